@@ -1,0 +1,47 @@
+// Internal declarations shared by the libfsim translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <mutex>
+#include <string>
+
+#include "../../include/fsim.h"
+
+namespace fsim {
+
+// ---- error reporting: status code + thread-local message ----------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+#define FSIM_HIP(call)                                                        \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess)                                                 \
+            return ::fsim::fail(FSIM_ERR_HIP, std::string("HIP: ") +          \
+                                hipGetErrorString(e_) + " at " + __FILE__ +  \
+                                ":" + std::to_string(__LINE__));              \
+    } while (0)
+
+// ---- device vectors (the Vec2d.Real / Vec2d.Cplx device side) -----------
+struct Vec {
+    int is_cplx;
+    int w, h;
+    void *d;       // float2* or float*
+    size_t bytes;
+};
+
+hipStream_t stream();            // the library's ordered stream
+int ensure_init();
+const float2 *twiddles(int L);   // exp(-2 pi i m / L), m < L, device resident
+int scratch_d(size_t bytes, void **p);   // growing device scratch (reductions, staging)
+std::mutex &scratch_mutex();            // held by every user of scratch_d for its whole use
+
+// ---- FFT entry (fft.hip) -------------------------------------------------
+bool fft_size_ok(int L);
+int fft2d_device(float2 *data, int w, int h, int inverse, hipStream_t s);
+
+// small helpers
+inline bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace fsim

@@ -1,0 +1,44 @@
+// Fused reconstruction: per-channel constants and job descriptors.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "filters.h"
+
+namespace fsim {
+
+// Per-channel constants read by every kernel of the pipeline (device copy).
+struct ReconConst {
+    int n, D, P, B, C;
+    int ns, otf_nb;
+    double otf_cpm, cutoff, vec_cpm;
+    const float2 *otf_tab;            // vals or valsAtt, otf_nb bands x ns
+    const float2 *wa;                 // (2n)^2 of (1/Wiener denominator, apodization)
+    const double *fade_lo, *fade_hi;  // fadeBorderCos factors, n each
+    float2 S[kMaxDir][kMaxComp][kMaxPhase];   // separation matrices (toFlt)
+    int offx[kMaxDir][kMaxComp], offy[kMaxDir][kMaxComp];   // integer paste offsets
+    double fx[kMaxDir][kMaxComp], fy[kMaxDir][kMaxComp];    // sub-pixel ramp shifts
+};
+
+// One stack: channel constants, raw input, work buffers and output.
+struct Job {
+    const ReconConst *c;
+    const uint16_t *raw;  // D*P frames of n*n
+    float2 *A;            // D*C components of n*n
+    float2 *B;            // (2n)^2
+    float *out;           // (2n)^2
+};
+
+// Either an explicit device array of jobs or a base job plus strides.
+struct JobList {
+    const Job *jobs;
+    Job base;
+    size_t raw_stride, a_stride, b_stride, out_stride;
+};
+
+bool recon_size_ok(int n);
+int launch_pipeline(int n, const JobList &jl, int njobs, int D, int C, hipStream_t s);
+size_t workspace_a_elems(int n, int D, int C);
+size_t workspace_b_elems(int n);
+
+}  // namespace fsim

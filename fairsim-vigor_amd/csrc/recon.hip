@@ -1,0 +1,507 @@
+// Fused fast-shift SIM reconstruction.
+//
+// Dataflow of one stack (livemode/ReconstructionRunner.java:456-542 and
+// tests/TestAccelSpeed.java:237-252, 563-703), regrouped into five HBM passes
+// over the raw side n and the doubled side m = 2n:
+//
+//  K1 k_sep_rows    raw u16 -> fade -> band separation (linear, so it moves in
+//                   front of the FFT) -> row FFT            writes A[d][c]
+//  K2 k_col_shift   shifted components only: column FFT -> * conj(OTF) ->
+//                   column IFFT                               in place on A
+//  K3 k_row_shift   shifted components only: row IFFT -> fast-shift phase ramp
+//                   (SimUtils.java:74-81) -> row FFT          in place on A
+//  K5 k_gather      per column tile of the m x m result: for every direction
+//                   and component, column FFT of the pasted column (the DC
+//                   component also * conj(OTF)), pasteFreq as a gather into a
+//                   register accumulator (SimUtils.java:83-86), then
+//                   * Wiener * apodization and the column IFFT  writes B
+//  K6 k_row_real    row IFFT of B, real part                  writes out
+//
+// HBM bytes per stack: 2*D*P*n^2 (raw) + 16*n^2 per complex pass per
+// component + 32*m^2 for B + 4*m^2 for the output -- the algorithmic count
+// of the fast-shift dataflow.
+#include "fsim_internal.h"
+#include "fft_core.h"
+#include "fft_launch.h"
+#include "filters.h"
+#include "recon.h"
+#include "simmath.h"
+
+#include <cmath>
+#include <vector>
+
+namespace fsim {
+
+__device__ __forceinline__ Job get_job(const JobList &jl, int j) {
+    if (jl.jobs) return jl.jobs[j];
+    Job J = jl.base;
+    J.raw += (size_t)j * jl.raw_stride;
+    J.A += (size_t)j * jl.a_stride;
+    J.B += (size_t)j * jl.b_stride;
+    J.out += (size_t)j * jl.out_stride;
+    return J;
+}
+
+// Fade factor of pixel (x, y): fadeBorderCos on a vector of ones
+// (SimUtils.java:162-185, ReconstructionRunner.java:781-783).
+__device__ __forceinline__ float fade_at(const double *__restrict__ lo, const double *__restrict__ hi, int x,
+                                         int y) {
+    float f = 1.0f;
+    f = (float)((double)f * lo[y]);
+    f = (float)((double)f * hi[y]);
+    f = (float)((double)f * lo[x]);
+    f = (float)((double)f * hi[x]);
+    return f;
+}
+
+// ---------------------------------------------------------------------------
+// K1: band separation + row FFT.  Block = RW rows of one component; the C
+// blocks of one row group are placed on the same XCD back to back (blocks
+// are dealt to the 8 XCDs round robin) so the raw rows they share hit L2.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(256) k_sep_rows(JobList jl, int ngroups, const float2 *__restrict__ tw) {
+    constexpr int E = FftE<N>::value, T = N / E;
+    constexpr int RW = T >= 256 ? 1 : 256 / T;
+    __shared__ float2 lds[RW * N];
+    const Job J = get_job(jl, blockIdx.y);
+    const ReconConst &K = *J.c;
+    const int C = K.C, P = K.P;
+    const int L = blockIdx.x;
+    const int g = (L / 8 / C) * 8 + (L % 8);
+    const int c = (L / 8) % C;
+    if (g >= ngroups) return;
+    constexpr int rows_per_dir = N / RW;
+    const int d = g / rows_per_dir, rb = g % rows_per_dir;
+    const int t = threadIdx.x % T, rr = threadIdx.x / T;
+    const int y = rb * RW + rr;
+    const size_t nn = (size_t)N * N;
+    const uint16_t *raw = J.raw + (size_t)d * P * nn + (size_t)y * N;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
+    float fy0 = 1.0f;
+    fy0 = (float)((double)fy0 * K.fade_lo[y]);
+    fy0 = (float)((double)fy0 * K.fade_hi[y]);
+    for (int p = 0; p < P; p++) {
+        const float2 s = K.S[d][c][p];
+        const uint16_t *rp = raw + (size_t)p * nn;
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int x = t + T * m;
+            float f = (float)((double)fy0 * K.fade_lo[x]);
+            f = (float)((double)f * K.fade_hi[x]);
+            const float px = (float)rp[x] * f;
+            v[m].x += s.x * px;
+            v[m].y += s.y * px;
+        }
+    }
+    fft_regs<N, E, false>(v, lds + rr * N, 1, t, tw);
+    float2 *out = J.A + ((size_t)(d * C + c) * N + y) * N;
+#pragma unroll
+    for (int m = 0; m < E; m++) out[t + T * m] = v[m];
+}
+
+// ---------------------------------------------------------------------------
+// K2: shifted components: column FFT, * conj(OTF_band), column IFFT (1/n).
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(1024) k_col_shift(JobList jl, const float2 *__restrict__ tw) {
+    constexpr int E = FftE<N>::value, T = N / E;
+    constexpr int CW = ColW<N>::value;
+    constexpr int tiles = N / CW;
+    __shared__ float2 lds[N * CW];
+    const Job J = get_job(jl, blockIdx.y);
+    const ReconConst &K = *J.c;
+    const int C = K.C, nsh = C - 1;
+    const int tile = blockIdx.x % tiles, dc = blockIdx.x / tiles;
+    const int d = dc / nsh, c = 1 + dc % nsh;
+    const int cl = threadIdx.x % CW, t = threadIdx.x / CW;
+    const int x = tile * CW + cl;
+    float2 *col = J.A + (size_t)(d * C + c) * N * N + x;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) v[m] = col[(size_t)(t + T * m) * N];
+    fft_regs<N, E, false>(v, lds + cl, CW, t, tw);
+    const int band = (c + 1) / 2;
+    const float2 *tab = K.otf_tab + (size_t)(K.otf_nb > 1 ? band : 0) * K.ns;
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const int y = t + T * m;
+        const float2 o = otf_pixel(tab, K.ns, K.otf_cpm, K.cutoff, x, y, N, N, 0, 0, K.vec_cpm);
+        v[m] = times_conj_java(v[m], o);
+    }
+    fft_regs<N, E, true>(v, lds + cl, CW, t, tw);
+    const float sc = 1.0f / N;
+#pragma unroll
+    for (int m = 0; m < E; m++) col[(size_t)(t + T * m) * N] = cscale(v[m], sc);
+}
+
+// ---------------------------------------------------------------------------
+// K3: shifted components: row IFFT (1/n), sub-pixel ramp, row FFT.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__restrict__ tw) {
+    constexpr int E = FftE<N>::value, T = N / E;
+    constexpr int RW = T >= 256 ? 1 : 256 / T;
+    constexpr int rblocks = N / RW;
+    __shared__ float2 lds[RW * N];
+    const Job J = get_job(jl, blockIdx.y);
+    const ReconConst &K = *J.c;
+    const int C = K.C, nsh = C - 1;
+    const int rb = blockIdx.x % rblocks, dc = blockIdx.x / rblocks;
+    const int d = dc / nsh, c = 1 + dc % nsh;
+    const int t = threadIdx.x % T, rr = threadIdx.x / T;
+    const int y = rb * RW + rr;
+    float2 *row = J.A + ((size_t)(d * C + c) * N + y) * N;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) v[m] = row[t + T * m];
+    fft_regs<N, E, true>(v, lds + rr * N, 1, t, tw);
+    const float sc = 1.0f / N;
+    const double fx = K.fx[d][c], fy = K.fy[d][c];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        float co, si;
+        shift_cos_sin(fx, fy, t + T * m, y, N, co, si);
+        v[m] = rot_java(cscale(v[m], sc), co, si);
+    }
+    fft_regs<N, E, false>(v, lds + rr * N, 1, t, tw);
+#pragma unroll
+    for (int m = 0; m < E; m++) row[t + T * m] = v[m];
+}
+
+// ---------------------------------------------------------------------------
+// K5: gather into the doubled grid, filter, column IFFT.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(1024) k_gather(JobList jl, const float2 *__restrict__ twn,
+                                                 const float2 *__restrict__ twm) {
+    constexpr int M = 2 * N;
+    constexpr int EN = FftE<N>::value, TN = N / EN;
+    constexpr int EM = FftE<M>::value, TM = M / EM;
+    constexpr int CW = ColW<M>::value;
+    __shared__ float2 lds[M * CW];
+    const Job J = get_job(jl, blockIdx.y);
+    const ReconConst &K = *J.c;
+    const int D = K.D, C = K.C;
+    const int cl = threadIdx.x % CW, t = threadIdx.x / CW;
+    const int xo = blockIdx.x * CW + cl;
+    float2 acc[EM];
+#pragma unroll
+    for (int m = 0; m < EM; m++) acc[m] = make_float2(0.f, 0.f);
+    const size_t nn = (size_t)N * N;
+    for (int d = 0; d < D; d++) {
+        for (int c = 0; c < C; c++) {
+            const int xs = paste_src(xo, M, N, K.offx[d][c]);
+            const bool act = (t < TN) && (xs >= 0);
+            float2 v[EN];
+            if (act) {
+                const float2 *col = J.A + (size_t)(d * C + c) * nn + xs;
+#pragma unroll
+                for (int m = 0; m < EN; m++) v[m] = col[(size_t)(t + TN * m) * N];
+            }
+            fft_regs<N, EN, false>(v, lds + cl, CW, t, twn, act);
+            if (act) {
+                if (c == 0) {
+                    const float2 *tab = K.otf_tab;
+#pragma unroll
+                    for (int m = 0; m < EN; m++) {
+                        const float2 o =
+                            otf_pixel(tab, K.ns, K.otf_cpm, K.cutoff, xs, t + TN * m, N, N, 0, 0, K.vec_cpm);
+                        v[m] = times_conj_java(v[m], o);
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < EN; m++) lds[(t + TN * m) * CW + cl] = v[m];
+            }
+            __syncthreads();
+            if (t < TM && xs >= 0) {
+                const int oy = K.offy[d][c];
+#pragma unroll
+                for (int m = 0; m < EM; m++) {
+                    const int ys = paste_src(t + TM * m, M, N, oy);
+                    if (ys >= 0) {
+                        const float2 s = lds[ys * CW + cl];
+                        acc[m].x += s.x;
+                        acc[m].y += s.y;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // fullResult.times(wienerDenom); fullResult.times(apo)
+    if (t < TM) {
+#pragma unroll
+        for (int m = 0; m < EM; m++) {
+            const float2 wa = K.wa[(size_t)(t + TM * m) * M + xo];
+            acc[m] = make_float2(acc[m].x * wa.x, acc[m].y * wa.x);
+            acc[m] = make_float2(acc[m].x * wa.y, acc[m].y * wa.y);
+        }
+    }
+    fft_regs<M, EM, true>(acc, lds + cl, CW, t, twm, t < TM);
+    if (t < TM) {
+        const float sc = 1.0f / M;
+#pragma unroll
+        for (int m = 0; m < EM; m++) J.B[(size_t)(t + TM * m) * M + xo] = cscale(acc[m], sc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K6: row IFFT of the doubled grid, real part out.
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ void __launch_bounds__(256) k_row_real(JobList jl, const float2 *__restrict__ tw) {
+    constexpr int E = FftE<M>::value, T = M / E;
+    constexpr int RW = T >= 256 ? 1 : 256 / T;
+    __shared__ float2 lds[RW * M];
+    const Job J = get_job(jl, blockIdx.y);
+    const int t = threadIdx.x % T, rr = threadIdx.x / T;
+    const int y = blockIdx.x * RW + rr;
+    const float2 *row = J.B + (size_t)y * M;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) v[m] = row[t + T * m];
+    fft_regs<M, E, true>(v, lds + rr * M, 1, t, tw);
+    const float sc = 1.0f / M;
+    float *o = J.out + (size_t)y * M;
+#pragma unroll
+    for (int m = 0; m < E; m++) o[t + T * m] = v[m].x * sc;
+}
+
+// pack Wiener reciprocal and apodization into one (w, a) table
+__global__ void k_pack_wa(float2 *__restrict__ wa, const float *__restrict__ w, const float *__restrict__ a,
+                          size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        wa[i] = make_float2(w[i], a[i]);
+}
+
+template <int N>
+static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, hipStream_t s) {
+    constexpr int M = 2 * N;
+    const float2 *twn = twiddles(N), *twm = twiddles(M);
+    if (!twn || !twm) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
+    {
+        constexpr int E = FftE<N>::value, T = N / E, RW = T >= 256 ? 1 : 256 / T;
+        const int ngroups = D * (N / RW);
+        const int gx = ((ngroups + 7) / 8) * 8 * C;
+        hipLaunchKernelGGL(k_sep_rows<N>, dim3(gx, njobs), dim3(T * RW), 0, s, jl, ngroups, twn);
+    }
+    if (C > 1) {
+        constexpr int E = FftE<N>::value, T = N / E, CW = ColW<N>::value;
+        hipLaunchKernelGGL(k_col_shift<N>, dim3(D * (C - 1) * (N / CW), njobs), dim3(T * CW), 0, s, jl, twn);
+        constexpr int RW = T >= 256 ? 1 : 256 / T;
+        hipLaunchKernelGGL(k_row_shift<N>, dim3(D * (C - 1) * (N / RW), njobs), dim3(T * RW), 0, s, jl, twn);
+    }
+    {
+        constexpr int EN = FftE<N>::value, TN = N / EN, EM = FftE<M>::value, TM = M / EM;
+        constexpr int TK = TN > TM ? TN : TM;
+        constexpr int CW = ColW<M>::value;
+        hipLaunchKernelGGL(k_gather<N>, dim3(M / CW, njobs), dim3(TK * CW), 0, s, jl, twn, twm);
+    }
+    {
+        constexpr int E = FftE<M>::value, T = M / E, RW = T >= 256 ? 1 : 256 / T;
+        hipLaunchKernelGGL(k_row_real<M>, dim3(M / RW, njobs), dim3(T * RW), 0, s, jl, twm);
+    }
+    FSIM_HIP(hipGetLastError());
+    return FSIM_OK;
+}
+
+bool recon_size_ok(int n) { return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048; }
+
+int launch_pipeline(int n, const JobList &jl, int njobs, int D, int C, hipStream_t s) {
+    switch (n) {
+        case 64: return launch_pipeline_t<64>(jl, njobs, D, C, s);
+        case 128: return launch_pipeline_t<128>(jl, njobs, D, C, s);
+        case 256: return launch_pipeline_t<256>(jl, njobs, D, C, s);
+        case 512: return launch_pipeline_t<512>(jl, njobs, D, C, s);
+        case 1024: return launch_pipeline_t<1024>(jl, njobs, D, C, s);
+        case 2048: return launch_pipeline_t<2048>(jl, njobs, D, C, s);
+        default: return fail(FSIM_ERR_ARG, "reconstruct: n must be a power of two in [64, 2048]");
+    }
+}
+
+size_t workspace_a_elems(int n, int D, int C) { return (size_t)D * C * n * n; }
+size_t workspace_b_elems(int n) { return (size_t)4 * n * n; }
+
+// ---------------------------------------------------------------------------
+// channel set-up: the FilterUpdateThread work (ReconstructionRunner.java:
+// 773-828) plus the per-direction constants of the reconstruction loop.
+// ---------------------------------------------------------------------------
+int build_channel_const(const fsim_recon_params &p, const Otf &o, ReconConst &K) {
+    const int n = p.n, D = p.ndir, P = p.nphase, B = p.nband, C = 2 * B - 1;
+    if (!recon_size_ok(n)) return fail(FSIM_ERR_ARG, "channel: n must be a power of two in [64, 2048]");
+    if (D < 1 || D > kMaxDir || B < 2 || B > kMaxBand || P < C || P > kMaxPhase)
+        return fail(FSIM_ERR_ARG, "channel: unsupported directions/phases/bands (phases >= 2*bands-1)");
+    if (p.vec_cpm <= 0) return fail(FSIM_ERR_ARG, "channel: vec_cpm must be positive");
+    if (o.nband != 1 && o.nband < B) return fail(FSIM_ERR_ARG, "channel: OTF has fewer bands than requested");
+    if (p.fade_px < 0 || 2 * p.fade_px > n) return fail(FSIM_ERR_ARG, "channel: fade border too wide");
+    K = ReconConst{};
+    K.n = n;
+    K.D = D;
+    K.P = P;
+    K.B = B;
+    K.C = C;
+    K.ns = o.ns;
+    K.otf_nb = o.nband;
+    K.otf_cpm = o.cpm;
+    K.cutoff = o.cutoff;
+    K.vec_cpm = p.vec_cpm;
+    K.otf_tab = p.use_att ? o.vals_att : o.vals;
+    for (int d = 0; d < D; d++) {
+        // SimParam.Dir.getModulations clamp (SimParam.java:598-607)
+        double mods[kMaxBand];
+        for (int b = 0; b < B; b++) {
+            double m = p.modul[d][b];
+            if (m < 0.3) m = 0.3;
+            if (m > 1.1) m = 1.1;
+            mods[b] = m;
+        }
+        double S[2 * kMaxComp * kMaxPhase];
+        int rc = separation_matrix_host(p.phases[d], P, B, mods, S);
+        if (rc) return rc;
+        for (int c = 0; c < C; c++)
+            for (int ph = 0; ph < P; ph++)
+                K.S[d][c][ph] = make_float2((float)S[2 * (c * P + ph)], (float)S[2 * (c * P + ph) + 1]);
+        K.offx[d][0] = K.offy[d][0] = 0;
+        K.fx[d][0] = K.fy[d][0] = 0;
+        for (int b = 1; b < B; b++) {
+            for (int s = 0; s < 2; s++) {
+                const int comp = s == 0 ? 2 * b : 2 * b - 1;  // pos, neg (SimAlgorithm.java:385-391)
+                const double kx = (s == 0 ? 1 : -1) * p.px[d] * b;
+                const double ky = (s == 0 ? 1 : -1) * p.py[d] * b;
+                // SimUtils.pasteAndFourierShift fast mode (SimUtils.java:74-86)
+                K.fx[d][comp] = kx - std::floor(kx);
+                K.fy[d][comp] = -ky - std::floor(-ky);
+                K.offx[d][comp] = (int)std::floor(kx);
+                K.offy[d][comp] = (int)std::floor(-ky);
+                if (K.offx[d][comp] < -2 * n || K.offy[d][comp] < -2 * n)
+                    return fail(FSIM_ERR_ARG, "channel: shift vector out of range");
+            }
+        }
+    }
+    return FSIM_OK;
+}
+
+}  // namespace fsim
+
+using namespace fsim;
+
+struct fsim_channel_s {
+    fsim_recon_params p;
+    ReconConst K;
+    ReconConst *dK = nullptr;
+    float2 *wa = nullptr;
+    double *fade = nullptr;
+    float2 *A = nullptr, *B = nullptr;
+    int cap = 0;
+};
+
+static void channel_free(fsim_channel_s *ch) {
+    (void)hipFree(ch->dK);
+    (void)hipFree(ch->wa);
+    (void)hipFree(ch->fade);
+    (void)hipFree(ch->A);
+    (void)hipFree(ch->B);
+}
+
+extern "C" {
+
+int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *out) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!p || !oh || !out) return fail(FSIM_ERR_ARG, "channel_create: null argument");
+    const Otf &o = oh->o;
+    fsim_channel_s *ch = new fsim_channel_s();
+    ch->p = *p;
+    rc = build_channel_const(*p, o, ch->K);
+    if (rc) {
+        delete ch;
+        return rc;
+    }
+    const int n = p->n, M = 2 * n;
+    const size_t mm = (size_t)M * M;
+    float *wtmp = nullptr, *atmp = nullptr;
+    if (hipMalloc(&ch->dK, sizeof(ReconConst)) != hipSuccess || hipMalloc(&ch->wa, mm * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&ch->fade, 2 * n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&wtmp, mm * sizeof(float)) != hipSuccess || hipMalloc(&atmp, mm * sizeof(float)) != hipSuccess) {
+        (void)hipFree(wtmp);
+        (void)hipFree(atmp);
+        channel_free(ch);
+        delete ch;
+        return fail(FSIM_ERR_OOM, "channel_create: device allocation failed");
+    }
+    std::vector<double> f(2 * n);
+    fade_factors_host(n, p->fade_px, f.data(), f.data() + n);
+    hipStream_t s = stream();
+    FSIM_HIP(hipMemcpyAsync(ch->fade, f.data(), 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
+    ch->K.fade_lo = ch->fade;
+    ch->K.fade_hi = ch->fade + n;
+    ch->K.wa = ch->wa;
+    ShiftSet sh{};
+    sh.ndir = p->ndir;
+    sh.nband = p->nband;
+    for (int d = 0; d < p->ndir; d++) {
+        sh.px[d] = p->px[d];
+        sh.py[d] = p->py[d];
+    }
+    rc = wiener_device(wtmp, M, M, o, p->vec_cpm, sh, p->use_att, p->wiener, s);
+    if (rc) return rc;
+    rc = write_apo_device(atmp, M, M, p->vec_cpm, o.cutoff, p->apo_bend, p->apo_cut, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pack_wa, dim3(1024), dim3(256), 0, s, ch->wa, wtmp, atmp, mm);
+    FSIM_HIP(hipGetLastError());
+    FSIM_HIP(hipMemcpyAsync(ch->dK, &ch->K, sizeof(ReconConst), hipMemcpyHostToDevice, s));
+    FSIM_HIP(hipStreamSynchronize(s));
+    (void)hipFree(wtmp);
+    (void)hipFree(atmp);
+    *out = ch;
+    return FSIM_OK;
+}
+
+int fsim_channel_destroy(fsim_channel ch) {
+    if (!ch) return FSIM_OK;
+    FSIM_HIP(hipStreamSynchronize(stream()));
+    channel_free(ch);
+    delete ch;
+    return FSIM_OK;
+}
+
+int fsim_reconstruct(fsim_channel ch, const uint16_t *raw, float *out, int count, void *strm) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!ch || !raw || !out || count < 1) return fail(FSIM_ERR_ARG, "reconstruct: bad arguments");
+    const ReconConst &K = ch->K;
+    const int n = K.n;
+    hipStream_t s = strm ? (hipStream_t)strm : stream();
+    if (count > ch->cap) {
+        FSIM_HIP(hipStreamSynchronize(s));
+        (void)hipFree(ch->A);
+        (void)hipFree(ch->B);
+        ch->A = ch->B = nullptr;
+        ch->cap = 0;
+        if (hipMalloc(&ch->A, workspace_a_elems(n, K.D, K.C) * count * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&ch->B, workspace_b_elems(n) * count * sizeof(float2)) != hipSuccess) {
+            (void)hipFree(ch->A);
+            ch->A = nullptr;
+            (void)hipGetLastError();
+            return fail(FSIM_ERR_OOM, "reconstruct: workspace allocation failed");
+        }
+        ch->cap = count;
+    }
+    JobList jl{};
+    jl.jobs = nullptr;
+    jl.base.c = ch->dK;
+    jl.base.raw = raw;
+    jl.base.A = ch->A;
+    jl.base.B = ch->B;
+    jl.base.out = out;
+    jl.raw_stride = (size_t)K.D * K.P * n * n;
+    jl.a_stride = workspace_a_elems(n, K.D, K.C);
+    jl.b_stride = workspace_b_elems(n);
+    jl.out_stride = (size_t)4 * n * n;
+    return launch_pipeline(n, jl, count, K.D, K.C, s);
+}
+
+}  // extern "C"

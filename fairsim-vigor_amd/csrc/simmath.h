@@ -1,0 +1,117 @@
+// Per-element SIM math shared by the vector-API kernels and the fused
+// reconstruction kernels.  Each function reproduces the Java arithmetic it
+// cites operation by operation (FMA contraction is switched off inside), so
+// that tables and integer maps match the reference bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fsim {
+
+// MTool.aproxSin01 / fsin / fcos (linalg/MTool.java:34-59), in double.
+__device__ __forceinline__ double aprox_sin01(double x) {
+#pragma clang fp contract(off)
+    double x2 = x * x;
+    return ((((.00015148419 * x2 - .00467376557) * x2 + .07968967928) * x2 - .64596371106) * x2 +
+            1.57079631847) *
+           x;
+}
+__device__ __forceinline__ double fsin_java(double x) {
+#pragma clang fp contract(off)
+    x = x / (M_PI / 2);
+    x -= floor(0.25 * x) * 4;
+    if (x > 3) return -aprox_sin01(4 - x);
+    if (x > 1) return aprox_sin01(2 - x);
+    return aprox_sin01(x);
+}
+
+// Phase of Vec2d.Cplx.fourierShift (BasicVectors.java:320-323):
+// phaVal = (float)(2*PI*(kx*x+ky*y)/N); co/si = (float)fcos/fsin(phaVal).
+__device__ __forceinline__ void shift_cos_sin(double kx, double ky, int x, int y, int N, float &co,
+                                              float &si) {
+#pragma clang fp contract(off)
+    float pha = (float)(2 * M_PI * (kx * x + ky * y) / N);
+    double p = (double)pha;
+    co = (float)fsin_java(p + M_PI / 2);
+    si = (float)fsin_java(p);
+}
+
+// Cplx.multReal / multImag of (re, im) and (co, si) (linalg/Cplx.java:325-332)
+__device__ __forceinline__ float2 rot_java(float2 v, float co, float si) {
+#pragma clang fp contract(off)
+    return make_float2((v.x * co) - (v.y * si), (v.y * co) + (v.x * si));
+}
+
+// Wrapped, y-flipped frequency coordinates (OtfProvider.java:345-346).
+__device__ __forceinline__ double xh_of(int x, int w) { return (double)((x < w / 2) ? x : x - w); }
+__device__ __forceinline__ double yh_of(int y, int h) { return (double)((y < h / 2) ? -y : h - y); }
+
+// OtfProvider.getOtfVal (OtfProvider.java:189-222) on a device table; both
+// interpolation weights use the lower sample, as the reference does.
+__device__ __forceinline__ float2 otf_val(const float2 *__restrict__ tab, int ns, double cpm, double cutoff,
+                                          double cycl) {
+#pragma clang fp contract(off)
+    if (cycl >= cutoff) return make_float2(0.f, 0.f);
+    const double pos = cycl / cpm;
+    if (ceil(pos) >= (double)ns) return make_float2(0.f, 0.f);
+    const int l = (int)floor(pos);
+    const float f = (float)(pos - l);
+    const float omf = 1 - f;
+    const float2 v = tab[l];
+    return make_float2(v.x * omf + v.x * f, v.y * omf + v.y * f);
+}
+
+// OtfProvider.getAttVal (OtfProvider.java:246-269)
+__device__ __forceinline__ float att_val(const float *__restrict__ only_att, double cpm, double cutoff,
+                                         double cycl) {
+#pragma clang fp contract(off)
+    if (cycl >= cutoff) return 1.f;
+    const double pos = cycl / cpm;
+    const int l = (int)floor(pos);
+    const float f = (float)(pos - l);
+    const float retl = only_att[l] * f;
+    const float reth = only_att[l] * (1 - f);
+    return retl + reth;
+}
+
+// Radius in cycles/micron of pixel (x, y) from centre (kx, ky):
+// MTool.fhypot(xh-kx, yh-ky) * vecCyclesPerMicron (OtfProvider.java:347-349).
+__device__ __forceinline__ double cycl_at(int x, int y, int w, int h, double kx, double ky, double vec_cpm) {
+#pragma clang fp contract(off)
+    const double dx = xh_of(x, w) - kx, dy = yh_of(y, h) - ky;
+    return sqrt(dx * dx + dy * dy) * vec_cpm;
+}
+
+// otfToVector value at a pixel (OtfProvider.java:351-365): 0 above cutoff.
+__device__ __forceinline__ float2 otf_pixel(const float2 *__restrict__ tab, int ns, double cpm, double cutoff,
+                                            int x, int y, int w, int h, double kx, double ky, double vec_cpm) {
+    const double cycl = cycl_at(x, y, w, h, kx, ky, vec_cpm);
+    if (cycl > cutoff) return make_float2(0.f, 0.f);
+    return otf_val(tab, ns, cpm, cutoff, cycl);
+}
+
+// Multiply by the conjugate: Cplx.Float.mult(val.conj()) (Cplx.java:298-300)
+__device__ __forceinline__ float2 mul_conj_java(float2 v, float2 o) {
+#pragma clang fp contract(off)
+    const float cr = o.x, ci = -o.y;
+    return make_float2(v.x * cr - v.y * ci, v.x * ci + v.y * cr);
+}
+// AbstractVectorCplx.times(in, conj=true) (AbstractVectorCplx.java:281-295):
+// this = multReal(in.re, -in.im, this.re, this.im), ...
+__device__ __forceinline__ float2 times_conj_java(float2 y, float2 x) {
+#pragma clang fp contract(off)
+    const float xr = x.x, xi = -x.y;
+    return make_float2((xr * y.x) - (xi * y.y), (xi * y.x) + (xr * y.y));
+}
+
+// Inverse of the pasteFreq index map (BasicVectors.java:273-278) along one
+// axis: output index -> input index, or -1 where nothing is pasted.
+// Valid for wo >= 2*wi, where the forward map is injective.
+__device__ __forceinline__ int paste_src(int o, int wo, int wi, int off) {
+    int p = (o - off) % wo;
+    if (p < 0) p += wo;
+    if (p < wi / 2) return p;
+    if (p >= wi / 2 + wo / 2 && p < wi + wo / 2) return p - wo / 2;
+    return -1;
+}
+
+}  // namespace fsim

@@ -1,0 +1,165 @@
+"""ctypes binding of libfsim.so (the C ABI declared in include/fsim.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if libfsim.so is missing or fails to load, every entry point raises, so a GPU
+run can never silently compute on the CPU.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfsim.so")
+
+FSIM_OK = 0
+FSIM_ERR_ARG = -1
+FSIM_ERR_OOM = -2
+FSIM_ERR_HIP = -3
+FSIM_ERR_STATE = -4
+FSIM_ERR_BUSY = -5
+
+
+class FsimError(RuntimeError):
+    """Mirror of the reference's RuntimeException("JNI-CUDA: ...")."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"fsim error {code}: {msg}")
+        self.code = code
+
+
+class FsimBusy(FsimError):
+    """Request refused by admission control (queue or memory full)."""
+
+
+class RECON_PARAMS(C.Structure):
+    """fsim_recon_params (include/fsim.h)."""
+    _fields_ = [
+        ("n", C.c_int),
+        ("ndir", C.c_int),
+        ("nphase", C.c_int),
+        ("nband", C.c_int),
+        ("px", C.c_double * 5),
+        ("py", C.c_double * 5),
+        ("phases", (C.c_double * 7) * 5),
+        ("modul", (C.c_double * 3) * 5),
+        ("vec_cpm", C.c_double),
+        ("use_att", C.c_int),
+        ("wiener", C.c_double),
+        ("apo_bend", C.c_double),
+        ("apo_cut", C.c_double),
+        ("fade_px", C.c_int),
+    ]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_F = C.c_float
+_L = C.c_int64
+_PD = C.POINTER(C.c_double)
+_PF = C.POINTER(C.c_float)
+_PI = C.POINTER(C.c_int)
+
+# name: (argtypes); every function returns int status unless listed in _RET
+_SIGS = {
+    "fsim_init": [_I],
+    "fsim_sync": [],
+    "fsim_alloc_device": [_L, C.POINTER(_P)],
+    "fsim_free_device": [_P],
+    "fsim_alloc_host": [_L, C.POINTER(_P)],
+    "fsim_free_host": [_P],
+    "fsim_memcpy": [_P, _P, _L, _I],
+    "fsim_vec_create": [_I, _I, _I, C.POINTER(_P)],
+    "fsim_vec_destroy": [_P],
+    "fsim_vec_shape": [_P, _PI, _PI, _PI],
+    "fsim_vec_upload": [_P, _P],
+    "fsim_vec_download": [_P, _P],
+    "fsim_vec_zero": [_P],
+    "fsim_vec_copy": [_P, _P],
+    "fsim_vec_add": [_P, _P],
+    "fsim_vec_add_const": [_P, _F, _F],
+    "fsim_vec_axpy": [_P, _F, _F, _P],
+    "fsim_vec_times": [_P, _P, _I],
+    "fsim_vec_scal": [_P, _F, _F],
+    "fsim_vec_norm2": [_P, _PD],
+    "fsim_vec_sum": [_P, _PD, _PD],
+    "fsim_vec_set_u16": [_P, _P],
+    "fsim_vec_set_u16_device": [_P, _P],
+    "fsim_vec_fft2d": [_P, _I],
+    "fsim_vec_fourier_shift": [_P, _D, _D],
+    "fsim_vec_paste_freq": [_P, _P, _I, _I],
+    "fsim_vec_get": [_P, _I, _I, _PF, _PF],
+    "fsim_vec_set": [_P, _I, _I, _F, _F],
+    "fsim_otf_from_estimate": [_D, _D, _D, _I, _D, _D, _P, _P, _P, _PD, _PD],
+    "fsim_otf_create": [_I, _I, _D, _D, _P, _P, _P, C.POINTER(_P)],
+    "fsim_otf_destroy": [_P],
+    "fsim_otf_to_vector": [_P, _P, _I, _D, _D, _D, _I, _I],
+    "fsim_mask_otf": [_P, _P, _D, _D, _D],
+    "fsim_write_apo": [_P, _P, _D, _D, _D],
+    "fsim_write_attenuation": [_P, _D, _D, _D, _D, _D],
+    "fsim_wiener_denominator": [_P, _P, _D, _I, _I, _PD, _PD, _I, _D],
+    "fsim_fade_border": [_P, _I],
+    "fsim_separation_matrix": [_PD, _I, _I, _PD, _PD],
+    "fsim_separate_bands": [C.POINTER(_P), _I, C.POINTER(_P), _I, _PF],
+    "fsim_locate_peak": [_P, _D, _PD],
+    "fsim_channel_create": [C.POINTER(RECON_PARAMS), _P, C.POINTER(_P)],
+    "fsim_channel_destroy": [_P],
+    "fsim_reconstruct": [_P, _P, _P, _I, _P],
+}
+_RET = {
+    "fsim_last_error": (C.c_char_p, []),
+    "fsim_device_bytes": (C.c_int64, []),
+    "fsim_vec_device_ptr": (_P, [_P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libfsim.so once; raise if it is missing (no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FsimError(FSIM_ERR_STATE, f"native library not built: {LIB_PATH} "
+                                        "(run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    for name, (ret, args) in _RET.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ret
+    _lib = L
+    return L
+
+
+def declared_symbols():
+    return list(_SIGS) + list(_RET)
+
+
+def check(rc):
+    """Raise the reference's exception type for a failing status."""
+    if rc == FSIM_OK:
+        return
+    msg = lib().fsim_last_error().decode(errors="replace")
+    if rc == FSIM_ERR_OOM:
+        raise MemoryError(msg)
+    if rc == FSIM_ERR_BUSY:
+        raise FsimBusy(rc, msg)
+    raise FsimError(rc, msg)
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args))
+
+
+_initialised = False
+
+
+def init(device=0):
+    global _initialised
+    if not _initialised:
+        call("fsim_init", device)
+        _initialised = True

@@ -1,0 +1,417 @@
+"""Host side of the SIM path: parameters, OTF provider and reconstructions.
+
+Mirrors org.fairsim.sim_algorithm for the 2D fast-shift reconstruction:
+``OtfProvider`` (sim_algorithm/OtfProvider.java), ``SimParam`` /
+``SimParam.Dir`` (sim_algorithm/SimParam.java), the ``SimUtils`` helpers
+used by the reconstruction (sim_algorithm/SimUtils.java:33-91) and two ways
+of running a reconstruction on the GPU:
+
+* ``reconstruct_plain`` -- the op sequence of
+  ``ReconstructionRunner.ReconstructionThread.run``
+  (livemode/ReconstructionRunner.java:456-542) issued one vector operation at
+  a time through the vector API, as the reference's accelerated path does;
+* ``Reconstructor`` -- the fused pipeline (fsim_channel / fsim_reconstruct)
+  that runs the same dataflow in five HBM passes for a batch of stacks.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import RECON_PARAMS, call, lib
+from .linalg import AccelVectorFactory, DeviceBuffer, Vec2dCplx, Vec2dReal
+
+
+class OtfProvider:
+    """OTF tables from an estimate (OtfProvider.fromEstimate,
+    OtfProvider.java:96-127) with attenuation (:276-287)."""
+
+    def __init__(self):
+        self.samples = 512
+        self.na = self.lam = self.a = None
+        self.att_str, self.att_fwhm = 0.99, 1.2
+        self.use_att = False
+        self.vec_cpm = -1.0
+        self._dev = None
+
+    @staticmethod
+    def fromEstimate(na, lam, a, samples=512):
+        o = OtfProvider()
+        o.na, o.lam, o.a, o.samples = float(na), float(lam), float(a), int(samples)
+        o._compute()
+        o.use_att = False
+        return o
+
+    def _compute(self):
+        n = self.samples
+        self.vals = np.zeros(2 * n, dtype=np.float32)
+        self.vals_att = np.zeros(2 * n, dtype=np.float32)
+        self.only_att = np.zeros(n, dtype=np.float32)
+        cut, cpm = C.c_double(), C.c_double()
+        call("fsim_otf_from_estimate", self.na, self.lam, self.a, n, self.att_str, self.att_fwhm,
+             self.vals.ctypes.data_as(C.c_void_p), self.vals_att.ctypes.data_as(C.c_void_p),
+             self.only_att.ctypes.data_as(C.c_void_p), C.byref(cut), C.byref(cpm))
+        self.cutoff, self.cpm = cut.value, cpm.value
+        self._release()
+
+    def _release(self):
+        if self._dev is not None and _lib._lib is not None:
+            lib().fsim_otf_destroy(self._dev)
+        self._dev = None
+
+    def __del__(self):
+        try:
+            self._release()
+        except Exception:
+            pass
+
+    def setAttenuation(self, strength, fwhm):
+        self.att_str, self.att_fwhm = float(strength), float(fwhm)
+        self._compute()
+
+    def switchAttenuation(self, on):
+        self.use_att = bool(on)
+
+    def isAttenuate(self):
+        return self.use_att
+
+    def getCutoff(self):
+        return self.cutoff
+
+    def setPixelSize(self, cycles_per_micron):
+        if cycles_per_micron <= 0:
+            raise ValueError("pxl size must be positive")
+        self.vec_cpm = float(cycles_per_micron)
+
+    def device(self):
+        """Device copy of the 1-D tables (fsim_otf)."""
+        if self._dev is None:
+            _lib.init()
+            h = C.c_void_p()
+            call("fsim_otf_create", 1, self.samples, self.cutoff, self.cpm,
+                 self.vals.ctypes.data_as(C.c_void_p), self.vals_att.ctypes.data_as(C.c_void_p),
+                 self.only_att.ctypes.data_as(C.c_void_p), C.byref(h))
+            self._dev = h
+        return self._dev
+
+    # --- table writers over device vectors (OtfProvider.java:331-529) ---
+    def otfToVector(self, vec, band, kx, ky, useAtt, write):
+        call("fsim_otf_to_vector", vec.handle, self.device(), int(band), float(kx), float(ky),
+             self.vec_cpm, 1 if useAtt else 0, 1 if write else 0)
+
+    def applyOtf(self, vec, band, kx=0.0, ky=0.0):
+        self.otfToVector(vec, band, kx, ky, self.use_att, False)
+
+    def writeOtfVector(self, vec, band, kx, ky):
+        self.otfToVector(vec, band, kx, ky, False, True)
+
+    def writeOtfWithAttVector(self, vec, band, kx, ky):
+        self.otfToVector(vec, band, kx, ky, self.use_att, True)
+
+    def writeApoVector(self, vec, bend, cutoff):
+        call("fsim_write_apo", vec.handle, self.device(), self.vec_cpm, float(bend), float(cutoff))
+
+    def writeAttenuationVector(self, vec, strength, fwhm, kx, ky):
+        call("fsim_write_attenuation", vec.handle, self.vec_cpm, float(strength), float(fwhm),
+             float(kx), float(ky))
+
+    def maskOtf(self, vec, kx, ky):
+        call("fsim_mask_otf", vec.handle, self.device(), float(kx), float(ky), self.vec_cpm)
+
+
+class Dir:
+    """SimParam.Dir (SimParam.java:378-615)."""
+
+    def __init__(self, bands, phases):
+        if bands < 2:
+            raise RuntimeError("bands < 2, not useful")
+        if phases < 2 * bands - 1:
+            raise RuntimeError(f"not enough phases for bands:{bands} {phases}")
+        self.nrBands, self.nrPhases = bands, phases
+        self.pX = self.pY = 0.0
+        self.phaOff = 0.0
+        self.phases = [(2 * math.pi / phases) * i for i in range(phases)]
+        self.modul = [1.0] * bands
+
+    def nrBand(self):
+        return self.nrBands
+
+    def nrComp(self):
+        return 2 * self.nrBands - 1
+
+    def nrPha(self):
+        return self.nrPhases
+
+    def setPxPy(self, x, y):
+        self.pX = x / (self.nrBands - 1)
+        self.pY = y / (self.nrBands - 1)
+
+    def px(self, band):
+        return self.pX * band
+
+    def py(self, band):
+        return self.pY * band
+
+    def setPhaOff(self, p):
+        self.phaOff = float(p)
+
+    def getPhaOff(self):
+        return self.phaOff
+
+    def setPhases(self, pha, reset):
+        if len(pha) != self.nrPhases:
+            raise RuntimeError("Length mismatch")
+        if reset:
+            self.phaOff = 0.0
+        self.phases = [float(p) for p in pha]
+
+    def getPhases(self):
+        return [p + self.phaOff for p in self.phases]
+
+    def setModulation(self, b, m):
+        self.modul[b] = float(m)
+        return 0.3 < m < 1.1
+
+    def getModulations(self):
+        return [min(max(m, 0.3), 1.1) for m in self.modul]
+
+
+class SimParam:
+    """SimParam.create(bands, dirs, phases, size, micronsPerPxl, otf)
+    (SimParam.java:103-130, 250-256)."""
+
+    def __init__(self, bands, dirs, phases, size, microns, otf):
+        self.nrBands, self.nrDirs, self.nrPhases = bands, dirs, phases
+        self.directions = [Dir(bands, phases) for _ in range(dirs)]
+        self.imgSize = int(size)
+        self.micronsPerPixel = float(microns)
+        self.cyclesPerMicron = 1 / (size * microns)
+        self.otf_ = otf
+        if otf is not None:
+            otf.setPixelSize(self.cyclesPerMicron)
+
+    @staticmethod
+    def create(bands, dirs, phases, size, microns, otf):
+        return SimParam(bands, dirs, phases, size, microns, otf)
+
+    def dir(self, i):
+        return self.directions[i]
+
+    def nrDir(self):
+        return self.nrDirs
+
+    def nrBand(self):
+        return self.nrBands
+
+    def otf(self):
+        return self.otf_
+
+    def pxlSizeCyclesMicron(self):
+        return self.cyclesPerMicron
+
+    def vectorWidth(self):
+        return self.imgSize
+
+    def vectorHeight(self):
+        return self.imgSize
+
+
+def separation_matrix(phases, bands, fac=None):
+    """BandSeparation.createSeparationMatrix (BandSeparation.java:40-114),
+    computed natively; returns a (C, P) complex128 array."""
+    P = len(phases)
+    Cc = 2 * bands - 1
+    ph = (C.c_double * P)(*phases)
+    out = (C.c_double * (2 * Cc * P))()
+    facp = (C.c_double * bands)(*fac) if fac is not None else None
+    call("fsim_separation_matrix", ph, P, bands, facp, out)
+    a = np.frombuffer(out, dtype=np.float64).reshape(Cc, P, 2)
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def separate_bands(inv, out, phases, bands, fac):
+    """BandSeparation.separateBands (BandSeparation.java:131-165) in one pass."""
+    S = separation_matrix(phases, bands, fac)
+    Sf = np.empty(S.shape + (2,), dtype=np.float32)
+    Sf[..., 0] = S.real
+    Sf[..., 1] = S.imag
+    ins = (C.c_void_p * len(inv))(*[v.handle.value for v in inv])
+    outs = (C.c_void_p * len(out))(*[v.handle.value for v in out])
+    call("fsim_separate_bands", ins, len(inv), outs, len(out),
+         Sf.ctypes.data_as(C.POINTER(C.c_float)))
+
+
+def wiener_denominator(vec, param, wparam):
+    """WienerFilter(param).getDenominator(wparam) (WienerFilter.java:32-122)."""
+    otf = param.otf()
+    D = param.nrDir()
+    px = (C.c_double * D)(*[param.dir(d).pX for d in range(D)])
+    py = (C.c_double * D)(*[param.dir(d).pY for d in range(D)])
+    call("fsim_wiener_denominator", vec.handle, otf.device(), param.pxlSizeCyclesMicron(), D,
+         param.nrBand(), px, py, 1 if otf.isAttenuate() else 0, float(wparam))
+
+
+def fade_border_cos(vec, px):
+    """SimUtils.fadeBorderCos (SimUtils.java:162-185)."""
+    call("fsim_fade_border", vec.handle, int(px))
+
+
+def locate_peak(vec, kmin):
+    """Correlation.locatePeak (Correlation.java:144-167)."""
+    r = (C.c_double * 4)()
+    call("fsim_locate_peak", vec.handle, float(kmin), r)
+    return list(r)
+
+
+def paste_and_fourier_shift(inV, outV, kx, ky, fast=True):
+    """SimUtils.pasteAndFourierShift (SimUtils.java:64-91) on device vectors."""
+    if not fast:
+        outV.pasteFreq(inV, 0, 0)
+        outV.fft2d(True)
+        outV.fourierShift(kx, -ky)
+        outV.fft2d(False)
+    else:
+        xf = kx - math.floor(kx)
+        yf = -ky - math.floor(-ky)
+        inV.fft2d(True)
+        inV.fourierShift(xf, yf)
+        inV.fft2d(False)
+        outV.pasteFreq(inV, int(math.floor(kx)), int(math.floor(-ky)))
+
+
+class PlainPipeline:
+    """Pre-allocated vectors and filters of one ReconstructionThread
+    (ReconstructionRunner.java:380-413) for one channel; ``run`` issues the
+    reconstruction loop body one vector op at a time."""
+
+    def __init__(self, param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10):
+        vf = AccelVectorFactory.getFactory()
+        self.param = param
+        n = param.vectorWidth()
+        D, P, B = param.nrDir(), param.dir(0).nrPha(), param.nrBand()
+        Cc = 2 * B - 1
+        self.n = n
+        otf = param.otf()
+        # FilterUpdateThread.setupFilters (ReconstructionRunner.java:773-813)
+        self.damp = vf.createReal2D(n, n)
+        self.damp.addConst(1.0)
+        if fade_px > 0:
+            fade_border_cos(self.damp, fade_px)
+        self.otfV = vf.createCplx2D(n, n)
+        otf.writeOtfWithAttVector(self.otfV, 0, 0, 0)
+        self.wien = vf.createReal2D(2 * n, 2 * n)
+        wiener_denominator(self.wien, param, wiener)
+        self.apo = vf.createReal2D(2 * n, 2 * n)
+        otf.writeApoVector(self.apo, apo_bend, apo_cut)
+        self.inFFT = vf.createArrayCplx2D(D, P, n, n)
+        self.sep = vf.createArrayCplx2D(D, Cc, n, n)
+        self.shifted = vf.createArrayCplx2D(D, Cc, 2 * n, 2 * n)
+        self.full = vf.createCplx2D(2 * n, 2 * n)
+        self.result = vf.createReal2D(2 * n, 2 * n)
+
+    def run(self, raw_dev_ptr):
+        """One reconstruction of a device-resident raw stack (uint16,
+        angle-major); the result stays in ``self.result``."""
+        par = self.param
+        n, D = self.n, par.nrDir()
+        P, B = par.dir(0).nrPha(), par.nrBand()
+        self.full.zero()
+        frame = n * n * 2
+        for a in range(D):
+            for p in range(P):
+                v = self.inFFT[a][p]
+                v.setFrom16bitPixelsDevice(raw_dev_ptr + (a * P + p) * frame)
+                v.times(self.damp)
+                v.fft2d(False)
+        for a in range(D):
+            d = par.dir(a)
+            separate_bands(self.inFFT[a], self.sep[a], d.getPhases(), B, d.getModulations())
+            for c in range(2 * B - 1):
+                self.sep[a][c].timesConj(self.otfV)
+            self.shifted[a][0].pasteFreq(self.sep[a][0])
+            for b in range(1, B):
+                pos, neg = 2 * b, 2 * b - 1
+                paste_and_fourier_shift(self.sep[a][pos], self.shifted[a][pos], d.px(b), d.py(b), True)
+                paste_and_fourier_shift(self.sep[a][neg], self.shifted[a][neg], -d.px(b), -d.py(b), True)
+            for c in range(2 * B - 1):
+                self.full.add(self.shifted[a][c])
+        self.full.times(self.wien)
+        self.full.times(self.apo)
+        self.full.fft2d(True)
+        self.result.copy(self.full)
+        return self.result
+
+
+def recon_params(param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10):
+    """Pack a SimParam into fsim_recon_params."""
+    p = RECON_PARAMS()
+    p.n = param.vectorWidth()
+    p.ndir = param.nrDir()
+    p.nphase = param.dir(0).nrPha()
+    p.nband = param.nrBand()
+    if p.ndir > 5 or p.nphase > 7 or p.nband > 3:
+        raise _lib.FsimError(_lib.FSIM_ERR_ARG, "at most 5 directions, 7 phases, 3 bands")
+    for d in range(p.ndir):
+        dd = param.dir(d)
+        p.px[d], p.py[d] = dd.pX, dd.pY
+        for i, ph in enumerate(dd.getPhases()):
+            p.phases[d][i] = ph
+        for b in range(p.nband):
+            p.modul[d][b] = dd.modul[b]
+    p.vec_cpm = param.pxlSizeCyclesMicron()
+    p.use_att = 1 if param.otf().isAttenuate() else 0
+    p.wiener = wiener
+    p.apo_bend = apo_bend
+    p.apo_cut = apo_cut
+    p.fade_px = fade_px
+    return p
+
+
+class Reconstructor:
+    """Fused reconstruction of one channel (fsim_channel): tables resident on
+    the device, ``reconstruct`` runs a batch of stacks in five passes."""
+
+    def __init__(self, param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10):
+        _lib.init()
+        self.param = param
+        self.n = param.vectorWidth()
+        self.D, self.P = param.nrDir(), param.dir(0).nrPha()
+        self.p = recon_params(param, wiener, apo_bend, apo_cut, fade_px)
+        self._h = C.c_void_p()
+        call("fsim_channel_create", C.byref(self.p), param.otf().device(), C.byref(self._h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            lib().fsim_channel_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def raw_bytes(self):
+        return self.D * self.P * self.n * self.n * 2
+
+    def out_bytes(self):
+        return 4 * self.n * self.n * 4
+
+    def reconstruct(self, raw_dev, out_dev, count=1, stream=None):
+        """raw_dev / out_dev: device addresses (int) of count stacks/images."""
+        call("fsim_reconstruct", self._h, C.c_void_p(int(raw_dev)), C.c_void_p(int(out_dev)),
+             int(count), C.c_void_p(stream) if stream else None)
+
+    def reconstruct_host(self, raw):
+        """Convenience: host uint16 stack(s) in, host float32 image(s) out."""
+        raw = np.ascontiguousarray(raw, dtype=np.uint16)
+        count = raw.size // (self.D * self.P * self.n * self.n)
+        rb = DeviceBuffer(raw.nbytes)
+        ob = DeviceBuffer(count * self.out_bytes())
+        rb.upload(raw)
+        self.reconstruct(rb.value, ob.value, count)
+        call("fsim_sync")
+        out = ob.download((count, 2 * self.n, 2 * self.n), np.float32)
+        rb.free()
+        ob.free()
+        return out[0] if count == 1 else out

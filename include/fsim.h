@@ -1,0 +1,217 @@
+/*
+ * libfsim -- MI355X (gfx950) implementation of fairSIM's 2D SR-SIM
+ * reconstruction path behind the org.fairsim.linalg vector API.
+ *
+ * C ABI only: plain pointers, sizes and opaque handles.  Every entry point
+ * returns FSIM_OK (0) or a negative status; fsim_last_error() returns the
+ * calling thread's last message.  The JNI shim maps FSIM_ERR_OOM to
+ * OutOfMemoryError and every other failure to RuntimeException, which is the
+ * reference's error convention (accel/cuda_common.h:33-63,
+ * org/fairsim/accel/AccelVectorCplx.java:45-54).
+ *
+ * Vectors are device resident, row-major [h][w], complex interleaved
+ * (re, im) float32 -- the layout of Vec2d (org/fairsim/linalg/Vec2d.java:23-24,
+ * org/fairsim/linalg/BasicVectors.java:233-245).  All operations are
+ * enqueued on one library stream in call order; results read back by
+ * fsim_vec_download / the reductions are synchronous.
+ */
+#ifndef FSIM_H
+#define FSIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSIM_OK 0
+#define FSIM_ERR_ARG (-1)     /* size mismatch / bad argument -> RuntimeException */
+#define FSIM_ERR_OOM (-2)     /* device allocation failed -> OutOfMemoryError */
+#define FSIM_ERR_HIP (-3)     /* HIP runtime error -> RuntimeException */
+#define FSIM_ERR_STATE (-4)   /* not initialised / unsupported -> RuntimeException */
+#define FSIM_ERR_BUSY (-5)    /* server: request refused (queue or memory full) */
+
+typedef struct fsim_vec_s *fsim_vec;
+typedef struct fsim_otf_s *fsim_otf;
+
+/* Thread-local message of the last failing call on this thread. */
+const char *fsim_last_error(void);
+
+/* ---- lifecycle ---------------------------------------------------------
+ * fsim_init: select the device and create the library stream.
+ *   replaces AccelVectorFactory's JNI_OnLoad / factory set-up
+ *   (accel/cuda_mgmt.cu:38-60, org/fairsim/accel/AccelVectorFactory.java:40-55)
+ * fsim_sync: AccelVectorFactory.nativeSync (AccelVectorFactory.java:139;
+ *   accel/cuda_mgmt.cu:38-44) */
+int fsim_init(int device);
+int fsim_sync(void);
+/* Bytes of device memory currently held by vectors and plans. */
+int64_t fsim_device_bytes(void);
+
+/* Raw buffers: AccelVectorFactory.nativeAllocMemory / nativeAllocMemoryHost
+ * (AccelVectorFactory.java:141-142; accel/cuda_mgmt.cu:46-60).  Host buffers
+ * are pinned.  fsim_memcpy copies on the library stream and returns when the
+ * copy is done; kind: 1 host->device, 2 device->host, 3 device->device. */
+int fsim_alloc_device(int64_t bytes, void **out);
+int fsim_free_device(void *p);
+int fsim_alloc_host(int64_t bytes, void **out);
+int fsim_free_host(void *p);
+int fsim_memcpy(void *dst, const void *src, int64_t bytes, int kind);
+
+/* ---- vectors: AccelVectorReal / AccelVectorCplx natives --------------------
+ * create: alloc(vf, n) (AccelVectorCplx.java:296, AccelVectorReal.java:218;
+ *   accel/cuda_cplx.cu:39-70), zero-initialised.
+ * destroy: dealloc (AccelVectorCplx.java:297; accel/cuda_cplx.cu:74-90).
+ * upload/download: copyBuffer(J[FJZI)V (AccelVectorCplx.java:298;
+ *   accel/cuda_cplx.cu:93-177); host layout = vectorData() float[]. */
+int fsim_vec_create(int is_cplx, int w, int h, fsim_vec *out);
+int fsim_vec_destroy(fsim_vec v);
+int fsim_vec_shape(fsim_vec v, int *is_cplx, int *w, int *h);
+void *fsim_vec_device_ptr(fsim_vec v);
+int fsim_vec_upload(fsim_vec v, const float *host);
+int fsim_vec_download(fsim_vec v, float *host);
+
+/* In-place arithmetic, `dst` is `this` of the Java method.
+ * zero:      nativeZero (AccelVectorCplx.java:309; cuda_cplx.cu:331-335)
+ * copy:      nativeCOPYCPLX / nativeCOPYREAL / Vec.Real.copy(Cplx)
+ *            (AccelVectorCplx.java:306-307; AbstractVectorReal.java:99-106):
+ *            cplx<-cplx, cplx<-real (im = 0), real<-cplx (real part), real<-real
+ * add:       nativeAdd (AccelVectorCplx.java:300; cuda_cplx.cu:263-276)
+ * add_const: nativeADDCONST (AccelVectorCplx.java:301; cuda_cplx.cu:279-288)
+ * axpy:      nativeAXPY (AccelVectorCplx.java:302; cuda_cplx.cu:291-304),
+ *            dst += (are + i aim) * x
+ * times:     nativeTIMES / nativeTIMESREAL (AccelVectorCplx.java:303-304;
+ *            cuda_cplx.cu:307-329); x complex or real; conj conjugates x
+ * scal:      nativeScal (AccelVectorCplx.java:310; cuda_cplx.cu:338-341)
+ * norm2:     nativeREDUCE(..., true) (AccelVectorCplx.java:305;
+ *            cuda_cplx.cu:344-364): sum of float squares, double accumulation
+ * sum:       AbstractVectorCplx.sumElements (AbstractVectorCplx.java:353-362),
+ *            absent from the reference accelerator (it copies to the host) */
+int fsim_vec_zero(fsim_vec v);
+int fsim_vec_copy(fsim_vec dst, fsim_vec src);
+int fsim_vec_add(fsim_vec dst, fsim_vec src);
+int fsim_vec_add_const(fsim_vec dst, float re, float im);
+int fsim_vec_axpy(fsim_vec dst, float are, float aim, fsim_vec x);
+int fsim_vec_times(fsim_vec dst, fsim_vec x, int conj);
+int fsim_vec_scal(fsim_vec dst, float re, float im);
+int fsim_vec_norm2(fsim_vec v, double *out);
+int fsim_vec_sum(fsim_vec v, double *re, double *im);
+
+/* Vec2d ops.
+ * set_u16:       nativeCOPYSHORT (AccelVectorCplx.java:483;
+ *                cuda_cplx.cu:217-259, 375-381): out = (float)(in & 0xFFFF)
+ *                (+0i); `host` points to w*h uint16 in host memory.
+ * set_u16_device: same with the pixels already in device memory.
+ * fft2d:         nativeFFT (AccelVectorCplx.java:487; cuda_cplx.cu:475-496);
+ *                w, h powers of two in [2, 4096]; forward unscaled, inverse
+ *                scaled by 1/(w*h).
+ * fourier_shift: nativeFourierShift (AccelVectorCplx.java:488;
+ *                cuda_cplx.cu:517-543), square vectors; the phase is rounded
+ *                to float as BasicVectors.java:320 does, sin/cos use the
+ *                MTool.fsin/fcos polynomial (linalg/MTool.java:34-59).
+ * paste_freq:    nativePasteFreq (AccelVectorCplx.java:489;
+ *                cuda_cplx.cu:547-580; BasicVectors.java:259-285).
+ * get/set:       nativeGet / nativeSet (AccelVectorCplx.java:485-486). */
+int fsim_vec_set_u16(fsim_vec v, const uint16_t *host);
+int fsim_vec_set_u16_device(fsim_vec v, const uint16_t *dev);
+int fsim_vec_fft2d(fsim_vec v, int inverse);
+int fsim_vec_fourier_shift(fsim_vec v, double kx, double ky);
+int fsim_vec_paste_freq(fsim_vec out, fsim_vec in, int xoff, int yoff);
+int fsim_vec_get(fsim_vec v, int x, int y, float *re, float *im);
+int fsim_vec_set(fsim_vec v, int x, int y, float re, float im);
+
+/* ---- OTF / filters (sim_algorithm/OtfProvider.java, WienerFilter.java) ----
+ * An fsim_otf holds OtfProvider's per-band 1-D tables on the device:
+ * vals, valsAtt (complex) and valsOnlyAtt (real), nsamples each
+ * (OtfProvider.java:39-41), plus cutoff and the table's cycles/micron per
+ * sample (OtfProvider.java:44-45).
+ *
+ * fsim_otf_from_estimate: OtfProvider.fromEstimate + setAttenuation
+ *   (OtfProvider.java:96-127, 276-287) computed on the host in double,
+ *   writing nband=1 tables into caller arrays (vals/vals_att: 2*n floats,
+ *   only_att: n floats) and cutoff/cpm.
+ * fsim_otf_create: upload such tables (nband * nsamples). */
+int fsim_otf_from_estimate(double na, double lambda_nm, double a, int nsamples,
+                           double att_str, double att_fwhm, float *vals,
+                           float *vals_att, float *only_att, double *cutoff,
+                           double *cpm);
+int fsim_otf_create(int nband, int nsamples, double cutoff, double cpm,
+                    const float *vals, const float *vals_att, const float *only_att,
+                    fsim_otf *out);
+int fsim_otf_destroy(fsim_otf o);
+
+/* otfToVector (OtfProvider.java:331-373): write (write != 0) or multiply by
+ * the conjugate (write == 0) the OTF of `band` centred at (kx, ky);
+ * vec_cpm = the vector's cycles/micron per pixel (setPixelSize). */
+int fsim_otf_to_vector(fsim_vec v, fsim_otf o, int band, double kx, double ky,
+                       double vec_cpm, int use_att, int write);
+/* maskOtf (OtfProvider.java:538-559) */
+int fsim_mask_otf(fsim_vec v, fsim_otf o, double kx, double ky, double vec_cpm);
+/* writeApoVector, real or complex vector (OtfProvider.java:437-492) */
+int fsim_write_apo(fsim_vec v, fsim_otf o, double vec_cpm, double bend, double cutoff_fac);
+/* writeAttenuationVector (OtfProvider.java:503-529) */
+int fsim_write_attenuation(fsim_vec v, double vec_cpm, double str, double fwhm,
+                           double kx, double ky);
+/* WienerFilter(sp).getDenominator(w) (WienerFilter.java:46-122) into a real
+ * vector: 1 / (sum_d sum_b |OTF_b(r1)|^2 att(r1) + |OTF_b(r2)|^2 att(r2) + w^2).
+ * px, py: band-1 shift per direction (SimParam.Dir pX, pY). */
+int fsim_wiener_denominator(fsim_vec out, fsim_otf o, double vec_cpm, int ndir,
+                            int nband, const double *px, const double *py,
+                            int use_att, double wparam);
+/* SimUtils.fadeBorderCos (SimUtils.java:162-185) on a real vector. */
+int fsim_fade_border(fsim_vec v, int px);
+
+/* BandSeparation.createSeparationMatrix (BandSeparation.java:40-114) on the
+ * host: phases[nphase] absolute band-1 phases, fac[nband] or NULL;
+ * out: (2*nband-1) x nphase complex doubles, interleaved. */
+int fsim_separation_matrix(const double *phases, int nphase, int nband,
+                           const double *fac, double *out);
+/* BandSeparation.separateBands multiply step (BandSeparation.java:152-163):
+ * out[c] = sum_p S[c][p] * in[p], S given as (ncomp x nphase) complex floats
+ * (the matrix after toFlt()).  One pass over the inputs. */
+int fsim_separate_bands(const fsim_vec *in, int nphase, const fsim_vec *out, int ncomp,
+                        const float *S);
+
+/* ---- estimation (sim_algorithm/Correlation.java) ------------------------
+ * locate_peak: Correlation.locatePeak (Correlation.java:144-167);
+ *   res = {xPos, yPos, max, phase} in the reference's coordinates. */
+int fsim_locate_peak(fsim_vec v, double kmin, double *res);
+
+/* ---- fused fast-shift reconstruction --------------------------------------
+ * One reconstruction = livemode/ReconstructionRunner.java:456-542 /
+ * tests/TestAccelSpeed.java:237-252, 563-703 for one channel: raw stack of
+ * ndir*nphase frames n x n uint16 (angle-major) -> 2n x 2n float32.
+ *
+ * fsim_channel: per-channel resident state built once (the FilterUpdateThread
+ * of ReconstructionRunner.java:773-828): OTF tables, reciprocal Wiener
+ * denominator and apodization for the (2n)^2 grid, the fade-border factors,
+ * and the per-direction separation matrices and shift parameters. */
+typedef struct fsim_channel_s *fsim_channel;
+
+typedef struct {
+    int n;                    /* raw side, power of two in [64, 2048] */
+    int ndir, nphase, nband;  /* ndir <= 5, nphase <= 7, nband <= 3 */
+    double px[5], py[5];      /* band-1 shift per direction (SimParam.Dir pX, pY) */
+    double phases[5][7];      /* absolute band-1 phases (SimParam.Dir.getPhases) */
+    double modul[5][3];       /* raw modulations; clamped as getModulations */
+    double vec_cpm;           /* 1 / (n * micronsPerPixel) */
+    int use_att;              /* OTF with attenuation */
+    double wiener;            /* Wiener parameter w */
+    double apo_bend, apo_cut; /* writeApoVector(bend, cutoff factor) */
+    int fade_px;              /* fadeBorderCos width (10), 0 = off */
+} fsim_recon_params;
+
+int fsim_channel_create(const fsim_recon_params *p, fsim_otf otf, fsim_channel *out);
+int fsim_channel_destroy(fsim_channel ch);
+
+/* Reconstruct `count` stacks of one channel.  raw_dev: count stacks back to
+ * back in device memory; out_dev: count images of (2n)^2 float.  Work buffers
+ * are held by the channel and grow to the largest count seen.  `stream` is a
+ * hipStream_t or NULL for the library stream.  Asynchronous. */
+int fsim_reconstruct(fsim_channel ch, const uint16_t *raw_dev, float *out_dev,
+                     int count, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSIM_H */
