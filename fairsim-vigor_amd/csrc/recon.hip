@@ -28,6 +28,9 @@
 #include "simmath.h"
 
 #include <cmath>
+#include <mutex>
+#include <set>
+#include <utility>
 #include <vector>
 
 namespace fsim {
@@ -277,32 +280,60 @@ __global__ void k_pack_wa(float2 *__restrict__ wa, const float *__restrict__ w, 
         wa[i] = make_float2(w[i], a[i]);
 }
 
+// Optional stage timing: HIP events recorded around each pipeline kernel on
+// the launch stream, summed per stage when read back.
+struct StageTimer {
+    std::mutex mu;
+    bool on = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+};
+static StageTimer g_timer;
+
+static void stage_mark(int stage, hipStream_t s, hipEvent_t &start) {
+    if (!g_timer.on) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, s);
+    if (stage >= 0) {
+        std::lock_guard<std::mutex> lk(g_timer.mu);
+        g_timer.ev.push_back({stage, {start, e}});
+    }
+    start = e;
+}
+
 template <int N>
 static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, hipStream_t s) {
     constexpr int M = 2 * N;
     const float2 *twn = twiddles(N), *twm = twiddles(M);
     if (!twn || !twm) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
+    hipEvent_t ev = nullptr;
+    stage_mark(-1, s, ev);
     {
         constexpr int E = FftE<N>::value, T = N / E, RW = T >= 256 ? 1 : 256 / T;
         const int ngroups = D * (N / RW);
         const int gx = ((ngroups + 7) / 8) * 8 * C;
         hipLaunchKernelGGL(k_sep_rows<N>, dim3(gx, njobs), dim3(T * RW), 0, s, jl, ngroups, twn);
+        stage_mark(0, s, ev);
     }
     if (C > 1) {
         constexpr int E = FftE<N>::value, T = N / E, CW = ColW<N>::value;
         hipLaunchKernelGGL(k_col_shift<N>, dim3(D * (C - 1) * (N / CW), njobs), dim3(T * CW), 0, s, jl, twn);
+        stage_mark(1, s, ev);
         constexpr int RW = T >= 256 ? 1 : 256 / T;
         hipLaunchKernelGGL(k_row_shift<N>, dim3(D * (C - 1) * (N / RW), njobs), dim3(T * RW), 0, s, jl, twn);
+        stage_mark(2, s, ev);
     }
     {
         constexpr int EN = FftE<N>::value, TN = N / EN, EM = FftE<M>::value, TM = M / EM;
         constexpr int TK = TN > TM ? TN : TM;
         constexpr int CW = ColW<M>::value;
         hipLaunchKernelGGL(k_gather<N>, dim3(M / CW, njobs), dim3(TK * CW), 0, s, jl, twn, twm);
+        stage_mark(3, s, ev);
     }
     {
         constexpr int E = FftE<M>::value, T = M / E, RW = T >= 256 ? 1 : 256 / T;
         hipLaunchKernelGGL(k_row_real<M>, dim3(M / RW, njobs), dim3(T * RW), 0, s, jl, twm);
+        stage_mark(4, s, ev);
     }
     FSIM_HIP(hipGetLastError());
     return FSIM_OK;
@@ -502,6 +533,34 @@ int fsim_reconstruct(fsim_channel ch, const uint16_t *raw, float *out, int count
     jl.b_stride = workspace_b_elems(n);
     jl.out_stride = (size_t)4 * n * n;
     return launch_pipeline(n, jl, count, K.D, K.C, s);
+}
+
+int fsim_stage_timing(int enable) {
+    std::lock_guard<std::mutex> lk(g_timer.mu);
+    g_timer.on = enable != 0;
+    return FSIM_OK;
+}
+
+int fsim_stage_times(double *ms, int *launches, int nstage) {
+    if (!ms || !launches || nstage < 5) return fail(FSIM_ERR_ARG, "stage_times: need 5 slots");
+    std::lock_guard<std::mutex> lk(g_timer.mu);
+    for (int i = 0; i < nstage; i++) {
+        ms[i] = 0;
+        launches[i] = 0;
+    }
+    std::set<hipEvent_t> owned;
+    for (auto &e : g_timer.ev) {
+        FSIM_HIP(hipEventSynchronize(e.second.second));
+        float t = 0;
+        FSIM_HIP(hipEventElapsedTime(&t, e.second.first, e.second.second));
+        ms[e.first] += t;
+        launches[e.first] += 1;
+        owned.insert(e.second.first);
+        owned.insert(e.second.second);
+    }
+    for (auto e : owned) (void)hipEventDestroy(e);
+    g_timer.ev.clear();
+    return FSIM_OK;
 }
 
 }  // extern "C"

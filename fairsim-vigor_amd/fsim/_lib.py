@@ -104,6 +104,8 @@ _SIGS = {
     "fsim_channel_create": [C.POINTER(RECON_PARAMS), _P, C.POINTER(_P)],
     "fsim_channel_destroy": [_P],
     "fsim_reconstruct": [_P, _P, _P, _I, _P],
+    "fsim_stage_timing": [_I],
+    "fsim_stage_times": [_PD, _PI, _I],
 }
 _RET = {
     "fsim_last_error": (C.c_char_p, []),

@@ -211,6 +211,15 @@ int fsim_channel_destroy(fsim_channel ch);
 int fsim_reconstruct(fsim_channel ch, const uint16_t *raw_dev, float *out_dev,
                      int count, void *stream);
 
+/* Stage timing of the fused pipeline (no counterpart in the reference, which
+ * only has Tool.Timer wall clocks, utils/Tool.java:210-245): while enabled,
+ * every fsim_reconstruct records HIP events around its kernels on its stream;
+ * fsim_stage_times waits for them and returns the summed milliseconds and
+ * launch counts of stages 0..4 = band separation + row FFT, shifted column
+ * pass, shifted row pass, gather + filter + column IFFT, row IFFT. */
+int fsim_stage_timing(int enable);
+int fsim_stage_times(double *ms, int *launches, int nstage);
+
 #ifdef __cplusplus
 }
 #endif
