@@ -47,6 +47,20 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q))
 
 
+def probe_copy(reps):
+    """y += x over two 64 MiB float vectors: 128 MiB read + 64 MiB written
+    per launch (k_add_f), the known-size kernel that calibrates FETCH_SIZE /
+    WRITE_SIZE."""
+    from fsim.linalg import Vec2dReal
+    from fsim import _lib
+    a, b = Vec2dReal(4096, 4096), Vec2dReal(4096, 4096)
+    for _ in range(reps):
+        a.add(b)
+    _lib.call("fsim_sync")
+    print(json.dumps({"probe_copy": True, "launches": reps, "read_bytes": 2 * 4096 * 4096 * 4,
+                      "write_bytes": 4096 * 4096 * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +72,10 @@ def main():
     ap.add_argument("--latency-runs", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the timed batched loop (for rocprofv3 runs)")
+    ap.add_argument("--probe-copy", action="store_true",
+                    help="run a plain streaming kernel of known size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -71,6 +89,9 @@ def main():
 
     n, B = args.n, args.batch
     _lib.init(0)
+    if args.probe_copy:
+        probe_copy(args.steps)
+        return
     log(f"generating synthetic stacks n={n}")
     stacks, sps = [], []
     for s in range(4):
@@ -83,6 +104,19 @@ def main():
     raw_buf = DeviceBuffer(frame_stack.nbytes)
     raw_buf.upload(frame_stack)
     out_buf = DeviceBuffer(B * rec.out_bytes())
+
+    if args.profile_only:
+        for _ in range(args.warmup):
+            rec.reconstruct(raw_buf.value, out_buf.value, B)
+        _lib.call("fsim_sync")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rec.reconstruct(raw_buf.value, out_buf.value, B)
+        _lib.call("fsim_sync")
+        el = time.perf_counter() - t0
+        print(json.dumps({"profile_only": True, "batch": B, "steps": args.steps,
+                          "recons_per_s": round(B * args.steps / el, 2)}), flush=True)
+        return
 
     # parity of the measured path on stack 0
     from oracle import fp64, ref

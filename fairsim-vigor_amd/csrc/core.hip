@@ -150,6 +150,16 @@ int fsim_vec_shape(fsim_vec vh, int *is_cplx, int *w, int *h) {
     return FSIM_OK;
 }
 
+int fsim_vec_reshape(fsim_vec vh, int w, int h) {
+    if (!vh) return fail(FSIM_ERR_ARG, "null vector");
+    Vec *v = reinterpret_cast<Vec *>(vh);
+    if (w <= 0 || h <= 0 || (size_t)w * h != (size_t)v->w * v->h)
+        return fail(FSIM_ERR_ARG, "reshape: element count mismatch");
+    v->w = w;
+    v->h = h;
+    return FSIM_OK;
+}
+
 void *fsim_vec_device_ptr(fsim_vec vh) { return vh ? reinterpret_cast<Vec *>(vh)->d : nullptr; }
 
 int fsim_vec_upload(fsim_vec vh, const float *host) {

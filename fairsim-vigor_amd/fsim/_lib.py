@@ -8,7 +8,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfsim.so")
+# FSIM_LIBRARY selects another build of the same library (e.g. a variant
+# compiled with different tile parameters); default: the in-tree build.
+LIB_PATH = os.environ.get("FSIM_LIBRARY") or os.path.join(_HERE, "libfsim.so")
 
 FSIM_OK = 0
 FSIM_ERR_ARG = -1
@@ -71,6 +73,7 @@ _SIGS = {
     "fsim_vec_create": [_I, _I, _I, C.POINTER(_P)],
     "fsim_vec_destroy": [_P],
     "fsim_vec_shape": [_P, _PI, _PI, _PI],
+    "fsim_vec_reshape": [_P, _I, _I],
     "fsim_vec_upload": [_P, _P],
     "fsim_vec_download": [_P, _P],
     "fsim_vec_zero": [_P],

@@ -67,6 +67,9 @@ int fsim_memcpy(void *dst, const void *src, int64_t bytes, int kind);
 int fsim_vec_create(int is_cplx, int w, int h, fsim_vec *out);
 int fsim_vec_destroy(fsim_vec v);
 int fsim_vec_shape(fsim_vec v, int *is_cplx, int *w, int *h);
+/* Reinterpret the 2-D shape (w*h must equal the element count): the Java side
+ * allocates by element count (alloc(vf, n)) and passes w, h with 2-D ops. */
+int fsim_vec_reshape(fsim_vec v, int w, int h);
 void *fsim_vec_device_ptr(fsim_vec v);
 int fsim_vec_upload(fsim_vec v, const float *host);
 int fsim_vec_download(fsim_vec v, float *host);

@@ -94,7 +94,7 @@ def test_fourier_shift_integer_is_roll(vf):
 @pytest.mark.parametrize("wi,wo,xo,yo", [(64, 128, 0, 0), (64, 128, 23, -17), (512, 1024, 185, -186),
                                           (512, 1024, -186, 185), (32, 96, 40, 5), (16, 32, -32, 31)])
 def test_paste_freq_bit_exact(vf, wi, wo, xo, yo):
-    rng = np.random.default_rng(wi + xo)
+    rng = np.random.default_rng(abs(wi + xo))
     z = rnd_c(rng, wi, wi)
     vi = cvec(vf, z)
     vo = vf.createCplx2D(wo, wo)
