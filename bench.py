@@ -61,6 +61,93 @@ def probe_copy(reps):
                       "write_bytes": 4096 * 4096 * 4}), flush=True)
 
 
+def serve_saturation(rec, raw_buf, out_buf, B, raw_bytes, out_bytes, nreq):
+    """Closed loop: keep up to B requests in flight on the server, stacks and
+    outputs resident in HBM; throughput and submit-to-done latency."""
+    from fsim.livemode import ReconstructionServer
+    srv = ReconstructionServer(mem_budget=16 << 30, max_batch=B // 2, max_queue=4 * B, nslots=4)
+    cid = srv.add_channel(rec)
+    inflight = []
+    for i in range(min(64, nreq)):                      # warm the slots
+        inflight.append(srv.submit(cid, raw_buf.value + (i % B) * raw_bytes,
+                                   out_buf.value + (i % B) * out_bytes, device=True))
+    for t in inflight:
+        srv.wait(t)
+    srv.stats(reset=True)
+    inflight = []
+    t0 = time.perf_counter()
+    for i in range(nreq):
+        if len(inflight) >= B:
+            srv.wait(inflight.pop(0))
+        inflight.append(srv.submit(cid, raw_buf.value + (i % B) * raw_bytes,
+                                   out_buf.value + (i % B) * out_bytes, device=True))
+    for t in inflight:
+        srv.wait(t)
+    el = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return {"form": "server, closed loop, 512^2 stacks resident in HBM, up to %d in flight" % B,
+            "recons_per_s": round(nreq / el, 2), "p50_ms": round(st["p50_ms"], 3),
+            "p99_ms": round(st["p99_ms"], 3), "mean_batch": round(st["mean_batch"], 2)}
+
+
+def serve_mixed_stream(nreq, rate):
+    """Open loop: Poisson arrivals at `rate` req/s, sizes {256: .5, 512: .35,
+    1024: .15}, 3 channels (525/600/670 nm), pinned host buffers (so the
+    latency includes both PCIe copies)."""
+    import numpy as np
+    from fsim import synthetic
+    from fsim.livemode import ReconstructionServer, pinned
+    from fsim.sim_algorithm import Reconstructor
+    rng = np.random.default_rng(1234)
+    sizes, probs = [256, 512, 1024], [0.5, 0.35, 0.15]
+    lams = [525.0, 600.0, 670.0]
+    srv = ReconstructionServer(mem_budget=16 << 30, max_batch=16, max_queue=4096, nslots=4)
+    chans, raws, outs = {}, {}, {}
+    for n in sizes:
+        for ci, lam in enumerate(lams):
+            raw, sp = synthetic.sim_stack(n=n, seed=100 + ci, lam=lam)
+            chans[(n, ci)] = Reconstructor(synthetic.make_param(sp, n, lam=lam))
+            buf, _ = pinned(raw.shape, np.uint16)
+            buf[...] = raw
+            raws[(n, ci)] = buf
+    ids = {k: srv.add_channel(v) for k, v in chans.items()}
+    pool = {n: [pinned((2 * n, 2 * n), np.float32)[0] for _ in range(8)] for n in sizes}
+    gaps = rng.exponential(1.0 / rate, nreq)
+    kinds = [(int(rng.choice(sizes, p=probs)), int(rng.integers(3))) for _ in range(nreq)]
+    tickets = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for i, k in enumerate(kinds):
+        t_next += gaps[i]
+        while time.perf_counter() < t_next:
+            pass
+        o = pool[k[0]][i % 8]
+        tickets.append(srv.submit(ids[k], raws[k], o))
+    for t in tickets:
+        srv.wait(t)
+    el = time.perf_counter() - t0
+    st = srv.stats(reset=True)
+    # saturation: the same mix submitted as fast as the server admits it
+    t0 = time.perf_counter()
+    tickets = []
+    for i, k in enumerate(kinds):
+        tickets.append(srv.submit(ids[k], raws[k], pool[k[0]][i % 8]))
+    for t in tickets:
+        srv.wait(t)
+    sat_el = time.perf_counter() - t0
+    st2 = srv.stats()
+    srv.close()
+    mean_n2 = sum(p * n * n for n, p in zip(sizes, probs))
+    return {"arrivals": f"Poisson {rate:g} req/s, {nreq} requests, sizes 256/512/1024 at .5/.35/.15, "
+                        "3 channels, pinned host buffers",
+            "offered_req_per_s": round(nreq / el, 1), "p50_ms": round(st["p50_ms"], 3),
+            "p99_ms": round(st["p99_ms"], 3), "mean_batch": round(st["mean_batch"], 2),
+            "saturation_req_per_s": round(nreq / sat_el, 1),
+            "saturation_p50_ms": round(st2["p50_ms"], 3), "saturation_p99_ms": round(st2["p99_ms"], 3),
+            "saturation_roofline_frac_pcie_inclusive": round(nreq / sat_el * 434 * mean_n2 / 8e12, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +159,10 @@ def main():
     ap.add_argument("--latency-runs", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--served-requests", type=int, default=2048)
+    ap.add_argument("--mix-requests", type=int, default=400)
+    ap.add_argument("--mix-rate", type=float, default=200.0)
+    ap.add_argument("--no-mix", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed batched loop (for rocprofv3 runs)")
     ap.add_argument("--probe-copy", action="store_true",
@@ -193,6 +284,15 @@ def main():
     plain_value = 1000.0 / float(np.mean(plat))
     plain_err = fp64.rel_l2(pipe.run(raw_buf.value).vectorData(), r32)
 
+    # served form: continuous batching through the request server
+    log("served: 512^2 closed loop, device-resident stacks")
+    served = serve_saturation(rec, raw_buf, out_buf, B, stacks[0].nbytes, rec.out_bytes(),
+                              args.served_requests)
+    log(f"served saturation {served['recons_per_s']:.1f} recons/s")
+    if not args.no_mix:
+        log("served: open-loop mixed stream at 200 req/s")
+        served["mixed_stream"] = serve_mixed_stream(args.mix_requests, args.mix_rate)
+
     # CPU baseline: the oracle restatement on the host cores, bounded sample
     cpu = None
     if not args.no_cpu:
@@ -233,7 +333,9 @@ def main():
         "latency_batch_ms": {"p50": round(pct(blat, 50), 3), "p99": round(pct(blat, 99), 3)},
         "plain": {"recons_per_s": round(plain_value, 2), "p50_ms": round(pct(plat, 50), 3),
                   "p99_ms": round(pct(plat, 99), 3), "rel_l2_vs_oracle": plain_err},
+        "served": served,
         "speedup_vs_plain": round(value / plain_value, 2),
+        "served_speedup_vs_plain": round(served["recons_per_s"] / plain_value, 2),
         "rel_l2_vs_oracle": parity,
         "cpu_baseline": cpu,
     }

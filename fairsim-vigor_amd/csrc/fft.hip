@@ -15,7 +15,8 @@ __global__ void __launch_bounds__(256) k_fft_rows(float2 *__restrict__ data, int
                                                   const float2 *__restrict__ tw, float scale) {
     constexpr int E = FftE<L>::value, T = L / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
-    __shared__ float2 lds[RW * L];
+    constexpr int LP = FftLds<L, E>::len;
+    __shared__ float2 lds[RW * LP];
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int row = blockIdx.x * RW + rr;
     const bool active = row < nrows;
@@ -25,7 +26,7 @@ __global__ void __launch_bounds__(256) k_fft_rows(float2 *__restrict__ data, int
 #pragma unroll
         for (int m = 0; m < E; m++) v[m] = p[t + T * m];
     }
-    fft_regs<L, E, INV>(v, lds + rr * L, 1, t, tw, active);
+    fft_regs<L, E, INV, RowWave<L>::value>(v, lds + rr * LP, 1, t, tw, active);
     if (active) {
 #pragma unroll
         for (int m = 0; m < E; m++) p[t + T * m] = INV ? cscale(v[m], scale) : v[m];
@@ -38,7 +39,7 @@ __global__ void __launch_bounds__(1024) k_fft_cols(float2 *__restrict__ data, in
                                                    const float2 *__restrict__ tw, float scale) {
     constexpr int E = FftE<L>::value, T = L / E;
     constexpr int CW = ColW<L>::value;
-    __shared__ float2 lds[L * CW];
+    __shared__ float2 lds[FftLds<L, E>::len * CW];
     const int c = threadIdx.x % CW, t = threadIdx.x / CW;
     const int col = blockIdx.x * CW + c;
     const bool active = col < ncols;

@@ -58,6 +58,21 @@ __device__ __forceinline__ float2 tw16(float2 d) {
     }
 }
 
+// Synchronisation between Stockham passes.  When one transform lives inside a
+// single wavefront (T <= 64, transforms packed along the wave), the LDS
+// exchange only needs the wave's own LDS operations to be complete and
+// ordered; otherwise a block barrier.
+template <bool WAVE>
+__device__ __forceinline__ void pass_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
 constexpr int bitrevc(int v, int bits) {
     int r = 0;
@@ -102,22 +117,57 @@ __device__ __forceinline__ void dft_reg(float2 (&v)[R]) {
     }
 }
 
-// Stockham pass NS (sub-transform size so far) of radix R = min(E, L/NS);
+// Elements per thread used for an L-point transform.
+template <int L>
+struct FftE {
+    static constexpr int value = L >= 1024 ? 16 : (L >= 8 ? 8 : L);
+};
+
+// Elements per thread in column kernels: 32 from L = 1024 up, so that the
+// 16-column tiles fit 512-thread blocks (256 VGPRs per thread available).
+template <int L>
+struct ColE {
+    static constexpr int value = L >= 1024 ? 32 : FftE<L>::value;
+};
+
+// LDS layout of a transform: element i lives at lds[pad(i) * ls] with one
+// spare slot every RE = min(E, 16) elements.  Stockham writes go out at stride
+// R between threads; the padding makes consecutive threads land in different
+// banks (stride RE + 1, odd), while the unit-stride reads stay conflict free.
+template <int L, int E>
+struct FftLds {
+    static constexpr int re = E > 16 ? 16 : E;
+    static constexpr int shift = re >= 16 ? 4 : re >= 8 ? 3 : re >= 4 ? 2 : re >= 2 ? 1 : 0;
+    static constexpr int len = L + (shift ? (L >> shift) : 0);   // padded length
+    static __device__ __forceinline__ int pad(int i) { return shift ? i + (i >> shift) : i; }
+};
+
+// Stockham pass NS (sub-transform size so far) of radix R = min(E, 16, L/NS);
 // recursion walks the passes at compile time.
-template <int L, int E, int NS, bool INV, bool FIRST>
+template <int L, int E, int NS, bool INV, bool FIRST, bool WAVE>
 __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, int ls, int t,
                                          const float2 *__restrict__ tw, bool active) {
     constexpr int REM = L / NS;
-    constexpr int R = REM >= E ? E : REM;
+    constexpr int RE = E > 16 ? 16 : E;  // in-register DFTs go up to radix 16
+    constexpr int R = REM >= RE ? RE : REM;
     constexpr bool LAST = (NS * R == L);
     constexpr int T = L / E;
     constexpr int Q = E / R;
+    using PL = FftLds<L, E>;
     if constexpr (!FIRST) {
         if (active) {
+            if constexpr (PL::shift == 0 || (T % PL::re) == 0) {
+                // pad(t + T m) = pad(t) + m (T + T / RE) when RE divides T
+                constexpr int TP = T + (PL::shift ? (T >> PL::shift) : 0);
+                const int pt = PL::pad(t);
 #pragma unroll
-            for (int m = 0; m < E; m++) v[m] = lds[(t + T * m) * ls];
+                for (int m = 0; m < E; m++) v[m] = lds[(pt + TP * m) * ls];
+            } else {
+#pragma unroll
+                for (int m = 0; m < E; m++) v[m] = lds[PL::pad(t + T * m) * ls];
+            }
         }
-        __syncthreads();
+        pass_sync<WAVE>();
     }
     if (active) {
 #pragma unroll
@@ -141,33 +191,43 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, int ls, in
 #pragma unroll
                 for (int r = 0; r < R; r++) v[q + r * Q] = u[r];
             } else {
-                const int base = (j - k) * R + k;
+                // pad(base + r NS) = pad(base) + r (NS + NS / RE): NS is 1 or a
+                // multiple of RE, and for NS = 1 base is a multiple of R = RE
+                const int pb = PL::pad((j - k) * R + k);
+                constexpr int NSP = NS + (PL::shift ? (NS >> PL::shift) : 0);
 #pragma unroll
-                for (int r = 0; r < R; r++) lds[(base + r * NS) * ls] = u[r];
+                for (int r = 0; r < R; r++) lds[(pb + r * NSP) * ls] = u[r];
             }
         }
     }
     if constexpr (!LAST) {
-        __syncthreads();
-        fft_pass<L, E, NS * R, INV, false>(v, lds, ls, t, tw, active);
+        pass_sync<WAVE>();
+        fft_pass<L, E, NS * R, INV, false, WAVE>(v, lds, ls, t, tw, active);
     }
 }
 
 // Full L-point FFT on registers.  Every thread of the block must call it
-// (it contains block barriers); threads with active == false only join the
-// barriers.  lds: this transform's region, element i at lds[i * ls].
-// On return the LDS region may be reused after a __syncthreads().
-template <int L, int E, bool INV>
+// (it contains barriers); threads with active == false only join the
+// barriers.  lds: this transform's region of FftLds<L, E>::len elements
+// (times ls), element i at lds[FftLds<L, E>::pad(i) * ls].
+// WAVE selects wavefront-level synchronisation, valid when every transform's
+// T = L/E threads lie inside one wavefront; the caller's own later LDS use
+// of the region is then ordered by the same wave.  On return the region may
+// be reused (the last pass ends with a synchronisation after its reads).
+template <int L, int E, bool INV, bool WAVE = false>
 __device__ __forceinline__ void fft_regs(float2 (&v)[E], float2 *lds, int ls, int t,
                                          const float2 *__restrict__ tw, bool active = true) {
     static_assert(L % E == 0, "E must divide L");
-    fft_pass<L, E, 1, INV, true>(v, lds, ls, t, tw, active);
+    static_assert(!WAVE || (L / E) <= 64, "wave synchronisation needs the transform inside one wavefront");
+    fft_pass<L, E, 1, INV, true, WAVE>(v, lds, ls, t, tw, active);
 }
 
-// Elements per thread used for an L-point transform.
+// Row layout (t fastest): one transform per T consecutive threads.
 template <int L>
-struct FftE {
-    static constexpr int value = L >= 1024 ? 16 : (L >= 8 ? 8 : L);
+struct RowWave {
+    static constexpr bool value = (L / FftE<L>::value) <= 64;
 };
+
+
 
 }  // namespace fsim

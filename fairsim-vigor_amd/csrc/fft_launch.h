@@ -11,4 +11,15 @@ struct ColW {
     static constexpr int value = v0 >= 16 ? 16 : (v0 < 1 ? 1 : v0);
 };
 
+// Tile width for a column kernel whose transform uses T threads and E elements
+// per thread: as ColW, capped so that the block stays within 1024 threads
+// (512 when E >= 32, which needs more than 128 VGPRs per thread).
+template <int L, int T, int E>
+struct ColTile {
+    static constexpr int maxthreads = E >= 32 ? 512 : 1024;
+    static constexpr int v0 = ColW<L>::value;
+    static constexpr int v1 = maxthreads / T;
+    static constexpr int value = v0 < v1 ? v0 : v1;
+};
+
 }  // namespace fsim

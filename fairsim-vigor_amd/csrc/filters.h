@@ -28,6 +28,9 @@ int wiener_device(float *out, int w, int h, const Otf &o, double vec_cpm, const 
 void fade_factors_host(int n, int px, double *lo, double *hi);
 int separation_matrix_host(const double *phases, int nphase, int nband, const double *fac, double *out);
 int reduce_device(const float *d, size_t nfloat, int mode, double2 *out, hipStream_t s);
+// otfToVector(write) of one band table into a w x h complex buffer
+int otf_table_device(float2 *out, int w, int h, const float2 *tab, int ns, double cpm, double cutoff,
+                     double vec_cpm, hipStream_t s);
 // writeApoVector into a real buffer (OtfProvider.java:469-492)
 int write_apo_device(float *out, int w, int h, double vec_cpm, double cutoff, double bend, double apo_cut,
                      hipStream_t s);

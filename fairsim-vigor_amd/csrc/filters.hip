@@ -320,6 +320,14 @@ int wiener_device(float *out, int w, int h, const Otf &o, double vec_cpm, const 
     return FSIM_OK;
 }
 
+int otf_table_device(float2 *out, int w, int h, const float2 *tab, int ns, double cpm, double cutoff,
+                     double vec_cpm, hipStream_t s) {
+    hipLaunchKernelGGL(k_otf_to_vector, grid_for((size_t)w * h), dim3(256), 0, s, out, w, h, tab, ns, cpm, cutoff,
+                       0.0, 0.0, vec_cpm, 1);
+    FSIM_HIP(hipGetLastError());
+    return FSIM_OK;
+}
+
 int write_apo_device(float *out, int w, int h, double vec_cpm, double cutoff, double bend, double apo_cut,
                      hipStream_t s) {
     hipLaunchKernelGGL(k_apo, grid_for((size_t)w * h), dim3(256), 0, s, (void *)out, 0, w, h, vec_cpm, cutoff, bend,

@@ -13,6 +13,7 @@ struct ReconConst {
     int ns, otf_nb;
     double otf_cpm, cutoff, vec_cpm;
     const float2 *otf_tab;            // vals or valsAtt, otf_nb bands x ns
+    const float2 *otf2d;              // otfToVector(band, 0, 0) on the n^2 grid, per OTF band
     const float2 *wa;                 // (2n)^2 of (1/Wiener denominator, apodization)
     const double *fade_lo, *fade_hi;  // fadeBorderCos factors, n each
     float2 S[kMaxDir][kMaxComp][kMaxPhase];   // separation matrices (toFlt)
@@ -41,4 +42,10 @@ int launch_pipeline(int n, const JobList &jl, int njobs, int D, int C, hipStream
 size_t workspace_a_elems(int n, int D, int C);
 size_t workspace_b_elems(int n);
 
+}  // namespace fsim
+
+struct fsim_channel_s;
+namespace fsim {
+const ReconConst *channel_device_const(const fsim_channel_s *ch);
+const ReconConst &channel_host_const(const fsim_channel_s *ch);
 }  // namespace fsim

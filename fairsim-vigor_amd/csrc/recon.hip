@@ -66,7 +66,8 @@ template <int N>
 __global__ void __launch_bounds__(256) k_sep_rows(JobList jl, int ngroups, const float2 *__restrict__ tw) {
     constexpr int E = FftE<N>::value, T = N / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
-    __shared__ float2 lds[RW * N];
+    constexpr int LP = FftLds<N, E>::len;
+    __shared__ float2 lds[RW * LP];
     const Job J = get_job(jl, blockIdx.y);
     const ReconConst &K = *J.c;
     const int C = K.C, P = K.P;
@@ -83,23 +84,20 @@ __global__ void __launch_bounds__(256) k_sep_rows(JobList jl, int ngroups, const
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
-    float fy0 = 1.0f;
-    fy0 = (float)((double)fy0 * K.fade_lo[y]);
-    fy0 = (float)((double)fy0 * K.fade_hi[y]);
+    float fade[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) fade[m] = fade_at(K.fade_lo, K.fade_hi, t + T * m, y);
     for (int p = 0; p < P; p++) {
         const float2 s = K.S[d][c][p];
         const uint16_t *rp = raw + (size_t)p * nn;
 #pragma unroll
         for (int m = 0; m < E; m++) {
-            const int x = t + T * m;
-            float f = (float)((double)fy0 * K.fade_lo[x]);
-            f = (float)((double)f * K.fade_hi[x]);
-            const float px = (float)rp[x] * f;
+            const float px = (float)rp[t + T * m] * fade[m];
             v[m].x += s.x * px;
             v[m].y += s.y * px;
         }
     }
-    fft_regs<N, E, false>(v, lds + rr * N, 1, t, tw);
+    fft_regs<N, E, false, RowWave<N>::value>(v, lds + rr * LP, 1, t, tw);
     float2 *out = J.A + ((size_t)(d * C + c) * N + y) * N;
 #pragma unroll
     for (int m = 0; m < E; m++) out[t + T * m] = v[m];
@@ -109,11 +107,17 @@ __global__ void __launch_bounds__(256) k_sep_rows(JobList jl, int ngroups, const
 // K2: shifted components: column FFT, * conj(OTF_band), column IFFT (1/n).
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ void __launch_bounds__(1024) k_col_shift(JobList jl, const float2 *__restrict__ tw) {
-    constexpr int E = FftE<N>::value, T = N / E;
-    constexpr int CW = ColW<N>::value;
+struct ColShiftBlock {
+    static constexpr int e = ColE<N>::value, t = N / e;
+    static constexpr int value = t * ColTile<N, t, e>::value;
+};
+
+template <int N>
+__global__ void __launch_bounds__(ColShiftBlock<N>::value) k_col_shift(JobList jl, const float2 *__restrict__ tw) {
+    constexpr int E = ColE<N>::value, T = N / E;
+    constexpr int CW = ColTile<N, T, E>::value;
     constexpr int tiles = N / CW;
-    __shared__ float2 lds[N * CW];
+    __shared__ float2 lds[FftLds<N, E>::len * CW];
     const Job J = get_job(jl, blockIdx.y);
     const ReconConst &K = *J.c;
     const int C = K.C, nsh = C - 1;
@@ -127,13 +131,9 @@ __global__ void __launch_bounds__(1024) k_col_shift(JobList jl, const float2 *__
     for (int m = 0; m < E; m++) v[m] = col[(size_t)(t + T * m) * N];
     fft_regs<N, E, false>(v, lds + cl, CW, t, tw);
     const int band = (c + 1) / 2;
-    const float2 *tab = K.otf_tab + (size_t)(K.otf_nb > 1 ? band : 0) * K.ns;
+    const float2 *otf = K.otf2d + (size_t)(K.otf_nb > 1 ? band : 0) * N * N + x;
 #pragma unroll
-    for (int m = 0; m < E; m++) {
-        const int y = t + T * m;
-        const float2 o = otf_pixel(tab, K.ns, K.otf_cpm, K.cutoff, x, y, N, N, 0, 0, K.vec_cpm);
-        v[m] = times_conj_java(v[m], o);
-    }
+    for (int m = 0; m < E; m++) v[m] = times_conj_java(v[m], otf[(size_t)(t + T * m) * N]);
     fft_regs<N, E, true>(v, lds + cl, CW, t, tw);
     const float sc = 1.0f / N;
 #pragma unroll
@@ -148,7 +148,8 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
     constexpr int E = FftE<N>::value, T = N / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int rblocks = N / RW;
-    __shared__ float2 lds[RW * N];
+    constexpr int LP = FftLds<N, E>::len;
+    __shared__ float2 lds[RW * LP];
     const Job J = get_job(jl, blockIdx.y);
     const ReconConst &K = *J.c;
     const int C = K.C, nsh = C - 1;
@@ -160,7 +161,7 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = row[t + T * m];
-    fft_regs<N, E, true>(v, lds + rr * N, 1, t, tw);
+    fft_regs<N, E, true, RowWave<N>::value>(v, lds + rr * LP, 1, t, tw);
     const float sc = 1.0f / N;
     const double fx = K.fx[d][c], fy = K.fy[d][c];
 #pragma unroll
@@ -169,27 +170,46 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
         shift_cos_sin(fx, fy, t + T * m, y, N, co, si);
         v[m] = rot_java(cscale(v[m], sc), co, si);
     }
-    fft_regs<N, E, false>(v, lds + rr * N, 1, t, tw);
+    fft_regs<N, E, false, RowWave<N>::value>(v, lds + rr * LP, 1, t, tw);
 #pragma unroll
     for (int m = 0; m < E; m++) row[t + T * m] = v[m];
 }
 
 // ---------------------------------------------------------------------------
 // K5: gather into the doubled grid, filter, column IFFT.
+// Elements per thread for the n-point (en) and 2n-point (em) column
+// transforms and the tile width: up to n = 512 a 1024-thread block with 16
+// elements per thread fits 128 VGPRs; from n = 1024 on, 32 elements per thread
+// in 512-thread blocks.
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ void __launch_bounds__(1024) k_gather(JobList jl, const float2 *__restrict__ twn,
+struct GatherE {
+    static constexpr int en = N >= 1024 ? 32 : FftE<N>::value;
+    static constexpr int em = N >= 1024 ? 32 : FftE<2 * N>::value;
+    static constexpr int tn = N / en, tm = 2 * N / em;
+    static constexpr int tk = tn > tm ? tn : tm;
+    static constexpr int cw = ColTile<2 * N, tk, (en > em ? en : em)>::value;
+};
+template <int N>
+__global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw) k_gather(JobList jl, const float2 *__restrict__ twn,
                                                  const float2 *__restrict__ twm) {
     constexpr int M = 2 * N;
-    constexpr int EN = FftE<N>::value, TN = N / EN;
-    constexpr int EM = FftE<M>::value, TM = M / EM;
-    constexpr int CW = ColW<M>::value;
-    __shared__ float2 lds[M * CW];
+    constexpr int EN = GatherE<N>::en, TN = N / EN;
+    constexpr int EM = GatherE<N>::em, TM = M / EM;
+    constexpr int CW = GatherE<N>::cw;
+    constexpr int LPN = FftLds<N, EN>::len, LPM = FftLds<M, EM>::len;
+    constexpr int LPK = LPM > LPN ? LPM : LPN;
+    __shared__ float2 lds[(LPK > N ? LPK : N) * CW];
     const Job J = get_job(jl, blockIdx.y);
     const ReconConst &K = *J.c;
     const int D = K.D, C = K.C;
     const int cl = threadIdx.x % CW, t = threadIdx.x / CW;
-    const int xo = blockIdx.x * CW + cl;
+    // blocks are dealt to the 8 XCDs round robin; give each XCD a contiguous
+    // run of column tiles so that source rows shared by neighbouring tiles
+    // (paste offsets are not tile aligned) are fetched once into its L2
+    constexpr int tiles = M / CW;
+    const int tile = (blockIdx.x % 8) * (tiles / 8) + blockIdx.x / 8;
+    const int xo = tile * CW + cl;
     float2 acc[EM];
 #pragma unroll
     for (int m = 0; m < EM; m++) acc[m] = make_float2(0.f, 0.f);
@@ -207,13 +227,9 @@ __global__ void __launch_bounds__(1024) k_gather(JobList jl, const float2 *__res
             fft_regs<N, EN, false>(v, lds + cl, CW, t, twn, act);
             if (act) {
                 if (c == 0) {
-                    const float2 *tab = K.otf_tab;
+                    const float2 *otf = K.otf2d + xs;
 #pragma unroll
-                    for (int m = 0; m < EN; m++) {
-                        const float2 o =
-                            otf_pixel(tab, K.ns, K.otf_cpm, K.cutoff, xs, t + TN * m, N, N, 0, 0, K.vec_cpm);
-                        v[m] = times_conj_java(v[m], o);
-                    }
+                    for (int m = 0; m < EN; m++) v[m] = times_conj_java(v[m], otf[(size_t)(t + TN * m) * N]);
                 }
 #pragma unroll
                 for (int m = 0; m < EN; m++) lds[(t + TN * m) * CW + cl] = v[m];
@@ -258,7 +274,8 @@ template <int M>
 __global__ void __launch_bounds__(256) k_row_real(JobList jl, const float2 *__restrict__ tw) {
     constexpr int E = FftE<M>::value, T = M / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
-    __shared__ float2 lds[RW * M];
+    constexpr int LP = FftLds<M, E>::len;
+    __shared__ float2 lds[RW * LP];
     const Job J = get_job(jl, blockIdx.y);
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int y = blockIdx.x * RW + rr;
@@ -266,7 +283,7 @@ __global__ void __launch_bounds__(256) k_row_real(JobList jl, const float2 *__re
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = row[t + T * m];
-    fft_regs<M, E, true>(v, lds + rr * M, 1, t, tw);
+    fft_regs<M, E, true, RowWave<M>::value>(v, lds + rr * LP, 1, t, tw);
     const float sc = 1.0f / M;
     float *o = J.out + (size_t)y * M;
 #pragma unroll
@@ -316,17 +333,18 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, hipStre
         stage_mark(0, s, ev);
     }
     if (C > 1) {
-        constexpr int E = FftE<N>::value, T = N / E, CW = ColW<N>::value;
-        hipLaunchKernelGGL(k_col_shift<N>, dim3(D * (C - 1) * (N / CW), njobs), dim3(T * CW), 0, s, jl, twn);
-        stage_mark(1, s, ev);
+        {
+            constexpr int E = ColE<N>::value, T = N / E, CW = ColTile<N, T, E>::value;
+            hipLaunchKernelGGL(k_col_shift<N>, dim3(D * (C - 1) * (N / CW), njobs), dim3(T * CW), 0, s, jl, twn);
+            stage_mark(1, s, ev);
+        }
+        constexpr int E = FftE<N>::value, T = N / E;
         constexpr int RW = T >= 256 ? 1 : 256 / T;
         hipLaunchKernelGGL(k_row_shift<N>, dim3(D * (C - 1) * (N / RW), njobs), dim3(T * RW), 0, s, jl, twn);
         stage_mark(2, s, ev);
     }
     {
-        constexpr int EN = FftE<N>::value, TN = N / EN, EM = FftE<M>::value, TM = M / EM;
-        constexpr int TK = TN > TM ? TN : TM;
-        constexpr int CW = ColW<M>::value;
+        constexpr int TK = GatherE<N>::tk, CW = GatherE<N>::cw;
         hipLaunchKernelGGL(k_gather<N>, dim3(M / CW, njobs), dim3(TK * CW), 0, s, jl, twn, twm);
         stage_mark(3, s, ev);
     }
@@ -424,14 +442,19 @@ struct fsim_channel_s {
     ReconConst K;
     ReconConst *dK = nullptr;
     float2 *wa = nullptr;
+    float2 *otf2d = nullptr;
     double *fade = nullptr;
     float2 *A = nullptr, *B = nullptr;
     int cap = 0;
 };
 
+const ReconConst *fsim::channel_device_const(const fsim_channel_s *ch) { return ch->dK; }
+const ReconConst &fsim::channel_host_const(const fsim_channel_s *ch) { return ch->K; }
+
 static void channel_free(fsim_channel_s *ch) {
     (void)hipFree(ch->dK);
     (void)hipFree(ch->wa);
+    (void)hipFree(ch->otf2d);
     (void)hipFree(ch->fade);
     (void)hipFree(ch->A);
     (void)hipFree(ch->B);
@@ -455,6 +478,7 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     const size_t mm = (size_t)M * M;
     float *wtmp = nullptr, *atmp = nullptr;
     if (hipMalloc(&ch->dK, sizeof(ReconConst)) != hipSuccess || hipMalloc(&ch->wa, mm * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&ch->otf2d, (size_t)o.nband * n * n * sizeof(float2)) != hipSuccess ||
         hipMalloc(&ch->fade, 2 * n * sizeof(double)) != hipSuccess ||
         hipMalloc(&wtmp, mm * sizeof(float)) != hipSuccess || hipMalloc(&atmp, mm * sizeof(float)) != hipSuccess) {
         (void)hipFree(wtmp);
@@ -470,6 +494,18 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     ch->K.fade_lo = ch->fade;
     ch->K.fade_hi = ch->fade + n;
     ch->K.wa = ch->wa;
+    {
+        // otfToVector(band, 0, 0, write) per OTF band on the n^2 grid
+        const int nb = o.nband;
+        const size_t nn = (size_t)n * n;
+        const float2 *tab = p->use_att ? o.vals_att : o.vals;
+        for (int b = 0; b < nb; b++) {
+            rc = otf_table_device(ch->otf2d + b * nn, n, n, tab + (size_t)b * o.ns, o.ns, o.cpm, o.cutoff,
+                                  p->vec_cpm, s);
+            if (rc) return rc;
+        }
+        ch->K.otf2d = ch->otf2d;
+    }
     ShiftSet sh{};
     sh.ndir = p->ndir;
     sh.nband = p->nband;

@@ -52,6 +52,22 @@ class RECON_PARAMS(C.Structure):
     ]
 
 
+class SERVER_CONFIG(C.Structure):
+    """fsim_server_config (include/fsim.h)."""
+    _fields_ = [("mem_budget", C.c_int64), ("max_batch", C.c_int), ("max_queue", C.c_int),
+                ("nslots", C.c_int)]
+
+
+class SERVER_STATS(C.Structure):
+    """fsim_server_stats (include/fsim.h)."""
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
+                ("failed", C.c_int64), ("batches", C.c_int64), ("pending", C.c_int64),
+                ("device_bytes", C.c_int64), ("mean_batch", C.c_double), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("max_ms", C.c_double)]
+
+
+FSIM_SUBMIT_DEVICE = 1
+
 _P = C.c_void_p
 _I = C.c_int
 _D = C.c_double
@@ -107,6 +123,12 @@ _SIGS = {
     "fsim_channel_create": [C.POINTER(RECON_PARAMS), _P, C.POINTER(_P)],
     "fsim_channel_destroy": [_P],
     "fsim_reconstruct": [_P, _P, _P, _I, _P],
+    "fsim_server_create": [C.POINTER(SERVER_CONFIG), C.POINTER(_P)],
+    "fsim_server_destroy": [_P],
+    "fsim_server_add_channel": [_P, _P, _PI],
+    "fsim_server_submit": [_P, _I, _P, _P, _I, C.POINTER(C.c_uint64)],
+    "fsim_server_wait": [_P, C.c_uint64, _I, _PD],
+    "fsim_server_get_stats": [_P, C.POINTER(SERVER_STATS), _I],
     "fsim_stage_timing": [_I],
     "fsim_stage_times": [_PD, _PI, _I],
 }

@@ -214,6 +214,41 @@ int fsim_channel_destroy(fsim_channel ch);
 int fsim_reconstruct(fsim_channel ch, const uint16_t *raw_dev, float *out_dev,
                      int count, void *stream);
 
+/* ---- request server (continuous batching) ---------------------------------
+ * The serving form of livemode/ReconstructionRunner.java: submit() is
+ * queueImage() (:140-150) -- it refuses with FSIM_ERR_BUSY when max_queue
+ * requests are pending, as the reference drops frames when its queue is full
+ * (:366-370) -- and wait() takes a result as finalRecon.take() does (:57-58).
+ * A scheduler thread batches pending requests of one size class (n, directions,
+ * phases, bands; channels may differ within a batch), admits a batch when a
+ * device slot of that class is free or mem_budget allows a new one (free slots
+ * of other classes are evicted first), and runs each batch on its own stream,
+ * so that up to nslots batches overlap on the device; requests arriving while
+ * batches run join the next free slot.  Channels stay owned by the caller and
+ * must outlive the server.  Host buffers are copied asynchronously: pass
+ * pinned memory (fsim_alloc_host) for overlap.  FSIM_SUBMIT_DEVICE: raw/out
+ * are device pointers.  Results are bitwise independent of batch composition. */
+typedef struct fsim_server_s *fsim_server;
+typedef struct {
+    int64_t mem_budget;  /* device bytes for slots (work buffers, staging) */
+    int max_batch;       /* stacks per launch */
+    int max_queue;       /* pending requests before submit refuses */
+    int nslots;          /* batches in flight at once (streams) */
+} fsim_server_config;
+typedef struct {
+    int64_t submitted, completed, rejected, failed, batches, pending, device_bytes;
+    double mean_batch, p50_ms, p99_ms, max_ms;  /* latency: submit -> result ready */
+} fsim_server_stats;
+#define FSIM_SUBMIT_DEVICE 1
+int fsim_server_create(const fsim_server_config *cfg, fsim_server *out);
+int fsim_server_destroy(fsim_server s);
+int fsim_server_add_channel(fsim_server s, fsim_channel ch, int *id);
+int fsim_server_submit(fsim_server s, int channel, const uint16_t *raw, float *out, int flags,
+                       uint64_t *ticket);
+/* timeout_ms < 0 waits forever; FSIM_ERR_BUSY on timeout */
+int fsim_server_wait(fsim_server s, uint64_t ticket, int timeout_ms, double *latency_ms);
+int fsim_server_get_stats(fsim_server s, fsim_server_stats *st, int reset);
+
 /* Stage timing of the fused pipeline (no counterpart in the reference, which
  * only has Tool.Timer wall clocks, utils/Tool.java:210-245): while enabled,
  * every fsim_reconstruct records HIP events around its kernels on its stream;
