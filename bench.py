@@ -27,14 +27,18 @@ HBM_PEAK_GBS = 8000.0
 STAGES = ["k_sep_rows", "k_col_shift", "k_row_shift", "k_gather", "k_row_real"]
 
 
-def stage_bytes(n, D=3, P=3, C=3):
-    """Algorithmic HBM bytes per stack of each pipeline kernel."""
+def stage_bytes(n, D=3, P=3, C=3, Cs=None):
+    """HBM bytes per stack each pipeline kernel must move.  Cs = stored
+    components per direction: C for the reference dataflow (the sum is
+    B_alg = 434 n^2 at D = P = C = 3), B when each band's neg component is
+    derived from its pos component."""
+    Cs = C if Cs is None else Cs
     nn = n * n
-    sh = D * (C - 1)
-    return [2 * D * P * nn + 8 * D * C * nn,     # raw u16 in, D*C components out
+    sh = D * (Cs - 1)
+    return [2 * D * P * nn + 8 * D * Cs * nn,    # raw u16 in, stored components out
             16 * sh * nn,                        # shifted components in + out
             16 * sh * nn,
-            8 * D * C * nn + 8 * 4 * nn,         # all components in, (2n)^2 complex out
+            8 * D * C * nn + 8 * 4 * nn,         # every contribution read once, (2n)^2 complex out
             8 * 4 * nn + 4 * 4 * nn]             # (2n)^2 complex in, real out
 
 
@@ -200,13 +204,21 @@ def main():
         for _ in range(args.warmup):
             rec.reconstruct(raw_buf.value, out_buf.value, B)
         _lib.call("fsim_sync")
+        _lib.call("fsim_stage_timing", 1)
         t0 = time.perf_counter()
         for _ in range(args.steps):
             rec.reconstruct(raw_buf.value, out_buf.value, B)
         _lib.call("fsim_sync")
         el = time.perf_counter() - t0
-        print(json.dumps({"profile_only": True, "batch": B, "steps": args.steps,
-                          "recons_per_s": round(B * args.steps / el, 2)}), flush=True)
+        _lib.call("fsim_stage_timing", 0)
+        import ctypes as C
+        ms = (C.c_double * 5)()
+        cnt = (C.c_int * 5)()
+        _lib.call("fsim_stage_times", ms, cnt, 5)
+        print(json.dumps({"profile_only": True, "batch": B, "steps": args.steps, "n": n,
+                          "recons_per_s": round(B * args.steps / el, 2),
+                          "stage_ms": {STAGES[i]: round(ms[i] / max(cnt[i], 1), 4) for i in range(5)}}),
+              flush=True)
         return
 
     # parity of the measured path on stack 0
@@ -243,7 +255,8 @@ def main():
     value = B * args.steps / elapsed
     log(f"batched: {value:.1f} recons/s, {ms_per_step:.3f} ms/step")
 
-    sb = stage_bytes(n)
+    cs = 2 if rec.p.mode == 0 else 3          # real estimated OTF: the pairing is used in mode 0
+    sb = stage_bytes(n, Cs=cs)
     stage_ms = [ms[i] / max(cnt[i], 1) for i in range(5)]          # per launch (= per batch)
     dom = int(np.argmax(stage_ms))
     dom_bytes = sb[dom] * B
@@ -251,8 +264,9 @@ def main():
     stages = {STAGES[i]: {"ms_per_launch": round(stage_ms[i], 4),
                           "GBps": round(sb[i] * B / (stage_ms[i] * 1e-3) / 1e9, 1) if stage_ms[i] else None}
               for i in range(5)}
-    b_alg = sum(sb)
+    b_alg = sum(stage_bytes(n))               # reference dataflow: 434 n^2
     pipeline_frac = value * b_alg / (HBM_PEAK_GBS * 1e9)
+    hbm_frac_moved = value * sum(sb) / (HBM_PEAK_GBS * 1e9)
 
     # single-stack latency of the fused path
     lat = []
@@ -328,6 +342,8 @@ def main():
                      "traffic": None},
         "pipeline_roofline_frac": round(pipeline_frac, 4),
         "alg_bytes_per_recon": b_alg,
+        "bytes_moved_per_recon": sum(sb),
+        "hbm_frac_of_bytes_moved": round(hbm_frac_moved, 4),
         "stages": stages,
         "latency_single_ms": {"p50": round(pct(lat, 50), 3), "p99": round(pct(lat, 99), 3)},
         "latency_batch_ms": {"p50": round(pct(blat, 50), 3), "p99": round(pct(blat, 99), 3)},

@@ -15,6 +15,13 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#ifndef FSIM_TW_MODE
+#define FSIM_TW_MODE 2
+#endif
+#ifndef FSIM_LDS_PAD
+#define FSIM_LDS_PAD 1
+#endif
+
 namespace fsim {
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
@@ -137,7 +144,7 @@ struct ColE {
 template <int L, int E>
 struct FftLds {
     static constexpr int re = E > 16 ? 16 : E;
-    static constexpr int shift = re >= 16 ? 4 : re >= 8 ? 3 : re >= 4 ? 2 : re >= 2 ? 1 : 0;
+    static constexpr int shift = !FSIM_LDS_PAD ? 0 : re >= 16 ? 4 : re >= 8 ? 3 : re >= 4 ? 2 : re >= 2 ? 1 : 0;
     static constexpr int len = L + (shift ? (L >> shift) : 0);   // padded length
     static __device__ __forceinline__ int pad(int i) { return shift ? i + (i >> shift) : i; }
 };
@@ -179,12 +186,32 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, int ls, in
             const int k = j & (NS - 1);
             if constexpr (NS > 1) {
                 const int st = (L / (NS * R)) * k;
+#if FSIM_TW_MODE == 1 || FSIM_TW_MODE == 2
+                // first twiddle of the butterfly, powers by complex products
+#if FSIM_TW_MODE == 2
+                // exp(-+2 pi i k / (NS R)): the argument k/(NS R) is exact in
+                // float (power-of-two denominator), sincospi is accurate
+                float sw, cw;
+                sincospif((INV ? 2.0f : -2.0f) * (float)k / (float)(NS * R), &sw, &cw);
+                float2 w1 = make_float2(cw, sw);
+#else
+                float2 w1 = tw[st];
+                if (INV) w1.y = -w1.y;
+#endif
+                float2 w = w1;
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    u[r] = cmul(u[r], w);
+                    w = cmul(w, w1);
+                }
+#else
 #pragma unroll
                 for (int r = 1; r < R; r++) {
                     float2 w = tw[r * st];
                     if (INV) w.y = -w.y;
                     u[r] = cmul(u[r], w);
                 }
+#endif
             }
             dft_reg<R, INV>(u);
             if constexpr (LAST) {

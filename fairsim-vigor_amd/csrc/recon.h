@@ -16,8 +16,17 @@ struct ReconConst {
     const float2 *otf2d;              // otfToVector(band, 0, 0) on the n^2 grid, per OTF band
     const float2 *wa;                 // (2n)^2 of (1/Wiener denominator, apodization)
     const double *fade_lo, *fade_hi;  // fadeBorderCos factors, n each
+    const float2 *ramp;               // [kMaxDir][kMaxComp][x|y][n] sub-pixel ramp factors
     float2 S[kMaxDir][kMaxComp][kMaxPhase];   // separation matrices (toFlt)
     int offx[kMaxDir][kMaxComp], offy[kMaxDir][kMaxComp];   // integer paste offsets
+    // Stored components: Cs per direction (C, or B with the Hermitian pairing:
+    // DC and the pos component of each band).  srow/sband/sramp: separation
+    // row, OTF band and ramp of stored component j; src[c]: stored component
+    // that contribution c reads; mir/mdx/mdy: contribution c is the mirrored,
+    // conjugated pos spectrum Z[(delta - k) mod n] with delta = (mdx, mdy).
+    int Cs;
+    int srow[kMaxComp], sband[kMaxComp], sramp[kMaxComp], src[kMaxComp];
+    int mir[kMaxDir][kMaxComp], mdx[kMaxDir][kMaxComp], mdy[kMaxDir][kMaxComp];
     double fx[kMaxDir][kMaxComp], fy[kMaxDir][kMaxComp];    // sub-pixel ramp shifts
 };
 
@@ -38,7 +47,7 @@ struct JobList {
 };
 
 bool recon_size_ok(int n);
-int launch_pipeline(int n, const JobList &jl, int njobs, int D, int C, hipStream_t s);
+int launch_pipeline(int n, const JobList &jl, int njobs, int D, int Cs, hipStream_t s);
 size_t workspace_a_elems(int n, int D, int C);
 size_t workspace_b_elems(int n);
 

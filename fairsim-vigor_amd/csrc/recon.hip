@@ -33,6 +33,19 @@
 #include <utility>
 #include <vector>
 
+#ifndef FSIM_RAMP_TABLE
+#define FSIM_RAMP_TABLE 1
+#endif
+#ifndef FSIM_K1_ALLC
+#define FSIM_K1_ALLC 0
+#endif
+#ifndef FSIM_GATHER_CW
+#define FSIM_GATHER_CW 8
+#endif
+#ifndef FSIM_COL_CW
+#define FSIM_COL_CW 16
+#endif
+
 namespace fsim {
 
 __device__ __forceinline__ Job get_job(const JobList &jl, int j) {
@@ -58,22 +71,24 @@ __device__ __forceinline__ float fade_at(const double *__restrict__ lo, const do
 }
 
 // ---------------------------------------------------------------------------
-// K1: band separation + row FFT.  Block = RW rows of one component; the C
-// blocks of one row group are placed on the same XCD back to back (blocks
-// are dealt to the 8 XCDs round robin) so the raw rows they share hit L2.
+// K1: band separation + row FFT.  One wavefront (T = n/E threads) per row of
+// one direction: the P raw rows are read once, faded, and every component
+// c = sum_p S[d][c][p] * raw_p is transformed and stored in turn.
 // ---------------------------------------------------------------------------
+#if !FSIM_K1_ALLC
 template <int N>
-__global__ void __launch_bounds__(256) k_sep_rows(JobList jl, int ngroups, const float2 *__restrict__ tw) {
+__global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, const float2 *__restrict__ tw) {
     constexpr int E = FftE<N>::value, T = N / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int LP = FftLds<N, E>::len;
     __shared__ float2 lds[RW * LP];
     const Job J = get_job(jl, blockIdx.y);
     const ReconConst &K = *J.c;
-    const int C = K.C, P = K.P;
+    const int C = K.Cs, P = K.P;
     const int L = blockIdx.x;
     const int g = (L / 8 / C) * 8 + (L % 8);
-    const int c = (L / 8) % C;
+    const int c = (L / 8) % C;           // stored component
+    const int crow = K.srow[c];          // its separation-matrix row
     if (g >= ngroups) return;
     constexpr int rows_per_dir = N / RW;
     const int d = g / rows_per_dir, rb = g % rows_per_dir;
@@ -88,7 +103,7 @@ __global__ void __launch_bounds__(256) k_sep_rows(JobList jl, int ngroups, const
 #pragma unroll
     for (int m = 0; m < E; m++) fade[m] = fade_at(K.fade_lo, K.fade_hi, t + T * m, y);
     for (int p = 0; p < P; p++) {
-        const float2 s = K.S[d][c][p];
+        const float2 s = K.S[d][crow][p];
         const uint16_t *rp = raw + (size_t)p * nn;
 #pragma unroll
         for (int m = 0; m < E; m++) {
@@ -102,27 +117,76 @@ __global__ void __launch_bounds__(256) k_sep_rows(JobList jl, int ngroups, const
 #pragma unroll
     for (int m = 0; m < E; m++) out[t + T * m] = v[m];
 }
+#endif
+
+template <int N>
+__global__ void __launch_bounds__(256) k_sep_rows(JobList jl, const float2 *__restrict__ tw) {
+    constexpr int E = FftE<N>::value, T = N / E;
+    constexpr int RW = T >= 256 ? 1 : 256 / T;
+    constexpr int LP = FftLds<N, E>::len;
+    constexpr int rows_per_dir = N / RW;
+    __shared__ float2 lds[RW * LP];
+    const Job J = get_job(jl, blockIdx.y);
+    const ReconConst &K = *J.c;
+    const int C = K.Cs, P = K.P;
+    const int d = blockIdx.x / rows_per_dir, rb = blockIdx.x % rows_per_dir;
+    const int t = threadIdx.x % T, rr = threadIdx.x / T;
+    const int y = rb * RW + rr;
+    const size_t nn = (size_t)N * N;
+    const uint16_t *raw = J.raw + (size_t)d * P * nn + (size_t)y * N;
+    float px[kMaxPhase][E];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const float f = fade_at(K.fade_lo, K.fade_hi, t + T * m, y);
+#pragma unroll
+        for (int p = 0; p < kMaxPhase; p++)
+            if (p < P) px[p][m] = (float)raw[(size_t)p * nn + t + T * m] * f;
+    }
+    for (int c = 0; c < C; c++) {
+        float2 v[E];
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int p = 0; p < kMaxPhase; p++) {
+            if (p < P) {
+                const float2 sp = K.S[d][K.srow[c]][p];
+#pragma unroll
+                for (int m = 0; m < E; m++) {
+                    v[m].x += sp.x * px[p][m];
+                    v[m].y += sp.y * px[p][m];
+                }
+            }
+        }
+        fft_regs<N, E, false, RowWave<N>::value>(v, lds + rr * LP, 1, t, tw);
+        if constexpr (!RowWave<N>::value) __syncthreads();
+        float2 *out = J.A + ((size_t)(d * C + c) * N + y) * N;
+#pragma unroll
+        for (int m = 0; m < E; m++) out[t + T * m] = v[m];
+    }
+}
 
 // ---------------------------------------------------------------------------
 // K2: shifted components: column FFT, * conj(OTF_band), column IFFT (1/n).
 // ---------------------------------------------------------------------------
 template <int N>
-struct ColShiftBlock {
+struct ColShiftTile {
     static constexpr int e = ColE<N>::value, t = N / e;
-    static constexpr int value = t * ColTile<N, t, e>::value;
+    static constexpr int cw0 = ColTile<N, t, e>::value;
+    static constexpr int cw = cw0 < FSIM_COL_CW ? cw0 : FSIM_COL_CW;
+    static constexpr int value = t * cw;
 };
 
 template <int N>
-__global__ void __launch_bounds__(ColShiftBlock<N>::value) k_col_shift(JobList jl, const float2 *__restrict__ tw) {
+__global__ void __launch_bounds__(ColShiftTile<N>::value) k_col_shift(JobList jl, const float2 *__restrict__ tw) {
     constexpr int E = ColE<N>::value, T = N / E;
-    constexpr int CW = ColTile<N, T, E>::value;
+    constexpr int CW = ColShiftTile<N>::cw;
     constexpr int tiles = N / CW;
     __shared__ float2 lds[FftLds<N, E>::len * CW];
     const Job J = get_job(jl, blockIdx.y);
     const ReconConst &K = *J.c;
-    const int C = K.C, nsh = C - 1;
+    const int C = K.Cs, nsh = C - 1;
     const int tile = blockIdx.x % tiles, dc = blockIdx.x / tiles;
-    const int d = dc / nsh, c = 1 + dc % nsh;
+    const int d = dc / nsh, c = 1 + dc % nsh;   // stored shifted component
     const int cl = threadIdx.x % CW, t = threadIdx.x / CW;
     const int x = tile * CW + cl;
     float2 *col = J.A + (size_t)(d * C + c) * N * N + x;
@@ -130,7 +194,7 @@ __global__ void __launch_bounds__(ColShiftBlock<N>::value) k_col_shift(JobList j
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = col[(size_t)(t + T * m) * N];
     fft_regs<N, E, false>(v, lds + cl, CW, t, tw);
-    const int band = (c + 1) / 2;
+    const int band = K.sband[c];
     const float2 *otf = K.otf2d + (size_t)(K.otf_nb > 1 ? band : 0) * N * N + x;
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = times_conj_java(v[m], otf[(size_t)(t + T * m) * N]);
@@ -152,9 +216,9 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
     __shared__ float2 lds[RW * LP];
     const Job J = get_job(jl, blockIdx.y);
     const ReconConst &K = *J.c;
-    const int C = K.C, nsh = C - 1;
+    const int C = K.Cs, nsh = C - 1;
     const int rb = blockIdx.x % rblocks, dc = blockIdx.x / rblocks;
-    const int d = dc / nsh, c = 1 + dc % nsh;
+    const int d = dc / nsh, c = 1 + dc % nsh;   // stored shifted component
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int y = rb * RW + rr;
     float2 *row = J.A + ((size_t)(d * C + c) * N + y) * N;
@@ -162,14 +226,26 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = row[t + T * m];
     fft_regs<N, E, true, RowWave<N>::value>(v, lds + rr * LP, 1, t, tw);
+    // sub-pixel ramp exp(2 pi i (fx x + fy y) / n) as a product of the
+    // per-channel row and column tables (SimUtils.java:74-81 shift phases)
     const float sc = 1.0f / N;
-    const double fx = K.fx[d][c], fy = K.fy[d][c];
+#if FSIM_RAMP_TABLE
+    const float2 *rx = K.ramp + ((size_t)(d * kMaxComp + K.sramp[c]) * 2) * N;
+    const float2 ry = rx[N + y];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const float2 r = cmul(rx[t + T * m], ry);
+        v[m] = rot_java(cscale(v[m], sc), r.x, r.y);
+    }
+#else
+    const double fx = K.fx[d][K.sramp[c]], fy = K.fy[d][K.sramp[c]];
 #pragma unroll
     for (int m = 0; m < E; m++) {
         float co, si;
         shift_cos_sin(fx, fy, t + T * m, y, N, co, si);
         v[m] = rot_java(cscale(v[m], sc), co, si);
     }
+#endif
     fft_regs<N, E, false, RowWave<N>::value>(v, lds + rr * LP, 1, t, tw);
 #pragma unroll
     for (int m = 0; m < E; m++) row[t + T * m] = v[m];
@@ -188,7 +264,8 @@ struct GatherE {
     static constexpr int em = N >= 1024 ? 32 : FftE<2 * N>::value;
     static constexpr int tn = N / en, tm = 2 * N / em;
     static constexpr int tk = tn > tm ? tn : tm;
-    static constexpr int cw = ColTile<2 * N, tk, (en > em ? en : em)>::value;
+    static constexpr int cw0 = ColTile<2 * N, tk, (en > em ? en : em)>::value;
+    static constexpr int cw = cw0 < FSIM_GATHER_CW ? cw0 : FSIM_GATHER_CW;
 };
 template <int N>
 __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw) k_gather(JobList jl, const float2 *__restrict__ twn,
@@ -202,13 +279,16 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw) k_gather(JobL
     __shared__ float2 lds[(LPK > N ? LPK : N) * CW];
     const Job J = get_job(jl, blockIdx.y);
     const ReconConst &K = *J.c;
-    const int D = K.D, C = K.C;
+    const int D = K.D, C = K.C, Cs = K.Cs;
     const int cl = threadIdx.x % CW, t = threadIdx.x / CW;
-    // blocks are dealt to the 8 XCDs round robin; give each XCD a contiguous
-    // run of column tiles so that source rows shared by neighbouring tiles
-    // (paste offsets are not tile aligned) are fetched once into its L2
+    // blocks are dealt to the 8 XCDs round robin; put adjacent tile pairs on
+    // the same XCD (bx and bx + 8 run back to back there) so that source rows
+    // shared by neighbouring tiles (paste offsets are not tile aligned) are
+    // fetched once into its L2, while the pairs stay spread over all XCDs
+    // (tiles differ in how many components land in them)
     constexpr int tiles = M / CW;
-    const int tile = (blockIdx.x % 8) * (tiles / 8) + blockIdx.x / 8;
+    const int bx = blockIdx.x;
+    const int tile = tiles % 16 ? bx : (bx / 16) * 16 + (bx % 8) * 2 + (bx / 8) % 2;
     const int xo = tile * CW + cl;
     float2 acc[EM];
 #pragma unroll
@@ -216,11 +296,14 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw) k_gather(JobL
     const size_t nn = (size_t)N * N;
     for (int d = 0; d < D; d++) {
         for (int c = 0; c < C; c++) {
-            const int xs = paste_src(xo, M, N, K.offx[d][c]);
+            // contribution c of direction d, in the reference's summation order
+            const int mir = K.mir[d][c];
+            int xs = paste_src(xo, M, N, K.offx[d][c]);
+            if (mir && xs >= 0) xs = (K.mdx[d][c] - xs + N) & (N - 1);
             const bool act = (t < TN) && (xs >= 0);
             float2 v[EN];
             if (act) {
-                const float2 *col = J.A + (size_t)(d * C + c) * nn + xs;
+                const float2 *col = J.A + (size_t)(d * Cs + K.src[c]) * nn + xs;
 #pragma unroll
                 for (int m = 0; m < EN; m++) v[m] = col[(size_t)(t + TN * m) * N];
             }
@@ -237,13 +320,15 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw) k_gather(JobL
             __syncthreads();
             if (t < TM && xs >= 0) {
                 const int oy = K.offy[d][c];
+                const int my = K.mdy[d][c];
 #pragma unroll
                 for (int m = 0; m < EM; m++) {
-                    const int ys = paste_src(t + TM * m, M, N, oy);
+                    int ys = paste_src(t + TM * m, M, N, oy);
                     if (ys >= 0) {
+                        if (mir) ys = (my - ys + N) & (N - 1);
                         const float2 s = lds[ys * CW + cl];
                         acc[m].x += s.x;
-                        acc[m].y += s.y;
+                        acc[m].y += mir ? -s.y : s.y;
                     }
                 }
             }
@@ -320,6 +405,7 @@ static void stage_mark(int stage, hipStream_t s, hipEvent_t &start) {
 
 template <int N>
 static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, hipStream_t s) {
+    // C: stored components per direction
     constexpr int M = 2 * N;
     const float2 *twn = twiddles(N), *twm = twiddles(M);
     if (!twn || !twm) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
@@ -327,14 +413,18 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, hipStre
     stage_mark(-1, s, ev);
     {
         constexpr int E = FftE<N>::value, T = N / E, RW = T >= 256 ? 1 : 256 / T;
+#if FSIM_K1_ALLC
+        hipLaunchKernelGGL(k_sep_rows<N>, dim3(D * (N / RW), njobs), dim3(T * RW), 0, s, jl, twn);
+#else
         const int ngroups = D * (N / RW);
         const int gx = ((ngroups + 7) / 8) * 8 * C;
-        hipLaunchKernelGGL(k_sep_rows<N>, dim3(gx, njobs), dim3(T * RW), 0, s, jl, ngroups, twn);
+        hipLaunchKernelGGL(k_sep_rows1<N>, dim3(gx, njobs), dim3(T * RW), 0, s, jl, ngroups, twn);
+#endif
         stage_mark(0, s, ev);
     }
     if (C > 1) {
         {
-            constexpr int E = ColE<N>::value, T = N / E, CW = ColTile<N, T, E>::value;
+            constexpr int E = ColE<N>::value, T = N / E, CW = ColShiftTile<N>::cw;
             hipLaunchKernelGGL(k_col_shift<N>, dim3(D * (C - 1) * (N / CW), njobs), dim3(T * CW), 0, s, jl, twn);
             stage_mark(1, s, ev);
         }
@@ -360,6 +450,7 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, hipStre
 bool recon_size_ok(int n) { return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048; }
 
 int launch_pipeline(int n, const JobList &jl, int njobs, int D, int C, hipStream_t s) {
+    // C: stored components per direction (ReconConst::Cs)
     switch (n) {
         case 64: return launch_pipeline_t<64>(jl, njobs, D, C, s);
         case 128: return launch_pipeline_t<128>(jl, njobs, D, C, s);
@@ -378,7 +469,7 @@ size_t workspace_b_elems(int n) { return (size_t)4 * n * n; }
 // channel set-up: the FilterUpdateThread work (ReconstructionRunner.java:
 // 773-828) plus the per-direction constants of the reconstruction loop.
 // ---------------------------------------------------------------------------
-int build_channel_const(const fsim_recon_params &p, const Otf &o, ReconConst &K) {
+int build_channel_const(const fsim_recon_params &p, const Otf &o, bool sym, ReconConst &K) {
     const int n = p.n, D = p.ndir, P = p.nphase, B = p.nband, C = 2 * B - 1;
     if (!recon_size_ok(n)) return fail(FSIM_ERR_ARG, "channel: n must be a power of two in [64, 2048]");
     if (D < 1 || D > kMaxDir || B < 2 || B > kMaxBand || P < C || P > kMaxPhase)
@@ -398,6 +489,14 @@ int build_channel_const(const fsim_recon_params &p, const Otf &o, ReconConst &K)
     K.cutoff = o.cutoff;
     K.vec_cpm = p.vec_cpm;
     K.otf_tab = p.use_att ? o.vals_att : o.vals;
+    K.Cs = sym ? B : C;
+    for (int j = 0; j < kMaxComp; j++) K.srow[j] = K.sband[j] = K.sramp[j] = K.src[j] = 0;
+    for (int j = 0; j < K.Cs; j++) {
+        K.srow[j] = sym ? 2 * j : j;          // DC, then pos of band j
+        K.sband[j] = sym ? j : (j + 1) / 2;
+        K.sramp[j] = K.srow[j];
+    }
+    for (int c = 0; c < C; c++) K.src[c] = sym ? (c + 1) / 2 : c;
     for (int d = 0; d < D; d++) {
         // SimParam.Dir.getModulations clamp (SimParam.java:598-607)
         double mods[kMaxBand];
@@ -425,9 +524,14 @@ int build_channel_const(const fsim_recon_params &p, const Otf &o, ReconConst &K)
                 K.fy[d][comp] = -ky - std::floor(-ky);
                 K.offx[d][comp] = (int)std::floor(kx);
                 K.offy[d][comp] = (int)std::floor(-ky);
+                K.mir[d][comp] = (sym && s == 1) ? 1 : 0;
                 if (K.offx[d][comp] < -2 * n || K.offy[d][comp] < -2 * n)
                     return fail(FSIM_ERR_ARG, "channel: shift vector out of range");
             }
+            // neg = conj(pos mirrored): delta = 1 per axis where the pos
+            // fractional shift is non-zero (the neg one is then 1 - f)
+            K.mdx[d][2 * b - 1] = K.fx[d][2 * b] != 0.0 ? 1 : 0;
+            K.mdy[d][2 * b - 1] = K.fy[d][2 * b] != 0.0 ? 1 : 0;
         }
     }
     return FSIM_OK;
@@ -443,6 +547,7 @@ struct fsim_channel_s {
     ReconConst *dK = nullptr;
     float2 *wa = nullptr;
     float2 *otf2d = nullptr;
+    float2 *ramp = nullptr;
     double *fade = nullptr;
     float2 *A = nullptr, *B = nullptr;
     int cap = 0;
@@ -455,6 +560,7 @@ static void channel_free(fsim_channel_s *ch) {
     (void)hipFree(ch->dK);
     (void)hipFree(ch->wa);
     (void)hipFree(ch->otf2d);
+    (void)hipFree(ch->ramp);
     (void)hipFree(ch->fade);
     (void)hipFree(ch->A);
     (void)hipFree(ch->B);
@@ -469,7 +575,16 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     const Otf &o = oh->o;
     fsim_channel_s *ch = new fsim_channel_s();
     ch->p = *p;
-    rc = build_channel_const(*p, o, ch->K);
+    // Hermitian pairing needs a real OTF (then conj(OTF) is symmetric in k)
+    bool sym = p->mode == 0;
+    if (sym) {
+        std::vector<float2> hv((size_t)o.nband * o.ns);
+        FSIM_HIP(hipMemcpy(hv.data(), p->use_att ? o.vals_att : o.vals, hv.size() * sizeof(float2),
+                           hipMemcpyDeviceToHost));
+        for (const float2 &z : hv)
+            if (z.y != 0.f) sym = false;
+    }
+    rc = build_channel_const(*p, o, sym, ch->K);
     if (rc) {
         delete ch;
         return rc;
@@ -480,6 +595,7 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     if (hipMalloc(&ch->dK, sizeof(ReconConst)) != hipSuccess || hipMalloc(&ch->wa, mm * sizeof(float2)) != hipSuccess ||
         hipMalloc(&ch->otf2d, (size_t)o.nband * n * n * sizeof(float2)) != hipSuccess ||
         hipMalloc(&ch->fade, 2 * n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ch->ramp, (size_t)kMaxDir * kMaxComp * 2 * n * sizeof(float2)) != hipSuccess ||
         hipMalloc(&wtmp, mm * sizeof(float)) != hipSuccess || hipMalloc(&atmp, mm * sizeof(float)) != hipSuccess) {
         (void)hipFree(wtmp);
         (void)hipFree(atmp);
@@ -494,6 +610,24 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     ch->K.fade_lo = ch->fade;
     ch->K.fade_hi = ch->fade + n;
     ch->K.wa = ch->wa;
+    {
+        // per direction/component ramp tables exp(2 pi i f t / n), t < n, for
+        // the x and y fractional shifts, from double precision
+        std::vector<float2> r((size_t)kMaxDir * kMaxComp * 2 * n, make_float2(1.f, 0.f));
+        for (int d = 0; d < p->ndir; d++)
+            for (int c = 0; c < ch->K.C; c++)
+                for (int a = 0; a < 2; a++) {
+                    const double f = a == 0 ? ch->K.fx[d][c] : ch->K.fy[d][c];
+                    float2 *row = r.data() + ((size_t)(d * kMaxComp + c) * 2 + a) * n;
+                    for (int t = 0; t < n; t++) {
+                        const double ph = 2 * M_PI * f * t / n;
+                        row[t] = make_float2((float)std::cos(ph), (float)std::sin(ph));
+                    }
+                }
+        FSIM_HIP(hipMemcpyAsync(ch->ramp, r.data(), r.size() * sizeof(float2), hipMemcpyHostToDevice, s));
+        FSIM_HIP(hipStreamSynchronize(s));
+        ch->K.ramp = ch->ramp;
+    }
     {
         // otfToVector(band, 0, 0, write) per OTF band on the n^2 grid
         const int nb = o.nband;
@@ -548,7 +682,7 @@ int fsim_reconstruct(fsim_channel ch, const uint16_t *raw, float *out, int count
         (void)hipFree(ch->B);
         ch->A = ch->B = nullptr;
         ch->cap = 0;
-        if (hipMalloc(&ch->A, workspace_a_elems(n, K.D, K.C) * count * sizeof(float2)) != hipSuccess ||
+        if (hipMalloc(&ch->A, workspace_a_elems(n, K.D, K.Cs) * count * sizeof(float2)) != hipSuccess ||
             hipMalloc(&ch->B, workspace_b_elems(n) * count * sizeof(float2)) != hipSuccess) {
             (void)hipFree(ch->A);
             ch->A = nullptr;
@@ -565,10 +699,10 @@ int fsim_reconstruct(fsim_channel ch, const uint16_t *raw, float *out, int count
     jl.base.B = ch->B;
     jl.base.out = out;
     jl.raw_stride = (size_t)K.D * K.P * n * n;
-    jl.a_stride = workspace_a_elems(n, K.D, K.C);
+    jl.a_stride = workspace_a_elems(n, K.D, K.Cs);
     jl.b_stride = workspace_b_elems(n);
     jl.out_stride = (size_t)4 * n * n;
-    return launch_pipeline(n, jl, count, K.D, K.C, s);
+    return launch_pipeline(n, jl, count, K.D, K.Cs, s);
 }
 
 int fsim_stage_timing(int enable) {

@@ -41,8 +41,10 @@ struct Request {
 };
 
 struct ClassKey {
-    int n, D, P, B;
-    bool operator==(const ClassKey &o) const { return n == o.n && D == o.D && P == o.P && B == o.B; }
+    int n, D, P, B, Cs;   // Cs: stored components per direction
+    bool operator==(const ClassKey &o) const {
+        return n == o.n && D == o.D && P == o.P && B == o.B && Cs == o.Cs;
+    }
 };
 
 struct Slot {
@@ -62,7 +64,7 @@ struct Slot {
 
 static size_t per_stack_bytes(const ClassKey &k) {
     const size_t nn = (size_t)k.n * k.n;
-    const int C = 2 * k.B - 1;
+    const int C = k.Cs;
     return 2 * (size_t)k.D * k.P * nn + 8 * (size_t)k.D * C * nn + 32 * nn + 16 * nn;
 }
 
@@ -113,7 +115,7 @@ struct Server {
         s->key = k;
         s->cap = cap_for(k);
         const size_t nn = (size_t)k.n * k.n;
-        const int C = 2 * k.B - 1;
+        const int C = k.Cs;
         s->bytes = per_stack_bytes(k) * s->cap + sizeof(Job) * s->cap;
         bool ok = hipMalloc(&s->raw, 2 * (size_t)k.D * k.P * nn * s->cap) == hipSuccess &&
                   hipMalloc(&s->out, 16 * nn * s->cap) == hipSuccess &&
@@ -160,13 +162,13 @@ struct Server {
 
     ClassKey key_of(const Request *r) const {
         const ReconConst &K = channel_host_const(channels[r->chan]);
-        return ClassKey{K.n, K.D, K.P, K.B};
+        return ClassKey{K.n, K.D, K.P, K.B, K.Cs};
     }
 
     void launch(Slot *s) {
         const ClassKey k = s->key;
         const size_t nn = (size_t)k.n * k.n;
-        const int C = 2 * k.B - 1;
+        const int C = k.Cs;
         const size_t raw_elems = (size_t)k.D * k.P * nn;
         int rc = FSIM_OK;
         for (size_t i = 0; i < s->reqs.size(); i++) {

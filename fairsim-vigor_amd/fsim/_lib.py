@@ -49,6 +49,7 @@ class RECON_PARAMS(C.Structure):
         ("apo_bend", C.c_double),
         ("apo_cut", C.c_double),
         ("fade_px", C.c_int),
+        ("mode", C.c_int),
     ]
 
 

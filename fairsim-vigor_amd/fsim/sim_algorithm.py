@@ -343,7 +343,7 @@ class PlainPipeline:
         return self.result
 
 
-def recon_params(param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10):
+def recon_params(param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10, mode=0):
     """Pack a SimParam into fsim_recon_params."""
     p = RECON_PARAMS()
     p.n = param.vectorWidth()
@@ -365,6 +365,7 @@ def recon_params(param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10):
     p.apo_bend = apo_bend
     p.apo_cut = apo_cut
     p.fade_px = fade_px
+    p.mode = mode
     return p
 
 
@@ -372,12 +373,14 @@ class Reconstructor:
     """Fused reconstruction of one channel (fsim_channel): tables resident on
     the device, ``reconstruct`` runs a batch of stacks in five passes."""
 
-    def __init__(self, param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10):
+    def __init__(self, param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10, mode=0):
+        """mode 0 derives each band's neg component from its pos component
+        when the OTF is real (exact identity); mode 1 computes all."""
         _lib.init()
         self.param = param
         self.n = param.vectorWidth()
         self.D, self.P = param.nrDir(), param.dir(0).nrPha()
-        self.p = recon_params(param, wiener, apo_bend, apo_cut, fade_px)
+        self.p = recon_params(param, wiener, apo_bend, apo_cut, fade_px, mode)
         self._h = C.c_void_p()
         call("fsim_channel_create", C.byref(self.p), param.otf().device(), C.byref(self._h))
 

@@ -202,6 +202,10 @@ typedef struct {
     double wiener;            /* Wiener parameter w */
     double apo_bend, apo_cut; /* writeApoVector(bend, cutoff factor) */
     int fade_px;              /* fadeBorderCos width (10), 0 = off */
+    int mode;                 /* 0: derive each band's neg component from its pos
+                                 component (exact for real raw data and a real
+                                 OTF; chosen only when the OTF is real),
+                                 1: compute every component separately */
 } fsim_recon_params;
 
 int fsim_channel_create(const fsim_recon_params *p, fsim_otf otf, fsim_channel *out);

@@ -29,31 +29,34 @@ def _setup(n, seed, nphase=3, nband=2, harmonic2=0.0, lam=525.0, fade_px=10, apo
     return raw, sp, param, p
 
 
-@pytest.mark.parametrize("n,seed", [(64, 1), (128, 2), (256, 3), (512, 4)])
-def test_fused_matches_oracle(fsim_lib, n, seed):
+@pytest.mark.parametrize("n,seed,mode", [(64, 1, 0), (128, 2, 0), (256, 3, 0), (512, 4, 0),
+                                         (64, 1, 1), (512, 4, 1)])
+def test_fused_matches_oracle(fsim_lib, n, seed, mode):
+    """mode 0: neg components derived from pos (Hermitian pairing), mode 1:
+    every component computed."""
     from fsim.sim_algorithm import Reconstructor
     raw, sp, param, p = _setup(n, seed)
-    rec = Reconstructor(param)
+    rec = Reconstructor(param, mode=mode)
     got = rec.reconstruct_host(raw)
     r32 = ref.recon32(p, raw, threads=8)
     e = fp64.rel_l2(got, r32)
     if n <= 256:
         r64 = ref.recon64(p, raw)
-        print(f"n={n} relL2 gpu-vs-ref32 {e:.3e} gpu-vs-fp64 {fp64.rel_l2(got, r64):.3e} "
+        print(f"n={n} mode={mode} relL2 gpu-vs-ref32 {e:.3e} gpu-vs-fp64 {fp64.rel_l2(got, r64):.3e} "
               f"ref32-vs-fp64 {fp64.rel_l2(r32, r64):.3e}")
     else:
-        print(f"n={n} relL2 gpu-vs-ref32 {e:.3e}")
+        print(f"n={n} mode={mode} relL2 gpu-vs-ref32 {e:.3e}")
     assert e < TOL
 
 
-@pytest.mark.parametrize("nphase,nband,h2", [(5, 2, 0.0), (5, 3, 0.3)])
-def test_fused_five_phase(fsim_lib, nphase, nband, h2):
+@pytest.mark.parametrize("nphase,nband,h2,mode", [(5, 2, 0.0, 0), (5, 3, 0.3, 0), (5, 3, 0.3, 1)])
+def test_fused_five_phase(fsim_lib, nphase, nband, h2, mode):
     from fsim.sim_algorithm import Reconstructor
     raw, sp, param, p = _setup(128, 11, nphase=nphase, nband=nband, harmonic2=h2)
-    got = Reconstructor(param).reconstruct_host(raw)
+    got = Reconstructor(param, mode=mode).reconstruct_host(raw)
     r32 = ref.recon32(p, raw, threads=8)
     e = fp64.rel_l2(got, r32)
-    print(f"P={nphase} B={nband} relL2 gpu-vs-ref32 {e:.3e}")
+    print(f"P={nphase} B={nband} mode={mode} relL2 gpu-vs-ref32 {e:.3e}")
     assert e < TOL
 
 
@@ -97,3 +100,23 @@ def test_no_fade_live_apodization(fsim_lib):
     raw, sp, param, p = _setup(64, 7, fade_px=0, apo_bend=1.0)
     got = Reconstructor(param, apo_bend=1.0, fade_px=0).reconstruct_host(raw)
     assert fp64.rel_l2(got, ref.recon32(p, raw)) < TOL
+
+
+def test_pairing_matches_full_components(fsim_lib):
+    """The Hermitian pairing (mode 0) and the full computation (mode 1) agree
+    to FP32 round-off on the same stacks, also with integer shift vectors
+    (zero fractional part: no one-pixel offset in the mirrored index)."""
+    from fsim import synthetic
+    from fsim.sim_algorithm import Reconstructor
+    for n, integer in [(128, False), (128, True), (256, False)]:
+        raw, sp = synthetic.sim_stack(n=n, seed=60 + n, nbeads=60, nfil=8)
+        if integer:
+            sp = dict(sp)
+            sp["px"] = [round(v) for v in sp["px"]]
+            sp["py"] = [round(v) for v in sp["py"]]
+        param = synthetic.make_param(sp, n)
+        a = Reconstructor(param, mode=0).reconstruct_host(raw)
+        b = Reconstructor(param, mode=1).reconstruct_host(raw)
+        e = fp64.rel_l2(a, b)
+        print(f"n={n} integer={integer} pairing vs full {e:.3e}")
+        assert e < 5e-6
