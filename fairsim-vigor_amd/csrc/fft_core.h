@@ -183,7 +183,11 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, int ls, in
 #pragma unroll
             for (int r = 0; r < R; r++) u[r] = v[q + r * Q];
             const int j = t + T * q;
-            const int k = j & (NS - 1);
+            int k = j & (NS - 1);
+            // keep the twiddles computed here: hoisted out of a caller's
+            // loop they stay live across it and push kernels into spills
+            // (the empty asm emits no instruction)
+            asm volatile("" : "+v"(k));
             if constexpr (NS > 1) {
                 const int st = (L / (NS * R)) * k;
 #if FSIM_TW_MODE == 1 || FSIM_TW_MODE == 2

@@ -7,13 +7,17 @@
 //  K1 k_sep_rows    raw u16 -> fade -> band separation (linear, so it moves in
 //                   front of the FFT) -> row FFT            writes A[d][c]
 //  K2 k_col_shift   shifted components only: column FFT -> * conj(OTF) ->
-//                   column IFFT                               in place on A
-//  K3 k_row_shift   shifted components only: row IFFT -> fast-shift phase ramp
-//                   (SimUtils.java:74-81) -> row FFT          in place on A
-//  K5 k_gather      per column tile of the m x m result: for every direction
-//                   and component, column FFT of the pasted column (the DC
-//                   component also * conj(OTF)), pasteFreq as a gather into a
-//                   register accumulator (SimUtils.java:83-86), then
+//                   column IFFT -> column factor of the fast-shift phase ramp
+//                   -> column FFT                             in place on A
+//  K3 k_row_shift   shifted components only: row IFFT -> row factor of the
+//                   ramp -> row FFT                           in place on A
+//                   The ramp exp(2 pi i (fx x + fy y) / n) of SimUtils.java:
+//                   74-81 is separable, so IFFT2 -> ramp -> FFT2 splits into
+//                   a column part (K2) and a row part (K3); A then holds the
+//                   full shifted spectrum of every shifted component.
+//  K5 k_gather      per column tile of the m x m result: DC component column
+//                   FFT * conj(OTF), pasteFreq of every component as a gather
+//                   into a register accumulator (SimUtils.java:83-86), then
 //                   * Wiener * apodization and the column IFFT  writes B
 //  K6 k_row_real    row IFFT of B, real part                  writes out
 //
@@ -33,24 +37,72 @@
 #include <utility>
 #include <vector>
 
-#ifndef FSIM_RAMP_TABLE
-#define FSIM_RAMP_TABLE 1
-#endif
-#ifndef FSIM_K1_ALLC
-#define FSIM_K1_ALLC 0
-#endif
 #ifndef FSIM_GATHER_CW
 #define FSIM_GATHER_CW 8
 #endif
-#ifndef FSIM_COL_CW
-#define FSIM_COL_CW 16
+#ifndef FSIM_JOBMAP
+#define FSIM_JOBMAP 1
 #endif
+#ifndef FSIM_COL_CW
+#define FSIM_COL_CW 8
+#endif
+#ifndef FSIM_COL_XCD
+#define FSIM_COL_XCD 1
+#endif
+#ifndef FSIM_COL_WPE
+#define FSIM_COL_WPE 6
+#endif
+
+// Global (address space 1) and constant (4) views of device pointers that
+// arrive inside structures.  With them the compiler emits global_* and s_load
+// accesses instead of flat_* ones, whose completion also counts against the
+// LDS counter: every LDS wait of an FFT exchange would otherwise wait for the
+// outstanding global loads as well.
+#define FSIM_GLB __attribute__((address_space(1)))
+#define FSIM_CST __attribute__((address_space(4)))
 
 namespace fsim {
 
+template <typename T>
+__device__ __forceinline__ FSIM_GLB T *glb(T *p) {
+    return (FSIM_GLB T *)p;
+}
+template <typename T>
+__device__ __forceinline__ const FSIM_CST T &cst(const T *p) {
+    return *(const FSIM_CST T *)p;
+}
+using KRef = const FSIM_CST ReconConst &;
+
+// float2 moves through a native vector type (the float2 class only binds
+// generic references)
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld(const FSIM_GLB float2 *p) {
+    const f2v x = *(const FSIM_GLB f2v *)p;
+    return make_float2(x.x, x.y);
+}
+__device__ __forceinline__ float2 ld(const FSIM_CST float2 *p) {
+    const f2v x = *(const FSIM_CST f2v *)p;
+    return make_float2(x.x, x.y);
+}
+__device__ __forceinline__ void st(FSIM_GLB float2 *p, float2 v) {
+    f2v x;
+    x.x = v.x;
+    x.y = v.y;
+    *(FSIM_GLB f2v *)p = x;
+}
+
 __device__ __forceinline__ Job get_job(const JobList &jl, int j) {
-    if (jl.jobs) return jl.jobs[j];
-    Job J = jl.base;
+    Job J;
+    if (jl.jobs) {
+        const FSIM_CST Job &g = cst(jl.jobs + j);
+        J.c = g.c;
+        J.raw = g.raw;
+        J.A = g.A;
+        J.B = g.B;
+        J.out = g.out;
+        return J;
+    }
+    J = jl.base;
     J.raw += (size_t)j * jl.raw_stride;
     J.A += (size_t)j * jl.a_stride;
     J.B += (size_t)j * jl.b_stride;
@@ -58,10 +110,19 @@ __device__ __forceinline__ Job get_job(const JobList &jl, int j) {
     return J;
 }
 
+// Work index of this block such that each of the 8 XCDs (workgroups are
+// dealt to them round robin by linear id) walks one contiguous range of
+// [0, total): neighbouring tiles of a component run on one XCD.  The grid
+// has xcd_grid(total) blocks; indices >= total have no work.
+__device__ __forceinline__ int xcd_linear(int total) {
+    const int per = (total + 7) / 8;
+    return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+}
+static inline unsigned xcd_grid(int total) { return (unsigned)((total + 7) / 8) * 8; }
+
 // Fade factor of pixel (x, y): fadeBorderCos on a vector of ones
 // (SimUtils.java:162-185, ReconstructionRunner.java:781-783).
-__device__ __forceinline__ float fade_at(const double *__restrict__ lo, const double *__restrict__ hi, int x,
-                                         int y) {
+__device__ __forceinline__ float fade_at(const FSIM_GLB double *lo, const FSIM_GLB double *hi, int x, int y) {
     float f = 1.0f;
     f = (float)((double)f * lo[y]);
     f = (float)((double)f * hi[y]);
@@ -72,10 +133,11 @@ __device__ __forceinline__ float fade_at(const double *__restrict__ lo, const do
 
 // ---------------------------------------------------------------------------
 // K1: band separation + row FFT.  One wavefront (T = n/E threads) per row of
-// one direction: the P raw rows are read once, faded, and every component
-// c = sum_p S[d][c][p] * raw_p is transformed and stored in turn.
+// one direction and one stored component: the P raw rows are read (the
+// blocks of the other components of the row run on the same XCD and find
+// them in L2), faded, combined as c = sum_p S[d][c][p] * raw_p and
+// transformed.
 // ---------------------------------------------------------------------------
-#if !FSIM_K1_ALLC
 template <int N>
 __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, const float2 *__restrict__ tw) {
     constexpr int E = FftE<N>::value, T = N / E;
@@ -83,7 +145,7 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, cons
     constexpr int LP = FftLds<N, E>::len;
     __shared__ float2 lds[RW * LP];
     const Job J = get_job(jl, blockIdx.y);
-    const ReconConst &K = *J.c;
+    KRef K = cst(J.c);
     const int C = K.Cs, P = K.P;
     const int L = blockIdx.x;
     const int g = (L / 8 / C) * 8 + (L % 8);
@@ -95,16 +157,17 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, cons
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int y = rb * RW + rr;
     const size_t nn = (size_t)N * N;
-    const uint16_t *raw = J.raw + (size_t)d * P * nn + (size_t)y * N;
+    const FSIM_GLB uint16_t *raw = glb(J.raw) + (size_t)d * P * nn + (size_t)y * N;
+    const FSIM_GLB double *flo = glb(K.fade_lo), *fhi = glb(K.fade_hi);
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
     float fade[E];
 #pragma unroll
-    for (int m = 0; m < E; m++) fade[m] = fade_at(K.fade_lo, K.fade_hi, t + T * m, y);
+    for (int m = 0; m < E; m++) fade[m] = fade_at(flo, fhi, t + T * m, y);
     for (int p = 0; p < P; p++) {
-        const float2 s = K.S[d][crow][p];
-        const uint16_t *rp = raw + (size_t)p * nn;
+        const float2 s = ld(&K.S[d][crow][p]);
+        const FSIM_GLB uint16_t *rp = raw + (size_t)p * nn;
 #pragma unroll
         for (int m = 0; m < E; m++) {
             const float px = (float)rp[t + T * m] * fade[m];
@@ -113,99 +176,81 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, cons
         }
     }
     fft_regs<N, E, false, RowWave<N>::value>(v, lds + rr * LP, 1, t, tw);
-    float2 *out = J.A + ((size_t)(d * C + c) * N + y) * N;
+    FSIM_GLB float2 *out = glb(J.A) + ((size_t)(d * C + c) * N + y) * N;
 #pragma unroll
-    for (int m = 0; m < E; m++) out[t + T * m] = v[m];
-}
-#endif
-
-template <int N>
-__global__ void __launch_bounds__(256) k_sep_rows(JobList jl, const float2 *__restrict__ tw) {
-    constexpr int E = FftE<N>::value, T = N / E;
-    constexpr int RW = T >= 256 ? 1 : 256 / T;
-    constexpr int LP = FftLds<N, E>::len;
-    constexpr int rows_per_dir = N / RW;
-    __shared__ float2 lds[RW * LP];
-    const Job J = get_job(jl, blockIdx.y);
-    const ReconConst &K = *J.c;
-    const int C = K.Cs, P = K.P;
-    const int d = blockIdx.x / rows_per_dir, rb = blockIdx.x % rows_per_dir;
-    const int t = threadIdx.x % T, rr = threadIdx.x / T;
-    const int y = rb * RW + rr;
-    const size_t nn = (size_t)N * N;
-    const uint16_t *raw = J.raw + (size_t)d * P * nn + (size_t)y * N;
-    float px[kMaxPhase][E];
-#pragma unroll
-    for (int m = 0; m < E; m++) {
-        const float f = fade_at(K.fade_lo, K.fade_hi, t + T * m, y);
-#pragma unroll
-        for (int p = 0; p < kMaxPhase; p++)
-            if (p < P) px[p][m] = (float)raw[(size_t)p * nn + t + T * m] * f;
-    }
-    for (int c = 0; c < C; c++) {
-        float2 v[E];
-#pragma unroll
-        for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
-#pragma unroll
-        for (int p = 0; p < kMaxPhase; p++) {
-            if (p < P) {
-                const float2 sp = K.S[d][K.srow[c]][p];
-#pragma unroll
-                for (int m = 0; m < E; m++) {
-                    v[m].x += sp.x * px[p][m];
-                    v[m].y += sp.y * px[p][m];
-                }
-            }
-        }
-        fft_regs<N, E, false, RowWave<N>::value>(v, lds + rr * LP, 1, t, tw);
-        if constexpr (!RowWave<N>::value) __syncthreads();
-        float2 *out = J.A + ((size_t)(d * C + c) * N + y) * N;
-#pragma unroll
-        for (int m = 0; m < E; m++) out[t + T * m] = v[m];
-    }
+    for (int m = 0; m < E; m++) st(out + t + T * m, v[m]);
 }
 
 // ---------------------------------------------------------------------------
-// K2: shifted components: column FFT, * conj(OTF_band), column IFFT (1/n).
+// K2: shifted components: column FFT, * conj(OTF_band), column IFFT (1/n),
+// column factor ry(y) of the sub-pixel ramp, column FFT.  Blocks own
+// CW-column tiles; the tiles of one component run in order on one XCD.
 // ---------------------------------------------------------------------------
+// Waves per SIMD a kernel's registers are planned for: the blocks that the
+// 160 KB of LDS per CU hold, clamped to [1, cap] (cap 4 = 128 VGPRs).
+constexpr int lds_waves_per_simd(int threads, int lds_bytes, int cap) {
+    const int w = (163840 / lds_bytes) * threads / 256;
+    return w < 1 ? 1 : (w > cap ? cap : w);
+}
+
 template <int N>
 struct ColShiftTile {
     static constexpr int e = ColE<N>::value, t = N / e;
     static constexpr int cw0 = ColTile<N, t, e>::value;
     static constexpr int cw = cw0 < FSIM_COL_CW ? cw0 : FSIM_COL_CW;
     static constexpr int value = t * cw;
+    static constexpr int lds = FftLds<N, e>::len * cw;
+    static constexpr int wpe = N > 512 ? 1 : lds_waves_per_simd(value, lds * 8, FSIM_COL_WPE);
 };
 
 template <int N>
-__global__ void __launch_bounds__(ColShiftTile<N>::value) k_col_shift(JobList jl, const float2 *__restrict__ tw) {
+__global__ void __launch_bounds__(ColShiftTile<N>::value, ColShiftTile<N>::wpe) k_col_shift(JobList jl, int nsh_all, int total,
+                                                                     const float2 *__restrict__ tw) {
     constexpr int E = ColE<N>::value, T = N / E;
     constexpr int CW = ColShiftTile<N>::cw;
     constexpr int tiles = N / CW;
-    __shared__ float2 lds[FftLds<N, E>::len * CW];
-    const Job J = get_job(jl, blockIdx.y);
-    const ReconConst &K = *J.c;
-    const int C = K.Cs, nsh = C - 1;
-    const int tile = blockIdx.x % tiles, dc = blockIdx.x / tiles;
+    __shared__ float2 lds[ColShiftTile<N>::lds];
+    // nsh_all = D * (Cs - 1) shifted components per stack; total = stacks *
+    // nsh_all * tiles
+    int w;
+    if (FSIM_COL_XCD) {
+        w = xcd_linear(total);
+        if (w >= total) return;
+    } else {
+        w = blockIdx.x;
+    }
+    const int tile = w % tiles, u = w / tiles;
+    const Job J = get_job(jl, u / nsh_all);
+    KRef K = cst(J.c);
+    const int C = K.Cs, nsh = C - 1, dc = u % nsh_all;
     const int d = dc / nsh, c = 1 + dc % nsh;   // stored shifted component
     const int cl = threadIdx.x % CW, t = threadIdx.x / CW;
     const int x = tile * CW + cl;
-    float2 *col = J.A + (size_t)(d * C + c) * N * N + x;
+    FSIM_GLB float2 *col = glb(J.A) + (size_t)(d * C + c) * N * N + x;
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = col[(size_t)(t + T * m) * N];
+    for (int m = 0; m < E; m++) v[m] = ld(col + (size_t)(t + T * m) * N);
     fft_regs<N, E, false>(v, lds + cl, CW, t, tw);
     const int band = K.sband[c];
-    const float2 *otf = K.otf2d + (size_t)(K.otf_nb > 1 ? band : 0) * N * N + x;
+    const FSIM_GLB float2 *otf = glb(K.otf2d) + (size_t)(K.otf_nb > 1 ? band : 0) * N * N + x;
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = times_conj_java(v[m], otf[(size_t)(t + T * m) * N]);
+    for (int m = 0; m < E; m++) v[m] = times_conj_java(v[m], ld(otf + (size_t)(t + T * m) * N));
     fft_regs<N, E, true>(v, lds + cl, CW, t, tw);
     const float sc = 1.0f / N;
+    const FSIM_GLB float2 *ry = glb(K.ramp) + ((size_t)(d * kMaxComp + K.sramp[c]) * 2 + 1) * N;
 #pragma unroll
-    for (int m = 0; m < E; m++) col[(size_t)(t + T * m) * N] = cscale(v[m], sc);
+    for (int m = 0; m < E; m++) {
+        const float2 r = ld(ry + t + T * m);
+        v[m] = rot_java(cscale(v[m], sc), r.x, r.y);
+    }
+    fft_regs<N, E, false>(v, lds + cl, CW, t, tw);
+#pragma unroll
+    for (int m = 0; m < E; m++) st(col + (size_t)(t + T * m) * N, v[m]);
 }
 
 // ---------------------------------------------------------------------------
-// K3: shifted components: row IFFT (1/n), sub-pixel ramp, row FFT.
+// K3: shifted components: row IFFT (1/n), row factor rx(x) of the sub-pixel
+// ramp, row FFT.
 // ---------------------------------------------------------------------------
 template <int N>
 __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__restrict__ tw) {
@@ -215,48 +260,60 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
     constexpr int LP = FftLds<N, E>::len;
     __shared__ float2 lds[RW * LP];
     const Job J = get_job(jl, blockIdx.y);
-    const ReconConst &K = *J.c;
+    KRef K = cst(J.c);
     const int C = K.Cs, nsh = C - 1;
     const int rb = blockIdx.x % rblocks, dc = blockIdx.x / rblocks;
     const int d = dc / nsh, c = 1 + dc % nsh;   // stored shifted component
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int y = rb * RW + rr;
-    float2 *row = J.A + ((size_t)(d * C + c) * N + y) * N;
+    FSIM_GLB float2 *row = glb(J.A) + ((size_t)(d * C + c) * N + y) * N;
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = row[t + T * m];
+    for (int m = 0; m < E; m++) v[m] = ld(row + t + T * m);
     fft_regs<N, E, true, RowWave<N>::value>(v, lds + rr * LP, 1, t, tw);
-    // sub-pixel ramp exp(2 pi i (fx x + fy y) / n) as a product of the
-    // per-channel row and column tables (SimUtils.java:74-81 shift phases)
+    // row factor rx(x) of the sub-pixel ramp (the column factor was applied
+    // by the column pass)
     const float sc = 1.0f / N;
-#if FSIM_RAMP_TABLE
-    const float2 *rx = K.ramp + ((size_t)(d * kMaxComp + K.sramp[c]) * 2) * N;
-    const float2 ry = rx[N + y];
+    const FSIM_GLB float2 *rx = glb(K.ramp) + ((size_t)(d * kMaxComp + K.sramp[c]) * 2) * N;
 #pragma unroll
     for (int m = 0; m < E; m++) {
-        const float2 r = cmul(rx[t + T * m], ry);
+        const float2 r = ld(rx + t + T * m);
         v[m] = rot_java(cscale(v[m], sc), r.x, r.y);
     }
-#else
-    const double fx = K.fx[d][K.sramp[c]], fy = K.fy[d][K.sramp[c]];
-#pragma unroll
-    for (int m = 0; m < E; m++) {
-        float co, si;
-        shift_cos_sin(fx, fy, t + T * m, y, N, co, si);
-        v[m] = rot_java(cscale(v[m], sc), co, si);
-    }
-#endif
     fft_regs<N, E, false, RowWave<N>::value>(v, lds + rr * LP, 1, t, tw);
 #pragma unroll
-    for (int m = 0; m < E; m++) row[t + T * m] = v[m];
+    for (int m = 0; m < E; m++) st(row + t + T * m, v[m]);
+}
+
+// Block -> (stack, column tile) for the gather kernel.  With >= 8 stacks in a
+// launch, XCD x takes stacks x, x+8, ... whole, tiles in order, so the cache
+// lines that neighbouring tiles share (paste offsets are not tile aligned)
+// stay in one L2.  With fewer stacks the tiles of a stack are spread over all
+// XCDs, with adjacent tile pairs on the same XCD.
+__device__ __forceinline__ bool gather_block(int tiles, int njobs, int &job, int &tile) {
+    const int L = blockIdx.x;
+    if (njobs >= 8 && FSIM_JOBMAP) {
+        const int xcd = L % 8, k = L / 8;
+        job = xcd + 8 * (k / tiles);
+        tile = k % tiles;
+        return job < njobs;
+    }
+    job = L / tiles;
+    const int bx = L % tiles;
+    tile = tiles % 16 ? bx : (bx / 16) * 16 + (bx % 8) * 2 + (bx / 8) % 2;
+    return true;
+}
+
+static inline unsigned gather_grid(int tiles, int njobs) {
+    return (unsigned)(njobs >= 8 && FSIM_JOBMAP ? tiles * ((njobs + 7) / 8) * 8 : tiles * njobs);
 }
 
 // ---------------------------------------------------------------------------
 // K5: gather into the doubled grid, filter, column IFFT.
 // Elements per thread for the n-point (en) and 2n-point (em) column
-// transforms and the tile width: up to n = 512 a 1024-thread block with 16
-// elements per thread fits 128 VGPRs; from n = 1024 on, 32 elements per thread
-// in 512-thread blocks.
+// transforms and the tile width: up to n = 512 a 512-thread block (8 columns,
+// 16 elements per thread for the 2n-point transform) within 128 VGPRs; from
+// n = 1024 on, 32 elements per thread.
 // ---------------------------------------------------------------------------
 template <int N>
 struct GatherE {
@@ -266,89 +323,138 @@ struct GatherE {
     static constexpr int tk = tn > tm ? tn : tm;
     static constexpr int cw0 = ColTile<2 * N, tk, (en > em ? en : em)>::value;
     static constexpr int cw = cw0 < FSIM_GATHER_CW ? cw0 : FSIM_GATHER_CW;
+    // LDS elements: the larger padded transform, or the n x cw staging tile
+    static constexpr int lpn = FftLds<N, en>::len, lpm = FftLds<2 * N, em>::len;
+    static constexpr int lpk = lpm > lpn ? lpm : lpn;
+    static constexpr int lds = (lpk > N ? lpk : N) * cw;
+    // at most 4 waves per SIMD (128 VGPRs); from n = 1024 on the transforms
+    // need the full register file
+    static constexpr int wpe = N > 512 ? 1 : lds_waves_per_simd(tk * cw, lds * 8, 4);
 };
+
+// pasteFreq from the n grid into the 2n grid (BasicVectors.java:273-278), one
+// axis: output index o, offset off.  p = (o - off) mod 2n; the output takes
+// input index p mod n when p lies in [0, n/2) or [3n/2, 2n).
 template <int N>
-__global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw) k_gather(JobList jl, const float2 *__restrict__ twn,
+__device__ __forceinline__ int paste_p(int o, int off) { return (o - off) & (2 * N - 1); }
+template <int N>
+__device__ __forceinline__ bool paste_ok(int p) { return ((p + N / 2) & (2 * N - 1)) < N; }
+
+// Does any column of [x0, x0 + cw) receive a pasted column (offset off)?
+template <int N>
+__device__ __forceinline__ bool tile_pasted(int x0, int cw, int off) {
+    bool hit = false;
+    for (int q = 0; q < cw; q++) hit |= paste_ok<N>(paste_p<N>(x0 + q, off));
+    return hit;
+}
+
+template <int N>
+__global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::wpe) k_gather(JobList jl, int njobs, const float2 *__restrict__ twn,
                                                  const float2 *__restrict__ twm) {
     constexpr int M = 2 * N;
     constexpr int EN = GatherE<N>::en, TN = N / EN;
     constexpr int EM = GatherE<N>::em, TM = M / EM;
     constexpr int CW = GatherE<N>::cw;
-    constexpr int LPN = FftLds<N, EN>::len, LPM = FftLds<M, EM>::len;
-    constexpr int LPK = LPM > LPN ? LPM : LPN;
-    __shared__ float2 lds[(LPK > N ? LPK : N) * CW];
-    const Job J = get_job(jl, blockIdx.y);
-    const ReconConst &K = *J.c;
+    __shared__ float2 lds[GatherE<N>::lds];
+    constexpr int tiles = M / CW;
+    int job, tile;
+    if (!gather_block(tiles, njobs, job, tile)) return;
+    const Job J = get_job(jl, job);
+    KRef K = cst(J.c);
     const int D = K.D, C = K.C, Cs = K.Cs;
     const int cl = threadIdx.x % CW, t = threadIdx.x / CW;
-    // blocks are dealt to the 8 XCDs round robin; put adjacent tile pairs on
-    // the same XCD (bx and bx + 8 run back to back there) so that source rows
-    // shared by neighbouring tiles (paste offsets are not tile aligned) are
-    // fetched once into its L2, while the pairs stay spread over all XCDs
-    // (tiles differ in how many components land in them)
-    constexpr int tiles = M / CW;
-    const int bx = blockIdx.x;
-    const int tile = tiles % 16 ? bx : (bx / 16) * 16 + (bx % 8) * 2 + (bx / 8) % 2;
     const int xo = tile * CW + cl;
+    const FSIM_GLB float2 *A = glb(J.A);
     float2 acc[EM];
 #pragma unroll
     for (int m = 0; m < EM; m++) acc[m] = make_float2(0.f, 0.f);
     const size_t nn = (size_t)N * N;
     for (int d = 0; d < D; d++) {
-        for (int c = 0; c < C; c++) {
-            // contribution c of direction d, in the reference's summation order
-            const int mir = K.mir[d][c];
-            int xs = paste_src(xo, M, N, K.offx[d][c]);
-            if (mir && xs >= 0) xs = (K.mdx[d][c] - xs + N) & (N - 1);
-            const bool act = (t < TN) && (xs >= 0);
+        // DC component (rows transformed): column FFT, * conj(OTF), staged
+        // through LDS for the row gather.  Skipped as a whole when the tile
+        // lies outside its paste window.
+        if (tile_pasted<N>(tile * CW, CW, K.offx[d][0])) {
+            const int px = paste_p<N>(xo, K.offx[d][0]);
+            const int xs = px & (N - 1);
+            const bool act = (t < TN) && paste_ok<N>(px);
             float2 v[EN];
             if (act) {
-                const float2 *col = J.A + (size_t)(d * Cs + K.src[c]) * nn + xs;
+                const FSIM_GLB float2 *col = A + (size_t)(d * Cs + K.src[0]) * nn + xs;
 #pragma unroll
-                for (int m = 0; m < EN; m++) v[m] = col[(size_t)(t + TN * m) * N];
+                for (int m = 0; m < EN; m++) v[m] = ld(col + (size_t)(t + TN * m) * N);
             }
             fft_regs<N, EN, false>(v, lds + cl, CW, t, twn, act);
             if (act) {
-                if (c == 0) {
-                    const float2 *otf = K.otf2d + xs;
+                const FSIM_GLB float2 *otf = glb(K.otf2d) + xs;
 #pragma unroll
-                    for (int m = 0; m < EN; m++) v[m] = times_conj_java(v[m], otf[(size_t)(t + TN * m) * N]);
+                for (int m = 0; m < EN; m++) {
+                    v[m] = times_conj_java(v[m], ld(otf + (size_t)(t + TN * m) * N));
+                    lds[(t + TN * m) * CW + cl] = v[m];
                 }
-#pragma unroll
-                for (int m = 0; m < EN; m++) lds[(t + TN * m) * CW + cl] = v[m];
             }
             __syncthreads();
-            if (t < TM && xs >= 0) {
-                const int oy = K.offy[d][c];
-                const int my = K.mdy[d][c];
+            if (t < TM && paste_ok<N>(px)) {
+                const int oy = K.offy[d][0];
 #pragma unroll
                 for (int m = 0; m < EM; m++) {
-                    int ys = paste_src(t + TM * m, M, N, oy);
-                    if (ys >= 0) {
-                        if (mir) ys = (my - ys + N) & (N - 1);
-                        const float2 s = lds[ys * CW + cl];
-                        acc[m].x += s.x;
-                        acc[m].y += mir ? -s.y : s.y;
-                    }
+                    const int py = paste_p<N>(t + TM * m, oy);
+                    if (paste_ok<N>(py)) acc[m] = cadd(acc[m], lds[(py & (N - 1)) * CW + cl]);
                 }
             }
             __syncthreads();
         }
+        // shifted components: full spectra in A, read in place; a mirrored
+        // (neg) contribution reads its pos partner at the mirrored index and
+        // conjugates.  Loads first (skipped where the row is not pasted; the
+        // register then stays zero), sums after, so the loads of a
+        // contribution are in flight together.
+        for (int c = 1; c < C; c++) {
+            const int offx = K.offx[d][c];
+            if (!tile_pasted<N>(tile * CW, CW, offx)) continue;
+            const int px = paste_p<N>(xo, offx);
+            const bool colok = (t < TM) && paste_ok<N>(px);
+            const int mir = K.mir[d][c];
+            const int oy = K.offy[d][c], my = K.mdy[d][c];
+            int xs = px & (N - 1);
+            if (mir) xs = (K.mdx[d][c] - xs) & (N - 1);
+            const FSIM_GLB float2 *comp = A + (size_t)(d * Cs + K.src[c]) * nn + xs;
+            const float sg = mir ? -1.0f : 1.0f;
+            // two halves of EM / 2 loads each bound the registers in flight
+#pragma unroll
+            for (int h = 0; h < EM; h += EM / 2) {
+                float2 sv[EM / 2];
+#pragma unroll
+                for (int i = 0; i < EM / 2; i++) {
+                    sv[i] = make_float2(0.f, 0.f);
+                    const int py = paste_p<N>(t + TM * (h + i), oy);
+                    int ys = py & (N - 1);
+                    if (mir) ys = (my - ys) & (N - 1);
+                    if (colok && paste_ok<N>(py)) sv[i] = ld(comp + (size_t)ys * N);
+                }
+#pragma unroll
+                for (int i = 0; i < EM / 2; i++) {
+                    acc[h + i].x += sv[i].x;
+                    acc[h + i].y += sg * sv[i].y;
+                }
+            }
+        }
     }
     // fullResult.times(wienerDenom); fullResult.times(apo)
     if (t < TM) {
+        const FSIM_GLB float2 *wa = glb(K.wa) + xo;
 #pragma unroll
         for (int m = 0; m < EM; m++) {
-            const float2 wa = K.wa[(size_t)(t + TM * m) * M + xo];
-            acc[m] = make_float2(acc[m].x * wa.x, acc[m].y * wa.x);
-            acc[m] = make_float2(acc[m].x * wa.y, acc[m].y * wa.y);
+            const float2 w = ld(wa + (size_t)(t + TM * m) * M);
+            acc[m] = make_float2(acc[m].x * w.x, acc[m].y * w.x);
+            acc[m] = make_float2(acc[m].x * w.y, acc[m].y * w.y);
         }
     }
     fft_regs<M, EM, true>(acc, lds + cl, CW, t, twm, t < TM);
     if (t < TM) {
         const float sc = 1.0f / M;
+        FSIM_GLB float2 *B = glb(J.B) + xo;
 #pragma unroll
-        for (int m = 0; m < EM; m++) J.B[(size_t)(t + TM * m) * M + xo] = cscale(acc[m], sc);
+        for (int m = 0; m < EM; m++) st(B + (size_t)(t + TM * m) * M, cscale(acc[m], sc));
     }
 }
 
@@ -364,13 +470,13 @@ __global__ void __launch_bounds__(256) k_row_real(JobList jl, const float2 *__re
     const Job J = get_job(jl, blockIdx.y);
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int y = blockIdx.x * RW + rr;
-    const float2 *row = J.B + (size_t)y * M;
+    const FSIM_GLB float2 *row = glb(J.B) + (size_t)y * M;
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = row[t + T * m];
+    for (int m = 0; m < E; m++) v[m] = ld(row + t + T * m);
     fft_regs<M, E, true, RowWave<M>::value>(v, lds + rr * LP, 1, t, tw);
     const float sc = 1.0f / M;
-    float *o = J.out + (size_t)y * M;
+    FSIM_GLB float *o = glb(J.out) + (size_t)y * M;
 #pragma unroll
     for (int m = 0; m < E; m++) o[t + T * m] = v[m].x * sc;
 }
@@ -413,19 +519,17 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, hipStre
     stage_mark(-1, s, ev);
     {
         constexpr int E = FftE<N>::value, T = N / E, RW = T >= 256 ? 1 : 256 / T;
-#if FSIM_K1_ALLC
-        hipLaunchKernelGGL(k_sep_rows<N>, dim3(D * (N / RW), njobs), dim3(T * RW), 0, s, jl, twn);
-#else
         const int ngroups = D * (N / RW);
         const int gx = ((ngroups + 7) / 8) * 8 * C;
         hipLaunchKernelGGL(k_sep_rows1<N>, dim3(gx, njobs), dim3(T * RW), 0, s, jl, ngroups, twn);
-#endif
         stage_mark(0, s, ev);
     }
     if (C > 1) {
         {
             constexpr int E = ColE<N>::value, T = N / E, CW = ColShiftTile<N>::cw;
-            hipLaunchKernelGGL(k_col_shift<N>, dim3(D * (C - 1) * (N / CW), njobs), dim3(T * CW), 0, s, jl, twn);
+            const int nsh_all = D * (C - 1), total = njobs * nsh_all * (N / CW);
+            hipLaunchKernelGGL(k_col_shift<N>, dim3(FSIM_COL_XCD ? xcd_grid(total) : (unsigned)total), dim3(T * CW),
+                               0, s, jl, nsh_all, total, twn);
             stage_mark(1, s, ev);
         }
         constexpr int E = FftE<N>::value, T = N / E;
@@ -435,7 +539,8 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, hipStre
     }
     {
         constexpr int TK = GatherE<N>::tk, CW = GatherE<N>::cw;
-        hipLaunchKernelGGL(k_gather<N>, dim3(M / CW, njobs), dim3(TK * CW), 0, s, jl, twn, twm);
+        hipLaunchKernelGGL(k_gather<N>, dim3(gather_grid(M / CW, njobs)), dim3(TK * CW), 0, s, jl, njobs, twn,
+                           twm);
         stage_mark(3, s, ev);
     }
     {
