@@ -17,6 +17,9 @@ __global__ void __launch_bounds__(256) k_fft_rows(float2 *__restrict__ data, int
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int LP = FftLds<L, E>::len;
     __shared__ float2 lds[RW * LP];
+    // row transforms: small blocks, twiddles from sincospi (no table fill)
+    using TW = TwLds<L, false>;
+    const float2 *tws = tw;
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int row = blockIdx.x * RW + rr;
     const bool active = row < nrows;
@@ -26,7 +29,7 @@ __global__ void __launch_bounds__(256) k_fft_rows(float2 *__restrict__ data, int
 #pragma unroll
         for (int m = 0; m < E; m++) v[m] = p[t + T * m];
     }
-    fft_regs<L, E, INV, RowWave<L>::value>(v, lds + rr * LP, 1, t, tw, active);
+    fft_regs<L, E, INV, RowWave<L>::value, TW::tl>(v, lds + rr * LP, 1, t, tws, active);
     if (active) {
 #pragma unroll
         for (int m = 0; m < E; m++) p[t + T * m] = INV ? cscale(v[m], scale) : v[m];
@@ -40,6 +43,10 @@ __global__ void __launch_bounds__(1024) k_fft_cols(float2 *__restrict__ data, in
     constexpr int E = FftE<L>::value, T = L / E;
     constexpr int CW = ColW<L>::value;
     __shared__ float2 lds[FftLds<L, E>::len * CW];
+    // the twiddle table where the LDS holds it beside the tile
+    using TW = TwLds<L, (FftLds<L, E>::len * CW + L) * 8 <= 163840>;
+    __shared__ float2 twl[TW::len];
+    const float2 *tws = tw_table<TW::tl, T * CW>(twl, tw);
     const int c = threadIdx.x % CW, t = threadIdx.x / CW;
     const int col = blockIdx.x * CW + c;
     const bool active = col < ncols;
@@ -48,7 +55,7 @@ __global__ void __launch_bounds__(1024) k_fft_cols(float2 *__restrict__ data, in
 #pragma unroll
         for (int m = 0; m < E; m++) v[m] = data[(size_t)(t + T * m) * pitch + col];
     }
-    fft_regs<L, E, INV>(v, lds + c, CW, t, tw, active);
+    fft_regs<L, E, INV, false, TW::tl>(v, lds + c, CW, t, tws, active);
     if (active) {
 #pragma unroll
         for (int m = 0; m < E; m++)

@@ -15,14 +15,52 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// Default twiddle source of the kernels: 3 = a block-shared LDS table
+// exp(-2 pi i s / TL) read per butterfly; 2 = sincospi of the first twiddle of
+// each butterfly and its powers by complex products.
 #ifndef FSIM_TW_MODE
-#define FSIM_TW_MODE 2
+#define FSIM_TW_MODE 3
 #endif
 #ifndef FSIM_LDS_PAD
 #define FSIM_LDS_PAD 1
 #endif
 
+// Global (address space 1) and constant (4) views of device pointers that
+// arrive inside structures.  With them the compiler emits global_* and s_load
+// accesses instead of flat_* ones, whose completion also counts against the
+// LDS counter: every LDS wait of an FFT exchange would otherwise wait for the
+// outstanding global loads as well.
+#define FSIM_GLB __attribute__((address_space(1)))
+#define FSIM_CST __attribute__((address_space(4)))
+
 namespace fsim {
+
+template <typename T>
+__device__ __forceinline__ FSIM_GLB T *glb(T *p) {
+    return (FSIM_GLB T *)p;
+}
+template <typename T>
+__device__ __forceinline__ const FSIM_CST T &cst(const T *p) {
+    return *(const FSIM_CST T *)p;
+}
+
+// float2 moves through a native vector type (the float2 class only binds
+// generic references)
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld(const FSIM_GLB float2 *p) {
+    const f2v x = *(const FSIM_GLB f2v *)p;
+    return make_float2(x.x, x.y);
+}
+__device__ __forceinline__ float2 ld(const FSIM_CST float2 *p) {
+    const f2v x = *(const FSIM_CST f2v *)p;
+    return make_float2(x.x, x.y);
+}
+__device__ __forceinline__ void st(FSIM_GLB float2 *p, float2 v) {
+    f2v x;
+    x.x = v.x;
+    x.y = v.y;
+    *(FSIM_GLB f2v *)p = x;
+}
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -150,8 +188,9 @@ struct FftLds {
 };
 
 // Stockham pass NS (sub-transform size so far) of radix R = min(E, 16, L/NS);
-// recursion walks the passes at compile time.
-template <int L, int E, int NS, bool INV, bool FIRST, bool WAVE>
+// recursion walks the passes at compile time.  TL > 0: tw is a twiddle table
+// of length TL (a multiple of L) in LDS; TL = 0: twiddles from sincospi.
+template <int L, int E, int NS, bool INV, bool FIRST, bool WAVE, int TL>
 __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, int ls, int t,
                                          const float2 *__restrict__ tw, bool active) {
     constexpr int REM = L / NS;
@@ -189,33 +228,28 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, int ls, in
             // (the empty asm emits no instruction)
             asm volatile("" : "+v"(k));
             if constexpr (NS > 1) {
-                const int st = (L / (NS * R)) * k;
-#if FSIM_TW_MODE == 1 || FSIM_TW_MODE == 2
-                // first twiddle of the butterfly, powers by complex products
-#if FSIM_TW_MODE == 2
-                // exp(-+2 pi i k / (NS R)): the argument k/(NS R) is exact in
-                // float (power-of-two denominator), sincospi is accurate
-                float sw, cw;
-                sincospif((INV ? 2.0f : -2.0f) * (float)k / (float)(NS * R), &sw, &cw);
-                float2 w1 = make_float2(cw, sw);
-#else
-                float2 w1 = tw[st];
-                if (INV) w1.y = -w1.y;
-#endif
-                float2 w = w1;
+                if constexpr (TL > 0) {
+                    // w^r = exp(-+2 pi i k r / (NS R)) = table[(TL / (NS R)) k r]
+                    constexpr int SC = TL / (NS * R);
 #pragma unroll
-                for (int r = 1; r < R; r++) {
-                    u[r] = cmul(u[r], w);
-                    w = cmul(w, w1);
-                }
-#else
+                    for (int r = 1; r < R; r++) {
+                        float2 w = tw[SC * k * r];
+                        if (INV) w.y = -w.y;
+                        u[r] = cmul(u[r], w);
+                    }
+                } else {
+                    // exp(-+2 pi i k / (NS R)): the argument k/(NS R) is exact
+                    // in float (power-of-two denominator), sincospi is accurate
+                    float sw, cw;
+                    sincospif((INV ? 2.0f : -2.0f) * (float)k / (float)(NS * R), &sw, &cw);
+                    const float2 w1 = make_float2(cw, sw);
+                    float2 w = w1;
 #pragma unroll
-                for (int r = 1; r < R; r++) {
-                    float2 w = tw[r * st];
-                    if (INV) w.y = -w.y;
-                    u[r] = cmul(u[r], w);
+                    for (int r = 1; r < R; r++) {
+                        u[r] = cmul(u[r], w);
+                        w = cmul(w, w1);
+                    }
                 }
-#endif
             }
             dft_reg<R, INV>(u);
             if constexpr (LAST) {
@@ -233,7 +267,7 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, int ls, in
     }
     if constexpr (!LAST) {
         pass_sync<WAVE>();
-        fft_pass<L, E, NS * R, INV, false, WAVE>(v, lds, ls, t, tw, active);
+        fft_pass<L, E, NS * R, INV, false, WAVE, TL>(v, lds, ls, t, tw, active);
     }
 }
 
@@ -245,12 +279,52 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, int ls, in
 // T = L/E threads lie inside one wavefront; the caller's own later LDS use
 // of the region is then ordered by the same wave.  On return the region may
 // be reused (the last pass ends with a synchronisation after its reads).
-template <int L, int E, bool INV, bool WAVE = false>
+// TL > 0: tw is the twiddle table returned by tw_table<TL> (TL a multiple of
+// L); TL = 0: no table, twiddles from sincospi (tw unused).
+template <int L, int E, bool INV, bool WAVE, int TL>
 __device__ __forceinline__ void fft_regs(float2 (&v)[E], float2 *lds, int ls, int t,
                                          const float2 *__restrict__ tw, bool active = true) {
     static_assert(L % E == 0, "E must divide L");
+    static_assert(TL == 0 || TL % L == 0, "twiddle table length must be a multiple of L");
     static_assert(!WAVE || (L / E) <= 64, "wave synchronisation needs the transform inside one wavefront");
-    fft_pass<L, E, 1, INV, true, WAVE>(v, lds, ls, t, tw, active);
+    fft_pass<L, E, 1, INV, true, WAVE, TL>(v, lds, ls, t, tw, active);
+}
+
+// Twiddle table of a kernel whose largest transform has L points: tl is the
+// table length passed to fft_regs (0: no table), len the LDS elements the
+// kernel reserves for it.  use = false forces the sincospi form (a kernel
+// whose LDS cannot hold the table).
+template <int L, bool use = true>
+struct TwLds {
+    static constexpr int tl = (FSIM_TW_MODE == 3 && use) ? L : 0;
+    static constexpr int len = tl > 0 ? tl : 1;
+};
+
+// Fills the LDS table twl of length TL (> 0) from the global table
+// g[s] = exp(-2 pi i s / TL) (rounded from double) and synchronises the
+// block; every one of the NT threads must call it.  All loads are issued
+// before the first LDS write.  With TL = 0 it does nothing.
+template <int TL, int NT>
+__device__ __forceinline__ const float2 *tw_table(float2 *twl, const float2 *g) {
+    if constexpr (TL > 0) {
+        constexpr int R = (TL + NT - 1) / NT;
+        const FSIM_GLB float2 *gg = glb(g);
+        float2 tmp[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = (int)threadIdx.x + r * NT;
+            if (TL % NT == 0 || i < TL) tmp[r] = ld(gg + i);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = (int)threadIdx.x + r * NT;
+            if (TL % NT == 0 || i < TL) twl[i] = tmp[r];
+        }
+        __syncthreads();
+        return twl;
+    } else {
+        return g;
+    }
 }
 
 // Row layout (t fastest): one transform per T consecutive threads.

@@ -16,6 +16,7 @@ struct ReconConst {
     const float2 *otf2d;              // otfToVector(band, 0, 0) on the n^2 grid, per OTF band
     const float2 *wa;                 // (2n)^2 of (1/Wiener denominator, apodization)
     const double *fade_lo, *fade_hi;  // fadeBorderCos factors, n each
+    int fade_px;                      // border width; factors are 1.0 inside
     const float2 *ramp;               // [kMaxDir][kMaxComp][x|y][n] sub-pixel ramp factors
     float2 S[kMaxDir][kMaxComp][kMaxPhase];   // separation matrices (toFlt)
     int offx[kMaxDir][kMaxComp], offy[kMaxDir][kMaxComp];   // integer paste offsets
@@ -40,10 +41,13 @@ struct Job {
 };
 
 // Either an explicit device array of jobs or a base job plus strides.
+// raw_aligned: every job's raw frames start on an 8-byte boundary (the
+// separation pass then reads 4 pixels per load).
 struct JobList {
     const Job *jobs;
     Job base;
     size_t raw_stride, a_stride, b_stride, out_stride;
+    bool raw_aligned;
 };
 
 bool recon_size_ok(int n);

@@ -53,43 +53,9 @@
 #define FSIM_COL_WPE 6
 #endif
 
-// Global (address space 1) and constant (4) views of device pointers that
-// arrive inside structures.  With them the compiler emits global_* and s_load
-// accesses instead of flat_* ones, whose completion also counts against the
-// LDS counter: every LDS wait of an FFT exchange would otherwise wait for the
-// outstanding global loads as well.
-#define FSIM_GLB __attribute__((address_space(1)))
-#define FSIM_CST __attribute__((address_space(4)))
-
 namespace fsim {
 
-template <typename T>
-__device__ __forceinline__ FSIM_GLB T *glb(T *p) {
-    return (FSIM_GLB T *)p;
-}
-template <typename T>
-__device__ __forceinline__ const FSIM_CST T &cst(const T *p) {
-    return *(const FSIM_CST T *)p;
-}
 using KRef = const FSIM_CST ReconConst &;
-
-// float2 moves through a native vector type (the float2 class only binds
-// generic references)
-typedef float f2v __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ float2 ld(const FSIM_GLB float2 *p) {
-    const f2v x = *(const FSIM_GLB f2v *)p;
-    return make_float2(x.x, x.y);
-}
-__device__ __forceinline__ float2 ld(const FSIM_CST float2 *p) {
-    const f2v x = *(const FSIM_CST f2v *)p;
-    return make_float2(x.x, x.y);
-}
-__device__ __forceinline__ void st(FSIM_GLB float2 *p, float2 v) {
-    f2v x;
-    x.x = v.x;
-    x.y = v.y;
-    *(FSIM_GLB f2v *)p = x;
-}
 
 __device__ __forceinline__ Job get_job(const JobList &jl, int j) {
     Job J;
@@ -121,24 +87,40 @@ __device__ __forceinline__ int xcd_linear(int total) {
 static inline unsigned xcd_grid(int total) { return (unsigned)((total + 7) / 8) * 8; }
 
 // Fade factor of pixel (x, y): fadeBorderCos on a vector of ones
-// (SimUtils.java:162-185, ReconstructionRunner.java:781-783).
-__device__ __forceinline__ float fade_at(const FSIM_GLB double *lo, const FSIM_GLB double *hi, int x, int y) {
+// (SimUtils.java:162-185, ReconstructionRunner.java:781-783): the products
+// 1 * lo[y] * hi[y] * lo[x] * hi[x], each rounded to float.
+// fade_rowv: (float)((float)(1 * lo[y]) * hi[y]).  In a row, lo[x] is 1.0
+// except in the left border and hi[x] except in the right one (the borders
+// do not overlap), so the column part is one rounded product with the
+// factor that is not 1 (multiplying by 1.0 is exact): fade_factor returns
+// that factor (1.0 inside), fade_colv applies it.
+__device__ __forceinline__ float fade_rowv(double lo_y, double hi_y) {
     float f = 1.0f;
-    f = (float)((double)f * lo[y]);
-    f = (float)((double)f * hi[y]);
-    f = (float)((double)f * lo[x]);
-    f = (float)((double)f * hi[x]);
+    f = (float)((double)f * lo_y);
+    f = (float)((double)f * hi_y);
     return f;
 }
+__device__ __forceinline__ double fade_factor(const FSIM_GLB double *lo, const FSIM_GLB double *hi, int x, int n,
+                                              int px) {
+    const bool left = x < px, right = x >= n - px;
+    const FSIM_GLB double *tab = left ? lo : hi;
+    const double v = tab[(left || right) ? x : 0];
+    return (left || right) ? v : 1.0;
+}
+__device__ __forceinline__ float fade_colv(float fy, double fx) { return (float)((double)fy * fx); }
 
 // ---------------------------------------------------------------------------
 // K1: band separation + row FFT.  One wavefront (T = n/E threads) per row of
 // one direction and one stored component: the P raw rows are read (the
 // blocks of the other components of the row run on the same XCD and find
 // them in L2), faded, combined as c = sum_p S[d][c][p] * raw_p and
-// transformed.
+// transformed.  VEC: each lane reads 4 adjacent pixels per load (8-byte
+// aligned rows) and the combined values reach the transform's element order
+// through the row's LDS region.
 // ---------------------------------------------------------------------------
-template <int N>
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+
+template <int N, bool VEC>
 __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, const float2 *__restrict__ tw) {
     constexpr int E = FftE<N>::value, T = N / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
@@ -152,6 +134,9 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, cons
     const int c = (L / 8) % C;           // stored component
     const int crow = K.srow[c];          // its separation-matrix row
     if (g >= ngroups) return;
+    // row kernels: small blocks, twiddles from sincospi (no table fill)
+    using TW = TwLds<N, false>;
+    const float2 *tws = tw;
     constexpr int rows_per_dir = N / RW;
     const int d = g / rows_per_dir, rb = g % rows_per_dir;
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
@@ -159,23 +144,107 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, cons
     const size_t nn = (size_t)N * N;
     const FSIM_GLB uint16_t *raw = glb(J.raw) + (size_t)d * P * nn + (size_t)y * N;
     const FSIM_GLB double *flo = glb(K.fade_lo), *fhi = glb(K.fade_hi);
+    const int fpx = K.fade_px;
     float2 v[E];
+    // The loads of the row (fade factors, the pixels of up to G phases) are
+    // issued before their first use, so a wave waits for memory once per G
+    // phases.
+    constexpr int G = 4;
+    if constexpr (VEC) {
+        static_assert(E % 4 == 0, "4 pixels per load");
+        constexpr int Q = E / 4;
+        using PL = FftLds<N, E>;
+        // lane t holds pixels x = 4 t + 4 T k + j, k < Q, j < 4
+        double fx[Q][4];
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
-    float fade[E];
+        for (int k = 0; k < Q; k++)
 #pragma unroll
-    for (int m = 0; m < E; m++) fade[m] = fade_at(flo, fhi, t + T * m, y);
-    for (int p = 0; p < P; p++) {
-        const float2 s = ld(&K.S[d][crow][p]);
-        const FSIM_GLB uint16_t *rp = raw + (size_t)p * nn;
+            for (int j = 0; j < 4; j++) fx[k][j] = fade_factor(flo, fhi, 4 * t + 4 * T * k + j, N, fpx);
+        const double fly = flo[y], fhy = fhi[y];
+        float2 w[Q][4];
+        float fd[Q][4];
+        for (int p0 = 0; p0 < P; p0 += G) {
+            u16x4 r[G][Q];
 #pragma unroll
-        for (int m = 0; m < E; m++) {
-            const float px = (float)rp[t + T * m] * fade[m];
-            v[m].x += s.x * px;
-            v[m].y += s.y * px;
+            for (int g = 0; g < G; g++)
+                if (p0 + g < P) {
+                    const FSIM_GLB u16x4 *rp = (const FSIM_GLB u16x4 *)(raw + (size_t)(p0 + g) * nn);
+#pragma unroll
+                    for (int k = 0; k < Q; k++) r[g][k] = rp[t + T * k];
+                }
+            if (p0 == 0) {
+                const float fy = fade_rowv(fly, fhy);
+#pragma unroll
+                for (int k = 0; k < Q; k++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        w[k][j] = make_float2(0.f, 0.f);
+                        fd[k][j] = fade_colv(fy, fx[k][j]);
+                    }
+            }
+#pragma unroll
+            for (int g = 0; g < G; g++)
+                if (p0 + g < P) {
+                    const float2 s = ld(&K.S[d][crow][p0 + g]);
+#pragma unroll
+                    for (int k = 0; k < Q; k++) {
+                        const float r4[4] = {(float)r[g][k].x, (float)r[g][k].y, (float)r[g][k].z,
+                                             (float)r[g][k].w};
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            const float px = r4[j] * fd[k][j];
+                            w[k][j].x += s.x * px;
+                            w[k][j].y += s.y * px;
+                        }
+                    }
+                }
+        }
+        float2 *row = lds + rr * LP;
+#pragma unroll
+        for (int k = 0; k < Q; k++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) row[PL::pad(4 * t + 4 * T * k + j)] = w[k][j];
+        pass_sync<RowWave<N>::value>();
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = row[PL::pad(t + T * m)];
+        pass_sync<RowWave<N>::value>();
+    } else {
+        double fx[E];
+#pragma unroll
+        for (int m = 0; m < E; m++) fx[m] = fade_factor(flo, fhi, t + T * m, N, fpx);
+        const double fly = flo[y], fhy = fhi[y];
+        float fade[E];
+        for (int p0 = 0; p0 < P; p0 += G) {
+            uint16_t r[G][E];
+#pragma unroll
+            for (int g = 0; g < G; g++)
+                if (p0 + g < P) {
+                    const FSIM_GLB uint16_t *rp = raw + (size_t)(p0 + g) * nn;
+#pragma unroll
+                    for (int m = 0; m < E; m++) r[g][m] = rp[t + T * m];
+                }
+            if (p0 == 0) {
+                const float fy = fade_rowv(fly, fhy);
+#pragma unroll
+                for (int m = 0; m < E; m++) {
+                    v[m] = make_float2(0.f, 0.f);
+                    fade[m] = fade_colv(fy, fx[m]);
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < G; g++)
+                if (p0 + g < P) {
+                    const float2 s = ld(&K.S[d][crow][p0 + g]);
+#pragma unroll
+                    for (int m = 0; m < E; m++) {
+                        const float px = (float)r[g][m] * fade[m];
+                        v[m].x += s.x * px;
+                        v[m].y += s.y * px;
+                    }
+                }
         }
     }
-    fft_regs<N, E, false, RowWave<N>::value>(v, lds + rr * LP, 1, t, tw);
+    fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
     FSIM_GLB float2 *out = glb(J.A) + ((size_t)(d * C + c) * N + y) * N;
 #pragma unroll
     for (int m = 0; m < E; m++) st(out + t + T * m, v[m]);
@@ -200,7 +269,8 @@ struct ColShiftTile {
     static constexpr int cw = cw0 < FSIM_COL_CW ? cw0 : FSIM_COL_CW;
     static constexpr int value = t * cw;
     static constexpr int lds = FftLds<N, e>::len * cw;
-    static constexpr int wpe = N > 512 ? 1 : lds_waves_per_simd(value, lds * 8, FSIM_COL_WPE);
+    static constexpr bool tw_fits = (lds + N) * 8 <= 163840;
+    static constexpr int wpe = N > 512 ? 1 : lds_waves_per_simd(value, (lds + N) * 8, FSIM_COL_WPE);
 };
 
 template <int N>
@@ -219,6 +289,9 @@ __global__ void __launch_bounds__(ColShiftTile<N>::value, ColShiftTile<N>::wpe) 
     } else {
         w = blockIdx.x;
     }
+    using TW = TwLds<N, ColShiftTile<N>::tw_fits>;
+    __shared__ float2 twl[TW::len];
+    const float2 *tws = tw_table<TW::tl, ColShiftTile<N>::value>(twl, tw);
     const int tile = w % tiles, u = w / tiles;
     const Job J = get_job(jl, u / nsh_all);
     KRef K = cst(J.c);
@@ -230,20 +303,28 @@ __global__ void __launch_bounds__(ColShiftTile<N>::value, ColShiftTile<N>::wpe) 
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = ld(col + (size_t)(t + T * m) * N);
-    fft_regs<N, E, false>(v, lds + cl, CW, t, tw);
+    fft_regs<N, E, false, false, TW::tl>(v, lds + cl, CW, t, tws);
     const int band = K.sband[c];
     const FSIM_GLB float2 *otf = glb(K.otf2d) + (size_t)(K.otf_nb > 1 ? band : 0) * N * N + x;
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = times_conj_java(v[m], ld(otf + (size_t)(t + T * m) * N));
-    fft_regs<N, E, true>(v, lds + cl, CW, t, tw);
-    const float sc = 1.0f / N;
+    // column ramp factors (in flight during the inverse transform where the
+    // registers allow)
     const FSIM_GLB float2 *ry = glb(K.ramp) + ((size_t)(d * kMaxComp + K.sramp[c]) * 2 + 1) * N;
+    constexpr int ER = E <= 8 ? E : 1;
+    float2 r[ER];
+    if constexpr (E <= 8) {
+#pragma unroll
+        for (int m = 0; m < E; m++) r[m] = ld(ry + t + T * m);
+    }
+    fft_regs<N, E, true, false, TW::tl>(v, lds + cl, CW, t, tws);
+    const float sc = 1.0f / N;
 #pragma unroll
     for (int m = 0; m < E; m++) {
-        const float2 r = ld(ry + t + T * m);
-        v[m] = rot_java(cscale(v[m], sc), r.x, r.y);
+        const float2 rm = E <= 8 ? r[m < ER ? m : 0] : ld(ry + t + T * m);
+        v[m] = rot_java(cscale(v[m], sc), rm.x, rm.y);
     }
-    fft_regs<N, E, false>(v, lds + cl, CW, t, tw);
+    fft_regs<N, E, false, false, TW::tl>(v, lds + cl, CW, t, tws);
 #pragma unroll
     for (int m = 0; m < E; m++) st(col + (size_t)(t + T * m) * N, v[m]);
 }
@@ -262,6 +343,8 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
     const Job J = get_job(jl, blockIdx.y);
     KRef K = cst(J.c);
     const int C = K.Cs, nsh = C - 1;
+    using TW = TwLds<N, false>;
+    const float2 *tws = tw;
     const int rb = blockIdx.x % rblocks, dc = blockIdx.x / rblocks;
     const int d = dc / nsh, c = 1 + dc % nsh;   // stored shifted component
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
@@ -270,7 +353,7 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = ld(row + t + T * m);
-    fft_regs<N, E, true, RowWave<N>::value>(v, lds + rr * LP, 1, t, tw);
+    fft_regs<N, E, true, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
     // row factor rx(x) of the sub-pixel ramp (the column factor was applied
     // by the column pass)
     const float sc = 1.0f / N;
@@ -280,7 +363,7 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
         const float2 r = ld(rx + t + T * m);
         v[m] = rot_java(cscale(v[m], sc), r.x, r.y);
     }
-    fft_regs<N, E, false, RowWave<N>::value>(v, lds + rr * LP, 1, t, tw);
+    fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
 #pragma unroll
     for (int m = 0; m < E; m++) st(row + t + T * m, v[m]);
 }
@@ -329,7 +412,7 @@ struct GatherE {
     static constexpr int lds = (lpk > N ? lpk : N) * cw;
     // at most 4 waves per SIMD (128 VGPRs); from n = 1024 on the transforms
     // need the full register file
-    static constexpr int wpe = N > 512 ? 1 : lds_waves_per_simd(tk * cw, lds * 8, 4);
+    static constexpr int wpe = N > 512 ? 1 : lds_waves_per_simd(tk * cw, (lds + 2 * N) * 8, 4);
 };
 
 // pasteFreq from the n grid into the 2n grid (BasicVectors.java:273-278), one
@@ -359,6 +442,11 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
     constexpr int tiles = M / CW;
     int job, tile;
     if (!gather_block(tiles, njobs, job, tile)) return;
+    // one table of length m serves both transforms (the n-point one at
+    // stride 2), where the LDS holds it beside the tile
+    using TW = TwLds<M, (GatherE<N>::lds + M) * 8 <= 163840>;
+    __shared__ float2 twl[TW::len];
+    const float2 *tws = tw_table<TW::tl, GatherE<N>::tk * GatherE<N>::cw>(twl, twm);
     const Job J = get_job(jl, job);
     KRef K = cst(J.c);
     const int D = K.D, C = K.C, Cs = K.Cs;
@@ -369,6 +457,7 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
 #pragma unroll
     for (int m = 0; m < EM; m++) acc[m] = make_float2(0.f, 0.f);
     const size_t nn = (size_t)N * N;
+    // Contributions in the reference's order: directions, then components.
     for (int d = 0; d < D; d++) {
         // DC component (rows transformed): column FFT, * conj(OTF), staged
         // through LDS for the row gather.  Skipped as a whole when the tile
@@ -383,7 +472,7 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
 #pragma unroll
                 for (int m = 0; m < EN; m++) v[m] = ld(col + (size_t)(t + TN * m) * N);
             }
-            fft_regs<N, EN, false>(v, lds + cl, CW, t, twn, act);
+            fft_regs<N, EN, false, false, TW::tl>(v, lds + cl, CW, t, tws, act);
             if (act) {
                 const FSIM_GLB float2 *otf = glb(K.otf2d) + xs;
 #pragma unroll
@@ -449,7 +538,7 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
             acc[m] = make_float2(acc[m].x * w.y, acc[m].y * w.y);
         }
     }
-    fft_regs<M, EM, true>(acc, lds + cl, CW, t, twm, t < TM);
+    fft_regs<M, EM, true, false, TW::tl>(acc, lds + cl, CW, t, tws, t < TM);
     if (t < TM) {
         const float sc = 1.0f / M;
         FSIM_GLB float2 *B = glb(J.B) + xo;
@@ -467,6 +556,8 @@ __global__ void __launch_bounds__(256) k_row_real(JobList jl, const float2 *__re
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int LP = FftLds<M, E>::len;
     __shared__ float2 lds[RW * LP];
+    using TW = TwLds<M, false>;
+    const float2 *tws = tw;
     const Job J = get_job(jl, blockIdx.y);
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int y = blockIdx.x * RW + rr;
@@ -474,7 +565,7 @@ __global__ void __launch_bounds__(256) k_row_real(JobList jl, const float2 *__re
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = ld(row + t + T * m);
-    fft_regs<M, E, true, RowWave<M>::value>(v, lds + rr * LP, 1, t, tw);
+    fft_regs<M, E, true, RowWave<M>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
     const float sc = 1.0f / M;
     FSIM_GLB float *o = glb(J.out) + (size_t)y * M;
 #pragma unroll
@@ -521,7 +612,10 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, hipStre
         constexpr int E = FftE<N>::value, T = N / E, RW = T >= 256 ? 1 : 256 / T;
         const int ngroups = D * (N / RW);
         const int gx = ((ngroups + 7) / 8) * 8 * C;
-        hipLaunchKernelGGL(k_sep_rows1<N>, dim3(gx, njobs), dim3(T * RW), 0, s, jl, ngroups, twn);
+        if (jl.raw_aligned)
+            hipLaunchKernelGGL((k_sep_rows1<N, true>), dim3(gx, njobs), dim3(T * RW), 0, s, jl, ngroups, twn);
+        else
+            hipLaunchKernelGGL((k_sep_rows1<N, false>), dim3(gx, njobs), dim3(T * RW), 0, s, jl, ngroups, twn);
         stage_mark(0, s, ev);
     }
     if (C > 1) {
@@ -714,6 +808,7 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     FSIM_HIP(hipMemcpyAsync(ch->fade, f.data(), 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
     ch->K.fade_lo = ch->fade;
     ch->K.fade_hi = ch->fade + n;
+    ch->K.fade_px = p->fade_px;
     ch->K.wa = ch->wa;
     {
         // per direction/component ramp tables exp(2 pi i f t / n), t < n, for
@@ -807,6 +902,7 @@ int fsim_reconstruct(fsim_channel ch, const uint16_t *raw, float *out, int count
     jl.a_stride = workspace_a_elems(n, K.D, K.Cs);
     jl.b_stride = workspace_b_elems(n);
     jl.out_stride = (size_t)4 * n * n;
+    jl.raw_aligned = ((uintptr_t)raw & 7) == 0;
     return launch_pipeline(n, jl, count, K.D, K.Cs, s);
 }
 

@@ -193,6 +193,8 @@ struct Server {
             rc = FSIM_ERR_HIP;
         JobList jl{};
         jl.jobs = s->jobs_d;
+        jl.raw_aligned = true;
+        for (int i = 0; i < count; i++) jl.raw_aligned &= ((uintptr_t)s->jobs_h[i].raw & 7) == 0;
         if (!rc) rc = launch_pipeline(k.n, jl, count, k.D, C, s->stream);
         for (int i = 0; i < count && !rc; i++) {
             Request *r = s->reqs[i];

@@ -47,17 +47,29 @@ __device__ __forceinline__ double yh_of(int y, int h) { return (double)((y < h /
 
 // OtfProvider.getOtfVal (OtfProvider.java:189-222) on a device table; both
 // interpolation weights use the lower sample, as the reference does.
+// otf_index: the table index l and fraction f at radius cycl, or -1 where the
+// value is zero; otf_weigh: the value from the sample tab[l].
+__device__ __forceinline__ int otf_index(int ns, double cpm, double cutoff, double cycl, float &f) {
+#pragma clang fp contract(off)
+    f = 0.f;
+    if (cycl >= cutoff) return -1;
+    const double pos = cycl / cpm;
+    if (ceil(pos) >= (double)ns) return -1;
+    const int l = (int)floor(pos);
+    f = (float)(pos - l);
+    return l;
+}
+__device__ __forceinline__ float2 otf_weigh(float2 v, float f) {
+#pragma clang fp contract(off)
+    const float omf = 1 - f;
+    return make_float2(v.x * omf + v.x * f, v.y * omf + v.y * f);
+}
 __device__ __forceinline__ float2 otf_val(const float2 *__restrict__ tab, int ns, double cpm, double cutoff,
                                           double cycl) {
-#pragma clang fp contract(off)
-    if (cycl >= cutoff) return make_float2(0.f, 0.f);
-    const double pos = cycl / cpm;
-    if (ceil(pos) >= (double)ns) return make_float2(0.f, 0.f);
-    const int l = (int)floor(pos);
-    const float f = (float)(pos - l);
-    const float omf = 1 - f;
-    const float2 v = tab[l];
-    return make_float2(v.x * omf + v.x * f, v.y * omf + v.y * f);
+    float f;
+    const int l = otf_index(ns, cpm, cutoff, cycl, f);
+    if (l < 0) return make_float2(0.f, 0.f);
+    return otf_weigh(tab[l], f);
 }
 
 // OtfProvider.getAttVal (OtfProvider.java:246-269)
