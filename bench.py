@@ -27,6 +27,26 @@ HBM_PEAK_GBS = 8000.0
 STAGES = ["k_sep_rows", "k_col_shift", "k_row_shift", "k_gather", "k_row_real"]
 
 
+def profiled_traffic(kernel, n, batch):
+    """HBM bytes per launch of `kernel` (read + write) from the committed
+    rocprofv3 counter summary of this workload (profiles/), or None when no
+    summary covers this n and batch.  Read bytes are 2 x FETCH_SIZE and write
+    bytes WRITE_SIZE, the factors the summary's copy-kernel calibration
+    gives."""
+    path = os.path.join(ROOT, "profiles", f"recon_pipeline_{n}x3x3_batch{batch}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        name = cells[0].strip("`") if cells else ""
+        if len(cells) == 7 and (name.startswith(f"fsim::{kernel}<") or name.startswith(f"fsim::{kernel}1<")):
+            try:
+                return int(round((float(cells[4]) + float(cells[6])) * 2 ** 20))
+            except ValueError:
+                return None
+    return None
+
+
 def stage_bytes(n, D=3, P=3, C=3, Cs=None):
     """HBM bytes per stack each pipeline kernel must move.  Cs = stored
     components per direction: C for the reference dataflow (the sum is
@@ -339,7 +359,7 @@ def main():
                    "n": n, "angles": 3, "phases": 3, "bands": 2, "batch": B},
         "roofline": {"bound": "hbm", "kernel": STAGES[dom], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": profiled_traffic(STAGES[dom], n, B)},
         "pipeline_roofline_frac": round(pipeline_frac, 4),
         "alg_bytes_per_recon": b_alg,
         "bytes_moved_per_recon": sum(sb),

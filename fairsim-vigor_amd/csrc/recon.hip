@@ -49,6 +49,9 @@
 #ifndef FSIM_COL_XCD
 #define FSIM_COL_XCD 1
 #endif
+#ifndef FSIM_TILE_GROUPS
+#define FSIM_TILE_GROUPS 1
+#endif
 #ifndef FSIM_COL_WPE
 #define FSIM_COL_WPE 6
 #endif
@@ -85,6 +88,19 @@ __device__ __forceinline__ int xcd_linear(int total) {
     return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
 }
 static inline unsigned xcd_grid(int total) { return (unsigned)((total + 7) / 8) * 8; }
+
+// Work index w -> (unit, tile) in tile-group-major order: groups of TG
+// adjacent tiles, and within a group every unit (component) with its TG
+// tiles in turn.  With xcd_linear, an XCD walks one group for all units
+// before the next, so the OTF tiles of that group stay in its L2 while the
+// components stream past (otherwise each component re-reads the table), and
+// neighbouring tiles, which share cache lines, still run back to back.
+template <int TG>
+__device__ __forceinline__ void tile_group_order(int w, int units, int &unit, int &tile) {
+    const int g = w / (units * TG), r = w % (units * TG);
+    unit = r / TG;
+    tile = g * TG + r % TG;
+}
 
 // Fade factor of pixel (x, y): fadeBorderCos on a vector of ones
 // (SimUtils.java:162-185, ReconstructionRunner.java:781-783): the products
@@ -292,7 +308,14 @@ __global__ void __launch_bounds__(ColShiftTile<N>::value, ColShiftTile<N>::wpe) 
     using TW = TwLds<N, ColShiftTile<N>::tw_fits>;
     __shared__ float2 twl[TW::len];
     const float2 *tws = tw_table<TW::tl, ColShiftTile<N>::value>(twl, tw);
-    const int tile = w % tiles, u = w / tiles;
+    int tile, u;
+    if (FSIM_TILE_GROUPS) {
+        constexpr int TG = tiles < 8 ? tiles : 8;
+        tile_group_order<TG>(w, total / tiles, u, tile);
+    } else {
+        tile = w % tiles;
+        u = w / tiles;
+    }
     const Job J = get_job(jl, u / nsh_all);
     KRef K = cst(J.c);
     const int C = K.Cs, nsh = C - 1, dc = u % nsh_all;
@@ -369,10 +392,14 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
 }
 
 // Block -> (stack, column tile) for the gather kernel.  With >= 8 stacks in a
-// launch, XCD x takes stacks x, x+8, ... whole, tiles in order, so the cache
-// lines that neighbouring tiles share (paste offsets are not tile aligned)
-// stay in one L2.  With fewer stacks the tiles of a stack are spread over all
-// XCDs, with adjacent tile pairs on the same XCD.
+// launch, XCD x takes stacks x, x+8, ... whole, tiles in order: the blocks
+// that run together on an XCD read the same rows of the same components (DRAM
+// row locality), and the cache lines that neighbouring tiles share (paste
+// offsets are not tile aligned) stay in one L2.  With fewer stacks the tiles
+// of a stack are spread over all XCDs, with adjacent tile pairs on the same
+// XCD.  (Tile-group order, as in the column pass, keeps the Wiener x
+// apodization tiles in L2 but spreads the concurrent blocks over 8 stacks;
+// measured slower.)
 __device__ __forceinline__ bool gather_block(int tiles, int njobs, int &job, int &tile) {
     const int L = blockIdx.x;
     if (njobs >= 8 && FSIM_JOBMAP) {
