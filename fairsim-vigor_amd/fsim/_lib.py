@@ -121,6 +121,9 @@ _SIGS = {
     "fsim_separation_matrix": [_PD, _I, _I, _PD, _PD],
     "fsim_separate_bands": [C.POINTER(_P), _I, C.POINTER(_P), _I, _PF],
     "fsim_locate_peak": [_P, _D, _PD],
+    "fsim_common_region": [_P, _P, _I, _I, _P, _D, _D, _D, _D, _I, _D],
+    "fsim_fit_peak": [_P, _P, _I, _I, _P, _D, _D, _D, _D, _D, C.POINTER(C.c_float), _PD],
+    "fsim_get_peak": [_P, _P, _I, _I, _P, _D, _D, _D, _D, _PD],
     "fsim_channel_create": [C.POINTER(RECON_PARAMS), _P, C.POINTER(_P)],
     "fsim_channel_destroy": [_P],
     "fsim_reconstruct": [_P, _P, _P, _I, _P],

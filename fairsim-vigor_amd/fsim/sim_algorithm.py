@@ -12,6 +12,11 @@ of running a reconstruction on the GPU:
   a time through the vector API, as the reference's accelerated path does;
 * ``Reconstructor`` -- the fused pipeline (fsim_channel / fsim_reconstruct)
   that runs the same dataflow in five HBM passes for a batch of stacks.
+
+Parameter estimation: ``Correlation`` (sim_algorithm/Correlation.java) with
+its per-pixel loops on the device, and ``estimate_parameters``
+(SimAlgorithm.estimateParameters, SimAlgorithm.java:37-256) over device
+vectors.
 """
 import ctypes as C
 import math
@@ -262,6 +267,126 @@ def locate_peak(vec, kmin):
     r = (C.c_double * 4)()
     call("fsim_locate_peak", vec.handle, float(kmin), r)
     return list(r)
+
+
+class Correlation:
+    """sim_algorithm/Correlation.java with the pixel loops on the device.
+    Bands are complex device vectors in Fourier space; the OTF's vector pixel
+    size (setPixelSize) must be set."""
+
+    @staticmethod
+    def locatePeak(vec, kMin):
+        """Correlation.locatePeak (Correlation.java:144-167): [x, y, mag, phase]."""
+        return locate_peak(vec, kMin)
+
+    @staticmethod
+    def commonRegion(band0, band1, bn0, bn1, otf, kx, ky, dist, weightLimit, divideByOtf):
+        """Correlation.commonRegion (Correlation.java:227-288), in place."""
+        call("fsim_common_region", band0.handle, band1.handle, int(bn0), int(bn1), otf.device(),
+             float(kx), float(ky), float(dist), float(weightLimit), 1 if divideByOtf else 0, otf.vec_cpm)
+
+    @staticmethod
+    def fitPeak(band0, band1, bn0, bn1, otf, kx, ky, weightLimit, search, cntrl=None):
+        """Correlation.fitPeak (Correlation.java:43-136): [kx, ky, phase, mag].
+        cntrl: None or a float32 array of shape (10, 30) to receive the
+        normalised correlation magnitudes of the three rounds."""
+        res = (C.c_double * 4)()
+        cp = None
+        if cntrl is not None:
+            if cntrl.dtype != np.float32 or cntrl.size != 300 or not cntrl.flags.c_contiguous:
+                raise ValueError("cntrl must be a contiguous float32 array of 10 x 30")
+            cp = cntrl.ctypes.data_as(C.POINTER(C.c_float))
+        call("fsim_fit_peak", band0.handle, band1.handle, int(bn0), int(bn1), otf.device(), float(kx),
+             float(ky), float(weightLimit), float(search), otf.vec_cpm, cp, res)
+        return list(res)
+
+    @staticmethod
+    def getPeak(band0, band1, bn0, bn1, otf, kx, ky, weightLimit):
+        """Correlation.getPeak (Correlation.java:183-209): a complex number
+        (phase = phase offset, magnitude = modulation estimate)."""
+        res = (C.c_double * 2)()
+        call("fsim_get_peak", band0.handle, band1.handle, int(bn0), int(bn1), otf.device(), float(kx),
+             float(ky), float(weightLimit), otf.vec_cpm, res)
+        return complex(res[0], res[1])
+
+
+def estimate_parameters(param, inFFT, fitBand, fitExclude):
+    """SimAlgorithm.estimateParameters (SimAlgorithm.java:37-256) without the
+    visual feedback: inFFT[d][p] are the FFT'd input images (complex device
+    vectors); per direction the shift vector (setPxPy), phase offset and
+    modulations of ``param`` are set.  Returns the per-direction results."""
+    w, h = inFFT[0][0].vectorWidth(), inFFT[0][0].vectorHeight()
+    otfPr = param.otf()
+    if fitBand != 1 and fitBand != 2:
+        raise RuntimeError("Fitband neither 1 nor 2")
+    # the attenuation vector fades out the DC component for the correlation
+    otfAtt = Vec2dReal(w, h)
+    otfPr.writeAttenuationVector(otfAtt, .99, 0.15 * otfPr.getCutoff(), 0, 0)
+    results = []
+    for angIdx in range(param.nrDir()):
+        d = param.dir(angIdx)
+        lb = 1
+        hb = 3 if d.nrBand() == 3 else 1
+        fb = lb if fitBand == 1 else hb
+        # band separation with equidistant phases (BandSeparation.java:183-197)
+        P = len(inFFT[angIdx])
+        separate = [Vec2dCplx(w, h) for _ in range(d.nrComp())]
+        separate_bands(inFFT[angIdx], separate, [(2 * math.pi * p) / P + 0.0 for p in range(P)],
+                       d.nrBand(), None)
+        # coarse correlation: ifft, multiply in real space, fft back
+        c0, c1, c2 = separate[0].duplicate(), separate[lb].duplicate(), separate[hb].duplicate()
+        for c in (c0, c1, c2):
+            c.times(otfAtt)
+            c.fft2d(True)
+        c1.timesConj(c0)
+        c2.timesConj(c0)
+        c1.fft2d(False)
+        c2.fft2d(False)
+        minDist = fitExclude * otfPr.getCutoff() / param.pxlSizeCyclesMicron()
+        peak = Correlation.locatePeak(c1 if fitBand == 1 else c2, minDist)
+        peak = Correlation.fitPeak(separate[0], separate[fb], 0, fitBand, otfPr, -peak[0], -peak[1], 0.05, 2.5)
+        r = {}
+        if lb != hb:
+            if fitBand == 1:
+                peak[0] *= 2
+                peak[1] *= 2
+            p1 = Correlation.getPeak(separate[0], separate[lb], 0, 1, otfPr, peak[0] / 2, peak[1] / 2, 0.05)
+            p2 = Correlation.getPeak(separate[0], separate[hb], 0, 2, otfPr, peak[0], peak[1], 0.05)
+            d.setPxPy(-peak[0], -peak[1])
+            d.setPhaOff(math.atan2(p1.imag, p1.real))
+            d.setModulation(1, abs(p1))
+            d.setModulation(2, abs(p2))
+            r.update(px=-peak[0], py=-peak[1], pha_off=math.atan2(p1.imag, p1.real), mod1=abs(p1), mod2=abs(p2))
+        else:
+            p1 = Correlation.getPeak(separate[0], separate[1], 0, 1, otfPr, peak[0], peak[1], 0.05)
+            d.setPxPy(-peak[0], -peak[1])
+            d.setPhaOff(math.atan2(p1.imag, p1.real))
+            d.setModulation(1, abs(p1))
+            r.update(px=-peak[0], py=-peak[1], pha_off=math.atan2(p1.imag, p1.real), mod1=abs(p1))
+        results.append(r)
+    return results
+
+
+def input_ffts(images, fade_px=10):
+    """The estimation input as the reference's GUI builds it
+    (sim_gui/ImageControl.java:816-840): each float image faded at the border
+    (fadeBorderCos), copied to a complex vector and transformed.  images:
+    float array [D, P, N, N] on the host; returns inFFT[d][p] device vectors."""
+    images = np.asarray(images, dtype=np.float32)
+    D, P, h, w = images.shape
+    out = []
+    for d in range(D):
+        row = []
+        for p in range(P):
+            r = Vec2dReal(w, h)
+            r.setData(images[d, p])
+            fade_border_cos(r, fade_px)
+            c = Vec2dCplx(w, h)
+            c.copy(r)
+            c.fft2d(False)
+            row.append(c)
+        out.append(row)
+    return out
 
 
 def paste_and_fourier_shift(inV, outV, kx, ky, fast=True):

@@ -180,6 +180,28 @@ int fsim_separate_bands(const fsim_vec *in, int nphase, const fsim_vec *out, int
  *   res = {xPos, yPos, max, phase} in the reference's coordinates. */
 int fsim_locate_peak(fsim_vec v, double kmin, double *res);
 
+/* common_region: Correlation.commonRegion (Correlation.java:227-288), in
+ *   place on both complex vectors.  The OTF weights (writeOtfVector, no
+ *   attenuation) of bands bn0/bn1 centred at 0, (kx, ky) and (-kx, -ky) at the
+ *   vector pixel size vec_cpm (cycles/micron per pixel); components below
+ *   weight_limit are zeroed, the others divided by the weight when
+ *   divide_by_otf; the DC ring ratio < dist or > 1 - dist is zeroed in band0
+ *   and, shifted by ((int)kx, (int)ky), in band1. */
+int fsim_common_region(fsim_vec band0, fsim_vec band1, int bn0, int bn1, fsim_otf otf, double kx, double ky,
+                       double dist, double weight_limit, int divide_by_otf, double vec_cpm);
+/* fit_peak: Correlation.fitPeak (Correlation.java:43-136).  Three rounds of
+ *   a 10 x 10 search (+- search around the guess, search / 5 per round) for
+ *   the shift of band1 against band0 (bands in Fourier space, unchanged);
+ *   res = {kx, ky, phase, magnitude}; cntrl: NULL or a 30 x 10 float image
+ *   (row-major, width 30) of the normalised correlation magnitudes. */
+int fsim_fit_peak(fsim_vec band0, fsim_vec band1, int bn0, int bn1, fsim_otf otf, double kx, double ky,
+                  double weight_limit, double search, double vec_cpm, float *cntrl, double *res);
+/* get_peak: Correlation.getPeak (Correlation.java:183-209): the
+ *   cross-correlation of band0 and band1 shifted to (kx, ky) over their common
+ *   region, scaled by 1/|band0|^2; res = {re, im} (Cplx.Double). */
+int fsim_get_peak(fsim_vec band0, fsim_vec band1, int bn0, int bn1, fsim_otf otf, double kx, double ky,
+                  double weight_limit, double vec_cpm, double *res);
+
 /* ---- fused fast-shift reconstruction --------------------------------------
  * One reconstruction = livemode/ReconstructionRunner.java:456-542 /
  * tests/TestAccelSpeed.java:237-252, 563-703 for one channel: raw stack of

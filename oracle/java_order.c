@@ -697,3 +697,150 @@ void ref_sum_elements(const cf *v, long n, double *res) {
     for (long i = 0; i < n; i++) { re += v[i].re; im += v[i].im; }
     res[0] = re; res[1] = im;
 }
+
+/* ------------------------------------------------------------------ */
+/* Parameter estimation (sim_algorithm/Correlation.java)               */
+/* ------------------------------------------------------------------ */
+
+/* Cplx.Float.abs (linalg/Cplx.java:159-161) */
+static float cf_abs(cf v) { return (float)sqrt((double)v.re * v.re + (double)v.im * v.im); }
+
+/* Cplx.Float.div = mult(a, reciproc(b)) (linalg/Cplx.java:298-319) */
+static cf cf_div(cf a, cf b) {
+    float s = b.re * b.re + b.im * b.im;
+    float rr = b.re / s, ri = -b.im / s;
+    cf r = {a.re * rr - a.im * ri, a.re * ri + a.im * rr};
+    return r;
+}
+
+/* Correlation.commonRegion (Correlation.java:227-288): the weights are
+ * writeOtfVector (no attenuation) of band bn0 at 0 and (kx, ky) and of band
+ * bn1 at 0 and (-kx, -ky); the pixel loop runs in the reference's order,
+ * y outer, x inner, including the zeros written into band1 at the shifted
+ * position of each DC-ring pixel. */
+void ref_common_region(cf *b0, cf *b1, int w, int h, const cf *tab0, const cf *tab1, int ns,
+                       double cpm, double cutoff, double vec_cpm, double kx, double ky,
+                       double dist, double wlim, int divide) {
+    long n = (long)w * h;
+    cf *g0 = malloc(sizeof(cf) * n), *g1 = malloc(sizeof(cf) * n);
+    cf *t0 = malloc(sizeof(cf) * n), *t1 = malloc(sizeof(cf) * n);
+    ref_otf_to_vector(g0, w, h, tab0, ns, cpm, cutoff, vec_cpm, 0, 0, 1);
+    ref_otf_to_vector(g1, w, h, tab1, ns, cpm, cutoff, vec_cpm, 0, 0, 1);
+    ref_otf_to_vector(t0, w, h, tab0, ns, cpm, cutoff, vec_cpm, kx, ky, 1);
+    ref_otf_to_vector(t1, w, h, tab1, ns, cpm, cutoff, vec_cpm, -kx, -ky, 1);
+    const cf zero = {0.f, 0.f};
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            long p = x + (long)y * w;
+            double rad = sqrt((double)(((x < w / 2) ? (x * x) : ((x - w) * (x - w))) +
+                                       ((y < h / 2) ? (y * y) : ((y - h) * (y - h)))));
+            double ratio = rad / sqrt(kx * kx + ky * ky);
+            if (cf_abs(g0[p]) < wlim || cf_abs(t0[p]) < wlim) b0[p] = zero;
+            else if (divide) b0[p] = cf_div(b0[p], g0[p]);
+            if (cf_abs(g1[p]) < wlim || cf_abs(t1[p]) < wlim) b1[p] = zero;
+            else if (divide) b1[p] = cf_div(b1[p], g1[p]);
+            if (ratio < dist || ratio > (1 - dist)) {
+                b0[p] = zero;
+                b1[(x - (int)kx + w) % w + (long)((y + (int)ky + h) % h) * w] = zero;
+            }
+        }
+    free(g0); free(g1); free(t0); free(t1);
+}
+
+/* sum over pixels of (b1 shifted by (kx, ky)) * conj(b0): Vec2d.Cplx
+ * fourierShift (BasicVectors.java:309-339), AbstractVectorCplx.timesConj
+ * (AbstractVectorCplx.java:281-304) and sumElements (:353-362), on copies,
+ * one candidate at a time in pixel order. */
+static void shifted_corr(const cf *b0, const cf *b1, int N, double kx, double ky, double *res) {
+    double re = 0, im = 0;
+    for (int y = 0; y < N; y++)
+        for (int x = 0; x < N; x++) {
+            long p = x + (long)y * N;
+            float pha = (float)(2 * M_PI * (kx * x + ky * y) / N);
+            float co = (float)ref_fcos(pha), si = (float)ref_fsin(pha);
+            float sr = (b1[p].re * co) - (b1[p].im * si);
+            float sm = (b1[p].im * co) + (b1[p].re * si);
+            float xr = b0[p].re, xi = -b0[p].im;
+            float qr = (xr * sr) - (xi * sm);
+            float qi = (xi * sr) + (xr * sm);
+            re += qr;
+            im += qi;
+        }
+    res[0] = re;
+    res[1] = im;
+}
+
+/* copies -> commonRegion(0.15, wlim, divide) -> inverse FFT of both */
+static void corr_pair(const cf *band0, const cf *band1, int N, const cf *tab0, const cf *tab1,
+                      int ns, double cpm, double cutoff, double vec_cpm, double kx, double ky,
+                      double wlim, cf *b0, cf *b1) {
+    long n = (long)N * N;
+    memcpy(b0, band0, sizeof(cf) * n);
+    memcpy(b1, band1, sizeof(cf) * n);
+    ref_common_region(b0, b1, N, N, tab0, tab1, ns, cpm, cutoff, vec_cpm, kx, ky, 0.15, wlim, 1);
+    ref_fft2d(b0, N, N, 1);
+    ref_fft2d(b1, N, N, 1);
+}
+
+/* Correlation.fitPeak (Correlation.java:43-136): res = {kx, ky, phase, mag};
+ * cntrl: NULL or 30 x 10 floats (row-major, width 30). */
+void ref_fit_peak(const cf *band0, const cf *band1, int N, const cf *tab0, const cf *tab1, int ns,
+                  double cpm, double cutoff, double vec_cpm, double kx, double ky, double wlim,
+                  double search, float *cntrl, double *res) {
+    long n = (long)N * N;
+    cf *b0 = malloc(sizeof(cf) * n), *b1 = malloc(sizeof(cf) * n);
+    double res_phase = 0, res_mag = 0;
+    for (int iter = 0; iter < 3; iter++) {
+        corr_pair(band0, band1, N, tab0, tab1, ns, cpm, cutoff, vec_cpm, kx, ky, wlim, b0, b1);
+        double scal = 1. / ref_norm2(b0, n);
+        double cre[100], cim[100];
+        const double tkx = kx, tky = ky, ts = search;
+#pragma omp parallel for schedule(dynamic)
+        for (int p = 0; p < 100; p++) {
+            int xi = p % 10, yi = p / 10;
+            double s[2];
+            shifted_corr(b0, b1, N, tkx + ((xi - 4.5) / 4.5) * ts, -(tky + ((yi - 4.5) / 4.5) * ts), s);
+            /* Cplx.mult(sum, Cplx.Double(scal)) */
+            cre[p] = s[0] * scal - s[1] * 0.0;
+            cim[p] = s[0] * 0.0 + s[1] * scal;
+        }
+        double max = 0, min = 1.7976931348623157e308, nkx = 0, nky = 0;
+        for (int yi = 0; yi < 10; yi++)
+            for (int xi = 0; xi < 10; xi++) {
+                double m = hypot(cre[yi * 10 + xi], cim[yi * 10 + xi]);
+                if (m > max) {
+                    max = m;
+                    nkx = tkx + ((xi - 4.5) / 4.5) * ts;
+                    nky = tky + ((yi - 4.5) / 4.5) * ts;
+                    res_phase = atan2(cim[yi * 10 + xi], cre[yi * 10 + xi]);
+                    res_mag = m;
+                }
+                if (m < min) min = m;
+            }
+        if (cntrl)
+            for (int yi = 0; yi < 10; yi++)
+                for (int xi = 0; xi < 10; xi++)
+                    cntrl[xi + iter * 10 + yi * 30] =
+                        (float)((hypot(cre[yi * 10 + xi], cim[yi * 10 + xi]) - min) / (max - min));
+        kx = nkx;
+        ky = nky;
+        search /= 5;
+    }
+    res[0] = kx; res[1] = ky; res[2] = res_phase; res[3] = res_mag;
+    free(b0); free(b1);
+}
+
+/* Correlation.getPeak (Correlation.java:183-209): res = {re, im} */
+void ref_get_peak(const cf *band0, const cf *band1, int N, const cf *tab0, const cf *tab1, int ns,
+                  double cpm, double cutoff, double vec_cpm, double kx, double ky, double wlim,
+                  double *res) {
+    long n = (long)N * N;
+    cf *b0 = malloc(sizeof(cf) * n), *b1 = malloc(sizeof(cf) * n);
+    corr_pair(band0, band1, N, tab0, tab1, ns, cpm, cutoff, vec_cpm, kx, ky, wlim, b0, b1);
+    double s[2];
+    shifted_corr(b0, b1, N, kx, -ky, s);
+    double scal = 1. / ref_norm2(b0, n);
+    res[0] = s[0] * scal - s[1] * 0.0;
+    res[1] = s[0] * 0.0 + s[1] * scal;
+    free(b0); free(b1);
+}

@@ -81,6 +81,9 @@ def lib():
         L.ref_norm2.argtypes = [P, C.c_long]
         L.ref_norm2.restype = D
         L.ref_sum_elements.argtypes = [P, C.c_long, P]
+        L.ref_common_region.argtypes = [P, P, I, I, P, P, I, D, D, D, D, D, D, D, I]
+        L.ref_fit_peak.argtypes = [P, P, I, P, P, I, D, D, D, D, D, D, D, P, P]
+        L.ref_get_peak.argtypes = [P, P, I, P, P, I, D, D, D, D, D, D, P]
         _lib = L
     return _lib
 
@@ -184,3 +187,115 @@ def recon64(p, raw):
     from . import fp64
     vals, cut, cpm = fp64.otf_from_estimate(p["na"], p["lam"], p["a"], p["otf_n"])
     return fp64.recon_fast(p, raw, vals)
+
+
+# ---------------------------------------------------------------------------
+# parameter estimation (sim_algorithm/Correlation.java, SimAlgorithm.java:37-256)
+# ---------------------------------------------------------------------------
+class Otf32:
+    """A single-band OtfProvider.fromEstimate table (no attenuation: the
+    estimation's writeOtfVector calls do not use it)."""
+
+    def __init__(self, na=1.4, lam=525.0, a=1.0, n=512):
+        self.vals, self.cutoff, self.cpm = otf_estimate(na, lam, a, n)
+        self.ns = n
+
+
+def common_region32(b0, b1, otf, vec_cpm, kx, ky, dist, wlim, divide=True):
+    """Correlation.commonRegion on float32 [N, N, 2] arrays, in place."""
+    h, w = b0.shape[:2]
+    lib().ref_common_region(ptr(b0), ptr(b1), w, h, ptr(otf.vals), ptr(otf.vals), otf.ns, otf.cpm,
+                            otf.cutoff, vec_cpm, kx, ky, dist, wlim, 1 if divide else 0)
+
+
+def fit_peak32(b0, b1, otf, vec_cpm, kx, ky, wlim, search, cntrl=False):
+    """Correlation.fitPeak: [kx, ky, phase, mag] (and the 10 x 30 control image)."""
+    n = b0.shape[0]
+    res = np.zeros(4)
+    c = np.zeros((10, 30), dtype=np.float32) if cntrl else None
+    lib().ref_fit_peak(ptr(b0), ptr(b1), n, ptr(otf.vals), ptr(otf.vals), otf.ns, otf.cpm, otf.cutoff,
+                       vec_cpm, kx, ky, wlim, search, ptr(c) if cntrl else None, ptr(res))
+    return (list(res), c) if cntrl else list(res)
+
+
+def get_peak32(b0, b1, otf, vec_cpm, kx, ky, wlim):
+    """Correlation.getPeak: complex."""
+    n = b0.shape[0]
+    res = np.zeros(2)
+    lib().ref_get_peak(ptr(b0), ptr(b1), n, ptr(otf.vals), ptr(otf.vals), otf.ns, otf.cpm, otf.cutoff,
+                       vec_cpm, kx, ky, wlim, ptr(res))
+    return complex(res[0], res[1])
+
+
+def times_conj32(y, x):
+    """y = y * conj(x) in AbstractVectorCplx.timesConj's float order."""
+    xr, xi = x[..., 0], -x[..., 1]
+    yr, yi = y[..., 0].copy(), y[..., 1].copy()
+    y[..., 0] = (xr * yr) - (xi * yi)
+    y[..., 1] = (xi * yr) + (xr * yi)
+
+
+def fft32(v, inverse):
+    lib().ref_fft2d(ptr(v), v.shape[1], v.shape[0], 1 if inverse else 0)
+
+
+def input_ffts32(images, fade_px=10):
+    """The estimation input as the GUI builds it (ImageControl.java:816-840):
+    float image, fadeBorderCos(10), to complex, forward FFT."""
+    n = images.shape[-1]
+    out = np.zeros(images.shape + (2,), dtype=np.float32)
+    for idx in np.ndindex(images.shape[:-2]):
+        img = np.ascontiguousarray(images[idx], dtype=np.float32)
+        lib().ref_fade_border(ptr(img), n, n, fade_px)
+        out[idx][..., 0] = img
+        fft32(out[idx], False)
+    return out
+
+
+def estimate32(in_fft, nband, fit_band, fit_exclude, otf, vec_cpm):
+    """SimAlgorithm.estimateParameters (SimAlgorithm.java:37-256, no visual
+    feedback) on FFT'd inputs [D, P, N, N, 2]: per direction the arguments it
+    stores, {px, py (setPxPy arguments), pha_off, mod1, mod2, peak}."""
+    D, P, n = in_fft.shape[0], in_fft.shape[1], in_fft.shape[2]
+    att = np.zeros((n, n), dtype=np.float32)
+    lib().ref_write_attenuation(ptr(att), n, n, vec_cpm, .99, 0.15 * otf.cutoff, 0.0, 0.0)
+    Cc = 2 * nband - 1
+    out = []
+    for d in range(D):
+        lb, hb = 1, (3 if nband == 3 else 1)
+        fb = lb if fit_band == 1 else hb
+        phases = np.array([(2 * np.pi * p) / P + 0.0 for p in range(P)])
+        Sm = np.zeros(2 * Cc * P)
+        assert lib().ref_separation_matrix(ptr(phases), P, nband, None, ptr(Sm)) == 0
+        Sf = Sm.astype(np.float32)
+        sep = np.zeros((Cc, n, n, 2), dtype=np.float32)
+        ins = (C.c_void_p * P)(*[in_fft[d, p].ctypes.data for p in range(P)])
+        outs = (C.c_void_p * Cc)(*[sep[c].ctypes.data for c in range(Cc)])
+        lib().ref_separate_bands(ins, outs, n * n, ptr(Sf), P, Cc)
+        c0, c1, c2 = sep[0].copy(), sep[lb].copy(), sep[hb].copy()
+        for c in (c0, c1, c2):
+            c[..., 0] *= att
+            c[..., 1] *= att
+            fft32(c, True)
+        times_conj32(c1, c0)
+        times_conj32(c2, c0)
+        fft32(c1, False)
+        fft32(c2, False)
+        min_dist = fit_exclude * otf.cutoff / vec_cpm
+        pk = np.zeros(4)
+        lib().ref_locate_peak(ptr(c1 if fit_band == 1 else c2), n, n, min_dist, ptr(pk))
+        peak = fit_peak32(sep[0], sep[fb], otf, vec_cpm, -pk[0], -pk[1], 0.05, 2.5)
+        r = {"peak": list(pk)}
+        if lb != hb:
+            if fit_band == 1:
+                peak[0] *= 2
+                peak[1] *= 2
+            p1 = get_peak32(sep[0], sep[lb], otf, vec_cpm, peak[0] / 2, peak[1] / 2, 0.05)
+            p2 = get_peak32(sep[0], sep[hb], otf, vec_cpm, peak[0], peak[1], 0.05)
+            r.update(px=-peak[0], py=-peak[1], pha_off=float(np.arctan2(p1.imag, p1.real)),
+                     mod1=abs(p1), mod2=abs(p2))
+        else:
+            p1 = get_peak32(sep[0], sep[1], otf, vec_cpm, peak[0], peak[1], 0.05)
+            r.update(px=-peak[0], py=-peak[1], pha_off=float(np.arctan2(p1.imag, p1.real)), mod1=abs(p1))
+        out.append(r)
+    return out

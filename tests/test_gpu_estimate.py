@@ -1,0 +1,120 @@
+"""GPU parity of the parameter estimation (fsim_common_region, fsim_fit_peak,
+fsim_get_peak and the estimateParameters flow) against the FP32 Java-order
+oracle.
+
+commonRegion is bit-exact (same per-pixel float arithmetic; the GPU kernel
+gathers the ring zeros the reference scatters).  fitPeak / getPeak follow the
+FFTs, whose rounding differs from the oracle's (JTransforms rounding is not
+reproducible, see test_gpu_recon), so their correlation values are compared to
+relative 1e-5; fitPeak's result positions are grid points and compare exactly.
+The estimation flow is fed the oracle's FFT'd inputs so that both sides start
+from the same bands.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import ref
+
+pytestmark = pytest.mark.gpu
+
+NA, LAM, A_OTF, PX_UM = 1.4, 525.0, 1.0, 0.08
+
+
+def _otf(n):
+    from fsim.sim_algorithm import OtfProvider
+    o = OtfProvider.fromEstimate(NA, LAM, A_OTF)
+    o.setPixelSize(1 / (n * PX_UM))
+    return o
+
+
+def _dev(a):
+    from fsim.linalg import Vec2dCplx
+    v = Vec2dCplx(a.shape[1], a.shape[0])
+    v.setData(a)
+    return v
+
+
+def _random_bands(n, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((n, n, 2)).astype(np.float32),
+            rng.standard_normal((n, n, 2)).astype(np.float32))
+
+
+@pytest.mark.parametrize("kx,ky", [(20.37, -11.62), (-25.8, 9.13), (0.5, 30.2)])
+@pytest.mark.parametrize("divide", [True, False])
+def test_common_region_matches_oracle(fsim_lib, kx, ky, divide):
+    from fsim.sim_algorithm import Correlation
+    n = 128
+    otf = _otf(n)
+    a0, a1 = _random_bands(n, 7)
+    v0, v1 = _dev(a0), _dev(a1)
+    Correlation.commonRegion(v0, v1, 0, 1, otf, kx, ky, 0.15, 0.05, divide)
+    r0, r1 = a0.copy(), a1.copy()
+    ref.common_region32(r0, r1, ref.Otf32(NA, LAM, A_OTF), 1 / (n * PX_UM), kx, ky, 0.15, 0.05, divide)
+    assert np.array_equal(v0.vectorData(), r0)
+    assert np.array_equal(v1.vectorData(), r1)
+
+
+def _shifted_pair(n, kx, ky, seed):
+    from test_oracle import _shifted_pair as pair
+    otf = ref.Otf32(NA, LAM, A_OTF)
+    return pair(n, kx, ky, otf.cutoff * n * PX_UM, seed)
+
+
+@pytest.mark.parametrize("n,kx,ky", [(128, 20.37, -11.62), (256, -41.8, 17.13)])
+def test_fit_peak_and_get_peak_match_oracle(fsim_lib, n, kx, ky):
+    from fsim.sim_algorithm import Correlation
+    otf = _otf(n)
+    b0, b1 = _shifted_pair(n, kx, ky, 9)
+    v0, v1 = _dev(b0), _dev(b1)
+    cntrl = np.zeros((10, 30), np.float32)
+    kx0, ky0 = round(kx) + 0.8, round(ky) - 0.6
+    got = Correlation.fitPeak(v0, v1, 0, 1, otf, kx0, ky0, 0.05, 2.5, cntrl)
+    want, wc = ref.fit_peak32(b0, b1, ref.Otf32(NA, LAM, A_OTF), 1 / (n * PX_UM), kx0, ky0, 0.05, 2.5,
+                              cntrl=True)
+    print(f"n={n} fitPeak gpu {got} oracle {want}")
+    assert got[0] == want[0] and got[1] == want[1]
+    assert abs(got[3] - want[3]) <= 1e-5 * want[3]
+    assert abs(got[2] - want[2]) <= 1e-5
+    assert np.max(np.abs(cntrl - wc)) < 1e-4
+    # the bands are unchanged
+    assert np.array_equal(v0.vectorData(), b0) and np.array_equal(v1.vectorData(), b1)
+    g = Correlation.getPeak(v0, v1, 0, 1, otf, got[0], got[1], 0.05)
+    w = ref.get_peak32(b0, b1, ref.Otf32(NA, LAM, A_OTF), 1 / (n * PX_UM), got[0], got[1], 0.05)
+    print(f"n={n} getPeak gpu {g} oracle {w}")
+    assert abs(g - w) <= 1e-5 * abs(w)
+
+
+def _estimate_case(nphase, nband, fit_band, exclude, k_frac, harmonic2, seed, n=256):
+    from fsim import synthetic
+    from fsim.linalg import Vec2dCplx
+    from fsim.sim_algorithm import SimParam, estimate_parameters
+    raw, sp = synthetic.sim_stack(n=n, nphase=nphase, seed=seed, nbeads=150, nfil=20, harmonic2=harmonic2,
+                                  k_frac=k_frac)
+    fft_in = ref.input_ffts32(raw.reshape(3, nphase, n, n))
+    want = ref.estimate32(fft_in, nband, fit_band, exclude, ref.Otf32(NA, LAM, A_OTF), 1 / (n * PX_UM))
+    param = SimParam.create(nband, 3, nphase, n, PX_UM, _otf(n))
+    in_dev = [[_dev(np.ascontiguousarray(fft_in[d, p])) for p in range(nphase)] for d in range(3)]
+    got = estimate_parameters(param, in_dev, fit_band, exclude)
+    return sp, param, got, want
+
+
+@pytest.mark.parametrize("nphase,nband,fit_band,exclude,k_frac,h2",
+                         [(3, 2, 1, 0.6, 0.85, 0.0), (5, 3, 2, 0.6, 0.45, 0.3), (5, 3, 1, 0.3, 0.45, 0.3)])
+def test_estimate_parameters_matches_oracle(fsim_lib, nphase, nband, fit_band, exclude, k_frac, h2):
+    sp, param, got, want = _estimate_case(nphase, nband, fit_band, exclude, k_frac, h2, seed=3)
+    for d in range(3):
+        g, w = got[d], want[d]
+        print(f"dir {d}: gpu {g} oracle {w}")
+        assert g["px"] == w["px"] and g["py"] == w["py"]
+        assert abs(g["pha_off"] - w["pha_off"]) < 1e-4
+        assert abs(g["mod1"] - w["mod1"]) <= 1e-4 * w["mod1"]
+        if nband == 3:
+            assert abs(g["mod2"] - w["mod2"]) <= 1e-4 * w["mod2"]
+        # SimParam.Dir holds the estimate: setPxPy stores the shift / (bands - 1)
+        assert param.dir(d).pX == g["px"] / (nband - 1)
+        assert param.dir(d).getPhaOff() == g["pha_off"]
+        scale = 1 if nband == 2 else 2
+        assert abs(param.dir(d).px(nband - 1) - scale * sp["px"][d]) < 0.1

@@ -314,3 +314,85 @@ def test_recon_parameters_known_answer():
     assert good > 0.6
     assert score(flip) < good - 0.2
     assert score(sp, np.pi / 2) < good - 0.2
+
+
+# --- parameter estimation (Correlation.java, SimAlgorithm.estimateParameters) ---
+
+def _shifted_pair(n, kx, ky, cut_px, seed):
+    """band0: a random spectrum inside the OTF support; band1: the same image
+    multiplied in real space by exp(-2 pi i (kx x - ky y) / n), i.e. what
+    fitPeak's Fourier shift by (kx, -ky) moves back onto band0."""
+    rng = np.random.default_rng(seed)
+    f = np.fft.fft2(rng.standard_normal((n, n)))
+    fr = np.fft.fftfreq(n) * n
+    fx, fy = np.meshgrid(fr, fr)
+    f[np.hypot(fx, fy) > 0.9 * cut_px] = 0
+    s0 = np.fft.ifft2(f)
+    y, x = np.mgrid[0:n, 0:n]
+    s1 = s0 * np.exp(-2j * np.pi * (kx * x - ky * y) / n)
+    return ref.cplx_f32(np.fft.fft2(s0)), ref.cplx_f32(np.fft.fft2(s1))
+
+
+@pytest.mark.parametrize("kx,ky", [(20.37, -11.62), (-25.8, 9.13)])
+def test_fit_peak_recovers_exact_shift(kx, ky):
+    """A band and its exactly shifted copy: fitPeak, started within a pixel,
+    returns the shift to the last round's grid step (2 * 0.1 / 9 px), and
+    getPeak there has phase ~0."""
+    n, vec_cpm = 128, 1 / (128 * 0.08)
+    otf = ref.Otf32(1.4, 525.0, 1.0)
+    b0, b1 = _shifted_pair(n, kx, ky, otf.cutoff / vec_cpm, 5)
+    r = ref.fit_peak32(b0, b1, otf, vec_cpm, round(kx) + 0.8, round(ky) - 0.6, 0.05, 2.5)
+    assert abs(r[0] - kx) < 0.03 and abs(r[1] - ky) < 0.03
+    g = ref.get_peak32(b0, b1, otf, vec_cpm, r[0], r[1], 0.05)
+    assert abs(np.angle(g)) < 0.1
+
+
+def test_common_region_zero_sets():
+    """Without weights (limit 0, no division) commonRegion only cuts the DC
+    ring 0.15 < |r| / |k| < 0.85 from band0 and the same ring moved by
+    (-(int)kx, +(int)ky) (truncated toward zero) from band1."""
+    n, vec_cpm = 64, 1 / (64 * 0.08)
+    otf = ref.Otf32(1.4, 525.0, 1.0)
+    kx, ky = -20.7, 9.4
+    b0 = np.ones((n, n, 2), np.float32)
+    b1 = np.ones((n, n, 2), np.float32)
+    ref.common_region32(b0, b1, otf, vec_cpm, kx, ky, 0.15, 0.0, divide=False)
+    fr = np.fft.fftfreq(n) * n
+    fx, fy = np.meshgrid(fr, fr)
+    ratio = np.hypot(fx, fy) / math.hypot(kx, ky)
+    ring = (ratio < 0.15) | (ratio > 0.85)
+    assert np.array_equal(b0[..., 0] == 0, ring)
+    ikx, iky = int(kx), int(ky)
+    moved = np.zeros_like(ring)
+    yy, xx = np.nonzero(ring)
+    moved[(yy + iky) % n, (xx - ikx) % n] = True
+    assert np.array_equal(b1[..., 0] == 0, moved)
+
+
+def test_estimate_known_answer_two_band():
+    """Seeded 3-phase data (|k0| = 0.85 cutoff, phases 0, 2pi/3, 4pi/3,
+    modulation 0.7): the estimation returns the true shift vectors, phase
+    offset ~0 and a modulation estimate of the right size."""
+    from fsim import synthetic
+    n = 256
+    raw, sp = synthetic.sim_stack(n=n, seed=3, nbeads=150, nfil=20)
+    otf = ref.Otf32(1.4, 525.0, 1.0)
+    est = ref.estimate32(ref.input_ffts32(raw.reshape(3, 3, n, n)), 2, 1, 0.6, otf, 1 / (n * 0.08))
+    for d, r in enumerate(est):
+        assert abs(r["px"] - sp["px"][d]) < 0.1 and abs(r["py"] - sp["py"][d]) < 0.1
+        assert abs(r["pha_off"]) < 0.2
+        assert 0.4 < r["mod1"] < 0.9
+
+
+@pytest.mark.parametrize("fit_band,exclude", [(1, 0.3), (2, 0.6)])
+def test_estimate_known_answer_three_band(fit_band, exclude):
+    """5-phase data with a 2 k0 harmonic (|k0| = 0.45 cutoff): both fit bands
+    return the band-2 shift 2 k0 (setPxPy argument)."""
+    from fsim import synthetic
+    n = 256
+    raw, sp = synthetic.sim_stack(n=n, nphase=5, seed=4, nbeads=150, nfil=20, harmonic2=0.3, k_frac=0.45)
+    otf = ref.Otf32(1.4, 525.0, 1.0)
+    est = ref.estimate32(ref.input_ffts32(raw.reshape(3, 5, n, n)), 3, fit_band, exclude, otf, 1 / (n * 0.08))
+    for d, r in enumerate(est):
+        assert abs(r["px"] - 2 * sp["px"][d]) < 0.1 and abs(r["py"] - 2 * sp["py"][d]) < 0.1
+        assert abs(r["pha_off"]) < 0.2
