@@ -172,6 +172,136 @@ def serve_mixed_stream(nreq, rate):
             "saturation_roofline_frac_pcie_inclusive": round(nreq / sat_el * 434 * mean_n2 / 8e12, 4)}
 
 
+def batched_mixed(nper=32):
+    """Config 2: one batch of 96 stacks -- 3 channels (525/600/670 nm) x sizes
+    256/512/1024, 32 of each size -- submitted at once to the request server
+    with stacks and outputs resident in HBM; time to the last result."""
+    import numpy as np
+    from fsim import synthetic
+    from fsim.linalg import DeviceBuffer
+    from fsim.livemode import ReconstructionServer
+    from fsim.sim_algorithm import Reconstructor
+    sizes, lams = [256, 512, 1024], [525.0, 600.0, 670.0]
+    srv = ReconstructionServer(mem_budget=32 << 30, max_batch=32, max_queue=4096, nslots=4)
+    ids, bufs, keep = {}, {}, []
+    for n in sizes:
+        for ci, lam in enumerate(lams):
+            raw, sp = synthetic.sim_stack(n=n, seed=200 + ci, lam=lam, nbeads=300 * n // 512,
+                                          nfil=40 * n // 512)
+            rec = Reconstructor(synthetic.make_param(sp, n, lam=lam))
+            keep.append(rec)
+            ids[(n, ci)] = srv.add_channel(rec)
+            rb = DeviceBuffer(raw.nbytes)
+            rb.upload(raw)
+            bufs[(n, ci)] = rb
+    jobs = [(n, i % 3) for n in sizes for i in range(nper)]
+    outs = {n: [DeviceBuffer(16 * n * n) for _ in range(nper)] for n in sizes}
+
+    def run():
+        t0 = time.perf_counter()
+        tickets = [srv.submit(ids[k], bufs[k].value, outs[k[0]][j % nper].value, device=True)
+                   for j, k in enumerate(jobs)]
+        for t in tickets:
+            srv.wait(t)
+        return time.perf_counter() - t0
+
+    run()
+    srv.stats(reset=True)
+    el = min(run() for _ in range(3))
+    srv.close()
+    b_alg = sum(434 * n * n for n, _ in jobs)
+    return {"batch": f"{len(jobs)} stacks: sizes 256/512/1024 x {nper}, 3 channels, resident in HBM",
+            "ms_per_batch": round(el * 1e3, 3), "recons_per_s": round(len(jobs) / el, 1),
+            "alg_bytes_per_batch": b_alg, "roofline_frac": round(b_alg / el / 8e12, 4)}
+
+
+def large_fov(batch=4, steps=5):
+    """Config 4: 2048^2 x 3 x 3 uint16 stacks -> 4096^2, fused pipeline,
+    `batch` stacks per launch resident in HBM; parity of one stack against the
+    FP32 oracle."""
+    import numpy as np
+    from fsim import synthetic
+    from fsim.linalg import DeviceBuffer
+    from fsim.sim_algorithm import Reconstructor
+    from oracle import fp64, ref
+    n = 2048
+    raw, sp = synthetic.sim_stack(n=n, seed=7, nbeads=1200, nfil=160)
+    rec = Reconstructor(synthetic.make_param(sp, n))
+    rb = DeviceBuffer(raw.nbytes * batch)
+    host = np.ascontiguousarray(np.broadcast_to(raw, (batch,) + raw.shape))
+    rb.upload(host)
+    ob = DeviceBuffer(rec.out_bytes() * batch)
+    rec.reconstruct(rb.value, ob.value, batch)
+    _lib_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rec.reconstruct(rb.value, ob.value, batch)
+    _lib_sync()
+    el = time.perf_counter() - t0
+    value = batch * steps / el
+    got = rec.reconstruct_host(raw)
+    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1))
+    err = fp64.rel_l2(got, ref.recon32(ref.recon_dict(sp, n), raw, threads=threads))
+    return {"workload": f"2048^2 x 3 x 3 uint16 -> 4096^2 float32, batch {batch}",
+            "recons_per_s": round(value, 2), "ms_per_recon": round(1e3 / value, 3),
+            "roofline_frac": round(value * 434 * n * n / 8e12, 4), "rel_l2_vs_oracle": err}
+
+
+def estimation(runs=3):
+    """Config 1: parameter estimation (SimAlgorithm.estimateParameters) of one
+    512^2 stack given as float images: 3 phases / 2 bands (fit band 1) and
+    5 phases / 3 bands with a 2 k0 harmonic (|k0| = 0.45 cutoff, fit band 2),
+    GPU (device vector flow) vs the FP32 oracle on the host cores; and the
+    estimate + fused reconstruction of the same stack."""
+    import numpy as np
+    from fsim import synthetic
+    from fsim.sim_algorithm import OtfProvider, Reconstructor, SimParam, estimate_parameters, input_ffts
+    from oracle import ref
+    n = 512
+    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1))
+    ref.lib().ref_set_threads(threads)
+    out = {}
+    for tag, nphase, nband, fb, k_frac, h2 in (("p3b2", 3, 2, 1, 0.85, 0.0), ("p5b3", 5, 3, 2, 0.45, 0.3)):
+        raw, sp = synthetic.sim_stack(n=n, nphase=nphase, seed=11, harmonic2=h2, k_frac=k_frac)
+        imgs = raw.reshape(3, nphase, n, n).astype(np.float32)
+
+        def est():
+            otf = OtfProvider.fromEstimate(1.4, 525.0, 1.0)
+            param = SimParam.create(nband, 3, nphase, n, 0.08, otf)
+            r = estimate_parameters(param, input_ffts(imgs), fb, 0.6)
+            _lib_sync()
+            return param, r
+
+        est()
+        t = []
+        for _ in range(runs):
+            a = time.perf_counter()
+            param, r = est()
+            t.append((time.perf_counter() - a) * 1e3)
+        err_k = max(abs(r[d]["px"] - (nband - 1) * sp["px"][d]) + abs(r[d]["py"] - (nband - 1) * sp["py"][d])
+                    for d in range(3))
+        # estimate + reconstruct (uint16 stack, fused path, estimated parameters)
+        a = time.perf_counter()
+        param, r = est()
+        rec = Reconstructor(param)
+        rec.reconstruct_host(raw)
+        er_ms = (time.perf_counter() - a) * 1e3
+        a = time.perf_counter()
+        otf32 = ref.Otf32(1.4, 525.0, 1.0)
+        ref.estimate32(ref.input_ffts32(imgs), nband, fb, 0.6, otf32, 1 / (n * 0.08))
+        cpu_ms = (time.perf_counter() - a) * 1e3
+        out[tag] = {"gpu_ms_per_estimate": round(float(np.median(t)), 2),
+                    "estimate_plus_recon_ms": round(er_ms, 2),
+                    "cpu_oracle_ms_per_estimate": round(cpu_ms, 1), "cpu_threads": threads,
+                    "max_k_error_px": round(err_k, 4)}
+    return out
+
+
+def _lib_sync():
+    from fsim import _lib
+    _lib.call("fsim_sync")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +317,8 @@ def main():
     ap.add_argument("--mix-requests", type=int, default=400)
     ap.add_argument("--mix-rate", type=float, default=200.0)
     ap.add_argument("--no-mix", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the estimation, mixed-batch and 2048^2 legs")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed batched loop (for rocprofv3 runs)")
     ap.add_argument("--probe-copy", action="store_true",
@@ -327,6 +459,15 @@ def main():
         log("served: open-loop mixed stream at 200 req/s")
         served["mixed_stream"] = serve_mixed_stream(args.mix_requests, args.mix_rate)
 
+    configs = None
+    if not args.no_configs:
+        log("config 1: parameter estimation")
+        configs = {"estimation_512": estimation()}
+        log("config 2: mixed batch of 96 stacks")
+        configs["batched_mixed_96"] = batched_mixed()
+        log("config 4: 2048^2 large field of view")
+        configs["large_fov_2048"] = large_fov()
+
     # CPU baseline: the oracle restatement on the host cores, bounded sample
     cpu = None
     if not args.no_cpu:
@@ -370,6 +511,7 @@ def main():
         "plain": {"recons_per_s": round(plain_value, 2), "p50_ms": round(pct(plat, 50), 3),
                   "p99_ms": round(pct(plat, 99), 3), "rel_l2_vs_oracle": plain_err},
         "served": served,
+        "configs": configs,
         "speedup_vs_plain": round(value / plain_value, 2),
         "served_speedup_vs_plain": round(served["recons_per_s"] / plain_value, 2),
         "rel_l2_vs_oracle": parity,

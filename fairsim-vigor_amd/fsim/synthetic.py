@@ -47,6 +47,21 @@ def make_object(nfine, rng, nbeads=300, nfil=40, bead_sigma=0.5):
     return np.real(np.fft.ifft2(np.fft.fft2(obj) * gauss))
 
 
+def _fft2(a):
+    """2-D FFT; multithreaded for the large grids (n >= 1024)."""
+    if a.shape[0] >= 2048:
+        import scipy.fft
+        return scipy.fft.fft2(a, workers=-1)
+    return np.fft.fft2(a)
+
+
+def _ifft2(a):
+    if a.shape[0] >= 2048:
+        import scipy.fft
+        return scipy.fft.ifft2(a, workers=-1)
+    return np.fft.ifft2(a)
+
+
 def sim_stack(n=512, ndir=3, nphase=3, na=1.4, lam=525.0, px_um=0.080, k_frac=0.85,
               mod=0.7, angle0_deg=3.0, peak=1000.0, bg=100.0, offset=100.0, seed=0,
               nbeads=300, nfil=40, harmonic2=0.0):
@@ -82,7 +97,7 @@ def sim_stack(n=512, ndir=3, nphase=3, na=1.4, lam=525.0, px_um=0.080, k_frac=0.
             illum = 1 + mod * np.cos(arg)
             if harmonic2 > 0:
                 illum = illum + harmonic2 * np.cos(2 * arg)
-            em = np.real(np.fft.ifft2(np.fft.fft2(obj * illum) * otf))
+            em = np.real(_ifft2(_fft2(obj * illum) * otf))
             binned = em.reshape(n, 2, n, 2).sum(axis=(1, 3))
             frames.append(binned)
     frames = np.array(frames)
