@@ -168,11 +168,11 @@ struct FftE {
     static constexpr int value = L >= 1024 ? 16 : (L >= 8 ? 8 : L);
 };
 
-// Elements per thread in column kernels: 32 from L = 1024 up, so that the
-// 16-column tiles fit 512-thread blocks (256 VGPRs per thread available).
+// Elements per thread in the fused column kernels: as FftE (16 from
+// L = 1024 up), which keeps a column's registers within 128 VGPRs.
 template <int L>
 struct ColE {
-    static constexpr int value = L >= 1024 ? 32 : FftE<L>::value;
+    static constexpr int value = FftE<L>::value;
 };
 
 // LDS layout of a transform: element i lives at lds[pad(i) * ls] with one

@@ -286,7 +286,7 @@ struct ColShiftTile {
     static constexpr int value = t * cw;
     static constexpr int lds = FftLds<N, e>::len * cw;
     static constexpr bool tw_fits = (lds + N) * 8 <= 163840;
-    static constexpr int wpe = N > 512 ? 1 : lds_waves_per_simd(value, (lds + N) * 8, FSIM_COL_WPE);
+    static constexpr int wpe = lds_waves_per_simd(value, (lds + N) * 8, FSIM_COL_WPE);
 };
 
 template <int N>
@@ -420,15 +420,18 @@ static inline unsigned gather_grid(int tiles, int njobs) {
 
 // ---------------------------------------------------------------------------
 // K5: gather into the doubled grid, filter, column IFFT.
-// Elements per thread for the n-point (en) and 2n-point (em) column
-// transforms and the tile width: up to n = 512 a 512-thread block (8 columns,
-// 16 elements per thread for the 2n-point transform) within 128 VGPRs; from
-// n = 1024 on, 32 elements per thread.
+// Elements per thread for the n-point (en, 8) and 2n-point (em, FftE: 8 or
+// 16) column transforms and the tile width: 8 columns up to n = 1024 (512
+// threads at n = 512, 1024 at n = 1024), 4 at n = 2048, all within 128 VGPRs
+// per thread.
 // ---------------------------------------------------------------------------
 template <int N>
 struct GatherE {
-    static constexpr int en = N >= 1024 ? 32 : FftE<N>::value;
-    static constexpr int em = N >= 1024 ? 32 : FftE<2 * N>::value;
+    // 8 elements for the n-point DC transform: from n = 512 on its n/8
+    // threads are exactly the 2n/16 of the 2n-point one, and the registers
+    // stay free for the accumulator
+    static constexpr int en = N >= 8 ? 8 : N;
+    static constexpr int em = FftE<2 * N>::value;
     static constexpr int tn = N / en, tm = 2 * N / em;
     static constexpr int tk = tn > tm ? tn : tm;
     static constexpr int cw0 = ColTile<2 * N, tk, (en > em ? en : em)>::value;
@@ -437,9 +440,8 @@ struct GatherE {
     static constexpr int lpn = FftLds<N, en>::len, lpm = FftLds<2 * N, em>::len;
     static constexpr int lpk = lpm > lpn ? lpm : lpn;
     static constexpr int lds = (lpk > N ? lpk : N) * cw;
-    // at most 4 waves per SIMD (128 VGPRs); from n = 1024 on the transforms
-    // need the full register file
-    static constexpr int wpe = N > 512 ? 1 : lds_waves_per_simd(tk * cw, (lds + 2 * N) * 8, 4);
+    // at most 4 waves per SIMD (128 VGPRs)
+    static constexpr int wpe = lds_waves_per_simd(tk * cw, (lds + 2 * N) * 8, 4);
 };
 
 // pasteFreq from the n grid into the 2n grid (BasicVectors.java:273-278), one
