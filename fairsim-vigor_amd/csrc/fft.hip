@@ -39,7 +39,10 @@ __global__ void __launch_bounds__(256) k_fft_rows(float2 *__restrict__ data, int
 
 template <int L, bool INV>
 __global__ void __launch_bounds__(1024) k_fft_cols(float2 *__restrict__ data, int ncols, int pitch,
-                                                   const float2 *__restrict__ tw, float scale) {
+                                                   size_t img_stride, const float2 *__restrict__ tw,
+                                                   float scale) {
+    // blockIdx.y: image of a batch of equally shaped images img_stride apart
+    data += (size_t)blockIdx.y * img_stride;
     constexpr int E = FftE<L>::value, T = L / E;
     constexpr int CW = ColW<L>::value;
     __shared__ float2 lds[FftLds<L, E>::len * CW];
@@ -79,16 +82,18 @@ static int launch_rows_t(float2 *d, int nrows, int pitch, int inverse, hipStream
 }
 
 template <int L>
-static int launch_cols_t(float2 *d, int ncols, int pitch, int inverse, hipStream_t s) {
+static int launch_cols_t(float2 *d, int ncols, int pitch, int nimg, size_t img_stride, int inverse,
+                         hipStream_t s) {
     constexpr int E = FftE<L>::value, T = L / E;
     constexpr int CW = ColW<L>::value;
     const float2 *tw = twiddles(L);
     if (!tw) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
-    dim3 grid(cdiv(ncols, CW)), block(T * CW);
+    dim3 grid(cdiv(ncols, CW), nimg), block(T * CW);
     if (inverse)
-        hipLaunchKernelGGL((k_fft_cols<L, true>), grid, block, 0, s, d, ncols, pitch, tw, 1.0f / L);
+        hipLaunchKernelGGL((k_fft_cols<L, true>), grid, block, 0, s, d, ncols, pitch, img_stride, tw,
+                           1.0f / L);
     else
-        hipLaunchKernelGGL((k_fft_cols<L, false>), grid, block, 0, s, d, ncols, pitch, tw, 1.0f);
+        hipLaunchKernelGGL((k_fft_cols<L, false>), grid, block, 0, s, d, ncols, pitch, img_stride, tw, 1.0f);
     FSIM_HIP(hipGetLastError());
     return FSIM_OK;
 }
@@ -115,16 +120,22 @@ bool fft_size_ok(int L) { return is_pow2(L) && L >= 2 && L <= 4096; }
 static int launch_rows(int L, float2 *d, int nrows, int pitch, int inverse, hipStream_t s) {
     FSIM_FFT_SWITCH(L, launch_rows_t, d, nrows, pitch, inverse, s)
 }
-static int launch_cols(int L, float2 *d, int ncols, int pitch, int inverse, hipStream_t s) {
-    FSIM_FFT_SWITCH(L, launch_cols_t, d, ncols, pitch, inverse, s)
+static int launch_cols(int L, float2 *d, int ncols, int pitch, int nimg, size_t img_stride, int inverse,
+                       hipStream_t s) {
+    FSIM_FFT_SWITCH(L, launch_cols_t, d, ncols, pitch, nimg, img_stride, inverse, s)
+}
+
+int fft2d_batch_device(float2 *data, int w, int h, int nimg, int inverse, hipStream_t s) {
+    if (!fft_size_ok(w) || !fft_size_ok(h))
+        return fail(FSIM_ERR_ARG, "fft2d: width and height must be powers of two in [2, 4096]");
+    if (nimg < 1 || nimg > 65535) return fail(FSIM_ERR_ARG, "fft2d: batch of 1 to 65535 images");
+    int rc = launch_rows(w, data, h * nimg, w, inverse, s);
+    if (rc) return rc;
+    return launch_cols(h, data, w, w, nimg, (size_t)w * h, inverse, s);
 }
 
 int fft2d_device(float2 *data, int w, int h, int inverse, hipStream_t s) {
-    if (!fft_size_ok(w) || !fft_size_ok(h))
-        return fail(FSIM_ERR_ARG, "fft2d: width and height must be powers of two in [2, 4096]");
-    int rc = launch_rows(w, data, h, w, inverse, s);
-    if (rc) return rc;
-    return launch_cols(h, data, w, w, inverse, s);
+    return fft2d_batch_device(data, w, h, 1, inverse, s);
 }
 
 }  // namespace fsim

@@ -202,8 +202,7 @@ __global__ void k_attenuation(void *__restrict__ v, int is_cplx, int w, int h, d
 #pragma clang fp contract(off)
     GRID_STRIDE(i, (size_t)w * h) {
         const int x = (int)(i % w), y = (int)(i / w);
-        const double cycl = cycl_at(x, y, w, h, kx, ky, vec_cpm);
-        const float val = (float)(1 - str * exp(-(cycl * cycl) / (2 * ((fwhm / 2.355) * (fwhm / 2.355)))));
+        const float val = attenuation_pixel(x, y, w, h, vec_cpm, str, fwhm, kx, ky);
         if (is_cplx)
             reinterpret_cast<float2 *>(v)[i] = make_float2(val, 0.f);
         else
@@ -273,11 +272,7 @@ __global__ void k_separate(SepArgs a, size_t n) {
         for (int p = 0; p < a.nphase; p++) x[p] = a.in[p][i];
         for (int c = 0; c < a.ncomp; c++) {
             float2 acc = make_float2(0.f, 0.f);
-            for (int p = 0; p < a.nphase; p++) {
-                const float2 s = a.S[c * 8 + p];
-                acc.x += (s.x * x[p].x) - (s.y * x[p].y);
-                acc.y += (s.y * x[p].x) + (s.x * x[p].y);
-            }
+            for (int p = 0; p < a.nphase; p++) acc = sep_axpy(acc, a.S[c * 8 + p], x[p]);
             a.out[c][i] = acc;
         }
     }
@@ -292,16 +287,8 @@ __global__ void k_locate_peak(const float2 *__restrict__ v, int w, int h, double
     unsigned long long k = 0;
     GRID_STRIDE(i, (size_t)w * h) {
         const int x = (int)(i % w), y = (int)(i / w);
-        const int dx2 = (x < w / 2) ? (x * x) : ((x - w) * (x - w));
-        const int dy2 = (y < h / 2) ? (y * y) : ((y - h) * (y - h));
-        const double rad = sqrt((double)(dx2 + dy2));
-        const float2 c = v[i];
-        const float a = (float)sqrt((double)c.x * c.x + (double)c.y * c.y);
-        if (rad > kmin && !(a != a)) {
-            const unsigned long long key =
-                ((unsigned long long)__float_as_uint(a) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)i);
-            if (key > k) k = key;
-        }
+        const unsigned long long key = peak_key(v[i], x, y, w, h, kmin, (unsigned)i);
+        if (key > k) k = key;
     }
     sh[threadIdx.x] = k;
     __syncthreads();

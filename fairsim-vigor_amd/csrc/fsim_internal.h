@@ -39,6 +39,8 @@ std::mutex &scratch_mutex();            // held by every user of scratch_d for i
 // ---- FFT entry (fft.hip) -------------------------------------------------
 bool fft_size_ok(int L);
 int fft2d_device(float2 *data, int w, int h, int inverse, hipStream_t s);
+// nimg images of w x h, w*h elements apart, transformed in the same launches
+int fft2d_batch_device(float2 *data, int w, int h, int nimg, int inverse, hipStream_t s);
 
 // small helpers
 inline bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }

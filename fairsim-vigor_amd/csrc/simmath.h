@@ -101,6 +101,36 @@ __device__ __forceinline__ float2 otf_pixel(const float2 *__restrict__ tab, int 
     return otf_val(tab, ns, cpm, cutoff, cycl);
 }
 
+// OtfProvider.writeAttenuationVector value at pixel (x, y)
+// (OtfProvider.java:503-529): 1 - str * exp(-cycl^2 / (2 (fwhm / 2.355)^2)).
+__device__ __forceinline__ float attenuation_pixel(int x, int y, int w, int h, double vec_cpm, double str,
+                                                   double fwhm, double kx, double ky) {
+#pragma clang fp contract(off)
+    const double cycl = cycl_at(x, y, w, h, kx, ky, vec_cpm);
+    return (float)(1 - str * exp(-(cycl * cycl) / (2 * ((fwhm / 2.355) * (fwhm / 2.355)))));
+}
+
+// Correlation.locatePeak (Correlation.java:144-167) ranking key of pixel i =
+// (x, y): |c| as (float)sqrt((double)re*re + (double)im*im) in the high
+// word, ~i in the low word, so that the largest key is the first maximum in
+// y-major order; 0 for pixels within kmin of DC (and NaN magnitudes).
+__device__ __forceinline__ unsigned long long peak_key(float2 c, int x, int y, int w, int h, double kmin,
+                                                       unsigned i) {
+    const int dx2 = (x < w / 2) ? (x * x) : ((x - w) * (x - w));
+    const int dy2 = (y < h / 2) ? (y * y) : ((y - h) * (y - h));
+    const double rad = sqrt((double)(dx2 + dy2));
+    const float a = (float)sqrt((double)c.x * c.x + (double)c.y * c.y);
+    if (!(rad > kmin) || a != a) return 0ull;
+    return ((unsigned long long)__float_as_uint(a) << 32) | (unsigned long long)(0xFFFFFFFFu - i);
+}
+
+// BandSeparation.separateBands accumulation (BandSeparation.java:157-163,
+// AbstractVectorCplx.axpy): acc += s * x in the reference's float order.
+__device__ __forceinline__ float2 sep_axpy(float2 acc, float2 s, float2 x) {
+#pragma clang fp contract(off)
+    return make_float2(acc.x + ((s.x * x.x) - (s.y * x.y)), acc.y + ((s.y * x.x) + (s.x * x.y)));
+}
+
 // Multiply by the conjugate: Cplx.Float.mult(val.conj()) (Cplx.java:298-300)
 __device__ __forceinline__ float2 mul_conj_java(float2 v, float2 o) {
 #pragma clang fp contract(off)
