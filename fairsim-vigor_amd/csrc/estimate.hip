@@ -15,7 +15,7 @@
 // the same kernel with one candidate.
 #include "fsim_internal.h"
 #include "filters.h"
-#include "simmath.h"
+#include "estimate_dev.h"
 
 #include <cmath>
 #include <mutex>
@@ -23,135 +23,35 @@
 
 namespace fsim {
 
-// |v| as Cplx.Float.abs (Cplx.java:159-161): sqrt in double, rounded to float.
-__device__ __forceinline__ float abs_java(float2 v) {
-    return (float)sqrt((double)v.x * (double)v.x + (double)v.y * (double)v.y);
-}
-
-// a / b as Cplx.Float.div = mult(a, reciproc(b)) (Cplx.java:311-319).
-__device__ __forceinline__ float2 div_java(float2 a, float2 b) {
-#pragma clang fp contract(off)
-    const float s = b.x * b.x + b.y * b.y;
-    const float rr = b.x / s, ri = -b.y / s;
-    return make_float2(a.x * rr - a.y * ri, a.x * ri + a.y * rr);
-}
-
-// Distance of pixel (x, y) to the DC component, in pixels, as the reference
-// computes it (integer squares, then sqrt in double).
-__device__ __forceinline__ double dc_rad(int x, int y, int w, int h) {
-    const int dx = x < w / 2 ? x : x - w, dy = y < h / 2 ? y : y - h;
-    return sqrt((double)(dx * dx + dy * dy));
-}
-
-struct RegionArgs {
-    const float2 *tab0, *tab1;   // radial OTF tables of bands bn0 and bn1 (no attenuation)
-    int ns;
-    double cpm, cutoff, vec_cpm;
-    double kx, ky, dist, wlim;
-    int ikx, iky;                // (int)kx, (int)ky: truncated toward zero
-    int divide;
-};
-
-// Correlation.commonRegion.  band0(x, y): zero where either OTF weight
-// (centred at 0 and at (kx, ky)) is below the limit, else divided by the
-// weight; zero in the ring ratio < dist or ratio > 1 - dist around DC.
-// band1(x, y): the same with the weights centred at 0 and (-kx, -ky); the
-// ring pixels (xq, yq) also zero band1 at ((xq - ikx) mod w, (yq + iky) mod h),
-// so band1(x, y) is zero when its source pixel ((x + ikx) mod w,
-// (y - iky) mod h) lies in the ring.  The reference's loop order decides only
-// the sign of such zeros.
+// Correlation.commonRegion over both bands in place.
 __global__ void k_common_region(float2 *__restrict__ b0, float2 *__restrict__ b1, int w, int h, RegionArgs a) {
     const double kmax = sqrt(a.kx * a.kx + a.ky * a.ky);
     const size_t n = (size_t)w * h;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int x = (int)(i % w), y = (int)(i / w);
-        const float2 w0 = otf_pixel(a.tab0, a.ns, a.cpm, a.cutoff, x, y, w, h, 0.0, 0.0, a.vec_cpm);
-        const float2 wt0 = otf_pixel(a.tab0, a.ns, a.cpm, a.cutoff, x, y, w, h, a.kx, a.ky, a.vec_cpm);
-        const float2 w1 = otf_pixel(a.tab1, a.ns, a.cpm, a.cutoff, x, y, w, h, 0.0, 0.0, a.vec_cpm);
-        const float2 wt1 = otf_pixel(a.tab1, a.ns, a.cpm, a.cutoff, x, y, w, h, -a.kx, -a.ky, a.vec_cpm);
         float2 v0 = b0[i], v1 = b1[i];
-        if (abs_java(w0) < a.wlim || abs_java(wt0) < a.wlim)
-            v0 = make_float2(0.f, 0.f);
-        else if (a.divide)
-            v0 = div_java(v0, w0);
-        if (abs_java(w1) < a.wlim || abs_java(wt1) < a.wlim)
-            v1 = make_float2(0.f, 0.f);
-        else if (a.divide)
-            v1 = div_java(v1, w1);
-        const double r0 = dc_rad(x, y, w, h) / kmax;
-        if (r0 < a.dist || r0 > 1 - a.dist) v0 = make_float2(0.f, 0.f);
-        const int xq = ((x + a.ikx) % w + w) % w, yq = ((y - a.iky) % h + h) % h;
-        const double rq = dc_rad(xq, yq, w, h) / kmax;
-        if (rq < a.dist || rq > 1 - a.dist) v1 = make_float2(0.f, 0.f);
+        region_pixel(a, x, y, w, h, kmax, v0, v1);
         b0[i] = v0;
         b1[i] = v1;
     }
 }
 
-// Candidate shifts: b1 shifted by (xpos, -ypos) (Vec2d.Cplx.fourierShift,
-// BasicVectors.java:309-339), times conj(b0) (AbstractVectorCplx.timesConj),
-// summed over all pixels in double (sumElements).  One block per chunk of
-// kPix pixels and candidate group; partial[c * nchunk + chunk].
-constexpr int kMaxCand = 100;
-constexpr int kCorrThreads = 256, kCorrPpt = 4, kCorrPix = kCorrThreads * kCorrPpt;
-constexpr int kCandPerBlock = 25;
+// Candidate list of one launch; partial[c * nchunk + chunk].
 struct Cands {
     double2 p[kMaxCand];   // (xpos, ypos)
     int n;
 };
 
-__device__ __forceinline__ double2 block_sum(double2 v, double2 *sh) {
-    for (int o = 32; o > 0; o >>= 1) {
-        v.x += __shfl_down(v.x, o, 64);
-        v.y += __shfl_down(v.y, o, 64);
-    }
-    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
-    if (lane == 0) sh[wv] = v;
-    __syncthreads();
-    double2 r = make_double2(0.0, 0.0);
-    if (threadIdx.x == 0)
-        for (int i = 0; i < kCorrThreads / 64; i++) {
-            r.x += sh[i].x;
-            r.y += sh[i].y;
-        }
-    __syncthreads();
-    return r;
-}
-
 __global__ void __launch_bounds__(kCorrThreads) k_corr_cands(const float2 *__restrict__ b0,
                                                              const float2 *__restrict__ b1, int N, Cands cands,
                                                              int nchunk, double2 *__restrict__ partial) {
     __shared__ double2 sh[kCorrThreads / 64];
-    const size_t n = (size_t)N * N;
     const int chunk = blockIdx.x;
-    float2 p0[kCorrPpt], p1[kCorrPpt];
-    int px[kCorrPpt], py[kCorrPpt];
-    bool ok[kCorrPpt];
-#pragma unroll
-    for (int j = 0; j < kCorrPpt; j++) {
-        const size_t i = (size_t)chunk * kCorrPix + (size_t)j * kCorrThreads + threadIdx.x;
-        ok[j] = i < n;
-        const size_t ic = ok[j] ? i : 0;
-        p0[j] = b0[ic];
-        p1[j] = b1[ic];
-        px[j] = (int)(ic % N);
-        py[j] = (int)(ic / N);
-    }
+    CorrPix px;
+    corr_load(px, b0, b1, N, chunk);
     const int c0 = blockIdx.y * kCandPerBlock;
     for (int c = c0; c < c0 + kCandPerBlock && c < cands.n; c++) {
-        const double xpos = cands.p[c].x, ypos = cands.p[c].y;
-        double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-        for (int j = 0; j < kCorrPpt; j++) {
-            float co, si;
-            shift_cos_sin(xpos, -ypos, px[j], py[j], N, co, si);
-            const float2 q = times_conj_java(rot_java(p1[j], co, si), p0[j]);
-            if (ok[j]) {
-                acc.x += (double)q.x;
-                acc.y += (double)q.y;
-            }
-        }
-        const double2 s = block_sum(acc, sh);
+        const double2 s = block_sum(corr_candidate(px, cands.p[c].x, cands.p[c].y, N), sh);
         if (threadIdx.x == 0) partial[(size_t)c * nchunk + chunk] = s;
     }
 }
@@ -322,7 +222,7 @@ extern "C" int fsim_fit_peak(fsim_vec band0, fsim_vec band1, int bn0, int bn1, f
         c.n = 100;
         for (int p = 0; p < 100; p++) {
             const int xi = p % 10, yi = p / 10;
-            c.p[p] = make_double2(kx + ((xi - 4.5) / 4.5) * search, ky + ((yi - 4.5) / 4.5) * search);
+            c.p[p] = make_double2(grid_pos(kx, xi, search), grid_pos(ky, yi, search));
         }
         std::vector<double2> sum;
         rc = corr_cands(N, c, sum);
@@ -340,8 +240,8 @@ extern "C" int fsim_fit_peak(fsim_vec band0, fsim_vec band1, int bn0, int bn1, f
                 const int p = yi * 10 + xi;
                 if (mag[p] > max) {
                     max = mag[p];
-                    newKx = kx + ((xi - 4.5) / 4.5) * search;
-                    newKy = ky + ((yi - 4.5) / 4.5) * search;
+                    newKx = grid_pos(kx, xi, search);
+                    newKy = grid_pos(ky, yi, search);
                     resPhase = std::atan2(corr_im[p], corr_re[p]);
                     resMag = mag[p];
                 }

@@ -1,0 +1,704 @@
+// Batched SIM parameter estimation: SimAlgorithm.estimateParameters
+// (SimAlgorithm.java:37-256) for many stacks in one stream-ordered sequence
+// of launches, with no host round trip between its steps.
+//
+// Work items: a group is one (stack, direction); a pair is one correlation of
+// band 0 with another band of a group (fitPeak: one pair per group; getPeak:
+// one or two).  Per-group and per-pair state (the peak, the current shift and
+// search radius, the correlation result) lives on the device, so the argmax
+// that chooses the next fitPeak grid runs in a kernel and the host only
+// launches.  Each step reuses the arithmetic of the per-call entry points
+// (region_pixel, corr_candidate, the FFT kernels), so a batch gives the same
+// results as the reference flow driven call by call.
+//
+// fitPeak's candidate search.  Evaluating all 100 candidates in the
+// reference's arithmetic (float-rounded shift phase, MTool polynomial, float
+// products) costs about 70 double-precision operations per pixel and
+// candidate.  Each round instead first evaluates every candidate's
+// correlation A(c) = sum q * e^{i phi_c} (q = b1 conj(b0)) separably, with
+// accurate phasors: a 10-frequency DFT along x per row, then 10 along y —
+// about 60 operations per pixel for all 100 candidates.  The reference value
+// J(c) differs from A(c) by at most
+//   D = sum|q| * (2^-24 * phi_max + 2e-6) * 1.5
+// (2^-24 |phi|: rounding of the phase to float; 2e-6 covers the polynomial,
+// the float products and the phasor tables with margin), so a candidate can
+// hold the maximum of |J| only if |A(c)| >= max|A| - 2D.  Those candidates
+// (typically a handful) are then evaluated in the reference's arithmetic and
+// the first maximum in y-outer, x-inner order is taken among them, which is
+// the candidate the full evaluation picks.
+//
+// Device layout for G = count * ndir groups, nn = n * n:
+//   X  [G][P][nn] complex: input spectra; after separation slots 0, 1 (, 2)
+//      hold components 0, 1 (, 3) in place
+//   W  [2][Q][nn] complex: coarse correlation (Q = G: attenuated component 0
+//      and fit-band component), then the commonRegion copies of Q pairs
+//      (band 0 copies first, then band 1 copies)
+#include "fsim_internal.h"
+#include "filters.h"
+#include "estimate_dev.h"
+
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+namespace fsim {
+
+struct EstPair {
+    double kx, ky;     // current shift
+    double search;     // > 0: fitPeak's 10 x 10 grid +- search around (kx, ky); 0: (kx, ky) alone
+    double re, im;     // correlation of the chosen candidate, scaled by 1 / |b0|^2
+    int grp;           // group (stack * ndir + direction)
+    int slot1;         // slot of band 1 in the group's X
+    int bn1;           // OTF band of band 1
+    int err;           // shift beyond the vector in commonRegion
+    int nsel;          // candidates evaluated in the reference's arithmetic
+    unsigned char sel[kMaxCand];   // their indices yi * 10 + xi, ascending
+};
+
+struct EstGroup {
+    unsigned long long key;   // locatePeak: best (magnitude, index) key
+};
+
+struct EbSep {
+    float2 S[3][kMaxPhase];   // rows of components 0, 1 (, 3)
+    int nphase, ncomp, fb;    // fb: slot of the fit band's component
+    double vec_cpm, att_fwhm;
+};
+
+struct EbRegion {
+    const float2 *tabs;       // radial OTF tables, band-major
+    int ns, otf_nband;
+    double cpm, cutoff, vec_cpm;
+};
+
+constexpr int kNormSlot = kMaxCand;            // partial slot of |b0|^2
+constexpr int kPairSlots = kMaxCand + 1;
+
+__global__ void k_eb_init(EstGroup *__restrict__ g, int G) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < G) g[i].key = 0ull;
+}
+
+// Input preparation of sim_gui/ImageControl.java:819-840: fadeBorderCos on
+// the float image (top, bottom, left, right; each `float *= double` rounds
+// once, SimUtils.java:162-185), copied to the real part of a complex image.
+// Four pixels per thread (n is a multiple of 4).
+__global__ void k_eb_load(const float4 *__restrict__ img, float4 *__restrict__ X, int n, size_t nquad,
+                          const double *__restrict__ lo, const double *__restrict__ hi) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < nquad; q += (size_t)gridDim.x * blockDim.x) {
+        const float4 v = img[q];
+        const size_t i = q * 4;
+        const int x0 = (int)(i % n), y = (int)((i / n) % n);
+        const double ly = lo[y], hy = hi[y];
+        float d[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            d[j] = (float)((double)d[j] * ly);
+            d[j] = (float)((double)d[j] * hy);
+            d[j] = (float)((double)d[j] * lo[x0 + j]);
+            d[j] = (float)((double)d[j] * hi[x0 + j]);
+        }
+        X[2 * q] = make_float4(d[0], 0.f, d[1], 0.f);
+        X[2 * q + 1] = make_float4(d[2], 0.f, d[3], 0.f);
+    }
+}
+
+// BandSeparation.separateBands with equidistant phases
+// (BandSeparation.java:131-165, 183-197) for the components the estimation
+// reads, in place; the attenuated copies of component 0 and of the fit
+// band's component (SimAlgorithm.java:84-92: duplicate, times(otfAtt) with
+// writeAttenuationVector(.99, 0.15 * cutoff)) go to W.  Grid (pixels, G).
+__global__ void k_eb_separate(float2 *__restrict__ X, float2 *__restrict__ W, int n, int G, EbSep a) {
+    const size_t nn = (size_t)n * n;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const int g = blockIdx.y;
+    float2 *in = X + (size_t)g * a.nphase * nn;
+    float2 x[kMaxPhase];
+#pragma unroll
+    for (int p = 0; p < kMaxPhase; p++)
+        if (p < a.nphase) x[p] = in[(size_t)p * nn + i];
+    float2 acc[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        acc[c] = make_float2(0.f, 0.f);
+        if (c < a.ncomp)
+#pragma unroll
+            for (int p = 0; p < kMaxPhase; p++)
+                if (p < a.nphase) acc[c] = sep_axpy(acc[c], a.S[c][p], x[p]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        if (c < a.ncomp) in[(size_t)c * nn + i] = acc[c];
+    const int px = (int)(i % n), py = (int)(i / n);
+    const float att = attenuation_pixel(px, py, n, n, a.vec_cpm, .99, a.att_fwhm, 0.0, 0.0);
+    const float2 f = a.fb == 2 ? acc[2] : acc[1];
+    W[(size_t)g * nn + i] = make_float2(acc[0].x * att, acc[0].y * att);
+    W[((size_t)G + g) * nn + i] = make_float2(f.x * att, f.y * att);
+}
+
+// c_fb.timesConj(c0) (AbstractVectorCplx.java:288-304) for all groups.
+__global__ void k_eb_times_conj(float2 *__restrict__ W, size_t Gnn) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < Gnn; i += (size_t)gridDim.x * blockDim.x)
+        W[Gnn + i] = times_conj_java(W[Gnn + i], W[i]);
+}
+
+// Correlation.locatePeak (Correlation.java:144-167) key of each group's
+// correlation map; grid (blocks, G).
+__global__ void __launch_bounds__(256) k_eb_peak(const float2 *__restrict__ C, int n, double kmin,
+                                                 EstGroup *__restrict__ grp) {
+    __shared__ unsigned long long sh[256];
+    const size_t nn = (size_t)n * n;
+    const float2 *v = C + (size_t)blockIdx.y * nn;
+    unsigned long long k = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x) {
+        const unsigned long long key = peak_key(v[i], (int)(i % n), (int)(i / n), n, n, kmin, (unsigned)i);
+        if (key > k) k = key;
+    }
+    sh[threadIdx.x] = k;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s && sh[threadIdx.x + s] > sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && sh[0]) atomicMax(&grp[blockIdx.y].key, sh[0]);
+}
+
+// locatePeak's result in its coordinate convention (Correlation.java:145-166):
+// (-0, -1) start values when no pixel qualifies, then x > w/2 -> x - w,
+// y > h/2 -> h - y, else -y.
+__device__ __forceinline__ void peak_coords(unsigned long long key, int n, double &xpos, double &ypos) {
+    xpos = -0.0;
+    ypos = -1;
+    if (key) {
+        const unsigned idx = 0xFFFFFFFFu - (unsigned)(key & 0xFFFFFFFFull);
+        xpos = idx % n;
+        ypos = idx / n;
+    }
+    if (xpos > n / 2) xpos = xpos - n;
+    if (ypos > n / 2)
+        ypos = n - ypos;
+    else
+        ypos *= -1;
+}
+
+// fitPeak(separate[0], separate[fb], 0, fitBand, otf, -peak[0], -peak[1],
+// 0.05, 2.5) (SimAlgorithm.java:114-115): one pair per group.
+__global__ void k_eb_fit_init(const EstGroup *__restrict__ grp, EstPair *__restrict__ pairs, int G, int n,
+                              int fb_slot, int fit_band, double *__restrict__ peaks) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    double xpos, ypos;
+    peak_coords(grp[g].key, n, xpos, ypos);
+    peaks[2 * g] = xpos;
+    peaks[2 * g + 1] = ypos;
+    EstPair p;
+    p.kx = -xpos;
+    p.ky = -ypos;
+    p.search = 2.5;
+    p.re = p.im = 0.0;
+    p.grp = g;
+    p.slot1 = fb_slot;
+    p.bn1 = fit_band;
+    p.err = 0;
+    p.nsel = 0;
+    pairs[g] = p;
+}
+
+// Correlation.commonRegion(b0, b1, 0, bn1, otf, kx, ky, 0.15, 0.05, true)
+// on copies of each pair's bands (Correlation.java:60-63, 189-192).
+// Grid (pixels, npairs).
+__global__ void k_eb_region(const float2 *__restrict__ X, float2 *__restrict__ T, EstPair *__restrict__ pairs,
+                            int npairs, int nphase, int n, EbRegion r) {
+    const size_t nn = (size_t)n * n;
+    const int p = blockIdx.y;
+    const double kx = pairs[p].kx, ky = pairs[p].ky;
+    const int grp = pairs[p].grp, slot1 = pairs[p].slot1, bn1 = pairs[p].bn1;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (fabs(kx) >= n || fabs(ky) >= n)) pairs[p].err = 1;
+    RegionArgs a;
+    a.tab0 = r.tabs;
+    a.tab1 = r.tabs + (size_t)(r.otf_nband == 1 ? 0 : bn1) * r.ns;
+    a.ns = r.ns;
+    a.cpm = r.cpm;
+    a.cutoff = r.cutoff;
+    a.vec_cpm = r.vec_cpm;
+    a.kx = kx;
+    a.ky = ky;
+    a.dist = 0.15;
+    a.wlim = 0.05;
+    a.ikx = (int)kx;
+    a.iky = (int)ky;
+    a.divide = 1;
+    const double kmax = sqrt(kx * kx + ky * ky);
+    const float2 *b0 = X + (size_t)grp * nphase * nn, *b1 = X + ((size_t)grp * nphase + slot1) * nn;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x) {
+        float2 v0 = b0[i], v1 = b1[i];
+        region_pixel(a, (int)(i % n), (int)(i / n), n, n, kmax, v0, v1);
+        T[(size_t)p * nn + i] = v0;
+        T[((size_t)npairs + p) * nn + i] = v1;
+    }
+}
+
+// Phasor tables of the approximate candidate evaluation, per pair:
+// TX[i][x] = e^{+2 pi i xpos_i x / n}, TY[j][y] = e^{-2 pi i ypos_j y / n}
+// (the shift of b1 by (xpos, -ypos)), accurate in double, stored as float.
+__global__ void k_eb_apx_tab(const EstPair *__restrict__ pairs, int n, float2 *__restrict__ tabs) {
+    const int p = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 20 * n) return;
+    const bool isy = k >= 10 * n;
+    const int kk = isy ? k - 10 * n : k, idx = kk / n, v = kk % n;
+    const double pos = isy ? grid_pos(pairs[p].ky, idx, pairs[p].search) : grid_pos(pairs[p].kx, idx, pairs[p].search);
+    double u = pos * v / n;
+    u -= floor(u);
+    double sn, cs;
+    sincospi(2 * u, &sn, &cs);
+    tabs[(size_t)p * 20 * n + k] = make_float2((float)cs, (float)(isy ? -sn : sn));
+}
+
+// Approximate correlations A(c) of all 100 candidates and sum|q| of each
+// pair; grid (n / rows, npairs), 256 threads as 16 row groups of 16 lanes.
+// Per row: R_i = sum_x TX_i(x) q(x, y) (lanes split x, then a 16-lane sum),
+// then lane i < 10 adds TY_j(y) R_i to candidate j * 10 + i.  Partial sums
+// apart[(pair * kPairSlots + c) * nblk + block], sum|q| in slot kNormSlot.
+constexpr int kApxTile = 512;
+__global__ void __launch_bounds__(256) k_eb_approx(const float2 *__restrict__ T, int n,
+                                                   const EstPair *__restrict__ pairs, int npairs,
+                                                   const float2 *__restrict__ tabs, int rows, int nblk,
+                                                   double2 *__restrict__ apart) {
+    __shared__ float2 tx[10 * kApxTile];
+    __shared__ float2 ty[10 * 32];
+    __shared__ double2 red[16][kMaxCand];
+    __shared__ double ared[4];
+    const int p = blockIdx.y, blk = blockIdx.x;
+    const int t = threadIdx.x, rg = t / 16, l = t % 16;
+    const size_t nn = (size_t)n * n;
+    const float2 *b0 = T + (size_t)p * nn, *b1 = T + ((size_t)npairs + p) * nn;
+    const float2 *TX = tabs + (size_t)p * 20 * n, *TY = TX + 10 * n;
+    const int y0 = blk * rows;
+    for (int k = t; k < 10 * rows; k += 256) ty[k] = TY[(k / rows) * n + y0 + k % rows];
+    double2 S[10];
+#pragma unroll
+    for (int j = 0; j < 10; j++) S[j] = make_double2(0.0, 0.0);
+    double aq = 0.0;
+    const int tw = n < kApxTile ? n : kApxTile;
+    for (int xt0 = 0; xt0 < n; xt0 += tw) {
+        __syncthreads();
+        for (int k = t; k < 10 * tw; k += 256) tx[(k / tw) * kApxTile + k % tw] = TX[(k / tw) * n + xt0 + k % tw];
+        __syncthreads();
+        for (int r = rg; r < rows; r += 16) {
+            const size_t row = (size_t)(y0 + r) * n + xt0;
+            double2 R[10];
+#pragma unroll
+            for (int i = 0; i < 10; i++) R[i] = make_double2(0.0, 0.0);
+            for (int x = l; x < tw; x += 16) {
+                const float2 a = b0[row + x], b = b1[row + x];
+                const double qr = (double)b.x * a.x + (double)b.y * a.y;
+                const double qi = (double)b.y * a.x - (double)b.x * a.y;
+                aq += sqrt(qr * qr + qi * qi);
+#pragma unroll
+                for (int i = 0; i < 10; i++) {
+                    const float2 e = tx[i * kApxTile + x];
+                    R[i].x += qr * e.x - qi * e.y;
+                    R[i].y += qr * e.y + qi * e.x;
+                }
+            }
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1)
+#pragma unroll
+                for (int i = 0; i < 10; i++) {
+                    R[i].x += __shfl_xor(R[i].x, o, 16);
+                    R[i].y += __shfl_xor(R[i].y, o, 16);
+                }
+            double2 Ri = R[0];
+#pragma unroll
+            for (int i = 1; i < 10; i++)
+                if (l == i) Ri = R[i];
+            if (l < 10)
+#pragma unroll
+                for (int j = 0; j < 10; j++) {
+                    const float2 e = ty[j * rows + r];
+                    S[j].x += Ri.x * e.x - Ri.y * e.y;
+                    S[j].y += Ri.x * e.y + Ri.y * e.x;
+                }
+        }
+    }
+    if (l < 10)
+#pragma unroll
+        for (int j = 0; j < 10; j++) red[rg][j * 10 + l] = S[j];
+    for (int o = 32; o > 0; o >>= 1) aq += __shfl_down(aq, o, 64);
+    if (t % 64 == 0) ared[t / 64] = aq;
+    __syncthreads();
+    double2 *out = apart + (size_t)p * kPairSlots * nblk;
+    if (t < kMaxCand) {
+        double2 s = make_double2(0.0, 0.0);
+        for (int g = 0; g < 16; g++) {
+            s.x += red[g][t].x;
+            s.y += red[g][t].y;
+        }
+        out[(size_t)t * nblk + blk] = s;
+    }
+    if (t == 0) out[(size_t)kNormSlot * nblk + blk] = make_double2(ared[0] + ared[1] + ared[2] + ared[3], 0.0);
+}
+
+// Candidates that can hold the maximum (see the top of this file); all 100
+// when the bound cannot separate (no positive threshold, or NaN).
+__global__ void __launch_bounds__(128) k_eb_select(EstPair *__restrict__ pairs, const double2 *__restrict__ apart,
+                                                   int nblk) {
+    __shared__ double mag[kPairSlots];
+    const int p = blockIdx.x, t = threadIdx.x;
+    const double2 *in = apart + (size_t)p * kPairSlots * nblk;
+    if (t < kPairSlots) {
+        double2 s = make_double2(0.0, 0.0);
+        for (int b = 0; b < nblk; b++) {
+            s.x += in[(size_t)t * nblk + b].x;
+            s.y += in[(size_t)t * nblk + b].y;
+        }
+        mag[t] = t == kNormSlot ? s.x : hypot(s.x, s.y);
+    }
+    __syncthreads();
+    if (t != 0) return;
+    double M = 0.0;
+    bool bad = false;
+    for (int c = 0; c < kMaxCand; c++) {
+        if (!(mag[c] >= 0.0)) bad = true;
+        if (mag[c] > M) M = mag[c];
+    }
+    const double phimax = 2 * M_PI * (fabs(pairs[p].kx) + fabs(pairs[p].ky) + 2 * pairs[p].search);
+    const double delta = 1.5 * mag[kNormSlot] * (phimax * 0x1p-24 * (1 + 1e-6) + 2e-6);
+    const double thr = M - 2 * delta - 1e-12 * M;
+    const bool all = bad || !(thr > 0.0) || !(delta == delta);
+    int ns = 0;
+    for (int c = 0; c < kMaxCand; c++)
+        if (all || mag[c] >= thr) pairs[p].sel[ns++] = (unsigned char)c;
+    pairs[p].nsel = ns;
+}
+
+// The selected candidates of every pair in the reference's arithmetic, and
+// |b0|^2: grid (nchunk, npairs); part[(pair * kPairSlots + c) * nchunk + chunk].
+// Candidate c = yi * 10 + xi of the grid, or (kx, ky) alone.
+__global__ void __launch_bounds__(kCorrThreads) k_eb_corr(const float2 *__restrict__ T, int n,
+                                                          const EstPair *__restrict__ pairs, int npairs, int nchunk,
+                                                          double2 *__restrict__ part) {
+    __shared__ double2 sh[kCorrThreads / 64];
+    const size_t nn = (size_t)n * n;
+    const int p = blockIdx.y, chunk = blockIdx.x;
+    const double kx = pairs[p].kx, ky = pairs[p].ky, search = pairs[p].search;
+    const int nsel = pairs[p].nsel;
+    CorrPix px;
+    corr_load(px, T + (size_t)p * nn, T + ((size_t)npairs + p) * nn, n, chunk);
+    double2 *out = part + (size_t)p * kPairSlots * nchunk;
+    const double2 s0 = block_sum(make_double2(corr_norm2(px), 0.0), sh);
+    if (threadIdx.x == 0) out[(size_t)kNormSlot * nchunk + chunk] = s0;
+    for (int k = 0; k < nsel; k++) {
+        const int c = pairs[p].sel[k];
+        const double xpos = search > 0 ? grid_pos(kx, c % 10, search) : kx;
+        const double ypos = search > 0 ? grid_pos(ky, c / 10, search) : ky;
+        const double2 s = block_sum(corr_candidate(px, xpos, ypos, n), sh);
+        if (threadIdx.x == 0) out[(size_t)c * nchunk + chunk] = s;
+    }
+}
+
+// Per pair: selected candidate sums in chunk order, corr = sum * (1 / |b0|^2)
+// as Cplx.mult with a real Cplx.Double; for a grid, the first maximum of
+// hypot(corr) in y-outer, x-inner order becomes the new shift and the search
+// radius shrinks by 5 (Correlation.java:99-131).  One block per pair.
+__global__ void __launch_bounds__(128) k_eb_step(EstPair *__restrict__ pairs, const double2 *__restrict__ part,
+                                                 int nchunk) {
+    __shared__ double2 sums[kPairSlots];
+    const int p = blockIdx.x, t = threadIdx.x;
+    const double search = pairs[p].search;
+    const int nsel = pairs[p].nsel;
+    const double2 *in = part + (size_t)p * kPairSlots * nchunk;
+    const int c = t < nsel ? pairs[p].sel[t] : (t == 127 ? kNormSlot : -1);
+    if (c >= 0) {
+        double2 s = make_double2(0.0, 0.0);
+        for (int k = 0; k < nchunk; k++) {
+            s.x += in[(size_t)c * nchunk + k].x;
+            s.y += in[(size_t)c * nchunk + k].y;
+        }
+        sums[c] = s;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    const double scal = 1. / sums[kNormSlot].x;
+    if (search <= 0) {
+        pairs[p].re = sums[0].x * scal - sums[0].y * 0.0;
+        pairs[p].im = sums[0].x * 0.0 + sums[0].y * scal;
+        return;
+    }
+    const double kx = pairs[p].kx, ky = pairs[p].ky;
+    double max = 0, newKx = 0, newKy = 0, bre = 0, bim = 0;
+    for (int k = 0; k < nsel; k++) {
+        const int cc = pairs[p].sel[k], xi = cc % 10, yi = cc / 10;
+        const double2 s = sums[cc];
+        const double re = s.x * scal - s.y * 0.0, im = s.x * 0.0 + s.y * scal;
+        const double mag = hypot(re, im);
+        if (mag > max) {
+            max = mag;
+            newKx = grid_pos(kx, xi, search);
+            newKy = grid_pos(ky, yi, search);
+            bre = re;
+            bim = im;
+        }
+    }
+    pairs[p].kx = newKx;
+    pairs[p].ky = newKy;
+    pairs[p].search = search / 5;
+    pairs[p].re = bre;
+    pairs[p].im = bim;
+}
+
+// getPeak pairs from the fitted peak (SimAlgorithm.java:118-159): two bands,
+// getPeak(sep0, sep1, 0, 1, peak); three bands, the peak doubled when fitted
+// on band 1, then getPeak(sep0, sep_lb, 0, 1, peak / 2) and
+// getPeak(sep0, sep_hb, 0, 2, peak).
+__global__ void k_eb_gp_init(const EstPair *__restrict__ fit, EstPair *__restrict__ gp, int G, int nband,
+                             int fit_band, double *__restrict__ shift) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    double kx = fit[g].kx, ky = fit[g].ky;
+    EstPair p;
+    p.search = 0;
+    p.re = p.im = 0.0;
+    p.grp = g;
+    p.err = 0;
+    p.nsel = 1;
+    p.sel[0] = 0;
+    if (nband == 3) {
+        if (fit_band == 1) {
+            kx *= 2;
+            ky *= 2;
+        }
+        p.kx = kx / 2;
+        p.ky = ky / 2;
+        p.slot1 = 1;
+        p.bn1 = 1;
+        gp[2 * g] = p;
+        p.kx = kx;
+        p.ky = ky;
+        p.slot1 = 2;
+        p.bn1 = 2;
+        gp[2 * g + 1] = p;
+    } else {
+        p.kx = kx;
+        p.ky = ky;
+        p.slot1 = 1;
+        p.bn1 = 1;
+        gp[g] = p;
+    }
+    shift[2 * g] = -kx;
+    shift[2 * g + 1] = -ky;
+}
+
+// Per-group results: setPxPy(-peak), setPhaOff(phase of p1), modulations |p1|
+// (, |p2|) as Cplx.Double.abs = hypot (Cplx.java:49).
+__global__ void k_eb_finish(const EstPair *__restrict__ fit, const EstPair *__restrict__ gp, int G, int npk,
+                            const double *__restrict__ peaks, const double *__restrict__ shift,
+                            fsim_estimate_result *__restrict__ out, int *__restrict__ err) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    const EstPair &p1 = gp[(size_t)g * npk];
+    fsim_estimate_result r;
+    r.px = shift[2 * g];
+    r.py = shift[2 * g + 1];
+    r.pha_off = atan2(p1.im, p1.re);
+    r.mod1 = hypot(p1.re, p1.im);
+    r.mod2 = 0.0;
+    int e = fit[g].err | p1.err;
+    if (npk == 2) {
+        const EstPair &p2 = gp[(size_t)g * npk + 1];
+        r.mod2 = hypot(p2.re, p2.im);
+        e |= p2.err;
+    }
+    r.peak_x = peaks[2 * g];
+    r.peak_y = peaks[2 * g + 1];
+    out[g] = r;
+    err[g] = e;
+}
+
+// Device workspace of the batched estimation, grown on demand; one batch at
+// a time.
+struct EbWork {
+    std::mutex mu;
+    float2 *X = nullptr, *W = nullptr;
+    size_t x_bytes = 0, w_bytes = 0;
+    void *small = nullptr;   // pairs, groups, partial sums, results
+    size_t small_bytes = 0;
+    std::vector<double> fade_h;
+};
+static EbWork g_eb;
+
+static int eb_grow(void **p, size_t &have, size_t need) {
+    if (need <= have) return FSIM_OK;
+    (void)hipFree(*p);
+    *p = nullptr;
+    have = 0;
+    if (hipMalloc(p, need) != hipSuccess) return fail(FSIM_ERR_OOM, "estimate: workspace allocation failed");
+    have = need;
+    return FSIM_OK;
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static void grid_stride_launch(size_t n, unsigned &blocks) {
+    size_t b = (n + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > 65535 * 8) b = 65535 * 8;
+    blocks = (unsigned)b;
+}
+
+// One commonRegion / inverse FFT / correlation round for npairs pairs; with
+// `grid`, the 10 x 10 candidates of fitPeak through the approximate filter
+// (tabs: 20 n phasors per pair, apart: partial sums of n / rows blocks).
+static int eb_round(const EbRegion &r, EstPair *pairs, int npairs, bool grid, int nphase, int n, int nchunk,
+                    double2 *part, float2 *tabs, double2 *apart, hipStream_t s) {
+    const size_t nn = (size_t)n * n;
+    const unsigned pb = cdiv(nn, 256);
+    hipLaunchKernelGGL(k_eb_region, dim3(pb > 1024 ? 1024 : pb, npairs), dim3(256), 0, s, g_eb.X, g_eb.W, pairs,
+                       npairs, nphase, n, r);
+    FSIM_HIP(hipGetLastError());
+    int rc = fft2d_batch_device(g_eb.W, n, n, 2 * npairs, 1, s);
+    if (rc) return rc;
+    if (grid) {
+        const int rows = n < 32 ? n : 32, nblk = n / rows;
+        hipLaunchKernelGGL(k_eb_apx_tab, dim3(cdiv((size_t)20 * n, 256), npairs), dim3(256), 0, s, pairs, n, tabs);
+        hipLaunchKernelGGL(k_eb_approx, dim3(nblk, npairs), dim3(256), 0, s, g_eb.W, n, pairs, npairs, tabs, rows,
+                           nblk, apart);
+        hipLaunchKernelGGL(k_eb_select, dim3(npairs), dim3(128), 0, s, pairs, apart, nblk);
+        FSIM_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_eb_corr, dim3(nchunk, npairs), dim3(kCorrThreads), 0, s, g_eb.W, n, pairs, npairs, nchunk,
+                       part);
+    hipLaunchKernelGGL(k_eb_step, dim3(npairs), dim3(128), 0, s, pairs, part, nchunk);
+    FSIM_HIP(hipGetLastError());
+    return FSIM_OK;
+}
+
+}  // namespace fsim
+
+using namespace fsim;
+
+extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate_params *p, fsim_otf otf,
+                             fsim_estimate_result *res) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!images || !p || !otf || !res || count < 1) return fail(FSIM_ERR_ARG, "estimate: null argument or count < 1");
+    const int n = p->n, D = p->ndir, P = p->nphase, B = p->nband;
+    if (!fft_size_ok(n) || n < 16) return fail(FSIM_ERR_ARG, "estimate: n must be a power of two >= 16");
+    if (reinterpret_cast<uintptr_t>(images) % 16) return fail(FSIM_ERR_ARG, "estimate: images must be 16-byte aligned");
+    if (B != 2 && B != 3) return fail(FSIM_ERR_ARG, "estimate: 2 or 3 bands");
+    if (p->fit_band != 1 && p->fit_band != 2) return fail(FSIM_ERR_ARG, "estimate: Fitband neither 1 nor 2");
+    if (D < 1 || P < 2 * B - 1 || P > kMaxPhase) return fail(FSIM_ERR_ARG, "estimate: bad direction/phase count");
+    if (!(p->vec_cpm > 0)) return fail(FSIM_ERR_STATE, "estimate: vector pixel size not set");
+    if (p->fade_px < 0 || 2 * p->fade_px > n) return fail(FSIM_ERR_ARG, "estimate: fade border too wide");
+    const Otf &o = otf->o;
+    if (o.nband != 1 && (p->fit_band >= o.nband || B - 1 >= o.nband))
+        return fail(FSIM_ERR_ARG, "estimate: band idx too high for the OTF");
+    const int G = count * D, npk = B == 3 ? 2 : 1;
+    const size_t nn = (size_t)n * n;
+    if ((size_t)G * P > 65535 || (size_t)2 * npk * G > 65535)
+        return fail(FSIM_ERR_ARG, "estimate: batch too large (at most 65535 images per transform)");
+
+    // separation matrix for phases 2 pi p / P, default factors
+    std::vector<double> ph(P), Sd((size_t)2 * (2 * B - 1) * P);
+    for (int i = 0; i < P; i++) ph[i] = (2 * M_PI * i) / P + 0.0;
+    rc = separation_matrix_host(ph.data(), P, B, nullptr, Sd.data());
+    if (rc) return rc;
+    EbSep sep{};
+    sep.nphase = P;
+    sep.ncomp = B == 3 ? 3 : 2;
+    sep.fb = (B == 3 && p->fit_band == 2) ? 2 : 1;
+    const int comps[3] = {0, 1, 3};
+    for (int c = 0; c < sep.ncomp; c++)
+        for (int q = 0; q < P; q++) {
+            const size_t k = (size_t)comps[c] * P + q;
+            sep.S[c][q] = make_float2((float)Sd[2 * k], (float)Sd[2 * k + 1]);
+        }
+    sep.vec_cpm = p->vec_cpm;
+    sep.att_fwhm = 0.15 * o.cutoff;
+    EbRegion reg{o.vals, o.ns, o.nband, o.cpm, o.cutoff, p->vec_cpm};
+
+    const int nchunk = (int)((nn + kCorrPix - 1) / kCorrPix);
+    const int npair_max = npk * G;
+    std::lock_guard<std::mutex> lk(g_eb.mu);
+    rc = eb_grow((void **)&g_eb.X, g_eb.x_bytes, (size_t)G * P * nn * sizeof(float2));
+    if (!rc) rc = eb_grow((void **)&g_eb.W, g_eb.w_bytes, (size_t)2 * npair_max * nn * sizeof(float2));
+    // small buffers: fade table, groups, fit pairs, getPeak pairs, partials, peaks, shifts, results, errors
+    const size_t o_fade = 0, o_grp = align256(o_fade + 2 * n * sizeof(double));
+    const size_t o_fit = align256(o_grp + G * sizeof(EstGroup));
+    const size_t o_gp = align256(o_fit + G * sizeof(EstPair));
+    const size_t o_part = align256(o_gp + (size_t)npair_max * sizeof(EstPair));
+    const size_t o_peak = align256(o_part + (size_t)npair_max * kPairSlots * nchunk * sizeof(double2));
+    const size_t o_shift = align256(o_peak + 2 * G * sizeof(double));
+    const size_t o_res = align256(o_shift + 2 * G * sizeof(double));
+    const size_t o_err = align256(o_res + G * sizeof(fsim_estimate_result));
+    const int nblk = n / (n < 32 ? n : 32);
+    const size_t o_tabs = align256(o_err + G * sizeof(int));
+    const size_t o_apart = align256(o_tabs + (size_t)G * 20 * n * sizeof(float2));
+    const size_t small = align256(o_apart + (size_t)G * kPairSlots * nblk * sizeof(double2));
+    if (!rc) rc = eb_grow(&g_eb.small, g_eb.small_bytes, small);
+    if (rc) return rc;
+    char *sb = reinterpret_cast<char *>(g_eb.small);
+    double *fade = reinterpret_cast<double *>(sb + o_fade);
+    EstGroup *grp = reinterpret_cast<EstGroup *>(sb + o_grp);
+    EstPair *fit = reinterpret_cast<EstPair *>(sb + o_fit);
+    EstPair *gp = reinterpret_cast<EstPair *>(sb + o_gp);
+    double2 *part = reinterpret_cast<double2 *>(sb + o_part);
+    double *peaks = reinterpret_cast<double *>(sb + o_peak);
+    double *shift = reinterpret_cast<double *>(sb + o_shift);
+    fsim_estimate_result *resd = reinterpret_cast<fsim_estimate_result *>(sb + o_res);
+    int *errd = reinterpret_cast<int *>(sb + o_err);
+    float2 *tabs = reinterpret_cast<float2 *>(sb + o_tabs);
+    double2 *apart = reinterpret_cast<double2 *>(sb + o_apart);
+
+    hipStream_t s = stream();
+    g_eb.fade_h.resize(2 * (size_t)n);
+    fade_factors_host(n, p->fade_px, g_eb.fade_h.data(), g_eb.fade_h.data() + n);
+    FSIM_HIP(hipMemcpyAsync(fade, g_eb.fade_h.data(), 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
+    const unsigned gb = cdiv(G, 64);
+    hipLaunchKernelGGL(k_eb_init, dim3(gb), dim3(64), 0, s, grp, G);
+    unsigned blocks;
+    const size_t nquad = (size_t)G * P * nn / 4;
+    grid_stride_launch(nquad, blocks);
+    hipLaunchKernelGGL(k_eb_load, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const float4 *>(images),
+                       reinterpret_cast<float4 *>(g_eb.X), n, nquad, fade, fade + n);
+    FSIM_HIP(hipGetLastError());
+    rc = fft2d_batch_device(g_eb.X, n, n, G * P, 0, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_eb_separate, dim3(cdiv(nn, 256), G), dim3(256), 0, s, g_eb.X, g_eb.W, n, G, sep);
+    FSIM_HIP(hipGetLastError());
+    // coarse correlation: both to real space, c_fb * conj(c0), back
+    rc = fft2d_batch_device(g_eb.W, n, n, 2 * G, 1, s);
+    if (rc) return rc;
+    grid_stride_launch((size_t)G * nn, blocks);
+    hipLaunchKernelGGL(k_eb_times_conj, dim3(blocks), dim3(256), 0, s, g_eb.W, (size_t)G * nn);
+    FSIM_HIP(hipGetLastError());
+    rc = fft2d_batch_device(g_eb.W + (size_t)G * nn, n, n, G, 0, s);
+    if (rc) return rc;
+    const double kmin = p->fit_exclude * o.cutoff / p->vec_cpm;
+    const unsigned pkb = cdiv(nn, 256 * 8);
+    hipLaunchKernelGGL(k_eb_peak, dim3(pkb, G), dim3(256), 0, s, g_eb.W + (size_t)G * nn, n, kmin, grp);
+    hipLaunchKernelGGL(k_eb_fit_init, dim3(gb), dim3(64), 0, s, grp, fit, G, n, sep.fb, p->fit_band, peaks);
+    FSIM_HIP(hipGetLastError());
+    // fitPeak: three rounds of commonRegion, inverse FFTs, candidate filter, argmax
+    for (int iter = 0; iter < 3; iter++) {
+        rc = eb_round(reg, fit, G, true, P, n, nchunk, part, tabs, apart, s);
+        if (rc) return rc;
+    }
+    // getPeak of band 1 (and band 2)
+    hipLaunchKernelGGL(k_eb_gp_init, dim3(gb), dim3(64), 0, s, fit, gp, G, B, p->fit_band, shift);
+    FSIM_HIP(hipGetLastError());
+    rc = eb_round(reg, gp, npk * G, false, P, n, nchunk, part, tabs, apart, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_eb_finish, dim3(gb), dim3(64), 0, s, fit, gp, G, npk, peaks, shift, resd, errd);
+    FSIM_HIP(hipGetLastError());
+    std::vector<int> err(G);
+    FSIM_HIP(hipMemcpyAsync(res, resd, G * sizeof(fsim_estimate_result), hipMemcpyDeviceToHost, s));
+    FSIM_HIP(hipMemcpyAsync(err.data(), errd, G * sizeof(int), hipMemcpyDeviceToHost, s));
+    FSIM_HIP(hipStreamSynchronize(s));
+    for (int g = 0; g < G; g++)
+        if (err[g]) return fail(FSIM_ERR_ARG, "estimate: commonRegion shift larger than the vector (stack " +
+                                                  std::to_string(g / D) + ", direction " + std::to_string(g % D) + ")");
+    return FSIM_OK;
+}

@@ -53,6 +53,19 @@ class RECON_PARAMS(C.Structure):
     ]
 
 
+class ESTIMATE_PARAMS(C.Structure):
+    """fsim_estimate_params (include/fsim.h)."""
+    _fields_ = [("n", C.c_int), ("ndir", C.c_int), ("nphase", C.c_int), ("nband", C.c_int),
+                ("fit_band", C.c_int), ("fit_exclude", C.c_double), ("vec_cpm", C.c_double),
+                ("fade_px", C.c_int)]
+
+
+class ESTIMATE_RESULT(C.Structure):
+    """fsim_estimate_result (include/fsim.h)."""
+    _fields_ = [("px", C.c_double), ("py", C.c_double), ("pha_off", C.c_double), ("mod1", C.c_double),
+                ("mod2", C.c_double), ("peak_x", C.c_double), ("peak_y", C.c_double)]
+
+
 class SERVER_CONFIG(C.Structure):
     """fsim_server_config (include/fsim.h)."""
     _fields_ = [("mem_budget", C.c_int64), ("max_batch", C.c_int), ("max_queue", C.c_int),
@@ -124,6 +137,7 @@ _SIGS = {
     "fsim_common_region": [_P, _P, _I, _I, _P, _D, _D, _D, _D, _I, _D],
     "fsim_fit_peak": [_P, _P, _I, _I, _P, _D, _D, _D, _D, _D, C.POINTER(C.c_float), _PD],
     "fsim_get_peak": [_P, _P, _I, _I, _P, _D, _D, _D, _D, _PD],
+    "fsim_estimate": [_P, _I, C.POINTER(ESTIMATE_PARAMS), _P, C.POINTER(ESTIMATE_RESULT)],
     "fsim_channel_create": [C.POINTER(RECON_PARAMS), _P, C.POINTER(_P)],
     "fsim_channel_destroy": [_P],
     "fsim_reconstruct": [_P, _P, _P, _I, _P],

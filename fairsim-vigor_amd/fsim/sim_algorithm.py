@@ -367,6 +367,62 @@ def estimate_parameters(param, inFFT, fitBand, fitExclude):
     return results
 
 
+def estimate_batch(param, fitBand, fitExclude, images=None, images_dev=None, count=None, fade_px=10):
+    """fsim_estimate: SimAlgorithm.estimateParameters (SimAlgorithm.java:37-256)
+    for a batch of stacks in one native call, with the GUI's input preparation
+    (input_ffts: fadeBorderCos, FFT) on the device.  ``images``: host float32
+    [count, D, P, N, N]; or ``images_dev``: the device address of such a block
+    plus ``count``.  Returns, per stack, the per-direction result dicts of
+    estimate_parameters (plus the coarse ``peak``); ``param`` is not changed
+    (apply_estimate stores one stack's results in it)."""
+    _lib.init()
+    D, P = param.nrDir(), param.dir(0).nrPha()
+    n = param.vectorWidth()
+    buf = None
+    if images is not None:
+        images = np.ascontiguousarray(images, dtype=np.float32)
+        count = images.size // (D * P * n * n)
+        if images.size != count * D * P * n * n or count < 1:
+            raise ValueError("images must hold count x D x P x N x N values")
+        buf = DeviceBuffer(images.nbytes)
+        buf.upload(images)
+        images_dev = buf.value
+    elif images_dev is None or count is None:
+        raise ValueError("give images, or images_dev and count")
+    p = _lib.ESTIMATE_PARAMS(n=n, ndir=D, nphase=P, nband=param.nrBand(), fit_band=int(fitBand),
+                             fit_exclude=float(fitExclude), vec_cpm=param.pxlSizeCyclesMicron(),
+                             fade_px=int(fade_px))
+    res = (_lib.ESTIMATE_RESULT * (count * D))()
+    try:
+        call("fsim_estimate", C.c_void_p(int(images_dev)), int(count), C.byref(p), param.otf().device(), res)
+    finally:
+        if buf is not None:
+            buf.free()
+    out = []
+    for s in range(count):
+        dirs = []
+        for d in range(D):
+            r = res[s * D + d]
+            e = dict(px=r.px, py=r.py, pha_off=r.pha_off, mod1=r.mod1, peak=(r.peak_x, r.peak_y))
+            if param.nrBand() == 3:
+                e["mod2"] = r.mod2
+            dirs.append(e)
+        out.append(dirs)
+    return out
+
+
+def apply_estimate(param, dirs):
+    """Store one stack's estimate in ``param`` as estimateParameters does
+    (SimAlgorithm.java:139-142, 155-158): setPxPy, setPhaOff, setModulation."""
+    for d, r in enumerate(dirs):
+        dd = param.dir(d)
+        dd.setPxPy(r["px"], r["py"])
+        dd.setPhaOff(r["pha_off"])
+        dd.setModulation(1, r["mod1"])
+        if "mod2" in r:
+            dd.setModulation(2, r["mod2"])
+
+
 def input_ffts(images, fade_px=10):
     """The estimation input as the reference's GUI builds it
     (sim_gui/ImageControl.java:816-840): each float image faded at the border

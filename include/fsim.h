@@ -202,6 +202,44 @@ int fsim_fit_peak(fsim_vec band0, fsim_vec band1, int bn0, int bn1, fsim_otf otf
 int fsim_get_peak(fsim_vec band0, fsim_vec band1, int bn0, int bn1, fsim_otf otf, double kx, double ky,
                   double weight_limit, double vec_cpm, double *res);
 
+/* ---- batched parameter estimation ------------------------------------------
+ * SimAlgorithm.estimateParameters (SimAlgorithm.java:37-256) for `count`
+ * stacks at once, with the input preparation of the GUI
+ * (sim_gui/ImageControl.java:816-840: fadeBorderCos, copy to a complex
+ * vector, forward FFT).  Per stack and direction: band separation with
+ * equidistant phases 2 pi p / nphase and default factors, the coarse
+ * correlation and locatePeak (exclusion fit_exclude * cutoff / vec_cpm),
+ * fitPeak(band 0, fit band, -peak, weight limit 0.05, search 2.5) and getPeak
+ * of band 1 (and band 2 for three bands) — the same arithmetic as the
+ * per-call entry points above, in about 35 stream-ordered launches for the
+ * whole batch and one synchronisation. */
+typedef struct {
+    int n;              /* image side; square, a power of two              */
+    int ndir;           /* pattern directions                              */
+    int nphase;         /* phases per direction (equidistant)              */
+    int nband;          /* 2 or 3                                          */
+    int fit_band;       /* 1 or 2                                          */
+    double fit_exclude; /* locatePeak exclusion, fraction of the OTF cutoff */
+    double vec_cpm;     /* cycles per micron of the n x n vector           */
+    int fade_px;        /* fadeBorderCos border applied to every image; 0: none */
+} fsim_estimate_params;
+
+typedef struct {
+    double px, py;          /* shift as SimParam.Dir.setPxPy receives it (-peak)  */
+    double pha_off;         /* phase of getPeak on band 1 (setPhaOff)             */
+    double mod1, mod2;      /* |getPeak| of bands 1 and 2 (mod2 = 0 with 2 bands)  */
+    double peak_x, peak_y;  /* coarse correlation peak (locatePeak convention)    */
+} fsim_estimate_result;
+
+/* estimate: images = device pointer (16-byte aligned) to count x ndir x
+ *   nphase x n x n float32 (direction-major, as the stacks are taken);
+ *   res = host array of
+ *   count x ndir results.  Errors: FSIM_ERR_ARG for unsupported sizes or
+ *   bands (bands beyond the OTF's), or a shift larger than the vector in
+ *   commonRegion (the reference fails there with an index exception). */
+int fsim_estimate(const float *images, int count, const fsim_estimate_params *p, fsim_otf otf,
+                  fsim_estimate_result *res);
+
 /* ---- fused fast-shift reconstruction --------------------------------------
  * One reconstruction = livemode/ReconstructionRunner.java:456-542 /
  * tests/TestAccelSpeed.java:237-252, 563-703 for one channel: raw stack of

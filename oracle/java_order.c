@@ -770,6 +770,11 @@ static void shifted_corr(const cf *b0, const cf *b1, int N, double kx, double ky
     res[1] = im;
 }
 
+/* one candidate of shifted_corr: sum(shift(b1, kx, ky) * conj(b0)) */
+void ref_shifted_corr(const cf *b0, const cf *b1, int N, double kx, double ky, double *res) {
+    shifted_corr(b0, b1, N, kx, ky, res);
+}
+
 /* copies -> commonRegion(0.15, wlim, divide) -> inverse FFT of both */
 static void corr_pair(const cf *band0, const cf *band1, int N, const cf *tab0, const cf *tab1,
                       int ns, double cpm, double cutoff, double vec_cpm, double kx, double ky,

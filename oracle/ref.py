@@ -84,6 +84,7 @@ def lib():
         L.ref_common_region.argtypes = [P, P, I, I, P, P, I, D, D, D, D, D, D, D, I]
         L.ref_fit_peak.argtypes = [P, P, I, P, P, I, D, D, D, D, D, D, D, P, P]
         L.ref_get_peak.argtypes = [P, P, I, P, P, I, D, D, D, D, D, D, P]
+        L.ref_shifted_corr.argtypes = [P, P, I, D, D, P]
         _lib = L
     return _lib
 
@@ -225,6 +226,14 @@ def get_peak32(b0, b1, otf, vec_cpm, kx, ky, wlim):
     lib().ref_get_peak(ptr(b0), ptr(b1), n, ptr(otf.vals), ptr(otf.vals), otf.ns, otf.cpm, otf.cutoff,
                        vec_cpm, kx, ky, wlim, ptr(res))
     return complex(res[0], res[1])
+
+
+def shifted_corr32(b0, b1, kx, ky):
+    """sum(b1.fourierShift(kx, ky) * conj(b0)) in the reference's arithmetic
+    (float phase, MTool polynomial, float products, double sum)."""
+    r = np.zeros(2)
+    lib().ref_shifted_corr(ptr(b0), ptr(b1), b0.shape[0], float(kx), float(ky), ptr(r))
+    return complex(r[0], r[1])
 
 
 def times_conj32(y, x):

@@ -118,3 +118,73 @@ def test_estimate_parameters_matches_oracle(fsim_lib, nphase, nband, fit_band, e
         assert param.dir(d).getPhaOff() == g["pha_off"]
         scale = 1 if nband == 2 else 2
         assert abs(param.dir(d).px(nband - 1) - scale * sp["px"][d]) < 0.1
+
+
+def _stacks(count, nphase, k_fracs, harmonic2, n=256, seed0=11):
+    from fsim import synthetic
+    imgs, sps = [], []
+    for s in range(count):
+        raw, sp = synthetic.sim_stack(n=n, nphase=nphase, seed=seed0 + s, nbeads=150, nfil=20,
+                                      harmonic2=harmonic2, k_frac=k_fracs[s % len(k_fracs)])
+        imgs.append(raw.reshape(3, nphase, n, n).astype(np.float32))
+        sps.append(sp)
+    return np.stack(imgs), sps
+
+
+# The batched estimation runs the same kernels as the per-call flow
+# (Correlation.* entry points over device vectors); only |b0|^2 is summed in a
+# different order, so positions agree exactly and phases/modulations to
+# double rounding.
+@pytest.mark.parametrize("nphase,nband,fit_band,exclude,k_fracs,h2",
+                         [(3, 2, 1, 0.6, (0.85, 0.8, 0.75), 0.0), (5, 3, 2, 0.6, (0.45, 0.42), 0.3),
+                          (5, 3, 1, 0.3, (0.45,), 0.3), (5, 2, 2, 0.6, (0.85,), 0.0)])
+def test_estimate_batch_matches_per_call_flow(fsim_lib, nphase, nband, fit_band, exclude, k_fracs, h2):
+    from fsim.sim_algorithm import SimParam, apply_estimate, estimate_batch, estimate_parameters, input_ffts
+    n, count = 256, 3
+    images, sps = _stacks(count, nphase, k_fracs, h2, n=n)
+    param = SimParam.create(nband, 3, nphase, n, PX_UM, _otf(n))
+    got = estimate_batch(param, fit_band, exclude, images=images)
+    assert len(got) == count
+    for s in range(count):
+        want = estimate_parameters(param, input_ffts(images[s]), fit_band, exclude)
+        for d in range(3):
+            g, w = got[s][d], want[d]
+            print(f"stack {s} dir {d}: batch {g} per-call {w}")
+            assert g["px"] == w["px"] and g["py"] == w["py"]
+            assert abs(g["pha_off"] - w["pha_off"]) <= 1e-9
+            assert abs(g["mod1"] - w["mod1"]) <= 1e-9 * w["mod1"]
+            if nband == 3:
+                assert abs(g["mod2"] - w["mod2"]) <= 1e-9 * w["mod2"]
+        apply_estimate(param, got[s])
+        scale = 1 if nband == 2 else 2
+        for d in range(3):
+            assert abs(param.dir(d).px(nband - 1) - scale * sps[s]["px"][d]) < 0.1
+            assert abs(param.dir(d).py(nband - 1) - scale * sps[s]["py"][d]) < 0.1
+
+
+def test_estimate_batch_alone_equals_in_batch(fsim_lib):
+    """Each stack's estimate is independent of the batch it runs in (bitwise)."""
+    from fsim.sim_algorithm import SimParam, estimate_batch
+    n = 256
+    images, _ = _stacks(4, 3, (0.85, 0.7), 0.0, n=n, seed0=40)
+    param = SimParam.create(2, 3, 3, n, PX_UM, _otf(n))
+    mixed = estimate_batch(param, 1, 0.6, images=images)
+    alone = estimate_batch(param, 1, 0.6, images=images[2:3])
+    assert alone[0] == mixed[2]
+
+
+def test_estimate_batch_matches_oracle(fsim_lib):
+    from fsim.sim_algorithm import SimParam, estimate_batch
+    n = 256
+    images, _ = _stacks(2, 3, (0.85, 0.78), 0.0, n=n, seed0=70)
+    param = SimParam.create(2, 3, 3, n, PX_UM, _otf(n))
+    got = estimate_batch(param, 1, 0.6, images=images)
+    for s in range(2):
+        want = ref.estimate32(ref.input_ffts32(images[s]), 2, 1, 0.6, ref.Otf32(NA, LAM, A_OTF), 1 / (n * PX_UM))
+        for d in range(3):
+            g, w = got[s][d], want[d]
+            print(f"stack {s} dir {d}: gpu {g} oracle {w}")
+            assert g["peak"] == (w["peak"][0], w["peak"][1])   # coarse peak pixel: bit-exact
+            assert g["px"] == w["px"] and g["py"] == w["py"]
+            assert abs(g["pha_off"] - w["pha_off"]) < 1e-4
+            assert abs(g["mod1"] - w["mod1"]) <= 1e-4 * w["mod1"]
