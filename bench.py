@@ -247,12 +247,13 @@ def large_fov(batch=4, steps=5):
             "roofline_frac": round(value * 434 * n * n / 8e12, 4), "rel_l2_vs_oracle": err}
 
 
-def estimation(runs=3):
+def estimation(runs=3, batch=64):
     """Config 1: parameter estimation (SimAlgorithm.estimateParameters) of one
     512^2 stack given as float images: 3 phases / 2 bands (fit band 1) and
     5 phases / 3 bands with a 2 k0 harmonic (|k0| = 0.45 cutoff, fit band 2),
-    GPU (device vector flow) vs the FP32 oracle on the host cores; and the
-    estimate + fused reconstruction of the same stack."""
+    GPU (device vector flow) vs the FP32 oracle on the host cores; the
+    estimate + fused reconstruction of the same stack; and the batched form
+    (estimation_batched)."""
     import numpy as np
     from fsim import synthetic
     from fsim.sim_algorithm import OtfProvider, Reconstructor, SimParam, estimate_parameters, input_ffts
@@ -294,7 +295,61 @@ def estimation(runs=3):
                     "estimate_plus_recon_ms": round(er_ms, 2),
                     "cpu_oracle_ms_per_estimate": round(cpu_ms, 1), "cpu_threads": threads,
                     "max_k_error_px": round(err_k, 4)}
+        out[tag]["batched"] = estimation_batched(nphase, nband, fb, k_frac, h2, batch, runs)
     return out
+
+
+def estimation_batched(nphase, nband, fb, k_frac, h2, batch=64, runs=3):
+    """Config 1 as a throughput: `batch` 512^2 stacks per call (4 distinct
+    seeded stacks repeated), float32 images and uint16 raw frames resident on
+    the device; fsim_estimate for all of them, then fsim_reconstruct_params
+    with every stack's own estimate.  Timed: the whole sequence including the
+    host step that turns estimates into per-stack parameters."""
+    import numpy as np
+    from fsim import synthetic
+    from fsim.linalg import DeviceBuffer
+    from fsim.sim_algorithm import Reconstructor, estimate_batch, params_from_estimate
+    n = 512
+    raws, sps = [], []
+    for s in range(4):
+        raw, sp = synthetic.sim_stack(n=n, nphase=nphase, seed=11 + s, harmonic2=h2, k_frac=k_frac)
+        raws.append(raw)
+        sps.append(sp)
+    raw_b = np.ascontiguousarray(np.concatenate([np.stack(raws)] * (batch // 4)))
+    img_b = raw_b.reshape(batch, 3, nphase, n, n).astype(np.float32)
+    param = synthetic.make_param(sps[0], n, nphase=nphase, nband=nband)
+    rec = Reconstructor(param)
+    rb, ib = DeviceBuffer(raw_b.nbytes), DeviceBuffer(img_b.nbytes)
+    ob = DeviceBuffer(batch * rec.out_bytes())
+    rb.upload(raw_b)
+    ib.upload(img_b)
+    del img_b
+
+    def run(recon):
+        est = estimate_batch(param, fb, 0.6, images_dev=ib.value, count=batch)
+        if recon:
+            rec.reconstruct_params(rb.value, ob.value, [params_from_estimate(param, e) for e in est])
+        _lib_sync()
+        return est
+
+    est = run(True)
+    scale = 1 if nband == 2 else 2
+    err_k = max(abs(est[s][d]["px"] - scale * sps[s % 4]["px"][d]) + abs(est[s][d]["py"] - scale * sps[s % 4]["py"][d])
+                for s in range(4) for d in range(3))
+    res = {"stacks_per_call": batch}
+    for recon, key in ((False, "estimates_per_s"), (True, "estimate_plus_recon_per_s")):
+        t = []
+        for _ in range(runs):
+            a = time.perf_counter()
+            run(recon)
+            t.append(time.perf_counter() - a)
+        res[key] = round(batch / float(np.median(t)), 1)
+    # estimate + reconstruction algorithmic bytes with float input: 1268 n^2
+    res["roofline_frac_estimate_plus_recon"] = round(res["estimate_plus_recon_per_s"] * 1268 * n * n / 8e12, 4)
+    res["max_k_error_px"] = round(err_k, 4)
+    for b in (rb, ib, ob):
+        b.free()
+    return res
 
 
 def _lib_sync():
@@ -321,6 +376,8 @@ def main():
                     help="skip the estimation, mixed-batch and 2048^2 legs")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed batched loop (for rocprofv3 runs)")
+    ap.add_argument("--estimation-only", action="store_true",
+                    help="only the batched estimation legs of config 1 (for rocprofv3 runs)")
     ap.add_argument("--probe-copy", action="store_true",
                     help="run a plain streaming kernel of known size (HBM counter calibration)")
     args = ap.parse_args()
@@ -338,6 +395,13 @@ def main():
     _lib.init(0)
     if args.probe_copy:
         probe_copy(args.steps)
+        return
+    if args.estimation_only:
+        out = {}
+        for tag, nphase, nband, fb, k_frac, h2 in (("p3b2", 3, 2, 1, 0.85, 0.0), ("p5b3", 5, 3, 2, 0.45, 0.3)):
+            log(f"batched estimation {tag}")
+            out[tag] = estimation_batched(nphase, nband, fb, k_frac, h2, B, 3)
+        print(json.dumps({"estimation_only": True, "n": 512, "batched": out}), flush=True)
         return
     log(f"generating synthetic stacks n={n}")
     stacks, sps = [], []

@@ -216,26 +216,8 @@ __global__ void k_wiener(float *__restrict__ out, int w, int h, const float2 *__
                          double vec_cpm, ShiftSet sh, int use_att, float w2) {
 #pragma clang fp contract(off)
     GRID_STRIDE(i, (size_t)w * h) {
-        const int x = (int)(i % w), y = (int)(i / w);
-        const double xh = xh_of(x, w), yh = yh_of(y, h);
-        float acc = 0.f;
-        for (int d = 0; d < sh.ndir; d++)
-            for (int b = 0; b < sh.nband; b++) {
-                const double bx = sh.px[d] * b, by = sh.py[d] * b;
-                const double r1 = sqrt((xh - bx) * (xh - bx) + (yh - by) * (yh - by)) * vec_cpm;
-                const double r2 = sqrt((xh + bx) * (xh + bx) + (yh + by) * (yh + by)) * vec_cpm;
-                // a single-band OTF serves every band (OtfProvider.java:192-193)
-                const size_t tb = (size_t)(otf_nb > 1 ? b : 0) * ns;
-                const float2 o1 = otf_val(vals + tb, ns, cpm, cutoff, r1);
-                const float2 o2 = otf_val(vals + tb, ns, cpm, cutoff, r2);
-                float v1 = o1.x * o1.x + o1.y * o1.y;
-                float v2 = o2.x * o2.x + o2.y * o2.y;
-                if (use_att) {
-                    v1 *= att_val(only_att + tb, cpm, cutoff, r1);
-                    v2 *= att_val(only_att + tb, cpm, cutoff, r2);
-                }
-                acc = acc + v1 + v2;
-            }
+        const float acc =
+            wiener_sum((int)(i % w), (int)(i / w), w, h, vals, only_att, ns, otf_nb, cpm, cutoff, vec_cpm, sh, use_att);
         out[i] = 1 / (acc + w2);
     }
 }

@@ -608,6 +608,38 @@ __global__ void k_pack_wa(float2 *__restrict__ wa, const float *__restrict__ w, 
         wa[i] = make_float2(w[i], a[i]);
 }
 
+// Sub-pixel ramp tables exp(2 pi i f t / n), t < n, of the x and y
+// fractional shifts f of every direction and component
+// (SimUtils.java:74-81), from double precision: ramp[k * n + t] for the
+// shift f[k].
+__global__ void k_ramps(float2 *__restrict__ ramp, const double *__restrict__ f, int n, size_t total) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int t = (int)(i % n);
+        const double ph = 2 * M_PI * f[i / n] * t / n;
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        ramp[i] = make_float2((float)cs, (float)sn);
+    }
+}
+
+// Per-stack (1 / Wiener denominator, apodization) tables: stack s uses its
+// own shifts sh[s] and the channel's apodization (wa0.y).  Grid (pixels,
+// stacks).
+__global__ void k_wiener_multi(float2 *__restrict__ wa, const float2 *__restrict__ wa0, int M,
+                               const ShiftSet *__restrict__ sh, const float2 *__restrict__ vals,
+                               const float *__restrict__ only_att, int ns, int otf_nb, double cpm, double cutoff,
+                               double vec_cpm, int use_att, float w2) {
+#pragma clang fp contract(off)
+    const size_t mm = (size_t)M * M;
+    const int st = blockIdx.y;
+    const ShiftSet S = sh[st];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < mm; i += (size_t)gridDim.x * blockDim.x) {
+        const float acc =
+            wiener_sum((int)(i % M), (int)(i / M), M, M, vals, only_att, ns, otf_nb, cpm, cutoff, vec_cpm, S, use_att);
+        wa[(size_t)st * mm + i] = make_float2(1 / (acc + w2), wa0[i].y);
+    }
+}
+
 // Optional stage timing: HIP events recorded around each pipeline kernel on
 // the launch stream, summed per stage when read back.
 struct StageTimer {
@@ -769,8 +801,26 @@ int build_channel_const(const fsim_recon_params &p, const Otf &o, bool sym, Reco
 
 using namespace fsim;
 
+// Device state of reconstructions with per-stack parameters
+// (fsim_reconstruct_params), grown on demand.
+struct MultiWork {
+    ReconConst *dK = nullptr;
+    float2 *wa = nullptr, *ramp = nullptr;
+    double *f = nullptr;
+    ShiftSet *sh = nullptr;
+    Job *jobs = nullptr;
+    int cap = 0;
+    hipEvent_t uploaded = nullptr;   // recorded after the uploads from the host arrays below
+    std::vector<ReconConst> hK;
+    std::vector<double> hf;
+    std::vector<ShiftSet> hsh;
+    std::vector<Job> hjobs;
+};
+
 struct fsim_channel_s {
     fsim_recon_params p;
+    Otf o;
+    bool sym = false;
     ReconConst K;
     ReconConst *dK = nullptr;
     float2 *wa = nullptr;
@@ -779,6 +829,7 @@ struct fsim_channel_s {
     double *fade = nullptr;
     float2 *A = nullptr, *B = nullptr;
     int cap = 0;
+    MultiWork mw;
 };
 
 const ReconConst *fsim::channel_device_const(const fsim_channel_s *ch) { return ch->dK; }
@@ -792,6 +843,53 @@ static void channel_free(fsim_channel_s *ch) {
     (void)hipFree(ch->fade);
     (void)hipFree(ch->A);
     (void)hipFree(ch->B);
+    (void)hipFree(ch->mw.dK);
+    (void)hipFree(ch->mw.wa);
+    (void)hipFree(ch->mw.ramp);
+    (void)hipFree(ch->mw.f);
+    (void)hipFree(ch->mw.sh);
+    (void)hipFree(ch->mw.jobs);
+    if (ch->mw.uploaded) (void)hipEventDestroy(ch->mw.uploaded);
+}
+
+// ramp tables of a set of stacks' constants: fx/fy of each direction and
+// component, computed on the device
+static int fill_ramps(float2 *ramp, double *fdev, std::vector<double> &hf, const ReconConst *K, int count, int n,
+                      hipStream_t s) {
+    const size_t per = (size_t)kMaxDir * kMaxComp * 2;
+    hf.assign(per * count, 0.0);
+    for (int st = 0; st < count; st++)
+        for (int d = 0; d < kMaxDir; d++)
+            for (int c = 0; c < kMaxComp; c++) {
+                hf[st * per + (d * kMaxComp + c) * 2] = K[st].fx[d][c];
+                hf[st * per + (d * kMaxComp + c) * 2 + 1] = K[st].fy[d][c];
+            }
+    FSIM_HIP(hipMemcpyAsync(fdev, hf.data(), hf.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    const size_t total = per * count * n;
+    hipLaunchKernelGGL(k_ramps, dim3((unsigned)std::min<size_t>((total + 255) / 256, 65535 * 4)), dim3(256), 0, s,
+                       ramp, fdev, n, total);
+    FSIM_HIP(hipGetLastError());
+    return FSIM_OK;
+}
+
+// A/B workspaces for `count` stacks
+static int ensure_workspace(fsim_channel_s *ch, int count, hipStream_t s) {
+    if (count <= ch->cap) return FSIM_OK;
+    const ReconConst &K = ch->K;
+    FSIM_HIP(hipStreamSynchronize(s));
+    (void)hipFree(ch->A);
+    (void)hipFree(ch->B);
+    ch->A = ch->B = nullptr;
+    ch->cap = 0;
+    if (hipMalloc(&ch->A, workspace_a_elems(K.n, K.D, K.Cs) * count * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&ch->B, workspace_b_elems(K.n) * count * sizeof(float2)) != hipSuccess) {
+        (void)hipFree(ch->A);
+        ch->A = nullptr;
+        (void)hipGetLastError();
+        return fail(FSIM_ERR_OOM, "reconstruct: workspace allocation failed");
+    }
+    ch->cap = count;
+    return FSIM_OK;
 }
 
 extern "C" {
@@ -817,16 +915,21 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
         delete ch;
         return rc;
     }
+    ch->o = o;
+    ch->sym = sym;
     const int n = p->n, M = 2 * n;
     const size_t mm = (size_t)M * M;
     float *wtmp = nullptr, *atmp = nullptr;
+    double *fdev = nullptr;
     if (hipMalloc(&ch->dK, sizeof(ReconConst)) != hipSuccess || hipMalloc(&ch->wa, mm * sizeof(float2)) != hipSuccess ||
         hipMalloc(&ch->otf2d, (size_t)o.nband * n * n * sizeof(float2)) != hipSuccess ||
         hipMalloc(&ch->fade, 2 * n * sizeof(double)) != hipSuccess ||
         hipMalloc(&ch->ramp, (size_t)kMaxDir * kMaxComp * 2 * n * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&fdev, (size_t)kMaxDir * kMaxComp * 2 * sizeof(double)) != hipSuccess ||
         hipMalloc(&wtmp, mm * sizeof(float)) != hipSuccess || hipMalloc(&atmp, mm * sizeof(float)) != hipSuccess) {
         (void)hipFree(wtmp);
         (void)hipFree(atmp);
+        (void)hipFree(fdev);
         channel_free(ch);
         delete ch;
         return fail(FSIM_ERR_OOM, "channel_create: device allocation failed");
@@ -840,20 +943,9 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     ch->K.fade_px = p->fade_px;
     ch->K.wa = ch->wa;
     {
-        // per direction/component ramp tables exp(2 pi i f t / n), t < n, for
-        // the x and y fractional shifts, from double precision
-        std::vector<float2> r((size_t)kMaxDir * kMaxComp * 2 * n, make_float2(1.f, 0.f));
-        for (int d = 0; d < p->ndir; d++)
-            for (int c = 0; c < ch->K.C; c++)
-                for (int a = 0; a < 2; a++) {
-                    const double f = a == 0 ? ch->K.fx[d][c] : ch->K.fy[d][c];
-                    float2 *row = r.data() + ((size_t)(d * kMaxComp + c) * 2 + a) * n;
-                    for (int t = 0; t < n; t++) {
-                        const double ph = 2 * M_PI * f * t / n;
-                        row[t] = make_float2((float)std::cos(ph), (float)std::sin(ph));
-                    }
-                }
-        FSIM_HIP(hipMemcpyAsync(ch->ramp, r.data(), r.size() * sizeof(float2), hipMemcpyHostToDevice, s));
+        std::vector<double> hf;
+        rc = fill_ramps(ch->ramp, fdev, hf, &ch->K, 1, n, s);
+        if (rc) return rc;
         FSIM_HIP(hipStreamSynchronize(s));
         ch->K.ramp = ch->ramp;
     }
@@ -886,6 +978,7 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     FSIM_HIP(hipStreamSynchronize(s));
     (void)hipFree(wtmp);
     (void)hipFree(atmp);
+    (void)hipFree(fdev);
     *out = ch;
     return FSIM_OK;
 }
@@ -905,21 +998,8 @@ int fsim_reconstruct(fsim_channel ch, const uint16_t *raw, float *out, int count
     const ReconConst &K = ch->K;
     const int n = K.n;
     hipStream_t s = strm ? (hipStream_t)strm : stream();
-    if (count > ch->cap) {
-        FSIM_HIP(hipStreamSynchronize(s));
-        (void)hipFree(ch->A);
-        (void)hipFree(ch->B);
-        ch->A = ch->B = nullptr;
-        ch->cap = 0;
-        if (hipMalloc(&ch->A, workspace_a_elems(n, K.D, K.Cs) * count * sizeof(float2)) != hipSuccess ||
-            hipMalloc(&ch->B, workspace_b_elems(n) * count * sizeof(float2)) != hipSuccess) {
-            (void)hipFree(ch->A);
-            ch->A = nullptr;
-            (void)hipGetLastError();
-            return fail(FSIM_ERR_OOM, "reconstruct: workspace allocation failed");
-        }
-        ch->cap = count;
-    }
+    rc = ensure_workspace(ch, count, s);
+    if (rc) return rc;
     JobList jl{};
     jl.jobs = nullptr;
     jl.base.c = ch->dK;
@@ -933,6 +1013,104 @@ int fsim_reconstruct(fsim_channel ch, const uint16_t *raw, float *out, int count
     jl.out_stride = (size_t)4 * n * n;
     jl.raw_aligned = ((uintptr_t)raw & 7) == 0;
     return launch_pipeline(n, jl, count, K.D, K.Cs, s);
+}
+
+int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw, float *out, int count, const fsim_sim_params *sp,
+                            void *strm) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!ch || !raw || !out || !sp || count < 1) return fail(FSIM_ERR_ARG, "reconstruct_params: bad arguments");
+    if (count > 65535) return fail(FSIM_ERR_ARG, "reconstruct_params: at most 65535 stacks per call");
+    const ReconConst &K0 = ch->K;
+    const int n = K0.n, M = 2 * n;
+    const size_t mm = (size_t)M * M, rper = (size_t)kMaxDir * kMaxComp * 2 * n;
+    hipStream_t s = strm ? (hipStream_t)strm : stream();
+    MultiWork &w = ch->mw;
+    // the previous call's uploads read the host arrays rewritten below
+    if (w.uploaded) FSIM_HIP(hipEventSynchronize(w.uploaded));
+    // host constants of every stack: the channel's settings with the stack's
+    // shifts, phases and modulations
+    w.hK.resize(count);
+    w.hsh.resize(count);
+    for (int st = 0; st < count; st++) {
+        fsim_recon_params p = ch->p;
+        for (int d = 0; d < p.ndir; d++) {
+            p.px[d] = sp[st].px[d];
+            p.py[d] = sp[st].py[d];
+            for (int ph = 0; ph < p.nphase; ph++) p.phases[d][ph] = sp[st].phases[d][ph];
+            for (int b = 0; b < p.nband; b++) p.modul[d][b] = sp[st].modul[d][b];
+        }
+        rc = build_channel_const(p, ch->o, ch->sym, w.hK[st]);
+        if (rc) return rc;
+        ShiftSet &S = w.hsh[st];
+        S = ShiftSet{};
+        S.ndir = p.ndir;
+        S.nband = p.nband;
+        for (int d = 0; d < p.ndir; d++) {
+            S.px[d] = p.px[d];
+            S.py[d] = p.py[d];
+        }
+    }
+    if (count > w.cap) {
+        FSIM_HIP(hipStreamSynchronize(s));
+        (void)hipFree(w.dK);
+        (void)hipFree(w.wa);
+        (void)hipFree(w.ramp);
+        (void)hipFree(w.f);
+        (void)hipFree(w.sh);
+        (void)hipFree(w.jobs);
+        hipEvent_t ev = w.uploaded;
+        w = MultiWork{};
+        w.uploaded = ev;
+        if (hipMalloc(&w.dK, count * sizeof(ReconConst)) != hipSuccess ||
+            hipMalloc(&w.wa, count * mm * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&w.ramp, count * rper * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&w.f, (size_t)count * kMaxDir * kMaxComp * 2 * sizeof(double)) != hipSuccess ||
+            hipMalloc(&w.sh, count * sizeof(ShiftSet)) != hipSuccess ||
+            hipMalloc(&w.jobs, count * sizeof(Job)) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(FSIM_ERR_OOM, "reconstruct_params: device allocation failed");
+        }
+        w.cap = count;
+        // the host constants were built before the reset
+        return fsim_reconstruct_params(ch, raw, out, count, sp, strm);
+    }
+    rc = ensure_workspace(ch, count, s);
+    if (rc) return rc;
+    w.hjobs.resize(count);
+    bool aligned = true;
+    for (int st = 0; st < count; st++) {
+        ReconConst &K = w.hK[st];
+        K.fade_lo = K0.fade_lo;
+        K.fade_hi = K0.fade_hi;
+        K.fade_px = K0.fade_px;
+        K.otf2d = K0.otf2d;
+        K.wa = w.wa + st * mm;
+        K.ramp = w.ramp + st * rper;
+        Job &j = w.hjobs[st];
+        j.c = w.dK + st;
+        j.raw = raw + (size_t)st * K0.D * K0.P * n * n;
+        j.A = ch->A + st * workspace_a_elems(n, K0.D, K0.Cs);
+        j.B = ch->B + st * workspace_b_elems(n);
+        j.out = out + (size_t)st * 4 * n * n;
+        aligned = aligned && ((uintptr_t)j.raw & 7) == 0;
+    }
+    FSIM_HIP(hipMemcpyAsync(w.dK, w.hK.data(), count * sizeof(ReconConst), hipMemcpyHostToDevice, s));
+    FSIM_HIP(hipMemcpyAsync(w.sh, w.hsh.data(), count * sizeof(ShiftSet), hipMemcpyHostToDevice, s));
+    FSIM_HIP(hipMemcpyAsync(w.jobs, w.hjobs.data(), count * sizeof(Job), hipMemcpyHostToDevice, s));
+    rc = fill_ramps(w.ramp, w.f, w.hf, w.hK.data(), count, n, s);
+    if (rc) return rc;
+    if (!w.uploaded) FSIM_HIP(hipEventCreateWithFlags(&w.uploaded, hipEventDisableTiming));
+    FSIM_HIP(hipEventRecord(w.uploaded, s));
+    const Otf &o = ch->o;
+    hipLaunchKernelGGL(k_wiener_multi, dim3((unsigned)std::min<size_t>((mm + 255) / 256, 2048), count), dim3(256), 0,
+                       s, w.wa, ch->wa, M, w.sh, o.vals, o.only_att, o.ns, o.nband, o.cpm, o.cutoff, ch->p.vec_cpm,
+                       ch->p.use_att, (float)(ch->p.wiener * ch->p.wiener));
+    FSIM_HIP(hipGetLastError());
+    JobList jl{};
+    jl.jobs = w.jobs;
+    jl.raw_aligned = aligned;
+    return launch_pipeline(n, jl, count, K0.D, K0.Cs, s);
 }
 
 int fsim_stage_timing(int enable) {

@@ -53,6 +53,12 @@ class RECON_PARAMS(C.Structure):
     ]
 
 
+class SIM_PARAMS(C.Structure):
+    """fsim_sim_params (include/fsim.h)."""
+    _fields_ = [("px", C.c_double * 5), ("py", C.c_double * 5), ("phases", (C.c_double * 7) * 5),
+                ("modul", (C.c_double * 3) * 5)]
+
+
 class ESTIMATE_PARAMS(C.Structure):
     """fsim_estimate_params (include/fsim.h)."""
     _fields_ = [("n", C.c_int), ("ndir", C.c_int), ("nphase", C.c_int), ("nband", C.c_int),
@@ -141,6 +147,7 @@ _SIGS = {
     "fsim_channel_create": [C.POINTER(RECON_PARAMS), _P, C.POINTER(_P)],
     "fsim_channel_destroy": [_P],
     "fsim_reconstruct": [_P, _P, _P, _I, _P],
+    "fsim_reconstruct_params": [_P, _P, _P, _I, C.POINTER(SIM_PARAMS), _P],
     "fsim_server_create": [C.POINTER(SERVER_CONFIG), C.POINTER(_P)],
     "fsim_server_destroy": [_P],
     "fsim_server_add_channel": [_P, _P, _PI],

@@ -524,6 +524,37 @@ class PlainPipeline:
         return self.result
 
 
+def sim_params(param):
+    """Pack the per-direction SIM parameters of a SimParam (pX, pY,
+    getPhases, modulations) into fsim_sim_params."""
+    p = _lib.SIM_PARAMS()
+    for d in range(param.nrDir()):
+        dd = param.dir(d)
+        p.px[d], p.py[d] = dd.pX, dd.pY
+        for i, ph in enumerate(dd.getPhases()):
+            p.phases[d][i] = ph
+        for b in range(param.nrBand()):
+            p.modul[d][b] = dd.modul[b]
+    return p
+
+
+def params_from_estimate(param, dirs):
+    """fsim_sim_params of one stack from its estimate (estimate_batch's
+    per-direction dicts) on top of ``param``'s phases and modulations, as
+    estimateParameters leaves a SimParam (setPxPy, setPhaOff, setModulation)."""
+    p = sim_params(param)
+    B = param.nrBand()
+    for d, r in enumerate(dirs):
+        dd = param.dir(d)
+        p.px[d], p.py[d] = r["px"] / (B - 1), r["py"] / (B - 1)
+        for i, ph in enumerate(dd.phases):
+            p.phases[d][i] = ph + r["pha_off"]
+        p.modul[d][1] = r["mod1"]
+        if B == 3:
+            p.modul[d][2] = r["mod2"]
+    return p
+
+
 def recon_params(param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10, mode=0):
     """Pack a SimParam into fsim_recon_params."""
     p = RECON_PARAMS()
@@ -585,6 +616,16 @@ class Reconstructor:
         """raw_dev / out_dev: device addresses (int) of count stacks/images."""
         call("fsim_reconstruct", self._h, C.c_void_p(int(raw_dev)), C.c_void_p(int(out_dev)),
              int(count), C.c_void_p(stream) if stream else None)
+
+    def reconstruct_params(self, raw_dev, out_dev, params, stream=None):
+        """fsim_reconstruct_params: len(params) stacks, stack i with the SIM
+        parameters params[i] (a SimParam, or an fsim_sim_params)."""
+        count = len(params)
+        arr = (_lib.SIM_PARAMS * count)()
+        for i, prm in enumerate(params):
+            arr[i] = prm if isinstance(prm, _lib.SIM_PARAMS) else sim_params(prm)
+        call("fsim_reconstruct_params", self._h, C.c_void_p(int(raw_dev)), C.c_void_p(int(out_dev)), count, arr,
+             C.c_void_p(stream) if stream else None)
 
     def reconstruct_host(self, raw):
         """Convenience: host uint16 stack(s) in, host float32 image(s) out."""

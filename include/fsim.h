@@ -278,6 +278,26 @@ int fsim_channel_destroy(fsim_channel ch);
 int fsim_reconstruct(fsim_channel ch, const uint16_t *raw_dev, float *out_dev,
                      int count, void *stream);
 
+/* SIM parameters of one stack (SimParam.Dir per direction): band-1 shift
+ * pX, pY; the phases used for the separation (getPhases: phases + phaOff);
+ * modulations per band (clamped to [0.3, 1.1] as getModulations). */
+typedef struct {
+    double px[5], py[5];
+    double phases[5][7];
+    double modul[5][3];
+} fsim_sim_params;
+
+/* reconstruct_params: fsim_reconstruct with each stack's own SIM parameters
+ *   (sp[count], host memory) -- the reconstruction that follows a per-stack
+ *   estimate (SimAlgorithm.runReconstruction after estimateParameters,
+ *   SimAlgorithm.java:268-709; the live refit, ReconstructionRunner.java:
+ *   642-746).  The channel supplies the OTF, the Wiener constant,
+ *   attenuation, apodization and fade; the Wiener denominator
+ *   (WienerFilter.java:46-122), separation matrices and shift ramps are built
+ *   on the device for every stack.  Asynchronous. */
+int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw_dev, float *out_dev, int count,
+                            const fsim_sim_params *sp, void *stream);
+
 /* ---- request server (continuous batching) ---------------------------------
  * The serving form of livemode/ReconstructionRunner.java: submit() is
  * queueImage() (:140-150) -- it refuses with FSIM_ERR_BUSY when max_queue

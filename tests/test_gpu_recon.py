@@ -144,3 +144,83 @@ def test_unaligned_raw_pointer_matches_aligned(fsim_lib):
     rb.free()
     ob.free()
     assert np.array_equal(got, aligned)
+
+
+def _multi_stacks(n, cases, nphase=3, nband=2):
+    """Stacks with different shifts / phase offsets / modulations; the
+    parameters of each as a SimParam on one OTF."""
+    from fsim import synthetic
+    raws, params, sps = [], [], []
+    for s, (k_frac, ang, pha_off, mod) in enumerate(cases):
+        raw, sp = synthetic.sim_stack(n=n, nphase=nphase, seed=60 + s, nbeads=max(30, 300 * n // 512),
+                                      nfil=max(5, 40 * n // 512), k_frac=k_frac, angle0_deg=ang)
+        prm = synthetic.make_param(sp, n, nphase=nphase, nband=nband, modulation=mod)
+        for d in range(3):
+            prm.dir(d).setPhaOff(pha_off)
+        raws.append(raw)
+        params.append(prm)
+        sps.append(sp)
+    return np.stack(raws), params, sps
+
+
+def test_reconstruct_params_equals_own_channel(fsim_lib):
+    """fsim_reconstruct_params: each stack of a batch with its own shifts,
+    phases and modulations gives the image of a channel built for exactly
+    those parameters (bitwise: same constants, same kernels)."""
+    from fsim.linalg import DeviceBuffer
+    from fsim.sim_algorithm import Reconstructor
+    from fsim._lib import call
+    n = 128
+    raws, params, _ = _multi_stacks(n, [(0.85, 3.0, 0.0, 0.7), (0.78, 11.0, 0.4, 0.55), (0.8, -7.0, -1.1, 0.9)])
+    rec = Reconstructor(params[0])
+    rb = DeviceBuffer(raws.nbytes)
+    rb.upload(raws)
+    ob = DeviceBuffer(len(params) * rec.out_bytes())
+    for _ in range(2):   # second call reuses the per-stack workspace
+        rec.reconstruct_params(rb.value, ob.value, params)
+        call("fsim_sync")
+        got = ob.download((len(params), 2 * n, 2 * n), np.float32)
+        for s, prm in enumerate(params):
+            own = Reconstructor(prm).reconstruct_host(raws[s])
+            assert np.array_equal(got[s], own), f"stack {s}"
+    # a smaller batch after a larger one
+    rec.reconstruct_params(rb.value, ob.value, params[:1])
+    call("fsim_sync")
+    assert np.array_equal(ob.download((1, 2 * n, 2 * n), np.float32)[0], rec.reconstruct_host(raws[0]))
+    rb.free()
+    ob.free()
+
+
+def test_estimate_then_reconstruct_matches_oracle(fsim_lib):
+    """Config 1 flow: parameters estimated per stack (fsim_estimate), then
+    every stack reconstructed with its own estimate (fsim_reconstruct_params);
+    compared with the FP32 oracle reconstructing with the same parameters."""
+    from fsim.linalg import DeviceBuffer
+    from fsim.sim_algorithm import Reconstructor, estimate_batch, params_from_estimate
+    from fsim import synthetic
+    from fsim._lib import call
+    n = 256
+    raws, params, sps = _multi_stacks(n, [(0.85, 3.0, 0.0, 0.7), (0.8, 9.0, 0.0, 0.7)])
+    base = synthetic.make_param(sps[0], n)
+    est = estimate_batch(base, 1, 0.6, images=raws.reshape(len(sps), 3, 3, n, n).astype(np.float32))
+    sim = [params_from_estimate(base, e) for e in est]
+    rec = Reconstructor(base)
+    rb = DeviceBuffer(raws.nbytes)
+    rb.upload(raws)
+    ob = DeviceBuffer(len(sim) * rec.out_bytes())
+    rec.reconstruct_params(rb.value, ob.value, sim)
+    call("fsim_sync")
+    got = ob.download((len(sim), 2 * n, 2 * n), np.float32)
+    for s in range(len(sim)):
+        p = ref.recon_dict(sps[s], n)
+        p["px"] = [sim[s].px[d] for d in range(3)]
+        p["py"] = [sim[s].py[d] for d in range(3)]
+        p["phases"] = [[sim[s].phases[d][i] for i in range(3)] for d in range(3)]
+        p["modul"] = [[sim[s].modul[d][b] for b in range(2)] for d in range(3)]
+        e = fp64.rel_l2(got[s], ref.recon32(p, raws[s], threads=8))
+        k_err = max(abs(sim[s].px[d] - sps[s]["px"][d]) + abs(sim[s].py[d] - sps[s]["py"][d]) for d in range(3))
+        print(f"stack {s}: |k - k_true| {k_err:.3f} px, relL2 gpu-vs-ref32 {e:.3e}")
+        assert k_err < 0.1
+        assert e < TOL
+    rb.free()
+    ob.free()
