@@ -308,7 +308,7 @@ def estimation_batched(nphase, nband, fb, k_frac, h2, batch=64, runs=3):
     import numpy as np
     from fsim import synthetic
     from fsim.linalg import DeviceBuffer
-    from fsim.sim_algorithm import Reconstructor, estimate_batch, params_from_estimate
+    from fsim.sim_algorithm import Reconstructor, estimate_batch, params_from_estimates
     n = 512
     raws, sps = [], []
     for s in range(4):
@@ -326,13 +326,14 @@ def estimation_batched(nphase, nband, fb, k_frac, h2, batch=64, runs=3):
     del img_b
 
     def run(recon):
-        est = estimate_batch(param, fb, 0.6, images_dev=ib.value, count=batch)
+        res = estimate_batch(param, fb, 0.6, images_dev=ib.value, count=batch, raw=True)
         if recon:
-            rec.reconstruct_params(rb.value, ob.value, [params_from_estimate(param, e) for e in est])
+            rec.reconstruct_params(rb.value, ob.value, params_from_estimates(param, res, batch))
         _lib_sync()
-        return est
+        return res
 
-    est = run(True)
+    res0 = run(True)
+    est = [[{"px": res0[s * 3 + d].px, "py": res0[s * 3 + d].py} for d in range(3)] for s in range(4)]
     scale = 1 if nband == 2 else 2
     err_k = max(abs(est[s][d]["px"] - scale * sps[s % 4]["px"][d]) + abs(est[s][d]["py"] - scale * sps[s % 4]["py"][d])
                 for s in range(4) for d in range(3))

@@ -34,6 +34,7 @@
 //      and fit-band component), then the commonRegion copies of Q pairs
 //      (band 0 copies first, then band 1 copies)
 #include "fsim_internal.h"
+#include "fft_core.h"
 #include "filters.h"
 #include "estimate_dev.h"
 
@@ -205,16 +206,64 @@ __global__ void k_eb_fit_init(const EstGroup *__restrict__ grp, EstPair *__restr
     pairs[g] = p;
 }
 
-// Correlation.commonRegion(b0, b1, 0, bn1, otf, kx, ky, 0.15, 0.05, true)
-// on copies of each pair's bands (Correlation.java:60-63, 189-192).
-// Grid (pixels, npairs).
-__global__ void k_eb_region(const float2 *__restrict__ X, float2 *__restrict__ T, EstPair *__restrict__ pairs,
-                            int npairs, int nphase, int n, EbRegion r) {
+// The DC-centred half of commonRegion's per-pixel work, which is the same
+// for every pair and round: for each OTF band, the weight w at (0, 0)
+// (otfToVector / writeOtfVector) reduced to what the pixel needs -- NaN
+// where |w| < 0.05 (the pixel is zeroed), else the reciprocal (rr, ri)
+// that Cplx.div multiplies by (Cplx.java:311-319).  OTF tables are finite
+// (fsim_otf_create checks), so a NaN entry only ever marks the cut.
+__global__ void k_eb_dc_tab(float2 *__restrict__ dct, int n, EbRegion r, int nb) {
+#pragma clang fp contract(off)
     const size_t nn = (size_t)n * n;
-    const int p = blockIdx.y;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn * nb; i += (size_t)gridDim.x * blockDim.x) {
+        const int b = (int)(i / nn);
+        const size_t k = i % nn;
+        const float2 w = otf_pixel(r.tabs + (size_t)b * r.ns, r.ns, r.cpm, r.cutoff, (int)(k % n), (int)(k / n), n, n,
+                                   0.0, 0.0, r.vec_cpm);
+        float2 e = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+        if (!(abs_java(w) < 0.05)) {
+            const float s = w.x * w.x + w.y * w.y;
+            e = make_float2(w.x / s, -w.y / s);
+        }
+        dct[i] = e;
+    }
+}
+
+// region_band0 / region_band1 with the DC weight from the table above.
+__device__ __forceinline__ float2 region_dc(float2 v, float2 dc, float2 wt, double r, double dist, double wlim) {
+#pragma clang fp contract(off)
+    if (dc.x != dc.x || abs_java(wt) < wlim)
+        v = make_float2(0.f, 0.f);
+    else
+        v = make_float2(v.x * dc.x - v.y * dc.y, v.x * dc.y + v.y * dc.x);
+    if (r < dist || r > 1 - dist) v = make_float2(0.f, 0.f);
+    return v;
+}
+
+// Correlation.commonRegion(b0, b1, 0, bn1, otf, kx, ky, 0.15, 0.05, true)
+// on copies of each pair's bands (Correlation.java:60-63, 189-192), fused
+// with the row pass of the inverse 2-D FFT that follows (the same row
+// transform as fft2d's, which runs rows first).  Grid (ceil(n / RW), 2 * npairs):
+// blockIdx.y = 2 * pair + band; band 0 rows go to T[pair], band 1 rows to
+// T[npairs + pair].
+template <int L>
+__global__ void __launch_bounds__(256) k_eb_region_rows(const float2 *__restrict__ X, float2 *__restrict__ T,
+                                                        EstPair *__restrict__ pairs, int npairs, int nphase,
+                                                        EbRegion r, const float2 *__restrict__ dct,
+                                                        const float2 *__restrict__ tw) {
+    constexpr int E = FftE<L>::value, Tn = L / E;
+    constexpr int RW = Tn >= 256 ? 1 : 256 / Tn;
+    constexpr int LP = FftLds<L, E>::len;
+    __shared__ float2 lds[RW * LP];
+    using TW = TwLds<L, false>;
+    constexpr size_t nn = (size_t)L * L;
+    const int t = threadIdx.x % Tn, rr = threadIdx.x / Tn;
+    const int row = blockIdx.x * RW + rr;
+    const bool active = row < L;   // small L: a block holds more row slots than rows
+    const int p = blockIdx.y >> 1, band = blockIdx.y & 1;
     const double kx = pairs[p].kx, ky = pairs[p].ky;
     const int grp = pairs[p].grp, slot1 = pairs[p].slot1, bn1 = pairs[p].bn1;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && (fabs(kx) >= n || fabs(ky) >= n)) pairs[p].err = 1;
+    if (blockIdx.x == 0 && band == 0 && threadIdx.x == 0 && (fabs(kx) >= L || fabs(ky) >= L)) pairs[p].err = 1;
     RegionArgs a;
     a.tab0 = r.tabs;
     a.tab1 = r.tabs + (size_t)(r.otf_nband == 1 ? 0 : bn1) * r.ns;
@@ -230,12 +279,57 @@ __global__ void k_eb_region(const float2 *__restrict__ X, float2 *__restrict__ T
     a.iky = (int)ky;
     a.divide = 1;
     const double kmax = sqrt(kx * kx + ky * ky);
-    const float2 *b0 = X + (size_t)grp * nphase * nn, *b1 = X + ((size_t)grp * nphase + slot1) * nn;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x) {
-        float2 v0 = b0[i], v1 = b1[i];
-        region_pixel(a, (int)(i % n), (int)(i / n), n, n, kmax, v0, v1);
-        T[(size_t)p * nn + i] = v0;
-        T[((size_t)npairs + p) * nn + i] = v1;
+    float2 v[E];
+    if (active) {
+        const float2 *src = X + ((size_t)grp * nphase + (band ? slot1 : 0)) * nn + (size_t)row * L;
+        const float2 *dc = dct + (size_t)(band && r.otf_nband > 1 ? bn1 : 0) * nn + (size_t)row * L;
+        const float2 *tab = band ? a.tab1 : a.tab0;
+        const double sx = band ? -kx : kx, sy = band ? -ky : ky;
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int x = t + Tn * m;
+            const float2 wt = otf_pixel(tab, a.ns, a.cpm, a.cutoff, x, row, L, L, sx, sy, a.vec_cpm);
+            // ring: band 0 at its own pixel, band 1 at its source pixel
+            // ((x + ikx) mod n, (y - iky) mod n)
+            const int xq = band ? ((x + a.ikx) % L + L) % L : x, yq = band ? ((row - a.iky) % L + L) % L : row;
+            v[m] = region_dc(src[x], dc[x], wt, dc_rad(xq, yq, L, L) / kmax, a.dist, a.wlim);
+        }
+    }
+    fft_regs<L, E, true, RowWave<L>::value, TW::tl>(v, lds + rr * LP, 1, t, tw, active);
+    if (active) {
+        float2 *dst = T + (size_t)(band ? npairs + p : p) * nn + (size_t)row * L;
+        constexpr float scale = 1.0f / L;
+#pragma unroll
+        for (int m = 0; m < E; m++) dst[t + Tn * m] = cscale(v[m], scale);
+    }
+}
+
+template <int L>
+static int launch_region_rows_t(const EbRegion &r, const float2 *dct, EstPair *pairs, int npairs, int nphase,
+                                float2 *X, float2 *W, hipStream_t s) {
+    constexpr int E = FftE<L>::value, Tn = L / E;
+    constexpr int RW = Tn >= 256 ? 1 : 256 / Tn;
+    const float2 *tw = twiddles(L);
+    if (!tw) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
+    hipLaunchKernelGGL(k_eb_region_rows<L>, dim3(cdiv(L, RW), 2 * npairs), dim3(Tn * RW), 0, s, X, W, pairs,
+                       npairs, nphase, r, dct, tw);
+    FSIM_HIP(hipGetLastError());
+    return FSIM_OK;
+}
+
+static int launch_region_rows(int n, const EbRegion &r, const float2 *dct, EstPair *pairs, int npairs, int nphase,
+                              float2 *X, float2 *W, hipStream_t s) {
+    switch (n) {
+        case 16: return launch_region_rows_t<16>(r, dct, pairs, npairs, nphase, X, W, s);
+        case 32: return launch_region_rows_t<32>(r, dct, pairs, npairs, nphase, X, W, s);
+        case 64: return launch_region_rows_t<64>(r, dct, pairs, npairs, nphase, X, W, s);
+        case 128: return launch_region_rows_t<128>(r, dct, pairs, npairs, nphase, X, W, s);
+        case 256: return launch_region_rows_t<256>(r, dct, pairs, npairs, nphase, X, W, s);
+        case 512: return launch_region_rows_t<512>(r, dct, pairs, npairs, nphase, X, W, s);
+        case 1024: return launch_region_rows_t<1024>(r, dct, pairs, npairs, nphase, X, W, s);
+        case 2048: return launch_region_rows_t<2048>(r, dct, pairs, npairs, nphase, X, W, s);
+        case 4096: return launch_region_rows_t<4096>(r, dct, pairs, npairs, nphase, X, W, s);
+        default: return fail(FSIM_ERR_ARG, "estimate: unsupported size");
     }
 }
 
@@ -551,14 +645,10 @@ static void grid_stride_launch(size_t n, unsigned &blocks) {
 // One commonRegion / inverse FFT / correlation round for npairs pairs; with
 // `grid`, the 10 x 10 candidates of fitPeak through the approximate filter
 // (tabs: 20 n phasors per pair, apart: partial sums of n / rows blocks).
-static int eb_round(const EbRegion &r, EstPair *pairs, int npairs, bool grid, int nphase, int n, int nchunk,
-                    double2 *part, float2 *tabs, double2 *apart, hipStream_t s) {
-    const size_t nn = (size_t)n * n;
-    const unsigned pb = cdiv(nn, 256);
-    hipLaunchKernelGGL(k_eb_region, dim3(pb > 1024 ? 1024 : pb, npairs), dim3(256), 0, s, g_eb.X, g_eb.W, pairs,
-                       npairs, nphase, n, r);
-    FSIM_HIP(hipGetLastError());
-    int rc = fft2d_batch_device(g_eb.W, n, n, 2 * npairs, 1, s);
+static int eb_round(const EbRegion &r, const float2 *dct, EstPair *pairs, int npairs, bool grid, int nphase, int n,
+                    int nchunk, double2 *part, float2 *tabs, double2 *apart, hipStream_t s) {
+    int rc = launch_region_rows(n, r, dct, pairs, npairs, nphase, g_eb.X, g_eb.W, s);
+    if (!rc) rc = fft_cols_batch_device(g_eb.W, n, n, 2 * npairs, 1, s);
     if (rc) return rc;
     if (grid) {
         const int rows = n < 32 ? n : 32, nblk = n / rows;
@@ -636,7 +726,9 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     const int nblk = n / (n < 32 ? n : 32);
     const size_t o_tabs = align256(o_err + G * sizeof(int));
     const size_t o_apart = align256(o_tabs + (size_t)G * 20 * n * sizeof(float2));
-    const size_t small = align256(o_apart + (size_t)G * kPairSlots * nblk * sizeof(double2));
+    const int ndc = o.nband == 1 ? 1 : (o.nband < 3 ? o.nband : 3);
+    const size_t o_dct = align256(o_apart + (size_t)G * kPairSlots * nblk * sizeof(double2));
+    const size_t small = align256(o_dct + (size_t)ndc * nn * sizeof(float2));
     if (!rc) rc = eb_grow(&g_eb.small, g_eb.small_bytes, small);
     if (rc) return rc;
     char *sb = reinterpret_cast<char *>(g_eb.small);
@@ -651,6 +743,7 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     int *errd = reinterpret_cast<int *>(sb + o_err);
     float2 *tabs = reinterpret_cast<float2 *>(sb + o_tabs);
     double2 *apart = reinterpret_cast<double2 *>(sb + o_apart);
+    float2 *dct = reinterpret_cast<float2 *>(sb + o_dct);
 
     hipStream_t s = stream();
     g_eb.fade_h.resize(2 * (size_t)n);
@@ -658,6 +751,8 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     FSIM_HIP(hipMemcpyAsync(fade, g_eb.fade_h.data(), 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
     const unsigned gb = cdiv(G, 64);
     hipLaunchKernelGGL(k_eb_init, dim3(gb), dim3(64), 0, s, grp, G);
+    hipLaunchKernelGGL(k_eb_dc_tab, dim3((unsigned)std::min<size_t>((ndc * nn + 255) / 256, 4096)), dim3(256), 0, s,
+                       dct, n, reg, ndc);
     unsigned blocks;
     const size_t nquad = (size_t)G * P * nn / 4;
     grid_stride_launch(nquad, blocks);
@@ -683,13 +778,13 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     FSIM_HIP(hipGetLastError());
     // fitPeak: three rounds of commonRegion, inverse FFTs, candidate filter, argmax
     for (int iter = 0; iter < 3; iter++) {
-        rc = eb_round(reg, fit, G, true, P, n, nchunk, part, tabs, apart, s);
+        rc = eb_round(reg, dct, fit, G, true, P, n, nchunk, part, tabs, apart, s);
         if (rc) return rc;
     }
     // getPeak of band 1 (and band 2)
     hipLaunchKernelGGL(k_eb_gp_init, dim3(gb), dim3(64), 0, s, fit, gp, G, B, p->fit_band, shift);
     FSIM_HIP(hipGetLastError());
-    rc = eb_round(reg, gp, npk * G, false, P, n, nchunk, part, tabs, apart, s);
+    rc = eb_round(reg, dct, gp, npk * G, false, P, n, nchunk, part, tabs, apart, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_eb_finish, dim3(gb), dim3(64), 0, s, fit, gp, G, npk, peaks, shift, resd, errd);
     FSIM_HIP(hipGetLastError());
@@ -700,5 +795,25 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     for (int g = 0; g < G; g++)
         if (err[g]) return fail(FSIM_ERR_ARG, "estimate: commonRegion shift larger than the vector (stack " +
                                                   std::to_string(g / D) + ", direction " + std::to_string(g % D) + ")");
+    return FSIM_OK;
+}
+
+extern "C" int fsim_params_from_estimate(const fsim_estimate_result *res, int count, int ndir, int nphase, int nband,
+                                         const fsim_sim_params *base, fsim_sim_params *out) {
+    if (!res || !base || !out || count < 0) return fail(FSIM_ERR_ARG, "params_from_estimate: null argument");
+    if (ndir < 1 || ndir > 5 || nphase < 1 || nphase > 7 || (nband != 2 && nband != 3))
+        return fail(FSIM_ERR_ARG, "params_from_estimate: 1-5 directions, 1-7 phases, 2 or 3 bands");
+    for (int s = 0; s < count; s++) {
+        fsim_sim_params p = *base;
+        for (int d = 0; d < ndir; d++) {
+            const fsim_estimate_result &r = res[(size_t)s * ndir + d];
+            p.px[d] = r.px / (nband - 1);
+            p.py[d] = r.py / (nband - 1);
+            for (int ph = 0; ph < nphase; ph++) p.phases[d][ph] = base->phases[d][ph] + r.pha_off;
+            p.modul[d][1] = r.mod1;
+            if (nband == 3) p.modul[d][2] = r.mod2;
+        }
+        out[s] = p;
+    }
     return FSIM_OK;
 }

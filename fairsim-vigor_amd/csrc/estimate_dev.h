@@ -46,25 +46,37 @@ struct RegionArgs {
 // source pixel ((x + ikx) mod w, (y - iky) mod h) lies in the ring.  The
 // reference's loop order decides only the sign of such zeros.
 // kmax = |(kx, ky)|.
-__device__ __forceinline__ void region_pixel(const RegionArgs &a, int x, int y, int w, int h, double kmax,
-                                             float2 &v0, float2 &v1) {
+__device__ __forceinline__ float2 region_band0(const RegionArgs &a, int x, int y, int w, int h, double kmax,
+                                               float2 v0) {
     const float2 w0 = otf_pixel(a.tab0, a.ns, a.cpm, a.cutoff, x, y, w, h, 0.0, 0.0, a.vec_cpm);
     const float2 wt0 = otf_pixel(a.tab0, a.ns, a.cpm, a.cutoff, x, y, w, h, a.kx, a.ky, a.vec_cpm);
-    const float2 w1 = otf_pixel(a.tab1, a.ns, a.cpm, a.cutoff, x, y, w, h, 0.0, 0.0, a.vec_cpm);
-    const float2 wt1 = otf_pixel(a.tab1, a.ns, a.cpm, a.cutoff, x, y, w, h, -a.kx, -a.ky, a.vec_cpm);
     if (abs_java(w0) < a.wlim || abs_java(wt0) < a.wlim)
         v0 = make_float2(0.f, 0.f);
     else if (a.divide)
         v0 = div_java(v0, w0);
+    const double r0 = dc_rad(x, y, w, h) / kmax;
+    if (r0 < a.dist || r0 > 1 - a.dist) v0 = make_float2(0.f, 0.f);
+    return v0;
+}
+
+__device__ __forceinline__ float2 region_band1(const RegionArgs &a, int x, int y, int w, int h, double kmax,
+                                               float2 v1) {
+    const float2 w1 = otf_pixel(a.tab1, a.ns, a.cpm, a.cutoff, x, y, w, h, 0.0, 0.0, a.vec_cpm);
+    const float2 wt1 = otf_pixel(a.tab1, a.ns, a.cpm, a.cutoff, x, y, w, h, -a.kx, -a.ky, a.vec_cpm);
     if (abs_java(w1) < a.wlim || abs_java(wt1) < a.wlim)
         v1 = make_float2(0.f, 0.f);
     else if (a.divide)
         v1 = div_java(v1, w1);
-    const double r0 = dc_rad(x, y, w, h) / kmax;
-    if (r0 < a.dist || r0 > 1 - a.dist) v0 = make_float2(0.f, 0.f);
     const int xq = ((x + a.ikx) % w + w) % w, yq = ((y - a.iky) % h + h) % h;
     const double rq = dc_rad(xq, yq, w, h) / kmax;
     if (rq < a.dist || rq > 1 - a.dist) v1 = make_float2(0.f, 0.f);
+    return v1;
+}
+
+__device__ __forceinline__ void region_pixel(const RegionArgs &a, int x, int y, int w, int h, double kmax,
+                                             float2 &v0, float2 &v1) {
+    v0 = region_band0(a, x, y, w, h, kmax, v0);
+    v1 = region_band1(a, x, y, w, h, kmax, v1);
 }
 
 // Candidate correlations: b1 shifted by (xpos, -ypos) (Vec2d.Cplx.fourierShift,

@@ -344,6 +344,10 @@ int fsim_otf_create(int nband, int ns, double cutoff, double cpm, const float *v
     if (rc) return rc;
     if (nband < 1 || ns < 2 || !vals || !vals_att || !only_att || !out || cpm <= 0)
         return fail(FSIM_ERR_ARG, "otf_create: bad arguments");
+    for (size_t i = 0; i < (size_t)nband * ns; i++)
+        if (!std::isfinite(vals[2 * i]) || !std::isfinite(vals[2 * i + 1]) || !std::isfinite(vals_att[2 * i]) ||
+            !std::isfinite(vals_att[2 * i + 1]) || !std::isfinite(only_att[i]))
+            return fail(FSIM_ERR_ARG, "otf_create: table values must be finite");
     fsim_otf_s *h = new fsim_otf_s();
     Otf &o = h->o;
     o.nband = nband;

@@ -41,6 +41,9 @@ bool fft_size_ok(int L);
 int fft2d_device(float2 *data, int w, int h, int inverse, hipStream_t s);
 // nimg images of w x h, w*h elements apart, transformed in the same launches
 int fft2d_batch_device(float2 *data, int w, int h, int nimg, int inverse, hipStream_t s);
+// the column pass alone (the second half of fft2d_batch_device, whose row
+// pass runs first): for callers that fuse their own row pass
+int fft_cols_batch_device(float2 *data, int w, int h, int nimg, int inverse, hipStream_t s);
 
 // small helpers
 inline bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }

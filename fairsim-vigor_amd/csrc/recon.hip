@@ -634,8 +634,37 @@ __global__ void k_wiener_multi(float2 *__restrict__ wa, const float2 *__restrict
     const int st = blockIdx.y;
     const ShiftSet S = sh[st];
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < mm; i += (size_t)gridDim.x * blockDim.x) {
-        const float acc =
-            wiener_sum((int)(i % M), (int)(i / M), M, M, vals, only_att, ns, otf_nb, cpm, cutoff, vec_cpm, S, use_att);
+        const int x = (int)(i % M), y = (int)(i / M);
+        const double xh = xh_of(x, M), yh = yh_of(y, M);
+        // band 0 (px(0) = 0): both distances are |(xh, yh)| in every
+        // direction, so one value serves all of them (the same operands as
+        // wiener_sum's, hence the same float)
+        const double r0 = sqrt(xh * xh + yh * yh) * vec_cpm;
+        float v0 = 0.f;
+        {
+            const float2 o = otf_val(vals, ns, cpm, cutoff, r0);
+            v0 = o.x * o.x + o.y * o.y;
+            if (use_att) v0 *= att_val(only_att, cpm, cutoff, r0);
+        }
+        float acc = 0.f;
+        for (int d = 0; d < S.ndir; d++) {
+            acc = acc + v0 + v0;
+            for (int b = 1; b < S.nband; b++) {
+                const double bx = S.px[d] * b, by = S.py[d] * b;
+                const double r1 = sqrt((xh - bx) * (xh - bx) + (yh - by) * (yh - by)) * vec_cpm;
+                const double r2 = sqrt((xh + bx) * (xh + bx) + (yh + by) * (yh + by)) * vec_cpm;
+                const size_t tb = (size_t)(otf_nb > 1 ? b : 0) * ns;
+                const float2 o1 = otf_val(vals + tb, ns, cpm, cutoff, r1);
+                const float2 o2 = otf_val(vals + tb, ns, cpm, cutoff, r2);
+                float v1 = o1.x * o1.x + o1.y * o1.y;
+                float v2 = o2.x * o2.x + o2.y * o2.y;
+                if (use_att) {
+                    v1 *= att_val(only_att + tb, cpm, cutoff, r1);
+                    v2 *= att_val(only_att + tb, cpm, cutoff, r2);
+                }
+                acc = acc + v1 + v2;
+            }
+        }
         wa[(size_t)st * mm + i] = make_float2(1 / (acc + w2), wa0[i].y);
     }
 }

@@ -144,6 +144,8 @@ _SIGS = {
     "fsim_fit_peak": [_P, _P, _I, _I, _P, _D, _D, _D, _D, _D, C.POINTER(C.c_float), _PD],
     "fsim_get_peak": [_P, _P, _I, _I, _P, _D, _D, _D, _D, _PD],
     "fsim_estimate": [_P, _I, C.POINTER(ESTIMATE_PARAMS), _P, C.POINTER(ESTIMATE_RESULT)],
+    "fsim_params_from_estimate": [C.POINTER(ESTIMATE_RESULT), _I, _I, _I, _I, C.POINTER(SIM_PARAMS),
+                                  C.POINTER(SIM_PARAMS)],
     "fsim_channel_create": [C.POINTER(RECON_PARAMS), _P, C.POINTER(_P)],
     "fsim_channel_destroy": [_P],
     "fsim_reconstruct": [_P, _P, _P, _I, _P],

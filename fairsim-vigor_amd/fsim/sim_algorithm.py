@@ -367,14 +367,15 @@ def estimate_parameters(param, inFFT, fitBand, fitExclude):
     return results
 
 
-def estimate_batch(param, fitBand, fitExclude, images=None, images_dev=None, count=None, fade_px=10):
+def estimate_batch(param, fitBand, fitExclude, images=None, images_dev=None, count=None, fade_px=10, raw=False):
     """fsim_estimate: SimAlgorithm.estimateParameters (SimAlgorithm.java:37-256)
     for a batch of stacks in one native call, with the GUI's input preparation
     (input_ffts: fadeBorderCos, FFT) on the device.  ``images``: host float32
     [count, D, P, N, N]; or ``images_dev``: the device address of such a block
     plus ``count``.  Returns, per stack, the per-direction result dicts of
     estimate_parameters (plus the coarse ``peak``); ``param`` is not changed
-    (apply_estimate stores one stack's results in it)."""
+    (apply_estimate stores one stack's results in it); ``raw``: return the
+    fsim_estimate_result array instead (for params_from_estimates)."""
     _lib.init()
     D, P = param.nrDir(), param.dir(0).nrPha()
     n = param.vectorWidth()
@@ -398,6 +399,8 @@ def estimate_batch(param, fitBand, fitExclude, images=None, images_dev=None, cou
     finally:
         if buf is not None:
             buf.free()
+    if raw:
+        return res
     out = []
     for s in range(count):
         dirs = []
@@ -408,6 +411,20 @@ def estimate_batch(param, fitBand, fitExclude, images=None, images_dev=None, cou
                 e["mod2"] = r.mod2
             dirs.append(e)
         out.append(dirs)
+    return out
+
+
+def params_from_estimates(param, res, count):
+    """fsim_params_from_estimate: per-stack fsim_sim_params (an array of
+    ``count``) from estimate_batch(..., raw=True) results, on top of
+    ``param``'s phases (without offset) and modulations."""
+    base = sim_params(param)
+    for d in range(param.nrDir()):
+        for i, ph in enumerate(param.dir(d).phases):
+            base.phases[d][i] = ph
+    out = (_lib.SIM_PARAMS * count)()
+    call("fsim_params_from_estimate", res, int(count), param.nrDir(), param.dir(0).nrPha(), param.nrBand(),
+         C.byref(base), out)
     return out
 
 
@@ -619,11 +636,15 @@ class Reconstructor:
 
     def reconstruct_params(self, raw_dev, out_dev, params, stream=None):
         """fsim_reconstruct_params: len(params) stacks, stack i with the SIM
-        parameters params[i] (a SimParam, or an fsim_sim_params)."""
+        parameters params[i] (a SimParam, or an fsim_sim_params; or a ctypes
+        array of fsim_sim_params)."""
         count = len(params)
-        arr = (_lib.SIM_PARAMS * count)()
-        for i, prm in enumerate(params):
-            arr[i] = prm if isinstance(prm, _lib.SIM_PARAMS) else sim_params(prm)
+        if isinstance(params, C.Array):
+            arr = params
+        else:
+            arr = (_lib.SIM_PARAMS * count)()
+            for i, prm in enumerate(params):
+                arr[i] = prm if isinstance(prm, _lib.SIM_PARAMS) else sim_params(prm)
         call("fsim_reconstruct_params", self._h, C.c_void_p(int(raw_dev)), C.c_void_p(int(out_dev)), count, arr,
              C.c_void_p(stream) if stream else None)
 

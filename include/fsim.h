@@ -133,7 +133,8 @@ int fsim_vec_set(fsim_vec v, int x, int y, float re, float im);
  *   (OtfProvider.java:96-127, 276-287) computed on the host in double,
  *   writing nband=1 tables into caller arrays (vals/vals_att: 2*n floats,
  *   only_att: n floats) and cutoff/cpm.
- * fsim_otf_create: upload such tables (nband * nsamples). */
+ * fsim_otf_create: upload such tables (nband * nsamples); every value must be
+ *   finite (FSIM_ERR_ARG otherwise). */
 int fsim_otf_from_estimate(double na, double lambda_nm, double a, int nsamples,
                            double att_str, double att_fwhm, float *vals,
                            float *vals_att, float *only_att, double *cutoff,
@@ -297,6 +298,17 @@ typedef struct {
  *   on the device for every stack.  Asynchronous. */
 int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw_dev, float *out_dev, int count,
                             const fsim_sim_params *sp, void *stream);
+
+/* params_from_estimate: what estimateParameters leaves in the SimParam
+ *   (SimAlgorithm.java:136-142, 155-158) as per-stack reconstruction
+ *   parameters: for stack s and direction d, pX, pY = (px, py) / (nband - 1)
+ *   (SimParam.Dir.setPxPy, SimParam.java:536-541), phases = base phases +
+ *   pha_off (setPhaOff; getPhases, :472-477), modulation 1 (and 2) from the
+ *   estimate, the others from base.  res: count x ndir results of
+ *   fsim_estimate; base: the parameters without phase offset; out: count
+ *   entries (host memory). */
+int fsim_params_from_estimate(const fsim_estimate_result *res, int count, int ndir, int nphase, int nband,
+                              const fsim_sim_params *base, fsim_sim_params *out);
 
 /* ---- request server (continuous batching) ---------------------------------
  * The serving form of livemode/ReconstructionRunner.java: submit() is

@@ -135,3 +135,37 @@ def test_synthetic_is_seeded_and_shaped():
     assert np.array_equal(a, b) and not np.array_equal(a, c)
     assert 100 <= a.min() and a.max() < 2000                  # offset + background + peak ~1000
     assert spa["k0"] == pytest.approx(0.85 * spa["cutoff_px"])
+
+
+@pytest.mark.parametrize("nband", [2, 3])
+def test_params_from_estimate_matches_simparam(nband):
+    """fsim_params_from_estimate leaves the parameters SimParam.Dir holds
+    after estimateParameters' setPxPy / setPhaOff / setModulation
+    (SimAlgorithm.java:139-142, 155-158), as the host mirror computes them."""
+    from fsim import _lib
+    from fsim.sim_algorithm import OtfProvider, SimParam, apply_estimate, params_from_estimates, sim_params
+    P, D, count = 5, 3, 3
+    otf = OtfProvider.fromEstimate(1.4, 525.0, 1.0)
+    param = SimParam.create(nband, D, P, 256, 0.08, otf)
+    param.dir(1).setModulation(1, 0.8)
+    rng = np.random.default_rng(4)
+    res = (_lib.ESTIMATE_RESULT * (count * D))()
+    dicts = []
+    for s in range(count):
+        dirs = []
+        for d in range(D):
+            r = res[s * D + d]
+            r.px, r.py = rng.uniform(-120, 120, 2)
+            r.pha_off, r.mod1, r.mod2 = rng.uniform(-3, 3), rng.uniform(0.2, 1.2), rng.uniform(0.1, 0.9)
+            e = dict(px=r.px, py=r.py, pha_off=r.pha_off, mod1=r.mod1)
+            if nband == 3:
+                e["mod2"] = r.mod2
+            dirs.append(e)
+        dicts.append(dirs)
+    got = params_from_estimates(param, res, count)
+    for s in range(count):
+        want_param = SimParam.create(nband, D, P, 256, 0.08, otf)
+        want_param.dir(1).setModulation(1, 0.8)
+        apply_estimate(want_param, dicts[s])
+        want = sim_params(want_param)
+        assert bytes(got[s]) == bytes(want), f"stack {s}"
