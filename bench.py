@@ -353,6 +353,91 @@ def estimation_batched(nphase, nband, fb, k_frac, h2, batch=64, runs=3):
     return res
 
 
+def stream_ingest(n=512, per_channel=48, runs=2):
+    """Live stream front end (fsim_ingest): camera packets of two channels
+    (128-byte header + 512^2 uint16 pixels, a sync gap before every stack,
+    interleaved) pushed from host memory.  Reported: the host-side assembly
+    rate alone (frames/s, sequences popped), and the end-to-end rate with the
+    request server attached (pinned staging, PCIe included), with a bitwise
+    check of one image against the channel's own reconstruction."""
+    import numpy as np
+    from fsim import synthetic
+    from fsim.livemode import ReconstructionServer, SimSequenceExtractor
+    from fsim.sim_algorithm import Reconstructor
+    from oracle import ingest_ref as ir
+    recs, packets, first = [], [], []
+    for c, lam in enumerate((525.0, 600.0)):
+        raws = []
+        for k in range(4):
+            raw, sp = synthetic.sim_stack(n=n, seed=200 + 10 * c + k, lam=lam)
+            raws.append(raw)
+        recs.append(Reconstructor(synthetic.make_param(sp, n, lam=lam)))
+        first.append(raws[0])
+        seq, t, pk = 1000 * (c + 1), 0, []
+        for k in range(per_channel):
+            t += 30000
+            for fr in raws[k % 4]:
+                pk.append(ir.write_header(seq, n, n, 2, pos1=c, t_cam=t) + fr.tobytes())
+                seq += 1
+                t += 1000
+        packets.append(pk)
+    stream = [np.frombuffer(p, np.uint8) for pair in zip(*packets) for p in pair]
+    nseq = 2 * per_channel
+
+    def assemble_only():
+        ing = SimSequenceExtractor([0, 1], sync_mode=0, sync_delay=20000, max_size=n)
+        a = time.perf_counter()
+        popped = 0
+        for i, p in enumerate(stream):
+            ing.push(p)
+            if i % 18 == 17:   # a consumer taking sequences as they complete
+                popped += sum(1 for c in (0, 1) for _ in iter(lambda: ing.pop(c), None))
+        popped += sum(1 for c in (0, 1) for _ in iter(lambda: ing.pop(c), None))
+        el = time.perf_counter() - a
+        ing.close()
+        return el, popped
+
+    def end_to_end():
+        srv = ReconstructionServer(mem_budget=8 << 30, max_batch=16, max_queue=256, nslots=4)
+        ids = [srv.add_channel(r) for r in recs]
+        ing = SimSequenceExtractor([0, 1], sync_mode=0, sync_delay=20000, max_size=n)
+        ing.attach(srv, ids, n, max_inflight=32)
+        a = time.perf_counter()
+        got = None
+        taken = 0
+        for i, p in enumerate(stream):
+            ing.push(p)
+            if i % 64 == 63:
+                while True:   # drain what is ready without blocking the stream
+                    try:
+                        img, info = ing.take(timeout_ms=0)
+                    except Exception:
+                        break
+                    taken += 1
+                    if info.channel == 0 and info.first_seq_nr == 1000:
+                        got = img
+        while taken < nseq:
+            img, info = ing.take(timeout_ms=60000)
+            taken += 1
+            if info.channel == 0 and info.first_seq_nr == 1000:
+                got = img
+        el = time.perf_counter() - a
+        ing.close()
+        srv.close()
+        return el, got
+
+    assemble_only()
+    ta = min(assemble_only()[0] for _ in range(runs))
+    end_to_end()
+    te, img = min((end_to_end() for _ in range(runs)), key=lambda r: r[0])
+    exact = bool(img is not None and np.array_equal(img, recs[0].reconstruct_host(first[0])))
+    return {"workload": f"2 channels x {per_channel} stacks of 9 x {n}^2 uint16 packets, interleaved",
+            "assembly_frames_per_s": round(len(stream) / ta, 1),
+            "assembly_sequences_per_s": round(nseq / ta, 1),
+            "end_to_end_recons_per_s": round(nseq / te, 1),
+            "pcie_inclusive": True, "bitwise_equal_to_direct": exact}
+
+
 def _lib_sync():
     from fsim import _lib
     _lib.call("fsim_sync")
@@ -377,6 +462,8 @@ def main():
                     help="skip the estimation, mixed-batch and 2048^2 legs")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed batched loop (for rocprofv3 runs)")
+    ap.add_argument("--stream-only", action="store_true",
+                    help="only the live stream ingest leg")
     ap.add_argument("--estimation-only", action="store_true",
                     help="only the batched estimation legs of config 1 (for rocprofv3 runs)")
     ap.add_argument("--probe-copy", action="store_true",
@@ -396,6 +483,9 @@ def main():
     _lib.init(0)
     if args.probe_copy:
         probe_copy(args.steps)
+        return
+    if args.stream_only:
+        print(json.dumps({"stream_only": True, "stream_ingest_512": stream_ingest()}), flush=True)
         return
     if args.estimation_only:
         out = {}
@@ -532,6 +622,8 @@ def main():
         configs["batched_mixed_96"] = batched_mixed()
         log("config 4: 2048^2 large field of view")
         configs["large_fov_2048"] = large_fov()
+        log("live stream ingest: packets -> sequences -> server")
+        configs["stream_ingest_512"] = stream_ingest()
 
     # CPU baseline: the oracle restatement on the host cores, bounded sample
     cpu = None

@@ -72,6 +72,25 @@ class ESTIMATE_RESULT(C.Structure):
                 ("mod2", C.c_double), ("peak_x", C.c_double), ("peak_y", C.c_double)]
 
 
+class INGEST_CONFIG(C.Structure):
+    """fsim_ingest_config (include/fsim.h)."""
+    _fields_ = [("ndir", C.c_int), ("nphase", C.c_int), ("sync_mode", C.c_int), ("sync_delay", C.c_int64),
+                ("sync_avr", C.c_int), ("seq_count", C.c_int), ("sort_capacity", C.c_int),
+                ("queue_capacity", C.c_int), ("max_width", C.c_int), ("max_height", C.c_int)]
+
+
+class INGEST_SEQ_INFO(C.Structure):
+    """fsim_ingest_seq_info (include/fsim.h)."""
+    _fields_ = [("channel", C.c_int), ("width", C.c_int), ("height", C.c_int), ("first_seq_nr", C.c_int64),
+                ("time_camera", C.c_int64), ("pos2", C.c_int32)]
+
+
+class INGEST_STATS(C.Structure):
+    """fsim_ingest_stats (include/fsim.h)."""
+    _fields_ = [("frames", C.c_int64), ("sequences", C.c_int64), ("missed_sequences", C.c_int64),
+                ("dropped_frames", C.c_int64), ("sync_frames", C.c_int64), ("no_sync_windows", C.c_int64)]
+
+
 class SERVER_CONFIG(C.Structure):
     """fsim_server_config (include/fsim.h)."""
     _fields_ = [("mem_budget", C.c_int64), ("max_batch", C.c_int), ("max_queue", C.c_int),
@@ -156,6 +175,15 @@ _SIGS = {
     "fsim_server_submit": [_P, _I, _P, _P, _I, C.POINTER(C.c_uint64)],
     "fsim_server_wait": [_P, C.c_uint64, _I, _PD],
     "fsim_server_get_stats": [_P, C.POINTER(SERVER_STATS), _I],
+    "fsim_wire_parse_header": [_P, _I, _I, _P],
+    "fsim_wire_write_header": [_P, _P],
+    "fsim_ingest_create": [_P, C.POINTER(_I), _I, C.POINTER(_P)],
+    "fsim_ingest_destroy": [_P],
+    "fsim_ingest_push": [_P, _P, C.c_int64],
+    "fsim_ingest_pop": [_P, _I, _P, C.c_int64, _P],
+    "fsim_ingest_get_stats": [_P, _I, _P],
+    "fsim_ingest_attach_server": [_P, _P, C.POINTER(_I), _I, _I],
+    "fsim_ingest_result": [_P, _I, _P, _P],
     "fsim_stage_timing": [_I],
     "fsim_stage_times": [_PD, _PI, _I],
 }

@@ -7,13 +7,20 @@
 a result (the ``finalRecon`` queue, :57-58, 564).  Channels are fused
 ``Reconstructor`` objects whose filters stay resident on the device, as the
 per-channel vectors of ``ReconstructionThread`` do (:380-413).
+
+``SimSequenceExtractor`` is the live stream's front end (fsim_ingest_*):
+camera packets in the transport/ImageWrapper.java wire format are sorted per
+channel and cut into SIM sequences at sync frames as
+livemode/SimSequenceExtractor.java does, and can feed a
+``ReconstructionServer`` directly.
 """
 import ctypes as C
 
 import numpy as np
 
 from . import _lib
-from ._lib import FsimBusy, SERVER_CONFIG, SERVER_STATS, call, lib
+from ._lib import (FsimBusy, INGEST_CONFIG, INGEST_SEQ_INFO, INGEST_STATS, SERVER_CONFIG, SERVER_STATS, call,
+                   lib)
 
 
 class ReconstructionServer:
@@ -87,3 +94,81 @@ def pinned(shape, dtype):
     buf = (C.c_char * nbytes).from_address(p.value)
     arr = np.frombuffer(buf, dtype=dtype).reshape(shape)
     return arr, p
+
+
+class SimSequenceExtractor:
+    """Per-channel sequence assembly of camera packets (fsim_ingest):
+    ``push`` one packet (128-byte header + pixels), ``pop`` an assembled
+    sequence, or ``attach`` a ReconstructionServer and ``take`` the
+    reconstructions in submission order.  The configuration names follow the
+    reference's Conf entries (SyncMode, SyncFrameDelay, SyncFrameAvr,
+    SyncFrameFreq; SimSequenceExtractor.java:59-90)."""
+
+    def __init__(self, channel_numbers, ndir=3, nphase=3, sync_mode=0, sync_delay=12000, sync_avr=3000,
+                 seq_count=1, sort_capacity=20, queue_capacity=None, max_size=2048):
+        raws = ndir * nphase
+        cfg = INGEST_CONFIG(ndir, nphase, int(sync_mode), int(sync_delay), int(sync_avr), int(seq_count),
+                            int(sort_capacity), int(queue_capacity or raws * raws * 10), int(max_size),
+                            int(max_size))
+        self.raws = raws
+        self.nch = len(channel_numbers)
+        nums = (C.c_int * self.nch)(*channel_numbers)
+        self._h = C.c_void_p()
+        call("fsim_ingest_create", C.byref(cfg), nums, self.nch, C.byref(self._h))
+        self._n = None
+        self._server = None
+        self._max = int(max_size)
+        self._popbuf = None
+
+    def close(self):
+        if self._h is not None and self._h.value and _lib._lib is not None:
+            lib().fsim_ingest_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def push(self, packet):
+        """One packet as bytes / bytearray / uint8 array."""
+        buf = np.frombuffer(packet, dtype=np.uint8) if not isinstance(packet, np.ndarray) else packet
+        call("fsim_ingest_push", self._h, C.c_void_p(buf.ctypes.data), int(buf.size))
+
+    def pop(self, channel):
+        """(frames [raws, h, w] uint16, info) of the next sequence of channel
+        index ``channel``, or None."""
+        mp = self.raws * self._max * self._max
+        if self._popbuf is None:
+            self._popbuf = np.empty(mp, np.uint16)
+        out = self._popbuf
+        info = INGEST_SEQ_INFO()
+        got = lib().fsim_ingest_pop(self._h, int(channel), C.c_void_p(out.ctypes.data), int(mp), C.byref(info))
+        if got < 0:
+            _lib.check(got)
+        if got == 0:
+            return None
+        npx = self.raws * info.width * info.height
+        return out[:npx].reshape(self.raws, info.height, info.width).copy(), info
+
+    def stats(self, channel):
+        st = INGEST_STATS()
+        call("fsim_ingest_get_stats", self._h, int(channel), C.byref(st))
+        return {f: getattr(st, f) for f, _ in INGEST_STATS._fields_}
+
+    def attach(self, server, server_channels, n, max_inflight=8):
+        """Feed ``server`` directly: channel index c goes to server channel
+        server_channels[c]; frames must be n x n."""
+        ids = (C.c_int * self.nch)(*server_channels)
+        call("fsim_ingest_attach_server", self._h, server._h, ids, int(n), int(max_inflight))
+        self._n = n
+        self._server = server
+
+    def take(self, timeout_ms=-1):
+        """The oldest submitted sequence's reconstruction ((2n)^2 float32) and
+        its info; raises FsimBusy on timeout or when nothing is in flight."""
+        out = np.empty((2 * self._n, 2 * self._n), np.float32)
+        info = INGEST_SEQ_INFO()
+        call("fsim_ingest_result", self._h, int(timeout_ms), C.c_void_p(out.ctypes.data), C.byref(info))
+        return out, info

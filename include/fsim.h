@@ -345,6 +345,88 @@ int fsim_server_submit(fsim_server s, int channel, const uint16_t *raw, float *o
 int fsim_server_wait(fsim_server s, uint64_t ticket, int timeout_ms, double *latency_ms);
 int fsim_server_get_stats(fsim_server s, fsim_server_stats *st, int reset);
 
+/* ---- live stream ingest (transport/ImageWrapper.java,
+ * livemode/SimSequenceExtractor.java) -----------------------------------------
+ * Wire format: a 128-byte little-endian header per camera frame, then the
+ * pixels (transport/header-definition.md; ImageWrapper.writeHeader /
+ * parseHeader, ImageWrapper.java:420-476): seq_nr @0 (int64), pixel count
+ * w*h @12 (int32), version @16 (1), bytes per pixel @17 (1 or 2), width @32,
+ * height @34, posA @36 (angle), posB @38 (phase), pos0 @40 (z), pos1 @42
+ * (channel), pos2 @44 (int32, time index), camera / capture / record time
+ * stamps @48/56/64 (int64, microseconds).  ImageWrapper neither writes nor
+ * checks the "LIVESIM" tag of the header description; neither does this. */
+typedef struct {
+    int64_t seq_nr;
+    int32_t npix;                 /* width * height */
+    int32_t version, bpp;
+    int32_t width, height;        /* int16 on the wire */
+    int32_t pos_a, pos_b, pos0, pos1;
+    int32_t pos2;
+    int64_t time_camera, time_capture, time_record;
+} fsim_frame_header;
+
+/* parse: ImageWrapper.parseHeader's checks (width/height beyond the maximum,
+ *   w*h != count, bpp not 1 or 2, version not 1) fail with FSIM_ERR_ARG.
+ * write: ImageWrapper.writeHeader (zero-filled, version 1). */
+int fsim_wire_parse_header(const uint8_t *hdr, int max_width, int max_height, fsim_frame_header *out);
+int fsim_wire_write_header(const fsim_frame_header *h, uint8_t *hdr);
+
+/* Sequence assembly: SimSequenceExtractor's per-channel threads
+ * (ImageSorter :146-178, PerChannelBuffer :211-470) as a push-driven state
+ * machine.  Frames are routed by pos1 to the channel with that number, put
+ * in sequence-number order through a sort buffer (:246-330; on overflow the
+ * frame is dropped and the buffer resynchronised to the lowest of the next
+ * ndir*nphase frames), a sync frame is searched (sync_mode 0: a camera-time
+ * gap > sync_delay before a frame, which then starts the sequence; 1: a
+ * camera-time gap within 50 of sync_delay, or a frame whose mean pixel
+ * exceeds sync_avr -- the frame after a bright one is skipped), and then
+ * seq_count sequences of ndir*nphase frames are taken (:369-470).  Frames of
+ * 8-bit packets are sign-extended to 16 bit as ImageWrapper.getPixels does.
+ * Assembled sequences wait in a per-channel queue of queue_capacity (further
+ * ones are dropped and counted, :457-460). */
+typedef struct fsim_ingest_s *fsim_ingest;
+typedef struct {
+    int ndir, nphase;
+    int sync_mode;          /* 0 or 1 (SyncMode) */
+    int64_t sync_delay;     /* SyncFrameDelay */
+    int sync_avr;           /* SyncFrameAvr */
+    int seq_count;          /* SyncFrameFreq: sequences per sync frame */
+    int sort_capacity;      /* reference: 20 */
+    int queue_capacity;     /* reference: (ndir*nphase)^2 * 10 */
+    int max_width, max_height;
+} fsim_ingest_config;
+typedef struct {
+    int64_t frames, sequences, missed_sequences, dropped_frames, sync_frames, no_sync_windows;
+} fsim_ingest_stats;
+typedef struct {
+    int channel;            /* index into channel_numbers */
+    int width, height;
+    int64_t first_seq_nr, time_camera;
+    int32_t pos2;           /* time index of the first frame */
+} fsim_ingest_seq_info;
+
+int fsim_ingest_create(const fsim_ingest_config *cfg, const int *channel_numbers, int nchannels, fsim_ingest *out);
+int fsim_ingest_destroy(fsim_ingest in);
+/* one packet (header + payload, len bytes); FSIM_ERR_ARG for a broken header,
+ * a short packet or an unknown channel (the packet is dropped, as the
+ * reference logs and drops it) */
+int fsim_ingest_push(fsim_ingest in, const uint8_t *packet, int64_t len);
+/* the next assembled sequence of channel index ch: 1 and the frames
+ * (ndir*nphase x w x h uint16, `out` holds max_pixels of them) when one is
+ * waiting, else 0 */
+int fsim_ingest_pop(fsim_ingest in, int ch, uint16_t *out, int64_t max_pixels, fsim_ingest_seq_info *info);
+int fsim_ingest_get_stats(fsim_ingest in, int ch, fsim_ingest_stats *st);
+
+/* Feeding the request server directly: sequences of channel index c are
+ * submitted to server channel server_channels[c] (frames must be n x n;
+ * others are dropped and counted) through pinned staging of up to
+ * max_inflight requests; push() and result() keep the pipe full.  result()
+ * waits (timeout_ms < 0: forever; FSIM_ERR_BUSY on timeout) for the oldest
+ * submitted sequence and copies its (2n)^2 image to out -- the
+ * ReconstructionRunner's finalRecon.take() (ReconstructionRunner.java:57-58). */
+int fsim_ingest_attach_server(fsim_ingest in, fsim_server srv, const int *server_channels, int n, int max_inflight);
+int fsim_ingest_result(fsim_ingest in, int timeout_ms, float *out, fsim_ingest_seq_info *info);
+
 /* Stage timing of the fused pipeline (no counterpart in the reference, which
  * only has Tool.Timer wall clocks, utils/Tool.java:210-245): while enabled,
  * every fsim_reconstruct records HIP events around its kernels on its stream;
