@@ -438,6 +438,51 @@ def stream_ingest(n=512, per_channel=48, runs=2):
             "pcie_inclusive": True, "bitwise_equal_to_direct": exact}
 
 
+def side_outputs(n=512, batch=64, reps=20):
+    """Live-mode side outputs on the device: widefield images of `batch`
+    512^2 stacks (9 uint16 frames read, one float image written: 22 B per
+    pixel) and the registration of `batch` reconstructed 1024^2 images
+    (source and result 8 B per pixel, the two tables 8 B per pixel once per
+    call), as HBM rates."""
+    import numpy as np
+    from fsim import synthetic
+    from fsim.linalg import DeviceBuffer
+    from fsim.livemode import Registration
+    from fsim.sim_algorithm import Reconstructor
+    raw, sp = synthetic.sim_stack(n=n, seed=5)
+    rec = Reconstructor(synthetic.make_param(sp, n))
+    rb = DeviceBuffer(batch * raw.nbytes)
+    rb.upload(np.ascontiguousarray(np.broadcast_to(raw, (batch,) + raw.shape)))
+    wb = DeviceBuffer(batch * n * n * 4)
+    rec.widefield(rb.value, wb.value, batch)
+    _lib_sync()
+    a = time.perf_counter()
+    for _ in range(reps):
+        rec.widefield(rb.value, wb.value, batch)
+    _lib_sync()
+    t_wf = (time.perf_counter() - a) / reps
+    M = 2 * n
+    gy, gx = np.mgrid[0:M, 0:M].astype(np.float32)
+    reg = Registration(gx + np.float32(0.3), gy - np.float32(0.45))
+    sb, db = DeviceBuffer(batch * M * M * 4), DeviceBuffer(batch * M * M * 4)
+    reg.register(sb.value, db.value, M, M, batch)
+    _lib_sync()
+    a = time.perf_counter()
+    for _ in range(reps):
+        reg.register(sb.value, db.value, M, M, batch)
+    _lib_sync()
+    t_reg = (time.perf_counter() - a) / reps
+    for b in (rb, wb, sb, db):
+        b.free()
+    reg.close()
+    wf_gbs = batch * n * n * 22 / t_wf / 1e9
+    reg_gbs = (batch * M * M * 8 + M * M * 8) / t_reg / 1e9   # images in and out; the tables once
+    return {"widefield": {"stacks_per_s": round(batch / t_wf, 1), "GB_per_s": round(wf_gbs, 1),
+                          "hbm_frac": round(wf_gbs / 8000, 4)},
+            "registration_1024": {"images_per_s": round(batch / t_reg, 1), "GB_per_s": round(reg_gbs, 1),
+                                  "hbm_frac": round(reg_gbs / 8000, 4)}}
+
+
 def _lib_sync():
     from fsim import _lib
     _lib.call("fsim_sync")
@@ -463,7 +508,7 @@ def main():
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed batched loop (for rocprofv3 runs)")
     ap.add_argument("--stream-only", action="store_true",
-                    help="only the live stream ingest leg")
+                    help="only the live stream ingest and side-output legs")
     ap.add_argument("--estimation-only", action="store_true",
                     help="only the batched estimation legs of config 1 (for rocprofv3 runs)")
     ap.add_argument("--probe-copy", action="store_true",
@@ -485,7 +530,8 @@ def main():
         probe_copy(args.steps)
         return
     if args.stream_only:
-        print(json.dumps({"stream_only": True, "stream_ingest_512": stream_ingest()}), flush=True)
+        print(json.dumps({"stream_only": True, "stream_ingest_512": stream_ingest(),
+                          "side_outputs_512": side_outputs()}), flush=True)
         return
     if args.estimation_only:
         out = {}
@@ -624,6 +670,8 @@ def main():
         configs["large_fov_2048"] = large_fov()
         log("live stream ingest: packets -> sequences -> server")
         configs["stream_ingest_512"] = stream_ingest()
+        log("live-mode side outputs: widefield, registration")
+        configs["side_outputs_512"] = side_outputs()
 
     # CPU baseline: the oracle restatement on the host cores, bounded sample
     cpu = None

@@ -1,0 +1,205 @@
+// Live-mode side outputs on the device: the widefield image of every stack
+// (livemode/ReconstructionRunner.java:452-481) and the registration of
+// widefield or reconstructed images through per-pixel translation tables
+// (registration/Registration.java).
+//
+// Both are one pass over HBM.  The widefield pass reads the D*P raw frames
+// of a stack (2 bytes per pixel each) and writes one float image; it
+// reproduces the reference's float arithmetic: each frame is faded by the
+// dampBorder vector (fadeBorderCos of a vector of ones), the frames are added
+// in (direction, phase) order and the sum is scaled by 1.f / (D*P).  The
+// registration pass is a bilinear gather with the reference's float
+// operation order.
+#include "fsim_internal.h"
+#include "recon.h"
+
+#include <cmath>
+#include <vector>
+
+namespace fsim {
+
+// Widefield images of `count` stacks: grid (blocks of 4-pixel groups,
+// stacks); n is a multiple of 4, so a group stays in one row.
+// raw_output in 1..D*P: that raw frame instead of the average
+// (ReconstructionRunner.java:473-479, setFrom16bitPixels of it).
+__global__ void k_widefield(const uint16_t *__restrict__ raw, float *__restrict__ out, int n, int nf,
+                            const double *__restrict__ lo, const double *__restrict__ hi, float scale,
+                            int raw_output) {
+#pragma clang fp contract(off)
+    const size_t nn = (size_t)n * n;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nn / 4) return;
+    const ushort4 *r = reinterpret_cast<const ushort4 *>(raw + (size_t)blockIdx.y * nf * nn);
+    float4 *o = reinterpret_cast<float4 *>(out + (size_t)blockIdx.y * nn);
+    const size_t nq = nn / 4;
+    if (raw_output > 0) {
+        const ushort4 v = r[(size_t)(raw_output - 1) * nq + q];
+        o[q] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+        return;
+    }
+    const int x0 = (int)((4 * q) % n), y = (int)((4 * q) / n);
+    // dampBorder: fadeBorderCos on a vector of ones, top, bottom, left, right
+    // (SimUtils.java:162-185; ReconstructionRunner.java:775-780)
+    float d[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        float e = 1.f;
+        e = (float)((double)e * lo[y]);
+        e = (float)((double)e * hi[y]);
+        e = (float)((double)e * lo[x0 + j]);
+        e = (float)((double)e * hi[x0 + j]);
+        d[j] = e;
+    }
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    for (int f = 0; f < nf; f++) {
+        const ushort4 v = r[(size_t)f * nq + q];
+        a0 = a0 + (float)v.x * d[0];
+        a1 = a1 + (float)v.y * d[1];
+        a2 = a2 + (float)v.z * d[2];
+        a3 = a3 + (float)v.w * d[3];
+    }
+    o[q] = make_float4(a0 * scale, a1 * scale, a2 * scale, a3 * scale);
+}
+
+// Registration.transPixel (Registration.java:239-268) at target (x, y) of a
+// w x h image in tables of width tw (the image centred in them at offx,
+// offy).  0 where the source falls outside [0, w-1) x [0, h-1) (the
+// reference's fresh result vector).
+__device__ __forceinline__ float trans_pixel(const float *__restrict__ s, const float *__restrict__ xt,
+                                             const float *__restrict__ yt, int tw, int w, int h, int offx, int offy,
+                                             int x, int y) {
+#pragma clang fp contract(off)
+    const size_t t = (size_t)(y + offy) * tw + (x + offx);
+    const float sx = xt[t] - offx, sy = yt[t] - offy;
+    float v = 0.f;
+    if (sx >= 0 && sy >= 0 && sx < w - 1 && sy < h - 1) {
+        const int left = (int)sx, top = (int)sy, right = (int)sx + 1, bottom = (int)sy + 1;
+        const float xf = sx - left, yf = sy - top;
+        const float tlf = (1 - xf) * (1 - yf), trf = xf * (1 - yf), blf = (1 - xf) * yf, brf = xf * yf;
+        const float tlv = s[(size_t)top * w + left] * tlf;
+        const float trv = s[(size_t)top * w + right] * trf;
+        const float blv = s[(size_t)bottom * w + left] * blf;
+        const float brv = s[(size_t)bottom * w + right] * brf;
+        v = tlv + trv + blv + brv;
+    }
+    return v;
+}
+
+// All target pixels of `count` images: grid (blocks of threads, images),
+// each thread kRegPix pixels a block width apart (coalesced table reads and
+// result writes, several gathers in flight per thread).
+constexpr int kRegPix = 4;
+__global__ void __launch_bounds__(256) k_register(float *__restrict__ dst, const float *__restrict__ src,
+                                                  const float *__restrict__ xt, const float *__restrict__ yt, int tw,
+                                                  int w, int h, int offx, int offy) {
+    const size_t nn = (size_t)w * h;
+    const float *s = src + (size_t)blockIdx.y * nn;
+    float *d = dst + (size_t)blockIdx.y * nn;
+    const size_t base = (size_t)blockIdx.x * 256 * kRegPix + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kRegPix; j++) {
+        const size_t i = base + (size_t)j * 256;
+        if (i < nn) d[i] = trans_pixel(s, xt, yt, tw, w, h, offx, offy, (int)(i % w), (int)(i / w));
+    }
+}
+
+}  // namespace fsim
+
+using namespace fsim;
+
+struct fsim_registration_s {
+    int w = 0, h = 0;            // reconstruction-size tables
+    float *xr = nullptr, *yr = nullptr;   // w x h
+    float *xw = nullptr, *yw = nullptr;   // w/2 x h/2 (widefield)
+};
+
+extern "C" {
+
+int fsim_widefield(fsim_channel ch, const uint16_t *raw, float *out, int count, int raw_output, void *strm) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!ch || !raw || !out || count < 1 || count > 65535) return fail(FSIM_ERR_ARG, "widefield: bad arguments");
+    const ReconConst &K = channel_host_const(ch);
+    const int nf = K.D * K.P;
+    if (raw_output < 0 || raw_output > nf)
+        return fail(FSIM_ERR_ARG, "widefield: raw_output must be 0 (average) or a frame number 1.." +
+                                      std::to_string(nf));
+    hipStream_t s = strm ? (hipStream_t)strm : stream();
+    const size_t nn = (size_t)K.n * K.n;
+    // 1.f / (nrDirs * nrPhases), in float as the reference divides
+    const float scale = 1.f / nf;
+    if (((uintptr_t)raw & 7) || ((uintptr_t)out & 15))
+        return fail(FSIM_ERR_ARG, "widefield: raw must be 8-byte and out 16-byte aligned");
+    hipLaunchKernelGGL(k_widefield, dim3(cdiv(nn / 4, 256), count), dim3(256), 0, s, raw, out, K.n, nf, K.fade_lo,
+                       K.fade_hi, scale, raw_output);
+    FSIM_HIP(hipGetLastError());
+    return FSIM_OK;
+}
+
+int fsim_registration_create(const float *xtrans, const float *ytrans, int width, int height,
+                             fsim_registration *out) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!xtrans || !ytrans || !out || width < 2 || height < 2)
+        return fail(FSIM_ERR_ARG, "registration_create: bad arguments");
+    fsim_registration_s *r = new fsim_registration_s();
+    r->w = width;
+    r->h = height;
+    const int ww = width / 2, wh = height / 2;
+    // widefield tables: the reconstruction tables at even (col, row), halved
+    // (Registration.writeRowToTransVec, Registration.java:371-385)
+    std::vector<float> xw((size_t)ww * wh), yw((size_t)ww * wh);
+    for (int y = 0; y < wh; y++)
+        for (int x = 0; x < ww; x++) {
+            xw[(size_t)y * ww + x] = xtrans[(size_t)(2 * y) * width + 2 * x] / 2;
+            yw[(size_t)y * ww + x] = ytrans[(size_t)(2 * y) * width + 2 * x] / 2;
+        }
+    const size_t nb = (size_t)width * height * sizeof(float), wb = xw.size() * sizeof(float);
+    if (hipMalloc(&r->xr, nb) != hipSuccess || hipMalloc(&r->yr, nb) != hipSuccess ||
+        hipMalloc(&r->xw, wb) != hipSuccess || hipMalloc(&r->yw, wb) != hipSuccess) {
+        (void)hipFree(r->xr);
+        (void)hipFree(r->yr);
+        (void)hipFree(r->xw);
+        (void)hipFree(r->yw);
+        delete r;
+        return fail(FSIM_ERR_OOM, "registration_create: device allocation failed");
+    }
+    FSIM_HIP(hipMemcpy(r->xr, xtrans, nb, hipMemcpyHostToDevice));
+    FSIM_HIP(hipMemcpy(r->yr, ytrans, nb, hipMemcpyHostToDevice));
+    FSIM_HIP(hipMemcpy(r->xw, xw.data(), wb, hipMemcpyHostToDevice));
+    FSIM_HIP(hipMemcpy(r->yw, yw.data(), wb, hipMemcpyHostToDevice));
+    *out = r;
+    return FSIM_OK;
+}
+
+int fsim_registration_destroy(fsim_registration r) {
+    if (!r) return FSIM_OK;
+    FSIM_HIP(hipStreamSynchronize(stream()));
+    (void)hipFree(r->xr);
+    (void)hipFree(r->yr);
+    (void)hipFree(r->xw);
+    (void)hipFree(r->yw);
+    delete r;
+    return FSIM_OK;
+}
+
+int fsim_register_images(fsim_registration r, int widefield, const float *src, float *dst, int w, int h, int count,
+                         void *strm) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!r || !src || !dst || w < 1 || h < 1 || count < 1 || count > 65535)
+        return fail(FSIM_ERR_ARG, "register_images: bad arguments");
+    if (src == dst) return fail(FSIM_ERR_ARG, "register_images: the result needs its own buffer");
+    const int tw = widefield ? r->w / 2 : r->w, th = widefield ? r->h / 2 : r->h;
+    if (w > tw || h > th)
+        return fail(FSIM_ERR_ARG, "register_images: Image registration failed, because of differences in dimensions");
+    hipStream_t s = strm ? (hipStream_t)strm : stream();
+    const size_t nn = (size_t)w * h;
+    hipLaunchKernelGGL(k_register, dim3(cdiv(nn, 256 * kRegPix), count), dim3(256), 0, s, dst, src,
+                       widefield ? r->xw : r->xr,
+                       widefield ? r->yw : r->yr, tw, w, h, (tw - w) / 2, (th - h) / 2);
+    FSIM_HIP(hipGetLastError());
+    return FSIM_OK;
+}
+
+}  // extern "C"

@@ -172,3 +172,79 @@ class SimSequenceExtractor:
         info = INGEST_SEQ_INFO()
         call("fsim_ingest_result", self._h, int(timeout_ms), C.c_void_p(out.ctypes.data), C.byref(info))
         return out, info
+
+
+def read_registration_file(path):
+    """The registration file as Registration reads it (Registration.java:
+    70-80, 334-385): ``...=width`` and ``...=height`` on the first two lines,
+    then an "X Trans" and a "Y Trans" block of space-separated rows of
+    source positions (tokens that do not parse are skipped), each ended by an
+    empty line.  Returns (xtrans, ytrans) float32 arrays [height, width]."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    width = int(lines[0].split("=")[1])
+    height = int(lines[1].split("=")[1])
+    tabs = {"x": np.zeros((height, width), np.float32), "y": np.zeros((height, width), np.float32)}
+    rows = {"x": 0, "y": 0}
+    cur = None
+    i = 2
+    while i < len(lines):
+        line = lines[i]
+        if line.startswith("X Trans") or line.startswith("Y Trans"):
+            cur = "x" if line.startswith("X") else "y"
+            i += 1
+            if i >= len(lines):
+                break
+            line = lines[i]
+        elif line == "":
+            cur = None
+        if cur is not None:
+            col = 0
+            for word in line.split(" "):
+                try:
+                    v = np.float32(float(word))
+                except ValueError:
+                    continue
+                tabs[cur][rows[cur], col] = v
+                col += 1
+            rows[cur] += 1
+        i += 1
+    return tabs["x"], tabs["y"]
+
+
+class Registration:
+    """Registration of widefield and reconstructed images on the device
+    (fsim_registration_*): ``register`` remaps ``count`` images through the
+    per-pixel source positions with the reference's bilinear transPixel."""
+
+    def __init__(self, xtrans, ytrans):
+        _lib.init()
+        xt = np.ascontiguousarray(xtrans, np.float32)
+        yt = np.ascontiguousarray(ytrans, np.float32)
+        if xt.shape != yt.shape:
+            raise ValueError("missmatch in dimensions")
+        self.height, self.width = xt.shape
+        self._h = C.c_void_p()
+        call("fsim_registration_create", C.c_void_p(xt.ctypes.data), C.c_void_p(yt.ctypes.data), self.width,
+             self.height, C.byref(self._h))
+
+    @staticmethod
+    def from_file(path):
+        return Registration(*read_registration_file(path))
+
+    def close(self):
+        if self._h is not None and self._h.value and _lib._lib is not None:
+            lib().fsim_registration_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def register(self, src_dev, dst_dev, w, h, count=1, widefield=False, stream=None):
+        """registerReconImage / registerWfImage for ``count`` images of w x h
+        (device addresses)."""
+        call("fsim_register_images", self._h, 1 if widefield else 0, C.c_void_p(int(src_dev)),
+             C.c_void_p(int(dst_dev)), int(w), int(h), int(count), C.c_void_p(stream) if stream else None)

@@ -648,6 +648,11 @@ class Reconstructor:
         call("fsim_reconstruct_params", self._h, C.c_void_p(int(raw_dev)), C.c_void_p(int(out_dev)), count, arr,
              C.c_void_p(stream) if stream else None)
 
+    def widefield(self, raw_dev, out_dev, count=1, raw_output=0, stream=None):
+        """fsim_widefield: the widefield images (n x n float) of count stacks."""
+        call("fsim_widefield", self._h, C.c_void_p(int(raw_dev)), C.c_void_p(int(out_dev)), int(count),
+             int(raw_output), C.c_void_p(stream) if stream else None)
+
     def reconstruct_host(self, raw):
         """Convenience: host uint16 stack(s) in, host float32 image(s) out."""
         raw = np.ascontiguousarray(raw, dtype=np.uint16)

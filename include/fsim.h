@@ -310,6 +310,30 @@ int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw_dev, float *out
 int fsim_params_from_estimate(const fsim_estimate_result *res, int count, int ndir, int nphase, int nband,
                               const fsim_sim_params *base, fsim_sim_params *out);
 
+/* ---- live-mode side outputs ---------------------------------------------
+ * widefield: the widefield image of each of `count` stacks (n x n float):
+ *   the faded raw frames (dampBorder, fadeBorderCos of ones) summed in
+ *   (direction, phase) order, times 1.f / (ndir * nphase)
+ *   (ReconstructionRunner.java:452-481); raw_output = k in 1..ndir*nphase
+ *   gives raw frame k instead (the reference's rawOutput).  The camera offset
+ *   of a channel (PerChannel.offset, :75, 92) is read by the reference but
+ *   never applied, so there is nothing to apply here either.
+ * registration: Registration (registration/Registration.java): per-pixel
+ *   source positions for reconstruction-size images (width x height tables,
+ *   the "X Trans" / "Y Trans" blocks of the registration file); widefield
+ *   images use the tables at even positions, halved (:371-385).  Images
+ *   smaller than the tables are centred in them.  register_images: bilinear
+ *   transPixel (:239-268) for `count` images of w x h (dst != src); pixels
+ *   whose source lies outside the image are 0. */
+int fsim_widefield(fsim_channel ch, const uint16_t *raw_dev, float *out_dev, int count, int raw_output,
+                   void *stream);
+typedef struct fsim_registration_s *fsim_registration;
+int fsim_registration_create(const float *xtrans, const float *ytrans, int width, int height,
+                             fsim_registration *out);
+int fsim_registration_destroy(fsim_registration r);
+int fsim_register_images(fsim_registration r, int widefield, const float *src_dev, float *dst_dev, int w, int h,
+                         int count, void *stream);
+
 /* ---- request server (continuous batching) ---------------------------------
  * The serving form of livemode/ReconstructionRunner.java: submit() is
  * queueImage() (:140-150) -- it refuses with FSIM_ERR_BUSY when max_queue
