@@ -483,6 +483,43 @@ def side_outputs(n=512, batch=64, reps=20):
                                   "hbm_frac": round(reg_gbs / 8000, 4)}}
 
 
+def filter_styles(n=512, iters=10, reps=5):
+    """Filter styles beyond Wiener: fsim_rl_deconvolve on a (2n)^2 spectrum
+    (the RL-on-output size) -- iterations/s and HBM rate against 168 B per
+    pixel and iteration (four 2-D FFTs at one read + write each, three fused
+    element-wise passes) -- and SimAlgorithm.runReconstruction's slow-shift
+    flow per filter style through the vector API (one 512^2 3x3 stack)."""
+    import numpy as np
+    from fsim import synthetic
+    from fsim.linalg import Vec2dCplx
+    from fsim.sim_algorithm import RLDeconvolution, input_ffts, run_reconstruction
+    M = 2 * n
+    rng = np.random.default_rng(0)
+    img, otf = Vec2dCplx(M, M), Vec2dCplx(M, M)
+    img.setData(rng.uniform(1, 2, (M, M, 2)).astype(np.float32))
+    otf.setData(rng.uniform(0, 1, (M, M, 2)).astype(np.float32))
+    RLDeconvolution.deconvolve(img, otf, 1)
+    a = time.perf_counter()
+    for _ in range(reps):
+        RLDeconvolution.deconvolve(img, otf, iters)
+    t = (time.perf_counter() - a) / (reps * iters)
+    gbs = 168 * M * M / t / 1e9
+    raw, sp = synthetic.sim_stack(n=n, seed=9)
+    param = synthetic.make_param(sp, n)
+    inFFT = input_ffts(raw.reshape(3, 3, n, n).astype(np.float32))
+    styles = {}
+    for style in ("Wiener", "RLin", "RLout", "RLboth"):
+        run_reconstruction(param, inFFT, filterStyle=style, rlIterations=5)
+        _lib_sync()
+        a = time.perf_counter()
+        run_reconstruction(param, inFFT, filterStyle=style, rlIterations=5)
+        _lib_sync()
+        styles[style] = round((time.perf_counter() - a) * 1e3, 2)
+    return {"rl_1024": {"iterations_per_s": round(1 / t, 1), "GB_per_s": round(gbs, 1),
+                        "hbm_frac": round(gbs / 8000, 4)},
+            "run_reconstruction_512_ms": styles}
+
+
 def _lib_sync():
     from fsim import _lib
     _lib.call("fsim_sync")
@@ -508,7 +545,7 @@ def main():
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed batched loop (for rocprofv3 runs)")
     ap.add_argument("--stream-only", action="store_true",
-                    help="only the live stream ingest and side-output legs")
+                    help="only the live stream ingest, side-output and filter-style legs")
     ap.add_argument("--estimation-only", action="store_true",
                     help="only the batched estimation legs of config 1 (for rocprofv3 runs)")
     ap.add_argument("--probe-copy", action="store_true",
@@ -531,7 +568,8 @@ def main():
         return
     if args.stream_only:
         print(json.dumps({"stream_only": True, "stream_ingest_512": stream_ingest(),
-                          "side_outputs_512": side_outputs()}), flush=True)
+                          "side_outputs_512": side_outputs(), "filter_styles_512": filter_styles()}),
+              flush=True)
         return
     if args.estimation_only:
         out = {}
@@ -672,6 +710,8 @@ def main():
         configs["stream_ingest_512"] = stream_ingest()
         log("live-mode side outputs: widefield, registration")
         configs["side_outputs_512"] = side_outputs()
+        log("filter styles: RL deconvolution, runReconstruction per style")
+        configs["filter_styles_512"] = filter_styles()
 
     # CPU baseline: the oracle restatement on the host cores, bounded sample
     cpu = None

@@ -175,6 +175,8 @@ _SIGS = {
     "fsim_server_submit": [_P, _I, _P, _P, _I, C.POINTER(C.c_uint64)],
     "fsim_server_wait": [_P, C.c_uint64, _I, _PD],
     "fsim_server_get_stats": [_P, C.POINTER(SERVER_STATS), _I],
+    "fsim_rl_deconvolve": [_P, _P, _I, _I],
+    "fsim_clip_scale": [_P, _I, _I],
     "fsim_widefield": [_P, _P, _P, _I, _I, _P],
     "fsim_registration_create": [_P, _P, _I, _I, C.POINTER(_P)],
     "fsim_registration_destroy": [_P],

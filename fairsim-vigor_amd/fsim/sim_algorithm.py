@@ -478,6 +478,120 @@ def paste_and_fourier_shift(inV, outV, kx, ky, fast=True):
         outV.pasteFreq(inV, int(math.floor(kx)), int(math.floor(-ky)))
 
 
+class RLDeconvolution:
+    """sim_algorithm/RLDeconvolution.java on the device (fsim_rl_deconvolve)."""
+
+    @staticmethod
+    def deconvolve(img, otf, steps, inputIsInFreqSpace=False):
+        """Richardson-Lucy iterations on ``img`` in place (RLDeconvolution.java:43-103)."""
+        call("fsim_rl_deconvolve", img.handle, otf.handle, int(steps), 1 if inputIsInFreqSpace else 0)
+
+
+FILTER_STYLES = ("Wiener", "RLin", "RLout", "RLboth")   # SimParam.FilterStyle (SimParam.java:33-43)
+
+
+def spatial(vec, clipScale="NONE"):
+    """SimUtils.spatial (SimUtils.java:124-153): inverse FFT of a copy, real
+    part; clipScale CLIP or BOTH runs clipAndScale (clip, or scale and clip)."""
+    tmp = vec.duplicate()
+    tmp.fft2d(True)
+    out = Vec2dReal(vec.vectorWidth(), vec.vectorHeight())
+    out.copy(tmp)
+    if clipScale in ("CLIP", "BOTH"):
+        call("fsim_clip_scale", out.handle, 1, 1 if clipScale == "BOTH" else 0)
+    return out
+
+
+def run_reconstruction(param, inFFT, otfBeforeShift=False, clipScale="NONE", filterStyle="Wiener",
+                       rlIterations=5, wiener=0.05, apoCutoff=2.0):
+    """SimAlgorithm.runReconstruction (SimAlgorithm.java:268-709) without the
+    intermediate displays: slow-shift reconstruction of FFT'd inputs
+    inFFT[d][p] (device vectors) with the filter style of SimParam
+    (Wiener, RL on input, RL on output, RL on both) and the otfBeforeShift
+    switch.  Returns the (2N)^2 real result on the device.
+
+    As in the reference: with otfBeforeShift the shifted bands are masked by
+    maskOtf(shifted[i], angIdx, i), i.e. around the pixel (angIdx, i)
+    (SimAlgorithm.java:408-410); RL on input deconvolves and separates the
+    first 2*bands-1 phase images, so it needs nphases == 2*bands-1 (the
+    reference fails with an index exception otherwise)."""
+    if filterStyle not in FILTER_STYLES:
+        raise ValueError(f"filter style {filterStyle} not in {FILTER_STYLES}")
+    useWiener = filterStyle in ("Wiener", "RLin")
+    useRLin = filterStyle in ("RLin", "RLboth")
+    useRLout = filterStyle in ("RLout", "RLboth")
+    w, h = inFFT[0][0].vectorWidth(), inFFT[0][0].vectorHeight()
+    apoB, apoF = .9, apoCutoff
+    otfPr = param.otf()
+    fullResult = Vec2dCplx(2 * w, 2 * h)
+    inputOtf = None
+    if useRLin:
+        inputOtf = Vec2dCplx(w, h)
+        otfPr.writeOtfVector(inputOtf, 0, 0, 0)
+    for angIdx in range(param.nrDir()):
+        par = param.dir(angIdx)
+        B = par.nrBand()
+        nc = 2 * B - 1
+        separate = [Vec2dCplx(w, h) for _ in range(nc)]
+        if useRLin:
+            if len(par.getPhases()) != nc:
+                raise IndexError("#input images != phases per band")
+            tmp = [inFFT[angIdx][i].duplicate() for i in range(nc)]
+            for t in tmp:
+                RLDeconvolution.deconvolve(t, inputOtf, rlIterations, True)
+            separate_bands(tmp, separate, par.getPhases(), B, par.getModulations())
+        else:
+            separate_bands(inFFT[angIdx], separate, par.getPhases(), B, par.getModulations())
+        if otfBeforeShift and useWiener:
+            for i in range(nc):
+                otfPr.applyOtf(separate[i], (i + 1) // 2)
+        shifted = [Vec2dCplx(2 * w, 2 * h) for _ in range(nc)]
+        shifted[0].pasteFreq(separate[0], 0, 0)
+        for b in range(1, B):
+            pos, neg = 2 * b, 2 * b - 1
+            paste_and_fourier_shift(separate[pos], shifted[pos], par.px(b), par.py(b), fast=False)
+            paste_and_fourier_shift(separate[neg], shifted[neg], -par.px(b), -par.py(b), fast=False)
+        if useWiener:
+            if not otfBeforeShift:
+                otfPr.applyOtf(shifted[0], 0)
+                for b in range(1, B):
+                    otfPr.applyOtf(shifted[2 * b], b, par.px(b), par.py(b))
+                    otfPr.applyOtf(shifted[2 * b - 1], b, -par.px(b), -par.py(b))
+            else:
+                for i in range(nc):
+                    otfPr.maskOtf(shifted[i], angIdx, i)
+        if useRLout:
+            shifted[0].scal(np.float32(1) / np.float32(param.nrDir()))
+        for i in range(nc):
+            fullResult.add(shifted[i])
+    image = None
+    if useWiener:
+        denom = Vec2dReal(2 * w, 2 * h)
+        wiener_denominator(denom, param, wiener)
+        fullResult.times(denom)
+        apo = Vec2dCplx(2 * w, 2 * h)
+        otfPr.writeApoVector(apo, apoB, apoF)
+        fullResult.times(apo)
+        image = spatial(fullResult, clipScale)
+    if useRLout:
+        otfSim = Vec2dCplx(2 * w, 2 * h)
+        pos_v, neg_v = Vec2dCplx(2 * w, 2 * h), Vec2dCplx(2 * w, 2 * h)
+        for ang in range(param.nrDir()):
+            par = param.dir(ang)
+            for b in range(param.nrBand()):
+                otfPr.writeOtfVector(pos_v, b, par.px(b), par.py(b))
+                otfPr.writeOtfVector(neg_v, b, -par.px(b), -par.py(b))
+                if b == 0:
+                    pos_v.scal(np.float32(1) / np.float32(param.nrDir()))
+                    otfSim.add(pos_v)
+                else:
+                    otfSim.add(pos_v)
+                    otfSim.add(neg_v)
+        RLDeconvolution.deconvolve(fullResult, otfSim, rlIterations, True)
+        image = spatial(fullResult, clipScale)
+    return image
+
+
 class PlainPipeline:
     """Pre-allocated vectors and filters of one ReconstructionThread
     (ReconstructionRunner.java:380-413) for one channel; ``run`` issues the

@@ -310,6 +310,17 @@ int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw_dev, float *out
 int fsim_params_from_estimate(const fsim_estimate_result *res, int count, int ndir, int nphase, int nband,
                               const fsim_sim_params *base, fsim_sim_params *out);
 
+/* ---- filter styles beyond Wiener ------------------------------------------
+ * rl_deconvolve: RLDeconvolution.deconvolve(img, otf, steps,
+ *   inputIsInFreqSpace) (sim_algorithm/RLDeconvolution.java:43-103), in
+ *   place; img and otf complex vectors of one power-of-two size.  Used by
+ *   SimAlgorithm.runReconstruction's "RL on input / output / both" filter
+ *   styles (SimAlgorithm.java:320-356, 587-636).
+ * clip_scale: SimUtils.clipAndScale (SimUtils.java:244-264) on a real
+ *   vector: scale by 255 / (largest positive value), then clip below 0. */
+int fsim_rl_deconvolve(fsim_vec img, fsim_vec otf, int steps, int input_in_freq);
+int fsim_clip_scale(fsim_vec v, int clip, int scale);
+
 /* ---- live-mode side outputs ---------------------------------------------
  * widefield: the widefield image of each of `count` stacks (n x n float):
  *   the faded raw frames (dampBorder, fadeBorderCos of ones) summed in

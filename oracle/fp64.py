@@ -244,3 +244,102 @@ def rel_l2(a, b):
     a = a.astype(dt)
     b = b.astype(dt)
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def rl_deconvolve64(img, otf, steps, in_freq=False):
+    """RLDeconvolution.deconvolve (RLDeconvolution.java:43-103) in FP64:
+    u_{j+1} = u_j x IFFT(FFT(d / IFFT(FFT(u_j) otf)) otf), u_0 = d."""
+    d = ifft2(img) if in_freq else np.asarray(img, dtype=np.complex128)
+    u = d.copy()
+    for _ in range(steps):
+        nxt = ifft2(fft2(u) * otf)
+        nxt = d / nxt
+        nxt = ifft2(fft2(nxt) * otf)
+        u = nxt * u
+    return fft2(u) if in_freq else u
+
+
+def mask_otf(w, h, cutoff, vec_cpm, kx, ky):
+    """OtfProvider.maskOtf (OtfProvider.java:538-559) as a 0/1 array."""
+    xh, yh = coords(w, h)
+    return np.where(np.sqrt((xh - kx) ** 2 + (yh - ky) ** 2) * vec_cpm > cutoff, 0.0, 1.0)
+
+
+def clip_and_scale(img, clip, scale):
+    """SimUtils.clipAndScale (SimUtils.java:244-264)."""
+    img = img.copy()
+    if scale:
+        mx = max(0.0, float(img.max()))
+        img = img * (255.0 / mx)
+    if clip:
+        img[img < 0] = 0
+    return img
+
+
+def run_reconstruction64(p, in_fft, vals, filter_style="Wiener", otf_before_shift=False, rl_iterations=5,
+                         clip_scale="NONE"):
+    """SimAlgorithm.runReconstruction (SimAlgorithm.java:268-709, no
+    intermediate displays) in FP64: slow-shift reconstruction of FFT'd
+    inputs in_fft[d][p] with the filter styles Wiener, RLin, RLout, RLboth
+    and the otfBeforeShift switch, including the reference's mask around
+    (angIdx, i) in the otfBeforeShift branch (:408-410)."""
+    n, D, P, B = p["n"], p["ndir"], p["nphase"], p["nband"]
+    C = 2 * B - 1
+    use_w = filter_style in ("Wiener", "RLin")
+    use_rin = filter_style in ("RLin", "RLboth")
+    use_rout = filter_style in ("RLout", "RLboth")
+    cpm, cut, vcpm = p["otf_cpm"], p["otf_cutoff"], p["vec_cpm"]
+    only_att = val_attenuation(np.arange(len(vals)) * cpm, p["att_str"], p["att_fwhm"])
+    tab = vals * only_att if p["use_att"] else vals     # applyOtf uses the attenuated table when on
+    full = np.zeros((2 * n, 2 * n), dtype=np.complex128)
+    input_otf = otf_to_vector(n, n, vals, cpm, cut, vcpm) if use_rin else None
+    for d in range(D):
+        mods = [clamp_mod(m) for m in p["modul"][d][:B]]
+        S = separation_matrix(p["phases"][d][:P], B, mods)
+        inputs = [np.asarray(v, dtype=np.complex128) for v in in_fft[d]]
+        if use_rin:
+            if P != C:
+                raise IndexError("#input images != phases per band")
+            inputs = [rl_deconvolve64(inputs[i], input_otf, rl_iterations, True) for i in range(C)]
+        sep = [sum(S[c, ph] * inputs[ph] for ph in range(P)) for c in range(C)]
+        if otf_before_shift and use_w:
+            o = np.conj(otf_to_vector(n, n, tab, cpm, cut, vcpm))
+            sep = [s * o for s in sep]
+        shifted = [paste_freq(sep[0], 2 * n, 2 * n)]
+        kxy = [(0.0, 0.0)]
+        for b in range(1, B):
+            for comp, sgn in ((2 * b - 1, -1.0), (2 * b, 1.0)):
+                kx, ky = sgn * p["px"][d] * b, sgn * p["py"][d] * b
+                s = paste_freq(sep[comp], 2 * n, 2 * n)
+                shifted.append(fft2(fourier_shift(ifft2(s), kx, -ky)))
+                kxy.append((kx, ky))
+        if use_w:
+            for i in range(C):
+                if not otf_before_shift:
+                    shifted[i] = shifted[i] * np.conj(otf_to_vector(2 * n, 2 * n, tab, cpm, cut, vcpm, *kxy[i]))
+                else:
+                    shifted[i] = shifted[i] * mask_otf(2 * n, 2 * n, cut, vcpm, d, i)
+        if use_rout:
+            shifted[0] = shifted[0] / D
+        for s in shifted:
+            full = full + s
+    img = None
+    if use_w:
+        wien = wiener_denominator(2 * n, 2 * n, vals, only_att, cpm, cut, vcpm, p["px"], p["py"], B,
+                                  p["use_att"], p["wiener"])
+        apo = apo_vector(2 * n, 2 * n, vcpm, cut, 0.9, p["apo_cut"])
+        img = ifft2(full * wien * apo).real
+    if use_rout:
+        sim = np.zeros((2 * n, 2 * n))
+        for d in range(D):
+            for b in range(B):
+                bx, by = p["px"][d] * b, p["py"][d] * b
+                if b == 0:
+                    sim = sim + otf_to_vector(2 * n, 2 * n, vals, cpm, cut, vcpm, bx, by) / D
+                else:
+                    sim = sim + otf_to_vector(2 * n, 2 * n, vals, cpm, cut, vcpm, bx, by)
+                    sim = sim + otf_to_vector(2 * n, 2 * n, vals, cpm, cut, vcpm, -bx, -by)
+        img = ifft2(rl_deconvolve64(full, sim, rl_iterations, True)).real
+    if clip_scale in ("CLIP", "BOTH"):
+        img = clip_and_scale(img, True, clip_scale == "BOTH")
+    return img
