@@ -520,6 +520,29 @@ def filter_styles(n=512, iters=10, reps=5):
             "run_reconstruction_512_ms": styles}
 
 
+def config_io(runs=5):
+    """XML parameter files: load the example live configuration
+    (tests/golden/vigor-fastsim-example.xml), build its three channels
+    (SimParam, OTF) and their device tables (fused reconstruction channels),
+    and write it back -- milliseconds per step."""
+    from fsim.conf import Conf
+    from fsim.livemode import load_live_channels
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "vigor-fastsim-example.xml")
+    t_load, t_chan = [], []
+    for _ in range(runs):
+        a = time.perf_counter()
+        cfg = Conf.loadFile(path)
+        vs = cfg.r().cd("vigor-settings")
+        chans = load_live_channels(vs, ["488", "568", "647"])
+        t_load.append((time.perf_counter() - a) * 1e3)
+        a = time.perf_counter()
+        recs = [c.reconstructor() for c in chans]
+        _lib_sync()
+        t_chan.append((time.perf_counter() - a) * 1e3)
+        del recs
+    return {"load_config_and_params_ms": round(min(t_load), 2), "channel_tables_3x512_ms": round(min(t_chan), 2)}
+
+
 def _lib_sync():
     from fsim import _lib
     _lib.call("fsim_sync")
@@ -568,7 +591,8 @@ def main():
         return
     if args.stream_only:
         print(json.dumps({"stream_only": True, "stream_ingest_512": stream_ingest(),
-                          "side_outputs_512": side_outputs(), "filter_styles_512": filter_styles()}),
+                          "side_outputs_512": side_outputs(), "filter_styles_512": filter_styles(),
+                          "config_io": config_io()}),
               flush=True)
         return
     if args.estimation_only:
@@ -712,6 +736,8 @@ def main():
         configs["side_outputs_512"] = side_outputs()
         log("filter styles: RL deconvolution, runReconstruction per style")
         configs["filter_styles_512"] = filter_styles()
+        log("configuration files: load, channel set-up")
+        configs["config_io"] = config_io()
 
     # CPU baseline: the oracle restatement on the host cores, bounded sample
     cpu = None

@@ -338,6 +338,21 @@ int fsim_otf_from_estimate(double na, double lambda, double a, int ns, double at
     return FSIM_OK;
 }
 
+int fsim_otf_attenuation(int nband, int ns, double cpm, double att_str, double att_fwhm, const float *vals,
+                         float *vals_att, float *only_att) {
+    if (nband < 1 || ns < 2 || !(cpm > 0) || !vals || !vals_att || !only_att)
+        return fail(FSIM_ERR_ARG, "otf_attenuation: bad arguments");
+    for (int b = 0; b < nband; b++)
+        for (int v = 0; v < ns; v++) {
+            const size_t i = (size_t)b * ns + v;
+            const double dist = v * cpm;
+            only_att[i] = val_attenuation(dist, att_str, att_fwhm);
+            vals_att[2 * i] = vals[2 * i] * only_att[i];
+            vals_att[2 * i + 1] = vals[2 * i + 1] * only_att[i];
+        }
+    return FSIM_OK;
+}
+
 int fsim_otf_create(int nband, int ns, double cutoff, double cpm, const float *vals, const float *vals_att,
                     const float *only_att, fsim_otf *out) {
     int rc = ensure_init();

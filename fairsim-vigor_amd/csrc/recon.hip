@@ -773,7 +773,7 @@ int build_channel_const(const fsim_recon_params &p, const Otf &o, bool sym, Reco
     K.B = B;
     K.C = C;
     K.ns = o.ns;
-    K.otf_nb = o.nband;
+    K.otf_nb = p.otf_band0 ? 1 : o.nband;   // bands of the N^2 OTF tables the components use
     K.otf_cpm = o.cpm;
     K.cutoff = o.cutoff;
     K.vec_cpm = p.vec_cpm;

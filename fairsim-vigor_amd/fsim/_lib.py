@@ -50,6 +50,7 @@ class RECON_PARAMS(C.Structure):
         ("apo_cut", C.c_double),
         ("fade_px", C.c_int),
         ("mode", C.c_int),
+        ("otf_band0", C.c_int),
     ]
 
 
@@ -148,6 +149,7 @@ _SIGS = {
     "fsim_vec_get": [_P, _I, _I, _PF, _PF],
     "fsim_vec_set": [_P, _I, _I, _F, _F],
     "fsim_otf_from_estimate": [_D, _D, _D, _I, _D, _D, _P, _P, _P, _PD, _PD],
+    "fsim_otf_attenuation": [_I, _I, _D, _D, _D, _P, _P, _P],
     "fsim_otf_create": [_I, _I, _D, _D, _P, _P, _P, C.POINTER(_P)],
     "fsim_otf_destroy": [_P],
     "fsim_otf_to_vector": [_P, _P, _I, _D, _D, _D, _I, _I],

@@ -248,3 +248,51 @@ class Registration:
         (device addresses)."""
         call("fsim_register_images", self._h, 1 if widefield else 0, C.c_void_p(int(src_dev)),
              C.c_void_p(int(dst_dev)), int(w), int(h), int(count), C.c_void_p(stream) if stream else None)
+
+
+class PerChannel:
+    """ReconstructionRunner.PerChannel (ReconstructionRunner.java:72-110): a
+    channel's SIM parameters and OTF from its configuration folder
+    ("channel-<label>"), the pixel size from the global settings, and the
+    live filter settings setupFilters applies (:773-813): Wiener 0.05,
+    attenuation 0.99 / 1.25 switched on, apodization bend 1 / cutoff 2, fade
+    10 px, band 0's OTF for every component.  ``offset`` is read as the
+    reference reads it (and, as there, not applied)."""
+
+    def __init__(self, fld, image_size, microns, label):
+        from .sim_algorithm import OtfProvider, SimParam
+        self.param = SimParam.loadConfig(fld)
+        self.param.setPxlSize(image_size, microns)
+        self.param.otf(OtfProvider.loadFromConfig(fld))
+        self.offset = float(np.float32(fld.getDbl("offset").val()))
+        self.chNumber = fld.getInt("ChannelNumber").val()
+        self.label = label
+        self.wienParam, self.attStr, self.attFWHM, self.useAttenuation = 0.05, 0.99, 1.25, True
+
+    def reconstructor(self, mode=0):
+        from .sim_algorithm import Reconstructor
+        otf = self.param.otf()
+        otf.setAttenuation(self.attStr, self.attFWHM)
+        otf.switchAttenuation(self.useAttenuation)
+        return Reconstructor(self.param, wiener=self.wienParam, apo_bend=1.0, apo_cut=2.0, fade_px=10, mode=mode,
+                             otf_band0=True)
+
+
+def load_live_channels(cfg, labels):
+    """The channel set-up of ReconstructionRunner's constructor
+    (ReconstructionRunner.java:160-200): ``cfg`` is the vigor-settings
+    folder, ``labels`` the channels to run (e.g. ["488", "568"])."""
+    size = cfg.getInt("RawPxlCount").val()
+    microns = cfg.getDbl("RawPxlSize").val()
+    return [PerChannel(cfg.cd("channel-" + lb), size, microns, lb) for lb in labels]
+
+
+def sequence_extractor_from_conf(cfg, channels, **kw):
+    """SimSequenceExtractor with the vigor-settings' SyncMode, SyncFrameFreq,
+    SyncFrameAvr, SyncFrameDelay, NrAngles and NrPhases
+    (SimSequenceExtractor.java:59-90); channels: PerChannel objects."""
+    return SimSequenceExtractor([c.chNumber for c in channels], ndir=cfg.getInt("NrAngles").val(),
+                                nphase=cfg.getInt("NrPhases").val(), sync_mode=cfg.getInt("SyncMode").val(),
+                                sync_delay=cfg.getInt("SyncFrameDelay").val(),
+                                sync_avr=cfg.getInt("SyncFrameAvr").val(),
+                                seq_count=cfg.getInt("SyncFrameFreq").val(), **kw)

@@ -30,10 +30,13 @@ from .linalg import AccelVectorFactory, DeviceBuffer, Vec2dCplx, Vec2dReal
 
 class OtfProvider:
     """OTF tables from an estimate (OtfProvider.fromEstimate,
-    OtfProvider.java:96-127) with attenuation (:276-287)."""
+    OtfProvider.java:96-127) or from measured per-band data (loadFromConfig,
+    :574-631), with attenuation (:276-287)."""
 
     def __init__(self):
         self.samples = 512
+        self.nband = 1
+        self.is_estimate = True
         self.na = self.lam = self.a = None
         self.att_str, self.att_fwhm = 0.99, 1.2
         self.use_att = False
@@ -48,7 +51,39 @@ class OtfProvider:
         o.use_att = False
         return o
 
+    @staticmethod
+    def fromData(na, lam, cycles_per_micron, vals):
+        """A measured OTF: vals complex [bands, samples] (one radial profile
+        per band, sample i at i * cycles_per_micron); cutoff 1000 / (lambda /
+        NA / 2) as the reference sets it (OtfProvider.java:593-597)."""
+        vals = np.asarray(vals, dtype=np.complex64)
+        if vals.ndim == 1:
+            vals = vals[None, :]
+        o = OtfProvider()
+        o.na, o.lam = float(na), float(lam)
+        o.is_estimate = False
+        o.nband, o.samples = vals.shape
+        o.cpm = float(cycles_per_micron)
+        o.cutoff = 1000 / (o.lam / o.na / 2)
+        o.vals = np.ascontiguousarray(np.stack([vals.real, vals.imag], -1).reshape(-1), dtype=np.float32)
+        o._compute()
+        o.use_att = False
+        return o
+
+    def band_values(self):
+        """vals as complex [bands, samples]."""
+        v = self.vals.reshape(self.nband, self.samples, 2)
+        return v[..., 0] + 1j * v[..., 1]
+
     def _compute(self):
+        if not self.is_estimate:
+            self.vals_att = np.zeros_like(self.vals)
+            self.only_att = np.zeros(self.nband * self.samples, dtype=np.float32)
+            call("fsim_otf_attenuation", self.nband, self.samples, self.cpm, self.att_str, self.att_fwhm,
+                 self.vals.ctypes.data_as(C.c_void_p), self.vals_att.ctypes.data_as(C.c_void_p),
+                 self.only_att.ctypes.data_as(C.c_void_p))
+            self._release()
+            return
         n = self.samples
         self.vals = np.zeros(2 * n, dtype=np.float32)
         self.vals_att = np.zeros(2 * n, dtype=np.float32)
@@ -94,7 +129,7 @@ class OtfProvider:
         if self._dev is None:
             _lib.init()
             h = C.c_void_p()
-            call("fsim_otf_create", 1, self.samples, self.cutoff, self.cpm,
+            call("fsim_otf_create", self.nband, self.samples, self.cutoff, self.cpm,
                  self.vals.ctypes.data_as(C.c_void_p), self.vals_att.ctypes.data_as(C.c_void_p),
                  self.only_att.ctypes.data_as(C.c_void_p), C.byref(h))
             self._dev = h
@@ -124,6 +159,54 @@ class OtfProvider:
     def maskOtf(self, vec, kx, ky):
         call("fsim_mask_otf", vec.handle, self.device(), float(kx), float(ky), self.vec_cpm)
 
+    # --- configuration files (OtfProvider.java:566-664) ---
+    @staticmethod
+    def loadFromConfig(cfg):
+        """OtfProvider.loadFromConfig from a Conf folder holding "otf2d":
+        an estimate (NA, emission, a-estimate) or data (bands, samples,
+        cycles, band-<b> as big-endian float32 (re, im) pairs); then the
+        attenuation folder, when present, sets and switches it on."""
+        from . import conf
+        fld = cfg.cd("otf2d")
+        if not fld.contains("data"):
+            ret = OtfProvider.fromEstimate(fld.getDbl("NA").val(), fld.getInt("emission").val(),
+                                           fld.getDbl("a-estimate").val())
+        else:
+            data = fld.cd("data")
+            nb, ns = data.getInt("bands").val(), data.getInt("samples").val()
+            vals = np.zeros((nb, ns), np.complex64)
+            for b in range(nb):
+                f = np.asarray(conf.from_byte(data.getData("band-%d" % b).val()), np.float32)
+                vals[b] = f[0:2 * ns:2] + 1j * f[1:2 * ns:2]
+            ret = OtfProvider.fromData(fld.getDbl("NA").val(), fld.getInt("emission").val(),
+                                       data.getDbl("cycles").val(), vals)
+        if fld.contains("attenuation"):
+            att = fld.cd("attenuation")
+            ret.setAttenuation(att.getDbl("strength").val(), att.getDbl("FWHM").val())
+            ret.switchAttenuation(True)
+        return ret
+
+    def saveConfig(self, cfg):
+        """OtfProvider.saveConfig into a Conf folder (creates "otf2d")."""
+        from . import conf
+        fld = cfg.mk("otf2d")
+        fld.newDbl("NA", self.na)
+        fld.newInt("emission", int(self.lam))
+        if self.is_estimate:
+            fld.newDbl("a-estimate", self.a)
+        else:
+            data = fld.mk("data")
+            data.newInt("bands", self.nband)
+            data.newInt("samples", self.samples)
+            data.newDbl("cycles", self.cpm)
+            for b in range(self.nband):
+                data.newData("band-%d" % b, conf.to_byte(
+                    self.vals[2 * b * self.samples:2 * (b + 1) * self.samples].tolist()))
+        if self.use_att:
+            att = fld.mk("attenuation")
+            att.newDbl("strength", self.att_str)
+            att.newDbl("FWHM", self.att_fwhm)
+
 
 class Dir:
     """SimParam.Dir (SimParam.java:378-615)."""
@@ -138,6 +221,7 @@ class Dir:
         self.phaOff = 0.0
         self.phases = [(2 * math.pi / phases) * i for i in range(phases)]
         self.modul = [1.0] * bands
+        self.hasIndividualPhases = False
 
     def nrBand(self):
         return self.nrBands
@@ -170,6 +254,7 @@ class Dir:
         if reset:
             self.phaOff = 0.0
         self.phases = [float(p) for p in pha]
+        self.hasIndividualPhases = True
 
     def getPhases(self):
         return [p + self.phaOff for p in self.phases]
@@ -186,19 +271,84 @@ class SimParam:
     """SimParam.create(bands, dirs, phases, size, micronsPerPxl, otf)
     (SimParam.java:103-130, 250-256)."""
 
+    IMGSEQ = ("PAZ", "PZA", "ZAP")   # SimParam.IMGSEQ (SimParam.java:753-832)
+
     def __init__(self, bands, dirs, phases, size, microns, otf):
         self.nrBands, self.nrDirs, self.nrPhases = bands, dirs, phases
         self.directions = [Dir(bands, phases) for _ in range(dirs)]
-        self.imgSize = int(size)
-        self.micronsPerPixel = float(microns)
-        self.cyclesPerMicron = 1 / (size * microns)
-        self.otf_ = otf
+        self.otf_ = None
+        self.wienerFilterParameter = 0.05
+        self.apoCutOff = 2.0
+        self.imgSeq = "PAZ"
+        self.paramDate = 0
+        self.setPxlSize(size, microns)
         if otf is not None:
-            otf.setPixelSize(self.cyclesPerMicron)
+            self.otf(otf)
 
     @staticmethod
     def create(bands, dirs, phases, size, microns, otf):
         return SimParam(bands, dirs, phases, size, microns, otf)
+
+    def setPxlSize(self, size, microns):
+        self.imgSize = int(size)
+        self.micronsPerPixel = float(microns)
+        self.cyclesPerMicron = 1 / (size * microns)
+        if self.otf_ is not None:
+            self.otf_.setPixelSize(self.cyclesPerMicron)
+
+    def getWienerFilter(self):
+        return self.wienerFilterParameter
+
+    def getApoCutoff(self):
+        return self.apoCutOff
+
+    @staticmethod
+    def loadConfig(cfg):
+        """SimParam.loadConfig (SimParam.java:676-726) from a Conf folder
+        holding "sim-param"."""
+        fd = cfg.cd("sim-param")
+        ret = SimParam(fd.getInt("nr-bands").val(), fd.getInt("nr-angles").val(), fd.getInt("nr-phases").val(),
+                       fd.getInt("img-size-pxl").val(), fd.getDbl("microns-per-pxl").val(), None)
+        if fd.contains("timestamp"):
+            ret.paramDate = fd.getTDE("timestamp").val()
+        seq = fd.getStr("img-seq").val().strip()
+        ret.imgSeq = seq if seq in SimParam.IMGSEQ else None
+        ret.wienerFilterParameter = fd.getDbl("wiener-parameter").val()
+        ret.apoCutOff = fd.getDbl("apodization-cutoff").val()
+        for d in range(ret.nrDirs):
+            df = fd.cd("dir-%d" % d)
+            dd = ret.dir(d)
+            shift = df.getDbl("shift").vals
+            if len(shift) < 2:
+                raise RuntimeError("Array too short")
+            dd.setPxPy(shift[0], shift[1])
+            dd.setPhaOff(df.getDbl("phase-offset").val())
+            for i, m in enumerate(df.getDbl("modulations").vals):
+                dd.setModulation(i, m)
+            if df.contains("phases"):
+                dd.setPhases(df.getDbl("phases").vals, False)
+        return ret
+
+    def saveConfig(self, cfg):
+        """SimParam.saveConfig (SimParam.java:645-674) into a Conf folder."""
+        fd = cfg.mk("sim-param")
+        fd.newInt("nr-angles", self.nrDirs)
+        fd.newInt("nr-bands", self.nrBands)
+        fd.newInt("nr-phases", self.nrPhases)
+        fd.newStr("img-seq", self.imgSeq)
+        fd.newInt("img-size-pxl", self.imgSize)
+        fd.newDbl("microns-per-pxl", self.micronsPerPixel)
+        fd.newDbl("wiener-parameter", self.wienerFilterParameter)
+        fd.newDbl("apodization-cutoff", self.apoCutOff)
+        fd.newTDE("timestamp", self.paramDate)
+        for d in range(self.nrDirs):
+            df = fd.mk("dir-%d" % d)
+            dd = self.dir(d)
+            df.newDbl("shift", dd.pX * (dd.nrBands - 1), dd.pY * (dd.nrBands - 1))
+            df.newDbl("phase-offset", dd.phaOff)
+            df.newDbl("modulations", *dd.modul)
+            if dd.hasIndividualPhases:
+                df.newDbl("phases", *dd.phases)
 
     def dir(self, i):
         return self.directions[i]
@@ -209,8 +359,13 @@ class SimParam:
     def nrBand(self):
         return self.nrBands
 
-    def otf(self):
-        return self.otf_
+    def otf(self, o=None):
+        """getter, or SimParam.otf(OtfProvider): set it, with this pixel size"""
+        if o is None:
+            return self.otf_
+        self.otf_ = o
+        o.setPixelSize(self.cyclesPerMicron)
+        return o
 
     def pxlSizeCyclesMicron(self):
         return self.cyclesPerMicron
@@ -686,7 +841,7 @@ def params_from_estimate(param, dirs):
     return p
 
 
-def recon_params(param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10, mode=0):
+def recon_params(param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10, mode=0, otf_band0=False):
     """Pack a SimParam into fsim_recon_params."""
     p = RECON_PARAMS()
     p.n = param.vectorWidth()
@@ -709,6 +864,7 @@ def recon_params(param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10, mode
     p.apo_cut = apo_cut
     p.fade_px = fade_px
     p.mode = mode
+    p.otf_band0 = 1 if otf_band0 else 0
     return p
 
 
@@ -716,14 +872,15 @@ class Reconstructor:
     """Fused reconstruction of one channel (fsim_channel): tables resident on
     the device, ``reconstruct`` runs a batch of stacks in five passes."""
 
-    def __init__(self, param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10, mode=0):
+    def __init__(self, param, wiener=0.05, apo_bend=0.9, apo_cut=2.0, fade_px=10, mode=0, otf_band0=False):
         """mode 0 derives each band's neg component from its pos component
-        when the OTF is real (exact identity); mode 1 computes all."""
+        when the OTF is real (exact identity); mode 1 computes all.
+        otf_band0: band 0's OTF for every component (the live runner)."""
         _lib.init()
         self.param = param
         self.n = param.vectorWidth()
         self.D, self.P = param.nrDir(), param.dir(0).nrPha()
-        self.p = recon_params(param, wiener, apo_bend, apo_cut, fade_px, mode)
+        self.p = recon_params(param, wiener, apo_bend, apo_cut, fade_px, mode, otf_band0)
         self._h = C.c_void_p()
         call("fsim_channel_create", C.byref(self.p), param.otf().device(), C.byref(self._h))
 

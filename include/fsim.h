@@ -139,6 +139,12 @@ int fsim_otf_from_estimate(double na, double lambda_nm, double a, int nsamples,
                            double att_str, double att_fwhm, float *vals,
                            float *vals_att, float *only_att, double *cutoff,
                            double *cpm);
+/* fsim_otf_attenuation: OtfProvider.setAttenuation's per-band tables
+ *   (OtfProvider.java:276-287) for OTFs read from data: only_att[b][v] =
+ *   valAttenuation(v * cpm, strength, fwhm), vals_att = vals * only_att
+ *   (nband x nsamples, complex interleaved). */
+int fsim_otf_attenuation(int nband, int nsamples, double cpm, double att_str, double att_fwhm,
+                         const float *vals, float *vals_att, float *only_att);
 int fsim_otf_create(int nband, int nsamples, double cutoff, double cpm,
                     const float *vals, const float *vals_att, const float *only_att,
                     fsim_otf *out);
@@ -267,6 +273,11 @@ typedef struct {
                                  component (exact for real raw data and a real
                                  OTF; chosen only when the OTF is real),
                                  1: compute every component separately */
+    int otf_band0;            /* 1: band 0's OTF for every component, as the live
+                                 ReconstructionRunner builds one OTF vector per
+                                 channel (ReconstructionRunner.java:786-789);
+                                 0: each band's own (tests/TestAccelSpeed.java:
+                                 506-514).  Equal for single-band OTFs. */
 } fsim_recon_params;
 
 int fsim_channel_create(const fsim_recon_params *p, fsim_otf otf, fsim_channel *out);

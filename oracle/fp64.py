@@ -66,6 +66,13 @@ def get_att_val(only_att, cpm, cutoff, cycl):
     return np.where(ok, v * f + v * (1 - f), 1.0)
 
 
+def band_table(vals, b):
+    """The radial table of band b: a single-band OTF serves every band
+    (OtfProvider.getOtfVal, OtfProvider.java:192-193)."""
+    vals = np.asarray(vals)
+    return vals[b] if vals.ndim == 2 else vals
+
+
 def coords(w, h):
     """Wrapped, y-flipped frequency coordinates (OtfProvider.java:345-346)."""
     x = np.arange(w)
@@ -108,11 +115,12 @@ def wiener_denominator(w, h, vals, only_att, cpm, cutoff, vec_cpm, px, py, nband
             bx, by = px[d] * b, py[d] * b
             r1 = np.sqrt((xh - bx) ** 2 + (yh - by) ** 2) * vec_cpm
             r2 = np.sqrt((xh + bx) ** 2 + (yh + by) ** 2) * vec_cpm
-            v1 = np.abs(get_otf_val(vals, cpm, cutoff, r1)) ** 2
-            v2 = np.abs(get_otf_val(vals, cpm, cutoff, r2)) ** 2
+            tb, ta = band_table(vals, b), band_table(only_att, b)
+            v1 = np.abs(get_otf_val(tb, cpm, cutoff, r1)) ** 2
+            v2 = np.abs(get_otf_val(tb, cpm, cutoff, r2)) ** 2
             if use_att:
-                v1 = v1 * get_att_val(only_att, cpm, cutoff, r1)
-                v2 = v2 * get_att_val(only_att, cpm, cutoff, r2)
+                v1 = v1 * get_att_val(ta, cpm, cutoff, r1)
+                v2 = v2 * get_att_val(ta, cpm, cutoff, r2)
             acc = acc + v1 + v2
     return 1 / (acc + wparam * wparam)
 
@@ -192,10 +200,17 @@ def build_tables(p, vals):
     as built by ReconstructionRunner.setupFilters (:773-797) and
     TestAccelSpeed (:506-529)."""
     n = p["n"]
-    only_att = val_attenuation(np.arange(len(vals)) * p["otf_cpm"], p["att_str"], p["att_fwhm"])
+    vals = np.asarray(vals)
+    only_att = val_attenuation(np.arange(vals.shape[-1]) * p["otf_cpm"], p["att_str"], p["att_fwhm"])
     vals_att = vals * only_att
-    otf = otf_to_vector(n, n, vals_att if p["use_att"] else vals, p["otf_cpm"],
-                        p["otf_cutoff"], p["vec_cpm"])
+    tabs = vals_att if p["use_att"] else vals
+    if vals.ndim == 2:
+        only_att = np.broadcast_to(only_att, vals.shape)
+        # per band, or band 0's for every component (the live runner)
+        nb = 1 if p.get("otf_band0") else vals.shape[0]
+        otf = [otf_to_vector(n, n, tabs[b], p["otf_cpm"], p["otf_cutoff"], p["vec_cpm"]) for b in range(nb)]
+    else:
+        otf = otf_to_vector(n, n, tabs, p["otf_cpm"], p["otf_cutoff"], p["vec_cpm"])
     wien = wiener_denominator(2 * n, 2 * n, vals, only_att, p["otf_cpm"], p["otf_cutoff"],
                               p["vec_cpm"], p["px"], p["py"], p["nband"], p["use_att"],
                               p["wiener"])
@@ -217,7 +232,8 @@ def recon_fast(p, raw, vals, tables=None, return_full=False):
         spec = [fft2(raw[d, ph] * fade) for ph in range(P)]
         mods = [clamp_mod(m) for m in p["modul"][d][:B]]
         S = separation_matrix(p["phases"][d][:P], B, mods)
-        sep = [sum(S[c, ph] * spec[ph] for ph in range(P)) * np.conj(otf) for c in range(C)]
+        sep = [sum(S[c, ph] * spec[ph] for ph in range(P)) *
+               np.conj(otf[min((c + 1) // 2, len(otf) - 1)] if isinstance(otf, list) else otf) for c in range(C)]
         full += paste_freq(sep[0], 2 * n, 2 * n)
         for b in range(1, B):
             for comp, sgn in ((2 * b, 1.0), (2 * b - 1, -1.0)):

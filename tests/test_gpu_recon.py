@@ -224,3 +224,49 @@ def test_estimate_then_reconstruct_matches_oracle(fsim_lib):
         assert e < TOL
     rb.free()
     ob.free()
+
+
+@pytest.mark.parametrize("band0", [False, True])
+def test_multiband_otf_matches_fp64(fsim_lib, band0, tmp_path):
+    """A measured three-band OTF (per-band radial tables, as an OTF file
+    holds them) through the fused path, with each band's own OTF (the
+    benchmark's order) or band 0's for every component (the live runner),
+    against the FP64 re-run with the same tables; and the OTF read back from
+    its XML form reconstructs bitwise identically."""
+    from fsim import synthetic
+    from fsim.conf import Conf
+    from fsim.sim_algorithm import OtfProvider, Reconstructor
+    n = 128
+    raw, sp = synthetic.sim_stack(n=n, nphase=5, seed=12, nbeads=75, nfil=10, harmonic2=0.3, k_frac=0.45)
+    ests = [OtfProvider.fromEstimate(1.4, 525.0, a) for a in (1.0, 0.7, 0.45)]
+    vals = np.stack([e.band_values()[0] for e in ests])
+    otf = OtfProvider.fromData(1.4, 525.0, ests[0].cpm, vals)
+    otf.setAttenuation(0.995, 1.2)
+    otf.switchAttenuation(True)
+    param = synthetic.make_param(sp, n, nphase=5, nband=3)
+    param.otf(otf)
+    got = Reconstructor(param, otf_band0=band0).reconstruct_host(raw)
+    p = ref.recon_dict(sp, n, nband=3)
+    p["px"] = [param.dir(d).pX for d in range(3)]
+    p["py"] = [param.dir(d).pY for d in range(3)]
+    p["otf_band0"] = band0
+    want = fp64.recon_fast(p, raw, vals.astype(np.complex128))
+    e = fp64.rel_l2(got, want)
+    other = fp64.recon_fast(dict(p, otf_band0=not band0), raw, vals.astype(np.complex128))
+    print(f"band0={band0}: rel L2 vs FP64 {e:.2e}; the other OTF choice differs by {fp64.rel_l2(other, want):.2e}")
+    assert e < TOL and fp64.rel_l2(other, want) > 1e-3
+    # the file keeps the band data exactly; "cycles" is a decimal written with
+    # six significant digits, as OtfProvider.saveConfig writes it
+    c = Conf("fairsim")
+    otf.saveConfig(c.r().mk("ch"))
+    c.saveFile(str(tmp_path / "otf.xml"))
+    loaded = OtfProvider.loadFromConfig(Conf.loadFile(str(tmp_path / "otf.xml")).r().cd("ch"))
+    assert loaded.cpm == float("%8.5e" % otf.cpm)
+    same = OtfProvider.fromData(1.4, 525.0, loaded.cpm, vals)
+    same.setAttenuation(0.995, 1.2)
+    same.switchAttenuation(True)
+    param.otf(loaded)
+    a = Reconstructor(param, otf_band0=band0).reconstruct_host(raw)
+    param.otf(same)
+    assert np.array_equal(a, Reconstructor(param, otf_band0=band0).reconstruct_host(raw))
+    assert fp64.rel_l2(a, got) < 1e-3
