@@ -47,19 +47,33 @@ def profiled_traffic(kernel, n, batch):
     return None
 
 
-def stage_bytes(n, D=3, P=3, C=3, Cs=None):
+def stage_bytes(n, D=3, P=3, C=3, Cs=None, half=False):
     """HBM bytes per stack each pipeline kernel must move.  Cs = stored
     components per direction: C for the reference dataflow (the sum is
     B_alg = 434 n^2 at D = P = C = 3), B when each band's neg component is
-    derived from its pos component."""
+    derived from its pos component.  half: the result spectrum is Hermitian
+    and only its columns 0..n are formed -- half of every contribution lands
+    there (the DC component's n/2 columns; of a pos/neg pair at offset kx,
+    n/2 + kx and n/2 - kx columns) and B is half as wide."""
     Cs = C if Cs is None else Cs
     nn = n * n
     sh = D * (Cs - 1)
+    if half:
+        gather = 4 * D * C * nn + 8 * 2 * nn     # half of each contribution, 2n x n complex out
+        last = 8 * 2 * nn + 4 * 4 * nn           # 2n x n complex in, (2n)^2 real out
+    else:
+        gather = 8 * D * C * nn + 8 * 4 * nn     # every contribution read once, (2n)^2 complex out
+        last = 8 * 4 * nn + 4 * 4 * nn           # (2n)^2 complex in, real out
     return [2 * D * P * nn + 8 * D * Cs * nn,    # raw u16 in, stored components out
             16 * sh * nn,                        # shifted components in + out
             16 * sh * nn,
-            8 * D * C * nn + 8 * 4 * nn,         # every contribution read once, (2n)^2 complex out
-            8 * 4 * nn + 4 * 4 * nn]             # (2n)^2 complex in, real out
+            gather, last]
+
+
+def stage_names(half):
+    """Kernel of each pipeline stage: the last pass is k_row_c2r when the
+    result spectrum is Hermitian, k_row_real otherwise."""
+    return STAGES[:4] + (["k_row_c2r"] if half else ["k_row_real"])
 
 
 def log(msg):
@@ -670,13 +684,15 @@ def main():
     value = B * args.steps / elapsed
     log(f"batched: {value:.1f} recons/s, {ms_per_step:.3f} ms/step")
 
-    cs = 2 if rec.p.mode == 0 else 3          # real estimated OTF: the pairing is used in mode 0
-    sb = stage_bytes(n, Cs=cs)
+    half = rec.p.mode == 0                    # real estimated OTF: the pairing is used in mode 0
+    cs = 2 if half else 3
+    sb = stage_bytes(n, Cs=cs, half=half)
+    names = stage_names(half)
     stage_ms = [ms[i] / max(cnt[i], 1) for i in range(5)]          # per launch (= per batch)
     dom = int(np.argmax(stage_ms))
     dom_bytes = sb[dom] * B
     achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
-    stages = {STAGES[i]: {"ms_per_launch": round(stage_ms[i], 4),
+    stages = {names[i]: {"ms_per_launch": round(stage_ms[i], 4),
                           "GBps": round(sb[i] * B / (stage_ms[i] * 1e-3) / 1e9, 1) if stage_ms[i] else None}
               for i in range(5)}
     b_alg = sum(stage_bytes(n))               # reference dataflow: 434 n^2
@@ -769,9 +785,9 @@ def main():
         "config": {"workload": f"fast-shift 2D SIM reconstruction, {n}^2 x 3 angles x 3 phases uint16 -> "
                                f"{2 * n}^2 float32, batched {B} stacks per step",
                    "n": n, "angles": 3, "phases": 3, "bands": 2, "batch": B},
-        "roofline": {"bound": "hbm", "kernel": STAGES[dom], "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(STAGES[dom], n, B)},
+                     "traffic": profiled_traffic(names[dom], n, B)},
         "pipeline_roofline_frac": round(pipeline_frac, 4),
         "alg_bytes_per_recon": b_alg,
         "bytes_moved_per_recon": sum(sb),

@@ -14,7 +14,8 @@ struct ReconConst {
     double otf_cpm, cutoff, vec_cpm;
     const float2 *otf_tab;            // vals or valsAtt, otf_nb bands x ns
     const float2 *otf2d;              // otfToVector(band, 0, 0) on the n^2 grid, per OTF band
-    const float2 *wa;                 // (2n)^2 of (1/Wiener denominator, apodization)
+    const float *wa;                  // (2n)^2 of (1/Wiener denominator) * apodization, tile-major:
+                                      // pixel (x, y) at ((x / cw) * 2n + y) * cw + x % cw, cw = gather_cw(n)
     const double *fade_lo, *fade_hi;  // fadeBorderCos factors, n each
     int fade_px;                      // border width; factors are 1.0 inside
     const float2 *ramp;               // [kMaxDir][kMaxComp][x|y][n] sub-pixel ramp factors
@@ -29,6 +30,10 @@ struct ReconConst {
     int srow[kMaxComp], sband[kMaxComp], sramp[kMaxComp], src[kMaxComp];
     int mir[kMaxDir][kMaxComp], mdx[kMaxDir][kMaxComp], mdy[kMaxDir][kMaxComp];
     double fx[kMaxDir][kMaxComp], fy[kMaxDir][kMaxComp];    // sub-pixel ramp shifts
+    // half: the m x m result spectrum is Hermitian (real raw data, real OTF,
+    // the Hermitian pairing), so the gather forms columns 0..n of it and the
+    // last pass is a half-length complex-to-real row transform.
+    int half;
 };
 
 // One stack: channel constants, raw input, work buffers and output.
@@ -51,7 +56,8 @@ struct JobList {
 };
 
 bool recon_size_ok(int n);
-int launch_pipeline(int n, const JobList &jl, int njobs, int D, int Cs, hipStream_t s);
+int gather_cw(int n);   // column tile width of the gather at raw side n
+int launch_pipeline(int n, const JobList &jl, int njobs, int D, int Cs, bool half, hipStream_t s);
 size_t workspace_a_elems(int n, int D, int C);
 size_t workspace_b_elems(int n);
 

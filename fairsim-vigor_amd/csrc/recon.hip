@@ -55,6 +55,9 @@
 #ifndef FSIM_COL_WPE
 #define FSIM_COL_WPE 6
 #endif
+#ifndef FSIM_B_TILED
+#define FSIM_B_TILED 1
+#endif
 
 namespace fsim {
 
@@ -460,15 +463,44 @@ __device__ __forceinline__ bool tile_pasted(int x0, int cw, int off) {
     return hit;
 }
 
+// Hermitian result.  The reference keeps the real part of the inverse
+// transform of the full spectrum S, i.e. transforms the Hermitian part
+// (S(k) + conj S(-k)) / 2.  The contributions come in conjugate-mirror pairs
+// (V_partner(k) = conj V_c(-k); the DC component is its own partner), so
+// that part is sum_c w_c(k) V_c(k) with w_c(k) = ([k - K_c in W] +
+// [-k - K_partner in W]) / 2, W the paste window [-n/2, n/2)^2: weight 1
+// inside both windows, 1/2 on the edge pixels (the window is asymmetric, and
+// the sub-pixel shift leaves values there), 0 outside.  Partner of
+// contribution c: c itself for DC, else the other member of its band's
+// (neg = 2b - 1, pos = 2b) pair.
+__device__ __forceinline__ int partner_comp(int c) { return c == 0 ? 0 : ((c & 1) ? c + 1 : c - 1); }
+
+// Does any column of [x0, x0 + cw) lie in the mirrored window (-x - off)?
 template <int N>
+__device__ __forceinline__ bool tile_pasted_mirror(int x0, int cw, int off) {
+    bool hit = false;
+    for (int q = 0; q < cw; q++) hit |= paste_ok<N>(paste_p<N>(-(x0 + q), off));
+    return hit;
+}
+
+// Column tiles of the gather: all m / cw, or with a Hermitian result the
+// n / cw tiles of columns [0, n) and one more whose first column is n (the
+// Nyquist column; the rest of that tile is formed but not read).
+template <int N, bool HALF>
+struct GatherTiles {
+    static constexpr int value = HALF ? N / GatherE<N>::cw + 1 : 2 * N / GatherE<N>::cw;
+};
+
+template <int N, bool HALF>
 __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::wpe) k_gather(JobList jl, int njobs, const float2 *__restrict__ twn,
                                                  const float2 *__restrict__ twm) {
     constexpr int M = 2 * N;
     constexpr int EN = GatherE<N>::en, TN = N / EN;
     constexpr int EM = GatherE<N>::em, TM = M / EM;
     constexpr int CW = GatherE<N>::cw;
+    static_assert(!HALF || N + CW <= M, "the Nyquist tile must fit the row");
     __shared__ float2 lds[GatherE<N>::lds];
-    constexpr int tiles = M / CW;
+    constexpr int tiles = GatherTiles<N, HALF>::value;
     int job, tile;
     if (!gather_block(tiles, njobs, job, tile)) return;
     // one table of length m serves both transforms (the n-point one at
@@ -486,41 +518,61 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
 #pragma unroll
     for (int m = 0; m < EM; m++) acc[m] = make_float2(0.f, 0.f);
     const size_t nn = (size_t)N * N;
-    // Contributions in the reference's order: directions, then components.
-    for (int d = 0; d < D; d++) {
-        // DC component (rows transformed): column FFT, * conj(OTF), staged
-        // through LDS for the row gather.  Skipped as a whole when the tile
-        // lies outside its paste window.
-        if (tile_pasted<N>(tile * CW, CW, K.offx[d][0])) {
-            const int px = paste_p<N>(xo, K.offx[d][0]);
-            const int xs = px & (N - 1);
-            const bool act = (t < TN) && paste_ok<N>(px);
-            float2 v[EN];
-            if (act) {
+    // DC components of every direction: all are pasted at offset 0 and
+    // filtered with the band-0 OTF, so their sum (formed on the
+    // row-transformed data) takes one column FFT and one OTF product, staged
+    // through LDS for the row gather.  Skipped as a whole when the tile lies
+    // outside the paste window.
+    if (tile_pasted<N>(tile * CW, CW, 0) || (HALF && tile_pasted_mirror<N>(tile * CW, CW, 0))) {
+        const int px = paste_p<N>(xo, 0);
+        const int xs = px & (N - 1);
+        const bool okx = paste_ok<N>(px);
+        const bool okx2 = HALF && paste_ok<N>(paste_p<N>(-xo, 0));
+        const bool act = (t < TN) && (okx || okx2);
+        float2 v[EN];
+#pragma unroll
+        for (int m = 0; m < EN; m++) v[m] = make_float2(0.f, 0.f);
+        if (act) {
+            for (int d = 0; d < D; d++) {
                 const FSIM_GLB float2 *col = A + (size_t)(d * Cs + K.src[0]) * nn + xs;
+                float2 u[EN];
 #pragma unroll
-                for (int m = 0; m < EN; m++) v[m] = ld(col + (size_t)(t + TN * m) * N);
-            }
-            fft_regs<N, EN, false, false, TW::tl>(v, lds + cl, CW, t, tws, act);
-            if (act) {
-                const FSIM_GLB float2 *otf = glb(K.otf2d) + xs;
+                for (int m = 0; m < EN; m++) u[m] = ld(col + (size_t)(t + TN * m) * N);
 #pragma unroll
-                for (int m = 0; m < EN; m++) {
-                    v[m] = times_conj_java(v[m], ld(otf + (size_t)(t + TN * m) * N));
-                    lds[(t + TN * m) * CW + cl] = v[m];
-                }
+                for (int m = 0; m < EN; m++) v[m] = cadd(v[m], u[m]);
             }
-            __syncthreads();
-            if (t < TM && paste_ok<N>(px)) {
-                const int oy = K.offy[d][0];
+        }
+        fft_regs<N, EN, false, false, TW::tl>(v, lds + cl, CW, t, tws, act);
+        if (act) {
+            const FSIM_GLB float2 *otf = glb(K.otf2d) + xs;
+#pragma unroll
+            for (int m = 0; m < EN; m++) {
+                v[m] = times_conj_java(v[m], ld(otf + (size_t)(t + TN * m) * N));
+                lds[(t + TN * m) * CW + cl] = v[m];
+            }
+        }
+        __syncthreads();
+        if constexpr (HALF) {
+            if (t < TM && (okx || okx2)) {
 #pragma unroll
                 for (int m = 0; m < EM; m++) {
-                    const int py = paste_p<N>(t + TM * m, oy);
-                    if (paste_ok<N>(py)) acc[m] = cadd(acc[m], lds[(py & (N - 1)) * CW + cl]);
+                    const int y = t + TM * m;
+                    const int py = paste_p<N>(y, 0);
+                    const int r = (okx && paste_ok<N>(py)) + (okx2 && paste_ok<N>(paste_p<N>(-y, 0)));
+                    if (r) acc[m] = cadd(acc[m], cscale(lds[(py & (N - 1)) * CW + cl], 0.5f * (float)r));
                 }
             }
-            __syncthreads();
+        } else if (t < TM && okx) {
+#pragma unroll
+            for (int m = 0; m < EM; m++) {
+                const int py = paste_p<N>(t + TM * m, 0);
+                if (paste_ok<N>(py)) acc[m] = cadd(acc[m], lds[(py & (N - 1)) * CW + cl]);
+            }
         }
+        __syncthreads();
+    }
+    // shifted contributions, directions then components
+    for (int d = 0; d < D; d++) {
         // shifted components: full spectra in A, read in place; a mirrored
         // (neg) contribution reads its pos partner at the mirrored index and
         // conjugates.  Loads first (skipped where the row is not pasted; the
@@ -528,15 +580,43 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
         // contribution are in flight together.
         for (int c = 1; c < C; c++) {
             const int offx = K.offx[d][c];
-            if (!tile_pasted<N>(tile * CW, CW, offx)) continue;
+            const int pc = partner_comp(c);
+            const int offx2 = K.offx[d][pc], offy2 = K.offy[d][pc];
+            if (!tile_pasted<N>(tile * CW, CW, offx) && !(HALF && tile_pasted_mirror<N>(tile * CW, CW, offx2)))
+                continue;
             const int px = paste_p<N>(xo, offx);
             const bool colok = (t < TM) && paste_ok<N>(px);
+            const bool colok2 = HALF && (t < TM) && paste_ok<N>(paste_p<N>(-xo, offx2));
             const int mir = K.mir[d][c];
             const int oy = K.offy[d][c], my = K.mdy[d][c];
             int xs = px & (N - 1);
             if (mir) xs = (K.mdx[d][c] - xs) & (N - 1);
             const FSIM_GLB float2 *comp = A + (size_t)(d * Cs + K.src[c]) * nn + xs;
             const float sg = mir ? -1.0f : 1.0f;
+            if constexpr (HALF) {
+#pragma unroll
+                for (int h = 0; h < EM; h += EM / 2) {
+                    float2 sv[EM / 2];
+                    float wt[EM / 2];
+#pragma unroll
+                    for (int i = 0; i < EM / 2; i++) {
+                        sv[i] = make_float2(0.f, 0.f);
+                        const int y = t + TM * (h + i);
+                        const int py = paste_p<N>(y, oy);
+                        int ys = py & (N - 1);
+                        if (mir) ys = (my - ys) & (N - 1);
+                        const int r = (colok && paste_ok<N>(py)) + (colok2 && paste_ok<N>(paste_p<N>(-y, offy2)));
+                        wt[i] = 0.5f * (float)r;
+                        if (r) sv[i] = ld(comp + (size_t)ys * N);
+                    }
+#pragma unroll
+                    for (int i = 0; i < EM / 2; i++) {
+                        acc[h + i].x += sv[i].x * wt[i];
+                        acc[h + i].y += sg * sv[i].y * wt[i];
+                    }
+                }
+                continue;
+            }
             // two halves of EM / 2 loads each bound the registers in flight
 #pragma unroll
             for (int h = 0; h < EM; h += EM / 2) {
@@ -557,22 +637,29 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
             }
         }
     }
-    // fullResult.times(wienerDenom); fullResult.times(apo)
+    // fullResult.times(wienerDenom); fullResult.times(apo), as one product
+    // table (tile-major: the tile's cw columns of a row are adjacent)
     if (t < TM) {
-        const FSIM_GLB float2 *wa = glb(K.wa) + xo;
+        const FSIM_GLB float *wa = glb(K.wa) + (size_t)tile * M * CW + cl;
+        float w[EM];
 #pragma unroll
-        for (int m = 0; m < EM; m++) {
-            const float2 w = ld(wa + (size_t)(t + TM * m) * M);
-            acc[m] = make_float2(acc[m].x * w.x, acc[m].y * w.x);
-            acc[m] = make_float2(acc[m].x * w.y, acc[m].y * w.y);
-        }
+        for (int m = 0; m < EM; m++) w[m] = wa[(size_t)(t + TM * m) * CW];
+#pragma unroll
+        for (int m = 0; m < EM; m++) acc[m] = cscale(acc[m], w[m]);
     }
     fft_regs<M, EM, true, false, TW::tl>(acc, lds + cl, CW, t, tws, t < TM);
     if (t < TM) {
         const float sc = 1.0f / M;
-        FSIM_GLB float2 *B = glb(J.B) + xo;
+        if constexpr (HALF && FSIM_B_TILED) {
+            // tile-major: the tile's rows are one contiguous run
+            FSIM_GLB float2 *B = glb(J.B) + (size_t)tile * M * CW + cl;
 #pragma unroll
-        for (int m = 0; m < EM; m++) st(B + (size_t)(t + TM * m) * M, cscale(acc[m], sc));
+            for (int m = 0; m < EM; m++) st(B + (size_t)(t + TM * m) * CW, cscale(acc[m], sc));
+        } else {
+            FSIM_GLB float2 *B = glb(J.B) + xo;
+#pragma unroll
+            for (int m = 0; m < EM; m++) st(B + (size_t)(t + TM * m) * M, cscale(acc[m], sc));
+        }
     }
 }
 
@@ -601,11 +688,117 @@ __global__ void __launch_bounds__(256) k_row_real(JobList jl, const float2 *__re
     for (int m = 0; m < E; m++) o[t + T * m] = v[m].x * sc;
 }
 
-// pack Wiener reciprocal and apodization into one (w, a) table
-__global__ void k_pack_wa(float2 *__restrict__ wa, const float *__restrict__ w, const float *__restrict__ a,
-                          size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        wa[i] = make_float2(w[i], a[i]);
+// K6 for a Hermitian result: B holds columns 0..n of the m-point row
+// spectra X (X[m - k] = conj(X[k])).  The m real outputs of a row come from
+// one n-point inverse transform: with E[k] = X[k] + conj(X[n-k]) and
+// O[k] = (X[k] - conj(X[n-k])) exp(2 pi i k / m), k < n, the inverse
+// transform of E + iO is x[2j] + i x[2j+1] (the even and odd samples fold
+// the m-point sum at k and k + n, and X[k + n] = conj(X[n - k])).  X[0] and
+// X[n] enter with their real parts: the transform is that of the Hermitian
+// extension, whose real part the full-width pass would keep.
+// B is tile-major as the half gather writes it: element (k, y) at
+// ((k / cw) * m + y) * cw + k % cw, so the RW rows of a block are, per tile,
+// one contiguous run of RW * cw elements.  The block loads those runs with
+// all its threads (whole cache lines per wave) into a row-major LDS copy,
+// from which each row's threads take X[k] and X[n - k].  Blocks walk the
+// rows in XCD-contiguous ranges (the grid's x extent is a multiple of 8),
+// so rows that share cache lines are read on one XCD.
+template <int N>
+__global__ void __launch_bounds__(256) k_row_c2r(JobList jl, const float2 *__restrict__ twm) {
+    constexpr int M = 2 * N;
+    constexpr int E = FftE<N>::value, T = N / E;
+    constexpr int RW = T >= 256 ? 1 : 256 / T;
+    constexpr int LP = FftLds<N, E>::len;
+    constexpr int CW = GatherE<N>::cw;
+    constexpr int NT = T * RW;
+    __shared__ float2 lds[RW * LP];
+    using TW = TwLds<N, false>;
+    const Job J = get_job(jl, blockIdx.y);
+    const int t = threadIdx.x % T, rr = threadIdx.x / T;
+    const int grp = (M / RW) % 8 ? (int)blockIdx.x : xcd_linear(M / RW);
+    const int y = grp * RW + rr;
+    const FSIM_GLB float2 *tg = glb(twm);
+    float2 a[E], b[E], w[E];
+    if constexpr (FSIM_B_TILED) {
+        // row-major LDS copy of columns [0, n + cw) of the block's rows
+        constexpr int RL = N + CW + 1;              // odd row pitch
+        constexpr int RUN = RW * CW;                // elements per tile run
+        constexpr int TOT = (N / CW + 1) * RUN;
+        constexpr int PER = (TOT + NT - 1) / NT;
+        __shared__ float2 rows[RW * RL];
+        const FSIM_GLB float2 *bt = glb(J.B) + (size_t)grp * RUN;
+        float2 u[PER];
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int e = (int)threadIdx.x + NT * r;
+            if (TOT % NT == 0 || e < TOT) u[r] = ld(bt + (size_t)(e / RUN) * M * CW + e % RUN);
+        }
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int e = (int)threadIdx.x + NT * r;
+            if (TOT % NT == 0 || e < TOT) {
+                const int q = e % RUN;
+                rows[(q / CW) * RL + (e / RUN) * CW + q % CW] = u[r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int k = t + T * m;
+            a[m] = rows[rr * RL + k];
+            b[m] = rows[rr * RL + N - k];
+            w[m] = ld(tg + k);   // exp(-2 pi i k / m)
+        }
+    } else {
+        const FSIM_GLB float2 *row = glb(J.B) + (size_t)y * M;
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int k = t + T * m;
+            a[m] = ld(row + k);
+            b[m] = ld(row + (N - k));
+            w[m] = ld(tg + k);   // exp(-2 pi i k / m)
+        }
+    }
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        float2 x = a[m], z = b[m];
+        if (t + T * m == 0) {
+            x.y = 0.f;
+            z.y = 0.f;
+        }
+        const float2 e = make_float2(x.x + z.x, x.y - z.y);
+        const float2 d = make_float2(x.x - z.x, x.y + z.y);
+        const float2 o = make_float2(d.x * w[m].x + d.y * w[m].y, d.y * w[m].x - d.x * w[m].y);
+        v[m] = make_float2(e.x - o.y, e.y + o.x);
+    }
+    fft_regs<N, E, true, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, twm);
+    const float sc = 1.0f / M;
+    FSIM_GLB float2 *o = reinterpret_cast<FSIM_GLB float2 *>(glb(J.out) + (size_t)y * M);
+#pragma unroll
+    for (int m = 0; m < E; m++) st(o + t + T * m, cscale(v[m], sc));
+}
+
+// Tile-major position of the tables the gather reads: element j of an
+// m x m table with tile width cw holds pixel (x, y).
+__device__ __forceinline__ void tile_major_xy(size_t j, int M, int cw, int &x, int &y) {
+    x = (int)(j / ((size_t)M * cw)) * cw + (int)(j % cw);
+    y = (int)((j / cw) % M);
+}
+
+// (1/Wiener denominator) * apodization, and the apodization alone (for the
+// per-stack tables), from row-major w and a into the tile-major layout
+__global__ void k_pack_wa(float *__restrict__ wa, float *__restrict__ apo, const float *__restrict__ w,
+                          const float *__restrict__ a, int M, int cw) {
+#pragma clang fp contract(off)
+    const size_t mm = (size_t)M * M;
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < mm; j += (size_t)gridDim.x * blockDim.x) {
+        int x, y;
+        tile_major_xy(j, M, cw, x, y);
+        const size_t i = (size_t)y * M + x;
+        wa[j] = w[i] * a[i];
+        apo[j] = a[i];
+    }
 }
 
 // Sub-pixel ramp tables exp(2 pi i f t / n), t < n, of the x and y
@@ -622,10 +815,11 @@ __global__ void k_ramps(float2 *__restrict__ ramp, const double *__restrict__ f,
     }
 }
 
-// Per-stack (1 / Wiener denominator, apodization) tables: stack s uses its
-// own shifts sh[s] and the channel's apodization (wa0.y).  Grid (pixels,
-// stacks).
-__global__ void k_wiener_multi(float2 *__restrict__ wa, const float2 *__restrict__ wa0, int M,
+// Per-stack (1 / Wiener denominator) * apodization tables, tile-major like
+// the channel's: stack s uses its own shifts sh[s] and the channel's
+// apodization apo.  Only the first `len` elements (the tiles the gather
+// reads) are formed.  Grid (pixels, stacks).
+__global__ void k_wiener_multi(float *__restrict__ wa, const float *__restrict__ apo, int M, int cw, size_t len,
                                const ShiftSet *__restrict__ sh, const float2 *__restrict__ vals,
                                const float *__restrict__ only_att, int ns, int otf_nb, double cpm, double cutoff,
                                double vec_cpm, int use_att, float w2) {
@@ -633,8 +827,9 @@ __global__ void k_wiener_multi(float2 *__restrict__ wa, const float2 *__restrict
     const size_t mm = (size_t)M * M;
     const int st = blockIdx.y;
     const ShiftSet S = sh[st];
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < mm; i += (size_t)gridDim.x * blockDim.x) {
-        const int x = (int)(i % M), y = (int)(i / M);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x) {
+        int x, y;
+        tile_major_xy(i, M, cw, x, y);
         const double xh = xh_of(x, M), yh = yh_of(y, M);
         // band 0 (px(0) = 0): both distances are |(xh, yh)| in every
         // direction, so one value serves all of them (the same operands as
@@ -665,7 +860,7 @@ __global__ void k_wiener_multi(float2 *__restrict__ wa, const float2 *__restrict
                 acc = acc + v1 + v2;
             }
         }
-        wa[(size_t)st * mm + i] = make_float2(1 / (acc + w2), wa0[i].y);
+        wa[(size_t)st * mm + i] = (1 / (acc + w2)) * apo[i];
     }
 }
 
@@ -691,7 +886,7 @@ static void stage_mark(int stage, hipStream_t s, hipEvent_t &start) {
 }
 
 template <int N>
-static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, hipStream_t s) {
+static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, bool half, hipStream_t s) {
     // C: stored components per direction
     constexpr int M = 2 * N;
     const float2 *twn = twiddles(N), *twm = twiddles(M);
@@ -723,11 +918,19 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, hipStre
     }
     {
         constexpr int TK = GatherE<N>::tk, CW = GatherE<N>::cw;
-        hipLaunchKernelGGL(k_gather<N>, dim3(gather_grid(M / CW, njobs)), dim3(TK * CW), 0, s, jl, njobs, twn,
-                           twm);
+        if (half)
+            hipLaunchKernelGGL((k_gather<N, true>), dim3(gather_grid(GatherTiles<N, true>::value, njobs)),
+                               dim3(TK * CW), 0, s, jl, njobs, twn, twm);
+        else
+            hipLaunchKernelGGL((k_gather<N, false>), dim3(gather_grid(GatherTiles<N, false>::value, njobs)),
+                               dim3(TK * CW), 0, s, jl, njobs, twn, twm);
         stage_mark(3, s, ev);
     }
-    {
+    if (half) {
+        constexpr int E = FftE<N>::value, T = N / E, RW = T >= 256 ? 1 : 256 / T;
+        hipLaunchKernelGGL(k_row_c2r<N>, dim3(M / RW, njobs), dim3(T * RW), 0, s, jl, twm);
+        stage_mark(4, s, ev);
+    } else {
         constexpr int E = FftE<M>::value, T = M / E, RW = T >= 256 ? 1 : 256 / T;
         hipLaunchKernelGGL(k_row_real<M>, dim3(M / RW, njobs), dim3(T * RW), 0, s, jl, twm);
         stage_mark(4, s, ev);
@@ -738,15 +941,27 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, hipStre
 
 bool recon_size_ok(int n) { return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048; }
 
-int launch_pipeline(int n, const JobList &jl, int njobs, int D, int C, hipStream_t s) {
+int gather_cw(int n) {
+    switch (n) {
+        case 64: return GatherE<64>::cw;
+        case 128: return GatherE<128>::cw;
+        case 256: return GatherE<256>::cw;
+        case 512: return GatherE<512>::cw;
+        case 1024: return GatherE<1024>::cw;
+        case 2048: return GatherE<2048>::cw;
+        default: return 1;
+    }
+}
+
+int launch_pipeline(int n, const JobList &jl, int njobs, int D, int C, bool half, hipStream_t s) {
     // C: stored components per direction (ReconConst::Cs)
     switch (n) {
-        case 64: return launch_pipeline_t<64>(jl, njobs, D, C, s);
-        case 128: return launch_pipeline_t<128>(jl, njobs, D, C, s);
-        case 256: return launch_pipeline_t<256>(jl, njobs, D, C, s);
-        case 512: return launch_pipeline_t<512>(jl, njobs, D, C, s);
-        case 1024: return launch_pipeline_t<1024>(jl, njobs, D, C, s);
-        case 2048: return launch_pipeline_t<2048>(jl, njobs, D, C, s);
+        case 64: return launch_pipeline_t<64>(jl, njobs, D, C, half, s);
+        case 128: return launch_pipeline_t<128>(jl, njobs, D, C, half, s);
+        case 256: return launch_pipeline_t<256>(jl, njobs, D, C, half, s);
+        case 512: return launch_pipeline_t<512>(jl, njobs, D, C, half, s);
+        case 1024: return launch_pipeline_t<1024>(jl, njobs, D, C, half, s);
+        case 2048: return launch_pipeline_t<2048>(jl, njobs, D, C, half, s);
         default: return fail(FSIM_ERR_ARG, "reconstruct: n must be a power of two in [64, 2048]");
     }
 }
@@ -779,6 +994,7 @@ int build_channel_const(const fsim_recon_params &p, const Otf &o, bool sym, Reco
     K.vec_cpm = p.vec_cpm;
     K.otf_tab = p.use_att ? o.vals_att : o.vals;
     K.Cs = sym ? B : C;
+    K.half = sym ? 1 : 0;
     for (int j = 0; j < kMaxComp; j++) K.srow[j] = K.sband[j] = K.sramp[j] = K.src[j] = 0;
     for (int j = 0; j < K.Cs; j++) {
         K.srow[j] = sym ? 2 * j : j;          // DC, then pos of band j
@@ -834,7 +1050,8 @@ using namespace fsim;
 // (fsim_reconstruct_params), grown on demand.
 struct MultiWork {
     ReconConst *dK = nullptr;
-    float2 *wa = nullptr, *ramp = nullptr;
+    float *wa = nullptr;
+    float2 *ramp = nullptr;
     double *f = nullptr;
     ShiftSet *sh = nullptr;
     Job *jobs = nullptr;
@@ -852,7 +1069,8 @@ struct fsim_channel_s {
     bool sym = false;
     ReconConst K;
     ReconConst *dK = nullptr;
-    float2 *wa = nullptr;
+    float *wa = nullptr;    // (1/Wiener denominator) * apodization, tile-major
+    float *apo = nullptr;   // apodization, tile-major
     float2 *otf2d = nullptr;
     float2 *ramp = nullptr;
     double *fade = nullptr;
@@ -867,6 +1085,7 @@ const ReconConst &fsim::channel_host_const(const fsim_channel_s *ch) { return ch
 static void channel_free(fsim_channel_s *ch) {
     (void)hipFree(ch->dK);
     (void)hipFree(ch->wa);
+    (void)hipFree(ch->apo);
     (void)hipFree(ch->otf2d);
     (void)hipFree(ch->ramp);
     (void)hipFree(ch->fade);
@@ -950,7 +1169,8 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     const size_t mm = (size_t)M * M;
     float *wtmp = nullptr, *atmp = nullptr;
     double *fdev = nullptr;
-    if (hipMalloc(&ch->dK, sizeof(ReconConst)) != hipSuccess || hipMalloc(&ch->wa, mm * sizeof(float2)) != hipSuccess ||
+    if (hipMalloc(&ch->dK, sizeof(ReconConst)) != hipSuccess || hipMalloc(&ch->wa, mm * sizeof(float)) != hipSuccess ||
+        hipMalloc(&ch->apo, mm * sizeof(float)) != hipSuccess ||
         hipMalloc(&ch->otf2d, (size_t)o.nband * n * n * sizeof(float2)) != hipSuccess ||
         hipMalloc(&ch->fade, 2 * n * sizeof(double)) != hipSuccess ||
         hipMalloc(&ch->ramp, (size_t)kMaxDir * kMaxComp * 2 * n * sizeof(float2)) != hipSuccess ||
@@ -1001,7 +1221,7 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     if (rc) return rc;
     rc = write_apo_device(atmp, M, M, p->vec_cpm, o.cutoff, p->apo_bend, p->apo_cut, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pack_wa, dim3(1024), dim3(256), 0, s, ch->wa, wtmp, atmp, mm);
+    hipLaunchKernelGGL(k_pack_wa, dim3(1024), dim3(256), 0, s, ch->wa, ch->apo, wtmp, atmp, M, gather_cw(n));
     FSIM_HIP(hipGetLastError());
     FSIM_HIP(hipMemcpyAsync(ch->dK, &ch->K, sizeof(ReconConst), hipMemcpyHostToDevice, s));
     FSIM_HIP(hipStreamSynchronize(s));
@@ -1041,7 +1261,10 @@ int fsim_reconstruct(fsim_channel ch, const uint16_t *raw, float *out, int count
     jl.b_stride = workspace_b_elems(n);
     jl.out_stride = (size_t)4 * n * n;
     jl.raw_aligned = ((uintptr_t)raw & 7) == 0;
-    return launch_pipeline(n, jl, count, K.D, K.Cs, s);
+    // the half-spectrum pass writes output pixel pairs (8-byte stores); an
+    // output that is only 4-byte aligned takes the full-width pass
+    const bool half = K.half && ((uintptr_t)out & 7) == 0;
+    return launch_pipeline(n, jl, count, K.D, K.Cs, half, s);
 }
 
 int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw, float *out, int count, const fsim_sim_params *sp,
@@ -1092,7 +1315,7 @@ int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw, float *out, in
         w = MultiWork{};
         w.uploaded = ev;
         if (hipMalloc(&w.dK, count * sizeof(ReconConst)) != hipSuccess ||
-            hipMalloc(&w.wa, count * mm * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&w.wa, count * mm * sizeof(float)) != hipSuccess ||
             hipMalloc(&w.ramp, count * rper * sizeof(float2)) != hipSuccess ||
             hipMalloc(&w.f, (size_t)count * kMaxDir * kMaxComp * 2 * sizeof(double)) != hipSuccess ||
             hipMalloc(&w.sh, count * sizeof(ShiftSet)) != hipSuccess ||
@@ -1132,14 +1355,18 @@ int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw, float *out, in
     if (!w.uploaded) FSIM_HIP(hipEventCreateWithFlags(&w.uploaded, hipEventDisableTiming));
     FSIM_HIP(hipEventRecord(w.uploaded, s));
     const Otf &o = ch->o;
-    hipLaunchKernelGGL(k_wiener_multi, dim3((unsigned)std::min<size_t>((mm + 255) / 256, 2048), count), dim3(256), 0,
-                       s, w.wa, ch->wa, M, w.sh, o.vals, o.only_att, o.ns, o.nband, o.cpm, o.cutoff, ch->p.vec_cpm,
-                       ch->p.use_att, (float)(ch->p.wiener * ch->p.wiener));
+    const bool half = K0.half && ((uintptr_t)out & 7) == 0;
+    const int cw = gather_cw(n);
+    // the half gather reads the tiles of columns [0, n + cw)
+    const size_t len = half ? (size_t)(n + cw) * M : mm;
+    hipLaunchKernelGGL(k_wiener_multi, dim3((unsigned)std::min<size_t>((len + 255) / 256, 2048), count), dim3(256),
+                       0, s, w.wa, ch->apo, M, cw, len, w.sh, o.vals, o.only_att, o.ns, o.nband, o.cpm, o.cutoff,
+                       ch->p.vec_cpm, ch->p.use_att, (float)(ch->p.wiener * ch->p.wiener));
     FSIM_HIP(hipGetLastError());
     JobList jl{};
     jl.jobs = w.jobs;
     jl.raw_aligned = aligned;
-    return launch_pipeline(n, jl, count, K0.D, K0.Cs, s);
+    return launch_pipeline(n, jl, count, K0.D, K0.Cs, half, s);
 }
 
 int fsim_stage_timing(int enable) {

@@ -42,8 +42,9 @@ struct Request {
 
 struct ClassKey {
     int n, D, P, B, Cs;   // Cs: stored components per direction
+    int half;             // Hermitian result (ReconConst::half)
     bool operator==(const ClassKey &o) const {
-        return n == o.n && D == o.D && P == o.P && B == o.B && Cs == o.Cs;
+        return n == o.n && D == o.D && P == o.P && B == o.B && Cs == o.Cs && half == o.half;
     }
 };
 
@@ -162,7 +163,7 @@ struct Server {
 
     ClassKey key_of(const Request *r) const {
         const ReconConst &K = channel_host_const(channels[r->chan]);
-        return ClassKey{K.n, K.D, K.P, K.B, K.Cs};
+        return ClassKey{K.n, K.D, K.P, K.B, K.Cs, K.half};
     }
 
     void launch(Slot *s) {
@@ -194,8 +195,13 @@ struct Server {
         JobList jl{};
         jl.jobs = s->jobs_d;
         jl.raw_aligned = true;
-        for (int i = 0; i < count; i++) jl.raw_aligned &= ((uintptr_t)s->jobs_h[i].raw & 7) == 0;
-        if (!rc) rc = launch_pipeline(k.n, jl, count, k.D, C, s->stream);
+        // the half-spectrum pass writes output pixel pairs (8-byte stores)
+        bool half = k.half != 0;
+        for (int i = 0; i < count; i++) {
+            jl.raw_aligned &= ((uintptr_t)s->jobs_h[i].raw & 7) == 0;
+            half &= ((uintptr_t)s->jobs_h[i].out & 7) == 0;
+        }
+        if (!rc) rc = launch_pipeline(k.n, jl, count, k.D, C, half, s->stream);
         for (int i = 0; i < count && !rc; i++) {
             Request *r = s->reqs[i];
             if (!r->device &&

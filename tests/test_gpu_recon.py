@@ -146,6 +146,55 @@ def test_unaligned_raw_pointer_matches_aligned(fsim_lib):
     assert np.array_equal(got, aligned)
 
 
+def test_half_spectrum_nyquist_column(fsim_lib):
+    """With a real OTF the result spectrum is Hermitian and only its columns
+    0..m/2 are formed, followed by a half-length complex-to-real row pass.
+    Shift vectors longer than n/2 along x paste contributions into the
+    Nyquist column m/2 (and, mirrored, across it); the image still matches
+    the FP32 oracle and the full-width computation (mode 1)."""
+    from fsim import synthetic
+    from fsim.sim_algorithm import Reconstructor
+    n = 64
+    raw, sp = synthetic.sim_stack(n=n, seed=83, nbeads=40, nfil=6)
+    sp = dict(sp)
+    sp["px"] = [0.56 * n, -0.61 * n, 0.3 * n]
+    sp["py"] = [0.05 * n, 0.1 * n, -0.52 * n]
+    param = synthetic.make_param(sp, n)
+    a = Reconstructor(param, mode=0).reconstruct_host(raw)
+    b = Reconstructor(param, mode=1).reconstruct_host(raw)
+    e_full = fp64.rel_l2(a, b)
+    e_ref = fp64.rel_l2(a, ref.recon32(ref.recon_dict(sp, n), raw))
+    print(f"half vs full {e_full:.3e}, half vs ref32 {e_ref:.3e}")
+    assert e_full < 5e-6
+    assert e_ref < TOL
+
+
+def test_unaligned_output_takes_full_width_pass(fsim_lib):
+    """The half-spectrum pass stores output pixel pairs; an output pointer
+    that is only 4-byte aligned is served by the full-width pass, which
+    agrees to FP32 round-off."""
+    from fsim import synthetic
+    from fsim._lib import call
+    from fsim.linalg import DeviceBuffer
+    from fsim.sim_algorithm import Reconstructor
+    n = 128
+    raw, sp = synthetic.sim_stack(n=n, seed=84, nbeads=60, nfil=8)
+    rec = Reconstructor(synthetic.make_param(sp, n))
+    aligned = rec.reconstruct_host(raw)
+    rb = DeviceBuffer(raw.nbytes)
+    rb.upload(raw)
+    ob = DeviceBuffer(rec.out_bytes() + 8)
+    rec.reconstruct(rb.value, ob.value + 4, 1)
+    call("fsim_sync")
+    whole = ob.download((rec.out_bytes() // 4 + 2,), np.float32)
+    got = whole[1:1 + (2 * n) ** 2].reshape(2 * n, 2 * n)
+    rb.free()
+    ob.free()
+    e = fp64.rel_l2(got, aligned)
+    print(f"unaligned output vs aligned {e:.3e}")
+    assert e < 5e-6
+
+
 def _multi_stacks(n, cases, nphase=3, nband=2):
     """Stacks with different shifts / phase offsets / modulations; the
     parameters of each as a SimParam on one OTF."""
