@@ -47,24 +47,29 @@ def profiled_traffic(kernel, n, batch):
     return None
 
 
-def stage_bytes(n, D=3, P=3, C=3, Cs=None, half=False):
+def stage_bytes(n, D=3, P=3, C=3, Cs=None, half=False, dc_sum=False):
     """HBM bytes per stack each pipeline kernel must move.  Cs = stored
     components per direction: C for the reference dataflow (the sum is
     B_alg = 434 n^2 at D = P = C = 3), B when each band's neg component is
-    derived from its pos component.  half: the result spectrum is Hermitian
-    and only its columns 0..n are formed -- half of every contribution lands
-    there (the DC component's n/2 columns; of a pos/neg pair at offset kx,
-    n/2 + kx and n/2 - kx columns) and B is half as wide."""
+    derived from its pos component.  dc_sum: the separation pass writes one
+    DC component (the sum over the directions) instead of one per direction.
+    half: the result spectrum is Hermitian and only its columns 0..n are
+    formed -- half of every contribution lands there (the DC component's n/2
+    columns; of a pos/neg pair at offset kx, n/2 + kx and n/2 - kx columns)
+    and B is half as wide."""
     Cs = C if Cs is None else Cs
     nn = n * n
     sh = D * (Cs - 1)
+    ndc = 1 if dc_sum else D                    # DC components written and read
+    dc_cols = 0.5 if half else 1.0              # share of a DC row spectrum kept
+    sep = 2 * D * P * nn + int(8 * ndc * dc_cols * nn) + 8 * sh * nn
     if half:
-        gather = 4 * D * C * nn + 8 * 2 * nn     # half of each contribution, 2n x n complex out
+        gather = 4 * D * (C - 1) * nn + 4 * ndc * nn + 8 * 2 * nn   # half of each contribution, 2n x n out
         last = 8 * 2 * nn + 4 * 4 * nn           # 2n x n complex in, (2n)^2 real out
     else:
-        gather = 8 * D * C * nn + 8 * 4 * nn     # every contribution read once, (2n)^2 complex out
+        gather = 8 * D * (C - 1) * nn + 8 * ndc * nn + 8 * 4 * nn   # every contribution once, (2n)^2 out
         last = 8 * 4 * nn + 4 * 4 * nn           # (2n)^2 complex in, real out
-    return [2 * D * P * nn + 8 * D * Cs * nn,    # raw u16 in, stored components out
+    return [sep,                                 # raw u16 in, stored components out
             16 * sh * nn,                        # shifted components in + out
             16 * sh * nn,
             gather, last]
@@ -686,7 +691,7 @@ def main():
 
     half = rec.p.mode == 0                    # real estimated OTF: the pairing is used in mode 0
     cs = 2 if half else 3
-    sb = stage_bytes(n, Cs=cs, half=half)
+    sb = stage_bytes(n, Cs=cs, half=half, dc_sum=True)
     names = stage_names(half)
     stage_ms = [ms[i] / max(cnt[i], 1) for i in range(5)]          # per launch (= per batch)
     dom = int(np.argmax(stage_ms))

@@ -57,7 +57,7 @@ struct JobList {
 
 bool recon_size_ok(int n);
 int gather_cw(int n);   // column tile width of the gather at raw side n
-int launch_pipeline(int n, const JobList &jl, int njobs, int D, int Cs, bool half, hipStream_t s);
+int launch_pipeline(int n, const JobList &jl, int njobs, int D, int P, int Cs, bool half, hipStream_t s);
 size_t workspace_a_elems(int n, int D, int C);
 size_t workspace_b_elems(int n);
 

@@ -58,6 +58,9 @@
 #ifndef FSIM_B_TILED
 #define FSIM_B_TILED 1
 #endif
+#ifndef FSIM_SEP_ALL
+#define FSIM_SEP_ALL 1
+#endif
 
 namespace fsim {
 
@@ -130,9 +133,13 @@ __device__ __forceinline__ float fade_colv(float fy, double fx) { return (float)
 
 // ---------------------------------------------------------------------------
 // K1: band separation + row FFT.  One wavefront (T = n/E threads) per row of
-// one direction and one stored component: the P raw rows are read (the
-// blocks of the other components of the row run on the same XCD and find
-// them in L2), faded, combined as c = sum_p S[d][c][p] * raw_p and
+// one unit: the DC component summed over all directions (the gather filters
+// the DC components of every direction with the same band-0 OTF at the same
+// offset, so by linearity their sum is all it needs; with a Hermitian result
+// only its columns 0..n/2), or one stored shifted component of one
+// direction.  The raw rows are read (the blocks of the other units of the
+// row run on the same XCD and find them in L2), faded, combined as
+// c = sum_f S[f / P][c][f % P] * raw_f over the unit's frames and
 // transformed.  VEC: each lane reads 4 adjacent pixels per load (8-byte
 // aligned rows) and the combined values reach the transform's element order
 // through the row's LDS region.
@@ -140,7 +147,8 @@ __device__ __forceinline__ float fade_colv(float fy, double fx) { return (float)
 typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 
 template <int N, bool VEC>
-__global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, const float2 *__restrict__ tw) {
+__global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int nrb, int units, int half,
+                                                   const float2 *__restrict__ tw) {
     constexpr int E = FftE<N>::value, T = N / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int LP = FftLds<N, E>::len;
@@ -148,26 +156,36 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, cons
     const Job J = get_job(jl, blockIdx.y);
     KRef K = cst(J.c);
     const int C = K.Cs, P = K.P;
+    // unit 0 of a row block: the DC component summed over the directions;
+    // unit 1 + d (C - 1) + (c - 1): stored component c >= 1 of direction d.
+    // The units of one row block run on one XCD (blocks L, L + 8, ...), so
+    // the raw rows they share are read from L2.
     const int L = blockIdx.x;
-    const int g = (L / 8 / C) * 8 + (L % 8);
-    const int c = (L / 8) % C;           // stored component
-    const int crow = K.srow[c];          // its separation-matrix row
-    if (g >= ngroups) return;
+    const int rb = (L / 8 / units) * 8 + (L % 8);
+    const int u = (L / 8) % units;
+    if (rb >= nrb) return;
+    int d = 0, c = 0, f0 = 0, nf = K.D * P;
+    if (u > 0) {
+        d = (u - 1) / (C - 1);
+        c = 1 + (u - 1) % (C - 1);
+        f0 = d * P;
+        nf = P;
+    }
+    const int crow = K.srow[c];          // separation-matrix row
     // row kernels: small blocks, twiddles from sincospi (no table fill)
     using TW = TwLds<N, false>;
     const float2 *tws = tw;
-    constexpr int rows_per_dir = N / RW;
-    const int d = g / rows_per_dir, rb = g % rows_per_dir;
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int y = rb * RW + rr;
     const size_t nn = (size_t)N * N;
-    const FSIM_GLB uint16_t *raw = glb(J.raw) + (size_t)d * P * nn + (size_t)y * N;
+    // frames are angle-major: frame f = d * P + p
+    const FSIM_GLB uint16_t *raw = glb(J.raw) + (size_t)f0 * nn + (size_t)y * N;
     const FSIM_GLB double *flo = glb(K.fade_lo), *fhi = glb(K.fade_hi);
     const int fpx = K.fade_px;
     float2 v[E];
-    // The loads of the row (fade factors, the pixels of up to G phases) are
+    // The loads of the row (fade factors, the pixels of up to G frames) are
     // issued before their first use, so a wave waits for memory once per G
-    // phases.
+    // frames.
     constexpr int G = 4;
     if constexpr (VEC) {
         static_assert(E % 4 == 0, "4 pixels per load");
@@ -182,11 +200,11 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, cons
         const double fly = flo[y], fhy = fhi[y];
         float2 w[Q][4];
         float fd[Q][4];
-        for (int p0 = 0; p0 < P; p0 += G) {
+        for (int p0 = 0; p0 < nf; p0 += G) {
             u16x4 r[G][Q];
 #pragma unroll
             for (int g = 0; g < G; g++)
-                if (p0 + g < P) {
+                if (p0 + g < nf) {
                     const FSIM_GLB u16x4 *rp = (const FSIM_GLB u16x4 *)(raw + (size_t)(p0 + g) * nn);
 #pragma unroll
                     for (int k = 0; k < Q; k++) r[g][k] = rp[t + T * k];
@@ -203,8 +221,9 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, cons
             }
 #pragma unroll
             for (int g = 0; g < G; g++)
-                if (p0 + g < P) {
-                    const float2 s = ld(&K.S[d][crow][p0 + g]);
+                if (p0 + g < nf) {
+                    const int f = f0 + p0 + g;
+                    const float2 s = ld(&K.S[f / P][crow][f % P]);
 #pragma unroll
                     for (int k = 0; k < Q; k++) {
                         const float r4[4] = {(float)r[g][k].x, (float)r[g][k].y, (float)r[g][k].z,
@@ -233,11 +252,11 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, cons
         for (int m = 0; m < E; m++) fx[m] = fade_factor(flo, fhi, t + T * m, N, fpx);
         const double fly = flo[y], fhy = fhi[y];
         float fade[E];
-        for (int p0 = 0; p0 < P; p0 += G) {
+        for (int p0 = 0; p0 < nf; p0 += G) {
             uint16_t r[G][E];
 #pragma unroll
             for (int g = 0; g < G; g++)
-                if (p0 + g < P) {
+                if (p0 + g < nf) {
                     const FSIM_GLB uint16_t *rp = raw + (size_t)(p0 + g) * nn;
 #pragma unroll
                     for (int m = 0; m < E; m++) r[g][m] = rp[t + T * m];
@@ -252,8 +271,9 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, cons
             }
 #pragma unroll
             for (int g = 0; g < G; g++)
-                if (p0 + g < P) {
-                    const float2 s = ld(&K.S[d][crow][p0 + g]);
+                if (p0 + g < nf) {
+                    const int f = f0 + p0 + g;
+                    const float2 s = ld(&K.S[f / P][crow][f % P]);
 #pragma unroll
                     for (int m = 0; m < E; m++) {
                         const float px = (float)r[g][m] * fade[m];
@@ -265,8 +285,104 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int ngroups, cons
     }
     fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
     FSIM_GLB float2 *out = glb(J.A) + ((size_t)(d * C + c) * N + y) * N;
+    // with a Hermitian result the gather reads DC columns 0..n/2 only
+    const int kmax = (u == 0 && half) ? N / 2 : N - 1;
 #pragma unroll
-    for (int m = 0; m < E; m++) st(out + t + T * m, v[m]);
+    for (int m = 0; m < E; m++)
+        if (t + T * m <= kmax) st(out + t + T * m, v[m]);
+}
+
+// K1, all units of a row in one wavefront: the row's D*P raw frames are
+// loaded once (4 pixels per load, all loads in flight together) and kept in
+// registers as 16-bit pixels; the DC sum and every stored shifted component
+// are then combined, transformed and stored in turn.  FR bounds the frames
+// held (D*P <= FR); used for n <= 512 (8 pixels per lane) and D*P <= 9,
+// where the frames fit the register budget.
+template <int N, int FR>
+__global__ void __launch_bounds__(256) k_sep_rows_all(JobList jl, int half, const float2 *__restrict__ tw) {
+    constexpr int E = FftE<N>::value, T = N / E;
+    constexpr int RW = T >= 256 ? 1 : 256 / T;
+    constexpr int LP = FftLds<N, E>::len;
+    static_assert(E % 4 == 0, "4 pixels per load");
+    constexpr int Q = E / 4;
+    using PL = FftLds<N, E>;
+    __shared__ float2 lds[RW * LP];
+    const Job J = get_job(jl, blockIdx.y);
+    KRef K = cst(J.c);
+    const int C = K.Cs, P = K.P, nf = K.D * P;
+    using TW = TwLds<N, false>;
+    const float2 *tws = tw;
+    const int t = threadIdx.x % T, rr = threadIdx.x / T;
+    const int y = blockIdx.x * RW + rr;
+    const size_t nn = (size_t)N * N;
+    const FSIM_GLB uint16_t *raw = glb(J.raw) + (size_t)y * N;
+    const FSIM_GLB double *flo = glb(K.fade_lo), *fhi = glb(K.fade_hi);
+    const int fpx = K.fade_px;
+    // lane t holds pixels x = 4 t + 4 T k + j, k < Q, j < 4
+    u16x4 r[FR][Q];
+#pragma unroll
+    for (int f = 0; f < FR; f++)
+        if (f < nf) {
+            const FSIM_GLB u16x4 *rp = (const FSIM_GLB u16x4 *)(raw + (size_t)f * nn);
+#pragma unroll
+            for (int k = 0; k < Q; k++) r[f][k] = rp[t + T * k];
+        }
+    float fd[Q][4];
+    {
+        const float fy = fade_rowv(flo[y], fhi[y]);
+#pragma unroll
+        for (int k = 0; k < Q; k++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) fd[k][j] = fade_colv(fy, fade_factor(flo, fhi, 4 * t + 4 * T * k + j, N, fpx));
+    }
+    float2 *row = lds + rr * LP;
+    const int units = 1 + K.D * (C - 1);
+    for (int u = 0; u < units; u++) {
+        // unit 0: DC summed over the directions; then (d, c), c >= 1
+        int d = 0, c = 0, f0 = 0, f1 = nf;
+        if (u > 0) {
+            d = (u - 1) / (C - 1);
+            c = 1 + (u - 1) % (C - 1);
+            f0 = d * P;
+            f1 = f0 + P;
+        }
+        const int crow = K.srow[c];
+        float2 w[Q][4];
+#pragma unroll
+        for (int k = 0; k < Q; k++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) w[k][j] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int f = 0; f < FR; f++)
+            if (f >= f0 && f < f1) {
+                const float2 sv = ld(&K.S[f / P][crow][f % P]);
+#pragma unroll
+                for (int k = 0; k < Q; k++) {
+                    const float r4[4] = {(float)r[f][k].x, (float)r[f][k].y, (float)r[f][k].z, (float)r[f][k].w};
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const float px = r4[j] * fd[k][j];
+                        w[k][j].x += sv.x * px;
+                        w[k][j].y += sv.y * px;
+                    }
+                }
+            }
+        float2 v[E];
+#pragma unroll
+        for (int k = 0; k < Q; k++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) row[PL::pad(4 * t + 4 * T * k + j)] = w[k][j];
+        pass_sync<RowWave<N>::value>();
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = row[PL::pad(t + T * m)];
+        pass_sync<RowWave<N>::value>();
+        fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, row, 1, t, tws);
+        FSIM_GLB float2 *out = glb(J.A) + ((size_t)(d * C + c) * N + y) * N;
+        const int kmax = (u == 0 && half) ? N / 2 : N - 1;
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            if (t + T * m <= kmax) st(out + t + T * m, v[m]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -283,7 +399,12 @@ constexpr int lds_waves_per_simd(int threads, int lds_bytes, int cap) {
 
 template <int N>
 struct ColShiftTile {
-    static constexpr int e = ColE<N>::value, t = N / e;
+#ifdef FSIM_COL_E
+    static constexpr int e = FSIM_COL_E < N / 2 ? FSIM_COL_E : ColE<N>::value;
+#else
+    static constexpr int e = ColE<N>::value;
+#endif
+    static constexpr int t = N / e;
     static constexpr int cw0 = ColTile<N, t, e>::value;
     static constexpr int cw = cw0 < FSIM_COL_CW ? cw0 : FSIM_COL_CW;
     static constexpr int value = t * cw;
@@ -295,7 +416,7 @@ struct ColShiftTile {
 template <int N>
 __global__ void __launch_bounds__(ColShiftTile<N>::value, ColShiftTile<N>::wpe) k_col_shift(JobList jl, int nsh_all, int total,
                                                                      const float2 *__restrict__ tw) {
-    constexpr int E = ColE<N>::value, T = N / E;
+    constexpr int E = ColShiftTile<N>::e, T = N / E;
     constexpr int CW = ColShiftTile<N>::cw;
     constexpr int tiles = N / CW;
     __shared__ float2 lds[ColShiftTile<N>::lds];
@@ -519,10 +640,10 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
     for (int m = 0; m < EM; m++) acc[m] = make_float2(0.f, 0.f);
     const size_t nn = (size_t)N * N;
     // DC components of every direction: all are pasted at offset 0 and
-    // filtered with the band-0 OTF, so their sum (formed on the
-    // row-transformed data) takes one column FFT and one OTF product, staged
-    // through LDS for the row gather.  Skipped as a whole when the tile lies
-    // outside the paste window.
+    // filtered with the band-0 OTF, so their sum (formed by the separation
+    // pass) takes one column FFT and one OTF product, staged through LDS for
+    // the row gather.  Skipped as a whole when the tile lies outside the
+    // paste window.
     if (tile_pasted<N>(tile * CW, CW, 0) || (HALF && tile_pasted_mirror<N>(tile * CW, CW, 0))) {
         const int px = paste_p<N>(xo, 0);
         const int xs = px & (N - 1);
@@ -533,14 +654,10 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
 #pragma unroll
         for (int m = 0; m < EN; m++) v[m] = make_float2(0.f, 0.f);
         if (act) {
-            for (int d = 0; d < D; d++) {
-                const FSIM_GLB float2 *col = A + (size_t)(d * Cs + K.src[0]) * nn + xs;
-                float2 u[EN];
+            // direction 0's DC slot holds the sum over the directions
+            const FSIM_GLB float2 *col = A + xs;
 #pragma unroll
-                for (int m = 0; m < EN; m++) u[m] = ld(col + (size_t)(t + TN * m) * N);
-#pragma unroll
-                for (int m = 0; m < EN; m++) v[m] = cadd(v[m], u[m]);
-            }
+            for (int m = 0; m < EN; m++) v[m] = ld(col + (size_t)(t + TN * m) * N);
         }
         fft_regs<N, EN, false, false, TW::tl>(v, lds + cl, CW, t, tws, act);
         if (act) {
@@ -886,7 +1003,7 @@ static void stage_mark(int stage, hipStream_t s, hipEvent_t &start) {
 }
 
 template <int N>
-static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, bool half, hipStream_t s) {
+static int launch_pipeline_t(const JobList &jl, int njobs, int D, int P, int C, bool half, hipStream_t s) {
     // C: stored components per direction
     constexpr int M = 2 * N;
     const float2 *twn = twiddles(N), *twm = twiddles(M);
@@ -895,17 +1012,27 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int C, bool ha
     stage_mark(-1, s, ev);
     {
         constexpr int E = FftE<N>::value, T = N / E, RW = T >= 256 ? 1 : 256 / T;
-        const int ngroups = D * (N / RW);
-        const int gx = ((ngroups + 7) / 8) * 8 * C;
-        if (jl.raw_aligned)
-            hipLaunchKernelGGL((k_sep_rows1<N, true>), dim3(gx, njobs), dim3(T * RW), 0, s, jl, ngroups, twn);
+        const int nrb = N / RW, units = 1 + D * (C - 1);
+        const int gx = ((nrb + 7) / 8) * 8 * units;
+        bool done = false;
+        if constexpr (N <= 512) {
+            if (FSIM_SEP_ALL && jl.raw_aligned && D * P <= 9) {
+                hipLaunchKernelGGL((k_sep_rows_all<N, 9>), dim3(nrb, njobs), dim3(T * RW), 0, s, jl, (int)half, twn);
+                done = true;
+            }
+        }
+        if (done) {
+        } else if (jl.raw_aligned)
+            hipLaunchKernelGGL((k_sep_rows1<N, true>), dim3(gx, njobs), dim3(T * RW), 0, s, jl, nrb, units,
+                               (int)half, twn);
         else
-            hipLaunchKernelGGL((k_sep_rows1<N, false>), dim3(gx, njobs), dim3(T * RW), 0, s, jl, ngroups, twn);
+            hipLaunchKernelGGL((k_sep_rows1<N, false>), dim3(gx, njobs), dim3(T * RW), 0, s, jl, nrb, units,
+                               (int)half, twn);
         stage_mark(0, s, ev);
     }
     if (C > 1) {
         {
-            constexpr int E = ColE<N>::value, T = N / E, CW = ColShiftTile<N>::cw;
+            constexpr int E = ColShiftTile<N>::e, T = N / E, CW = ColShiftTile<N>::cw;
             const int nsh_all = D * (C - 1), total = njobs * nsh_all * (N / CW);
             hipLaunchKernelGGL(k_col_shift<N>, dim3(FSIM_COL_XCD ? xcd_grid(total) : (unsigned)total), dim3(T * CW),
                                0, s, jl, nsh_all, total, twn);
@@ -953,15 +1080,15 @@ int gather_cw(int n) {
     }
 }
 
-int launch_pipeline(int n, const JobList &jl, int njobs, int D, int C, bool half, hipStream_t s) {
+int launch_pipeline(int n, const JobList &jl, int njobs, int D, int P, int C, bool half, hipStream_t s) {
     // C: stored components per direction (ReconConst::Cs)
     switch (n) {
-        case 64: return launch_pipeline_t<64>(jl, njobs, D, C, half, s);
-        case 128: return launch_pipeline_t<128>(jl, njobs, D, C, half, s);
-        case 256: return launch_pipeline_t<256>(jl, njobs, D, C, half, s);
-        case 512: return launch_pipeline_t<512>(jl, njobs, D, C, half, s);
-        case 1024: return launch_pipeline_t<1024>(jl, njobs, D, C, half, s);
-        case 2048: return launch_pipeline_t<2048>(jl, njobs, D, C, half, s);
+        case 64: return launch_pipeline_t<64>(jl, njobs, D, P, C, half, s);
+        case 128: return launch_pipeline_t<128>(jl, njobs, D, P, C, half, s);
+        case 256: return launch_pipeline_t<256>(jl, njobs, D, P, C, half, s);
+        case 512: return launch_pipeline_t<512>(jl, njobs, D, P, C, half, s);
+        case 1024: return launch_pipeline_t<1024>(jl, njobs, D, P, C, half, s);
+        case 2048: return launch_pipeline_t<2048>(jl, njobs, D, P, C, half, s);
         default: return fail(FSIM_ERR_ARG, "reconstruct: n must be a power of two in [64, 2048]");
     }
 }
@@ -1264,7 +1391,7 @@ int fsim_reconstruct(fsim_channel ch, const uint16_t *raw, float *out, int count
     // the half-spectrum pass writes output pixel pairs (8-byte stores); an
     // output that is only 4-byte aligned takes the full-width pass
     const bool half = K.half && ((uintptr_t)out & 7) == 0;
-    return launch_pipeline(n, jl, count, K.D, K.Cs, half, s);
+    return launch_pipeline(n, jl, count, K.D, K.P, K.Cs, half, s);
 }
 
 int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw, float *out, int count, const fsim_sim_params *sp,
@@ -1366,7 +1493,7 @@ int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw, float *out, in
     JobList jl{};
     jl.jobs = w.jobs;
     jl.raw_aligned = aligned;
-    return launch_pipeline(n, jl, count, K0.D, K0.Cs, half, s);
+    return launch_pipeline(n, jl, count, K0.D, K0.P, K0.Cs, half, s);
 }
 
 int fsim_stage_timing(int enable) {

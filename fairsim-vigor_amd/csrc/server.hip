@@ -201,7 +201,7 @@ struct Server {
             jl.raw_aligned &= ((uintptr_t)s->jobs_h[i].raw & 7) == 0;
             half &= ((uintptr_t)s->jobs_h[i].out & 7) == 0;
         }
-        if (!rc) rc = launch_pipeline(k.n, jl, count, k.D, C, half, s->stream);
+        if (!rc) rc = launch_pipeline(k.n, jl, count, k.D, k.P, C, half, s->stream);
         for (int i = 0; i < count && !rc; i++) {
             Request *r = s->reqs[i];
             if (!r->device &&
