@@ -39,7 +39,7 @@ def profiled_traffic(kernel, n, batch):
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         name = cells[0].strip("`") if cells else ""
-        if len(cells) == 7 and (name.startswith(f"fsim::{kernel}<") or name.startswith(f"fsim::{kernel}1<")):
+        if len(cells) == 7 and name.split("<")[0] in (f"fsim::{kernel}", f"fsim::{kernel}1", f"fsim::{kernel}_all"):
             try:
                 return int(round((float(cells[4]) + float(cells[6])) * 2 ** 20))
             except ValueError:
