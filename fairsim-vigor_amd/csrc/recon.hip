@@ -61,6 +61,12 @@
 #ifndef FSIM_SEP_ALL
 #define FSIM_SEP_ALL 1
 #endif
+#ifndef FSIM_STAGE_STORE
+#define FSIM_STAGE_STORE 0   // separation pass: stores from registers
+#endif
+#ifndef FSIM_STAGE_STORE_K3
+#define FSIM_STAGE_STORE_K3 1   // row shift pass: stores through LDS
+#endif
 
 namespace fsim {
 
@@ -145,6 +151,87 @@ __device__ __forceinline__ float fade_colv(float fy, double fx) { return (float)
 // through the row's LDS region.
 // ---------------------------------------------------------------------------
 typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+
+// Component spectra in A are tile-major: 8-column tiles, each one contiguous
+// run over the n rows, element (x, y) at ((x / 8) * n + y) * 8 + x % 8.  The
+// column kernels then read and write whole runs; the row kernels move the
+// runs of their RW rows (RW * 8 elements per tile) through LDS.
+constexpr int kAT = 8;
+#ifndef FSIM_AT_RH
+#define FSIM_AT_RH 0   // tile height in rows; 0: the whole column
+#endif
+template <int N>
+struct ATile {
+    // at least the rows of one row-kernel block (256 threads, n / E per row)
+    static constexpr int rw = (N / FftE<N>::value) >= 256 ? 1 : 256 / (N / FftE<N>::value);
+    static constexpr int req = (FSIM_AT_RH > 0 && FSIM_AT_RH < N) ? FSIM_AT_RH : N;
+    static constexpr int rh = req > rw ? req : rw;
+};
+template <int N>
+__device__ __forceinline__ size_t at_col(int x) {
+    return (size_t)(x / kAT) * ATile<N>::rh * kAT + (x % kAT);
+}
+template <int N>
+__device__ __forceinline__ size_t at_row(int y) {
+    constexpr int RH = ATile<N>::rh;
+    return (size_t)(y / RH) * N * RH + (size_t)(y % RH) * kAT;
+}
+
+// Rows [y0, y0 + RW) of a tile-major n x n array, tiles [0, ntiles), into
+// row-major LDS rows (row r at rows + r * RP) / back from them.  Every thread
+// of the block (NT = n * RW / E threads, E elements each) calls them; the
+// caller places the block barriers.
+template <int N, int RW, int NT>
+__device__ __forceinline__ void tiled_rows_load(const FSIM_GLB float2 *a, int y0, float2 *rows, int RP, int ntiles) {
+    constexpr int RUN = RW * kAT, PER = N * RW / NT;
+    static_assert(ATile<N>::rh % RW == 0, "a block's rows lie in one tile row");
+    const FSIM_GLB float2 *base = a + at_row<N>(y0);
+    const int lim = ntiles * RUN;
+    float2 u[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int e = (int)threadIdx.x + NT * r;
+        if (e < lim) u[r] = ld(base + at_col<N>((e / RUN) * kAT) + e % RUN);
+    }
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int e = (int)threadIdx.x + NT * r;
+        if (e < lim) {
+            const int q = e % RUN;
+            rows[(q / kAT) * RP + (e / RUN) * kAT + q % kAT] = u[r];
+        }
+    }
+}
+// Row y of a tile-major n x n array from the registers of its transform
+// (element k = t + T m): 8 lanes write one tile's 64 bytes of the row; the
+// other half of each 128-byte line is the neighbouring row, written by the
+// neighbouring wave, and the halves meet in L2.
+template <int N, int E>
+__device__ __forceinline__ void tiled_row_store_regs(FSIM_GLB float2 *a, int y, int t, const float2 (&v)[E], int kmax) {
+    constexpr int T = N / E;
+    FSIM_GLB float2 *base = a + at_row<N>(y);
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const int k = t + T * m;
+        if (k <= kmax) st(base + at_col<N>(k), v[m]);
+    }
+}
+
+template <int N, int RW, int NT>
+__device__ __forceinline__ void tiled_rows_store(FSIM_GLB float2 *a, int y0, const float2 *rows, int RP, int ntiles) {
+    constexpr int RUN = RW * kAT, PER = N * RW / NT;
+    static_assert(ATile<N>::rh % RW == 0, "a block's rows lie in one tile row");
+    FSIM_GLB float2 *base = a + at_row<N>(y0);
+    const int lim = ntiles * RUN;
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int e = (int)threadIdx.x + NT * r;
+        if (e < lim) {
+            const int q = e % RUN;
+            st(base + at_col<N>((e / RUN) * kAT) + q, rows[(q / kAT) * RP + (e / RUN) * kAT + q % kAT]);
+        }
+    }
+}
 
 template <int N, bool VEC>
 __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int nrb, int units, int half,
@@ -284,12 +371,16 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int nrb, int unit
         }
     }
     fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
-    FSIM_GLB float2 *out = glb(J.A) + ((size_t)(d * C + c) * N + y) * N;
     // with a Hermitian result the gather reads DC columns 0..n/2 only
-    const int kmax = (u == 0 && half) ? N / 2 : N - 1;
+    const int ntiles = (u == 0 && half) ? N / 2 / kAT + 1 : N / kAT;
+    if constexpr (FSIM_STAGE_STORE) {
 #pragma unroll
-    for (int m = 0; m < E; m++)
-        if (t + T * m <= kmax) st(out + t + T * m, v[m]);
+        for (int m = 0; m < E; m++) lds[rr * LP + t + T * m] = v[m];
+        __syncthreads();
+        tiled_rows_store<N, RW, T * RW>(glb(J.A) + (size_t)(d * C + c) * nn, rb * RW, lds, LP, ntiles);
+    } else {
+        tiled_row_store_regs<N, E>(glb(J.A) + (size_t)(d * C + c) * nn, y, t, v, ntiles * kAT - 1);
+    }
 }
 
 // K1, all units of a row in one wavefront: the row's D*P raw frames are
@@ -377,11 +468,17 @@ __global__ void __launch_bounds__(256) k_sep_rows_all(JobList jl, int half, cons
         for (int m = 0; m < E; m++) v[m] = row[PL::pad(t + T * m)];
         pass_sync<RowWave<N>::value>();
         fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, row, 1, t, tws);
-        FSIM_GLB float2 *out = glb(J.A) + ((size_t)(d * C + c) * N + y) * N;
-        const int kmax = (u == 0 && half) ? N / 2 : N - 1;
+        // with a Hermitian result the gather reads DC columns 0..n/2 only
+        const int ntiles = (u == 0 && half) ? N / 2 / kAT + 1 : N / kAT;
+        if constexpr (FSIM_STAGE_STORE) {
 #pragma unroll
-        for (int m = 0; m < E; m++)
-            if (t + T * m <= kmax) st(out + t + T * m, v[m]);
+            for (int m = 0; m < E; m++) row[t + T * m] = v[m];
+            __syncthreads();
+            tiled_rows_store<N, RW, T * RW>(glb(J.A) + (size_t)(d * C + c) * nn, blockIdx.x * RW, lds, LP, ntiles);
+            __syncthreads();
+        } else {
+            tiled_row_store_regs<N, E>(glb(J.A) + (size_t)(d * C + c) * nn, y, t, v, ntiles * kAT - 1);
+        }
     }
 }
 
@@ -446,10 +543,10 @@ __global__ void __launch_bounds__(ColShiftTile<N>::value, ColShiftTile<N>::wpe) 
     const int d = dc / nsh, c = 1 + dc % nsh;   // stored shifted component
     const int cl = threadIdx.x % CW, t = threadIdx.x / CW;
     const int x = tile * CW + cl;
-    FSIM_GLB float2 *col = glb(J.A) + (size_t)(d * C + c) * N * N + x;
+    FSIM_GLB float2 *col = glb(J.A) + (size_t)(d * C + c) * N * N + at_col<N>(x);
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = ld(col + (size_t)(t + T * m) * N);
+    for (int m = 0; m < E; m++) v[m] = ld(col + at_row<N>(t + T * m));
     fft_regs<N, E, false, false, TW::tl>(v, lds + cl, CW, t, tws);
     const int band = K.sband[c];
     const FSIM_GLB float2 *otf = glb(K.otf2d) + (size_t)(K.otf_nb > 1 ? band : 0) * N * N + x;
@@ -473,7 +570,7 @@ __global__ void __launch_bounds__(ColShiftTile<N>::value, ColShiftTile<N>::wpe) 
     }
     fft_regs<N, E, false, false, TW::tl>(v, lds + cl, CW, t, tws);
 #pragma unroll
-    for (int m = 0; m < E; m++) st(col + (size_t)(t + T * m) * N, v[m]);
+    for (int m = 0; m < E; m++) st(col + at_row<N>(t + T * m), v[m]);
 }
 
 // ---------------------------------------------------------------------------
@@ -495,11 +592,12 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
     const int rb = blockIdx.x % rblocks, dc = blockIdx.x / rblocks;
     const int d = dc / nsh, c = 1 + dc % nsh;   // stored shifted component
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
-    const int y = rb * RW + rr;
-    FSIM_GLB float2 *row = glb(J.A) + ((size_t)(d * C + c) * N + y) * N;
+    FSIM_GLB float2 *comp = glb(J.A) + (size_t)(d * C + c) * N * N;
+    tiled_rows_load<N, RW, T * RW>(comp, rb * RW, lds, LP, N / kAT);
+    __syncthreads();
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = ld(row + t + T * m);
+    for (int m = 0; m < E; m++) v[m] = lds[rr * LP + t + T * m];
     fft_regs<N, E, true, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
     // row factor rx(x) of the sub-pixel ramp (the column factor was applied
     // by the column pass)
@@ -511,8 +609,14 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
         v[m] = rot_java(cscale(v[m], sc), r.x, r.y);
     }
     fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
+    if constexpr (FSIM_STAGE_STORE_K3) {
 #pragma unroll
-    for (int m = 0; m < E; m++) st(row + t + T * m, v[m]);
+        for (int m = 0; m < E; m++) lds[rr * LP + t + T * m] = v[m];
+        __syncthreads();
+        tiled_rows_store<N, RW, T * RW>(comp, rb * RW, lds, LP, N / kAT);
+    } else {
+        tiled_row_store_regs<N, E>(comp, rb * RW + rr, t, v, N - 1);
+    }
 }
 
 // Block -> (stack, column tile) for the gather kernel.  With >= 8 stacks in a
@@ -655,9 +759,9 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
         for (int m = 0; m < EN; m++) v[m] = make_float2(0.f, 0.f);
         if (act) {
             // direction 0's DC slot holds the sum over the directions
-            const FSIM_GLB float2 *col = A + xs;
+            const FSIM_GLB float2 *col = A + at_col<N>(xs);
 #pragma unroll
-            for (int m = 0; m < EN; m++) v[m] = ld(col + (size_t)(t + TN * m) * N);
+            for (int m = 0; m < EN; m++) v[m] = ld(col + at_row<N>(t + TN * m));
         }
         fft_regs<N, EN, false, false, TW::tl>(v, lds + cl, CW, t, tws, act);
         if (act) {
@@ -708,7 +812,7 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
             const int oy = K.offy[d][c], my = K.mdy[d][c];
             int xs = px & (N - 1);
             if (mir) xs = (K.mdx[d][c] - xs) & (N - 1);
-            const FSIM_GLB float2 *comp = A + (size_t)(d * Cs + K.src[c]) * nn + xs;
+            const FSIM_GLB float2 *comp = A + (size_t)(d * Cs + K.src[c]) * nn + at_col<N>(xs);
             const float sg = mir ? -1.0f : 1.0f;
             if constexpr (HALF) {
 #pragma unroll
@@ -724,7 +828,7 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
                         if (mir) ys = (my - ys) & (N - 1);
                         const int r = (colok && paste_ok<N>(py)) + (colok2 && paste_ok<N>(paste_p<N>(-y, offy2)));
                         wt[i] = 0.5f * (float)r;
-                        if (r) sv[i] = ld(comp + (size_t)ys * N);
+                        if (r) sv[i] = ld(comp + at_row<N>(ys));
                     }
 #pragma unroll
                     for (int i = 0; i < EM / 2; i++) {
@@ -744,7 +848,7 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
                     const int py = paste_p<N>(t + TM * (h + i), oy);
                     int ys = py & (N - 1);
                     if (mir) ys = (my - ys) & (N - 1);
-                    if (colok && paste_ok<N>(py)) sv[i] = ld(comp + (size_t)ys * N);
+                    if (colok && paste_ok<N>(py)) sv[i] = ld(comp + at_row<N>(ys));
                 }
 #pragma unroll
                 for (int i = 0; i < EM / 2; i++) {
