@@ -191,32 +191,56 @@ def serve_mixed_stream(nreq, rate):
             "saturation_roofline_frac_pcie_inclusive": round(nreq / sat_el * 434 * mean_n2 / 8e12, 4)}
 
 
-def batched_mixed(nper=32):
+def batched_mixed(nper=32, reps=10):
     """Config 2: one batch of 96 stacks -- 3 channels (525/600/670 nm) x sizes
-    256/512/1024, 32 of each size -- submitted at once to the request server
-    with stacks and outputs resident in HBM; time to the last result."""
+    256/512/1024, 32 of each size, stacks and outputs resident in HBM.  The
+    headline form is one fixed-shape batch (fsim_batch_*: the three size
+    classes run concurrently, captured once as a hipGraph and replayed; the
+    slot addresses travel in the device job lists); time per replay from
+    host timestamps around run + sync, median of `reps`.  Beside it the same
+    96 stacks submitted one by one to the request server (time to the last
+    result)."""
     import numpy as np
     from fsim import synthetic
+    from fsim._lib import call
     from fsim.linalg import DeviceBuffer
-    from fsim.livemode import ReconstructionServer
+    from fsim.livemode import ReconstructionBatch, ReconstructionServer
     from fsim.sim_algorithm import Reconstructor
     sizes, lams = [256, 512, 1024], [525.0, 600.0, 670.0]
-    srv = ReconstructionServer(mem_budget=32 << 30, max_batch=32, max_queue=4096, nslots=4)
-    ids, bufs, keep = {}, {}, []
+    recs, bufs, keep = {}, {}, []
     for n in sizes:
         for ci, lam in enumerate(lams):
             raw, sp = synthetic.sim_stack(n=n, seed=200 + ci, lam=lam, nbeads=300 * n // 512,
                                           nfil=40 * n // 512)
             rec = Reconstructor(synthetic.make_param(sp, n, lam=lam))
             keep.append(rec)
-            ids[(n, ci)] = srv.add_channel(rec)
+            recs[(n, ci)] = rec
             rb = DeviceBuffer(raw.nbytes)
             rb.upload(raw)
             bufs[(n, ci)] = rb
     jobs = [(n, i % 3) for n in sizes for i in range(nper)]
     outs = {n: [DeviceBuffer(16 * n * n) for _ in range(nper)] for n in sizes}
+    out_ptrs = [outs[k[0]][j % nper].value for j, k in enumerate(jobs)]
+    raw_ptrs = [bufs[k].value for k in jobs]
+    b_alg = sum(434 * n * n for n, _ in jobs)
 
-    def run():
+    batch = ReconstructionBatch([recs[k] for k in jobs], use_graph=True)
+    batch.run(raw_ptrs, out_ptrs)            # captures the graph
+    call("fsim_sync")
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        batch.run(raw_ptrs, out_ptrs)
+        call("fsim_sync")
+        times.append(time.perf_counter() - t0)
+    el = float(np.median(times))
+    info = batch.info()
+    batch.close()
+
+    srv = ReconstructionServer(mem_budget=32 << 30, max_batch=32, max_queue=4096, nslots=4)
+    ids = {k: srv.add_channel(r) for k, r in recs.items()}
+
+    def run_served():
         t0 = time.perf_counter()
         tickets = [srv.submit(ids[k], bufs[k].value, outs[k[0]][j % nper].value, device=True)
                    for j, k in enumerate(jobs)]
@@ -224,14 +248,18 @@ def batched_mixed(nper=32):
             srv.wait(t)
         return time.perf_counter() - t0
 
-    run()
+    run_served()
     srv.stats(reset=True)
-    el = min(run() for _ in range(3))
+    el_s = min(run_served() for _ in range(3))
     srv.close()
-    b_alg = sum(434 * n * n for n, _ in jobs)
     return {"batch": f"{len(jobs)} stacks: sizes 256/512/1024 x {nper}, 3 channels, resident in HBM",
+            "form": f"one fixed-shape batch, {info['classes']} size classes on concurrent streams, "
+                    f"replayed hipGraph (captured={info['captured']})",
             "ms_per_batch": round(el * 1e3, 3), "recons_per_s": round(len(jobs) / el, 1),
-            "alg_bytes_per_batch": b_alg, "roofline_frac": round(b_alg / el / 8e12, 4)}
+            "alg_bytes_per_batch": b_alg, "roofline_frac": round(b_alg / el / 8e12, 4),
+            "served_one_by_one": {"ms_per_batch": round(el_s * 1e3, 3),
+                                  "recons_per_s": round(len(jobs) / el_s, 1),
+                                  "roofline_frac": round(b_alg / el_s / 8e12, 4)}}
 
 
 def large_fov(batch=4, steps=5):
@@ -590,6 +618,8 @@ def main():
                     help="only the live stream ingest, side-output and filter-style legs")
     ap.add_argument("--estimation-only", action="store_true",
                     help="only the batched estimation legs of config 1 (for rocprofv3 runs)")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="only the mixed-size batch (config 2)")
     ap.add_argument("--probe-copy", action="store_true",
                     help="run a plain streaming kernel of known size (HBM counter calibration)")
     args = ap.parse_args()
@@ -613,6 +643,9 @@ def main():
                           "side_outputs_512": side_outputs(), "filter_styles_512": filter_styles(),
                           "config_io": config_io()}),
               flush=True)
+        return
+    if args.mixed_only:
+        print(json.dumps({"mixed_only": True, "batched_mixed_96": batched_mixed()}), flush=True)
         return
     if args.estimation_only:
         out = {}

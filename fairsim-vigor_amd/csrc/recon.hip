@@ -1094,6 +1094,8 @@ struct StageTimer {
 };
 static StageTimer g_timer;
 
+bool stage_timing_enabled() { return g_timer.on; }
+
 static void stage_mark(int stage, hipStream_t s, hipEvent_t &start) {
     if (!g_timer.on) return;
     hipEvent_t e;

@@ -86,6 +86,47 @@ class ReconstructionServer:
         return {k: getattr(st, k) for k, _ in SERVER_STATS._fields_}
 
 
+class ReconstructionBatch:
+    """Fixed-shape batch (fsim_batch_*): slot i reconstructs with
+    ``reconstructors[i]``; sizes and channels may be mixed.  ``run`` takes
+    one device raw address and one device output address per slot (8-byte
+    aligned); the class pipelines run concurrently and, with
+    ``use_graph``, as one replayed hipGraph whose job lists carry the
+    addresses of the run."""
+
+    def __init__(self, reconstructors, use_graph=True):
+        _lib.init()
+        self._recs = list(reconstructors)
+        n = len(self._recs)
+        hs = (C.c_void_p * n)(*[r.handle.value for r in self._recs])
+        self._h = C.c_void_p()
+        call("fsim_batch_create", hs, n, 1 if use_graph else 0, C.byref(self._h))
+        self.count = n
+
+    def run(self, raw_ptrs, out_ptrs, stream=None):
+        if len(raw_ptrs) != self.count or len(out_ptrs) != self.count:
+            raise ValueError("one raw and one output address per slot")
+        rp = (C.c_void_p * self.count)(*[int(p) for p in raw_ptrs])
+        op = (C.c_void_p * self.count)(*[int(p) for p in out_ptrs])
+        call("fsim_batch_run", self._h, rp, op, C.c_void_p(stream))
+
+    def info(self):
+        nc, cap, nb = C.c_int(), C.c_int(), C.c_int64()
+        call("fsim_batch_info", self._h, C.byref(nc), C.byref(cap), C.byref(nb))
+        return {"classes": nc.value, "captured": bool(cap.value), "device_bytes": nb.value}
+
+    def close(self):
+        if self._h is not None and self._h.value and _lib._lib is not None:
+            lib().fsim_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def pinned(shape, dtype):
     """A numpy array over pinned host memory (fsim_alloc_host)."""
     nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize

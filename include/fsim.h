@@ -391,6 +391,26 @@ int fsim_server_submit(fsim_server s, int channel, const uint16_t *raw, float *o
 int fsim_server_wait(fsim_server s, uint64_t ticket, int timeout_ms, double *latency_ms);
 int fsim_server_get_stats(fsim_server s, fsim_server_stats *st, int reset);
 
+/* ---- fixed-shape batches: mixed sizes and channels, one graph launch -------
+ * No single counterpart in the reference: it reconstructs image by image from
+ * several threads (livemode/ReconstructionRunner.java:243-251, 380-413).
+ * fsim_batch_create binds slot i to channels[i]; the slots are grouped into
+ * size classes (raw side, directions, phases, stored components), each with
+ * its own work buffers, device job list and stream.  fsim_batch_run
+ * reconstructs raw[i] (device, D*P frames of n*n uint16) into out[i] (device,
+ * (2n)^2 float32) for every slot, stream-ordered on `stream` (NULL: the
+ * library stream): the class pipelines run concurrently, and with
+ * use_graph != 0 the launch sequence is captured as a hipGraph on the first
+ * run and replayed afterwards, the slot addresses being rewritten in the
+ * device job lists before each replay.  raw[i] and out[i] must be 8-byte
+ * aligned.  One run at a time per batch (runs on one stream serialise).
+ * Results are bitwise those of fsim_reconstruct on the same stacks. */
+typedef struct fsim_batch_s *fsim_batch;
+int fsim_batch_create(const fsim_channel *channels, int count, int use_graph, fsim_batch *out);
+int fsim_batch_run(fsim_batch b, const uint16_t *const *raw, float *const *out, void *stream);
+int fsim_batch_info(fsim_batch b, int *nclasses, int *captured, int64_t *device_bytes);
+int fsim_batch_destroy(fsim_batch b);
+
 /* ---- live stream ingest (transport/ImageWrapper.java,
  * livemode/SimSequenceExtractor.java) -----------------------------------------
  * Wire format: a 128-byte little-endian header per camera frame, then the
