@@ -25,6 +25,7 @@ for _p in (ROOT, os.path.join(ROOT, "fairsim-vigor_amd")):
 
 HBM_PEAK_GBS = 8000.0
 STAGES = ["k_sep_rows", "k_col_shift", "k_row_shift", "k_gather", "k_row_real"]
+TUNE = True   # run the autotuner for each workload's size class (--no-tune: defaults)
 
 
 def profiled_traffic(kernel, n, batch):
@@ -218,6 +219,11 @@ def batched_mixed(nper=32, reps=10):
             rb = DeviceBuffer(raw.nbytes)
             rb.upload(raw)
             bufs[(n, ci)] = rb
+    tuned = {}
+    if TUNE:
+        from fsim import autotune
+        for n in sizes:
+            tuned[n] = autotune.tune(recs[(n, 0)], count=nper, reps=3)
     jobs = [(n, i % 3) for n in sizes for i in range(nper)]
     outs = {n: [DeviceBuffer(16 * n * n) for _ in range(nper)] for n in sizes}
     out_ptrs = [outs[k[0]][j % nper].value for j, k in enumerate(jobs)]
@@ -257,6 +263,7 @@ def batched_mixed(nper=32, reps=10):
                     f"replayed hipGraph (captured={info['captured']})",
             "ms_per_batch": round(el * 1e3, 3), "recons_per_s": round(len(jobs) / el, 1),
             "alg_bytes_per_batch": b_alg, "roofline_frac": round(b_alg / el / 8e12, 4),
+            "autotune": {n: (t["sep"], t["col_elems"], t["col_width"]) for n, t in tuned.items()},
             "served_one_by_one": {"ms_per_batch": round(el_s * 1e3, 3),
                                   "recons_per_s": round(len(jobs) / el_s, 1),
                                   "roofline_frac": round(b_alg / el_s / 8e12, 4)}}
@@ -274,6 +281,9 @@ def large_fov(batch=4, steps=5):
     n = 2048
     raw, sp = synthetic.sim_stack(n=n, seed=7, nbeads=1200, nfil=160)
     rec = Reconstructor(synthetic.make_param(sp, n))
+    if TUNE:
+        from fsim import autotune
+        autotune.tune(rec, count=batch, reps=2)
     rb = DeviceBuffer(raw.nbytes * batch)
     host = np.ascontiguousarray(np.broadcast_to(raw, (batch,) + raw.shape))
     rb.upload(host)
@@ -598,8 +608,10 @@ def _lib_sync():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    # the warm-up also brings the device out of the idle state the host-side
+    # parity check leaves it in (clocks ramp over tens of milliseconds)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--plain-runs", type=int, default=20)
@@ -618,6 +630,8 @@ def main():
                     help="only the live stream ingest, side-output and filter-style legs")
     ap.add_argument("--estimation-only", action="store_true",
                     help="only the batched estimation legs of config 1 (for rocprofv3 runs)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (default launch shapes)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="only the mixed-size batch (config 2)")
     ap.add_argument("--probe-copy", action="store_true",
@@ -634,6 +648,8 @@ def main():
     from fsim.sim_algorithm import PlainPipeline, Reconstructor
 
     n, B = args.n, args.batch
+    global TUNE
+    TUNE = not args.no_tune
     _lib.init(0)
     if args.probe_copy:
         probe_copy(args.steps)
@@ -662,6 +678,11 @@ def main():
         sps.append(sp)
     param = synthetic.make_param(sps[0], n)
     rec = Reconstructor(param)
+    tuned = None
+    if TUNE:
+        from fsim import autotune
+        tuned = autotune.tune(rec, count=B, reps=3)
+        log(f"autotune: {tuned}")
     frame_stack = np.stack([stacks[i % 4] for i in range(B)])
     raw_buf = DeviceBuffer(frame_stack.nbytes)
     raw_buf.upload(frame_stack)
@@ -684,7 +705,8 @@ def main():
         _lib.call("fsim_stage_times", ms, cnt, 5)
         print(json.dumps({"profile_only": True, "batch": B, "steps": args.steps, "n": n,
                           "recons_per_s": round(B * args.steps / el, 2),
-                          "stage_ms": {STAGES[i]: round(ms[i] / max(cnt[i], 1), 4) for i in range(5)}}),
+                          "stage_ms": {STAGES[i]: round(ms[i] / max(cnt[i], 1), 4) for i in range(5)},
+                          "autotune": tuned}),
               flush=True)
         return
 
@@ -831,6 +853,7 @@ def main():
         "bytes_moved_per_recon": sum(sb),
         "hbm_frac_of_bytes_moved": round(hbm_frac_moved, 4),
         "stages": stages,
+        "autotune": tuned,
         "latency_single_ms": {"p50": round(pct(lat, 50), 3), "p99": round(pct(lat, 99), 3)},
         "latency_batch_ms": {"p50": round(pct(blat, 50), 3), "p99": round(pct(blat, 99), 3)},
         "plain": {"recons_per_s": round(plain_value, 2), "p50_ms": round(pct(plat, 50), 3),

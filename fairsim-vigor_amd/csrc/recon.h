@@ -55,6 +55,21 @@ struct JobList {
     bool raw_aligned;
 };
 
+// Launch choices of the pipeline that the autotuner picks per size class:
+// sep = 1: the separation with all units of a row in one wavefront (where the
+// frames fit: n <= 512, D*P <= 9), 0: one unit per block; col: column-pass
+// candidate (elements per thread, columns per block; col_shape).
+constexpr int kColCands = 4;
+struct PipeCfg {
+    int sep = 1;
+    int col = 0;
+};
+bool pipe_cfg_lookup(int n, int D, int P, int Cs, bool half, PipeCfg &cfg);
+int col_shape(int n, int col, int *e, int *cw);
+bool sep_all_eligible(int n, int D, int P);
+int launch_pipeline_cfg(int n, const JobList &jl, int njobs, int D, int P, int Cs, bool half, const PipeCfg &cfg,
+                        hipStream_t s);
+
 bool recon_size_ok(int n);
 int gather_cw(int n);   // column tile width of the gather at raw side n
 int launch_pipeline(int n, const JobList &jl, int njobs, int D, int P, int Cs, bool half, hipStream_t s);

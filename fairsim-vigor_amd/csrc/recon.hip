@@ -58,9 +58,6 @@
 #ifndef FSIM_B_TILED
 #define FSIM_B_TILED 1
 #endif
-#ifndef FSIM_SEP_ALL
-#define FSIM_SEP_ALL 1
-#endif
 #ifndef FSIM_STAGE_STORE
 #define FSIM_STAGE_STORE 0   // separation pass: stores from registers
 #endif
@@ -494,29 +491,29 @@ constexpr int lds_waves_per_simd(int threads, int lds_bytes, int cap) {
     return w < 1 ? 1 : (w > cap ? cap : w);
 }
 
-template <int N>
+// E0: elements per thread (the Stockham radix; 0: ColE), CW0: columns per
+// block (capped by the 1024-thread block).  The pipeline picks one of the
+// candidates kColE / kColCW per size (tuned or default).
+template <int N, int E0, int CW0>
 struct ColShiftTile {
-#ifdef FSIM_COL_E
-    static constexpr int e = FSIM_COL_E < N / 2 ? FSIM_COL_E : ColE<N>::value;
-#else
-    static constexpr int e = ColE<N>::value;
-#endif
+    static constexpr int e = (E0 > 0 && E0 <= 16 && E0 < N / 2) ? E0 : ColE<N>::value;
     static constexpr int t = N / e;
     static constexpr int cw0 = ColTile<N, t, e>::value;
-    static constexpr int cw = cw0 < FSIM_COL_CW ? cw0 : FSIM_COL_CW;
+    static constexpr int cw = cw0 < CW0 ? cw0 : CW0;
     static constexpr int value = t * cw;
     static constexpr int lds = FftLds<N, e>::len * cw;
     static constexpr bool tw_fits = (lds + N) * 8 <= 163840;
     static constexpr int wpe = lds_waves_per_simd(value, (lds + N) * 8, FSIM_COL_WPE);
 };
 
-template <int N>
-__global__ void __launch_bounds__(ColShiftTile<N>::value, ColShiftTile<N>::wpe) k_col_shift(JobList jl, int nsh_all, int total,
-                                                                     const float2 *__restrict__ tw) {
-    constexpr int E = ColShiftTile<N>::e, T = N / E;
-    constexpr int CW = ColShiftTile<N>::cw;
+template <int N, int E0, int CW0>
+__global__ void __launch_bounds__((ColShiftTile<N, E0, CW0>::value), (ColShiftTile<N, E0, CW0>::wpe))
+    k_col_shift(JobList jl, int nsh_all, int total, const float2 *__restrict__ tw) {
+    using CT = ColShiftTile<N, E0, CW0>;
+    constexpr int E = CT::e, T = N / E;
+    constexpr int CW = CT::cw;
     constexpr int tiles = N / CW;
-    __shared__ float2 lds[ColShiftTile<N>::lds];
+    __shared__ float2 lds[CT::lds];
     // nsh_all = D * (Cs - 1) shifted components per stack; total = stacks *
     // nsh_all * tiles
     int w;
@@ -526,9 +523,9 @@ __global__ void __launch_bounds__(ColShiftTile<N>::value, ColShiftTile<N>::wpe) 
     } else {
         w = blockIdx.x;
     }
-    using TW = TwLds<N, ColShiftTile<N>::tw_fits>;
+    using TW = TwLds<N, CT::tw_fits>;
     __shared__ float2 twl[TW::len];
-    const float2 *tws = tw_table<TW::tl, ColShiftTile<N>::value>(twl, tw);
+    const float2 *tws = tw_table<TW::tl, CT::value>(twl, tw);
     int tile, u;
     if (FSIM_TILE_GROUPS) {
         constexpr int TG = tiles < 8 ? tiles : 8;
@@ -598,6 +595,9 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = lds[rr * LP + t + T * m];
+    // the transform's first exchange rewrites the row's region: every thread
+    // of the row (two wavefronts from n = 2048 on) has read it first
+    pass_sync<RowWave<N>::value>();
     fft_regs<N, E, true, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
     // row factor rx(x) of the sub-pixel ramp (the column factor was applied
     // by the column pass)
@@ -1108,8 +1108,30 @@ static void stage_mark(int stage, hipStream_t s, hipEvent_t &start) {
     start = e;
 }
 
+// Column-pass candidates (elements per thread E0, columns per block CW0; E0 =
+// 0: the size's default element count): 0 (0, 8), 1 (16, 8), 2 (16, 16),
+// 3 (8, 16).
+template <int N, int E0, int CW0>
+static void launch_col_shift(const JobList &jl, int njobs, int D, int C, const float2 *twn, hipStream_t s) {
+    using CT = ColShiftTile<N, E0, CW0>;
+    const int nsh_all = D * (C - 1), total = njobs * nsh_all * (N / CT::cw);
+    hipLaunchKernelGGL((k_col_shift<N, E0, CW0>), dim3(FSIM_COL_XCD ? xcd_grid(total) : (unsigned)total),
+                       dim3(CT::value), 0, s, jl, nsh_all, total, twn);
+}
+
 template <int N>
-static int launch_pipeline_t(const JobList &jl, int njobs, int D, int P, int C, bool half, hipStream_t s) {
+static void col_candidate_shape(int col, int &e, int &cw) {
+    switch (col) {
+        case 1: e = ColShiftTile<N, 16, 8>::e; cw = ColShiftTile<N, 16, 8>::cw; break;
+        case 2: e = ColShiftTile<N, 16, 16>::e; cw = ColShiftTile<N, 16, 16>::cw; break;
+        case 3: e = ColShiftTile<N, 8, 16>::e; cw = ColShiftTile<N, 8, 16>::cw; break;
+        default: e = ColShiftTile<N, 0, FSIM_COL_CW>::e; cw = ColShiftTile<N, 0, FSIM_COL_CW>::cw; break;
+    }
+}
+
+template <int N>
+static int launch_pipeline_t(const JobList &jl, int njobs, int D, int P, int C, bool half, const PipeCfg &cfg,
+                             hipStream_t s) {
     // C: stored components per direction
     constexpr int M = 2 * N;
     const float2 *twn = twiddles(N), *twm = twiddles(M);
@@ -1122,7 +1144,7 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int P, int C, 
         const int gx = ((nrb + 7) / 8) * 8 * units;
         bool done = false;
         if constexpr (N <= 512) {
-            if (FSIM_SEP_ALL && jl.raw_aligned && D * P <= 9) {
+            if (cfg.sep != 0 && jl.raw_aligned && D * P <= 9) {
                 hipLaunchKernelGGL((k_sep_rows_all<N, 9>), dim3(nrb, njobs), dim3(T * RW), 0, s, jl, (int)half, twn);
                 done = true;
             }
@@ -1137,13 +1159,13 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int P, int C, 
         stage_mark(0, s, ev);
     }
     if (C > 1) {
-        {
-            constexpr int E = ColShiftTile<N>::e, T = N / E, CW = ColShiftTile<N>::cw;
-            const int nsh_all = D * (C - 1), total = njobs * nsh_all * (N / CW);
-            hipLaunchKernelGGL(k_col_shift<N>, dim3(FSIM_COL_XCD ? xcd_grid(total) : (unsigned)total), dim3(T * CW),
-                               0, s, jl, nsh_all, total, twn);
-            stage_mark(1, s, ev);
+        switch (cfg.col) {
+            case 1: launch_col_shift<N, 16, 8>(jl, njobs, D, C, twn, s); break;
+            case 2: launch_col_shift<N, 16, 16>(jl, njobs, D, C, twn, s); break;
+            case 3: launch_col_shift<N, 8, 16>(jl, njobs, D, C, twn, s); break;
+            default: launch_col_shift<N, 0, FSIM_COL_CW>(jl, njobs, D, C, twn, s); break;
         }
+        stage_mark(1, s, ev);
         constexpr int E = FftE<N>::value, T = N / E;
         constexpr int RW = T >= 256 ? 1 : 256 / T;
         hipLaunchKernelGGL(k_row_shift<N>, dim3(D * (C - 1) * (N / RW), njobs), dim3(T * RW), 0, s, jl, twn);
@@ -1186,15 +1208,37 @@ int gather_cw(int n) {
     }
 }
 
-int launch_pipeline(int n, const JobList &jl, int njobs, int D, int P, int C, bool half, hipStream_t s) {
-    // C: stored components per direction (ReconConst::Cs)
+int col_shape(int n, int col, int *e, int *cw) {
     switch (n) {
-        case 64: return launch_pipeline_t<64>(jl, njobs, D, P, C, half, s);
-        case 128: return launch_pipeline_t<128>(jl, njobs, D, P, C, half, s);
-        case 256: return launch_pipeline_t<256>(jl, njobs, D, P, C, half, s);
-        case 512: return launch_pipeline_t<512>(jl, njobs, D, P, C, half, s);
-        case 1024: return launch_pipeline_t<1024>(jl, njobs, D, P, C, half, s);
-        case 2048: return launch_pipeline_t<2048>(jl, njobs, D, P, C, half, s);
+        case 64: col_candidate_shape<64>(col, *e, *cw); return FSIM_OK;
+        case 128: col_candidate_shape<128>(col, *e, *cw); return FSIM_OK;
+        case 256: col_candidate_shape<256>(col, *e, *cw); return FSIM_OK;
+        case 512: col_candidate_shape<512>(col, *e, *cw); return FSIM_OK;
+        case 1024: col_candidate_shape<1024>(col, *e, *cw); return FSIM_OK;
+        case 2048: col_candidate_shape<2048>(col, *e, *cw); return FSIM_OK;
+        default: return fail(FSIM_ERR_ARG, "col_shape: unsupported size");
+    }
+}
+
+bool sep_all_eligible(int n, int D, int P) { return n <= 512 && D * P <= 9; }
+
+int launch_pipeline(int n, const JobList &jl, int njobs, int D, int P, int C, bool half, hipStream_t s) {
+    PipeCfg cfg;
+    if (!pipe_cfg_lookup(n, D, P, C, half, cfg)) cfg = PipeCfg{};
+    return launch_pipeline_cfg(n, jl, njobs, D, P, C, half, cfg, s);
+}
+
+int launch_pipeline_cfg(int n, const JobList &jl, int njobs, int D, int P, int C, bool half, const PipeCfg &cfg,
+                        hipStream_t s) {
+    // C: stored components per direction (ReconConst::Cs)
+    if (cfg.col < 0 || cfg.col >= kColCands) return fail(FSIM_ERR_ARG, "pipeline: bad column-pass candidate");
+    switch (n) {
+        case 64: return launch_pipeline_t<64>(jl, njobs, D, P, C, half, cfg, s);
+        case 128: return launch_pipeline_t<128>(jl, njobs, D, P, C, half, cfg, s);
+        case 256: return launch_pipeline_t<256>(jl, njobs, D, P, C, half, cfg, s);
+        case 512: return launch_pipeline_t<512>(jl, njobs, D, P, C, half, cfg, s);
+        case 1024: return launch_pipeline_t<1024>(jl, njobs, D, P, C, half, cfg, s);
+        case 2048: return launch_pipeline_t<2048>(jl, njobs, D, P, C, half, cfg, s);
         default: return fail(FSIM_ERR_ARG, "reconstruct: n must be a power of two in [64, 2048]");
     }
 }
@@ -1462,6 +1506,17 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     (void)hipFree(atmp);
     (void)hipFree(fdev);
     *out = ch;
+    return FSIM_OK;
+}
+
+int fsim_channel_class(fsim_channel ch, int *n, int *D, int *P, int *Cs, int *half) {
+    if (!ch || !n || !D || !P || !Cs || !half) return fail(FSIM_ERR_ARG, "channel_class: bad arguments");
+    const ReconConst &K = ch->K;
+    *n = K.n;
+    *D = K.D;
+    *P = K.P;
+    *Cs = K.Cs;
+    *half = K.half;
     return FSIM_OK;
 }
 

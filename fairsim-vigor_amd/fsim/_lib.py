@@ -106,6 +106,12 @@ class SERVER_STATS(C.Structure):
                 ("p99_ms", C.c_double), ("max_ms", C.c_double)]
 
 
+class TUNE_RESULT(C.Structure):
+    """fsim_tune_result (include/fsim.h)."""
+    _fields_ = [("sep", C.c_int), ("col", C.c_int), ("col_elems", C.c_int), ("col_width", C.c_int),
+                ("candidates", C.c_int), ("ms_best", C.c_double), ("ms_default", C.c_double)]
+
+
 FSIM_SUBMIT_DEVICE = 1
 
 _P = C.c_void_p
@@ -181,6 +187,13 @@ _SIGS = {
     "fsim_batch_run": [_P, _P, _P, _P],
     "fsim_batch_info": [_P, _P, _P, _P],
     "fsim_batch_destroy": [_P],
+    "fsim_channel_class": [_P, _PI, _PI, _PI, _PI, _PI],
+    "fsim_autotune": [_P, _I, _I, C.POINTER(TUNE_RESULT)],
+    "fsim_autotune_set": [_I, _I, _I, _I, _I, _I, _I],
+    "fsim_autotune_get": [_I, _I, _I, _I, _I, _PI, _PI],
+    "fsim_autotune_clear": [],
+    "fsim_autotune_save": [C.c_char_p],
+    "fsim_autotune_load": [C.c_char_p],
     "fsim_rl_deconvolve": [_P, _P, _I, _I],
     "fsim_clip_scale": [_P, _I, _I],
     "fsim_widefield": [_P, _P, _P, _I, _I, _P],

@@ -282,6 +282,11 @@ typedef struct {
 
 int fsim_channel_create(const fsim_recon_params *p, fsim_otf otf, fsim_channel *out);
 int fsim_channel_destroy(fsim_channel ch);
+/* The channel's size class: raw side, directions, phases, stored components
+ * per direction (bands with the Hermitian pairing, 2*bands-1 without) and
+ * whether the result is Hermitian (real OTF, mode 0).  No counterpart in the
+ * reference; the key of fsim_batch_* classes and of fsim_autotune_*. */
+int fsim_channel_class(fsim_channel ch, int *n, int *D, int *P, int *Cs, int *half);
 
 /* Reconstruct `count` stacks of one channel.  raw_dev: count stacks back to
  * back in device memory; out_dev: count images of (2n)^2 float.  Work buffers
@@ -410,6 +415,34 @@ int fsim_batch_create(const fsim_channel *channels, int count, int use_graph, fs
 int fsim_batch_run(fsim_batch b, const uint16_t *const *raw, float *const *out, void *stream);
 int fsim_batch_info(fsim_batch b, int *nclasses, int *captured, int64_t *device_bytes);
 int fsim_batch_destroy(fsim_batch b);
+
+/* ---- autotuner of the pipeline's launch shapes --------------------------
+ * No counterpart in the reference (its cuFFT plans are fixed per size,
+ * accel/FFTProvider.java:63-102).  Per size class (n, D, P, stored
+ * components Cs, Hermitian result half) the pipeline chooses: sep = 1 the
+ * separation with all units of a row in one wavefront (n <= 512, D*P <= 9),
+ * 0 one unit per block; col = the column pass's candidate 0..3 (elements per
+ * thread = Stockham radix 8 or 16, columns per block 8 or 16; the shape is
+ * reported in col_elems / col_width).  fsim_autotune times every candidate on
+ * the channel's class (count zero-filled stacks; after a 30 ms warm-up, three
+ * interleaved rounds of `reps` timed launches each, HIP events, best round
+ * per candidate) and caches the fastest -- the default unless a candidate is
+ * more than 2% faster -- for the process; every later launch of
+ * that class (fsim_reconstruct, _params, the server, batches created after)
+ * uses it.  save/load persist the cache as text ("fsim-autotune 1" then
+ * "n D P Cs half sep col count ms" per line); load merges and rejects a
+ * malformed file whole.  set pins a choice by hand, get returns -1s when the
+ * class has no entry.  These calls do not need a GPU except fsim_autotune. */
+typedef struct {
+    int sep, col, col_elems, col_width, candidates;
+    double ms_best, ms_default;   /* per launch of `count` stacks */
+} fsim_tune_result;
+int fsim_autotune(fsim_channel ch, int count, int reps, fsim_tune_result *res);
+int fsim_autotune_set(int n, int D, int P, int Cs, int half, int sep, int col);
+int fsim_autotune_get(int n, int D, int P, int Cs, int half, int *sep, int *col);
+int fsim_autotune_clear(void);
+int fsim_autotune_save(const char *path);
+int fsim_autotune_load(const char *path);
 
 /* ---- live stream ingest (transport/ImageWrapper.java,
  * livemode/SimSequenceExtractor.java) -----------------------------------------

@@ -1,0 +1,56 @@
+"""The autotuner's cache without a GPU: set/get, save/load round trip, merge,
+and refusal of malformed files (fsim_autotune_set/get/save/load/clear)."""
+import pytest
+
+from fsim import autotune
+from fsim._lib import FsimError
+
+
+@pytest.fixture(autouse=True)
+def _clean():
+    autotune.clear()
+    yield
+    autotune.clear()
+
+
+def test_set_get_and_unknown_class():
+    assert autotune.get_choice(512, 3, 3, 2, 1) is None
+    autotune.set_choice(512, 3, 3, 2, 1, 0, 2)
+    assert autotune.get_choice(512, 3, 3, 2, 1) == (0, 2)
+    assert autotune.get_choice(512, 3, 3, 2, 0) is None      # another class
+    with pytest.raises(FsimError):
+        autotune.set_choice(500, 3, 3, 2, 1, 0, 0)             # unsupported size
+    with pytest.raises(FsimError):
+        autotune.set_choice(512, 3, 3, 2, 1, 2, 0)             # sep not 0/1
+    with pytest.raises(FsimError):
+        autotune.set_choice(512, 3, 3, 2, 1, 1, 4)             # no such column candidate
+
+
+def test_save_load_round_trip_and_merge(tmp_path):
+    autotune.set_choice(512, 3, 3, 2, 1, 1, 0)
+    autotune.set_choice(1024, 3, 5, 3, 1, 0, 3)
+    path = tmp_path / "tune.txt"
+    autotune.save(path)
+    lines = path.read_text().splitlines()
+    assert lines[0] == "fsim-autotune 1" and len(lines) == 3
+    autotune.clear()
+    assert autotune.get_choice(512, 3, 3, 2, 1) is None
+    autotune.set_choice(256, 3, 3, 2, 1, 0, 1)                 # kept by the merge
+    autotune.load(path)
+    assert autotune.get_choice(512, 3, 3, 2, 1) == (1, 0)
+    assert autotune.get_choice(1024, 3, 5, 3, 1) == (0, 3)
+    assert autotune.get_choice(256, 3, 3, 2, 1) == (0, 1)
+
+
+@pytest.mark.parametrize("text", ["", "fsim-autotune 2\n", "other 1\n",
+                                  "fsim-autotune 1\n512 3 3 2 1 1 9 16 0.5\n",
+                                  "fsim-autotune 1\n512 3 3 2 1 1 0 16\n",
+                                  "fsim-autotune 1\n512 3 3 2 7 1 0 16 0.5\n"])
+def test_malformed_file_changes_nothing(tmp_path, text):
+    autotune.set_choice(128, 3, 3, 2, 1, 0, 1)
+    path = tmp_path / "bad.txt"
+    path.write_text(text)
+    with pytest.raises(FsimError):
+        autotune.load(path)
+    assert autotune.get_choice(128, 3, 3, 2, 1) == (0, 1)
+    assert autotune.get_choice(512, 3, 3, 2, 1) is None

@@ -1,0 +1,67 @@
+"""GPU test of the autotuner: every candidate launch shape gives the same
+image (to FP32 round-off: the candidates differ in FFT radix), the tuned
+choice is one of them and the tuned pipeline matches the FP32 oracle."""
+import numpy as np
+import pytest
+
+from oracle import fp64, ref
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def tuned_channel(fsim_lib):
+    from fsim import autotune, synthetic
+    from fsim.sim_algorithm import Reconstructor
+    autotune.clear()
+    n = 128
+    raw, sp = synthetic.sim_stack(n=n, seed=95, nbeads=60, nfil=8)
+    rec = Reconstructor(synthetic.make_param(sp, n))
+    yield n, raw, sp, rec
+    autotune.clear()
+
+
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_candidates_agree_large(fsim_lib, n):
+    """At the sizes whose column transforms take 16 elements per thread by
+    default, the 8-element candidate runs four Stockham passes."""
+    from fsim import autotune, synthetic
+    from fsim.sim_algorithm import Reconstructor
+    autotune.clear()
+    raw, sp = synthetic.sim_stack(n=n, seed=96, nbeads=300 * n // 512, nfil=40 * n // 512)
+    rec = Reconstructor(synthetic.make_param(sp, n))
+    cls = autotune.class_of(rec)
+    default = rec.reconstruct_host(raw)
+    try:
+        for col in range(4):
+            print(f"[autotune] n={n} col={col}", flush=True)
+            autotune.set_choice(*cls, 0, col)
+            e = fp64.rel_l2(rec.reconstruct_host(raw), default)
+            print(f"  vs default {e:.3e}", flush=True)
+            assert e < 5e-6
+    finally:
+        autotune.clear()
+
+
+def test_candidates_agree_and_tuned_matches_oracle(tuned_channel):
+    from fsim import autotune
+    n, raw, sp, rec = tuned_channel
+    cls = autotune.class_of(rec)
+    assert cls == (n, 3, 3, 2, 1)
+    default = rec.reconstruct_host(raw)
+    for sep in (0, 1):
+        for col in range(4):
+            print(f"[autotune] n={n} sep={sep} col={col}", flush=True)
+            autotune.set_choice(*cls, sep, col)
+            got = rec.reconstruct_host(raw)
+            e = fp64.rel_l2(got, default)
+            print(f"  vs default {e:.3e}")
+            assert e < 5e-6
+    autotune.clear()
+    res = autotune.tune(rec, count=4, reps=2)
+    print(res)
+    assert res["candidates"] >= 2 and 0 <= res["col"] < 4 and res["sep"] in (0, 1)
+    assert res["ms_best"] > 0 and res["ms_best"] <= res["ms_default"]
+    assert autotune.get_choice(*cls) == (res["sep"], res["col"])
+    got = rec.reconstruct_host(raw)
+    assert fp64.rel_l2(got, ref.recon32(ref.recon_dict(sp, n), raw)) < 1e-4

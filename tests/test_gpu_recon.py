@@ -30,7 +30,7 @@ def _setup(n, seed, nphase=3, nband=2, harmonic2=0.0, lam=525.0, fade_px=10, apo
 
 
 @pytest.mark.parametrize("n,seed,mode", [(64, 1, 0), (128, 2, 0), (256, 3, 0), (512, 4, 0),
-                                         (1024, 5, 0), (64, 1, 1), (512, 4, 1)])
+                                         (1024, 5, 0), (2048, 6, 0), (64, 1, 1), (512, 4, 1)])
 def test_fused_matches_oracle(fsim_lib, n, seed, mode):
     """mode 0: neg components derived from pos (Hermitian pairing), mode 1:
     every component computed."""
