@@ -192,6 +192,125 @@ def serve_mixed_stream(nreq, rate):
             "saturation_roofline_frac_pcie_inclusive": round(nreq / sat_el * 434 * mean_n2 / 8e12, 4)}
 
 
+_STACKS = {}
+
+
+def _stack(n, seed, **kw):
+    """Seeded synthetic stack, generated once per process (2048^2 takes
+    seconds on the host)."""
+    from fsim import synthetic
+    key = (n, seed, tuple(sorted(kw.items())))
+    if key not in _STACKS:
+        _STACKS[key] = synthetic.sim_stack(n=n, seed=seed, **kw)
+    return _STACKS[key]
+
+
+def serve_large_fov_mix(nreq=400, rate=200.0, frac=0.05, tps=8):
+    """Config 4 served: the config-3 stream (Poisson `rate` req/s, sizes
+    256/512/1024 at .5/.35/.15, 3 channels, pinned host buffers) with a share
+    `frac` of large-FOV requests: 2048^2 x 3 x 3 uint16 stacks, `tps`
+    timepoints each, resident on the device; the k0/phase fit on the first
+    timepoint (fsim_estimate on its float32 frames), then the request's
+    timepoints through fsim_server_submit_params with that fit, admitted
+    against the server's memory budget.  A worker thread runs the large
+    requests (fit, submit, wait) so the arrival loop keeps its schedule;
+    latency of a large request = arrival -> its last image ready."""
+    import queue
+    import threading
+    import numpy as np
+    from fsim import synthetic
+    from fsim.linalg import DeviceBuffer
+    from fsim.livemode import ReconstructionServer, pinned
+    from fsim.sim_algorithm import Reconstructor, estimate_batch, params_from_estimates
+    rng = np.random.default_rng(4321)
+    sizes, probs = [256, 512, 1024], [0.5, 0.35, 0.15]
+    lams = [525.0, 600.0, 670.0]
+    srv = ReconstructionServer(mem_budget=24 << 30, max_batch=16, max_queue=4096, nslots=5)
+    chans, raws = {}, {}
+    for n in sizes:
+        for ci, lam in enumerate(lams):
+            raw, sp = synthetic.sim_stack(n=n, seed=100 + ci, lam=lam)
+            chans[(n, ci)] = Reconstructor(synthetic.make_param(sp, n, lam=lam))
+            buf, _ = pinned(raw.shape, np.uint16)
+            buf[...] = raw
+            raws[(n, ci)] = buf
+    ids = {k: srv.add_channel(v) for k, v in chans.items()}
+    pool = {n: [pinned((2 * n, 2 * n), np.float32)[0] for _ in range(8)] for n in sizes}
+    # the large-FOV channel: 65 nm pixels, 2048^2; its parameters come from
+    # the fit of each request's first timepoint
+    nl = 2048
+    raw_l, sp_l = _stack(nl, 7, nbeads=1200, nfil=160)
+    param_l = synthetic.make_param(sp_l, nl)
+    rec_l = Reconstructor(param_l)
+    lid = srv.add_channel(rec_l)
+    rb_l = DeviceBuffer(tps * raw_l.nbytes)
+    rb_l.upload(np.ascontiguousarray(np.broadcast_to(raw_l, (tps,) + raw_l.shape)))
+    ib_l = DeviceBuffer(raw_l.size * 4)
+    ib_l.upload(raw_l.astype(np.float32))
+    ob_l = DeviceBuffer(tps * rec_l.out_bytes())
+    jobs = queue.Queue()
+    large_lat, k_err = [], []
+
+    def worker():
+        while True:
+            item = jobs.get()
+            if item is None:
+                return
+            t_arr = item
+            res = estimate_batch(param_l, 1, 0.6, images_dev=ib_l.value, count=1, raw=True)
+            k_err.append(max(abs(res[d].px - sp_l["px"][d]) + abs(res[d].py - sp_l["py"][d]) for d in range(3)))
+            prm = params_from_estimates(param_l, res, 1)[0]
+            t = srv.submit_params(lid, rb_l.value, ob_l.value, tps, prm, device=True)
+            srv.wait(t)
+            large_lat.append((time.perf_counter() - t_arr) * 1e3)
+
+    th = threading.Thread(target=worker)
+    th.start()
+    # one large request before the stream: the first fit allocates the
+    # estimation buffers, the first submit the server slot
+    jobs.put(time.perf_counter())
+    while not large_lat:
+        time.sleep(0.01)
+    large_lat.clear()
+    k_err.clear()
+    srv.stats(reset=True)
+    gaps = rng.exponential(1.0 / rate, nreq)
+    kinds = []
+    for _ in range(nreq):
+        if rng.random() < frac:
+            kinds.append(None)
+        else:
+            kinds.append((int(rng.choice(sizes, p=probs)), int(rng.integers(3))))
+    tickets = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for i, k in enumerate(kinds):
+        t_next += gaps[i]
+        while time.perf_counter() < t_next:
+            pass
+        if k is None:
+            jobs.put(time.perf_counter())
+        else:
+            tickets.append(srv.submit(ids[k], raws[k], pool[k[0]][i % 8]))
+    small_lat = [srv.wait(t) for t in tickets]
+    jobs.put(None)
+    th.join()
+    el = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    nlarge = sum(1 for k in kinds if k is None)
+    return {"arrivals": f"Poisson {rate:g} req/s, {nreq} requests: {nlarge} large-FOV ({tps} x 2048^2 timepoints, "
+                        f"k0 fit on the first) and {nreq - nlarge} of sizes 256/512/1024 at .5/.35/.15, 3 channels",
+            "offered_req_per_s": round(nreq / el, 1),
+            "small_p50_ms": round(float(np.percentile(small_lat, 50)), 3),
+            "small_p99_ms": round(float(np.percentile(small_lat, 99)), 3),
+            "large_p50_ms": round(float(np.percentile(large_lat, 50)), 3) if large_lat else None,
+            "large_p99_ms": round(float(np.percentile(large_lat, 99)), 3) if large_lat else None,
+            "large_fit_max_k_error_px": round(float(max(k_err)), 4) if k_err else None,
+            "server_device_bytes": st.get("device_bytes"), "failed": st["failed"],
+            "rejected": st["rejected"]}
+
+
 def batched_mixed(nper=32, reps=10):
     """Config 2: one batch of 96 stacks -- 3 channels (525/600/670 nm) x sizes
     256/512/1024, 32 of each size, stacks and outputs resident in HBM.  The
@@ -279,7 +398,7 @@ def large_fov(batch=4, steps=5):
     from fsim.sim_algorithm import Reconstructor
     from oracle import fp64, ref
     n = 2048
-    raw, sp = synthetic.sim_stack(n=n, seed=7, nbeads=1200, nfil=160)
+    raw, sp = _stack(n, 7, nbeads=1200, nfil=160)
     rec = Reconstructor(synthetic.make_param(sp, n))
     if TUNE:
         from fsim import autotune
@@ -632,6 +751,8 @@ def main():
                     help="only the batched estimation legs of config 1 (for rocprofv3 runs)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (default launch shapes)")
+    ap.add_argument("--serve-fov-only", action="store_true",
+                    help="only the served stream with large-FOV requests (config 4 served)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="only the mixed-size batch (config 2)")
     ap.add_argument("--probe-copy", action="store_true",
@@ -659,6 +780,9 @@ def main():
                           "side_outputs_512": side_outputs(), "filter_styles_512": filter_styles(),
                           "config_io": config_io()}),
               flush=True)
+        return
+    if args.serve_fov_only:
+        print(json.dumps({"serve_fov_only": True, "served_large_fov_mix": serve_large_fov_mix()}), flush=True)
         return
     if args.mixed_only:
         print(json.dumps({"mixed_only": True, "batched_mixed_96": batched_mixed()}), flush=True)
@@ -806,6 +930,8 @@ def main():
         configs["batched_mixed_96"] = batched_mixed()
         log("config 4: 2048^2 large field of view")
         configs["large_fov_2048"] = large_fov()
+        log("config 4 served: large-FOV requests in the config-3 stream")
+        configs["served_large_fov_mix"] = serve_large_fov_mix()
         log("live stream ingest: packets -> sequences -> server")
         configs["stream_ingest_512"] = stream_ingest()
         log("live-mode side outputs: widefield, registration")

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 #include "filters.h"
 
@@ -82,4 +83,26 @@ struct fsim_channel_s;
 namespace fsim {
 const ReconConst *channel_device_const(const fsim_channel_s *ch);
 const ReconConst &channel_host_const(const fsim_channel_s *ch);
+
+// Device state of reconstructions with their own SIM parameters, grown on
+// demand: per parameter set constants, (1/Wiener) * apodization table, ramp
+// tables; per stack jobs.
+struct MultiWork {
+    ReconConst *dK = nullptr;
+    float *wa = nullptr;
+    float2 *ramp = nullptr;
+    double *f = nullptr;
+    ShiftSet *sh = nullptr;
+    Job *jobs = nullptr;
+    int cap_sets = 0, cap_jobs = 0;
+    hipEvent_t uploaded = nullptr;   // recorded after the uploads from the host arrays below
+    std::vector<ReconConst> hK;
+    std::vector<double> hf;
+    std::vector<ShiftSet> hsh;
+    std::vector<Job> hjobs;
+};
+void multiwork_release(MultiWork &w);
+size_t multiwork_bytes(int n, int count, int nsets);
+int reconstruct_params_with(const fsim_channel_s *ch, MultiWork &w, float2 *A, float2 *B, const uint16_t *raw, float *out,
+                            int count, const fsim_sim_params *sp, int sp_step, hipStream_t s);
 }  // namespace fsim

@@ -32,6 +32,7 @@
 #include "simmath.h"
 
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <set>
 #include <utility>
@@ -1323,23 +1324,6 @@ int build_channel_const(const fsim_recon_params &p, const Otf &o, bool sym, Reco
 
 using namespace fsim;
 
-// Device state of reconstructions with per-stack parameters
-// (fsim_reconstruct_params), grown on demand.
-struct MultiWork {
-    ReconConst *dK = nullptr;
-    float *wa = nullptr;
-    float2 *ramp = nullptr;
-    double *f = nullptr;
-    ShiftSet *sh = nullptr;
-    Job *jobs = nullptr;
-    int cap = 0;
-    hipEvent_t uploaded = nullptr;   // recorded after the uploads from the host arrays below
-    std::vector<ReconConst> hK;
-    std::vector<double> hf;
-    std::vector<ShiftSet> hsh;
-    std::vector<Job> hjobs;
-};
-
 struct fsim_channel_s {
     fsim_recon_params p;
     Otf o;
@@ -1368,13 +1352,7 @@ static void channel_free(fsim_channel_s *ch) {
     (void)hipFree(ch->fade);
     (void)hipFree(ch->A);
     (void)hipFree(ch->B);
-    (void)hipFree(ch->mw.dK);
-    (void)hipFree(ch->mw.wa);
-    (void)hipFree(ch->mw.ramp);
-    (void)hipFree(ch->mw.f);
-    (void)hipFree(ch->mw.sh);
-    (void)hipFree(ch->mw.jobs);
-    if (ch->mw.uploaded) (void)hipEventDestroy(ch->mw.uploaded);
+    multiwork_release(ch->mw);
 }
 
 // ramp tables of a set of stacks' constants: fx/fy of each direction and
@@ -1416,6 +1394,142 @@ static int ensure_workspace(fsim_channel_s *ch, int count, hipStream_t s) {
     ch->cap = count;
     return FSIM_OK;
 }
+
+namespace fsim {
+
+void multiwork_release(MultiWork &w) {
+    (void)hipFree(w.dK);
+    (void)hipFree(w.wa);
+    (void)hipFree(w.ramp);
+    (void)hipFree(w.f);
+    (void)hipFree(w.sh);
+    (void)hipFree(w.jobs);
+    if (w.uploaded) (void)hipEventDestroy(w.uploaded);
+    w = MultiWork{};
+}
+
+size_t multiwork_bytes(int n, int count, int nsets) {
+    const size_t M = 2 * (size_t)n;
+    return (size_t)nsets * (sizeof(ReconConst) + M * M * sizeof(float) +
+                            (size_t)kMaxDir * kMaxComp * 2 * n * sizeof(float2) +
+                            (size_t)kMaxDir * kMaxComp * 2 * sizeof(double) + sizeof(ShiftSet)) +
+           (size_t)count * sizeof(Job);
+}
+
+// Reconstruction of `count` stacks with their own SIM parameters: sp[st *
+// sp_step] for stack st (sp_step 0: one set for all).  Constants, Wiener
+// tables and ramps are built once per run of equal consecutive sets (the
+// timepoints of one fit share them).  A and B: work buffers for `count`
+// stacks; w: the state that grows with the sets and jobs.
+int reconstruct_params_with(const fsim_channel_s *ch, MultiWork &w, float2 *A, float2 *B, const uint16_t *raw, float *out,
+                            int count, const fsim_sim_params *sp, int sp_step, hipStream_t s) {
+    const ReconConst &K0 = ch->K;
+    const int n = K0.n, M = 2 * n;
+    const size_t mm = (size_t)M * M, rper = (size_t)kMaxDir * kMaxComp * 2 * n;
+    // the previous call's uploads read the host arrays rewritten below
+    if (w.uploaded) FSIM_HIP(hipEventSynchronize(w.uploaded));
+    std::vector<int> set_of(count);
+    std::vector<int> first;
+    for (int st = 0; st < count; st++) {
+        const fsim_sim_params *cur = sp + (size_t)st * sp_step;
+        const bool same = st > 0 && (sp_step == 0 || std::memcmp(cur, cur - sp_step, sizeof(fsim_sim_params)) == 0);
+        if (!same) first.push_back(st);
+        set_of[st] = (int)first.size() - 1;
+    }
+    const int nset = (int)first.size();
+    // host constants of every set: the channel's settings with the set's
+    // shifts, phases and modulations
+    w.hK.resize(nset);
+    w.hsh.resize(nset);
+    for (int k = 0; k < nset; k++) {
+        const fsim_sim_params &q = sp[(size_t)first[k] * sp_step];
+        fsim_recon_params p = ch->p;
+        for (int d = 0; d < p.ndir; d++) {
+            p.px[d] = q.px[d];
+            p.py[d] = q.py[d];
+            for (int ph = 0; ph < p.nphase; ph++) p.phases[d][ph] = q.phases[d][ph];
+            for (int b = 0; b < p.nband; b++) p.modul[d][b] = q.modul[d][b];
+        }
+        const int rc = build_channel_const(p, ch->o, ch->sym, w.hK[k]);
+        if (rc) return rc;
+        ShiftSet &S = w.hsh[k];
+        S = ShiftSet{};
+        S.ndir = p.ndir;
+        S.nband = p.nband;
+        for (int d = 0; d < p.ndir; d++) {
+            S.px[d] = p.px[d];
+            S.py[d] = p.py[d];
+        }
+    }
+    if (nset > w.cap_sets || count > w.cap_jobs) {
+        FSIM_HIP(hipStreamSynchronize(s));
+        const int cs = std::max(nset, w.cap_sets), cj = std::max(count, w.cap_jobs);
+        hipEvent_t ev = w.uploaded;
+        w.uploaded = nullptr;
+        std::vector<ReconConst> hK;
+        std::vector<ShiftSet> hsh;
+        hK.swap(w.hK);
+        hsh.swap(w.hsh);
+        multiwork_release(w);
+        w.uploaded = ev;
+        w.hK.swap(hK);
+        w.hsh.swap(hsh);
+        if (hipMalloc(&w.dK, cs * sizeof(ReconConst)) != hipSuccess ||
+            hipMalloc(&w.wa, cs * mm * sizeof(float)) != hipSuccess ||
+            hipMalloc(&w.ramp, cs * rper * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&w.f, (size_t)cs * kMaxDir * kMaxComp * 2 * sizeof(double)) != hipSuccess ||
+            hipMalloc(&w.sh, cs * sizeof(ShiftSet)) != hipSuccess ||
+            hipMalloc(&w.jobs, cj * sizeof(Job)) != hipSuccess) {
+            (void)hipGetLastError();
+            multiwork_release(w);
+            return fail(FSIM_ERR_OOM, "reconstruct_params: device allocation failed");
+        }
+        w.cap_sets = cs;
+        w.cap_jobs = cj;
+    }
+    for (int k = 0; k < nset; k++) {
+        ReconConst &K = w.hK[k];
+        K.fade_lo = K0.fade_lo;
+        K.fade_hi = K0.fade_hi;
+        K.fade_px = K0.fade_px;
+        K.otf2d = K0.otf2d;
+        K.wa = w.wa + k * mm;
+        K.ramp = w.ramp + k * rper;
+    }
+    w.hjobs.resize(count);
+    bool aligned = true;
+    for (int st = 0; st < count; st++) {
+        Job &j = w.hjobs[st];
+        j.c = w.dK + set_of[st];
+        j.raw = raw + (size_t)st * K0.D * K0.P * n * n;
+        j.A = A + st * workspace_a_elems(n, K0.D, K0.Cs);
+        j.B = B + st * workspace_b_elems(n);
+        j.out = out + (size_t)st * 4 * n * n;
+        aligned = aligned && ((uintptr_t)j.raw & 7) == 0;
+    }
+    FSIM_HIP(hipMemcpyAsync(w.dK, w.hK.data(), nset * sizeof(ReconConst), hipMemcpyHostToDevice, s));
+    FSIM_HIP(hipMemcpyAsync(w.sh, w.hsh.data(), nset * sizeof(ShiftSet), hipMemcpyHostToDevice, s));
+    FSIM_HIP(hipMemcpyAsync(w.jobs, w.hjobs.data(), count * sizeof(Job), hipMemcpyHostToDevice, s));
+    int rc = fill_ramps(w.ramp, w.f, w.hf, w.hK.data(), nset, n, s);
+    if (rc) return rc;
+    if (!w.uploaded) FSIM_HIP(hipEventCreateWithFlags(&w.uploaded, hipEventDisableTiming));
+    FSIM_HIP(hipEventRecord(w.uploaded, s));
+    const Otf &o = ch->o;
+    const bool half = K0.half && ((uintptr_t)out & 7) == 0;
+    const int cw = gather_cw(n);
+    // the half gather reads the tiles of columns [0, n + cw)
+    const size_t len = half ? (size_t)(n + cw) * M : mm;
+    hipLaunchKernelGGL(k_wiener_multi, dim3((unsigned)std::min<size_t>((len + 255) / 256, 2048), nset), dim3(256),
+                       0, s, w.wa, ch->apo, M, cw, len, w.sh, o.vals, o.only_att, o.ns, o.nband, o.cpm, o.cutoff,
+                       ch->p.vec_cpm, ch->p.use_att, (float)(ch->p.wiener * ch->p.wiener));
+    FSIM_HIP(hipGetLastError());
+    JobList jl{};
+    jl.jobs = w.jobs;
+    jl.raw_aligned = aligned;
+    return launch_pipeline(n, jl, count, K0.D, K0.P, K0.Cs, half, s);
+}
+
+}  // namespace fsim
 
 extern "C" {
 
@@ -1561,100 +1675,10 @@ int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw, float *out, in
     if (rc) return rc;
     if (!ch || !raw || !out || !sp || count < 1) return fail(FSIM_ERR_ARG, "reconstruct_params: bad arguments");
     if (count > 65535) return fail(FSIM_ERR_ARG, "reconstruct_params: at most 65535 stacks per call");
-    const ReconConst &K0 = ch->K;
-    const int n = K0.n, M = 2 * n;
-    const size_t mm = (size_t)M * M, rper = (size_t)kMaxDir * kMaxComp * 2 * n;
     hipStream_t s = strm ? (hipStream_t)strm : stream();
-    MultiWork &w = ch->mw;
-    // the previous call's uploads read the host arrays rewritten below
-    if (w.uploaded) FSIM_HIP(hipEventSynchronize(w.uploaded));
-    // host constants of every stack: the channel's settings with the stack's
-    // shifts, phases and modulations
-    w.hK.resize(count);
-    w.hsh.resize(count);
-    for (int st = 0; st < count; st++) {
-        fsim_recon_params p = ch->p;
-        for (int d = 0; d < p.ndir; d++) {
-            p.px[d] = sp[st].px[d];
-            p.py[d] = sp[st].py[d];
-            for (int ph = 0; ph < p.nphase; ph++) p.phases[d][ph] = sp[st].phases[d][ph];
-            for (int b = 0; b < p.nband; b++) p.modul[d][b] = sp[st].modul[d][b];
-        }
-        rc = build_channel_const(p, ch->o, ch->sym, w.hK[st]);
-        if (rc) return rc;
-        ShiftSet &S = w.hsh[st];
-        S = ShiftSet{};
-        S.ndir = p.ndir;
-        S.nband = p.nband;
-        for (int d = 0; d < p.ndir; d++) {
-            S.px[d] = p.px[d];
-            S.py[d] = p.py[d];
-        }
-    }
-    if (count > w.cap) {
-        FSIM_HIP(hipStreamSynchronize(s));
-        (void)hipFree(w.dK);
-        (void)hipFree(w.wa);
-        (void)hipFree(w.ramp);
-        (void)hipFree(w.f);
-        (void)hipFree(w.sh);
-        (void)hipFree(w.jobs);
-        hipEvent_t ev = w.uploaded;
-        w = MultiWork{};
-        w.uploaded = ev;
-        if (hipMalloc(&w.dK, count * sizeof(ReconConst)) != hipSuccess ||
-            hipMalloc(&w.wa, count * mm * sizeof(float)) != hipSuccess ||
-            hipMalloc(&w.ramp, count * rper * sizeof(float2)) != hipSuccess ||
-            hipMalloc(&w.f, (size_t)count * kMaxDir * kMaxComp * 2 * sizeof(double)) != hipSuccess ||
-            hipMalloc(&w.sh, count * sizeof(ShiftSet)) != hipSuccess ||
-            hipMalloc(&w.jobs, count * sizeof(Job)) != hipSuccess) {
-            (void)hipGetLastError();
-            return fail(FSIM_ERR_OOM, "reconstruct_params: device allocation failed");
-        }
-        w.cap = count;
-        // the host constants were built before the reset
-        return fsim_reconstruct_params(ch, raw, out, count, sp, strm);
-    }
     rc = ensure_workspace(ch, count, s);
     if (rc) return rc;
-    w.hjobs.resize(count);
-    bool aligned = true;
-    for (int st = 0; st < count; st++) {
-        ReconConst &K = w.hK[st];
-        K.fade_lo = K0.fade_lo;
-        K.fade_hi = K0.fade_hi;
-        K.fade_px = K0.fade_px;
-        K.otf2d = K0.otf2d;
-        K.wa = w.wa + st * mm;
-        K.ramp = w.ramp + st * rper;
-        Job &j = w.hjobs[st];
-        j.c = w.dK + st;
-        j.raw = raw + (size_t)st * K0.D * K0.P * n * n;
-        j.A = ch->A + st * workspace_a_elems(n, K0.D, K0.Cs);
-        j.B = ch->B + st * workspace_b_elems(n);
-        j.out = out + (size_t)st * 4 * n * n;
-        aligned = aligned && ((uintptr_t)j.raw & 7) == 0;
-    }
-    FSIM_HIP(hipMemcpyAsync(w.dK, w.hK.data(), count * sizeof(ReconConst), hipMemcpyHostToDevice, s));
-    FSIM_HIP(hipMemcpyAsync(w.sh, w.hsh.data(), count * sizeof(ShiftSet), hipMemcpyHostToDevice, s));
-    FSIM_HIP(hipMemcpyAsync(w.jobs, w.hjobs.data(), count * sizeof(Job), hipMemcpyHostToDevice, s));
-    rc = fill_ramps(w.ramp, w.f, w.hf, w.hK.data(), count, n, s);
-    if (rc) return rc;
-    if (!w.uploaded) FSIM_HIP(hipEventCreateWithFlags(&w.uploaded, hipEventDisableTiming));
-    FSIM_HIP(hipEventRecord(w.uploaded, s));
-    const Otf &o = ch->o;
-    const bool half = K0.half && ((uintptr_t)out & 7) == 0;
-    const int cw = gather_cw(n);
-    // the half gather reads the tiles of columns [0, n + cw)
-    const size_t len = half ? (size_t)(n + cw) * M : mm;
-    hipLaunchKernelGGL(k_wiener_multi, dim3((unsigned)std::min<size_t>((len + 255) / 256, 2048), count), dim3(256),
-                       0, s, w.wa, ch->apo, M, cw, len, w.sh, o.vals, o.only_att, o.ns, o.nband, o.cpm, o.cutoff,
-                       ch->p.vec_cpm, ch->p.use_att, (float)(ch->p.wiener * ch->p.wiener));
-    FSIM_HIP(hipGetLastError());
-    JobList jl{};
-    jl.jobs = w.jobs;
-    jl.raw_aligned = aligned;
-    return launch_pipeline(n, jl, count, K0.D, K0.P, K0.Cs, half, s);
+    return reconstruct_params_with(ch, ch->mw, ch->A, ch->B, raw, out, count, sp, 1, s);
 }
 
 int fsim_stage_timing(int enable) {

@@ -35,6 +35,11 @@ struct Request {
     const uint16_t *raw = nullptr;
     float *out = nullptr;
     bool device = false;
+    // a request with its own SIM parameters: `count` stacks (timepoints),
+    // all reconstructed with sp
+    bool params = false;
+    int count = 1;
+    fsim_sim_params sp{};
     double t_submit = 0, t_done = 0;
     int state = 0;  // 0 pending, 1 running, 2 done, 3 failed
     int rc = 0;
@@ -43,8 +48,9 @@ struct Request {
 struct ClassKey {
     int n, D, P, B, Cs;   // Cs: stored components per direction
     int half;             // Hermitian result (ReconConst::half)
+    int pcount;           // requests with their own parameters: stacks per request (one request per slot)
     bool operator==(const ClassKey &o) const {
-        return n == o.n && D == o.D && P == o.P && B == o.B && Cs == o.Cs && half == o.half;
+        return n == o.n && D == o.D && P == o.P && B == o.B && Cs == o.Cs && half == o.half && pcount == o.pcount;
     }
 };
 
@@ -56,6 +62,7 @@ struct Slot {
     float *out = nullptr;
     float2 *A = nullptr, *B = nullptr;
     Job *jobs_d = nullptr, *jobs_h = nullptr;
+    MultiWork *mw = nullptr;   // slots of requests with their own parameters
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     std::vector<Request *> reqs;
@@ -67,6 +74,12 @@ static size_t per_stack_bytes(const ClassKey &k) {
     const size_t nn = (size_t)k.n * k.n;
     const int C = k.Cs;
     return 2 * (size_t)k.D * k.P * nn + 8 * (size_t)k.D * C * nn + 32 * nn + 16 * nn;
+}
+
+// Device bytes of a slot of class k holding cap requests.
+static size_t slot_bytes(const ClassKey &k, int cap) {
+    if (k.pcount > 0) return per_stack_bytes(k) * k.pcount + multiwork_bytes(k.n, k.pcount, 1);
+    return (per_stack_bytes(k) + sizeof(Job)) * cap;
 }
 
 struct Server {
@@ -99,12 +112,17 @@ struct Server {
         (void)hipFree(s->B);
         (void)hipFree(s->jobs_d);
         if (s->jobs_h) (void)hipHostFree(s->jobs_h);
+        if (s->mw) {
+            multiwork_release(*s->mw);
+            delete s->mw;
+        }
         if (s->done) (void)hipEventDestroy(s->done);
         if (s->stream) (void)hipStreamDestroy(s->stream);
         delete s;
     }
 
     int cap_for(const ClassKey &k) const {
+        if (k.pcount > 0) return 1;
         const size_t per = per_stack_bytes(k);
         const size_t share = (size_t)cfg.mem_budget / (size_t)std::max(cfg.nslots, 1);
         int cap = (int)std::min<size_t>((size_t)cfg.max_batch, std::max<size_t>(share / per, 1));
@@ -117,11 +135,13 @@ struct Server {
         s->cap = cap_for(k);
         const size_t nn = (size_t)k.n * k.n;
         const int C = k.Cs;
-        s->bytes = per_stack_bytes(k) * s->cap + sizeof(Job) * s->cap;
-        bool ok = hipMalloc(&s->raw, 2 * (size_t)k.D * k.P * nn * s->cap) == hipSuccess &&
-                  hipMalloc(&s->out, 16 * nn * s->cap) == hipSuccess &&
-                  hipMalloc(&s->A, 8 * (size_t)k.D * C * nn * s->cap) == hipSuccess &&
-                  hipMalloc(&s->B, 32 * nn * s->cap) == hipSuccess &&
+        const size_t st = (size_t)s->cap * (k.pcount > 0 ? k.pcount : 1);   // stacks
+        s->bytes = slot_bytes(k, s->cap);
+        if (k.pcount > 0) s->mw = new MultiWork();
+        bool ok = hipMalloc(&s->raw, 2 * (size_t)k.D * k.P * nn * st) == hipSuccess &&
+                  hipMalloc(&s->out, 16 * nn * st) == hipSuccess &&
+                  hipMalloc(&s->A, 8 * (size_t)k.D * C * nn * st) == hipSuccess &&
+                  hipMalloc(&s->B, 32 * nn * st) == hipSuccess &&
                   hipMalloc(&s->jobs_d, sizeof(Job) * s->cap) == hipSuccess &&
                   hipHostMalloc(&s->jobs_h, sizeof(Job) * s->cap, hipHostMallocDefault) == hipSuccess &&
                   hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess &&
@@ -139,7 +159,7 @@ struct Server {
     Slot *acquire_slot(const ClassKey &k) {
         for (Slot *s : slots)
             if (!s->busy && s->key == k) return s;
-        const size_t need = per_stack_bytes(k) * cap_for(k) + sizeof(Job) * cap_for(k);
+        const size_t need = slot_bytes(k, cap_for(k));
         if ((int)slots.size() >= cfg.nslots || bytes + need > (size_t)cfg.mem_budget) {
             for (size_t i = 0; i < slots.size();) {
                 Slot *s = slots[i];
@@ -163,10 +183,40 @@ struct Server {
 
     ClassKey key_of(const Request *r) const {
         const ReconConst &K = channel_host_const(channels[r->chan]);
-        return ClassKey{K.n, K.D, K.P, K.B, K.Cs, K.half};
+        return ClassKey{K.n, K.D, K.P, K.B, K.Cs, K.half, r->params ? r->count : 0};
+    }
+
+    // One request with its own parameters: its stacks through the
+    // per-parameter pipeline on the slot's buffers.
+    void launch_params(Slot *s) {
+        const ClassKey k = s->key;
+        const size_t nn = (size_t)k.n * k.n;
+        const size_t raw_elems = (size_t)k.D * k.P * nn * k.pcount, out_elems = 4 * nn * k.pcount;
+        Request *r = s->reqs[0];
+        int rc = FSIM_OK;
+        const uint16_t *raw = r->raw;
+        float *out = r->out;
+        if (!r->device) {
+            raw = s->raw;
+            out = s->out;
+            if (hipMemcpyAsync((void *)raw, r->raw, raw_elems * 2, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+                rc = FSIM_ERR_HIP;
+        }
+        if (!rc)
+            rc = reconstruct_params_with(channels[r->chan], *s->mw, s->A, s->B, raw, out, k.pcount, &r->sp, 0,
+                                         s->stream);
+        if (!rc && !r->device &&
+            hipMemcpyAsync(r->out, out, out_elems * 4, hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+            rc = FSIM_ERR_HIP;
+        if (hipEventRecord(s->done, s->stream) != hipSuccess) rc = FSIM_ERR_HIP;
+        if (rc) r->rc = rc;
     }
 
     void launch(Slot *s) {
+        if (s->key.pcount > 0) {
+            launch_params(s);
+            return;
+        }
         const ClassKey k = s->key;
         const size_t nn = (size_t)k.n * k.n;
         const int C = k.Cs;
@@ -355,6 +405,44 @@ int fsim_server_submit(fsim_server h, int chan, const uint16_t *raw, float *out,
     r->raw = raw;
     r->out = out;
     r->device = (flags & FSIM_SUBMIT_DEVICE) != 0;
+    r->t_submit = now_ms();
+    s.reqs[r->ticket] = r;
+    s.pending.push_back(r);
+    s.submitted++;
+    *ticket = r->ticket;
+    s.cv_work.notify_one();
+    return FSIM_OK;
+}
+
+int fsim_server_submit_params(fsim_server h, int chan, const uint16_t *raw, float *out, int count,
+                              const fsim_sim_params *sp, int flags, uint64_t *ticket) {
+    if (!h || !raw || !out || !sp || !ticket) return fail(FSIM_ERR_ARG, "server_submit_params: null argument");
+    if (count < 1 || count > 4096) return fail(FSIM_ERR_ARG, "server_submit_params: count must be in [1, 4096]");
+    Server &s = h->s;
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (chan < 0 || chan >= (int)s.channels.size())
+        return fail(FSIM_ERR_ARG, "server_submit_params: unknown channel");
+    if ((int)s.pending.size() >= s.cfg.max_queue) {
+        s.rejected++;
+        return fail(FSIM_ERR_BUSY, "server_submit_params: queue full, request refused");
+    }
+    Request probe;
+    probe.chan = chan;
+    probe.params = true;
+    probe.count = count;
+    if (slot_bytes(s.key_of(&probe), 1) > (size_t)s.cfg.mem_budget) {
+        s.rejected++;
+        return fail(FSIM_ERR_BUSY, "server_submit_params: request larger than the memory budget");
+    }
+    Request *r = new Request();
+    r->ticket = s.next_ticket++;
+    r->chan = chan;
+    r->raw = raw;
+    r->out = out;
+    r->device = (flags & FSIM_SUBMIT_DEVICE) != 0;
+    r->params = true;
+    r->count = count;
+    r->sp = *sp;
     r->t_submit = now_ms();
     s.reqs[r->ticket] = r;
     s.pending.push_back(r);

@@ -183,6 +183,7 @@ _SIGS = {
     "fsim_server_submit": [_P, _I, _P, _P, _I, C.POINTER(C.c_uint64)],
     "fsim_server_wait": [_P, C.c_uint64, _I, _PD],
     "fsim_server_get_stats": [_P, C.POINTER(SERVER_STATS), _I],
+    "fsim_server_submit_params": [_P, _I, _P, _P, _I, C.POINTER(SIM_PARAMS), _I, C.POINTER(C.c_uint64)],
     "fsim_batch_create": [_P, _I, _I, _P],
     "fsim_batch_run": [_P, _P, _P, _P],
     "fsim_batch_info": [_P, _P, _P, _P],

@@ -62,6 +62,21 @@ class ReconstructionServer:
             self._keep[t.value] = (raw, out)
         return t.value
 
+    def submit_params(self, channel, raw, out, count, params, device=False):
+        """fsim_server_submit_params: ``count`` consecutive stacks in ``raw``
+        reconstructed into ``count`` consecutive images in ``out`` with the
+        SIM parameters ``params`` (an fsim_sim_params or a SimParam)."""
+        from .sim_algorithm import sim_params
+        sp = params if isinstance(params, _lib.SIM_PARAMS) else sim_params(params)
+        t = C.c_uint64()
+        rp = raw if isinstance(raw, int) else raw.ctypes.data
+        op = out if isinstance(out, int) else out.ctypes.data
+        call("fsim_server_submit_params", self._h, int(channel), C.c_void_p(rp), C.c_void_p(op), int(count),
+             C.byref(sp), _lib.FSIM_SUBMIT_DEVICE if device else 0, C.byref(t))
+        if not isinstance(raw, int):
+            self._keep[t.value] = (raw, out)
+        return t.value
+
     def queueImage(self, channel, raw, out):
         """ReconstructionRunner.queueImage: False when the request is refused."""
         try:

@@ -392,6 +392,18 @@ int fsim_server_destroy(fsim_server s);
 int fsim_server_add_channel(fsim_server s, fsim_channel ch, int *id);
 int fsim_server_submit(fsim_server s, int channel, const uint16_t *raw, float *out, int flags,
                        uint64_t *ticket);
+/* submit_params: one request of `count` stacks (e.g. the timepoints of one
+ * acquisition, raw = count consecutive stacks, out = count consecutive
+ * images) reconstructed with its own SIM parameters *sp (copied) instead of
+ * the channel's -- the live refit (ReconstructionRunner.java:642-746) or a
+ * fit on the first timepoint applied to the following ones.  The request
+ * gets a slot of its own, sized by its footprint (work buffers for count
+ * stacks plus one set of parameter tables) and admitted against mem_budget
+ * like the batches; a request larger than the budget is refused
+ * (FSIM_ERR_BUSY).  Results are bitwise those of fsim_reconstruct_params
+ * with sp repeated count times. */
+int fsim_server_submit_params(fsim_server s, int channel, const uint16_t *raw, float *out, int count,
+                              const fsim_sim_params *sp, int flags, uint64_t *ticket);
 /* timeout_ms < 0 waits forever; FSIM_ERR_BUSY on timeout */
 int fsim_server_wait(fsim_server s, uint64_t ticket, int timeout_ms, double *latency_ms);
 int fsim_server_get_stats(fsim_server s, fsim_server_stats *st, int reset);

@@ -188,3 +188,19 @@ def test_estimate_batch_matches_oracle(fsim_lib):
             assert g["px"] == w["px"] and g["py"] == w["py"]
             assert abs(g["pha_off"] - w["pha_off"]) < 1e-4
             assert abs(g["mod1"] - w["mod1"]) <= 1e-4 * w["mod1"]
+
+
+def test_estimate_batch_large_size_known_answer(fsim_lib):
+    """1024^2 stacks (row transforms of two wavefronts in the estimation's
+    FFTs): the fitted shift is within a tenth of a pixel of the pattern's."""
+    from fsim import synthetic
+    from fsim.sim_algorithm import SimParam, estimate_batch
+    n = 1024
+    raw, sp = synthetic.sim_stack(n=n, seed=21, nbeads=600, nfil=80)
+    images = raw.reshape(1, 3, 3, n, n).astype(np.float32)
+    param = SimParam.create(2, 3, 3, n, PX_UM, _otf(n))
+    got = estimate_batch(param, 1, 0.6, images=images)
+    for d in range(3):
+        g = got[0][d]
+        print(f"n={n} dir {d}: {g} true ({sp['px'][d]:.3f}, {sp['py'][d]:.3f})")
+        assert abs(g["px"] - sp["px"][d]) < 0.1 and abs(g["py"] - sp["py"][d]) < 0.1

@@ -108,3 +108,112 @@ def test_device_resident_submit(setup):
     got = ob.download((128, 128), np.float32)
     assert np.array_equal(got, rec.reconstruct_host(raw))
     srv.close()
+
+
+def _shifted_params(rec_param, scale):
+    """The channel's parameters with the shifts scaled and the phases offset:
+    a stand-in for a per-acquisition fit."""
+    from fsim.sim_algorithm import sim_params
+    p = sim_params(rec_param)
+    for d in range(3):
+        p.px[d] *= scale
+        p.py[d] *= scale
+        for i in range(3):
+            p.phases[d][i] += 0.1 * (d + 1)
+    return p
+
+
+def test_params_request_equals_reconstruct_params(setup):
+    """A request with its own parameters (3 timepoints) gives bitwise the
+    images of fsim_reconstruct_params with those parameters repeated, from
+    host and from device buffers, alone and among ordinary requests."""
+    from fsim import synthetic
+    from fsim.linalg import DeviceBuffer
+    from fsim.livemode import ReconstructionServer
+    n, lam = 128, 600.0
+    raw0, rec = setup[(n, lam)]
+    stacks = np.stack([raw0] + [synthetic.sim_stack(n=n, seed=300 + i, lam=lam, nbeads=60, nfil=8)[0]
+                                for i in range(2)])
+    sp = synthetic.sim_stack(n=n, seed=41, lam=lam, nbeads=60, nfil=8)[1]
+    prm = _shifted_params(synthetic.make_param(sp, n, lam=lam), 1.01)
+    rb = DeviceBuffer(stacks.nbytes)
+    rb.upload(stacks)
+    ob = DeviceBuffer(3 * rec.out_bytes())
+    rec.reconstruct_params(rb.value, ob.value, [prm] * 3)
+    from fsim._lib import call
+    call("fsim_sync")
+    exp = ob.download((3, 2 * n, 2 * n), np.float32)
+    # each timepoint alone with the same parameters
+    for t in range(3):
+        o1 = DeviceBuffer(rec.out_bytes())
+        rec.reconstruct_params(rb.value + t * stacks[0].nbytes, o1.value, [prm])
+        call("fsim_sync")
+        assert np.array_equal(o1.download((2 * n, 2 * n), np.float32), exp[t])
+    srv = ReconstructionServer(mem_budget=2 << 30, max_batch=8, max_queue=256, nslots=3)
+    ids = {k: srv.add_channel(v[1]) for k, v in setup.items()}
+    out_h = np.zeros((3, 2 * n, 2 * n), np.float32)
+    ob2 = DeviceBuffer(3 * rec.out_bytes())
+    tickets = []
+    for i, k in enumerate(list(setup) * 3):
+        if i == 4:
+            tickets.append(("h", srv.submit_params(ids[(n, lam)], stacks, out_h, 3, prm)))
+        if i == 7:
+            tickets.append(("d", srv.submit_params(ids[(n, lam)], rb.value, ob2.value, 3, prm, device=True)))
+        o = np.zeros((2 * k[0], 2 * k[0]), np.float32)
+        tickets.append((k, srv.submit(ids[k], setup[k][0], o), o))
+    for tk in tickets:
+        srv.wait(tk[1])
+    for tk in tickets:
+        if tk[0] not in ("h", "d"):
+            assert np.array_equal(tk[2], setup[tk[0]][1].reconstruct_host(setup[tk[0]][0]))
+    assert np.array_equal(out_h, exp)
+    assert np.array_equal(ob2.download((3, 2 * n, 2 * n), np.float32), exp)
+    st = srv.stats()
+    assert st["failed"] == 0
+    srv.close()
+
+
+def test_params_request_larger_than_budget_is_refused(setup):
+    from fsim._lib import FsimBusy
+    from fsim.livemode import ReconstructionServer
+    from fsim.sim_algorithm import sim_params
+    from fsim import synthetic
+    n, lam = 256, 525.0
+    raw, rec = setup[(n, lam)]
+    # 8 stacks of 256^2 need about 60 MB of work buffers
+    srv = ReconstructionServer(mem_budget=32 << 20, max_batch=4, max_queue=16, nslots=1)
+    cid = srv.add_channel(rec)
+    sp = synthetic.sim_stack(n=n, seed=43, lam=lam, nbeads=30, nfil=5)[1]
+    prm = sim_params(synthetic.make_param(sp, n, lam=lam))
+    big = np.zeros((8,) + raw.shape, np.uint16)
+    out = np.zeros((8, 2 * n, 2 * n), np.float32)
+    with pytest.raises(FsimBusy):
+        srv.submit_params(cid, big, out, 8, prm)
+    srv.close()
+
+
+def test_reconstruct_params_shares_tables_of_equal_sets(setup):
+    """Runs of equal parameter sets share one set of tables; every stack's
+    image is bitwise the one computed with its set alone."""
+    from fsim import synthetic
+    from fsim._lib import call
+    from fsim.linalg import DeviceBuffer
+    n, lam = 64, 525.0
+    raw, rec = setup[(n, lam)]
+    sp = synthetic.sim_stack(n=n, seed=40, lam=lam, nbeads=30, nfil=5)[1]
+    base = synthetic.make_param(sp, n, lam=lam)
+    p1, p2 = _shifted_params(base, 1.0), _shifted_params(base, 1.02)
+    order = [p1, p1, p2, p2, p1]
+    stacks = np.stack([raw] * 5)
+    rb = DeviceBuffer(stacks.nbytes)
+    rb.upload(stacks)
+    ob = DeviceBuffer(5 * rec.out_bytes())
+    rec.reconstruct_params(rb.value, ob.value, order)
+    call("fsim_sync")
+    got = ob.download((5, 2 * n, 2 * n), np.float32)
+    for t, p in enumerate(order):
+        o1 = DeviceBuffer(rec.out_bytes())
+        rec.reconstruct_params(rb.value, o1.value, [p])
+        call("fsim_sync")
+        assert np.array_equal(o1.download((2 * n, 2 * n), np.float32), got[t])
+    assert not np.array_equal(got[0], got[2])
