@@ -80,27 +80,74 @@ __global__ void k_eb_init(EstGroup *__restrict__ g, int G) {
     if (i < G) g[i].key = 0ull;
 }
 
-// Input preparation of sim_gui/ImageControl.java:819-840: fadeBorderCos on
-// the float image (top, bottom, left, right; each `float *= double` rounds
-// once, SimUtils.java:162-185), copied to the real part of a complex image.
-// Four pixels per thread (n is a multiple of 4).
-__global__ void k_eb_load(const float4 *__restrict__ img, float4 *__restrict__ X, int n, size_t nquad,
-                          const double *__restrict__ lo, const double *__restrict__ hi) {
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < nquad; q += (size_t)gridDim.x * blockDim.x) {
-        const float4 v = img[q];
-        const size_t i = q * 4;
-        const int x0 = (int)(i % n), y = (int)((i / n) % n);
+// Input preparation of every frame (sim_gui/ImageControl.java:819-840) fused
+// with the forward row transform: fadeBorderCos on the float image (top,
+// bottom, left, right; each `float *= double` rounds once,
+// SimUtils.java:162-185) and the copy to the real part of a complex image, then
+// the row pass of the 2-D FFT (the same transform code as
+// fft2d_batch_device's row pass, hence the same values).  One row of one
+// frame per T = n/E threads.
+template <int L>
+__global__ void __launch_bounds__(256) k_eb_load_rows(const float *__restrict__ img, float2 *__restrict__ X,
+                                                      int nrows, const double *__restrict__ lo,
+                                                      const double *__restrict__ hi) {
+    constexpr int E = FftE<L>::value, T = L / E;
+    constexpr int RW = T >= 256 ? 1 : 256 / T;
+    constexpr int LP = FftLds<L, E>::len;
+    __shared__ float2 lds[RW * LP];
+    using TW = TwLds<L, false>;
+    const int t = threadIdx.x % T, rr = threadIdx.x / T;
+    const int row = blockIdx.x * RW + rr;
+    const bool active = row < nrows;
+    const int y = row % L;
+    float2 v[E];
+    if (active) {
+        const float *src = img + (size_t)row * L;
         const double ly = lo[y], hy = hi[y];
-        float d[4] = {v.x, v.y, v.z, v.w};
+        float d[E];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            d[j] = (float)((double)d[j] * ly);
-            d[j] = (float)((double)d[j] * hy);
-            d[j] = (float)((double)d[j] * lo[x0 + j]);
-            d[j] = (float)((double)d[j] * hi[x0 + j]);
+        for (int m = 0; m < E; m++) d[m] = src[t + T * m];
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int x = t + T * m;
+            float e = (float)((double)d[m] * ly);
+            e = (float)((double)e * hy);
+            e = (float)((double)e * lo[x]);
+            e = (float)((double)e * hi[x]);
+            v[m] = make_float2(e, 0.f);
         }
-        X[2 * q] = make_float4(d[0], 0.f, d[1], 0.f);
-        X[2 * q + 1] = make_float4(d[2], 0.f, d[3], 0.f);
+    }
+    fft_regs<L, E, false, RowWave<L>::value, TW::tl>(v, lds + rr * LP, 1, t, nullptr, active);
+    if (active) {
+        float2 *dst = X + (size_t)row * L;
+#pragma unroll
+        for (int m = 0; m < E; m++) dst[t + T * m] = v[m];
+    }
+}
+
+template <int L>
+static int launch_load_rows_t(const float *img, float2 *X, int nrows, const double *lo, const double *hi,
+                              hipStream_t s) {
+    constexpr int E = FftE<L>::value, T = L / E;
+    constexpr int RW = T >= 256 ? 1 : 256 / T;
+    hipLaunchKernelGGL(k_eb_load_rows<L>, dim3(cdiv(nrows, RW)), dim3(T * RW), 0, s, img, X, nrows, lo, hi);
+    FSIM_HIP(hipGetLastError());
+    return FSIM_OK;
+}
+
+static int launch_load_rows(int L, const float *img, float2 *X, int nrows, const double *lo, const double *hi,
+                            hipStream_t s) {
+    switch (L) {
+        case 16: return launch_load_rows_t<16>(img, X, nrows, lo, hi, s);
+        case 32: return launch_load_rows_t<32>(img, X, nrows, lo, hi, s);
+        case 64: return launch_load_rows_t<64>(img, X, nrows, lo, hi, s);
+        case 128: return launch_load_rows_t<128>(img, X, nrows, lo, hi, s);
+        case 256: return launch_load_rows_t<256>(img, X, nrows, lo, hi, s);
+        case 512: return launch_load_rows_t<512>(img, X, nrows, lo, hi, s);
+        case 1024: return launch_load_rows_t<1024>(img, X, nrows, lo, hi, s);
+        case 2048: return launch_load_rows_t<2048>(img, X, nrows, lo, hi, s);
+        case 4096: return launch_load_rows_t<4096>(img, X, nrows, lo, hi, s);
+        default: return fail(FSIM_ERR_ARG, "estimate: unsupported size");
     }
 }
 
@@ -754,12 +801,9 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     hipLaunchKernelGGL(k_eb_dc_tab, dim3((unsigned)std::min<size_t>((ndc * nn + 255) / 256, 4096)), dim3(256), 0, s,
                        dct, n, reg, ndc);
     unsigned blocks;
-    const size_t nquad = (size_t)G * P * nn / 4;
-    grid_stride_launch(nquad, blocks);
-    hipLaunchKernelGGL(k_eb_load, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const float4 *>(images),
-                       reinterpret_cast<float4 *>(g_eb.X), n, nquad, fade, fade + n);
-    FSIM_HIP(hipGetLastError());
-    rc = fft2d_batch_device(g_eb.X, n, n, G * P, 0, s);
+    rc = launch_load_rows(n, images, g_eb.X, G * P * n, fade, fade + n, s);
+    if (rc) return rc;
+    rc = fft_cols_batch_device(g_eb.X, n, n, G * P, 0, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_eb_separate, dim3(cdiv(nn, 256), G), dim3(256), 0, s, g_eb.X, g_eb.W, n, G, sep);
     FSIM_HIP(hipGetLastError());

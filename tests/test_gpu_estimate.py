@@ -131,10 +131,13 @@ def _stacks(count, nphase, k_fracs, harmonic2, n=256, seed0=11):
     return np.stack(imgs), sps
 
 
-# The batched estimation runs the same kernels as the per-call flow
-# (Correlation.* entry points over device vectors); only |b0|^2 is summed in a
-# different order, so positions agree exactly and phases/modulations to
-# double rounding.
+# The batched estimation runs the same correlation kernels as the per-call
+# flow (Correlation.* entry points over device vectors); its input pass fuses
+# the fade into the row transform (the same transform code, compiled into
+# another kernel, where the compiler may form other fused multiply-adds) and
+# |b0|^2 is summed in a different order, so positions agree exactly and
+# phases/modulations to FP32 rounding of the input transform (observed
+# <= 3.2e-9 rad, 3.2e-9 relative).
 @pytest.mark.parametrize("nphase,nband,fit_band,exclude,k_fracs,h2",
                          [(3, 2, 1, 0.6, (0.85, 0.8, 0.75), 0.0), (5, 3, 2, 0.6, (0.45, 0.42), 0.3),
                           (5, 3, 1, 0.3, (0.45,), 0.3), (5, 2, 2, 0.6, (0.85,), 0.0)])
@@ -151,10 +154,10 @@ def test_estimate_batch_matches_per_call_flow(fsim_lib, nphase, nband, fit_band,
             g, w = got[s][d], want[d]
             print(f"stack {s} dir {d}: batch {g} per-call {w}")
             assert g["px"] == w["px"] and g["py"] == w["py"]
-            assert abs(g["pha_off"] - w["pha_off"]) <= 1e-9
-            assert abs(g["mod1"] - w["mod1"]) <= 1e-9 * w["mod1"]
+            assert abs(g["pha_off"] - w["pha_off"]) <= 1e-7
+            assert abs(g["mod1"] - w["mod1"]) <= 1e-7 * w["mod1"]
             if nband == 3:
-                assert abs(g["mod2"] - w["mod2"]) <= 1e-9 * w["mod2"]
+                assert abs(g["mod2"] - w["mod2"]) <= 1e-7 * w["mod2"]
         apply_estimate(param, got[s])
         scale = 1 if nband == 2 else 2
         for d in range(3):
