@@ -432,16 +432,32 @@ __global__ void __launch_bounds__(256) k_eb_approx(const float2 *__restrict__ T,
             double2 R[10];
 #pragma unroll
             for (int i = 0; i < 10; i++) R[i] = make_double2(0.0, 0.0);
-            for (int x = l; x < tw; x += 16) {
-                const float2 a = b0[row + x], b = b1[row + x];
-                const double qr = (double)b.x * a.x + (double)b.y * a.y;
-                const double qi = (double)b.y * a.x - (double)b.x * a.y;
-                aq += sqrt(qr * qr + qi * qi);
+            // four pixels' loads in flight before their sums (the sums keep
+            // the pixel order: x, x + 16, ...)
+            for (int x0 = l; x0 < tw; x0 += 64) {
+                float2 av[4], bv[4];
 #pragma unroll
-                for (int i = 0; i < 10; i++) {
-                    const float2 e = tx[i * kApxTile + x];
-                    R[i].x += qr * e.x - qi * e.y;
-                    R[i].y += qr * e.y + qi * e.x;
+                for (int u = 0; u < 4; u++) {
+                    const int x = x0 + 16 * u;
+                    if (x < tw) {
+                        av[u] = b0[row + x];
+                        bv[u] = b1[row + x];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int x = x0 + 16 * u;
+                    if (x >= tw) break;
+                    const float2 a = av[u], b = bv[u];
+                    const double qr = (double)b.x * a.x + (double)b.y * a.y;
+                    const double qi = (double)b.y * a.x - (double)b.x * a.y;
+                    aq += sqrt(qr * qr + qi * qi);
+#pragma unroll
+                    for (int i = 0; i < 10; i++) {
+                        const float2 e = tx[i * kApxTile + x];
+                        R[i].x += qr * e.x - qi * e.y;
+                        R[i].y += qr * e.y + qi * e.x;
+                    }
                 }
             }
 #pragma unroll
