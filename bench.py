@@ -388,7 +388,7 @@ def batched_mixed(nper=32, reps=10):
                                   "roofline_frac": round(b_alg / el_s / 8e12, 4)}}
 
 
-def large_fov(batch=4, steps=5):
+def large_fov(batch=8, steps=5):
     """Config 4: 2048^2 x 3 x 3 uint16 stacks -> 4096^2, fused pipeline,
     `batch` stacks per launch resident in HBM; parity of one stack against the
     FP32 oracle."""

@@ -60,7 +60,7 @@ struct JobList {
 // sep = 1: the separation with all units of a row in one wavefront (where the
 // frames fit: n <= 512, D*P <= 9), 0: one unit per block; col: column-pass
 // candidate (elements per thread, columns per block; col_shape).
-constexpr int kColCands = 4;
+constexpr int kColCands = 6;
 struct PipeCfg {
     int sep = 1;
     int col = 0;

@@ -1111,7 +1111,7 @@ static void stage_mark(int stage, hipStream_t s, hipEvent_t &start) {
 
 // Column-pass candidates (elements per thread E0, columns per block CW0; E0 =
 // 0: the size's default element count): 0 (0, 8), 1 (16, 8), 2 (16, 16),
-// 3 (8, 16).
+// 3 (8, 16), 4 (16, 4), 5 (8, 8).
 template <int N, int E0, int CW0>
 static void launch_col_shift(const JobList &jl, int njobs, int D, int C, const float2 *twn, hipStream_t s) {
     using CT = ColShiftTile<N, E0, CW0>;
@@ -1126,6 +1126,8 @@ static void col_candidate_shape(int col, int &e, int &cw) {
         case 1: e = ColShiftTile<N, 16, 8>::e; cw = ColShiftTile<N, 16, 8>::cw; break;
         case 2: e = ColShiftTile<N, 16, 16>::e; cw = ColShiftTile<N, 16, 16>::cw; break;
         case 3: e = ColShiftTile<N, 8, 16>::e; cw = ColShiftTile<N, 8, 16>::cw; break;
+        case 4: e = ColShiftTile<N, 16, 4>::e; cw = ColShiftTile<N, 16, 4>::cw; break;
+        case 5: e = ColShiftTile<N, 8, 8>::e; cw = ColShiftTile<N, 8, 8>::cw; break;
         default: e = ColShiftTile<N, 0, FSIM_COL_CW>::e; cw = ColShiftTile<N, 0, FSIM_COL_CW>::cw; break;
     }
 }
@@ -1164,6 +1166,8 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int P, int C, 
             case 1: launch_col_shift<N, 16, 8>(jl, njobs, D, C, twn, s); break;
             case 2: launch_col_shift<N, 16, 16>(jl, njobs, D, C, twn, s); break;
             case 3: launch_col_shift<N, 8, 16>(jl, njobs, D, C, twn, s); break;
+            case 4: launch_col_shift<N, 16, 4>(jl, njobs, D, C, twn, s); break;
+            case 5: launch_col_shift<N, 8, 8>(jl, njobs, D, C, twn, s); break;
             default: launch_col_shift<N, 0, FSIM_COL_CW>(jl, njobs, D, C, twn, s); break;
         }
         stage_mark(1, s, ev);
