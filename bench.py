@@ -195,6 +195,19 @@ def serve_mixed_stream(nreq, rate):
 _STACKS = {}
 
 
+def parity_errors(got, sp, n, raw, lam=525.0, threads=None):
+    """rel-L2 of the GPU image against the FP32 Java-order oracle and the FP64
+    re-run of the same dataflow, and of the FP32 oracle against FP64 (the
+    error budget of the reference arithmetic itself)."""
+    from oracle import fp64, ref
+    p = ref.recon_dict(sp, n, lam=lam)
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
+    r32 = ref.recon32(p, raw, threads=threads)
+    r64 = ref.recon64(p, raw)
+    return {"gpu_vs_ref32": fp64.rel_l2(got, r32), "gpu_vs_fp64": fp64.rel_l2(got, r64),
+            "ref32_vs_fp64": fp64.rel_l2(r32, r64)}
+
+
 def _stack(n, seed, **kw):
     """Seeded synthetic stack, generated once per process (2048^2 takes
     seconds on the host)."""
@@ -327,11 +340,12 @@ def batched_mixed(nper=32, reps=10):
     from fsim.livemode import ReconstructionBatch, ReconstructionServer
     from fsim.sim_algorithm import Reconstructor
     sizes, lams = [256, 512, 1024], [525.0, 600.0, 670.0]
-    recs, bufs, keep = {}, {}, []
+    recs, bufs, keep, host = {}, {}, [], {}
     for n in sizes:
         for ci, lam in enumerate(lams):
             raw, sp = synthetic.sim_stack(n=n, seed=200 + ci, lam=lam, nbeads=300 * n // 512,
                                           nfil=40 * n // 512)
+            host[(n, ci)] = (raw, sp, lam)
             rec = Reconstructor(synthetic.make_param(sp, n, lam=lam))
             keep.append(rec)
             recs[(n, ci)] = rec
@@ -361,6 +375,12 @@ def batched_mixed(nper=32, reps=10):
     el = float(np.median(times))
     info = batch.info()
     batch.close()
+    # parity of the first slot of each size (channel 0) from the batch output
+    parity = {}
+    for n in sizes:
+        raw, sp, lam = host[(n, 0)]
+        got = outs[n][0].download((2 * n, 2 * n), np.float32)
+        parity[n] = {k: float(f"{v:.3e}") for k, v in parity_errors(got, sp, n, raw, lam=lam).items()}
 
     srv = ReconstructionServer(mem_budget=32 << 30, max_batch=32, max_queue=4096, nslots=4)
     ids = {k: srv.add_channel(r) for k, r in recs.items()}
@@ -383,6 +403,7 @@ def batched_mixed(nper=32, reps=10):
             "ms_per_batch": round(el * 1e3, 3), "recons_per_s": round(len(jobs) / el, 1),
             "alg_bytes_per_batch": b_alg, "roofline_frac": round(b_alg / el / 8e12, 4),
             "autotune": {n: (t["sep"], t["col_elems"], t["col_width"]) for n, t in tuned.items()},
+            "rel_l2": parity,
             "served_one_by_one": {"ms_per_batch": round(el_s * 1e3, 3),
                                   "recons_per_s": round(len(jobs) / el_s, 1),
                                   "roofline_frac": round(b_alg / el_s / 8e12, 4)}}
@@ -417,10 +438,11 @@ def large_fov(batch=8, steps=5):
     value = batch * steps / el
     got = rec.reconstruct_host(raw)
     threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1))
-    err = fp64.rel_l2(got, ref.recon32(ref.recon_dict(sp, n), raw, threads=threads))
+    par = parity_errors(got, sp, n, raw, threads=threads)
     return {"workload": f"2048^2 x 3 x 3 uint16 -> 4096^2 float32, batch {batch}",
             "recons_per_s": round(value, 2), "ms_per_recon": round(1e3 / value, 3),
-            "roofline_frac": round(value * 434 * n * n / 8e12, 4), "rel_l2_vs_oracle": err}
+            "roofline_frac": round(value * 434 * n * n / 8e12, 4), "rel_l2_vs_oracle": par["gpu_vs_ref32"],
+            "rel_l2": par}
 
 
 def estimation(runs=3, batch=64):
@@ -509,6 +531,20 @@ def estimation_batched(nphase, nband, fb, k_frac, h2, batch=64, runs=3):
         return res
 
     res0 = run(True)
+    # parity of stack 0's reconstruction with its own estimate
+    from oracle import fp64, ref
+    prm = params_from_estimates(param, res0, 1)[0]
+    got0 = ob.download((2 * n, 2 * n), np.float32)
+    p = ref.recon_dict(sps[0], n, nband=nband)
+    p["px"] = [prm.px[d] for d in range(3)]
+    p["py"] = [prm.py[d] for d in range(3)]
+    p["phases"] = [[prm.phases[d][i] for i in range(nphase)] for d in range(3)]
+    p["modul"] = [[prm.modul[d][b] for b in range(nband)] for d in range(3)]
+    r32 = ref.recon32(p, raws[0], threads=max(1, min(16, os.cpu_count() or 1)))
+    r64 = ref.recon64(p, raws[0])
+    parity = {"gpu_vs_ref32": float(f"{fp64.rel_l2(got0, r32):.3e}"),
+              "gpu_vs_fp64": float(f"{fp64.rel_l2(got0, r64):.3e}"),
+              "ref32_vs_fp64": float(f"{fp64.rel_l2(r32, r64):.3e}")}
     est = [[{"px": res0[s * 3 + d].px, "py": res0[s * 3 + d].py} for d in range(3)] for s in range(4)]
     scale = 1 if nband == 2 else 2
     err_k = max(abs(est[s][d]["px"] - scale * sps[s % 4]["px"][d]) + abs(est[s][d]["py"] - scale * sps[s % 4]["py"][d])
@@ -524,6 +560,7 @@ def estimation_batched(nphase, nband, fb, k_frac, h2, batch=64, runs=3):
     # estimate + reconstruction algorithmic bytes with float input: 1268 n^2
     res["roofline_frac_estimate_plus_recon"] = round(res["estimate_plus_recon_per_s"] * 1268 * n * n / 8e12, 4)
     res["max_k_error_px"] = round(err_k, 4)
+    res["rel_l2_stack0"] = parity
     for b in (rb, ib, ob):
         b.free()
     return res
@@ -845,7 +882,9 @@ def main():
     r32 = ref.recon32(p32, stacks[0], threads=threads)
     first_cpu = time.perf_counter() - t0
     parity = fp64.rel_l2(got, r32)
-    log(f"rel-L2 gpu vs oracle {parity:.3e}")
+    r64 = ref.recon64(p32, stacks[0])
+    parity64 = {"gpu_vs_ref32": parity, "gpu_vs_fp64": fp64.rel_l2(got, r64), "ref32_vs_fp64": fp64.rel_l2(r32, r64)}
+    log(f"rel-L2 gpu vs oracle {parity:.3e}, vs FP64 {parity64['gpu_vs_fp64']:.3e}")
 
     # batched fused pipeline
     log(f"warmup {args.warmup} steps of batch {B}")
@@ -989,6 +1028,7 @@ def main():
         "speedup_vs_plain": round(value / plain_value, 2),
         "served_speedup_vs_plain": round(served["recons_per_s"] / plain_value, 2),
         "rel_l2_vs_oracle": parity,
+        "rel_l2": parity64,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
