@@ -59,8 +59,9 @@ struct JobList {
 // Launch choices of the pipeline that the autotuner picks per size class:
 // sep = 1: the separation with all units of a row in one wavefront (where the
 // frames fit: n <= 512, D*P <= 9), 0: one unit per block; col: column-pass
-// candidate (elements per thread, columns per block; col_shape).
-constexpr int kColCands = 6;
+// candidate (elements per thread, columns per block; col_shape reports the
+// columns negated for the one-column-per-wavefront layout).
+constexpr int kColCands = 8;
 struct PipeCfg {
     int sep = 1;
     int col = 0;

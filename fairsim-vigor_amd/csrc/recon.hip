@@ -495,10 +495,19 @@ constexpr int lds_waves_per_simd(int threads, int lds_bytes, int cap) {
 // E0: elements per thread (the Stockham radix; 0: ColE), CW0: columns per
 // block (capped by the 1024-thread block).  The pipeline picks one of the
 // candidates kColE / kColCW per size (tuned or default).
-template <int N, int E0, int CW0>
+// WC: one column's transform per wavefront (threads t-fastest, E chosen so
+// that T = n/E <= 64): the transforms synchronise at wavefront level and
+// the waves of a block run independently; the loads of a wave are 64-byte
+// strided (one column of a tile run), the block's waves read the same lines.
+template <int N>
+struct WaveColE {
+    static constexpr int value = N / 64 > 8 ? N / 64 : 8;
+};
+template <int N, int E0, int CW0, bool WC = false>
 struct ColShiftTile {
-    static constexpr int e = (E0 > 0 && E0 <= 16 && E0 < N / 2) ? E0 : ColE<N>::value;
+    static constexpr int e = WC ? WaveColE<N>::value : ((E0 > 0 && E0 <= 16 && E0 < N / 2) ? E0 : ColE<N>::value);
     static constexpr int t = N / e;
+    static_assert(!WC || t <= 64, "a column's transform inside one wavefront");
     static constexpr int cw0 = ColTile<N, t, e>::value;
     static constexpr int cw = cw0 < CW0 ? cw0 : CW0;
     static constexpr int value = t * cw;
@@ -507,13 +516,14 @@ struct ColShiftTile {
     static constexpr int wpe = lds_waves_per_simd(value, (lds + N) * 8, FSIM_COL_WPE);
 };
 
-template <int N, int E0, int CW0>
-__global__ void __launch_bounds__((ColShiftTile<N, E0, CW0>::value), (ColShiftTile<N, E0, CW0>::wpe))
+template <int N, int E0, int CW0, bool WC>
+__global__ void __launch_bounds__((ColShiftTile<N, E0, CW0, WC>::value), (ColShiftTile<N, E0, CW0, WC>::wpe))
     k_col_shift(JobList jl, int nsh_all, int total, const float2 *__restrict__ tw) {
-    using CT = ColShiftTile<N, E0, CW0>;
+    using CT = ColShiftTile<N, E0, CW0, WC>;
     constexpr int E = CT::e, T = N / E;
     constexpr int CW = CT::cw;
     constexpr int tiles = N / CW;
+    constexpr int LP = FftLds<N, E>::len;
     __shared__ float2 lds[CT::lds];
     // nsh_all = D * (Cs - 1) shifted components per stack; total = stacks *
     // nsh_all * tiles
@@ -539,13 +549,18 @@ __global__ void __launch_bounds__((ColShiftTile<N, E0, CW0>::value), (ColShiftTi
     KRef K = cst(J.c);
     const int C = K.Cs, nsh = C - 1, dc = u % nsh_all;
     const int d = dc / nsh, c = 1 + dc % nsh;   // stored shifted component
-    const int cl = threadIdx.x % CW, t = threadIdx.x / CW;
+    const int cl = WC ? (int)threadIdx.x / T : (int)threadIdx.x % CW;
+    const int t = WC ? (int)threadIdx.x % T : (int)threadIdx.x / CW;
+    // LDS region of the column: interleaved with the tile's other columns,
+    // or (WC) a contiguous one per column
+    float2 *reg = WC ? lds + cl * LP : lds + cl;
+    constexpr int LS = WC ? 1 : CW;
     const int x = tile * CW + cl;
     FSIM_GLB float2 *col = glb(J.A) + (size_t)(d * C + c) * N * N + at_col<N>(x);
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = ld(col + at_row<N>(t + T * m));
-    fft_regs<N, E, false, false, TW::tl>(v, lds + cl, CW, t, tws);
+    fft_regs<N, E, false, WC, TW::tl>(v, reg, LS, t, tws);
     const int band = K.sband[c];
     const FSIM_GLB float2 *otf = glb(K.otf2d) + (size_t)(K.otf_nb > 1 ? band : 0) * N * N + x;
 #pragma unroll
@@ -559,14 +574,14 @@ __global__ void __launch_bounds__((ColShiftTile<N, E0, CW0>::value), (ColShiftTi
 #pragma unroll
         for (int m = 0; m < E; m++) r[m] = ld(ry + t + T * m);
     }
-    fft_regs<N, E, true, false, TW::tl>(v, lds + cl, CW, t, tws);
+    fft_regs<N, E, true, WC, TW::tl>(v, reg, LS, t, tws);
     const float sc = 1.0f / N;
 #pragma unroll
     for (int m = 0; m < E; m++) {
         const float2 rm = E <= 8 ? r[m < ER ? m : 0] : ld(ry + t + T * m);
         v[m] = rot_java(cscale(v[m], sc), rm.x, rm.y);
     }
-    fft_regs<N, E, false, false, TW::tl>(v, lds + cl, CW, t, tws);
+    fft_regs<N, E, false, WC, TW::tl>(v, reg, LS, t, tws);
 #pragma unroll
     for (int m = 0; m < E; m++) st(col + at_row<N>(t + T * m), v[m]);
 }
@@ -1111,12 +1126,13 @@ static void stage_mark(int stage, hipStream_t s, hipEvent_t &start) {
 
 // Column-pass candidates (elements per thread E0, columns per block CW0; E0 =
 // 0: the size's default element count): 0 (0, 8), 1 (16, 8), 2 (16, 16),
-// 3 (8, 16), 4 (16, 4), 5 (8, 8).
-template <int N, int E0, int CW0>
+// 3 (8, 16), 4 (16, 4), 5 (8, 8); 6 and 7: one column per wavefront
+// (WaveColE elements per thread), 8 and 4 columns per block.
+template <int N, int E0, int CW0, bool WC = false>
 static void launch_col_shift(const JobList &jl, int njobs, int D, int C, const float2 *twn, hipStream_t s) {
-    using CT = ColShiftTile<N, E0, CW0>;
+    using CT = ColShiftTile<N, E0, CW0, WC>;
     const int nsh_all = D * (C - 1), total = njobs * nsh_all * (N / CT::cw);
-    hipLaunchKernelGGL((k_col_shift<N, E0, CW0>), dim3(FSIM_COL_XCD ? xcd_grid(total) : (unsigned)total),
+    hipLaunchKernelGGL((k_col_shift<N, E0, CW0, WC>), dim3(FSIM_COL_XCD ? xcd_grid(total) : (unsigned)total),
                        dim3(CT::value), 0, s, jl, nsh_all, total, twn);
 }
 
@@ -1128,6 +1144,8 @@ static void col_candidate_shape(int col, int &e, int &cw) {
         case 3: e = ColShiftTile<N, 8, 16>::e; cw = ColShiftTile<N, 8, 16>::cw; break;
         case 4: e = ColShiftTile<N, 16, 4>::e; cw = ColShiftTile<N, 16, 4>::cw; break;
         case 5: e = ColShiftTile<N, 8, 8>::e; cw = ColShiftTile<N, 8, 8>::cw; break;
+        case 6: e = ColShiftTile<N, 0, 8, true>::e; cw = -ColShiftTile<N, 0, 8, true>::cw; break;
+        case 7: e = ColShiftTile<N, 0, 4, true>::e; cw = -ColShiftTile<N, 0, 4, true>::cw; break;
         default: e = ColShiftTile<N, 0, FSIM_COL_CW>::e; cw = ColShiftTile<N, 0, FSIM_COL_CW>::cw; break;
     }
 }
@@ -1168,6 +1186,8 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int P, int C, 
             case 3: launch_col_shift<N, 8, 16>(jl, njobs, D, C, twn, s); break;
             case 4: launch_col_shift<N, 16, 4>(jl, njobs, D, C, twn, s); break;
             case 5: launch_col_shift<N, 8, 8>(jl, njobs, D, C, twn, s); break;
+            case 6: launch_col_shift<N, 0, 8, true>(jl, njobs, D, C, twn, s); break;
+            case 7: launch_col_shift<N, 0, 4, true>(jl, njobs, D, C, twn, s); break;
             default: launch_col_shift<N, 0, FSIM_COL_CW>(jl, njobs, D, C, twn, s); break;
         }
         stage_mark(1, s, ev);

@@ -433,9 +433,10 @@ int fsim_batch_destroy(fsim_batch b);
  * accel/FFTProvider.java:63-102).  Per size class (n, D, P, stored
  * components Cs, Hermitian result half) the pipeline chooses: sep = 1 the
  * separation with all units of a row in one wavefront (n <= 512, D*P <= 9),
- * 0 one unit per block; col = the column pass's candidate 0..5 (elements per
- * thread = Stockham radix 8 or 16, columns per block 4, 8 or 16; the shape is
- * reported in col_elems / col_width).  fsim_autotune times every candidate on
+ * 0 one unit per block; col = the column pass's candidate 0..7 (elements per
+ * thread = Stockham radix 8, 16 or 32, columns per block 4, 8 or 16; the
+ * shape is reported in col_elems / col_width, col_width negated for the
+ * layouts with one column's transform per wavefront).  fsim_autotune times every candidate on
  * the channel's class (count zero-filled stacks; after a 30 ms warm-up, three
  * interleaved rounds of `reps` timed launches each, HIP events, best round
  * per candidate) and caches the fastest -- the default unless a candidate is

@@ -23,7 +23,7 @@ def test_set_get_and_unknown_class():
     with pytest.raises(FsimError):
         autotune.set_choice(512, 3, 3, 2, 1, 2, 0)             # sep not 0/1
     with pytest.raises(FsimError):
-        autotune.set_choice(512, 3, 3, 2, 1, 1, 6)             # no such column candidate
+        autotune.set_choice(512, 3, 3, 2, 1, 1, 8)             # no such column candidate
 
 
 def test_save_load_round_trip_and_merge(tmp_path):

@@ -33,7 +33,7 @@ def test_candidates_agree_large(fsim_lib, n):
     cls = autotune.class_of(rec)
     default = rec.reconstruct_host(raw)
     try:
-        for col in range(6):
+        for col in range(8):
             print(f"[autotune] n={n} col={col}", flush=True)
             autotune.set_choice(*cls, 0, col)
             e = fp64.rel_l2(rec.reconstruct_host(raw), default)
@@ -50,7 +50,7 @@ def test_candidates_agree_and_tuned_matches_oracle(tuned_channel):
     assert cls == (n, 3, 3, 2, 1)
     default = rec.reconstruct_host(raw)
     for sep in (0, 1):
-        for col in range(6):
+        for col in range(8):
             print(f"[autotune] n={n} sep={sep} col={col}", flush=True)
             autotune.set_choice(*cls, sep, col)
             got = rec.reconstruct_host(raw)
@@ -60,7 +60,7 @@ def test_candidates_agree_and_tuned_matches_oracle(tuned_channel):
     autotune.clear()
     res = autotune.tune(rec, count=4, reps=2)
     print(res)
-    assert res["candidates"] >= 2 and 0 <= res["col"] < 6 and res["sep"] in (0, 1)
+    assert res["candidates"] >= 2 and 0 <= res["col"] < 8 and res["sep"] in (0, 1)
     assert res["ms_best"] > 0 and res["ms_best"] <= res["ms_default"]
     assert autotune.get_choice(*cls) == (res["sep"], res["col"])
     got = rec.reconstruct_host(raw)
