@@ -60,6 +60,21 @@ def test_fused_five_phase(fsim_lib, nphase, nband, h2, mode):
     assert e < TOL
 
 
+@pytest.mark.parametrize("ndir", [2, 4])
+def test_fused_other_direction_counts(fsim_lib, ndir):
+    """Two directions (2 x 3 frames: the separation with all units of a row in
+    one wavefront) and four (4 x 3 frames: one unit per block)."""
+    from fsim import synthetic
+    from fsim.sim_algorithm import Reconstructor
+    n = 128
+    raw, sp = synthetic.sim_stack(n=n, ndir=ndir, seed=30 + ndir, nbeads=60, nfil=8)
+    param = synthetic.make_param(sp, n, ndir=ndir)
+    got = Reconstructor(param).reconstruct_host(raw)
+    e = fp64.rel_l2(got, ref.recon32(ref.recon_dict(sp, n), raw, threads=8))
+    print(f"ndir={ndir} relL2 gpu-vs-ref32 {e:.3e}")
+    assert e < TOL
+
+
 def test_fused_batch_equals_single(fsim_lib):
     """Each stack of a batch gives the same image as alone (bitwise: the
     kernels are deterministic and independent per stack)."""
