@@ -56,15 +56,6 @@
 #ifndef FSIM_COL_WPE
 #define FSIM_COL_WPE 6
 #endif
-#ifndef FSIM_B_TILED
-#define FSIM_B_TILED 1
-#endif
-#ifndef FSIM_STAGE_STORE
-#define FSIM_STAGE_STORE 0   // separation pass: stores from registers
-#endif
-#ifndef FSIM_STAGE_STORE_K3
-#define FSIM_STAGE_STORE_K3 1   // row shift pass: stores through LDS
-#endif
 
 namespace fsim {
 
@@ -155,15 +146,11 @@ typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 // column kernels then read and write whole runs; the row kernels move the
 // runs of their RW rows (RW * 8 elements per tile) through LDS.
 constexpr int kAT = 8;
-#ifndef FSIM_AT_RH
-#define FSIM_AT_RH 0   // tile height in rows; 0: the whole column
-#endif
+// Tile height: the whole column (row bands of 16-64 rows measured no
+// different).
 template <int N>
 struct ATile {
-    // at least the rows of one row-kernel block (256 threads, n / E per row)
-    static constexpr int rw = (N / FftE<N>::value) >= 256 ? 1 : 256 / (N / FftE<N>::value);
-    static constexpr int req = (FSIM_AT_RH > 0 && FSIM_AT_RH < N) ? FSIM_AT_RH : N;
-    static constexpr int rh = req > rw ? req : rw;
+    static constexpr int rh = N;
 };
 template <int N>
 __device__ __forceinline__ size_t at_col(int x) {
@@ -371,14 +358,7 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int nrb, int unit
     fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
     // with a Hermitian result the gather reads DC columns 0..n/2 only
     const int ntiles = (u == 0 && half) ? N / 2 / kAT + 1 : N / kAT;
-    if constexpr (FSIM_STAGE_STORE) {
-#pragma unroll
-        for (int m = 0; m < E; m++) lds[rr * LP + t + T * m] = v[m];
-        __syncthreads();
-        tiled_rows_store<N, RW, T * RW>(glb(J.A) + (size_t)(d * C + c) * nn, rb * RW, lds, LP, ntiles);
-    } else {
-        tiled_row_store_regs<N, E>(glb(J.A) + (size_t)(d * C + c) * nn, y, t, v, ntiles * kAT - 1);
-    }
+    tiled_row_store_regs<N, E>(glb(J.A) + (size_t)(d * C + c) * nn, y, t, v, ntiles * kAT - 1);
 }
 
 // K1, all units of a row in one wavefront: the row's D*P raw frames are
@@ -468,15 +448,7 @@ __global__ void __launch_bounds__(256) k_sep_rows_all(JobList jl, int half, cons
         fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, row, 1, t, tws);
         // with a Hermitian result the gather reads DC columns 0..n/2 only
         const int ntiles = (u == 0 && half) ? N / 2 / kAT + 1 : N / kAT;
-        if constexpr (FSIM_STAGE_STORE) {
-#pragma unroll
-            for (int m = 0; m < E; m++) row[t + T * m] = v[m];
-            __syncthreads();
-            tiled_rows_store<N, RW, T * RW>(glb(J.A) + (size_t)(d * C + c) * nn, blockIdx.x * RW, lds, LP, ntiles);
-            __syncthreads();
-        } else {
-            tiled_row_store_regs<N, E>(glb(J.A) + (size_t)(d * C + c) * nn, y, t, v, ntiles * kAT - 1);
-        }
+        tiled_row_store_regs<N, E>(glb(J.A) + (size_t)(d * C + c) * nn, y, t, v, ntiles * kAT - 1);
     }
 }
 
@@ -625,14 +597,10 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
         v[m] = rot_java(cscale(v[m], sc), r.x, r.y);
     }
     fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
-    if constexpr (FSIM_STAGE_STORE_K3) {
 #pragma unroll
-        for (int m = 0; m < E; m++) lds[rr * LP + t + T * m] = v[m];
-        __syncthreads();
-        tiled_rows_store<N, RW, T * RW>(comp, rb * RW, lds, LP, N / kAT);
-    } else {
-        tiled_row_store_regs<N, E>(comp, rb * RW + rr, t, v, N - 1);
-    }
+    for (int m = 0; m < E; m++) lds[rr * LP + t + T * m] = v[m];
+    __syncthreads();
+    tiled_rows_store<N, RW, T * RW>(comp, rb * RW, lds, LP, N / kAT);
 }
 
 // Block -> (stack, column tile) for the gather kernel.  With >= 8 stacks in a
@@ -887,7 +855,7 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
     fft_regs<M, EM, true, false, TW::tl>(acc, lds + cl, CW, t, tws, t < TM);
     if (t < TM) {
         const float sc = 1.0f / M;
-        if constexpr (HALF && FSIM_B_TILED) {
+        if constexpr (HALF) {
             // tile-major: the tile's rows are one contiguous run
             FSIM_GLB float2 *B = glb(J.B) + (size_t)tile * M * CW + cl;
 #pragma unroll
@@ -956,45 +924,34 @@ __global__ void __launch_bounds__(256) k_row_c2r(JobList jl, const float2 *__res
     const int y = grp * RW + rr;
     const FSIM_GLB float2 *tg = glb(twm);
     float2 a[E], b[E], w[E];
-    if constexpr (FSIM_B_TILED) {
-        // row-major LDS copy of columns [0, n + cw) of the block's rows
-        constexpr int RL = N + CW + 1;              // odd row pitch
-        constexpr int RUN = RW * CW;                // elements per tile run
-        constexpr int TOT = (N / CW + 1) * RUN;
-        constexpr int PER = (TOT + NT - 1) / NT;
-        __shared__ float2 rows[RW * RL];
-        const FSIM_GLB float2 *bt = glb(J.B) + (size_t)grp * RUN;
-        float2 u[PER];
+    // row-major LDS copy of columns [0, n + cw) of the block's rows
+    constexpr int RL = N + CW + 1;              // odd row pitch
+    constexpr int RUN = RW * CW;                // elements per tile run
+    constexpr int TOT = (N / CW + 1) * RUN;
+    constexpr int PER = (TOT + NT - 1) / NT;
+    __shared__ float2 rows[RW * RL];
+    const FSIM_GLB float2 *bt = glb(J.B) + (size_t)grp * RUN;
+    float2 u[PER];
 #pragma unroll
-        for (int r = 0; r < PER; r++) {
-            const int e = (int)threadIdx.x + NT * r;
-            if (TOT % NT == 0 || e < TOT) u[r] = ld(bt + (size_t)(e / RUN) * M * CW + e % RUN);
-        }
+    for (int r = 0; r < PER; r++) {
+        const int e = (int)threadIdx.x + NT * r;
+        if (TOT % NT == 0 || e < TOT) u[r] = ld(bt + (size_t)(e / RUN) * M * CW + e % RUN);
+    }
 #pragma unroll
-        for (int r = 0; r < PER; r++) {
-            const int e = (int)threadIdx.x + NT * r;
-            if (TOT % NT == 0 || e < TOT) {
-                const int q = e % RUN;
-                rows[(q / CW) * RL + (e / RUN) * CW + q % CW] = u[r];
-            }
+    for (int r = 0; r < PER; r++) {
+        const int e = (int)threadIdx.x + NT * r;
+        if (TOT % NT == 0 || e < TOT) {
+            const int q = e % RUN;
+            rows[(q / CW) * RL + (e / RUN) * CW + q % CW] = u[r];
         }
-        __syncthreads();
+    }
+    __syncthreads();
 #pragma unroll
-        for (int m = 0; m < E; m++) {
-            const int k = t + T * m;
-            a[m] = rows[rr * RL + k];
-            b[m] = rows[rr * RL + N - k];
-            w[m] = ld(tg + k);   // exp(-2 pi i k / m)
-        }
-    } else {
-        const FSIM_GLB float2 *row = glb(J.B) + (size_t)y * M;
-#pragma unroll
-        for (int m = 0; m < E; m++) {
-            const int k = t + T * m;
-            a[m] = ld(row + k);
-            b[m] = ld(row + (N - k));
-            w[m] = ld(tg + k);   // exp(-2 pi i k / m)
-        }
+    for (int m = 0; m < E; m++) {
+        const int k = t + T * m;
+        a[m] = rows[rr * RL + k];
+        b[m] = rows[rr * RL + N - k];
+        w[m] = ld(tg + k);   // exp(-2 pi i k / m)
     }
     float2 v[E];
 #pragma unroll
