@@ -2,6 +2,8 @@
 // twiddle tables, device scratch and the vector lifecycle.
 #include "fsim_internal.h"
 
+#include <cstdlib>
+
 #include <atomic>
 #include <cmath>
 #include <map>
@@ -95,6 +97,10 @@ int fsim_init(int device) {
     FSIM_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     g_device = device;
     g_init = true;
+    // a saved autotuning cache named by FSIM_AUTOTUNE_CACHE is merged in; a
+    // missing or malformed file leaves the defaults (and the message in
+    // fsim_last_error) without failing the initialisation
+    if (const char *path = std::getenv("FSIM_AUTOTUNE_CACHE")) (void)fsim_autotune_load(path);
     return FSIM_OK;
 }
 

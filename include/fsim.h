@@ -445,7 +445,9 @@ int fsim_batch_destroy(fsim_batch b);
  * uses it.  save/load persist the cache as text ("fsim-autotune 1" then
  * "n D P Cs half sep col count ms" per line); load merges and rejects a
  * malformed file whole.  set pins a choice by hand, get returns -1s when the
- * class has no entry.  These calls do not need a GPU except fsim_autotune. */
+ * class has no entry.  These calls do not need a GPU except fsim_autotune.
+ * fsim_init loads the file named by the environment variable
+ * FSIM_AUTOTUNE_CACHE, when set (a missing or malformed file is ignored). */
 typedef struct {
     int sep, col, col_elems, col_width, candidates;
     double ms_best, ms_default;   /* per launch of `count` stacks */

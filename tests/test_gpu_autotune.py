@@ -65,3 +65,22 @@ def test_candidates_agree_and_tuned_matches_oracle(tuned_channel):
     assert autotune.get_choice(*cls) == (res["sep"], res["col"])
     got = rec.reconstruct_host(raw)
     assert fp64.rel_l2(got, ref.recon32(ref.recon_dict(sp, n), raw)) < 1e-4
+
+
+def test_init_loads_cache_named_by_environment(fsim_lib, tmp_path):
+    """fsim_init merges the cache file named by FSIM_AUTOTUNE_CACHE (a fresh
+    process: the library initialises once per process)."""
+    import os
+    import subprocess
+    import sys
+    path = tmp_path / "tune.txt"
+    path.write_text("fsim-autotune 1\n256 3 3 2 1 0 5 16 0.25\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys; sys.path[:0] = [%r, %r]\n"
+            "from fsim import _lib, autotune\n"
+            "_lib.init(0)\n"
+            "print('choice', autotune.get_choice(256, 3, 3, 2, 1))\n") % (root, os.path.join(root, "fairsim-vigor_amd"))
+    env = dict(os.environ, FSIM_AUTOTUNE_CACHE=str(path))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr[-500:])
+    assert "choice (0, 5)" in out.stdout

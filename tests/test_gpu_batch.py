@@ -127,3 +127,26 @@ def test_batch_rejects_unaligned_addresses(setup):
         b.run([rb.value], [ob.value + 4])
     assert not b.info()["captured"]
     b.close()
+
+
+def test_batch_and_params_argument_errors(setup):
+    """Refusals before any device work: a null channel or an empty slot list,
+    and server requests with their own parameters that name no channel or no
+    stacks."""
+    from fsim._lib import FsimError, call, SIM_PARAMS
+    from fsim.livemode import ReconstructionServer
+    chans, _ = setup
+    raw, rec = chans[(64, 525.0)]
+    h = C.c_void_p()
+    with pytest.raises(FsimError):
+        call("fsim_batch_create", (C.c_void_p * 1)(None), 1, 1, C.byref(h))
+    with pytest.raises(FsimError):
+        call("fsim_batch_create", (C.c_void_p * 1)(rec.handle.value), 0, 1, C.byref(h))
+    srv = ReconstructionServer(mem_budget=1 << 30, max_batch=4, max_queue=16, nslots=1)
+    cid = srv.add_channel(rec)
+    out = np.zeros((2 * 64, 2 * 64), np.float32)
+    with pytest.raises(FsimError):
+        srv.submit_params(cid + 5, raw, out, 1, SIM_PARAMS())
+    with pytest.raises(FsimError):
+        srv.submit_params(cid, raw, out, 0, SIM_PARAMS())
+    srv.close()
