@@ -400,16 +400,20 @@ __global__ void k_eb_apx_tab(const EstPair *__restrict__ pairs, int n, float2 *_
 // Approximate correlations A(c) of all 100 candidates and sum|q| of each
 // pair; grid (n / rows, npairs), 256 threads as 16 row groups of 16 lanes.
 // Per row: R_i = sum_x TX_i(x) q(x, y) (lanes split x, then a 16-lane sum),
-// then lane i < 10 adds TY_j(y) R_i to candidate j * 10 + i.  Partial sums
+// kept per row in LDS; then thread j * 10 + i forms candidate (i, j) as
+// sum_y TY_j(y) R_i(y) over the block's rows.  Partial sums
 // apart[(pair * kPairSlots + c) * nblk + block], sum|q| in slot kNormSlot.
-constexpr int kApxTile = 512;
-__global__ void __launch_bounds__(256) k_eb_approx(const float2 *__restrict__ T, int n,
-                                                   const EstPair *__restrict__ pairs, int npairs,
-                                                   const float2 *__restrict__ tabs, int rows, int nblk,
-                                                   double2 *__restrict__ apart) {
+// (Row sums in LDS rather than 10 per-lane candidate accumulators, and x in
+// tiles of 256: fewer registers and less LDS per block, so more waves hide
+// the latency of the double-precision chains.)
+constexpr int kApxTile = 256, kApxRows = 32;
+__global__ void __launch_bounds__(256, 4) k_eb_approx(const float2 *__restrict__ T, int n,
+                                                      const EstPair *__restrict__ pairs, int npairs,
+                                                      const float2 *__restrict__ tabs, int rows, int nblk,
+                                                      double2 *__restrict__ apart) {
     __shared__ float2 tx[10 * kApxTile];
-    __shared__ float2 ty[10 * 32];
-    __shared__ double2 red[16][kMaxCand];
+    __shared__ float2 ty[10 * kApxRows];
+    __shared__ double2 rsum[kApxRows][10];
     __shared__ double ared[4];
     const int p = blockIdx.y, blk = blockIdx.x;
     const int t = threadIdx.x, rg = t / 16, l = t % 16;
@@ -418,9 +422,7 @@ __global__ void __launch_bounds__(256) k_eb_approx(const float2 *__restrict__ T,
     const float2 *TX = tabs + (size_t)p * 20 * n, *TY = TX + 10 * n;
     const int y0 = blk * rows;
     for (int k = t; k < 10 * rows; k += 256) ty[k] = TY[(k / rows) * n + y0 + k % rows];
-    double2 S[10];
-#pragma unroll
-    for (int j = 0; j < 10; j++) S[j] = make_double2(0.0, 0.0);
+    for (int k = t; k < 10 * rows; k += 256) rsum[k / 10][k % 10] = make_double2(0.0, 0.0);
     double aq = 0.0;
     const int tw = n < kApxTile ? n : kApxTile;
     for (int xt0 = 0; xt0 < n; xt0 += tw) {
@@ -471,27 +473,25 @@ __global__ void __launch_bounds__(256) k_eb_approx(const float2 *__restrict__ T,
 #pragma unroll
             for (int i = 1; i < 10; i++)
                 if (l == i) Ri = R[i];
-            if (l < 10)
-#pragma unroll
-                for (int j = 0; j < 10; j++) {
-                    const float2 e = ty[j * rows + r];
-                    S[j].x += Ri.x * e.x - Ri.y * e.y;
-                    S[j].y += Ri.x * e.y + Ri.y * e.x;
-                }
+            // row r belongs to this row group in every x tile
+            if (l < 10) {
+                rsum[r][l].x += Ri.x;
+                rsum[r][l].y += Ri.y;
+            }
         }
     }
-    if (l < 10)
-#pragma unroll
-        for (int j = 0; j < 10; j++) red[rg][j * 10 + l] = S[j];
     for (int o = 32; o > 0; o >>= 1) aq += __shfl_down(aq, o, 64);
     if (t % 64 == 0) ared[t / 64] = aq;
     __syncthreads();
     double2 *out = apart + (size_t)p * kPairSlots * nblk;
     if (t < kMaxCand) {
+        const int i = t % 10, j = t / 10;
         double2 s = make_double2(0.0, 0.0);
-        for (int g = 0; g < 16; g++) {
-            s.x += red[g][t].x;
-            s.y += red[g][t].y;
+        for (int r = 0; r < rows; r++) {
+            const float2 e = ty[j * rows + r];
+            const double2 R = rsum[r][i];
+            s.x += R.x * e.x - R.y * e.y;
+            s.y += R.x * e.y + R.y * e.x;
         }
         out[(size_t)t * nblk + blk] = s;
     }
@@ -714,7 +714,7 @@ static int eb_round(const EbRegion &r, const float2 *dct, EstPair *pairs, int np
     if (!rc) rc = fft_cols_batch_device(g_eb.W, n, n, 2 * npairs, 1, s);
     if (rc) return rc;
     if (grid) {
-        const int rows = n < 32 ? n : 32, nblk = n / rows;
+        const int rows = n < kApxRows ? n : kApxRows, nblk = n / rows;
         hipLaunchKernelGGL(k_eb_apx_tab, dim3(cdiv((size_t)20 * n, 256), npairs), dim3(256), 0, s, pairs, n, tabs);
         hipLaunchKernelGGL(k_eb_approx, dim3(nblk, npairs), dim3(256), 0, s, g_eb.W, n, pairs, npairs, tabs, rows,
                            nblk, apart);

@@ -1020,7 +1020,9 @@ __global__ void k_wiener_multi(float *__restrict__ wa, const float *__restrict__
 #pragma clang fp contract(off)
     const size_t mm = (size_t)M * M;
     const int st = blockIdx.y;
-    const ShiftSet S = sh[st];
+    // read in place (uniform loads): a local copy indexed by the direction
+    // loop lives in scratch memory
+    const ShiftSet &S = sh[st];
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x) {
         int x, y;
         tile_major_xy(i, M, cw, x, y);

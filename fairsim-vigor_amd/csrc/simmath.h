@@ -7,6 +7,22 @@
 
 namespace fsim {
 
+// a / b rounded to nearest, from ib = 1 / b rounded to nearest: q = a * ib is
+// within an ulp of a / b, the residual a - b * q is exact in one FMA, and
+// RN(q + (a - b * q) * ib) = RN(a / b) (Markstein's theorem), so the result is
+// the IEEE quotient bit for bit.  Two FMAs and a product instead of the
+// division sequence; a NaN, infinite, huge or tiny a (where the residual could
+// overflow or underflow) divides.  Callers pass b's reciprocal as an
+// expression of loop-invariant values (a constant, a kernel argument), which
+// the compiler folds or computes once.
+__device__ __forceinline__ double div_rn(double a, double b, double ib) {
+    const double m = fabs(a);
+    if (!(m >= 0x1p-900 && m <= 0x1p900)) return a / b;
+    const double q = a * ib;
+    const double r = fma(-q, b, a);
+    return fma(r, ib, q);
+}
+
 // MTool.aproxSin01 / fsin / fcos (linalg/MTool.java:34-59), in double.
 __device__ __forceinline__ double aprox_sin01(double x) {
 #pragma clang fp contract(off)
@@ -17,7 +33,7 @@ __device__ __forceinline__ double aprox_sin01(double x) {
 }
 __device__ __forceinline__ double fsin_java(double x) {
 #pragma clang fp contract(off)
-    x = x / (M_PI / 2);
+    x = div_rn(x, M_PI / 2, 1 / (M_PI / 2));
     x -= floor(0.25 * x) * 4;
     if (x > 3) return -aprox_sin01(4 - x);
     if (x > 1) return aprox_sin01(2 - x);
@@ -29,7 +45,7 @@ __device__ __forceinline__ double fsin_java(double x) {
 __device__ __forceinline__ void shift_cos_sin(double kx, double ky, int x, int y, int N, float &co,
                                               float &si) {
 #pragma clang fp contract(off)
-    float pha = (float)(2 * M_PI * (kx * x + ky * y) / N);
+    float pha = (float)div_rn(2 * M_PI * (kx * x + ky * y), (double)N, 1.0 / N);
     double p = (double)pha;
     co = (float)fsin_java(p + M_PI / 2);
     si = (float)fsin_java(p);
@@ -53,7 +69,7 @@ __device__ __forceinline__ int otf_index(int ns, double cpm, double cutoff, doub
 #pragma clang fp contract(off)
     f = 0.f;
     if (cycl >= cutoff) return -1;
-    const double pos = cycl / cpm;
+    const double pos = div_rn(cycl, cpm, 1 / cpm);
     if (ceil(pos) >= (double)ns) return -1;
     const int l = (int)floor(pos);
     f = (float)(pos - l);
@@ -77,7 +93,7 @@ __device__ __forceinline__ float att_val(const float *__restrict__ only_att, dou
                                          double cycl) {
 #pragma clang fp contract(off)
     if (cycl >= cutoff) return 1.f;
-    const double pos = cycl / cpm;
+    const double pos = div_rn(cycl, cpm, 1 / cpm);
     const int l = (int)floor(pos);
     const float f = (float)(pos - l);
     const float retl = only_att[l] * f;
