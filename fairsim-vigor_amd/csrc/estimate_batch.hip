@@ -35,6 +35,7 @@
 //      (band 0 copies first, then band 1 copies)
 #include "fsim_internal.h"
 #include "fft_core.h"
+#include "fft_launch.h"
 #include "filters.h"
 #include "estimate_dev.h"
 
@@ -185,10 +186,66 @@ __global__ void k_eb_separate(float2 *__restrict__ X, float2 *__restrict__ W, in
     W[((size_t)G + g) * nn + i] = make_float2(f.x * att, f.y * att);
 }
 
-// c_fb.timesConj(c0) (AbstractVectorCplx.java:288-304) for all groups.
-__global__ void k_eb_times_conj(float2 *__restrict__ W, size_t Gnn) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < Gnn; i += (size_t)gridDim.x * blockDim.x)
-        W[Gnn + i] = times_conj_java(W[Gnn + i], W[i]);
+// The column pass of the coarse correlation's inverse transforms fused with
+// c_fb.timesConj(c0) (SimAlgorithm.java:85-101, AbstractVectorCplx.java:
+// 288-304): one block transforms the same column tile of both images of a
+// group, each column exactly as the stand-alone column pass does (scaled by
+// 1 / L), and writes the product over c_fb; c0 in real space is not read
+// again.  Saves the product's pass over three images and the write of c0.
+// Grid (L / CW, G); W holds c0 of the G groups, then c_fb (Gnn = G * L * L
+// apart).
+template <int L>
+__global__ void __launch_bounds__(1024) k_eb_cols_conj(float2 *__restrict__ W, size_t Gnn,
+                                                       const float2 *__restrict__ tw) {
+    constexpr int E = FftE<L>::value, T = L / E;
+    constexpr int CW = ColW<L>::value;
+    __shared__ float2 lds[FftLds<L, E>::len * CW];
+    using TW = TwLds<L, (FftLds<L, E>::len * CW + L) * 8 <= 163840>;
+    __shared__ float2 twl[TW::len];
+    const float2 *tws = tw_table<TW::tl, T * CW>(twl, tw);
+    const int c = threadIdx.x % CW, t = threadIdx.x / CW;
+    const float2 *a = W + (size_t)blockIdx.y * L * L + blockIdx.x * CW + c;
+    float2 *b = W + Gnn + (size_t)blockIdx.y * L * L + blockIdx.x * CW + c;
+    float2 v[E], u[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        v[m] = a[(size_t)(t + T * m) * L];
+        u[m] = b[(size_t)(t + T * m) * L];
+    }
+    // fft_regs ends with a barrier after its last LDS reads, so the second
+    // transform may reuse the region at once
+    fft_regs<L, E, true, false, TW::tl>(v, lds + c, CW, t, tws);
+    fft_regs<L, E, true, false, TW::tl>(u, lds + c, CW, t, tws);
+    constexpr float sc = 1.0f / L;
+#pragma unroll
+    for (int m = 0; m < E; m++) b[(size_t)(t + T * m) * L] = times_conj_java(cscale(u[m], sc), cscale(v[m], sc));
+}
+
+template <int L>
+static int launch_cols_conj_t(float2 *W, int G, hipStream_t s) {
+    constexpr int E = FftE<L>::value, T = L / E;
+    constexpr int CW = ColW<L>::value;
+    static_assert(L % CW == 0, "column tiles cover the image");
+    const float2 *tw = twiddles(L);
+    if (!tw) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
+    hipLaunchKernelGGL(k_eb_cols_conj<L>, dim3(L / CW, G), dim3(T * CW), 0, s, W, (size_t)G * L * L, tw);
+    FSIM_HIP(hipGetLastError());
+    return FSIM_OK;
+}
+
+static int launch_cols_conj(int n, float2 *W, int G, hipStream_t s) {
+    switch (n) {
+        case 16: return launch_cols_conj_t<16>(W, G, s);
+        case 32: return launch_cols_conj_t<32>(W, G, s);
+        case 64: return launch_cols_conj_t<64>(W, G, s);
+        case 128: return launch_cols_conj_t<128>(W, G, s);
+        case 256: return launch_cols_conj_t<256>(W, G, s);
+        case 512: return launch_cols_conj_t<512>(W, G, s);
+        case 1024: return launch_cols_conj_t<1024>(W, G, s);
+        case 2048: return launch_cols_conj_t<2048>(W, G, s);
+        case 4096: return launch_cols_conj_t<4096>(W, G, s);
+        default: return fail(FSIM_ERR_ARG, "estimate: unsupported size");
+    }
 }
 
 // Correlation.locatePeak (Correlation.java:144-167) key of each group's
@@ -339,7 +396,7 @@ __global__ void __launch_bounds__(256) k_eb_region_rows(const float2 *__restrict
             // ring: band 0 at its own pixel, band 1 at its source pixel
             // ((x + ikx) mod n, (y - iky) mod n)
             const int xq = band ? ((x + a.ikx) % L + L) % L : x, yq = band ? ((row - a.iky) % L + L) % L : row;
-            v[m] = region_dc(src[x], dc[x], wt, dc_rad(xq, yq, L, L) / kmax, a.dist, a.wlim);
+            v[m] = region_dc(src[x], dc[x], wt, div_rn(dc_rad(xq, yq, L, L), kmax, 1 / kmax), a.dist, a.wlim);
         }
     }
     fft_regs<L, E, true, RowWave<L>::value, TW::tl>(v, lds + rr * LP, 1, t, tw, active);
@@ -698,13 +755,6 @@ static int eb_grow(void **p, size_t &have, size_t need) {
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-static void grid_stride_launch(size_t n, unsigned &blocks) {
-    size_t b = (n + 255) / 256;
-    if (b < 1) b = 1;
-    if (b > 65535 * 8) b = 65535 * 8;
-    blocks = (unsigned)b;
-}
-
 // One commonRegion / inverse FFT / correlation round for npairs pairs; with
 // `grid`, the 10 x 10 candidates of fitPeak through the approximate filter
 // (tabs: 20 n phasors per pair, apart: partial sums of n / rows blocks).
@@ -816,19 +866,17 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     hipLaunchKernelGGL(k_eb_init, dim3(gb), dim3(64), 0, s, grp, G);
     hipLaunchKernelGGL(k_eb_dc_tab, dim3((unsigned)std::min<size_t>((ndc * nn + 255) / 256, 4096)), dim3(256), 0, s,
                        dct, n, reg, ndc);
-    unsigned blocks;
     rc = launch_load_rows(n, images, g_eb.X, G * P * n, fade, fade + n, s);
     if (rc) return rc;
     rc = fft_cols_batch_device(g_eb.X, n, n, G * P, 0, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_eb_separate, dim3(cdiv(nn, 256), G), dim3(256), 0, s, g_eb.X, g_eb.W, n, G, sep);
     FSIM_HIP(hipGetLastError());
-    // coarse correlation: both to real space, c_fb * conj(c0), back
-    rc = fft2d_batch_device(g_eb.W, n, n, 2 * G, 1, s);
+    // coarse correlation: both to real space (row pass, then the column pass
+    // fused with c_fb * conj(c0)), back
+    rc = fft_rows_batch_device(g_eb.W, n, n, 2 * G, 1, s);
+    if (!rc) rc = launch_cols_conj(n, g_eb.W, G, s);
     if (rc) return rc;
-    grid_stride_launch((size_t)G * nn, blocks);
-    hipLaunchKernelGGL(k_eb_times_conj, dim3(blocks), dim3(256), 0, s, g_eb.W, (size_t)G * nn);
-    FSIM_HIP(hipGetLastError());
     rc = fft2d_batch_device(g_eb.W + (size_t)G * nn, n, n, G, 0, s);
     if (rc) return rc;
     const double kmin = p->fit_exclude * o.cutoff / p->vec_cpm;

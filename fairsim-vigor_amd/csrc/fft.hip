@@ -141,6 +141,13 @@ int fft_cols_batch_device(float2 *data, int w, int h, int nimg, int inverse, hip
     return launch_cols(h, data, w, w, nimg, (size_t)w * h, inverse, s);
 }
 
+int fft_rows_batch_device(float2 *data, int w, int h, int nimg, int inverse, hipStream_t s) {
+    if (!fft_size_ok(w) || !fft_size_ok(h))
+        return fail(FSIM_ERR_ARG, "fft2d: width and height must be powers of two in [2, 4096]");
+    if (nimg < 1 || nimg > 65535) return fail(FSIM_ERR_ARG, "fft2d: batch of 1 to 65535 images");
+    return launch_rows(w, data, h * nimg, w, inverse, s);
+}
+
 int fft2d_device(float2 *data, int w, int h, int inverse, hipStream_t s) {
     return fft2d_batch_device(data, w, h, 1, inverse, s);
 }

@@ -44,6 +44,8 @@ int fft2d_batch_device(float2 *data, int w, int h, int nimg, int inverse, hipStr
 // the column pass alone (the second half of fft2d_batch_device, whose row
 // pass runs first): for callers that fuse their own row pass
 int fft_cols_batch_device(float2 *data, int w, int h, int nimg, int inverse, hipStream_t s);
+// the row pass alone (the first half of fft2d_batch_device)
+int fft_rows_batch_device(float2 *data, int w, int h, int nimg, int inverse, hipStream_t s);
 
 // small helpers
 inline bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }

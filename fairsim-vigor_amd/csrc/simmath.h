@@ -11,14 +11,15 @@ namespace fsim {
 // within an ulp of a / b, the residual a - b * q is exact in one FMA, and
 // RN(q + (a - b * q) * ib) = RN(a / b) (Markstein's theorem), so the result is
 // the IEEE quotient bit for bit.  Two FMAs and a product instead of the
-// division sequence; a NaN, infinite, huge or tiny a (where the residual could
-// overflow or underflow) divides.  Callers pass b's reciprocal as an
-// expression of loop-invariant values (a constant, a kernel argument), which
-// the compiler folds or computes once.
+// division sequence; where a or q is zero, NaN, infinite, huge or tiny (the
+// residual could overflow or underflow; b zero, infinite or NaN lands here
+// too) it divides.  Callers pass b's reciprocal as an expression of
+// loop-invariant values (a constant, a kernel argument), which the compiler
+// folds or computes once.
 __device__ __forceinline__ double div_rn(double a, double b, double ib) {
-    const double m = fabs(a);
-    if (!(m >= 0x1p-900 && m <= 0x1p900)) return a / b;
     const double q = a * ib;
+    const double m = fabs(a), mq = fabs(q);
+    if (!(m >= 0x1p-900 && m <= 0x1p900 && mq >= 0x1p-900 && mq <= 0x1p900)) return a / b;
     const double r = fma(-q, b, a);
     return fma(r, ib, q);
 }
