@@ -437,38 +437,52 @@ static int launch_region_rows(int n, const EbRegion &r, const float2 *dct, EstPa
     }
 }
 
-// Phasor tables of the approximate candidate evaluation, per pair:
-// TX[i][x] = e^{+2 pi i xpos_i x / n}, TY[j][y] = e^{-2 pi i ypos_j y / n}
-// (the shift of b1 by (xpos, -ypos)), accurate in double, stored as float.
+// Phasor tables of the approximate candidate evaluation, per pair (20 n
+// float2 reserved): TY[j][y] = e^{-2 pi i ypos_j y / n} as float (10 n
+// float2), then as double2 TXB[x] = e^{+2 pi i xpos_0 x / n} and
+// TXS[x] = e^{+2 pi i (search / 4.5) x / n}, so that
+// TX_i(x) = e^{+2 pi i xpos_i x / n} = TXB[x] TXS[x]^i (the grid's positions
+// are xpos_0 + i search / 4.5; the shift of b1 by (xpos, -ypos)).  Phases
+// reduced mod 1 and evaluated in double.
+__device__ __forceinline__ const double2 *apx_txb(const float2 *tabs, int p, int n) {
+    return reinterpret_cast<const double2 *>(tabs + (size_t)p * 20 * n + 10 * n);
+}
 __global__ void k_eb_apx_tab(const EstPair *__restrict__ pairs, int n, float2 *__restrict__ tabs) {
     const int p = blockIdx.y;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= 20 * n) return;
-    const bool isy = k >= 10 * n;
-    const int kk = isy ? k - 10 * n : k, idx = kk / n, v = kk % n;
-    const double pos = isy ? grid_pos(pairs[p].ky, idx, pairs[p].search) : grid_pos(pairs[p].kx, idx, pairs[p].search);
+    if (k >= 12 * n) return;
+    const double search = pairs[p].search;
+    double pos, sn, cs;
+    int v;
+    if (k < 10 * n) {
+        pos = grid_pos(pairs[p].ky, k / n, search);
+        v = k % n;
+    } else {
+        v = (k - 10 * n) % n;
+        pos = k < 11 * n ? grid_pos(pairs[p].kx, 0, search) : search / 4.5;
+    }
     double u = pos * v / n;
     u -= floor(u);
-    double sn, cs;
     sincospi(2 * u, &sn, &cs);
-    tabs[(size_t)p * 20 * n + k] = make_float2((float)cs, (float)(isy ? -sn : sn));
+    if (k < 10 * n)
+        tabs[(size_t)p * 20 * n + k] = make_float2((float)cs, (float)-sn);
+    else
+        const_cast<double2 *>(apx_txb(tabs, p, n))[k - 10 * n] = make_double2(cs, sn);
 }
 
 // Approximate correlations A(c) of all 100 candidates and sum|q| of each
 // pair; grid (n / rows, npairs), 256 threads as 16 row groups of 16 lanes.
-// Per row: R_i = sum_x TX_i(x) q(x, y) (lanes split x, then a 16-lane sum),
-// kept per row in LDS; then thread j * 10 + i forms candidate (i, j) as
+// Per row: R_i = sum_x TX_i(x) q(x, y) (lanes split x, then a 16-lane sum;
+// TX_i(x) q(x, y) by the recurrence z_0 = TXB q, z_{i+1} = z_i TXS), kept per
+// row in LDS; then thread j * 10 + i forms candidate (i, j) as
 // sum_y TY_j(y) R_i(y) over the block's rows.  Partial sums
 // apart[(pair * kPairSlots + c) * nblk + block], sum|q| in slot kNormSlot.
-// (Row sums in LDS rather than 10 per-lane candidate accumulators, and x in
-// tiles of 256: fewer registers and less LDS per block, so more waves hide
-// the latency of the double-precision chains.)
 constexpr int kApxTile = 256, kApxRows = 32;
 __global__ void __launch_bounds__(256, 4) k_eb_approx(const float2 *__restrict__ T, int n,
                                                       const EstPair *__restrict__ pairs, int npairs,
                                                       const float2 *__restrict__ tabs, int rows, int nblk,
                                                       double2 *__restrict__ apart) {
-    __shared__ float2 tx[10 * kApxTile];
+    __shared__ double2 txb[kApxTile], txs[kApxTile];
     __shared__ float2 ty[10 * kApxRows];
     __shared__ double2 rsum[kApxRows][10];
     __shared__ double ared[4];
@@ -476,7 +490,8 @@ __global__ void __launch_bounds__(256, 4) k_eb_approx(const float2 *__restrict__
     const int t = threadIdx.x, rg = t / 16, l = t % 16;
     const size_t nn = (size_t)n * n;
     const float2 *b0 = T + (size_t)p * nn, *b1 = T + ((size_t)npairs + p) * nn;
-    const float2 *TX = tabs + (size_t)p * 20 * n, *TY = TX + 10 * n;
+    const float2 *TY = tabs + (size_t)p * 20 * n;
+    const double2 *TXB = apx_txb(tabs, p, n), *TXS = TXB + n;
     const int y0 = blk * rows;
     for (int k = t; k < 10 * rows; k += 256) ty[k] = TY[(k / rows) * n + y0 + k % rows];
     for (int k = t; k < 10 * rows; k += 256) rsum[k / 10][k % 10] = make_double2(0.0, 0.0);
@@ -484,7 +499,10 @@ __global__ void __launch_bounds__(256, 4) k_eb_approx(const float2 *__restrict__
     const int tw = n < kApxTile ? n : kApxTile;
     for (int xt0 = 0; xt0 < n; xt0 += tw) {
         __syncthreads();
-        for (int k = t; k < 10 * tw; k += 256) tx[(k / tw) * kApxTile + k % tw] = TX[(k / tw) * n + xt0 + k % tw];
+        for (int k = t; k < tw; k += 256) {
+            txb[k] = TXB[xt0 + k];
+            txs[k] = TXS[xt0 + k];
+        }
         __syncthreads();
         for (int r = rg; r < rows; r += 16) {
             const size_t row = (size_t)(y0 + r) * n + xt0;
@@ -511,11 +529,17 @@ __global__ void __launch_bounds__(256, 4) k_eb_approx(const float2 *__restrict__
                     const double qr = (double)b.x * a.x + (double)b.y * a.y;
                     const double qi = (double)b.y * a.x - (double)b.x * a.y;
                     aq += sqrt(qr * qr + qi * qi);
+                    const double2 e = txb[x], w = txs[x];
+                    double zr = qr * e.x - qi * e.y, zi = qr * e.y + qi * e.x;
 #pragma unroll
                     for (int i = 0; i < 10; i++) {
-                        const float2 e = tx[i * kApxTile + x];
-                        R[i].x += qr * e.x - qi * e.y;
-                        R[i].y += qr * e.y + qi * e.x;
+                        R[i].x += zr;
+                        R[i].y += zi;
+                        if (i < 9) {
+                            const double nr = zr * w.x - zi * w.y;
+                            zi = zr * w.y + zi * w.x;
+                            zr = nr;
+                        }
                     }
                 }
             }
@@ -765,7 +789,7 @@ static int eb_round(const EbRegion &r, const float2 *dct, EstPair *pairs, int np
     if (rc) return rc;
     if (grid) {
         const int rows = n < kApxRows ? n : kApxRows, nblk = n / rows;
-        hipLaunchKernelGGL(k_eb_apx_tab, dim3(cdiv((size_t)20 * n, 256), npairs), dim3(256), 0, s, pairs, n, tabs);
+        hipLaunchKernelGGL(k_eb_apx_tab, dim3(cdiv((size_t)12 * n, 256), npairs), dim3(256), 0, s, pairs, n, tabs);
         hipLaunchKernelGGL(k_eb_approx, dim3(nblk, npairs), dim3(256), 0, s, g_eb.W, n, pairs, npairs, tabs, rows,
                            nblk, apart);
         hipLaunchKernelGGL(k_eb_select, dim3(npairs), dim3(128), 0, s, pairs, apart, nblk);
