@@ -25,7 +25,7 @@ namespace fsim {
 
 // Correlation.commonRegion over both bands in place.
 __global__ void k_common_region(float2 *__restrict__ b0, float2 *__restrict__ b1, int w, int h, RegionArgs a) {
-    const double kmax = sqrt(a.kx * a.kx + a.ky * a.ky);
+    const double kmax = k_len(a.kx, a.ky);
     const size_t n = (size_t)w * h;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int x = (int)(i % w), y = (int)(i / w);

@@ -53,6 +53,8 @@ struct EstPair {
     int slot1;         // slot of band 1 in the group's X
     int bn1;           // OTF band of band 1
     int err;           // shift beyond the vector in commonRegion
+    int ring_ok;       // ring_lo / ring_hi valid (|k| positive and finite)
+    int ring_lo, ring_hi;   // commonRegion's DC ring keeps squared radii in [ring_lo, ring_hi]
     int nsel;          // candidates evaluated in the reference's arithmetic
     unsigned char sel[kMaxCand];   // their indices yi * 10 + xi, ascending
 };
@@ -333,15 +335,61 @@ __global__ void k_eb_dc_tab(float2 *__restrict__ dct, int n, EbRegion r, int nb)
     }
 }
 
-// region_band0 / region_band1 with the DC weight from the table above.
-__device__ __forceinline__ float2 region_dc(float2 v, float2 dc, float2 wt, double r, double dist, double wlim) {
+// region_band0 / region_band1 with the DC weight from the table above;
+// `cut`: the pixel lies in the DC ring's cut (ratio < dist or > 1 - dist).
+__device__ __forceinline__ float2 region_dc(float2 v, float2 dc, float2 wt, bool cut, double wlim) {
 #pragma clang fp contract(off)
     if (dc.x != dc.x || abs_java(wt) < wlim)
         v = make_float2(0.f, 0.f);
     else
         v = make_float2(v.x * dc.x - v.y * dc.y, v.x * dc.y + v.y * dc.x);
-    if (r < dist || r > 1 - dist) v = make_float2(0.f, 0.f);
+    if (cut) v = make_float2(0.f, 0.f);
     return v;
+}
+
+// The ring ratio of commonRegion at squared radius d2 (an integer):
+// sqrt(d2) / |k|, rounded as the reference rounds it.
+__device__ __forceinline__ double ring_ratio(int d2, double kmax) {
+    return div_rn(sqrt((double)d2), kmax, 1 / kmax);
+}
+
+// Per pair, the squared radii the DC ring keeps.  The ratio is a
+// non-decreasing function of d2 (a rounded square root, then a rounded
+// division by a positive |k|), so ratio < dist holds exactly for d2 < lo and
+// ratio > 1 - dist exactly for d2 > hi; two binary searches over
+// d2 in [0, n^2 / 2] find lo and hi with the same arithmetic the per-pixel
+// test uses, and the pass then compares integers.  |k| zero or not finite
+// keeps the per-pixel test.
+__global__ void k_eb_ring(EstPair *__restrict__ pairs, int npairs, int n, double dist) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    const double kmax = k_len(pairs[p].kx, pairs[p].ky);
+    if (!(kmax > 0.0 && kmax <= 1e300)) {
+        pairs[p].ring_ok = 0;
+        return;
+    }
+    const int dmax = 2 * (n / 2) * (n / 2);
+    int a = 0, b = dmax + 1;   // first d2 with ratio >= dist
+    while (a < b) {
+        const int m = a + (b - a) / 2;
+        if (ring_ratio(m, kmax) < dist)
+            a = m + 1;
+        else
+            b = m;
+    }
+    const int lo = a;
+    a = 0;
+    b = dmax + 1;   // first d2 with ratio > 1 - dist
+    while (a < b) {
+        const int m = a + (b - a) / 2;
+        if (ring_ratio(m, kmax) > 1 - dist)
+            b = m;
+        else
+            a = m + 1;
+    }
+    pairs[p].ring_lo = lo;
+    pairs[p].ring_hi = a - 1;
+    pairs[p].ring_ok = 1;
 }
 
 // Correlation.commonRegion(b0, b1, 0, bn1, otf, kx, ky, 0.15, 0.05, true)
@@ -382,7 +430,9 @@ __global__ void __launch_bounds__(256) k_eb_region_rows(const float2 *__restrict
     a.ikx = (int)kx;
     a.iky = (int)ky;
     a.divide = 1;
-    const double kmax = sqrt(kx * kx + ky * ky);
+    const double kmax = k_len(kx, ky);
+    const bool ring_ok = pairs[p].ring_ok;
+    const int ring_lo = pairs[p].ring_lo, ring_hi = pairs[p].ring_hi;
     float2 v[E];
     if (active) {
         const float2 *src = X + ((size_t)grp * nphase + (band ? slot1 : 0)) * nn + (size_t)row * L;
@@ -396,7 +446,10 @@ __global__ void __launch_bounds__(256) k_eb_region_rows(const float2 *__restrict
             // ring: band 0 at its own pixel, band 1 at its source pixel
             // ((x + ikx) mod n, (y - iky) mod n)
             const int xq = band ? ((x + a.ikx) % L + L) % L : x, yq = band ? ((row - a.iky) % L + L) % L : row;
-            v[m] = region_dc(src[x], dc[x], wt, div_rn(dc_rad(xq, yq, L, L), kmax, 1 / kmax), a.dist, a.wlim);
+            const int dx = xq < L / 2 ? xq : xq - L, dy = yq < L / 2 ? yq : yq - L, d2 = dx * dx + dy * dy;
+            const bool cut = ring_ok ? (d2 < ring_lo || d2 > ring_hi)
+                                     : (ring_ratio(d2, kmax) < a.dist || ring_ratio(d2, kmax) > 1 - a.dist);
+            v[m] = region_dc(src[x], dc[x], wt, cut, a.wlim);
         }
     }
     fft_regs<L, E, true, RowWave<L>::value, TW::tl>(v, lds + rr * LP, 1, t, tw, active);
@@ -784,6 +837,7 @@ static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 // (tabs: 20 n phasors per pair, apart: partial sums of n / rows blocks).
 static int eb_round(const EbRegion &r, const float2 *dct, EstPair *pairs, int npairs, bool grid, int nphase, int n,
                     int nchunk, double2 *part, float2 *tabs, double2 *apart, hipStream_t s) {
+    hipLaunchKernelGGL(k_eb_ring, dim3(cdiv(npairs, 64)), dim3(64), 0, s, pairs, npairs, n, 0.15);
     int rc = launch_region_rows(n, r, dct, pairs, npairs, nphase, g_eb.X, g_eb.W, s);
     if (!rc) rc = fft_cols_batch_device(g_eb.W, n, n, 2 * npairs, 1, s);
     if (rc) return rc;

@@ -28,6 +28,12 @@ __device__ __forceinline__ double dc_rad(int x, int y, int w, int h) {
     return sqrt((double)(dx * dx + dy * dy));
 }
 
+// |(kx, ky)| as Math.sqrt(kx * kx + ky * ky), without fused multiply-adds.
+__device__ __forceinline__ double k_len(double kx, double ky) {
+#pragma clang fp contract(off)
+    return sqrt(kx * kx + ky * ky);
+}
+
 struct RegionArgs {
     const float2 *tab0, *tab1;   // radial OTF tables of bands bn0 and bn1 (no attenuation)
     int ns;

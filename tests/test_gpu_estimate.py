@@ -165,6 +165,23 @@ def test_estimate_batch_matches_per_call_flow(fsim_lib, nphase, nband, fit_band,
             assert abs(param.dir(d).py(nband - 1) - scale * sps[s]["py"][d]) < 0.1
 
 
+@pytest.mark.parametrize("n", [64, 128])
+def test_estimate_batch_small_sizes_match_per_call_flow(fsim_lib, n):
+    """Sizes below the candidate pass's 256-pixel x tile and 32-row blocks."""
+    from fsim.sim_algorithm import SimParam, estimate_batch, estimate_parameters, input_ffts
+    images, _ = _stacks(2, 3, (0.85, 0.8), 0.0, n=n, seed0=90)
+    param = SimParam.create(2, 3, 3, n, PX_UM, _otf(n))
+    got = estimate_batch(param, 1, 0.6, images=images)
+    for s in range(2):
+        want = estimate_parameters(param, input_ffts(images[s]), 1, 0.6)
+        for d in range(3):
+            g, w = got[s][d], want[d]
+            print(f"n={n} stack {s} dir {d}: batch {g} per-call {w}")
+            assert g["px"] == w["px"] and g["py"] == w["py"]
+            assert abs(g["pha_off"] - w["pha_off"]) <= 1e-6
+            assert abs(g["mod1"] - w["mod1"]) <= 1e-6 * w["mod1"]
+
+
 def test_estimate_batch_alone_equals_in_batch(fsim_lib):
     """Each stack's estimate is independent of the batch it runs in (bitwise)."""
     from fsim.sim_algorithm import SimParam, estimate_batch
