@@ -90,7 +90,7 @@ __device__ __forceinline__ void region_pixel(const RegionArgs &a, int x, int y, 
 // summed over all pixels in double (sumElements).  A block holds kCorrPix
 // pixels of both bands and evaluates up to kCandPerBlock candidates.
 constexpr int kMaxCand = 100;
-constexpr int kCorrThreads = 256, kCorrPpt = 4, kCorrPix = kCorrThreads * kCorrPpt;
+constexpr int kCorrThreads = 256, kCorrPpt = 8, kCorrPix = kCorrThreads * kCorrPpt;
 constexpr int kCandPerBlock = 25;
 
 // Block-wide double2 sum, result valid in thread 0.  Called by all threads.
@@ -137,10 +137,14 @@ __device__ __forceinline__ void corr_load(CorrPix &c, const float2 *__restrict__
 // This thread's part of sum(shift(b1, xpos, -ypos) * conj(b0)).
 __device__ __forceinline__ double2 corr_candidate(const CorrPix &c, double xpos, double ypos, int N) {
     double2 acc = make_double2(0.0, 0.0);
+    const bool pow2 = (N & (N - 1)) == 0;
 #pragma unroll
     for (int j = 0; j < kCorrPpt; j++) {
         float co, si;
-        shift_cos_sin(xpos, -ypos, c.px[j], c.py[j], N, co, si);
+        if (pow2)
+            shift_cos_sin_pow2(xpos, -ypos, c.px[j], c.py[j], 1.0 / N, co, si);
+        else
+            shift_cos_sin(xpos, -ypos, c.px[j], c.py[j], N, co, si);
         const float2 q = times_conj_java(rot_java(c.p1[j], co, si), c.p0[j]);
         if (c.ok[j]) {
             acc.x += (double)q.x;

@@ -52,6 +52,17 @@ __device__ __forceinline__ void shift_cos_sin(double kx, double ky, int x, int y
     si = (float)fsin_java(p);
 }
 
+// The same for a power-of-two N, from inv_n = 1 / N: the product with the
+// exact reciprocal of a power of two is the quotient, bit for bit.
+__device__ __forceinline__ void shift_cos_sin_pow2(double kx, double ky, int x, int y, double inv_n, float &co,
+                                                   float &si) {
+#pragma clang fp contract(off)
+    float pha = (float)(2 * M_PI * (kx * x + ky * y) * inv_n);
+    double p = (double)pha;
+    co = (float)fsin_java(p + M_PI / 2);
+    si = (float)fsin_java(p);
+}
+
 // Cplx.multReal / multImag of (re, im) and (co, si) (linalg/Cplx.java:325-332)
 __device__ __forceinline__ float2 rot_java(float2 v, float co, float si) {
 #pragma clang fp contract(off)
