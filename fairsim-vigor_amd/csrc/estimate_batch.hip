@@ -41,6 +41,7 @@
 
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 namespace fsim {
@@ -497,8 +498,10 @@ static int launch_region_rows(int n, const EbRegion &r, const float2 *dct, EstPa
 // TX_i(x) = e^{+2 pi i xpos_i x / n} = TXB[x] TXS[x]^i (the grid's positions
 // are xpos_0 + i search / 4.5; the shift of b1 by (xpos, -ypos)).  Phases
 // reduced mod 1 and evaluated in double.
-__device__ __forceinline__ const double2 *apx_txb(const float2 *tabs, int p, int n) {
-    return reinterpret_cast<const double2 *>(tabs + (size_t)p * 20 * n + 10 * n);
+template <typename F>
+__device__ __forceinline__ auto apx_txb(F *tabs, int p, int n) {
+    using D = std::conditional_t<std::is_const_v<F>, const double2, double2>;
+    return reinterpret_cast<D *>(tabs + (size_t)p * 20 * n + 10 * n);
 }
 __global__ void k_eb_apx_tab(const EstPair *__restrict__ pairs, int n, float2 *__restrict__ tabs) {
     const int p = blockIdx.y;
@@ -520,7 +523,7 @@ __global__ void k_eb_apx_tab(const EstPair *__restrict__ pairs, int n, float2 *_
     if (k < 10 * n)
         tabs[(size_t)p * 20 * n + k] = make_float2((float)cs, (float)-sn);
     else
-        const_cast<double2 *>(apx_txb(tabs, p, n))[k - 10 * n] = make_double2(cs, sn);
+        apx_txb(tabs, p, n)[k - 10 * n] = make_double2(cs, sn);
 }
 
 // Approximate correlations A(c) of all 100 candidates and sum|q| of each
@@ -581,7 +584,10 @@ __global__ void __launch_bounds__(256, 4) k_eb_approx(const float2 *__restrict__
                     const float2 a = av[u], b = bv[u];
                     const double qr = (double)b.x * a.x + (double)b.y * a.y;
                     const double qi = (double)b.y * a.x - (double)b.x * a.y;
-                    aq += sqrt(qr * qr + qi * qi);
+                    // |q| for the bound's sum|q| in single precision: its
+                    // relative error (~1e-7) is far inside the bound's
+                    // factor 1.5
+                    aq += (double)sqrtf((float)(qr * qr + qi * qi));
                     const double2 e = txb[x], w = txs[x];
                     double zr = qr * e.x - qi * e.y, zi = qr * e.y + qi * e.x;
 #pragma unroll
