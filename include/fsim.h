@@ -218,7 +218,7 @@ int fsim_get_peak(fsim_vec band0, fsim_vec band1, int bn0, int bn1, fsim_otf otf
  * correlation and locatePeak (exclusion fit_exclude * cutoff / vec_cpm),
  * fitPeak(band 0, fit band, -peak, weight limit 0.05, search 2.5) and getPeak
  * of band 1 (and band 2 for three bands) — the same arithmetic as the
- * per-call entry points above, in about 35 stream-ordered launches for the
+ * per-call entry points above, in about 40 stream-ordered launches for the
  * whole batch and one synchronisation. */
 typedef struct {
     int n;              /* image side; square, a power of two              */
