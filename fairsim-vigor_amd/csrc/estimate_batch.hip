@@ -443,14 +443,24 @@ __global__ void __launch_bounds__(256) k_eb_region_rows(const float2 *__restrict
 #pragma unroll
         for (int m = 0; m < E; m++) {
             const int x = t + Tn * m;
-            const float2 wt = otf_pixel(tab, a.ns, a.cpm, a.cutoff, x, row, L, L, sx, sy, a.vec_cpm);
             // ring: band 0 at its own pixel, band 1 at its source pixel
             // ((x + ikx) mod n, (y - iky) mod n)
             const int xq = band ? ((x + a.ikx) % L + L) % L : x, yq = band ? ((row - a.iky) % L + L) % L : row;
             const int dx = xq < L / 2 ? xq : xq - L, dy = yq < L / 2 ? yq : yq - L, d2 = dx * dx + dy * dy;
             const bool cut = ring_ok ? (d2 < ring_lo || d2 > ring_hi)
                                      : (ring_ratio(d2, kmax) < a.dist || ring_ratio(d2, kmax) > 1 - a.dist);
-            v[m] = region_dc(src[x], dc[x], wt, cut, a.wlim);
+            // a pixel in the ring's cut or outside the DC weight's support is
+            // zero whatever the shifted weight: the weight (FP64 radius and
+            // table lookup) and the band value are formed only where they
+            // matter (a wavefront's lanes hold 64 adjacent pixels, so whole
+            // runs outside the common region skip it)
+            const float2 dcv = dc[x];
+            float2 out = make_float2(0.f, 0.f);
+            if (!cut && dcv.x == dcv.x) {
+                const float2 wt = otf_pixel(tab, a.ns, a.cpm, a.cutoff, x, row, L, L, sx, sy, a.vec_cpm);
+                out = region_dc(src[x], dcv, wt, false, a.wlim);
+            }
+            v[m] = out;
         }
     }
     fft_regs<L, E, true, RowWave<L>::value, TW::tl>(v, lds + rr * LP, 1, t, tw, active);
