@@ -1019,6 +1019,10 @@ __global__ void k_wiener_multi(float *__restrict__ wa, const float *__restrict__
                                double vec_cpm, int use_att, float w2) {
 #pragma clang fp contract(off)
     const size_t mm = (size_t)M * M;
+    // squared distance beyond which every OTF term is zero (with margin);
+    // a cutoff or scale that makes it not finite disables the skip
+    double far2 = (cutoff / vec_cpm) * (cutoff / vec_cpm) * (1 + 1e-9);
+    if (!(far2 < __builtin_inf())) far2 = __builtin_inf();
     const int st = blockIdx.y;
     // read in place (uniform loads): a local copy indexed by the direction
     // loop lives in scratch memory
@@ -1027,32 +1031,35 @@ __global__ void k_wiener_multi(float *__restrict__ wa, const float *__restrict__
         int x, y;
         tile_major_xy(i, M, cw, x, y);
         const double xh = xh_of(x, M), yh = yh_of(y, M);
+        // |OTF|^2 (times the attenuation) at squared distance q from a band
+        // centre: zero beyond the cutoff.  q > far2 puts the radius
+        // sqrt(q) * vec_cpm above the cutoff by a relative 5e-10, far more
+        // than the roundings of the exact path, which would return +0 there
+        // too; so those terms skip the FP64 square root, division and table
+        // lookups (most of the 2n x 2n grid lies outside a shifted band's
+        // support).
+        auto term = [&](double q, size_t tb) -> float {
+            float v = 0.f;
+            if (q <= far2) {
+                const double r = sqrt(q) * vec_cpm;
+                const float2 o = otf_val(vals + tb, ns, cpm, cutoff, r);
+                v = o.x * o.x + o.y * o.y;
+                if (use_att) v *= att_val(only_att + tb, cpm, cutoff, r);
+            }
+            return v;
+        };
         // band 0 (px(0) = 0): both distances are |(xh, yh)| in every
         // direction, so one value serves all of them (the same operands as
         // wiener_sum's, hence the same float)
-        const double r0 = sqrt(xh * xh + yh * yh) * vec_cpm;
-        float v0 = 0.f;
-        {
-            const float2 o = otf_val(vals, ns, cpm, cutoff, r0);
-            v0 = o.x * o.x + o.y * o.y;
-            if (use_att) v0 *= att_val(only_att, cpm, cutoff, r0);
-        }
+        const float v0 = term(xh * xh + yh * yh, 0);
         float acc = 0.f;
         for (int d = 0; d < S.ndir; d++) {
             acc = acc + v0 + v0;
             for (int b = 1; b < S.nband; b++) {
                 const double bx = S.px[d] * b, by = S.py[d] * b;
-                const double r1 = sqrt((xh - bx) * (xh - bx) + (yh - by) * (yh - by)) * vec_cpm;
-                const double r2 = sqrt((xh + bx) * (xh + bx) + (yh + by) * (yh + by)) * vec_cpm;
                 const size_t tb = (size_t)(otf_nb > 1 ? b : 0) * ns;
-                const float2 o1 = otf_val(vals + tb, ns, cpm, cutoff, r1);
-                const float2 o2 = otf_val(vals + tb, ns, cpm, cutoff, r2);
-                float v1 = o1.x * o1.x + o1.y * o1.y;
-                float v2 = o2.x * o2.x + o2.y * o2.y;
-                if (use_att) {
-                    v1 *= att_val(only_att + tb, cpm, cutoff, r1);
-                    v2 *= att_val(only_att + tb, cpm, cutoff, r2);
-                }
+                const float v1 = term((xh - bx) * (xh - bx) + (yh - by) * (yh - by), tb);
+                const float v2 = term((xh + bx) * (xh + bx) + (yh + by) * (yh + by), tb);
                 acc = acc + v1 + v2;
             }
         }
