@@ -52,6 +52,53 @@ const float2 *twiddles(int L) {
     return d;
 }
 
+// Counted device allocations of the library's own state (channel tables,
+// work buffers): their bytes are part of fsim_device_bytes, and a test hook
+// (fsim_debug_fail_alloc) can make the k-th of them fail to exercise the
+// error paths.
+static std::map<void *, size_t> g_owned;
+static int g_fail_after = -1;
+
+int dev_alloc(void **p, size_t bytes) {
+    *p = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (g_fail_after == 0) {
+            g_fail_after = -1;
+            return FSIM_ERR_OOM;
+        }
+        if (g_fail_after > 0) g_fail_after--;
+    }
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return FSIM_ERR_OOM;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_owned[*p] = bytes;
+    g_bytes += (int64_t)bytes;
+    return FSIM_OK;
+}
+
+void dev_free(void *p) {
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_owned.find(p);
+        if (it != g_owned.end()) {
+            g_bytes -= (int64_t)it->second;
+            g_owned.erase(it);
+        }
+    }
+    (void)hipFree(p);
+}
+
+const float2 *twiddles_cached(int L) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tw.find(L);
+    return it == g_tw.end() ? nullptr : it->second;
+}
+
 std::mutex &scratch_mutex() {
     static std::mutex m;
     return m;
@@ -112,6 +159,12 @@ int fsim_sync(void) {
 }
 
 int64_t fsim_device_bytes(void) { return g_bytes.load(); }
+
+int fsim_debug_fail_alloc(int nth) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_fail_after = nth < 0 ? -1 : nth;
+    return FSIM_OK;
+}
 
 int fsim_vec_create(int is_cplx, int w, int h, fsim_vec *out) {
     int rc = ensure_init();

@@ -34,6 +34,18 @@ hipStream_t stream();            // the library's ordered stream
 int ensure_init();
 const float2 *twiddles(int L);   // exp(-2 pi i m / L), m < L, device resident
 int scratch_d(size_t bytes, void **p);   // growing device scratch (reductions, staging)
+// counted device allocations (fsim_device_bytes; fsim_debug_fail_alloc)
+int dev_alloc(void **p, size_t bytes);
+void dev_free(void *p);
+template <class T>
+inline int dev_alloc_n(T **p, size_t n) {
+    void *v = nullptr;
+    const int rc = dev_alloc(&v, n * sizeof(T));
+    *p = static_cast<T *>(v);
+    return rc;
+}
+// a twiddle table already created by twiddles(L), or nullptr (never allocates)
+const float2 *twiddles_cached(int L);
 std::mutex &scratch_mutex();            // held by every user of scratch_d for its whole use
 
 // ---- FFT entry (fft.hip) -------------------------------------------------

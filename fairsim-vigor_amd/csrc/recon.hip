@@ -34,6 +34,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <map>
 #include <set>
 #include <utility>
 #include <vector>
@@ -1314,6 +1315,17 @@ int build_channel_const(const fsim_recon_params &p, const Otf &o, bool sym, Reco
 
 using namespace fsim;
 
+// Work buffers of one stream's calls on a channel: A/B for `cap` stacks and
+// the per-parameter state of fsim_reconstruct_params.  Each stream that
+// calls into a channel gets its own, so calls on different streams (the
+// reference's concurrent ReconstructionThreads, each with its own vectors,
+// ReconstructionRunner.java:243-251, 398-413) never share intermediates.
+struct Workspace {
+    float2 *A = nullptr, *B = nullptr;
+    int cap = 0;
+    MultiWork mw;
+};
+
 struct fsim_channel_s {
     fsim_recon_params p;
     Otf o;
@@ -1325,24 +1337,39 @@ struct fsim_channel_s {
     float2 *otf2d = nullptr;
     float2 *ramp = nullptr;
     double *fade = nullptr;
-    float2 *A = nullptr, *B = nullptr;
-    int cap = 0;
-    MultiWork mw;
+    std::mutex ws_mu;
+    std::map<hipStream_t, Workspace *> ws;
 };
 
 const ReconConst *fsim::channel_device_const(const fsim_channel_s *ch) { return ch->dK; }
 const ReconConst &fsim::channel_host_const(const fsim_channel_s *ch) { return ch->K; }
 
+static void workspace_release(Workspace *w) {
+    dev_free(w->A);
+    dev_free(w->B);
+    w->A = w->B = nullptr;
+    w->cap = 0;
+    multiwork_release(w->mw);
+}
+
+// Frees every device buffer of the channel (tables and workspaces); the
+// caller has synchronised the work that reads them.
 static void channel_free(fsim_channel_s *ch) {
-    (void)hipFree(ch->dK);
-    (void)hipFree(ch->wa);
-    (void)hipFree(ch->apo);
-    (void)hipFree(ch->otf2d);
-    (void)hipFree(ch->ramp);
-    (void)hipFree(ch->fade);
-    (void)hipFree(ch->A);
-    (void)hipFree(ch->B);
-    multiwork_release(ch->mw);
+    dev_free(ch->dK);
+    dev_free(ch->wa);
+    dev_free(ch->apo);
+    dev_free(ch->otf2d);
+    dev_free(ch->ramp);
+    dev_free(ch->fade);
+    ch->dK = nullptr;
+    ch->wa = ch->apo = nullptr;
+    ch->otf2d = ch->ramp = nullptr;
+    ch->fade = nullptr;
+    for (auto &kv : ch->ws) {
+        workspace_release(kv.second);
+        delete kv.second;
+    }
+    ch->ws.clear();
 }
 
 // ramp tables of a set of stacks' constants: fx/fy of each direction and
@@ -1365,23 +1392,32 @@ static int fill_ramps(float2 *ramp, double *fdev, std::vector<double> &hf, const
     return FSIM_OK;
 }
 
-// A/B workspaces for `count` stacks
-static int ensure_workspace(fsim_channel_s *ch, int count, hipStream_t s) {
-    if (count <= ch->cap) return FSIM_OK;
+// The calling stream's A/B workspace, grown to `count` stacks.  Growing
+// waits for that stream's earlier work (the only work that reads the old
+// buffers).
+static int ensure_workspace(fsim_channel_s *ch, int count, hipStream_t s, Workspace **out) {
+    Workspace *w;
+    {
+        std::lock_guard<std::mutex> lk(ch->ws_mu);
+        auto it = ch->ws.find(s);
+        if (it == ch->ws.end()) it = ch->ws.emplace(s, new Workspace()).first;
+        w = it->second;
+    }
+    *out = w;
+    if (count <= w->cap) return FSIM_OK;
     const ReconConst &K = ch->K;
     FSIM_HIP(hipStreamSynchronize(s));
-    (void)hipFree(ch->A);
-    (void)hipFree(ch->B);
-    ch->A = ch->B = nullptr;
-    ch->cap = 0;
-    if (hipMalloc(&ch->A, workspace_a_elems(K.n, K.D, K.Cs) * count * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&ch->B, workspace_b_elems(K.n) * count * sizeof(float2)) != hipSuccess) {
-        (void)hipFree(ch->A);
-        ch->A = nullptr;
-        (void)hipGetLastError();
+    dev_free(w->A);
+    dev_free(w->B);
+    w->A = w->B = nullptr;
+    w->cap = 0;
+    if (dev_alloc_n(&w->A, workspace_a_elems(K.n, K.D, K.Cs) * count) ||
+        dev_alloc_n(&w->B, workspace_b_elems(K.n) * count)) {
+        dev_free(w->A);
+        w->A = nullptr;
         return fail(FSIM_ERR_OOM, "reconstruct: workspace allocation failed");
     }
-    ch->cap = count;
+    w->cap = count;
     return FSIM_OK;
 }
 
@@ -1521,6 +1557,70 @@ int reconstruct_params_with(const fsim_channel_s *ch, MultiWork &w, float2 *A, f
 
 }  // namespace fsim
 
+// Device temporaries of the channel set-up, freed on every exit.
+struct ChannelTmp {
+    float *wtmp = nullptr, *atmp = nullptr;
+    double *fdev = nullptr;
+    ~ChannelTmp() {
+        dev_free(wtmp);
+        dev_free(atmp);
+        dev_free(fdev);
+    }
+};
+
+// Tables of a channel: the FilterUpdateThread work of
+// ReconstructionRunner.java:773-828 on the device.  On failure the caller
+// frees what was allocated (channel_free).
+static int channel_setup(fsim_channel_s *ch, const fsim_recon_params *p, const Otf &o, ChannelTmp &tmp) {
+    const int n = p->n, M = 2 * n;
+    const size_t mm = (size_t)M * M;
+    if (dev_alloc_n(&ch->dK, 1) || dev_alloc_n(&ch->wa, mm) || dev_alloc_n(&ch->apo, mm) ||
+        dev_alloc_n(&ch->otf2d, (size_t)o.nband * n * n) || dev_alloc_n(&ch->fade, 2 * (size_t)n) ||
+        dev_alloc_n(&ch->ramp, (size_t)kMaxDir * kMaxComp * 2 * n) ||
+        dev_alloc_n(&tmp.fdev, (size_t)kMaxDir * kMaxComp * 2) || dev_alloc_n(&tmp.wtmp, mm) ||
+        dev_alloc_n(&tmp.atmp, mm))
+        return fail(FSIM_ERR_OOM, "channel_create: device allocation failed");
+    // the transforms' twiddle tables exist before the first reconstruction
+    if (!twiddles(n) || !twiddles(M)) return fail(FSIM_ERR_OOM, "channel_create: twiddle table allocation failed");
+    std::vector<double> f(2 * n);
+    fade_factors_host(n, p->fade_px, f.data(), f.data() + n);
+    hipStream_t s = stream();
+    FSIM_HIP(hipMemcpyAsync(ch->fade, f.data(), 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
+    ch->K.fade_lo = ch->fade;
+    ch->K.fade_hi = ch->fade + n;
+    ch->K.fade_px = p->fade_px;
+    ch->K.wa = ch->wa;
+    std::vector<double> hf;
+    int rc = fill_ramps(ch->ramp, tmp.fdev, hf, &ch->K, 1, n, s);
+    if (rc) return rc;
+    ch->K.ramp = ch->ramp;
+    // otfToVector(band, 0, 0, write) per OTF band on the n^2 grid
+    const size_t nn = (size_t)n * n;
+    const float2 *tab = p->use_att ? o.vals_att : o.vals;
+    for (int b = 0; b < o.nband; b++) {
+        rc = otf_table_device(ch->otf2d + b * nn, n, n, tab + (size_t)b * o.ns, o.ns, o.cpm, o.cutoff, p->vec_cpm,
+                              s);
+        if (rc) return rc;
+    }
+    ch->K.otf2d = ch->otf2d;
+    ShiftSet sh{};
+    sh.ndir = p->ndir;
+    sh.nband = p->nband;
+    for (int d = 0; d < p->ndir; d++) {
+        sh.px[d] = p->px[d];
+        sh.py[d] = p->py[d];
+    }
+    rc = wiener_device(tmp.wtmp, M, M, o, p->vec_cpm, sh, p->use_att, p->wiener, s);
+    if (rc) return rc;
+    rc = write_apo_device(tmp.atmp, M, M, p->vec_cpm, o.cutoff, p->apo_bend, p->apo_cut, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pack_wa, dim3(1024), dim3(256), 0, s, ch->wa, ch->apo, tmp.wtmp, tmp.atmp, M, gather_cw(n));
+    FSIM_HIP(hipGetLastError());
+    FSIM_HIP(hipMemcpyAsync(ch->dK, &ch->K, sizeof(ReconConst), hipMemcpyHostToDevice, s));
+    FSIM_HIP(hipStreamSynchronize(s));
+    return FSIM_OK;
+}
+
 extern "C" {
 
 int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *out) {
@@ -1528,8 +1628,6 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     if (rc) return rc;
     if (!p || !oh || !out) return fail(FSIM_ERR_ARG, "channel_create: null argument");
     const Otf &o = oh->o;
-    fsim_channel_s *ch = new fsim_channel_s();
-    ch->p = *p;
     // Hermitian pairing needs a real OTF (then conj(OTF) is symmetric in k)
     bool sym = p->mode == 0;
     if (sym) {
@@ -1539,6 +1637,8 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
         for (const float2 &z : hv)
             if (z.y != 0.f) sym = false;
     }
+    fsim_channel_s *ch = new fsim_channel_s();
+    ch->p = *p;
     rc = build_channel_const(*p, o, sym, ch->K);
     if (rc) {
         delete ch;
@@ -1546,69 +1646,20 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     }
     ch->o = o;
     ch->sym = sym;
-    const int n = p->n, M = 2 * n;
-    const size_t mm = (size_t)M * M;
-    float *wtmp = nullptr, *atmp = nullptr;
-    double *fdev = nullptr;
-    if (hipMalloc(&ch->dK, sizeof(ReconConst)) != hipSuccess || hipMalloc(&ch->wa, mm * sizeof(float)) != hipSuccess ||
-        hipMalloc(&ch->apo, mm * sizeof(float)) != hipSuccess ||
-        hipMalloc(&ch->otf2d, (size_t)o.nband * n * n * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&ch->fade, 2 * n * sizeof(double)) != hipSuccess ||
-        hipMalloc(&ch->ramp, (size_t)kMaxDir * kMaxComp * 2 * n * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&fdev, (size_t)kMaxDir * kMaxComp * 2 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&wtmp, mm * sizeof(float)) != hipSuccess || hipMalloc(&atmp, mm * sizeof(float)) != hipSuccess) {
-        (void)hipFree(wtmp);
-        (void)hipFree(atmp);
-        (void)hipFree(fdev);
+    {
+        ChannelTmp tmp;
+        rc = channel_setup(ch, p, o, tmp);
+        if (rc) {
+            // the set-up's launches may still read the buffers
+            (void)hipStreamSynchronize(stream());
+            (void)hipGetLastError();
+        }
+    }
+    if (rc) {
         channel_free(ch);
         delete ch;
-        return fail(FSIM_ERR_OOM, "channel_create: device allocation failed");
+        return rc;
     }
-    std::vector<double> f(2 * n);
-    fade_factors_host(n, p->fade_px, f.data(), f.data() + n);
-    hipStream_t s = stream();
-    FSIM_HIP(hipMemcpyAsync(ch->fade, f.data(), 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
-    ch->K.fade_lo = ch->fade;
-    ch->K.fade_hi = ch->fade + n;
-    ch->K.fade_px = p->fade_px;
-    ch->K.wa = ch->wa;
-    {
-        std::vector<double> hf;
-        rc = fill_ramps(ch->ramp, fdev, hf, &ch->K, 1, n, s);
-        if (rc) return rc;
-        FSIM_HIP(hipStreamSynchronize(s));
-        ch->K.ramp = ch->ramp;
-    }
-    {
-        // otfToVector(band, 0, 0, write) per OTF band on the n^2 grid
-        const int nb = o.nband;
-        const size_t nn = (size_t)n * n;
-        const float2 *tab = p->use_att ? o.vals_att : o.vals;
-        for (int b = 0; b < nb; b++) {
-            rc = otf_table_device(ch->otf2d + b * nn, n, n, tab + (size_t)b * o.ns, o.ns, o.cpm, o.cutoff,
-                                  p->vec_cpm, s);
-            if (rc) return rc;
-        }
-        ch->K.otf2d = ch->otf2d;
-    }
-    ShiftSet sh{};
-    sh.ndir = p->ndir;
-    sh.nband = p->nband;
-    for (int d = 0; d < p->ndir; d++) {
-        sh.px[d] = p->px[d];
-        sh.py[d] = p->py[d];
-    }
-    rc = wiener_device(wtmp, M, M, o, p->vec_cpm, sh, p->use_att, p->wiener, s);
-    if (rc) return rc;
-    rc = write_apo_device(atmp, M, M, p->vec_cpm, o.cutoff, p->apo_bend, p->apo_cut, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_pack_wa, dim3(1024), dim3(256), 0, s, ch->wa, ch->apo, wtmp, atmp, M, gather_cw(n));
-    FSIM_HIP(hipGetLastError());
-    FSIM_HIP(hipMemcpyAsync(ch->dK, &ch->K, sizeof(ReconConst), hipMemcpyHostToDevice, s));
-    FSIM_HIP(hipStreamSynchronize(s));
-    (void)hipFree(wtmp);
-    (void)hipFree(atmp);
-    (void)hipFree(fdev);
     *out = ch;
     return FSIM_OK;
 }
@@ -1626,7 +1677,8 @@ int fsim_channel_class(fsim_channel ch, int *n, int *D, int *P, int *Cs, int *ha
 
 int fsim_channel_destroy(fsim_channel ch) {
     if (!ch) return FSIM_OK;
-    FSIM_HIP(hipStreamSynchronize(stream()));
+    // calls on any stream may still read the channel's tables and workspaces
+    FSIM_HIP(hipDeviceSynchronize());
     channel_free(ch);
     delete ch;
     return FSIM_OK;
@@ -1639,14 +1691,15 @@ int fsim_reconstruct(fsim_channel ch, const uint16_t *raw, float *out, int count
     const ReconConst &K = ch->K;
     const int n = K.n;
     hipStream_t s = strm ? (hipStream_t)strm : stream();
-    rc = ensure_workspace(ch, count, s);
+    Workspace *w = nullptr;
+    rc = ensure_workspace(ch, count, s, &w);
     if (rc) return rc;
     JobList jl{};
     jl.jobs = nullptr;
     jl.base.c = ch->dK;
     jl.base.raw = raw;
-    jl.base.A = ch->A;
-    jl.base.B = ch->B;
+    jl.base.A = w->A;
+    jl.base.B = w->B;
     jl.base.out = out;
     jl.raw_stride = (size_t)K.D * K.P * n * n;
     jl.a_stride = workspace_a_elems(n, K.D, K.Cs);
@@ -1666,9 +1719,10 @@ int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw, float *out, in
     if (!ch || !raw || !out || !sp || count < 1) return fail(FSIM_ERR_ARG, "reconstruct_params: bad arguments");
     if (count > 65535) return fail(FSIM_ERR_ARG, "reconstruct_params: at most 65535 stacks per call");
     hipStream_t s = strm ? (hipStream_t)strm : stream();
-    rc = ensure_workspace(ch, count, s);
+    Workspace *w = nullptr;
+    rc = ensure_workspace(ch, count, s, &w);
     if (rc) return rc;
-    return reconstruct_params_with(ch, ch->mw, ch->A, ch->B, raw, out, count, sp, 1, s);
+    return reconstruct_params_with(ch, w->mw, w->A, w->B, raw, out, count, sp, 1, s);
 }
 
 int fsim_stage_timing(int enable) {

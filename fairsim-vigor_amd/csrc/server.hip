@@ -273,6 +273,20 @@ struct Server {
             const ClassKey k = key_of(pending.front());
             Slot *s = acquire_slot(k);
             if (!s) {
+                if (inflight.empty()) {
+                    // nothing on the device will free a slot or memory: the
+                    // slot could not be allocated, so the request fails
+                    // (the reference turns a failed allocation into
+                    // OutOfMemoryError, AccelVectorCplx.java:45-54)
+                    Request *r = pending.front();
+                    pending.pop_front();
+                    r->rc = FSIM_ERR_OOM;
+                    r->state = 3;
+                    r->t_done = now_ms();
+                    failed++;
+                    cv_done.notify_all();
+                    continue;
+                }
                 cv_work.wait(lk);  // a slot completes and notifies
                 continue;
             }
@@ -399,6 +413,17 @@ int fsim_server_submit(fsim_server h, int chan, const uint16_t *raw, float *out,
         s.rejected++;
         return fail(FSIM_ERR_BUSY, "server_submit: queue full, request refused");
     }
+    {
+        // a request whose slot can never fit the budget is refused now (it
+        // would otherwise wait at the head of the queue for memory that no
+        // completion frees)
+        Request probe;
+        probe.chan = chan;
+        if (slot_bytes(s.key_of(&probe), 1) > (size_t)s.cfg.mem_budget) {
+            s.rejected++;
+            return fail(FSIM_ERR_BUSY, "server_submit: request larger than the memory budget");
+        }
+    }
     Request *r = new Request();
     r->ticket = s.next_ticket++;
     r->chan = chan;
@@ -468,6 +493,7 @@ int fsim_server_wait(fsim_server h, uint64_t ticket, int timeout_ms, double *lat
     const int rc = r->rc;
     s.reqs.erase(it);
     delete r;
+    if (rc == FSIM_ERR_OOM) return fail(rc, "server_wait: no device memory for the request");
     if (rc) return fail(rc, "server_wait: reconstruction failed on the device");
     return FSIM_OK;
 }

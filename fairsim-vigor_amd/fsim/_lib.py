@@ -212,6 +212,7 @@ _SIGS = {
     "fsim_ingest_result": [_P, _I, _P, _P],
     "fsim_stage_timing": [_I],
     "fsim_stage_times": [_PD, _PI, _I],
+    "fsim_debug_fail_alloc": [_I],
 }
 _RET = {
     "fsim_last_error": (C.c_char_p, []),

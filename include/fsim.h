@@ -45,8 +45,14 @@ const char *fsim_last_error(void);
  *   accel/cuda_mgmt.cu:38-44) */
 int fsim_init(int device);
 int fsim_sync(void);
-/* Bytes of device memory currently held by vectors and plans. */
+/* Bytes of device memory currently held by vectors, raw buffers, channels
+ * (tables and work buffers). */
 int64_t fsim_device_bytes(void);
+/* Test hook, no counterpart in the reference: the nth (0-based) channel or
+ * work-buffer device allocation from now on fails as if the device were out
+ * of memory (FSIM_ERR_OOM); nth < 0 switches the hook off.  For tests of the
+ * error paths (every failure frees what the call had allocated). */
+int fsim_debug_fail_alloc(int nth);
 
 /* Raw buffers: AccelVectorFactory.nativeAllocMemory / nativeAllocMemoryHost
  * (AccelVectorFactory.java:141-142; accel/cuda_mgmt.cu:46-60).  Host buffers
@@ -289,9 +295,15 @@ int fsim_channel_destroy(fsim_channel ch);
 int fsim_channel_class(fsim_channel ch, int *n, int *D, int *P, int *Cs, int *half);
 
 /* Reconstruct `count` stacks of one channel.  raw_dev: count stacks back to
- * back in device memory; out_dev: count images of (2n)^2 float.  Work buffers
- * are held by the channel and grow to the largest count seen.  `stream` is a
- * hipStream_t or NULL for the library stream.  Asynchronous. */
+ * back in device memory; out_dev: count images of (2n)^2 float.  `stream` is
+ * a hipStream_t or NULL for the library stream.  Asynchronous.  Work buffers
+ * are held by the channel per stream and grow to the largest count seen on
+ * that stream, so calls on different streams (one per thread, as the
+ * reference's ReconstructionThreads each own their vectors,
+ * livemode/ReconstructionRunner.java:243-251, 398-413) run concurrently;
+ * calls on one stream from several threads must be serialised by the caller.
+ * Replaces, fused, the per-frame op sequence of
+ * ReconstructionRunner.ReconstructionThread.run (:415-594). */
 int fsim_reconstruct(fsim_channel ch, const uint16_t *raw_dev, float *out_dev,
                      int count, void *stream);
 
@@ -374,7 +386,11 @@ int fsim_register_images(fsim_registration r, int widefield, const float *src_de
  * batches run join the next free slot.  Channels stay owned by the caller and
  * must outlive the server.  Host buffers are copied asynchronously: pass
  * pinned memory (fsim_alloc_host) for overlap.  FSIM_SUBMIT_DEVICE: raw/out
- * are device pointers.  Results are bitwise independent of batch composition. */
+ * are device pointers.  Results are bitwise independent of batch composition.
+ * A request whose slot alone exceeds mem_budget is refused at submit
+ * (FSIM_ERR_BUSY); one whose slot cannot be allocated while nothing else is
+ * on the device fails (wait returns FSIM_ERR_OOM) instead of blocking the
+ * queue. */
 typedef struct fsim_server_s *fsim_server;
 typedef struct {
     int64_t mem_budget;  /* device bytes for slots (work buffers, staging) */

@@ -334,3 +334,144 @@ def test_multiband_otf_matches_fp64(fsim_lib, band0, tmp_path):
     param.otf(same)
     assert np.array_equal(a, Reconstructor(param, otf_band0=band0).reconstruct_host(raw))
     assert fp64.rel_l2(a, got) < 1e-3
+
+
+def test_complex_otf_takes_all_components(fsim_lib):
+    """A measured three-band OTF with non-zero imaginary parts (as an OTF
+    file may hold them, OtfProvider.java:592-628): conj(OTF) is then not
+    symmetric, so the channel stores every component (no Hermitian pairing,
+    full-width result) even when the pairing is asked for, and the image
+    matches the FP64 re-run with the same complex tables."""
+    import ctypes as C
+    from fsim import synthetic
+    from fsim._lib import call
+    from fsim.sim_algorithm import OtfProvider, Reconstructor
+    n = 128
+    raw, sp = synthetic.sim_stack(n=n, nphase=5, seed=14, nbeads=75, nfil=10, harmonic2=0.3, k_frac=0.45)
+    ests = [OtfProvider.fromEstimate(1.4, 525.0, a) for a in (1.0, 0.7, 0.45)]
+    ns = ests[0].band_values().shape[1]
+    r = np.arange(ns) / ns
+    # a radial phase per band (a defocus-like aberration), |OTF| unchanged
+    phase = np.stack([0.6 * r ** 2, -0.4 * r, 0.9 * r ** 3])
+    vals = (np.stack([e.band_values()[0] for e in ests]) * np.exp(1j * phase)).astype(np.complex64)
+    assert np.abs(vals.imag).max() > 0.05
+    otf = OtfProvider.fromData(1.4, 525.0, ests[0].cpm, vals)
+    otf.setAttenuation(0.995, 1.2)
+    otf.switchAttenuation(True)
+    param = synthetic.make_param(sp, n, nphase=5, nband=3)
+    param.otf(otf)
+    paired = Reconstructor(param, mode=0)
+    full = Reconstructor(param, mode=1)
+    cls = [C.c_int() for _ in range(5)]
+    call("fsim_channel_class", paired.handle, *[C.byref(c) for c in cls])
+    n_, D, P, Cs, half = [c.value for c in cls]
+    assert (n_, D, P, Cs, half) == (n, 3, 5, 5, 0)
+    got = paired.reconstruct_host(raw)
+    assert np.array_equal(got, full.reconstruct_host(raw))
+    p = ref.recon_dict(sp, n, nband=3)
+    p["px"] = [param.dir(d).pX for d in range(3)]
+    p["py"] = [param.dir(d).pY for d in range(3)]
+    want = fp64.recon_fast(p, raw, vals.astype(np.complex128))
+    real_only = fp64.recon_fast(p, raw, vals.real.astype(np.complex128))
+    e = fp64.rel_l2(got, want)
+    print(f"complex OTF: rel L2 vs FP64 {e:.2e}; the real part alone differs by {fp64.rel_l2(real_only, want):.2e}")
+    assert e < TOL and fp64.rel_l2(real_only, want) > 1e-3
+
+
+def test_channel_create_frees_on_allocation_failure(fsim_lib):
+    """Every device allocation of the channel set-up and of the work buffers
+    is made to fail in turn (fsim_debug_fail_alloc): each failure raises
+    MemoryError (the reference's OutOfMemoryError) and leaves the library's
+    device bytes where they were."""
+    from fsim import synthetic
+    from fsim._lib import lib
+    from fsim.linalg import DeviceBuffer
+    from fsim.sim_algorithm import Reconstructor
+    L = lib()
+    n = 128
+    raw, sp = synthetic.sim_stack(n=n, seed=15, nbeads=60, nfil=8)
+    param = synthetic.make_param(sp, n)
+    ok = Reconstructor(param)          # OTF tables and twiddles exist from here on
+    want = ok.reconstruct_host(raw)
+    del ok
+    base = L.fsim_device_bytes()
+    try:
+        for k in range(9):             # the set-up's nine device buffers
+            L.fsim_debug_fail_alloc(k)
+            with pytest.raises(MemoryError):
+                Reconstructor(param)
+            assert L.fsim_device_bytes() == base, f"allocation {k}"
+        L.fsim_debug_fail_alloc(-1)
+        rec = Reconstructor(param)
+        held = L.fsim_device_bytes()
+        assert held > base
+        rb = DeviceBuffer(raw.nbytes)
+        ob = DeviceBuffer(rec.out_bytes())
+        rb.upload(raw)
+        held = L.fsim_device_bytes()
+        for k in range(2):             # the work buffers A, B
+            L.fsim_debug_fail_alloc(k)
+            with pytest.raises(MemoryError):
+                rec.reconstruct(rb.value, ob.value, 1)
+            assert L.fsim_device_bytes() == held
+        L.fsim_debug_fail_alloc(-1)
+        assert np.array_equal(rec.reconstruct_host(raw), want)
+        rb.free()
+        ob.free()
+        del rec
+        assert L.fsim_device_bytes() == base
+    finally:
+        L.fsim_debug_fail_alloc(-1)
+
+
+def _hip():
+    import ctypes as C
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            h = C.CDLL(name)
+            break
+        except OSError:
+            h = None
+    assert h is not None, "HIP runtime library not found"
+    h.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    h.hipStreamSynchronize.argtypes = [C.c_void_p]
+    h.hipStreamDestroy.argtypes = [C.c_void_p]
+    return h
+
+
+def test_streams_have_own_workspaces(fsim_lib):
+    """Two streams reconstructing on one channel at the same time (the
+    reference's concurrent ReconstructionThreads) each use their own work
+    buffers: every image is bitwise the one of a call alone."""
+    import ctypes as C
+    from fsim import synthetic
+    from fsim.linalg import DeviceBuffer
+    from fsim.sim_algorithm import Reconstructor
+    n = 256
+    raws = []
+    for s in range(4):
+        raw, sp = synthetic.sim_stack(n=n, seed=110 + s, nbeads=150, nfil=20)
+        raws.append(raw)
+    rec = Reconstructor(synthetic.make_param(sp, n))
+    want = [rec.reconstruct_host(r) for r in raws]
+    hip = _hip()
+    streams = [C.c_void_p(), C.c_void_p()]
+    for st in streams:
+        assert hip.hipStreamCreate(C.byref(st)) == 0
+    try:
+        ins = [DeviceBuffer(2 * raws[0].nbytes) for _ in range(2)]
+        outs = [DeviceBuffer(2 * rec.out_bytes()) for _ in range(2)]
+        ins[0].upload(np.stack(raws[:2]))
+        ins[1].upload(np.stack(raws[2:]))
+        for _ in range(3):
+            for i in range(2):
+                rec.reconstruct(ins[i].value, outs[i].value, 2, stream=streams[i].value)
+        for st in streams:
+            assert hip.hipStreamSynchronize(st) == 0
+        got = [outs[i].download((2, 2 * n, 2 * n), np.float32) for i in range(2)]
+        for i in range(2):
+            for j in range(2):
+                assert np.array_equal(got[i][j], want[2 * i + j]), f"stream {i} stack {j}"
+    finally:
+        for st in streams:
+            hip.hipStreamDestroy(st)

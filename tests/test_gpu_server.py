@@ -217,3 +217,25 @@ def test_reconstruct_params_shares_tables_of_equal_sets(setup):
         call("fsim_sync")
         assert np.array_equal(o1.download((2 * n, 2 * n), np.float32), got[t])
     assert not np.array_equal(got[0], got[2])
+
+
+def test_request_larger_than_budget_is_refused(setup):
+    """A plain request whose slot alone exceeds the memory budget is refused
+    at submit (FsimBusy) instead of waiting at the head of the queue for
+    memory that no completion frees; smaller requests behind it still run."""
+    from fsim._lib import FsimBusy
+    from fsim.livemode import ReconstructionServer
+    # one 256^2 stack needs about 7.3 MB of slot memory, a 64^2 one 0.5 MB
+    srv = ReconstructionServer(mem_budget=4 << 20, max_batch=4, max_queue=16, nslots=1)
+    raw, rec = setup[(256, 525.0)]
+    big = srv.add_channel(rec)
+    with pytest.raises(FsimBusy):
+        srv.submit(big, raw, np.zeros((512, 512), np.float32))
+    raw64, rec64 = setup[(64, 525.0)]
+    small = srv.add_channel(rec64)
+    o = np.zeros((128, 128), np.float32)
+    srv.wait(srv.submit(small, raw64, o), timeout_ms=60000)
+    assert np.array_equal(o, rec64.reconstruct_host(raw64))
+    st = srv.stats()
+    assert st["rejected"] == 1 and st["completed"] == 1 and st["failed"] == 0
+    srv.close()
