@@ -12,6 +12,17 @@
 
 namespace fsim {
 
+// Work buffers of fsim_reconstruct_noalloc (the A/B intermediates of the
+// fused pipeline), for callers that cannot allocate: one array per GPU,
+// held until the process ends.  Host code never passes `scratch` itself
+// (on the host it is a stand-in, not a device address): fsim_init looks up
+// the device address once, with the device current, and everything uses
+// that pointer.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static constexpr int kMaxDevices = 64;
+static void *g_scratch_dev[kMaxDevices] = {};
+static std::string g_scratch_err = "fsim_init has not been called";
+
 static thread_local std::string t_err;
 static std::mutex g_mu;
 static bool g_init = false;
@@ -93,6 +104,13 @@ void dev_free(void *p) {
     (void)hipFree(p);
 }
 
+void *noalloc_scratch(std::string *why) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    void *p = g_init ? g_scratch_dev[g_device] : nullptr;
+    if (!p && why) *why = g_scratch_err;
+    return p;
+}
+
 const float2 *twiddles_cached(int L) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_tw.find(L);
@@ -140,7 +158,20 @@ int fsim_init(int device) {
     int n = 0;
     FSIM_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(FSIM_ERR_ARG, "fsim_init: no such device");
+    if (device >= kMaxDevices) return fail(FSIM_ERR_ARG, "fsim_init: device index out of range");
     FSIM_HIP(hipSetDevice(device));
+    if (!g_scratch_dev[device]) {
+        // a failed lookup leaves the pointer unset: fsim_reconstruct_noalloc
+        // then refuses with this message, every other entry point works
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            g_scratch_err = std::string("no device address for the work-buffer array: ") + hipGetErrorString(e);
+        } else {
+            g_scratch_dev[device] = ptr;
+        }
+    }
     FSIM_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     g_device = device;
     g_init = true;

@@ -44,6 +44,10 @@ inline int dev_alloc_n(T **p, size_t n) {
     *p = static_cast<T *>(v);
     return rc;
 }
+// device address of the fixed work-buffer array on the bound device
+// (kNoAllocScratchBytes bytes), nullptr before fsim_init
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+void *noalloc_scratch(std::string *why);   // why: the reason when it returns nullptr
 // a twiddle table already created by twiddles(L), or nullptr (never allocates)
 const float2 *twiddles_cached(int L);
 std::mutex &scratch_mutex();            // held by every user of scratch_d for its whole use

@@ -1077,10 +1077,14 @@ struct StageTimer {
 };
 static StageTimer g_timer;
 
+// Set while fsim_reconstruct_noalloc launches: the pipeline then only looks
+// up existing twiddle tables and records no timing events.
+static thread_local bool t_noalloc = false;
+
 bool stage_timing_enabled() { return g_timer.on; }
 
 static void stage_mark(int stage, hipStream_t s, hipEvent_t &start) {
-    if (!g_timer.on) return;
+    if (!g_timer.on || t_noalloc) return;
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
     (void)hipEventRecord(e, s);
@@ -1122,8 +1126,12 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int P, int C, 
                              hipStream_t s) {
     // C: stored components per direction
     constexpr int M = 2 * N;
-    const float2 *twn = twiddles(N), *twm = twiddles(M);
-    if (!twn || !twm) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
+    const float2 *twn = t_noalloc ? twiddles_cached(N) : twiddles(N);
+    const float2 *twm = t_noalloc ? twiddles_cached(M) : twiddles(M);
+    if (!twn || !twm)
+        return fail(t_noalloc ? FSIM_ERR_STATE : FSIM_ERR_OOM,
+                    t_noalloc ? "reconstruct_noalloc: twiddle tables missing (create the channel first)"
+                              : "twiddle table allocation failed");
     hipEvent_t ev = nullptr;
     stage_mark(-1, s, ev);
     {
@@ -1581,6 +1589,7 @@ static int channel_setup(fsim_channel_s *ch, const fsim_recon_params *p, const O
         dev_alloc_n(&tmp.atmp, mm))
         return fail(FSIM_ERR_OOM, "channel_create: device allocation failed");
     // the transforms' twiddle tables exist before the first reconstruction
+    // (fsim_reconstruct_noalloc only looks them up)
     if (!twiddles(n) || !twiddles(M)) return fail(FSIM_ERR_OOM, "channel_create: twiddle table allocation failed");
     std::vector<double> f(2 * n);
     fade_factors_host(n, p->fade_px, f.data(), f.data() + n);
@@ -1710,6 +1719,42 @@ int fsim_reconstruct(fsim_channel ch, const uint16_t *raw, float *out, int count
     // output that is only 4-byte aligned takes the full-width pass
     const bool half = K.half && ((uintptr_t)out & 7) == 0;
     return launch_pipeline(n, jl, count, K.D, K.P, K.Cs, half, s);
+}
+
+int fsim_reconstruct_noalloc(fsim_channel ch, const uint16_t *raw, float *out, int count, void *strm) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!ch || !raw || !out || count < 1) return fail(FSIM_ERR_ARG, "reconstruct_noalloc: bad arguments");
+    const ReconConst &K = ch->K;
+    const int n = K.n;
+    const size_t a_stride = workspace_a_elems(n, K.D, K.Cs), b_stride = workspace_b_elems(n);
+    const size_t need = (a_stride + b_stride) * sizeof(float2) * (size_t)count;
+    if (need > kNoAllocScratchBytes)
+        return fail(FSIM_ERR_ARG, "reconstruct_noalloc: " + std::to_string(count) + " stacks of " +
+                                      std::to_string(n) + "^2 need " + std::to_string(need) +
+                                      " bytes of work buffers, more than the fixed " +
+                                      std::to_string(kNoAllocScratchBytes) + " (use fewer stacks per call)");
+    std::string why;
+    float2 *ptr = static_cast<float2 *>(noalloc_scratch(&why));
+    if (!ptr) return fail(FSIM_ERR_STATE, "reconstruct_noalloc: " + why);
+    hipStream_t s = strm ? (hipStream_t)strm : stream();
+    JobList jl{};
+    jl.jobs = nullptr;
+    jl.base.c = ch->dK;
+    jl.base.raw = raw;
+    jl.base.A = ptr;
+    jl.base.B = ptr + a_stride * (size_t)count;
+    jl.base.out = out;
+    jl.raw_stride = (size_t)K.D * K.P * n * n;
+    jl.a_stride = a_stride;
+    jl.b_stride = b_stride;
+    jl.out_stride = (size_t)4 * n * n;
+    jl.raw_aligned = ((uintptr_t)raw & 7) == 0;
+    const bool half = K.half && ((uintptr_t)out & 7) == 0;
+    t_noalloc = true;
+    rc = launch_pipeline(n, jl, count, K.D, K.P, K.Cs, half, s);
+    t_noalloc = false;
+    return rc;
 }
 
 int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw, float *out, int count, const fsim_sim_params *sp,

@@ -177,6 +177,7 @@ _SIGS = {
     "fsim_channel_destroy": [_P],
     "fsim_reconstruct": [_P, _P, _P, _I, _P],
     "fsim_reconstruct_params": [_P, _P, _P, _I, C.POINTER(SIM_PARAMS), _P],
+    "fsim_reconstruct_noalloc": [_P, _P, _P, _I, _P],
     "fsim_server_create": [C.POINTER(SERVER_CONFIG), C.POINTER(_P)],
     "fsim_server_destroy": [_P],
     "fsim_server_add_channel": [_P, _P, _PI],

@@ -307,6 +307,22 @@ int fsim_channel_class(fsim_channel ch, int *n, int *D, int *P, int *Cs, int *ha
 int fsim_reconstruct(fsim_channel ch, const uint16_t *raw_dev, float *out_dev,
                      int count, void *stream);
 
+/* reconstruct_noalloc: fsim_reconstruct with the work buffers (the A/B
+ * intermediates, 10..16 * (2n)^2 bytes per stack) in a fixed device array of
+ * the library instead of allocated ones.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, hosts that forbid it).  The call allocates
+ * nothing and frees nothing; the array (128 MiB per GPU, held until the
+ * process ends) bounds count: a call whose work buffers exceed it fails with
+ * FSIM_ERR_ARG before anything is launched (at 512^2, up to 6 stacks per
+ * call; at 1024^2, 1; 2048^2 does not fit).  The array is shared by every
+ * call on the GPU: calls that overlap in time (on different streams or
+ * threads) race and must be serialised by the caller.  Nothing in it needs to
+ * start at zero.  The channel must have been created (its tables and the
+ * transform tables exist from then on).  Asynchronous on `stream`. */
+int fsim_reconstruct_noalloc(fsim_channel ch, const uint16_t *raw_dev, float *out_dev,
+                             int count, void *stream);
+
 /* SIM parameters of one stack (SimParam.Dir per direction): band-1 shift
  * pX, pY; the phases used for the separation (getPhases: phases + phaOff);
  * modulations per band (clamped to [0.3, 1.1] as getModulations). */
