@@ -187,12 +187,122 @@ struct FftLds {
     static __device__ __forceinline__ int pad(int i) { return shift ? i + (i >> shift) : i; }
 };
 
+// Radix of the Stockham pass that follows sub-transforms of size NS.
+constexpr int tw_radix(int L, int E, int NS) {
+    return (L / NS) >= (E > 16 ? 16 : E) ? (E > 16 ? 16 : E) : L / NS;
+}
+// Twiddle factors one thread applies in the passes from NS on: every pass
+// after the first has Q = E/R butterflies per thread with R-1 twiddles each.
+constexpr int tw_count_from(int L, int E, int NS) {
+    return NS >= L ? 0
+                   : ((NS > 1 ? (E / tw_radix(L, E, NS)) * (tw_radix(L, E, NS) - 1) : 0) +
+                      tw_count_from(L, E, NS * tw_radix(L, E, NS)));
+}
+
+// A thread's twiddle factors for every pass of an L-point transform with E
+// elements per thread (thread t, element order of fft_regs), held in
+// registers: loaded once per kernel from the exact-rounded table
+// exp(-2 pi i s / TL) and reused by all of the kernel's transforms of that
+// length (an inverse transform conjugates them), so the passes do no table
+// reads, no index arithmetic and no sincos.  Worth it where the count is
+// small (<= 14 for L <= 512 with E = 8); the values equal the table entries,
+// so results are bitwise those of the table path.
+template <int L, int E>
+struct TwReg {
+    static constexpr int count = tw_count_from(L, E, 1);
+    float2 w[count > 0 ? count : 1];
+};
+template <int L, int E>
+constexpr bool tw_reg_fits() {
+    return E <= 8 && TwReg<L, E>::count <= 16;
+}
+
+__device__ __forceinline__ float2 tw_fetch(const float2 *p, int i) { return p[i]; }
+__device__ __forceinline__ float2 tw_fetch(const FSIM_GLB float2 *p, int i) { return ld(p + i); }
+
+template <int L, int E, int TL, int NS, typename P>
+__device__ __forceinline__ void tw_reg_load_pass(TwReg<L, E> &tw, int t, P tab) {
+    if constexpr (NS < L) {
+        constexpr int R = tw_radix(L, E, NS), Q = E / R, T = L / E;
+        if constexpr (NS > 1) {
+            constexpr int off = tw_count_from(L, E, 1) - tw_count_from(L, E, NS);
+            constexpr int SC = TL / (NS * R);
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const int k = (t + T * q) & (NS - 1);
+#pragma unroll
+                for (int r = 1; r < R; r++) tw.w[off + q * (R - 1) + r - 1] = tw_fetch(tab, SC * k * r);
+            }
+        }
+        tw_reg_load_pass<L, E, TL, NS * R>(tw, t, tab);
+    }
+}
+
+// Fills tw for thread t from the table tab (LDS or global) of length TL, a
+// multiple of L: entry s = exp(-2 pi i s / TL).
+template <int L, int E, int TL, typename P>
+__device__ __forceinline__ void tw_reg_load(TwReg<L, E> &tw, int t, P tab) {
+    static_assert(TL % L == 0, "twiddle table length must be a multiple of L");
+    tw_reg_load_pass<L, E, TL, 1>(tw, t, tab);
+}
+
+// Twiddle sources of fft_pass: a table (TL > 0: LDS table of length TL; TL =
+// 0: sincospi of the first twiddle and its powers) or TwReg registers.
+template <int TL>
+struct TwTab {
+    const float2 *tw;
+    template <int L, int E, int NS, int R, bool INV>
+    __device__ __forceinline__ void apply(float2 (&u)[R], int k, int q) const {
+        if constexpr (TL > 0) {
+            // w^r = exp(-+2 pi i k r / (NS R)) = table[(TL / (NS R)) k r];
+            // the index advances by additions (k is opaque to the compiler,
+            // which would otherwise form k r with quarter-rate multiplies)
+            constexpr int SC = TL / (NS * R);
+            const int step = SC * k;
+            int idx = step;
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                float2 w = tw[idx];
+                idx += step;
+                asm volatile("" : "+v"(idx));   // keeps the addition (no multiply by r)
+                if (INV) w.y = -w.y;
+                u[r] = cmul(u[r], w);
+            }
+        } else {
+            // exp(-+2 pi i k / (NS R)): the argument k/(NS R) is exact
+            // in float (power-of-two denominator), sincospi is accurate
+            float sw, cw;
+            sincospif((INV ? 2.0f : -2.0f) * (float)k / (float)(NS * R), &sw, &cw);
+            const float2 w1 = make_float2(cw, sw);
+            float2 w = w1;
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                u[r] = cmul(u[r], w);
+                w = cmul(w, w1);
+            }
+        }
+    }
+};
+template <int L0, int E0>
+struct TwRegRef {
+    const TwReg<L0, E0> &tw;
+    template <int L, int E, int NS, int R, bool INV>
+    __device__ __forceinline__ void apply(float2 (&u)[R], int, int q) const {
+        constexpr int off = tw_count_from(L, E, 1) - tw_count_from(L, E, NS);
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            float2 w = tw.w[off + q * (R - 1) + r - 1];
+            if (INV) w.y = -w.y;
+            u[r] = cmul(u[r], w);
+        }
+    }
+};
+
 // Stockham pass NS (sub-transform size so far) of radix R = min(E, 16, L/NS);
-// recursion walks the passes at compile time.  TL > 0: tw is a twiddle table
-// of length TL (a multiple of L) in LDS; TL = 0: twiddles from sincospi.
-template <int L, int E, int NS, bool INV, bool FIRST, bool WAVE, int TL>
-__device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, int ls, int t,
-                                         const float2 *__restrict__ tw, bool active) {
+// recursion walks the passes at compile time.  TS supplies the twiddles
+// (TwTab or TwRegRef).
+template <int L, int E, int NS, bool INV, bool FIRST, bool WAVE, typename TS>
+__device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, int ls, int t, const TS &ts, bool active) {
     constexpr int REM = L / NS;
     constexpr int RE = E > 16 ? 16 : E;  // in-register DFTs go up to radix 16
     constexpr int R = REM >= RE ? RE : REM;
@@ -227,30 +337,7 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, int ls, in
             // loop they stay live across it and push kernels into spills
             // (the empty asm emits no instruction)
             asm volatile("" : "+v"(k));
-            if constexpr (NS > 1) {
-                if constexpr (TL > 0) {
-                    // w^r = exp(-+2 pi i k r / (NS R)) = table[(TL / (NS R)) k r]
-                    constexpr int SC = TL / (NS * R);
-#pragma unroll
-                    for (int r = 1; r < R; r++) {
-                        float2 w = tw[SC * k * r];
-                        if (INV) w.y = -w.y;
-                        u[r] = cmul(u[r], w);
-                    }
-                } else {
-                    // exp(-+2 pi i k / (NS R)): the argument k/(NS R) is exact
-                    // in float (power-of-two denominator), sincospi is accurate
-                    float sw, cw;
-                    sincospif((INV ? 2.0f : -2.0f) * (float)k / (float)(NS * R), &sw, &cw);
-                    const float2 w1 = make_float2(cw, sw);
-                    float2 w = w1;
-#pragma unroll
-                    for (int r = 1; r < R; r++) {
-                        u[r] = cmul(u[r], w);
-                        w = cmul(w, w1);
-                    }
-                }
-            }
+            if constexpr (NS > 1) ts.template apply<L, E, NS, R, INV>(u, k, q);
             dft_reg<R, INV>(u);
             if constexpr (LAST) {
 #pragma unroll
@@ -267,7 +354,7 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, int ls, in
     }
     if constexpr (!LAST) {
         pass_sync<WAVE>();
-        fft_pass<L, E, NS * R, INV, false, WAVE, TL>(v, lds, ls, t, tw, active);
+        fft_pass<L, E, NS * R, INV, false, WAVE, TS>(v, lds, ls, t, ts, active);
     }
 }
 
@@ -287,7 +374,18 @@ __device__ __forceinline__ void fft_regs(float2 (&v)[E], float2 *lds, int ls, in
     static_assert(L % E == 0, "E must divide L");
     static_assert(TL == 0 || TL % L == 0, "twiddle table length must be a multiple of L");
     static_assert(!WAVE || (L / E) <= 64, "wave synchronisation needs the transform inside one wavefront");
-    fft_pass<L, E, 1, INV, true, WAVE, TL>(v, lds, ls, t, tw, active);
+    const TwTab<TL> ts{tw};
+    fft_pass<L, E, 1, INV, true, WAVE, TwTab<TL>>(v, lds, ls, t, ts, active);
+}
+
+// fft_regs with the thread's twiddles in registers (tw_reg_load).
+template <int L, int E, bool INV, bool WAVE>
+__device__ __forceinline__ void fft_regs_r(float2 (&v)[E], float2 *lds, int ls, int t, const TwReg<L, E> &tw,
+                                           bool active = true) {
+    static_assert(L % E == 0, "E must divide L");
+    static_assert(!WAVE || (L / E) <= 64, "wave synchronisation needs the transform inside one wavefront");
+    const TwRegRef<L, E> ts{tw};
+    fft_pass<L, E, 1, INV, true, WAVE, TwRegRef<L, E>>(v, lds, ls, t, ts, active);
 }
 
 // Twiddle table of a kernel whose largest transform has L points: tl is the

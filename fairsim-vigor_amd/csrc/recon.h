@@ -35,6 +35,10 @@ struct ReconConst {
     // the Hermitian pairing), so the gather forms columns 0..n of it and the
     // last pass is a half-length complex-to-real row transform.
     int half;
+    // otf_real: every OTF table value is real, so conj(OTF) multiplies as a
+    // real scale (the same values as the complex product without its zero
+    // terms)
+    int otf_real;
 };
 
 // One stack: channel constants, raw input, work buffers and output.
@@ -62,10 +66,35 @@ struct JobList {
 // candidate (elements per thread, columns per block; col_shape reports the
 // columns negated for the one-column-per-wavefront layout).
 constexpr int kColCands = 8;
+// chunk: stacks per launch when a call's stacks run as consecutive chunks
+// (-1: the size's default, 0: all in one launch), streams: chunks run
+// round-robin on this many library streams, each reusing its own work
+// buffers (kept in the 256 MB Infinity Cache between the passes when a
+// chunk's buffers are small enough, and one chunk's tail overlaps the next
+// chunk's head).
+constexpr int kMaxChunkStreams = 4;
 struct PipeCfg {
     int sep = 1;
     int col = 0;
+    int chunk = -1;
+    int streams = 2;
 };
+// chunk size and stream count a call of `count` stacks uses: (count, 1) when
+// it runs as one launch
+void chunk_plan(int n, int D, int Cs, int count, const PipeCfg &cfg, int &K, int &S);
+// The library streams of chunked launches and the lock a caller holds from
+// looking up their work buffers until its chunks are enqueued (the fork /
+// join events are shared, and a call that grows a stream's buffers waits for
+// that stream first).  chunk_streams_locked: with the lock held, the
+// kMaxChunkStreams streams (created on first use), nullptr if they could not
+// be created.
+std::mutex &chunk_mutex();
+const hipStream_t *chunk_streams_locked();
+// With chunk_mutex() held: runs `count` stacks (jl: base job plus strides;
+// its A/B unused) as chunks of K on the first S chunk streams, forked from
+// and joined into s; ws_a / ws_b: the work buffers (K stacks each) of stream i.
+int launch_chunked(int n, const JobList &jl, int count, int D, int P, int Cs, bool half, const PipeCfg &cfg, int K,
+                   int S, hipStream_t s, float2 *const *ws_a, float2 *const *ws_b);
 bool pipe_cfg_lookup(int n, int D, int P, int Cs, bool half, PipeCfg &cfg);
 int col_shape(int n, int col, int *e, int *cw);
 bool sep_all_eligible(int n, int D, int P);

@@ -31,6 +31,7 @@
 #include "recon.h"
 #include "simmath.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -56,6 +57,29 @@
 #endif
 #ifndef FSIM_COL_WPE
 #define FSIM_COL_WPE 6
+#endif
+// Twiddles of the row kernels: 0 sincospi per pass, 1 a block-shared LDS
+// table read per pass, 2 the thread's twiddles in registers, read once from
+// that table.  Column pass: 1 LDS table per pass, 2 registers from the LDS
+// table.  Defaults measured per kernel at 512^2 (64 stacks): the all-units
+// separation runs four transforms per thread (registers: 0.166 ms against
+// 0.178 with sincospi), the row shift is no faster with fewer VALU
+// instructions (0.17 ms in all modes; the table leaves more issue slots to
+// concurrent kernels), both separation kernels share one mode (the same
+// arithmetic, so the two forms give bitwise equal spectra), the complex-to-real pass loses to a table fill
+// (0.093 ms sincospi, 0.114 table), the column pass loses with register
+// twiddles (0.195 ms table, 0.24 registers: occupancy).
+#ifndef FSIM_ROW_TW
+#define FSIM_ROW_TW 1
+#endif
+#ifndef FSIM_SEP_TW
+#define FSIM_SEP_TW 2
+#endif
+#ifndef FSIM_C2R_TW
+#define FSIM_C2R_TW 0
+#endif
+#ifndef FSIM_COL_TW
+#define FSIM_COL_TW 1
 #endif
 
 namespace fsim {
@@ -245,9 +269,9 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int nrb, int unit
         nf = P;
     }
     const int crow = K.srow[c];          // separation-matrix row
-    // row kernels: small blocks, twiddles from sincospi (no table fill)
-    using TW = TwLds<N, false>;
-    const float2 *tws = tw;
+    using TW = TwLds<N, FSIM_SEP_TW != 0>;
+    __shared__ float2 twl[TW::len];
+    const float2 *tws = tw_table<TW::tl, T * RW>(twl, tw);
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int y = rb * RW + rr;
     const size_t nn = (size_t)N * N;
@@ -356,7 +380,13 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int nrb, int unit
                 }
         }
     }
-    fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
+    if constexpr (FSIM_SEP_TW == 2 && tw_reg_fits<N, E>()) {
+        TwReg<N, E> twr;
+        tw_reg_load<N, E, TW::tl>(twr, t, tws);
+        fft_regs_r<N, E, false, RowWave<N>::value>(v, lds + rr * LP, 1, t, twr);
+    } else {
+        fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
+    }
     // with a Hermitian result the gather reads DC columns 0..n/2 only
     const int ntiles = (u == 0 && half) ? N / 2 / kAT + 1 : N / kAT;
     tiled_row_store_regs<N, E>(glb(J.A) + (size_t)(d * C + c) * nn, y, t, v, ntiles * kAT - 1);
@@ -380,8 +410,9 @@ __global__ void __launch_bounds__(256) k_sep_rows_all(JobList jl, int half, cons
     const Job J = get_job(jl, blockIdx.y);
     KRef K = cst(J.c);
     const int C = K.Cs, P = K.P, nf = K.D * P;
-    using TW = TwLds<N, false>;
-    const float2 *tws = tw;
+    using TW = TwLds<N, FSIM_SEP_TW != 0>;
+    __shared__ float2 twl[TW::len];
+    const float2 *tws = tw_table<TW::tl, T * RW>(twl, tw);
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int y = blockIdx.x * RW + rr;
     const size_t nn = (size_t)N * N;
@@ -407,6 +438,10 @@ __global__ void __launch_bounds__(256) k_sep_rows_all(JobList jl, int half, cons
     }
     float2 *row = lds + rr * LP;
     const int units = 1 + K.D * (C - 1);
+    // one set of twiddles serves every unit's transform
+    constexpr bool RT = FSIM_SEP_TW == 2 && tw_reg_fits<N, E>();
+    TwReg<N, E> twr;
+    if constexpr (RT) tw_reg_load<N, E, TW::tl>(twr, t, tws);
     for (int u = 0; u < units; u++) {
         // unit 0: DC summed over the directions; then (d, c), c >= 1
         int d = 0, c = 0, f0 = 0, f1 = nf;
@@ -446,7 +481,10 @@ __global__ void __launch_bounds__(256) k_sep_rows_all(JobList jl, int half, cons
 #pragma unroll
         for (int m = 0; m < E; m++) v[m] = row[PL::pad(t + T * m)];
         pass_sync<RowWave<N>::value>();
-        fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, row, 1, t, tws);
+        if constexpr (RT)
+            fft_regs_r<N, E, false, RowWave<N>::value>(v, row, 1, t, twr);
+        else
+            fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, row, 1, t, tws);
         // with a Hermitian result the gather reads DC columns 0..n/2 only
         const int ntiles = (u == 0 && half) ? N / 2 / kAT + 1 : N / kAT;
         tiled_row_store_regs<N, E>(glb(J.A) + (size_t)(d * C + c) * nn, y, t, v, ntiles * kAT - 1);
@@ -485,8 +523,11 @@ struct ColShiftTile {
     static constexpr int cw = cw0 < CW0 ? cw0 : CW0;
     static constexpr int value = t * cw;
     static constexpr int lds = FftLds<N, e>::len * cw;
+    // register twiddles (no LDS table; about 28 more VGPRs: at most 5 waves
+    // per SIMD keep them without spills)
     static constexpr bool tw_fits = (lds + N) * 8 <= 163840;
-    static constexpr int wpe = lds_waves_per_simd(value, (lds + N) * 8, FSIM_COL_WPE);
+    static constexpr bool rt = FSIM_COL_TW == 2 && tw_fits && tw_reg_fits<N, e>();
+    static constexpr int wpe = lds_waves_per_simd(value, (lds + N) * 8, rt ? 5 : FSIM_COL_WPE);
 };
 
 template <int N, int E0, int CW0, bool WC>
@@ -507,6 +548,9 @@ __global__ void __launch_bounds__((ColShiftTile<N, E0, CW0, WC>::value), (ColShi
     } else {
         w = blockIdx.x;
     }
+    // twiddles: in registers, loaded once from the global table, where
+    // they fit (L <= 512); else the LDS table
+    constexpr bool RT = CT::rt;
     using TW = TwLds<N, CT::tw_fits>;
     __shared__ float2 twl[TW::len];
     const float2 *tws = tw_table<TW::tl, CT::value>(twl, tw);
@@ -533,11 +577,23 @@ __global__ void __launch_bounds__((ColShiftTile<N, E0, CW0, WC>::value), (ColShi
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = ld(col + at_row<N>(t + T * m));
-    fft_regs<N, E, false, WC, TW::tl>(v, reg, LS, t, tws);
+    TwReg<N, E> twr;
+    if constexpr (RT) {
+        tw_reg_load<N, E, TW::tl>(twr, t, tws);
+        fft_regs_r<N, E, false, WC>(v, reg, LS, t, twr);
+    } else {
+        fft_regs<N, E, false, WC, TW::tl>(v, reg, LS, t, tws);
+    }
     const int band = K.sband[c];
     const FSIM_GLB float2 *otf = glb(K.otf2d) + (size_t)(K.otf_nb > 1 ? band : 0) * N * N + x;
+    if (K.otf_real) {
+        // conj(OTF) real: the product's value without the zero terms
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = times_conj_java(v[m], ld(otf + (size_t)(t + T * m) * N));
+        for (int m = 0; m < E; m++) v[m] = cscale(v[m], ld(otf + (size_t)(t + T * m) * N).x);
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = times_conj_java(v[m], ld(otf + (size_t)(t + T * m) * N));
+    }
     // column ramp factors (in flight during the inverse transform where the
     // registers allow)
     const FSIM_GLB float2 *ry = glb(K.ramp) + ((size_t)(d * kMaxComp + K.sramp[c]) * 2 + 1) * N;
@@ -547,14 +603,20 @@ __global__ void __launch_bounds__((ColShiftTile<N, E0, CW0, WC>::value), (ColShi
 #pragma unroll
         for (int m = 0; m < E; m++) r[m] = ld(ry + t + T * m);
     }
-    fft_regs<N, E, true, WC, TW::tl>(v, reg, LS, t, tws);
-    const float sc = 1.0f / N;
+    if constexpr (RT)
+        fft_regs_r<N, E, true, WC>(v, reg, LS, t, twr);
+    else
+        fft_regs<N, E, true, WC, TW::tl>(v, reg, LS, t, tws);
+    // the ramp factors carry the inverse transform's 1/n (exact: a power of two)
 #pragma unroll
     for (int m = 0; m < E; m++) {
         const float2 rm = E <= 8 ? r[m < ER ? m : 0] : ld(ry + t + T * m);
-        v[m] = rot_java(cscale(v[m], sc), rm.x, rm.y);
+        v[m] = rot_java(v[m], rm.x, rm.y);
     }
-    fft_regs<N, E, false, WC, TW::tl>(v, reg, LS, t, tws);
+    if constexpr (RT)
+        fft_regs_r<N, E, false, WC>(v, reg, LS, t, twr);
+    else
+        fft_regs<N, E, false, WC, TW::tl>(v, reg, LS, t, tws);
 #pragma unroll
     for (int m = 0; m < E; m++) st(col + at_row<N>(t + T * m), v[m]);
 }
@@ -573,31 +635,40 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
     const Job J = get_job(jl, blockIdx.y);
     KRef K = cst(J.c);
     const int C = K.Cs, nsh = C - 1;
-    using TW = TwLds<N, false>;
-    const float2 *tws = tw;
+    using TW = TwLds<N, FSIM_ROW_TW != 0>;
+    __shared__ float2 twl[TW::len];
+    const float2 *tws = tw_table<TW::tl, T * RW>(twl, tw);
     const int rb = blockIdx.x % rblocks, dc = blockIdx.x / rblocks;
     const int d = dc / nsh, c = 1 + dc % nsh;   // stored shifted component
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     FSIM_GLB float2 *comp = glb(J.A) + (size_t)(d * C + c) * N * N;
     tiled_rows_load<N, RW, T * RW>(comp, rb * RW, lds, LP, N / kAT);
     __syncthreads();
+    constexpr bool RT = FSIM_ROW_TW == 2 && tw_reg_fits<N, E>();
+    TwReg<N, E> twr;
+    if constexpr (RT) tw_reg_load<N, E, TW::tl>(twr, t, tws);
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = lds[rr * LP + t + T * m];
     // the transform's first exchange rewrites the row's region: every thread
     // of the row (two wavefronts from n = 2048 on) has read it first
     pass_sync<RowWave<N>::value>();
-    fft_regs<N, E, true, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
+    if constexpr (RT)
+        fft_regs_r<N, E, true, RowWave<N>::value>(v, lds + rr * LP, 1, t, twr);
+    else
+        fft_regs<N, E, true, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
     // row factor rx(x) of the sub-pixel ramp (the column factor was applied
-    // by the column pass)
-    const float sc = 1.0f / N;
+    // by the column pass); it carries the inverse transform's 1/n
     const FSIM_GLB float2 *rx = glb(K.ramp) + ((size_t)(d * kMaxComp + K.sramp[c]) * 2) * N;
 #pragma unroll
     for (int m = 0; m < E; m++) {
         const float2 r = ld(rx + t + T * m);
-        v[m] = rot_java(cscale(v[m], sc), r.x, r.y);
+        v[m] = rot_java(v[m], r.x, r.y);
     }
-    fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
+    if constexpr (RT)
+        fft_regs_r<N, E, false, RowWave<N>::value>(v, lds + rr * LP, 1, t, twr);
+    else
+        fft_regs<N, E, false, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
 #pragma unroll
     for (int m = 0; m < E; m++) lds[rr * LP + t + T * m] = v[m];
     __syncthreads();
@@ -751,9 +822,11 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
         fft_regs<N, EN, false, false, TW::tl>(v, lds + cl, CW, t, tws, act);
         if (act) {
             const FSIM_GLB float2 *otf = glb(K.otf2d) + xs;
+            const bool re = K.otf_real != 0;
 #pragma unroll
             for (int m = 0; m < EN; m++) {
-                v[m] = times_conj_java(v[m], ld(otf + (size_t)(t + TN * m) * N));
+                const float2 o = ld(otf + (size_t)(t + TN * m) * N);
+                v[m] = re ? cscale(v[m], o.x) : times_conj_java(v[m], o);
                 lds[(t + TN * m) * CW + cl] = v[m];
             }
         }
@@ -918,7 +991,11 @@ __global__ void __launch_bounds__(256) k_row_c2r(JobList jl, const float2 *__res
     constexpr int CW = GatherE<N>::cw;
     constexpr int NT = T * RW;
     __shared__ float2 lds[RW * LP];
-    using TW = TwLds<N, false>;
+    // the twiddles of the n-point transform from the m-point table
+    // (TL = m, a multiple of n)
+    using TW = TwLds<M, FSIM_C2R_TW != 0>;
+    __shared__ float2 twl[TW::len];
+    const float2 *tws = tw_table<TW::tl, NT>(twl, twm);
     const Job J = get_job(jl, blockIdx.y);
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int grp = (M / RW) % 8 ? (int)blockIdx.x : xcd_linear(M / RW);
@@ -967,7 +1044,13 @@ __global__ void __launch_bounds__(256) k_row_c2r(JobList jl, const float2 *__res
         const float2 o = make_float2(d.x * w[m].x + d.y * w[m].y, d.y * w[m].x - d.x * w[m].y);
         v[m] = make_float2(e.x - o.y, e.y + o.x);
     }
-    fft_regs<N, E, true, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, twm);
+    if constexpr (FSIM_C2R_TW == 2 && tw_reg_fits<N, E>()) {
+        TwReg<N, E> twr;
+        tw_reg_load<N, E, TW::tl>(twr, t, tws);
+        fft_regs_r<N, E, true, RowWave<N>::value>(v, lds + rr * LP, 1, t, twr);
+    } else {
+        fft_regs<N, E, true, RowWave<N>::value, TW::tl>(v, lds + rr * LP, 1, t, tws);
+    }
     const float sc = 1.0f / M;
     FSIM_GLB float2 *o = reinterpret_cast<FSIM_GLB float2 *>(glb(J.out) + (size_t)y * M);
 #pragma unroll
@@ -996,7 +1079,7 @@ __global__ void k_pack_wa(float *__restrict__ wa, float *__restrict__ apo, const
     }
 }
 
-// Sub-pixel ramp tables exp(2 pi i f t / n), t < n, of the x and y
+// Sub-pixel ramp tables exp(2 pi i f t / n) / n, t < n, of the x and y
 // fractional shifts f of every direction and component
 // (SimUtils.java:74-81), from double precision: ramp[k * n + t] for the
 // shift f[k].
@@ -1006,7 +1089,10 @@ __global__ void k_ramps(float2 *__restrict__ ramp, const double *__restrict__ f,
         const double ph = 2 * M_PI * f[i / n] * t / n;
         double sn, cs;
         sincos(ph, &sn, &cs);
-        ramp[i] = make_float2((float)cs, (float)sn);
+        // times the 1/n of the inverse transform it follows (exact: n is a
+        // power of two), so the kernels apply one product
+        const float inv_n = 1.0f / (float)n;
+        ramp[i] = make_float2((float)cs * inv_n, (float)sn * inv_n);
     }
 }
 
@@ -1246,11 +1332,82 @@ int launch_pipeline_cfg(int n, const JobList &jl, int njobs, int D, int P, int C
 size_t workspace_a_elems(int n, int D, int C) { return (size_t)D * C * n * n; }
 size_t workspace_b_elems(int n) { return (size_t)4 * n * n; }
 
+void chunk_plan(int n, int D, int Cs, int count, const PipeCfg &cfg, int &K, int &S) {
+    K = count;
+    S = 1;
+    int k = cfg.chunk;
+    if (k < 0) {
+        // default: up to 512^2, the largest power of two of stacks whose work
+        // buffers take at most 160 MiB (8 stacks at 512^2, 3 x 3 frames)
+        if (n > 512) return;
+        const size_t per = (workspace_a_elems(n, D, Cs) + workspace_b_elems(n)) * sizeof(float2);
+        k = 1;
+        while ((size_t)(2 * k) * per <= ((size_t)160 << 20)) k *= 2;
+    }
+    if (k <= 0 || k >= count) return;
+    const int chunks = (count + k - 1) / k;
+    K = k;
+    S = std::max(1, std::min({cfg.streams, kMaxChunkStreams, chunks}));
+}
+
+// Library streams of chunked launches and their fork / join events; `mu`
+// is held while a call enqueues (the events are reused).
+struct ChunkStreams {
+    std::mutex mu;
+    bool made = false, failed = false;
+    hipStream_t s[kMaxChunkStreams] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kMaxChunkStreams] = {};
+};
+static ChunkStreams g_chunk;
+
+static bool chunk_streams_ready() {
+    if (g_chunk.made) return true;
+    if (g_chunk.failed) return false;
+    bool ok = hipEventCreateWithFlags(&g_chunk.fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; i < kMaxChunkStreams && ok; i++)
+        ok = hipStreamCreateWithFlags(&g_chunk.s[i], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&g_chunk.join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        g_chunk.failed = true;
+        return false;
+    }
+    g_chunk.made = true;
+    return true;
+}
+
+std::mutex &chunk_mutex() { return g_chunk.mu; }
+
+const hipStream_t *chunk_streams_locked() { return chunk_streams_ready() ? g_chunk.s : nullptr; }
+
+int launch_chunked(int n, const JobList &jl, int count, int D, int P, int Cs, bool half, const PipeCfg &cfg, int K,
+                   int S, hipStream_t s, float2 *const *ws_a, float2 *const *ws_b) {
+    if (!chunk_streams_ready()) return fail(FSIM_ERR_HIP, "reconstruct: chunk streams could not be created");
+    FSIM_HIP(hipEventRecord(g_chunk.fork, s));
+    for (int i = 0; i < S; i++) FSIM_HIP(hipStreamWaitEvent(g_chunk.s[i], g_chunk.fork, 0));
+    for (int c = 0, first = 0; first < count; c++, first += K) {
+        const int i = c % S, kc = std::min(K, count - first);
+        JobList cj = jl;
+        cj.base.raw += (size_t)first * jl.raw_stride;
+        cj.base.out += (size_t)first * jl.out_stride;
+        cj.base.A = ws_a[i];
+        cj.base.B = ws_b[i];
+        const int rc = launch_pipeline_cfg(n, cj, kc, D, P, Cs, half, cfg, g_chunk.s[i]);
+        if (rc) return rc;
+    }
+    for (int i = 0; i < S; i++) {
+        FSIM_HIP(hipEventRecord(g_chunk.join[i], g_chunk.s[i]));
+        FSIM_HIP(hipStreamWaitEvent(s, g_chunk.join[i], 0));
+    }
+    return FSIM_OK;
+}
+
 // ---------------------------------------------------------------------------
 // channel set-up: the FilterUpdateThread work (ReconstructionRunner.java:
 // 773-828) plus the per-direction constants of the reconstruction loop.
 // ---------------------------------------------------------------------------
-int build_channel_const(const fsim_recon_params &p, const Otf &o, bool sym, ReconConst &K) {
+int build_channel_const(const fsim_recon_params &p, const Otf &o, bool sym, bool otf_real, ReconConst &K) {
     const int n = p.n, D = p.ndir, P = p.nphase, B = p.nband, C = 2 * B - 1;
     if (!recon_size_ok(n)) return fail(FSIM_ERR_ARG, "channel: n must be a power of two in [64, 2048]");
     if (D < 1 || D > kMaxDir || B < 2 || B > kMaxBand || P < C || P > kMaxPhase)
@@ -1272,6 +1429,7 @@ int build_channel_const(const fsim_recon_params &p, const Otf &o, bool sym, Reco
     K.otf_tab = p.use_att ? o.vals_att : o.vals;
     K.Cs = sym ? B : C;
     K.half = sym ? 1 : 0;
+    K.otf_real = otf_real ? 1 : 0;
     for (int j = 0; j < kMaxComp; j++) K.srow[j] = K.sband[j] = K.sramp[j] = K.src[j] = 0;
     for (int j = 0; j < K.Cs; j++) {
         K.srow[j] = sym ? 2 * j : j;          // DC, then pos of band j
@@ -1338,6 +1496,7 @@ struct fsim_channel_s {
     fsim_recon_params p;
     Otf o;
     bool sym = false;
+    bool otf_real = false;
     ReconConst K;
     ReconConst *dK = nullptr;
     float *wa = nullptr;    // (1/Wiener denominator) * apodization, tile-major
@@ -1484,7 +1643,7 @@ int reconstruct_params_with(const fsim_channel_s *ch, MultiWork &w, float2 *A, f
             for (int ph = 0; ph < p.nphase; ph++) p.phases[d][ph] = q.phases[d][ph];
             for (int b = 0; b < p.nband; b++) p.modul[d][b] = q.modul[d][b];
         }
-        const int rc = build_channel_const(p, ch->o, ch->sym, w.hK[k]);
+        const int rc = build_channel_const(p, ch->o, ch->sym, ch->otf_real, w.hK[k]);
         if (rc) return rc;
         ShiftSet &S = w.hsh[k];
         S = ShiftSet{};
@@ -1638,17 +1797,19 @@ int fsim_channel_create(const fsim_recon_params *p, fsim_otf oh, fsim_channel *o
     if (!p || !oh || !out) return fail(FSIM_ERR_ARG, "channel_create: null argument");
     const Otf &o = oh->o;
     // Hermitian pairing needs a real OTF (then conj(OTF) is symmetric in k)
-    bool sym = p->mode == 0;
-    if (sym) {
+    bool otf_real = true;
+    {
         std::vector<float2> hv((size_t)o.nband * o.ns);
         FSIM_HIP(hipMemcpy(hv.data(), p->use_att ? o.vals_att : o.vals, hv.size() * sizeof(float2),
                            hipMemcpyDeviceToHost));
         for (const float2 &z : hv)
-            if (z.y != 0.f) sym = false;
+            if (z.y != 0.f) otf_real = false;
     }
+    const bool sym = p->mode == 0 && otf_real;
     fsim_channel_s *ch = new fsim_channel_s();
     ch->p = *p;
-    rc = build_channel_const(*p, o, sym, ch->K);
+    ch->otf_real = otf_real;
+    rc = build_channel_const(*p, o, sym, otf_real, ch->K);
     if (rc) {
         delete ch;
         return rc;
@@ -1700,15 +1861,10 @@ int fsim_reconstruct(fsim_channel ch, const uint16_t *raw, float *out, int count
     const ReconConst &K = ch->K;
     const int n = K.n;
     hipStream_t s = strm ? (hipStream_t)strm : stream();
-    Workspace *w = nullptr;
-    rc = ensure_workspace(ch, count, s, &w);
-    if (rc) return rc;
     JobList jl{};
     jl.jobs = nullptr;
     jl.base.c = ch->dK;
     jl.base.raw = raw;
-    jl.base.A = w->A;
-    jl.base.B = w->B;
     jl.base.out = out;
     jl.raw_stride = (size_t)K.D * K.P * n * n;
     jl.a_stride = workspace_a_elems(n, K.D, K.Cs);
@@ -1718,7 +1874,31 @@ int fsim_reconstruct(fsim_channel ch, const uint16_t *raw, float *out, int count
     // the half-spectrum pass writes output pixel pairs (8-byte stores); an
     // output that is only 4-byte aligned takes the full-width pass
     const bool half = K.half && ((uintptr_t)out & 7) == 0;
-    return launch_pipeline(n, jl, count, K.D, K.P, K.Cs, half, s);
+    PipeCfg cfg;
+    if (!pipe_cfg_lookup(n, K.D, K.P, K.Cs, half, cfg)) cfg = PipeCfg{};
+    int kc = count, ns = 1;
+    chunk_plan(n, K.D, K.Cs, count, cfg, kc, ns);
+    if (kc < count) {
+        // chunks on the library streams, each with its own work buffers
+        std::lock_guard<std::mutex> lk(chunk_mutex());
+        const hipStream_t *cs = chunk_streams_locked();
+        if (!cs) return fail(FSIM_ERR_HIP, "reconstruct: chunk streams could not be created");
+        float2 *wa[kMaxChunkStreams], *wb[kMaxChunkStreams];
+        for (int i = 0; i < ns; i++) {
+            Workspace *w = nullptr;
+            rc = ensure_workspace(ch, kc, cs[i], &w);
+            if (rc) return rc;
+            wa[i] = w->A;
+            wb[i] = w->B;
+        }
+        return launch_chunked(n, jl, count, K.D, K.P, K.Cs, half, cfg, kc, ns, s, wa, wb);
+    }
+    Workspace *w = nullptr;
+    rc = ensure_workspace(ch, count, s, &w);
+    if (rc) return rc;
+    jl.base.A = w->A;
+    jl.base.B = w->B;
+    return launch_pipeline_cfg(n, jl, count, K.D, K.P, K.Cs, half, cfg, s);
 }
 
 int fsim_reconstruct_noalloc(fsim_channel ch, const uint16_t *raw, float *out, int count, void *strm) {

@@ -109,7 +109,9 @@ class SERVER_STATS(C.Structure):
 class TUNE_RESULT(C.Structure):
     """fsim_tune_result (include/fsim.h)."""
     _fields_ = [("sep", C.c_int), ("col", C.c_int), ("col_elems", C.c_int), ("col_width", C.c_int),
-                ("candidates", C.c_int), ("ms_best", C.c_double), ("ms_default", C.c_double)]
+                ("candidates", C.c_int), ("ms_best", C.c_double), ("ms_default", C.c_double),
+                ("chunk", C.c_int), ("streams", C.c_int), ("ms_whole", C.c_double),
+                ("ms_chunk_default", C.c_double)]
 
 
 FSIM_SUBMIT_DEVICE = 1
@@ -193,6 +195,8 @@ _SIGS = {
     "fsim_autotune": [_P, _I, _I, C.POINTER(TUNE_RESULT)],
     "fsim_autotune_set": [_I, _I, _I, _I, _I, _I, _I],
     "fsim_autotune_get": [_I, _I, _I, _I, _I, _PI, _PI],
+    "fsim_autotune_set_chunks": [_I, _I, _I, _I, _I, _I, _I],
+    "fsim_autotune_get_chunks": [_I, _I, _I, _I, _I, _I, _PI, _PI],
     "fsim_autotune_clear": [],
     "fsim_autotune_save": [C.c_char_p],
     "fsim_autotune_load": [C.c_char_p],

@@ -474,19 +474,34 @@ int fsim_batch_destroy(fsim_batch b);
  * per candidate) and caches the fastest -- the default unless a candidate is
  * more than 2% faster -- for the process; every later launch of
  * that class (fsim_reconstruct, _params, the server, batches created after)
- * uses it.  save/load persist the cache as text ("fsim-autotune 1" then
- * "n D P Cs half sep col count ms" per line); load merges and rejects a
+ * uses it.  save/load persist the cache as text ("fsim-autotune 2" then
+ * "n D P Cs half sep col chunk streams count ms" per line; version 1 files
+ * without the chunk columns load too); load merges and rejects a
  * malformed file whole.  set pins a choice by hand, get returns -1s when the
  * class has no entry.  These calls do not need a GPU except fsim_autotune.
  * fsim_init loads the file named by the environment variable
  * FSIM_AUTOTUNE_CACHE, when set (a missing or malformed file is ignored). */
 typedef struct {
     int sep, col, col_elems, col_width, candidates;
-    double ms_best, ms_default;   /* per launch of `count` stacks */
+    double ms_best, ms_default;   /* per call of `count` stacks: chosen, all defaults */
+    int chunk, streams;           /* chosen chunking: stacks per chunk (0: one launch), streams */
+    double ms_whole, ms_chunk_default;  /* chosen shapes as one launch / with the size's default chunking */
 } fsim_tune_result;
 int fsim_autotune(fsim_channel ch, int count, int reps, fsim_tune_result *res);
 int fsim_autotune_set(int n, int D, int P, int Cs, int half, int sep, int col);
 int fsim_autotune_get(int n, int D, int P, int Cs, int half, int *sep, int *col);
+/* Chunked calls: fsim_reconstruct runs a call's stacks as consecutive chunks
+ * of `chunk` stacks, round-robin on `streams` library streams forked from and
+ * joined into the caller's stream, each stream reusing its own chunk-sized
+ * work buffers (which then stay in the GPU's 256 MB Infinity Cache between
+ * the passes, and one chunk's tail overlaps the next one's head).  chunk =
+ * -1: the size default (up to 512^2: the largest power of two of stacks
+ * whose work buffers fit 160 MiB, on 2 streams; one launch above), 0: one
+ * launch.  Results are bitwise those of one launch.  fsim_autotune times the
+ * chunkings after the shapes.  get_chunks reports the plan a call of `count`
+ * stacks of the class uses (chunk = count, streams = 1 for one launch). */
+int fsim_autotune_set_chunks(int n, int D, int P, int Cs, int half, int chunk, int streams);
+int fsim_autotune_get_chunks(int n, int D, int P, int Cs, int half, int count, int *chunk, int *streams);
 int fsim_autotune_clear(void);
 int fsim_autotune_save(const char *path);
 int fsim_autotune_load(const char *path);
