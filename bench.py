@@ -28,13 +28,13 @@ STAGES = ["k_sep_rows", "k_col_shift", "k_row_shift", "k_gather", "k_row_real"]
 TUNE = True   # run the autotuner for each workload's size class (--no-tune: defaults)
 
 
-def profiled_traffic(kernel, n, batch):
+def profiled_traffic(kernel, n, per_launch):
     """HBM bytes per launch of `kernel` (read + write) from the committed
     rocprofv3 counter summary of this workload (profiles/), or None when no
-    summary covers this n and batch.  Read bytes are 2 x FETCH_SIZE and write
-    bytes WRITE_SIZE, the factors the summary's copy-kernel calibration
-    gives."""
-    path = os.path.join(ROOT, "profiles", f"recon_pipeline_{n}x3x3_batch{batch}.md")
+    summary covers this n and number of stacks per launch.  Read bytes are
+    2 x FETCH_SIZE and write bytes WRITE_SIZE, the factors the summary's
+    copy-kernel calibration gives."""
+    path = os.path.join(ROOT, "profiles", f"recon_pipeline_{n}x3x3_{per_launch}_stacks_per_launch.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -195,12 +195,12 @@ def serve_mixed_stream(nreq, rate):
 _STACKS = {}
 
 
-def parity_errors(got, sp, n, raw, lam=525.0, threads=None):
+def parity_errors(got, sp, n, raw, lam=525.0, threads=None, px_um=0.080):
     """rel-L2 of the GPU image against the FP32 Java-order oracle and the FP64
     re-run of the same dataflow, and of the FP32 oracle against FP64 (the
     error budget of the reference arithmetic itself)."""
     from oracle import fp64, ref
-    p = ref.recon_dict(sp, n, lam=lam)
+    p = ref.recon_dict(sp, n, lam=lam, px_um=px_um)
     threads = threads or max(1, min(16, os.cpu_count() or 1))
     r32 = ref.recon32(p, raw, threads=threads)
     r64 = ref.recon64(p, raw)
@@ -219,7 +219,7 @@ def _stack(n, seed, **kw):
 
 
 def serve_large_fov_mix(nreq=400, rate=200.0, frac=0.05, tps=8):
-    """Config 4 served: the config-3 stream (Poisson `rate` req/s, sizes
+    """Served large-FOV mix: the mixed Poisson request stream (`rate` req/s, sizes
     256/512/1024 at .5/.35/.15, 3 channels, pinned host buffers) with a share
     `frac` of large-FOV requests: 2048^2 x 3 x 3 uint16 stacks, `tps`
     timepoints each, resident on the device; the k0/phase fit on the first
@@ -252,12 +252,13 @@ def serve_large_fov_mix(nreq=400, rate=200.0, frac=0.05, tps=8):
     # the large-FOV channel: 65 nm pixels, 2048^2; its parameters come from
     # the fit of each request's first timepoint
     nl = 2048
-    raw_l, sp_l = _stack(nl, 7, nbeads=1200, nfil=160)
-    param_l = synthetic.make_param(sp_l, nl)
+    series_l, sp_l = _stack(nl, 7, nbeads=1200, nfil=160, px_um=0.065, timepoints=8)
+    raw_l = series_l[0]
+    param_l = synthetic.make_param(sp_l, nl, px_um=0.065)
     rec_l = Reconstructor(param_l)
     lid = srv.add_channel(rec_l)
     rb_l = DeviceBuffer(tps * raw_l.nbytes)
-    rb_l.upload(np.ascontiguousarray(np.broadcast_to(raw_l, (tps,) + raw_l.shape)))
+    rb_l.upload(np.ascontiguousarray(series_l[np.arange(tps) % len(series_l)]))
     ib_l = DeviceBuffer(raw_l.size * 4)
     ib_l.upload(raw_l.astype(np.float32))
     ob_l = DeviceBuffer(tps * rec_l.out_bytes())
@@ -312,8 +313,9 @@ def serve_large_fov_mix(nreq=400, rate=200.0, frac=0.05, tps=8):
     st = srv.stats()
     srv.close()
     nlarge = sum(1 for k in kinds if k is None)
-    return {"arrivals": f"Poisson {rate:g} req/s, {nreq} requests: {nlarge} large-FOV ({tps} x 2048^2 timepoints, "
-                        f"k0 fit on the first) and {nreq - nlarge} of sizes 256/512/1024 at .5/.35/.15, 3 channels",
+    return {"arrivals": f"Poisson {rate:g} req/s, {nreq} requests: {nlarge} large-FOV ({tps} x 2048^2 timepoints "
+                        f"at 65 nm, k0 fit on the first) and {nreq - nlarge} of sizes 256/512/1024 at "
+                        f".5/.35/.15, 3 channels",
             "offered_req_per_s": round(nreq / el, 1),
             "small_p50_ms": round(float(np.percentile(small_lat, 50)), 3),
             "small_p99_ms": round(float(np.percentile(small_lat, 99)), 3),
@@ -325,7 +327,7 @@ def serve_large_fov_mix(nreq=400, rate=200.0, frac=0.05, tps=8):
 
 
 def batched_mixed(nper=32, reps=10):
-    """Config 2: one batch of 96 stacks -- 3 channels (525/600/670 nm) x sizes
+    """Mixed-size batch: one batch of 96 stacks -- 3 channels (525/600/670 nm) x sizes
     256/512/1024, 32 of each size, stacks and outputs resident in HBM.  The
     headline form is one fixed-shape batch (fsim_batch_*: the three size
     classes run concurrently, captured once as a hipGraph and replayed; the
@@ -341,27 +343,35 @@ def batched_mixed(nper=32, reps=10):
     from fsim.sim_algorithm import Reconstructor
     sizes, lams = [256, 512, 1024], [525.0, 600.0, 670.0]
     recs, bufs, keep, host = {}, {}, [], {}
+    ntp = (nper + 2) // 3                       # timepoints per (size, channel)
     for n in sizes:
         for ci, lam in enumerate(lams):
+            # one sample per (size, channel) imaged ntp times with independent
+            # seeded photon noise: every slot of the batch has its own stack
             raw, sp = synthetic.sim_stack(n=n, seed=200 + ci, lam=lam, nbeads=300 * n // 512,
-                                          nfil=40 * n // 512)
-            host[(n, ci)] = (raw, sp, lam)
+                                          nfil=40 * n // 512, timepoints=ntp)
+            host[(n, ci)] = (raw[0], sp, lam)
             rec = Reconstructor(synthetic.make_param(sp, n, lam=lam))
             keep.append(rec)
             recs[(n, ci)] = rec
-            rb = DeviceBuffer(raw.nbytes)
-            rb.upload(raw)
-            bufs[(n, ci)] = rb
+            for t in range(ntp):
+                rb = DeviceBuffer(raw[t].nbytes)
+                rb.upload(raw[t])
+                bufs[(n, ci, t)] = rb
     tuned = {}
     if TUNE:
         from fsim import autotune
         for n in sizes:
             tuned[n] = autotune.tune(recs[(n, 0)], count=nper, reps=3)
-    jobs = [(n, i % 3) for n in sizes for i in range(nper)]
+    slots = [(n, i % 3, i // 3) for n in sizes for i in range(nper)]
+    jobs = [s[:2] for s in slots]
     outs = {n: [DeviceBuffer(16 * n * n) for _ in range(nper)] for n in sizes}
     out_ptrs = [outs[k[0]][j % nper].value for j, k in enumerate(jobs)]
-    raw_ptrs = [bufs[k].value for k in jobs]
+    raw_ptrs = [bufs[s].value for s in slots]
+    assert len(set(raw_ptrs)) == len(slots)     # distinct raw stacks
+    raw_bytes = sum(bufs[s].nbytes for s in slots)
     b_alg = sum(434 * n * n for n, _ in jobs)
+    b_moved = sum(sum(stage_bytes(n, Cs=2, half=True, dc_sum=True)) for n, _ in jobs)
 
     batch = ReconstructionBatch([recs[k] for k in jobs], use_graph=True)
     batch.run(raw_ptrs, out_ptrs)            # captures the graph
@@ -387,7 +397,7 @@ def batched_mixed(nper=32, reps=10):
 
     def run_served():
         t0 = time.perf_counter()
-        tickets = [srv.submit(ids[k], bufs[k].value, outs[k[0]][j % nper].value, device=True)
+        tickets = [srv.submit(ids[k], raw_ptrs[j], outs[k[0]][j % nper].value, device=True)
                    for j, k in enumerate(jobs)]
         for t in tickets:
             srv.wait(t)
@@ -397,20 +407,25 @@ def batched_mixed(nper=32, reps=10):
     srv.stats(reset=True)
     el_s = min(run_served() for _ in range(3))
     srv.close()
-    return {"batch": f"{len(jobs)} stacks: sizes 256/512/1024 x {nper}, 3 channels, resident in HBM",
+    return {"batch": f"{len(jobs)} distinct stacks: sizes 256/512/1024 x {nper}, 3 channels, resident in HBM "
+                     f"({raw_bytes / 1e6:.0f} MB raw in; per size and channel one sample imaged {ntp} times with "
+                     f"independent seeded photon noise)",
             "form": f"one fixed-shape batch, {info['classes']} size classes on concurrent streams, "
                     f"replayed hipGraph (captured={info['captured']})",
             "ms_per_batch": round(el * 1e3, 3), "recons_per_s": round(len(jobs) / el, 1),
-            "alg_bytes_per_batch": b_alg, "roofline_frac": round(b_alg / el / 8e12, 4),
+            "executed_bytes_per_batch": b_moved, "executed_bytes_frac": round(b_moved / el / 8e12, 4),
+            "reference_dataflow_bytes_per_batch": b_alg,
+            "reference_dataflow_equivalent_frac": round(b_alg / el / 8e12, 4),
             "autotune": {n: (t["sep"], t["col_elems"], t["col_width"]) for n, t in tuned.items()},
             "rel_l2": parity,
             "served_one_by_one": {"ms_per_batch": round(el_s * 1e3, 3),
                                   "recons_per_s": round(len(jobs) / el_s, 1),
-                                  "roofline_frac": round(b_alg / el_s / 8e12, 4)}}
+                                  "executed_bytes_frac": round(b_moved / el_s / 8e12, 4),
+                                  "reference_dataflow_equivalent_frac": round(b_alg / el_s / 8e12, 4)}}
 
 
 def large_fov(batch=8, steps=5):
-    """Config 4: 2048^2 x 3 x 3 uint16 stacks -> 4096^2, fused pipeline,
+    """Large field of view: 2048^2 x 3 x 3 uint16 stacks at 65 nm raw pixels -> 4096^2, fused pipeline,
     `batch` stacks per launch resident in HBM; parity of one stack against the
     FP32 oracle."""
     import numpy as np
@@ -418,14 +433,15 @@ def large_fov(batch=8, steps=5):
     from fsim.linalg import DeviceBuffer
     from fsim.sim_algorithm import Reconstructor
     from oracle import fp64, ref
-    n = 2048
-    raw, sp = _stack(n, 7, nbeads=1200, nfil=160)
-    rec = Reconstructor(synthetic.make_param(sp, n))
+    n, px = 2048, 0.065
+    series, sp = _stack(n, 7, nbeads=1200, nfil=160, px_um=px, timepoints=8)
+    raw = series[0]
+    rec = Reconstructor(synthetic.make_param(sp, n, px_um=px))
     if TUNE:
         from fsim import autotune
         autotune.tune(rec, count=batch, reps=2)
     rb = DeviceBuffer(raw.nbytes * batch)
-    host = np.ascontiguousarray(np.broadcast_to(raw, (batch,) + raw.shape))
+    host = np.ascontiguousarray(series[np.arange(batch) % len(series)])
     rb.upload(host)
     ob = DeviceBuffer(rec.out_bytes() * batch)
     rec.reconstruct(rb.value, ob.value, batch)
@@ -438,15 +454,18 @@ def large_fov(batch=8, steps=5):
     value = batch * steps / el
     got = rec.reconstruct_host(raw)
     threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1))
-    par = parity_errors(got, sp, n, raw, threads=threads)
-    return {"workload": f"2048^2 x 3 x 3 uint16 -> 4096^2 float32, batch {batch}",
+    par = parity_errors(got, sp, n, raw, threads=threads, px_um=px)
+    return {"workload": f"2048^2 x 3 x 3 uint16 at 65 nm raw pixels (|k0| = {sp['k0']:.1f} px) -> 4096^2 "
+                        f"float32, batch {batch} distinct timepoints",
             "recons_per_s": round(value, 2), "ms_per_recon": round(1e3 / value, 3),
-            "roofline_frac": round(value * 434 * n * n / 8e12, 4), "rel_l2_vs_oracle": par["gpu_vs_ref32"],
+            "executed_bytes_frac": round(value * sum(stage_bytes(n, Cs=2, half=True, dc_sum=True)) / 8e12, 4),
+            "reference_dataflow_equivalent_frac": round(value * 434 * n * n / 8e12, 4),
+            "rel_l2_vs_oracle": par["gpu_vs_ref32"],
             "rel_l2": par}
 
 
 def estimation(runs=3, batch=64):
-    """Config 1: parameter estimation (SimAlgorithm.estimateParameters) of one
+    """Parameter estimation (SimAlgorithm.estimateParameters) of one
     512^2 stack given as float images: 3 phases / 2 bands (fit band 1) and
     5 phases / 3 bands with a 2 k0 harmonic (|k0| = 0.45 cutoff, fit band 2),
     GPU (device vector flow) vs the FP32 oracle on the host cores; the
@@ -498,7 +517,7 @@ def estimation(runs=3, batch=64):
 
 
 def estimation_batched(nphase, nband, fb, k_frac, h2, batch=64, runs=3):
-    """Config 1 as a throughput: `batch` 512^2 stacks per call (4 distinct
+    """Estimation plus reconstruction as a throughput: `batch` 512^2 stacks per call (4 distinct
     seeded stacks repeated), float32 images and uint16 raw frames resident on
     the device; fsim_estimate for all of them, then fsim_reconstruct_params
     with every stack's own estimate.  Timed: the whole sequence including the
@@ -778,20 +797,22 @@ def main():
     ap.add_argument("--mix-requests", type=int, default=400)
     ap.add_argument("--mix-rate", type=float, default=200.0)
     ap.add_argument("--no-mix", action="store_true")
-    ap.add_argument("--no-configs", action="store_true",
+    ap.add_argument("--no-workloads", action="store_true",
                     help="skip the estimation, mixed-batch and 2048^2 legs")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed batched loop (for rocprofv3 runs)")
+    ap.add_argument("--one-launch", action="store_true",
+                    help="with --profile-only: the batch as one launch on one stream (no chunks)")
     ap.add_argument("--stream-only", action="store_true",
                     help="only the live stream ingest, side-output and filter-style legs")
     ap.add_argument("--estimation-only", action="store_true",
-                    help="only the batched estimation legs of config 1 (for rocprofv3 runs)")
+                    help="only the batched 512^2 estimation legs (for rocprofv3 runs)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (default launch shapes)")
     ap.add_argument("--serve-fov-only", action="store_true",
-                    help="only the served stream with large-FOV requests (config 4 served)")
+                    help="only the served stream with large-FOV requests")
     ap.add_argument("--mixed-only", action="store_true",
-                    help="only the mixed-size batch (config 2)")
+                    help="only the 96-stack mixed-size batch")
     ap.add_argument("--probe-copy", action="store_true",
                     help="run a plain streaming kernel of known size (HBM counter calibration)")
     args = ap.parse_args()
@@ -832,11 +853,13 @@ def main():
         print(json.dumps({"estimation_only": True, "n": 512, "batched": out}), flush=True)
         return
     log(f"generating synthetic stacks n={n}")
-    stacks, sps = [], []
-    for s in range(4):
-        raw, sp = synthetic.sim_stack(n=n, seed=s)
-        stacks.append(raw)
-        sps.append(sp)
+    # B distinct stacks: 4 seeded samples, each imaged B/4 times with
+    # independent photon noise (stacks[i] is stack i of the batch)
+    per = max(2, (B + 3) // 4)
+    series = [synthetic.sim_stack(n=n, seed=s, timepoints=per) for s in range(4)]
+    nst = max(B, 4)
+    stacks = [series[i % 4][0][i // 4] for i in range(nst)]
+    sps = [series[i % 4][1] for i in range(nst)]
     param = synthetic.make_param(sps[0], n)
     rec = Reconstructor(param)
     tuned = None
@@ -844,12 +867,15 @@ def main():
         from fsim import autotune
         tuned = autotune.tune(rec, count=B, reps=3)
         log(f"autotune: {tuned}")
-    frame_stack = np.stack([stacks[i % 4] for i in range(B)])
+    frame_stack = np.stack([stacks[i % len(stacks)] for i in range(B)])
     raw_buf = DeviceBuffer(frame_stack.nbytes)
     raw_buf.upload(frame_stack)
     out_buf = DeviceBuffer(B * rec.out_bytes())
 
     if args.profile_only:
+        if args.one_launch:
+            from fsim import autotune
+            autotune.set_chunks(*autotune.class_of(rec), 0, 1)
         for _ in range(args.warmup):
             rec.reconstruct(raw_buf.value, out_buf.value, B)
         _lib.call("fsim_sync")
@@ -886,38 +912,72 @@ def main():
     parity64 = {"gpu_vs_ref32": parity, "gpu_vs_fp64": fp64.rel_l2(got, r64), "ref32_vs_fp64": fp64.rel_l2(r32, r64)}
     log(f"rel-L2 gpu vs oracle {parity:.3e}, vs FP64 {parity64['gpu_vs_fp64']:.3e}")
 
-    # batched fused pipeline
-    log(f"warmup {args.warmup} steps of batch {B}")
+    # batched fused pipeline: the timed region runs without per-kernel
+    # events; an identical second region with HIP events around every kernel
+    # (on the stream each kernel is launched on) gives the per-kernel times
+    from fsim import autotune
+    cls = autotune.class_of(rec)
+    per_launch, nstreams = autotune.chunk_plan(*cls, B)
+    log(f"warmup {args.warmup} steps of batch {B} ({per_launch} stacks per launch on {nstreams} streams)")
     for _ in range(args.warmup):
         rec.reconstruct(raw_buf.value, out_buf.value, B)
     _lib.call("fsim_sync")
-    _lib.call("fsim_stage_timing", 1)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         rec.reconstruct(raw_buf.value, out_buf.value, B)
     _lib.call("fsim_sync")
     t1 = time.perf_counter()
-    _lib.call("fsim_stage_timing", 0)
-    import ctypes as C
-    ms = (C.c_double * 5)()
-    cnt = (C.c_int * 5)()
-    _lib.call("fsim_stage_times", ms, cnt, 5)
     elapsed = t1 - t0
     ms_per_step = 1000.0 * elapsed / args.steps
     value = B * args.steps / elapsed
     log(f"batched: {value:.1f} recons/s, {ms_per_step:.3f} ms/step")
+    import ctypes as C
+
+    def stage_timed(steps):
+        """`steps` calls with a HIP event around every kernel on its launch
+        stream: (recons/s, ms per launch and launches per stage)."""
+        _lib.call("fsim_stage_timing", 1)
+        a = time.perf_counter()
+        for _ in range(steps):
+            rec.reconstruct(raw_buf.value, out_buf.value, B)
+        _lib.call("fsim_sync")
+        b = time.perf_counter()
+        _lib.call("fsim_stage_timing", 0)
+        ms = (C.c_double * 5)()
+        cnt = (C.c_int * 5)()
+        _lib.call("fsim_stage_times", ms, cnt, 5)
+        return B * steps / (b - a), [ms[i] / max(cnt[i], 1) for i in range(5)], [int(c) for c in cnt]
 
     half = rec.p.mode == 0                    # real estimated OTF: the pairing is used in mode 0
     cs = 2 if half else 3
     sb = stage_bytes(n, Cs=cs, half=half, dc_sum=True)
     names = stage_names(half)
-    stage_ms = [ms[i] / max(cnt[i], 1) for i in range(5)]          # per launch (= per batch)
+
+    def stage_table(stage_ms, cnt, k):
+        return {names[i]: {"ms_per_launch": round(stage_ms[i], 4), "launches": cnt[i], "stacks_per_launch": k,
+                           "bytes_per_stack": sb[i],
+                           "GBps": round(sb[i] * k / (stage_ms[i] * 1e-3) / 1e9, 1) if stage_ms[i] else None}
+                for i in range(5)}
+
+    # the timed configuration with kernel events (concurrent chunks: a
+    # kernel's event time includes the other stream's share of the chip)
+    value_timed, ms_chunked, cnt_chunked = stage_timed(args.steps)
+    stages_chunked = stage_table(ms_chunked, cnt_chunked, per_launch)
+    # the kernels alone: the same B stacks as one launch on one stream, each
+    # kernel's event time is its own execution; the roofline of the dominant
+    # kernel comes from this region (profiles/ holds rocprofv3 of the same)
+    autotune.set_chunks(*cls, 0, 1)
+    for _ in range(10):
+        rec.reconstruct(raw_buf.value, out_buf.value, B)
+    value_whole, stage_ms, cnt_whole = stage_timed(args.steps)
+    if tuned:
+        autotune.set_chunks(*cls, tuned["chunk"], tuned["streams"])
+    else:
+        autotune.set_chunks(*cls, -1, 2)
     dom = int(np.argmax(stage_ms))
     dom_bytes = sb[dom] * B
     achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
-    stages = {names[i]: {"ms_per_launch": round(stage_ms[i], 4),
-                          "GBps": round(sb[i] * B / (stage_ms[i] * 1e-3) / 1e9, 1) if stage_ms[i] else None}
-              for i in range(5)}
+    stages = stage_table(stage_ms, cnt_whole, B)
     b_alg = sum(stage_bytes(n))               # reference dataflow: 434 n^2
     pipeline_frac = value * b_alg / (HBM_PEAK_GBS * 1e9)
     hbm_frac_moved = value * sum(sb) / (HBM_PEAK_GBS * 1e9)
@@ -961,38 +1021,48 @@ def main():
         log("served: open-loop mixed stream at 200 req/s")
         served["mixed_stream"] = serve_mixed_stream(args.mix_requests, args.mix_rate)
 
-    configs = None
-    if not args.no_configs:
-        log("config 1: parameter estimation")
-        configs = {"estimation_512": estimation()}
-        log("config 2: mixed batch of 96 stacks")
-        configs["batched_mixed_96"] = batched_mixed()
-        log("config 4: 2048^2 large field of view")
-        configs["large_fov_2048"] = large_fov()
-        log("config 4 served: large-FOV requests in the config-3 stream")
-        configs["served_large_fov_mix"] = serve_large_fov_mix()
+    workloads = None
+    if not args.no_workloads:
+        log("512^2 parameter estimation")
+        workloads = {"estimation_512": estimation()}
+        log("mixed batch of 96 stacks (256/512/1024, 3 channels)")
+        workloads["batched_mixed_96"] = batched_mixed()
+        log("2048^2 large field of view at 65 nm")
+        workloads["large_fov_2048"] = large_fov()
+        log("served: large-FOV requests in the mixed Poisson stream")
+        workloads["served_large_fov_mix"] = serve_large_fov_mix()
         log("live stream ingest: packets -> sequences -> server")
-        configs["stream_ingest_512"] = stream_ingest()
+        workloads["stream_ingest_512"] = stream_ingest()
         log("live-mode side outputs: widefield, registration")
-        configs["side_outputs_512"] = side_outputs()
+        workloads["side_outputs_512"] = side_outputs()
         log("filter styles: RL deconvolution, runReconstruction per style")
-        configs["filter_styles_512"] = filter_styles()
+        workloads["filter_styles_512"] = filter_styles()
         log("configuration files: load, channel set-up")
-        configs["config_io"] = config_io()
+        workloads["config_io"] = config_io()
 
     # CPU baseline: the oracle restatement on the host cores, bounded sample
     cpu = None
     if not args.no_cpu:
+        def cpu_rate(nthreads, seconds, first):
+            runs = min(200, max(3, int(seconds / max(first, 1e-3))))
+            a = time.perf_counter()
+            for i in range(runs):
+                ref.recon32(p32, stacks[i % len(stacks)], threads=nthreads)
+            return runs, time.perf_counter() - a
         log(f"cpu baseline, {threads} threads")
-        runs = max(3, int(args.cpu_seconds / max(first_cpu, 1e-3)))
-        runs = min(runs, 200)
-        a = time.perf_counter()
-        for i in range(runs):
-            ref.recon32(p32, stacks[i % 4], threads=threads)
-        cpu_el = time.perf_counter() - a
+        runs, cpu_el = cpu_rate(threads, args.cpu_seconds, first_cpu)
         cpu = {"value": round(runs / cpu_el, 3), "unit": "recons/s", "cores": threads, "kind": "port",
-               "sample": f"{runs} reconstructions of the same 512^2 3x3 stacks, FP32 Java-order C "
-                         f"restatement (oracle/java_order.c), OpenMP {threads} threads, {cpu_el:.1f} s"}
+               "sample": f"{runs} reconstructions of the 512^2 3x3 stacks of the GPU workload, FP32 Java-order "
+                         f"C restatement of the reference CPU path (oracle/java_order.c, not the JVM), "
+                         f"OpenMP {threads} threads, {cpu_el:.1f} s"}
+        if threads > 1:
+            log("cpu baseline, 1 thread")
+            a = time.perf_counter()
+            ref.recon32(p32, stacks[0], threads=1)
+            first1 = time.perf_counter() - a
+            runs1, el1 = cpu_rate(1, args.cpu_seconds, first1)
+            cpu["one_core"] = {"value": round(runs1 / el1, 3), "unit": "recons/s", "cores": 1,
+                               "sample": f"{runs1} reconstructions, 1 thread, {el1:.1f} s"}
 
     res = {
         "metric": "SIM recons/s + p50/p99 latency, 512² 3×3 raw; served vs plain GPU",
@@ -1012,19 +1082,27 @@ def main():
                    "n": n, "angles": 3, "phases": 3, "bands": 2, "batch": B},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(names[dom], n, B)},
-        "pipeline_roofline_frac": round(pipeline_frac, 4),
-        "alg_bytes_per_recon": b_alg,
+                     "traffic": profiled_traffic(names[dom], n, B),
+                     "bytes_per_launch": dom_bytes, "stacks_per_launch": B,
+                     "ms_per_launch": round(stage_ms[dom], 4),
+                     "measured": f"HIP events around each kernel, {B} stacks as one launch on one stream "
+                                 f"(the kernel alone on the GPU)"},
+        "pipeline_executed_bytes_frac": round(hbm_frac_moved, 4),
         "bytes_moved_per_recon": sum(sb),
-        "hbm_frac_of_bytes_moved": round(hbm_frac_moved, 4),
+        "reference_dataflow_equivalent_frac": round(pipeline_frac, 4),
+        "reference_dataflow_bytes_per_recon": b_alg,
+        "chunking": {"stacks_per_launch": per_launch, "streams": nstreams},
         "stages": stages,
+        "one_launch": {"recons_per_s_with_kernel_events": round(value_whole, 2)},
+        "stages_in_timed_configuration": stages_chunked,
+        "value_with_kernel_events": round(value_timed, 2),
         "autotune": tuned,
         "latency_single_ms": {"p50": round(pct(lat, 50), 3), "p99": round(pct(lat, 99), 3)},
         "latency_batch_ms": {"p50": round(pct(blat, 50), 3), "p99": round(pct(blat, 99), 3)},
         "plain": {"recons_per_s": round(plain_value, 2), "p50_ms": round(pct(plat, 50), 3),
                   "p99_ms": round(pct(plat, 99), 3), "rel_l2_vs_oracle": plain_err},
         "served": served,
-        "configs": configs,
+        "workloads": workloads,
         "speedup_vs_plain": round(value / plain_value, 2),
         "served_speedup_vs_plain": round(served["recons_per_s"] / plain_value, 2),
         "rel_l2_vs_oracle": parity,

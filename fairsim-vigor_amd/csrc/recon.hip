@@ -1155,13 +1155,18 @@ __global__ void k_wiener_multi(float *__restrict__ wa, const float *__restrict__
 }
 
 // Optional stage timing: HIP events recorded around each pipeline kernel on
-// the launch stream, summed per stage when read back.
+// the launch stream, summed per stage when read back.  The events come from
+// a pool created when timing is switched on (creating an event per launch
+// would slow the launches being timed); marks beyond the pool are skipped.
 struct StageTimer {
     std::mutex mu;
     bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
 };
 static StageTimer g_timer;
+static constexpr size_t kTimerEvents = 16384;
 
 // Set while fsim_reconstruct_noalloc launches: the pipeline then only looks
 // up existing twiddle tables and records no timing events.
@@ -1171,13 +1176,14 @@ bool stage_timing_enabled() { return g_timer.on; }
 
 static void stage_mark(int stage, hipStream_t s, hipEvent_t &start) {
     if (!g_timer.on || t_noalloc) return;
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return;
-    (void)hipEventRecord(e, s);
-    if (stage >= 0) {
-        std::lock_guard<std::mutex> lk(g_timer.mu);
-        g_timer.ev.push_back({stage, {start, e}});
+    std::lock_guard<std::mutex> lk(g_timer.mu);
+    if (g_timer.used >= g_timer.pool.size()) {
+        start = nullptr;
+        return;
     }
+    hipEvent_t e = g_timer.pool[g_timer.used++];
+    (void)hipEventRecord(e, s);
+    if (stage >= 0 && start) g_timer.ev.push_back({stage, {start, e}});
     start = e;
 }
 
@@ -1952,6 +1958,17 @@ int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw, float *out, in
 
 int fsim_stage_timing(int enable) {
     std::lock_guard<std::mutex> lk(g_timer.mu);
+    if (enable && g_timer.pool.empty()) {
+        g_timer.pool.resize(kTimerEvents, nullptr);
+        for (auto &e : g_timer.pool)
+            if (hipEventCreate(&e) != hipSuccess) {
+                (void)hipGetLastError();
+                for (auto &x : g_timer.pool)
+                    if (x) (void)hipEventDestroy(x);
+                g_timer.pool.clear();
+                return fail(FSIM_ERR_HIP, "stage_timing: event creation failed");
+            }
+    }
     g_timer.on = enable != 0;
     return FSIM_OK;
 }
@@ -1963,18 +1980,16 @@ int fsim_stage_times(double *ms, int *launches, int nstage) {
         ms[i] = 0;
         launches[i] = 0;
     }
-    std::set<hipEvent_t> owned;
     for (auto &e : g_timer.ev) {
         FSIM_HIP(hipEventSynchronize(e.second.second));
         float t = 0;
         FSIM_HIP(hipEventElapsedTime(&t, e.second.first, e.second.second));
         ms[e.first] += t;
         launches[e.first] += 1;
-        owned.insert(e.second.first);
-        owned.insert(e.second.second);
     }
-    for (auto e : owned) (void)hipEventDestroy(e);
+    // the pool's events are reused by the next timed launches
     g_timer.ev.clear();
+    g_timer.used = 0;
     return FSIM_OK;
 }
 

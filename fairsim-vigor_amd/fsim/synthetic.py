@@ -64,12 +64,15 @@ def _ifft2(a):
 
 def sim_stack(n=512, ndir=3, nphase=3, na=1.4, lam=525.0, px_um=0.080, k_frac=0.85,
               mod=0.7, angle0_deg=3.0, peak=1000.0, bg=100.0, offset=100.0, seed=0,
-              nbeads=300, nfil=40, harmonic2=0.0):
+              nbeads=300, nfil=40, harmonic2=0.0, timepoints=1):
     """Return (raw uint16 [ndir*nphase, n, n], params dict).
 
     params: 'px','py' (band-1 shift per direction, reference convention),
     'phases' (per direction, absolute), 'modul', 'k0' (|k| in raw pixels),
-    'cutoff_px'.  harmonic2 > 0 adds a 2k component (three-beam data)."""
+    'cutoff_px'.  harmonic2 > 0 adds a 2k component (three-beam data).
+    timepoints > 1: raw is [timepoints, ndir*nphase, n, n], the same sample
+    imaged repeatedly with independent seeded photon noise (the first
+    timepoint equals the single stack of the same seed)."""
     rng = np.random.default_rng(seed)
     nf = 2 * n
     fine_um = px_um / 2
@@ -103,11 +106,17 @@ def sim_stack(n=512, ndir=3, nphase=3, na=1.4, lam=525.0, px_um=0.080, k_frac=0.
     frames = np.array(frames)
     frames = np.clip(frames, 0, None)
     frames = frames * (peak / frames.max()) + bg
-    noisy = rng.poisson(frames).astype(np.float64) + offset
-    raw = np.clip(noisy, 0, 65535).astype(np.uint16)
+    raws = [_camera(rng, frames, offset) for _ in range(timepoints)]
+    raw = raws[0] if timepoints == 1 else np.stack(raws)
     params = dict(px=pxs, py=pys, phases=phs, modul=mod, k0=k0,
                   cutoff_px=cutoff / cpm_raw, na=na, lam=lam, px_um=px_um)
     return raw, params
+
+
+def _camera(rng, frames, offset):
+    """Poisson photon counts plus the camera offset, as uint16."""
+    noisy = rng.poisson(frames).astype(np.float64) + offset
+    return np.clip(noisy, 0, 65535).astype(np.uint16)
 
 
 def make_param(sp, n, ndir=3, nphase=3, nband=2, na=1.4, lam=525.0, px_um=0.080, a=1.0,

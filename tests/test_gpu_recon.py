@@ -256,7 +256,7 @@ def test_reconstruct_params_equals_own_channel(fsim_lib):
 
 
 def test_estimate_then_reconstruct_matches_oracle(fsim_lib):
-    """Config 1 flow: parameters estimated per stack (fsim_estimate), then
+    """Estimate-then-reconstruct flow: parameters estimated per stack (fsim_estimate), then
     every stack reconstructed with its own estimate (fsim_reconstruct_params);
     compared with the FP32 oracle reconstructing with the same parameters."""
     from fsim.linalg import DeviceBuffer
