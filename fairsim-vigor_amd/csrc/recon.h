@@ -39,6 +39,10 @@ struct ReconConst {
     // real scale (the same values as the complex product without its zero
     // terms)
     int otf_real;
+    // columns x of the n x n grid with zlo <= x <= zhi lie beyond the OTF
+    // cutoff along x (|xh| * vec_cpm > cutoff, so every OTF value of the
+    // column is zero): the column pass writes zeros there (empty: zlo > zhi)
+    int zlo, zhi;
 };
 
 // One stack: channel constants, raw input, work buffers and output.

@@ -558,6 +558,11 @@ __global__ void __launch_bounds__((ColShiftTile<N, E0, CW0, WC>::value), (ColShi
     if (FSIM_TILE_GROUPS) {
         constexpr int TG = tiles < 8 ? tiles : 8;
         tile_group_order<TG>(w, total / tiles, u, tile);
+        // group g holds tiles g, g + G, g + 2G, ... (G = tiles / TG groups):
+        // each XCD's share spans the whole column range, so the zero tiles
+        // beyond the OTF cutoff (a contiguous range) spread evenly over the
+        // XCDs instead of emptying one XCD's share
+        tile = tile / TG + (tiles / TG) * (tile % TG);
     } else {
         tile = w % tiles;
         u = w / tiles;
@@ -574,6 +579,13 @@ __global__ void __launch_bounds__((ColShiftTile<N, E0, CW0, WC>::value), (ColShi
     constexpr int LS = WC ? 1 : CW;
     const int x = tile * CW + cl;
     FSIM_GLB float2 *col = glb(J.A) + (size_t)(d * C + c) * N * N + at_col<N>(x);
+    if (tile * CW >= K.zlo && tile * CW + CW - 1 <= K.zhi) {
+        // every column of the tile lies beyond the OTF cutoff: the product
+        // with conj(OTF) is zero, and so are its transforms
+#pragma unroll
+        for (int m = 0; m < E; m++) st(col + at_row<N>(t + T * m), make_float2(0.f, 0.f));
+        return;
+    }
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = ld(col + at_row<N>(t + T * m));
@@ -1436,6 +1448,18 @@ int build_channel_const(const fsim_recon_params &p, const Otf &o, bool sym, bool
     K.Cs = sym ? B : C;
     K.half = sym ? 1 : 0;
     K.otf_real = otf_real ? 1 : 0;
+    {
+        // |xh| > R with R = cutoff / vec_cpm (a relative margin of 1e-9 keeps
+        // columns whose radius rounds to the cutoff): otfToVector writes zero
+        // for every row of such a column (OtfProvider.java:352)
+        const double R = o.cutoff / p.vec_cpm * (1 + 1e-9);
+        K.zlo = 1;
+        K.zhi = 0;
+        if (std::isfinite(R) && R >= 0 && R < n / 2) {
+            K.zlo = (int)std::floor(R) + 1;
+            K.zhi = n - K.zlo;
+        }
+    }
     for (int j = 0; j < kMaxComp; j++) K.srow[j] = K.sband[j] = K.sramp[j] = K.src[j] = 0;
     for (int j = 0; j < K.Cs; j++) {
         K.srow[j] = sym ? 2 * j : j;          // DC, then pos of band j
