@@ -292,6 +292,25 @@ def clip_and_scale(img, clip, scale):
     return img
 
 
+def sim_otf64(p, vals):
+    """The effective SIM OTF of runReconstruction's RL-on-output branch
+    (SimAlgorithm.java:592-615) on the (2n)^2 grid: per direction band 0 at
+    weight 1/D, every higher band b at +-(px, py) * b (writeOtfVector, the
+    unattenuated table)."""
+    n, D, B = p["n"], p["ndir"], p["nband"]
+    cpm, cut, vcpm = p["otf_cpm"], p["otf_cutoff"], p["vec_cpm"]
+    sim = np.zeros((2 * n, 2 * n), dtype=np.complex128)
+    for d in range(D):
+        for b in range(B):
+            bx, by = p["px"][d] * b, p["py"][d] * b
+            if b == 0:
+                sim = sim + otf_to_vector(2 * n, 2 * n, vals, cpm, cut, vcpm, bx, by) / D
+            else:
+                sim = sim + otf_to_vector(2 * n, 2 * n, vals, cpm, cut, vcpm, bx, by)
+                sim = sim + otf_to_vector(2 * n, 2 * n, vals, cpm, cut, vcpm, -bx, -by)
+    return sim
+
+
 def run_reconstruction64(p, in_fft, vals, filter_style="Wiener", otf_before_shift=False, rl_iterations=5,
                          clip_scale="NONE"):
     """SimAlgorithm.runReconstruction (SimAlgorithm.java:268-709, no
@@ -346,16 +365,7 @@ def run_reconstruction64(p, in_fft, vals, filter_style="Wiener", otf_before_shif
         apo = apo_vector(2 * n, 2 * n, vcpm, cut, 0.9, p["apo_cut"])
         img = ifft2(full * wien * apo).real
     if use_rout:
-        sim = np.zeros((2 * n, 2 * n))
-        for d in range(D):
-            for b in range(B):
-                bx, by = p["px"][d] * b, p["py"][d] * b
-                if b == 0:
-                    sim = sim + otf_to_vector(2 * n, 2 * n, vals, cpm, cut, vcpm, bx, by) / D
-                else:
-                    sim = sim + otf_to_vector(2 * n, 2 * n, vals, cpm, cut, vcpm, bx, by)
-                    sim = sim + otf_to_vector(2 * n, 2 * n, vals, cpm, cut, vcpm, -bx, -by)
-        img = ifft2(rl_deconvolve64(full, sim, rl_iterations, True)).real
+        img = ifft2(rl_deconvolve64(full, sim_otf64(p, vals), rl_iterations, True)).real
     if clip_scale in ("CLIP", "BOTH"):
         img = clip_and_scale(img, True, clip_scale == "BOTH")
     return img
