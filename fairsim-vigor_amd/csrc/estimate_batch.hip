@@ -91,7 +91,12 @@ __global__ void k_eb_init(EstGroup *__restrict__ g, int G) {
 // the row pass of the 2-D FFT (the same transform code as
 // fft2d_batch_device's row pass, hence the same values).  One row of one
 // frame per T = n/E threads.
-template <int L>
+// LIVE: the live refit's preparation instead (ReconstructionRunner.java:
+// 614-617, 661-664): the image times the float vector borderDampen, a vector
+// of ones faded by fadeBorderCos -- per pixel
+// (float)((float)((float)((float)(1 * lo[y]) * hi[y]) * lo[x]) * hi[x]),
+// each factor applied as `float *= double` -- in one float product.
+template <int L, bool LIVE>
 __global__ void __launch_bounds__(256) k_eb_load_rows(const float *__restrict__ img, float2 *__restrict__ X,
                                                       int nrows, const double *__restrict__ lo,
                                                       const double *__restrict__ hi) {
@@ -114,11 +119,20 @@ __global__ void __launch_bounds__(256) k_eb_load_rows(const float *__restrict__ 
 #pragma unroll
         for (int m = 0; m < E; m++) {
             const int x = t + T * m;
-            float e = (float)((double)d[m] * ly);
-            e = (float)((double)e * hy);
-            e = (float)((double)e * lo[x]);
-            e = (float)((double)e * hi[x]);
-            v[m] = make_float2(e, 0.f);
+            if constexpr (LIVE) {
+                float b = 1.0f;
+                b = (float)((double)b * ly);
+                b = (float)((double)b * hy);
+                b = (float)((double)b * lo[x]);
+                b = (float)((double)b * hi[x]);
+                v[m] = make_float2(d[m] * b, 0.f);
+            } else {
+                float e = (float)((double)d[m] * ly);
+                e = (float)((double)e * hy);
+                e = (float)((double)e * lo[x]);
+                e = (float)((double)e * hi[x]);
+                v[m] = make_float2(e, 0.f);
+            }
         }
     }
     fft_regs<L, E, false, RowWave<L>::value, TW::tl>(v, lds + rr * LP, 1, t, nullptr, active);
@@ -130,27 +144,32 @@ __global__ void __launch_bounds__(256) k_eb_load_rows(const float *__restrict__ 
 }
 
 template <int L>
-static int launch_load_rows_t(const float *img, float2 *X, int nrows, const double *lo, const double *hi,
+static int launch_load_rows_t(const float *img, float2 *X, int nrows, const double *lo, const double *hi, bool live,
                               hipStream_t s) {
     constexpr int E = FftE<L>::value, T = L / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
-    hipLaunchKernelGGL(k_eb_load_rows<L>, dim3(cdiv(nrows, RW)), dim3(T * RW), 0, s, img, X, nrows, lo, hi);
+    if (live)
+        hipLaunchKernelGGL((k_eb_load_rows<L, true>), dim3(cdiv(nrows, RW)), dim3(T * RW), 0, s, img, X, nrows, lo,
+                           hi);
+    else
+        hipLaunchKernelGGL((k_eb_load_rows<L, false>), dim3(cdiv(nrows, RW)), dim3(T * RW), 0, s, img, X, nrows, lo,
+                           hi);
     FSIM_HIP(hipGetLastError());
     return FSIM_OK;
 }
 
 static int launch_load_rows(int L, const float *img, float2 *X, int nrows, const double *lo, const double *hi,
-                            hipStream_t s) {
+                            bool live, hipStream_t s) {
     switch (L) {
-        case 16: return launch_load_rows_t<16>(img, X, nrows, lo, hi, s);
-        case 32: return launch_load_rows_t<32>(img, X, nrows, lo, hi, s);
-        case 64: return launch_load_rows_t<64>(img, X, nrows, lo, hi, s);
-        case 128: return launch_load_rows_t<128>(img, X, nrows, lo, hi, s);
-        case 256: return launch_load_rows_t<256>(img, X, nrows, lo, hi, s);
-        case 512: return launch_load_rows_t<512>(img, X, nrows, lo, hi, s);
-        case 1024: return launch_load_rows_t<1024>(img, X, nrows, lo, hi, s);
-        case 2048: return launch_load_rows_t<2048>(img, X, nrows, lo, hi, s);
-        case 4096: return launch_load_rows_t<4096>(img, X, nrows, lo, hi, s);
+        case 16: return launch_load_rows_t<16>(img, X, nrows, lo, hi, live, s);
+        case 32: return launch_load_rows_t<32>(img, X, nrows, lo, hi, live, s);
+        case 64: return launch_load_rows_t<64>(img, X, nrows, lo, hi, live, s);
+        case 128: return launch_load_rows_t<128>(img, X, nrows, lo, hi, live, s);
+        case 256: return launch_load_rows_t<256>(img, X, nrows, lo, hi, live, s);
+        case 512: return launch_load_rows_t<512>(img, X, nrows, lo, hi, live, s);
+        case 1024: return launch_load_rows_t<1024>(img, X, nrows, lo, hi, live, s);
+        case 2048: return launch_load_rows_t<2048>(img, X, nrows, lo, hi, live, s);
+        case 4096: return launch_load_rows_t<4096>(img, X, nrows, lo, hi, live, s);
         default: return fail(FSIM_ERR_ARG, "estimate: unsupported size");
     }
 }
@@ -759,9 +778,11 @@ __global__ void __launch_bounds__(128) k_eb_step(EstPair *__restrict__ pairs, co
 // getPeak pairs from the fitted peak (SimAlgorithm.java:118-159): two bands,
 // getPeak(sep0, sep1, 0, 1, peak); three bands, the peak doubled when fitted
 // on band 1, then getPeak(sep0, sep_lb, 0, 1, peak / 2) and
-// getPeak(sep0, sep_hb, 0, 2, peak).
+// getPeak(sep0, sep_hb, 0, 2, peak).  live: the refit's single getPeak of
+// band 1 at peak / (bands - 1) (ReconstructionRunner.java:722-724; its
+// second getPeak only feeds a trace line).
 __global__ void k_eb_gp_init(const EstPair *__restrict__ fit, EstPair *__restrict__ gp, int G, int nband,
-                             int fit_band, double *__restrict__ shift) {
+                             int fit_band, int live, double *__restrict__ shift) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= G) return;
     double kx = fit[g].kx, ky = fit[g].ky;
@@ -772,7 +793,13 @@ __global__ void k_eb_gp_init(const EstPair *__restrict__ fit, EstPair *__restric
     p.err = 0;
     p.nsel = 1;
     p.sel[0] = 0;
-    if (nband == 3) {
+    if (live) {
+        p.kx = kx / (nband - 1);
+        p.ky = ky / (nband - 1);
+        p.slot1 = 1;
+        p.bn1 = 1;
+        gp[g] = p;
+    } else if (nband == 3) {
         if (fit_band == 1) {
             kx *= 2;
             ky *= 2;
@@ -801,7 +828,7 @@ __global__ void k_eb_gp_init(const EstPair *__restrict__ fit, EstPair *__restric
 // Per-group results: setPxPy(-peak), setPhaOff(phase of p1), modulations |p1|
 // (, |p2|) as Cplx.Double.abs = hypot (Cplx.java:49).
 __global__ void k_eb_finish(const EstPair *__restrict__ fit, const EstPair *__restrict__ gp, int G, int npk,
-                            const double *__restrict__ peaks, const double *__restrict__ shift,
+                            int live, const double *__restrict__ peaks, const double *__restrict__ shift,
                             fsim_estimate_result *__restrict__ out, int *__restrict__ err) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= G) return;
@@ -811,7 +838,7 @@ __global__ void k_eb_finish(const EstPair *__restrict__ fit, const EstPair *__re
     r.py = shift[2 * g + 1];
     r.pha_off = atan2(p1.im, p1.re);
     r.mod1 = hypot(p1.re, p1.im);
-    r.mod2 = 0.0;
+    r.mod2 = live ? __builtin_nan("") : 0.0;
     int e = fit[g].err | p1.err;
     if (npk == 2) {
         const EstPair &p2 = gp[(size_t)g * npk + 1];
@@ -885,14 +912,19 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     if (!fft_size_ok(n) || n < 16) return fail(FSIM_ERR_ARG, "estimate: n must be a power of two >= 16");
     if (reinterpret_cast<uintptr_t>(images) % 16) return fail(FSIM_ERR_ARG, "estimate: images must be 16-byte aligned");
     if (B != 2 && B != 3) return fail(FSIM_ERR_ARG, "estimate: 2 or 3 bands");
-    if (p->fit_band != 1 && p->fit_band != 2) return fail(FSIM_ERR_ARG, "estimate: Fitband neither 1 nor 2");
+    const bool live = p->live != 0;
+    if (p->live != 0 && p->live != 1) return fail(FSIM_ERR_ARG, "estimate: live must be 0 or 1");
+    if (!live && p->fit_band != 1 && p->fit_band != 2) return fail(FSIM_ERR_ARG, "estimate: Fitband neither 1 nor 2");
+    // the live refit fits the highest band with OTF band number 1
+    const int fit_band = live ? (B == 3 ? 2 : 1) : p->fit_band;
+    const int fit_otf_band = live ? 1 : p->fit_band;
     if (D < 1 || P < 2 * B - 1 || P > kMaxPhase) return fail(FSIM_ERR_ARG, "estimate: bad direction/phase count");
     if (!(p->vec_cpm > 0)) return fail(FSIM_ERR_STATE, "estimate: vector pixel size not set");
     if (p->fade_px < 0 || 2 * p->fade_px > n) return fail(FSIM_ERR_ARG, "estimate: fade border too wide");
     const Otf &o = otf->o;
-    if (o.nband != 1 && (p->fit_band >= o.nband || B - 1 >= o.nband))
+    if (o.nband != 1 && (fit_otf_band >= o.nband || B - 1 >= o.nband))
         return fail(FSIM_ERR_ARG, "estimate: band idx too high for the OTF");
-    const int G = count * D, npk = B == 3 ? 2 : 1;
+    const int G = count * D, npk = (B == 3 && !live) ? 2 : 1;
     const size_t nn = (size_t)n * n;
     if ((size_t)G * P > 65535 || (size_t)2 * npk * G > 65535)
         return fail(FSIM_ERR_ARG, "estimate: batch too large (at most 65535 images per transform)");
@@ -905,7 +937,7 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     EbSep sep{};
     sep.nphase = P;
     sep.ncomp = B == 3 ? 3 : 2;
-    sep.fb = (B == 3 && p->fit_band == 2) ? 2 : 1;
+    sep.fb = (B == 3 && fit_band == 2) ? 2 : 1;
     const int comps[3] = {0, 1, 3};
     for (int c = 0; c < sep.ncomp; c++)
         for (int q = 0; q < P; q++) {
@@ -960,7 +992,7 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     hipLaunchKernelGGL(k_eb_init, dim3(gb), dim3(64), 0, s, grp, G);
     hipLaunchKernelGGL(k_eb_dc_tab, dim3((unsigned)std::min<size_t>((ndc * nn + 255) / 256, 4096)), dim3(256), 0, s,
                        dct, n, reg, ndc);
-    rc = launch_load_rows(n, images, g_eb.X, G * P * n, fade, fade + n, s);
+    rc = launch_load_rows(n, images, g_eb.X, G * P * n, fade, fade + n, live, s);
     if (rc) return rc;
     rc = fft_cols_batch_device(g_eb.X, n, n, G * P, 0, s);
     if (rc) return rc;
@@ -976,7 +1008,7 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     const double kmin = p->fit_exclude * o.cutoff / p->vec_cpm;
     const unsigned pkb = cdiv(nn, 256 * 8);
     hipLaunchKernelGGL(k_eb_peak, dim3(pkb, G), dim3(256), 0, s, g_eb.W + (size_t)G * nn, n, kmin, grp);
-    hipLaunchKernelGGL(k_eb_fit_init, dim3(gb), dim3(64), 0, s, grp, fit, G, n, sep.fb, p->fit_band, peaks);
+    hipLaunchKernelGGL(k_eb_fit_init, dim3(gb), dim3(64), 0, s, grp, fit, G, n, sep.fb, fit_otf_band, peaks);
     FSIM_HIP(hipGetLastError());
     // fitPeak: three rounds of commonRegion, inverse FFTs, candidate filter, argmax
     for (int iter = 0; iter < 3; iter++) {
@@ -984,11 +1016,12 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
         if (rc) return rc;
     }
     // getPeak of band 1 (and band 2)
-    hipLaunchKernelGGL(k_eb_gp_init, dim3(gb), dim3(64), 0, s, fit, gp, G, B, p->fit_band, shift);
+    hipLaunchKernelGGL(k_eb_gp_init, dim3(gb), dim3(64), 0, s, fit, gp, G, B, fit_band, live ? 1 : 0, shift);
     FSIM_HIP(hipGetLastError());
     rc = eb_round(reg, dct, gp, npk * G, false, P, n, nchunk, part, tabs, apart, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_eb_finish, dim3(gb), dim3(64), 0, s, fit, gp, G, npk, peaks, shift, resd, errd);
+    hipLaunchKernelGGL(k_eb_finish, dim3(gb), dim3(64), 0, s, fit, gp, G, npk, live ? 1 : 0, peaks, shift, resd,
+                       errd);
     FSIM_HIP(hipGetLastError());
     std::vector<int> err(G);
     FSIM_HIP(hipMemcpyAsync(res, resd, G * sizeof(fsim_estimate_result), hipMemcpyDeviceToHost, s));
@@ -1013,7 +1046,7 @@ extern "C" int fsim_params_from_estimate(const fsim_estimate_result *res, int co
             p.py[d] = r.py / (nband - 1);
             for (int ph = 0; ph < nphase; ph++) p.phases[d][ph] = base->phases[d][ph] + r.pha_off;
             p.modul[d][1] = r.mod1;
-            if (nband == 3) p.modul[d][2] = r.mod2;
+            if (nband == 3 && !std::isnan(r.mod2)) p.modul[d][2] = r.mod2;
         }
         out[s] = p;
     }

@@ -64,7 +64,7 @@ class ESTIMATE_PARAMS(C.Structure):
     """fsim_estimate_params (include/fsim.h)."""
     _fields_ = [("n", C.c_int), ("ndir", C.c_int), ("nphase", C.c_int), ("nband", C.c_int),
                 ("fit_band", C.c_int), ("fit_exclude", C.c_double), ("vec_cpm", C.c_double),
-                ("fade_px", C.c_int)]
+                ("fade_px", C.c_int), ("live", C.c_int)]
 
 
 class ESTIMATE_RESULT(C.Structure):

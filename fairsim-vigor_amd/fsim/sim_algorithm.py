@@ -522,7 +522,68 @@ def estimate_parameters(param, inFFT, fitBand, fitExclude):
     return results
 
 
-def estimate_batch(param, fitBand, fitExclude, images=None, images_dev=None, count=None, fade_px=10, raw=False):
+def refit_parameters(param, raw_stack, fitExclude=0.8, fade_px=10):
+    """ReconstructionRunner.ParameterRefitThread.doRefit
+    (livemode/ReconstructionRunner.java:642-746) over device vectors: the
+    live-mode refit of one channel from its latest raw stack (uint16 [D*P,
+    N, N], direction-major).  Each frame is set from its 16-bit pixels,
+    multiplied by borderDampen (a vector of ones faded by fadeBorderCos) and
+    transformed; per direction the bands are separated with equidistant
+    phases, the coarse correlation of band 0 with the highest band is
+    searched outside fitExclude * cutoff (0.8 in the reference), fitPeak
+    refines it with OTF band numbers (0, 1) whatever the band, and getPeak of
+    band 1 at peak / (bands - 1) gives the phase offset and modulation 1;
+    modulation 2 is left as it was.  ``param`` receives setPxPy, setPhaOff
+    and setModulation(1) per direction; returns the per-direction results."""
+    raw_stack = np.ascontiguousarray(raw_stack, dtype=np.uint16)
+    D, P = param.nrDir(), param.dir(0).nrPha()
+    n = param.vectorWidth()
+    otfPr = param.otf()
+    borderDampen = Vec2dReal(n, n)
+    borderDampen.addConst(1)
+    fade_border_cos(borderDampen, fade_px)
+    inFFT = []
+    count = 0
+    for a in range(D):
+        row = []
+        for _ in range(P):
+            v = Vec2dCplx(n, n)
+            v.setFrom16bitPixels(raw_stack[count])
+            count += 1
+            v.times(borderDampen)
+            v.fft2d(False)
+            row.append(v)
+        inFFT.append(row)
+    otfAtt = Vec2dReal(n, n)
+    otfPr.writeAttenuationVector(otfAtt, .99, 0.15 * otfPr.getCutoff(), 0, 0)
+    results = []
+    for angIdx in range(D):
+        par = param.dir(angIdx)
+        separate = [Vec2dCplx(n, n) for _ in range(par.nrComp())]
+        separate_bands(inFFT[angIdx], separate, [(2 * math.pi * p) / P + 0.0 for p in range(P)], par.nrBand(), None)
+        lb = 1
+        hb = 3 if par.nrBand() == 3 else 1
+        nBand = par.nrBand() - 1
+        c0, c1 = separate[0].duplicate(), separate[hb].duplicate()
+        c0.times(otfAtt)
+        c1.times(otfAtt)
+        c0.fft2d(True)
+        c1.fft2d(True)
+        c1.timesConj(c0)
+        c1.fft2d(False)
+        minDist = fitExclude * otfPr.getCutoff() / param.pxlSizeCyclesMicron()
+        peak = Correlation.locatePeak(c1, minDist)
+        peak = Correlation.fitPeak(separate[0], separate[hb], 0, 1, otfPr, -peak[0], -peak[1], 0.05, 2.5)
+        p1 = Correlation.getPeak(separate[0], separate[lb], 0, 1, otfPr, peak[0] / nBand, peak[1] / nBand, 0.05)
+        par.setPxPy(-peak[0], -peak[1])
+        par.setPhaOff(math.atan2(p1.imag, p1.real))
+        par.setModulation(1, abs(p1))
+        results.append(dict(px=-peak[0], py=-peak[1], pha_off=math.atan2(p1.imag, p1.real), mod1=abs(p1)))
+    return results
+
+
+def estimate_batch(param, fitBand, fitExclude, images=None, images_dev=None, count=None, fade_px=10, raw=False,
+                   live=False):
     """fsim_estimate: SimAlgorithm.estimateParameters (SimAlgorithm.java:37-256)
     for a batch of stacks in one native call, with the GUI's input preparation
     (input_ffts: fadeBorderCos, FFT) on the device.  ``images``: host float32
@@ -530,7 +591,11 @@ def estimate_batch(param, fitBand, fitExclude, images=None, images_dev=None, cou
     plus ``count``.  Returns, per stack, the per-direction result dicts of
     estimate_parameters (plus the coarse ``peak``); ``param`` is not changed
     (apply_estimate stores one stack's results in it); ``raw``: return the
-    fsim_estimate_result array instead (for params_from_estimates)."""
+    fsim_estimate_result array instead (for params_from_estimates).
+    ``live``: the live-mode refit instead (refit_parameters' semantics,
+    ReconstructionRunner.java:642-746): highest band, OTF band numbers (0, 1),
+    modulation 1 only, input faded by the borderDampen product; ``fitBand``
+    is ignored (pass fitExclude = 0.8 for the reference's constant)."""
     _lib.init()
     D, P = param.nrDir(), param.dir(0).nrPha()
     n = param.vectorWidth()
@@ -547,7 +612,7 @@ def estimate_batch(param, fitBand, fitExclude, images=None, images_dev=None, cou
         raise ValueError("give images, or images_dev and count")
     p = _lib.ESTIMATE_PARAMS(n=n, ndir=D, nphase=P, nband=param.nrBand(), fit_band=int(fitBand),
                              fit_exclude=float(fitExclude), vec_cpm=param.pxlSizeCyclesMicron(),
-                             fade_px=int(fade_px))
+                             fade_px=int(fade_px), live=1 if live else 0)
     res = (_lib.ESTIMATE_RESULT * (count * D))()
     try:
         call("fsim_estimate", C.c_void_p(int(images_dev)), int(count), C.byref(p), param.otf().device(), res)
@@ -562,7 +627,7 @@ def estimate_batch(param, fitBand, fitExclude, images=None, images_dev=None, cou
         for d in range(D):
             r = res[s * D + d]
             e = dict(px=r.px, py=r.py, pha_off=r.pha_off, mod1=r.mod1, peak=(r.peak_x, r.peak_y))
-            if param.nrBand() == 3:
+            if param.nrBand() == 3 and not live:
                 e["mod2"] = r.mod2
             dirs.append(e)
         out.append(dirs)
@@ -836,7 +901,7 @@ def params_from_estimate(param, dirs):
         for i, ph in enumerate(dd.phases):
             p.phases[d][i] = ph + r["pha_off"]
         p.modul[d][1] = r["mod1"]
-        if B == 3:
+        if B == 3 and "mod2" in r:
             p.modul[d][2] = r["mod2"]
     return p
 

@@ -235,12 +235,24 @@ typedef struct {
     double fit_exclude; /* locatePeak exclusion, fraction of the OTF cutoff */
     double vec_cpm;     /* cycles per micron of the n x n vector           */
     int fade_px;        /* fadeBorderCos border applied to every image; 0: none */
+    int live;           /* 0: SimAlgorithm.estimateParameters (SimAlgorithm.java:37-256);
+                         * 1: the live-mode refit ReconstructionRunner.ParameterRefitThread.
+                         * doRefit (livemode/ReconstructionRunner.java:642-746): the peak is
+                         * searched on the highest band's correlation (fit_band ignored;
+                         * the reference uses fit_exclude = 0.8), fitPeak correlates it with
+                         * OTF band numbers (0, 1) whatever the band, getPeak of band 1 at
+                         * peak / (bands - 1) gives phase and modulation 1, modulation 2 is
+                         * not estimated (mod2 = NaN: fsim_params_from_estimate keeps the
+                         * base value); the input is faded by multiplying with the faded
+                         * vector of ones (times(borderDampen), :614-617, 661-664) instead of
+                         * fadeBorderCos on each image */
 } fsim_estimate_params;
 
 typedef struct {
     double px, py;          /* shift as SimParam.Dir.setPxPy receives it (-peak)  */
     double pha_off;         /* phase of getPeak on band 1 (setPhaOff)             */
-    double mod1, mod2;      /* |getPeak| of bands 1 and 2 (mod2 = 0 with 2 bands)  */
+    double mod1, mod2;      /* |getPeak| of bands 1 and 2 (mod2 = 0 with 2 bands,
+                             * NaN when not estimated: the live refit)             */
     double peak_x, peak_y;  /* coarse correlation peak (locatePeak convention)    */
 } fsim_estimate_result;
 
@@ -348,7 +360,8 @@ int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw_dev, float *out
  *   parameters: for stack s and direction d, pX, pY = (px, py) / (nband - 1)
  *   (SimParam.Dir.setPxPy, SimParam.java:536-541), phases = base phases +
  *   pha_off (setPhaOff; getPhases, :472-477), modulation 1 (and 2) from the
- *   estimate, the others from base.  res: count x ndir results of
+ *   estimate, the others from base (also modulation 2 when the estimate
+ *   holds NaN for it, as the live refit leaves it).  res: count x ndir results of
  *   fsim_estimate; base: the parameters without phase offset; out: count
  *   entries (host memory). */
 int fsim_params_from_estimate(const fsim_estimate_result *res, int count, int ndir, int nphase, int nband,

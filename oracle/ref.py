@@ -201,6 +201,28 @@ class Otf32:
         self.vals, self.cutoff, self.cpm = otf_estimate(na, lam, a, n)
         self.ns = n
 
+    def tab(self, band):
+        """The table of OTF band `band` (a single-band OTF serves every band,
+        OtfProvider.getOtfVal: `if (!isMultiBand) band = 0`)."""
+        return self.vals
+
+
+class OtfBands32:
+    """A measured multi-band OTF (OtfProvider.fromData / loadFromConfig,
+    OtfProvider.java:592-628): one complex table of `ns` samples per band,
+    float32 [nb, ns, 2]."""
+
+    def __init__(self, vals, cutoff, cpm):
+        vals = np.asarray(vals)
+        self.tabs = [np.ascontiguousarray(cplx_f32(v)) for v in vals]
+        self.vals = self.tabs[0]
+        self.cutoff, self.cpm, self.ns = cutoff, cpm, vals.shape[1]
+
+    def tab(self, band):
+        if band >= len(self.tabs):
+            raise IndexError("band idx too high or <0")
+        return self.tabs[band]
+
 
 def common_region32(b0, b1, otf, vec_cpm, kx, ky, dist, wlim, divide=True):
     """Correlation.commonRegion on float32 [N, N, 2] arrays, in place."""
@@ -209,21 +231,22 @@ def common_region32(b0, b1, otf, vec_cpm, kx, ky, dist, wlim, divide=True):
                             otf.cutoff, vec_cpm, kx, ky, dist, wlim, 1 if divide else 0)
 
 
-def fit_peak32(b0, b1, otf, vec_cpm, kx, ky, wlim, search, cntrl=False):
-    """Correlation.fitPeak: [kx, ky, phase, mag] (and the 10 x 30 control image)."""
+def fit_peak32(b0, b1, otf, vec_cpm, kx, ky, wlim, search, cntrl=False, bn0=0, bn1=1):
+    """Correlation.fitPeak(b0, b1, bn0, bn1, otf, ...): [kx, ky, phase, mag]
+    (and the 10 x 30 control image); bn0 / bn1 select the OTF bands."""
     n = b0.shape[0]
     res = np.zeros(4)
     c = np.zeros((10, 30), dtype=np.float32) if cntrl else None
-    lib().ref_fit_peak(ptr(b0), ptr(b1), n, ptr(otf.vals), ptr(otf.vals), otf.ns, otf.cpm, otf.cutoff,
+    lib().ref_fit_peak(ptr(b0), ptr(b1), n, ptr(otf.tab(bn0)), ptr(otf.tab(bn1)), otf.ns, otf.cpm, otf.cutoff,
                        vec_cpm, kx, ky, wlim, search, ptr(c) if cntrl else None, ptr(res))
     return (list(res), c) if cntrl else list(res)
 
 
-def get_peak32(b0, b1, otf, vec_cpm, kx, ky, wlim):
-    """Correlation.getPeak: complex."""
+def get_peak32(b0, b1, otf, vec_cpm, kx, ky, wlim, bn0=0, bn1=1):
+    """Correlation.getPeak(b0, b1, bn0, bn1, otf, ...): complex."""
     n = b0.shape[0]
     res = np.zeros(2)
-    lib().ref_get_peak(ptr(b0), ptr(b1), n, ptr(otf.vals), ptr(otf.vals), otf.ns, otf.cpm, otf.cutoff,
+    lib().ref_get_peak(ptr(b0), ptr(b1), n, ptr(otf.tab(bn0)), ptr(otf.tab(bn1)), otf.ns, otf.cpm, otf.cutoff,
                        vec_cpm, kx, ky, wlim, ptr(res))
     return complex(res[0], res[1])
 
@@ -293,18 +316,74 @@ def estimate32(in_fft, nband, fit_band, fit_exclude, otf, vec_cpm):
         min_dist = fit_exclude * otf.cutoff / vec_cpm
         pk = np.zeros(4)
         lib().ref_locate_peak(ptr(c1 if fit_band == 1 else c2), n, n, min_dist, ptr(pk))
-        peak = fit_peak32(sep[0], sep[fb], otf, vec_cpm, -pk[0], -pk[1], 0.05, 2.5)
+        peak = fit_peak32(sep[0], sep[fb], otf, vec_cpm, -pk[0], -pk[1], 0.05, 2.5, bn1=fit_band)
         r = {"peak": list(pk)}
         if lb != hb:
             if fit_band == 1:
                 peak[0] *= 2
                 peak[1] *= 2
             p1 = get_peak32(sep[0], sep[lb], otf, vec_cpm, peak[0] / 2, peak[1] / 2, 0.05)
-            p2 = get_peak32(sep[0], sep[hb], otf, vec_cpm, peak[0], peak[1], 0.05)
+            p2 = get_peak32(sep[0], sep[hb], otf, vec_cpm, peak[0], peak[1], 0.05, bn1=2)
             r.update(px=-peak[0], py=-peak[1], pha_off=float(np.arctan2(p1.imag, p1.real)),
                      mod1=abs(p1), mod2=abs(p2))
         else:
             p1 = get_peak32(sep[0], sep[1], otf, vec_cpm, peak[0], peak[1], 0.05)
             r.update(px=-peak[0], py=-peak[1], pha_off=float(np.arctan2(p1.imag, p1.real)), mod1=abs(p1))
         out.append(r)
+    return out
+
+
+def input_ffts32_live(raw, fade_px=10):
+    """The live refit's input (ReconstructionRunner.java:614-617, 661-664):
+    each 16-bit frame to a complex vector (setFrom16bitPixels), times
+    borderDampen -- a float vector of ones faded by fadeBorderCos -- and the
+    forward FFT.  raw: uint16 [D, P, N, N]."""
+    n = raw.shape[-1]
+    damp = np.ones((n, n), dtype=np.float32)
+    lib().ref_fade_border(ptr(damp), n, n, fade_px)
+    out = np.zeros(raw.shape + (2,), dtype=np.float32)
+    for idx in np.ndindex(raw.shape[:-2]):
+        out[idx][..., 0] = (raw[idx] & 0xFFFF).astype(np.float32) * damp
+        fft32(out[idx], False)
+    return out
+
+
+def refit32(in_fft, nband, otf, vec_cpm, fit_exclude=0.8):
+    """ReconstructionRunner.ParameterRefitThread.doRefit
+    (livemode/ReconstructionRunner.java:642-746) on FFT'd inputs
+    [D, P, N, N, 2]: per direction the band separation with equidistant
+    phases, the coarse correlation of band 0 with the highest band (hb),
+    locatePeak outside fit_exclude * cutoff, fitPeak(sep0, sep_hb, 0, 1, ...)
+    -- OTF band 1 whatever hb is -- and getPeak(sep0, sep1, 0, 1, peak /
+    (bands - 1)); returns {px, py (setPxPy arguments), pha_off, mod1, peak}
+    (modulation 2 is not refitted)."""
+    D, P, n = in_fft.shape[0], in_fft.shape[1], in_fft.shape[2]
+    att = np.zeros((n, n), dtype=np.float32)
+    lib().ref_write_attenuation(ptr(att), n, n, vec_cpm, .99, 0.15 * otf.cutoff, 0.0, 0.0)
+    Cc = 2 * nband - 1
+    out = []
+    for d in range(D):
+        lb, hb = 1, (3 if nband == 3 else 1)
+        nb = nband - 1
+        phases = np.array([(2 * np.pi * p) / P + 0.0 for p in range(P)])
+        Sm = np.zeros(2 * Cc * P)
+        assert lib().ref_separation_matrix(ptr(phases), P, nband, None, ptr(Sm)) == 0
+        Sf = Sm.astype(np.float32)
+        sep = np.zeros((Cc, n, n, 2), dtype=np.float32)
+        ins = (C.c_void_p * P)(*[in_fft[d, p].ctypes.data for p in range(P)])
+        outs = (C.c_void_p * Cc)(*[sep[c].ctypes.data for c in range(Cc)])
+        lib().ref_separate_bands(ins, outs, n * n, ptr(Sf), P, Cc)
+        c0, c1 = sep[0].copy(), sep[hb].copy()
+        for c in (c0, c1):
+            c[..., 0] *= att
+            c[..., 1] *= att
+            fft32(c, True)
+        times_conj32(c1, c0)
+        fft32(c1, False)
+        pk = np.zeros(4)
+        lib().ref_locate_peak(ptr(c1), n, n, fit_exclude * otf.cutoff / vec_cpm, ptr(pk))
+        peak = fit_peak32(sep[0], sep[hb], otf, vec_cpm, -pk[0], -pk[1], 0.05, 2.5, bn0=0, bn1=1)
+        p1 = get_peak32(sep[0], sep[lb], otf, vec_cpm, peak[0] / nb, peak[1] / nb, 0.05, bn0=0, bn1=1)
+        out.append({"peak": list(pk), "px": -peak[0], "py": -peak[1],
+                    "pha_off": float(np.arctan2(p1.imag, p1.real)), "mod1": abs(p1)})
     return out

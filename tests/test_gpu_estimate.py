@@ -224,3 +224,89 @@ def test_estimate_batch_large_size_known_answer(fsim_lib):
         g = got[0][d]
         print(f"n={n} dir {d}: {g} true ({sp['px'][d]:.3f}, {sp['py'][d]:.3f})")
         assert abs(g["px"] - sp["px"][d]) < 0.1 and abs(g["py"] - sp["py"][d]) < 0.1
+
+
+def _live_otf(kind, n):
+    """fromEstimate (one band) or a measured-style three-band OTF (bands of
+    different curvature, so OTF band 1 and band 2 weights differ), with the
+    oracle's copy of the same tables."""
+    from fsim.sim_algorithm import OtfProvider
+    if kind == "estimate":
+        o = OtfProvider.fromEstimate(NA, LAM, A_OTF)
+        o.setPixelSize(1 / (n * PX_UM))
+        return o, ref.Otf32(NA, LAM, A_OTF)
+    ests = [OtfProvider.fromEstimate(NA, LAM, a) for a in (1.0, 0.6, 0.3)]
+    vals = np.stack([e.band_values()[0] for e in ests])
+    o = OtfProvider.fromData(NA, LAM, ests[0].cpm, vals)
+    o.setPixelSize(1 / (n * PX_UM))
+    return o, ref.OtfBands32(vals, o.getCutoff(), o.cpm)
+
+
+@pytest.mark.parametrize("nphase,nband,k_frac,h2,kind", [(3, 2, 0.85, 0.0, "estimate"),
+                                                         (5, 3, 0.45, 0.3, "three_band"),
+                                                         (5, 3, 0.45, 0.3, "estimate")])
+def test_live_refit_matches_oracle(fsim_lib, nphase, nband, k_frac, h2, kind):
+    """The live refit (fsim_estimate with live = 1, ReconstructionRunner.
+    doRefit) on 16-bit stacks against the oracle's restatement: coarse peaks
+    and fitted shifts identical, phase and modulation 1 to 1e-4, modulation 2
+    not estimated; the per-call flow over device vectors (refit_parameters)
+    gives the same shifts."""
+    from fsim import synthetic
+    from fsim.sim_algorithm import SimParam, estimate_batch, refit_parameters
+    n = 256
+    raws, sps = [], []
+    for s in range(2):
+        raw, sp = synthetic.sim_stack(n=n, nphase=nphase, seed=150 + s, nbeads=150, nfil=20, harmonic2=h2,
+                                      k_frac=k_frac)
+        raws.append(raw)
+        sps.append(sp)
+    otf, otf32 = _live_otf(kind, n)
+    param = SimParam.create(nband, 3, nphase, n, PX_UM, otf)
+    images = np.stack(raws).reshape(2, 3, nphase, n, n).astype(np.float32)
+    got = estimate_batch(param, 1, 0.8, images=images, live=True)
+    scale = nband - 1
+    for s in range(2):
+        want = ref.refit32(ref.input_ffts32_live(raws[s].reshape(3, nphase, n, n)), nband, otf32, 1 / (n * PX_UM))
+        for d in range(3):
+            g, w = got[s][d], want[d]
+            print(f"{kind} stack {s} dir {d}: gpu {g} oracle {w}")
+            assert g["peak"] == (w["peak"][0], w["peak"][1])
+            assert g["px"] == w["px"] and g["py"] == w["py"]
+            assert abs(g["pha_off"] - w["pha_off"]) < 1e-4
+            assert abs(g["mod1"] - w["mod1"]) <= 1e-4 * w["mod1"]
+            assert "mod2" not in g
+            assert abs(g["px"] - scale * sps[s]["px"][d]) < 0.1 and abs(g["py"] - scale * sps[s]["py"][d]) < 0.1
+    # the per-call flow through the vector API: the same shifts; modulation
+    # 2 of the parameters is left as it was
+    p2 = SimParam.create(nband, 3, nphase, n, PX_UM, otf)
+    if nband == 3:
+        for d in range(3):
+            p2.dir(d).setModulation(2, 0.55)
+    per_call = refit_parameters(p2, raws[0])
+    for d in range(3):
+        assert per_call[d]["px"] == got[0][d]["px"] and per_call[d]["py"] == got[0][d]["py"]
+        assert abs(per_call[d]["pha_off"] - got[0][d]["pha_off"]) < 1e-6
+        if nband == 3:
+            assert p2.dir(d).getModulations()[2] == 0.55
+
+
+def test_live_refit_keeps_modulation_two(fsim_lib):
+    """fsim_params_from_estimate keeps the base modulation 2 where the live
+    refit did not estimate it (NaN), and takes the estimate's otherwise."""
+    from fsim import synthetic
+    from fsim.sim_algorithm import SimParam, estimate_batch, params_from_estimates
+    n = 128
+    raw, sp = synthetic.sim_stack(n=n, nphase=5, seed=160, nbeads=60, nfil=8, harmonic2=0.3, k_frac=0.45)
+    otf, _ = _live_otf("estimate", n)
+    param = SimParam.create(3, 3, 5, n, PX_UM, otf)
+    for d in range(3):
+        param.dir(d).setModulation(2, 0.42)
+    imgs = raw.reshape(1, 3, 5, n, n).astype(np.float32)
+    res_live = estimate_batch(param, 1, 0.8, images=imgs, raw=True, live=True)
+    res_est = estimate_batch(param, 2, 0.6, images=imgs, raw=True)
+    live = params_from_estimates(param, res_live, 1)[0]
+    est = params_from_estimates(param, res_est, 1)[0]
+    for d in range(3):
+        assert np.isnan(res_live[d].mod2) and not np.isnan(res_est[d].mod2)
+        assert live.modul[d][2] == 0.42 and est.modul[d][2] == res_est[d].mod2
+        assert live.modul[d][1] == res_live[d].mod1 and live.px[d] == res_live[d].px / 2

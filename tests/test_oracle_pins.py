@@ -19,7 +19,11 @@ the oracle's own formula, and is chosen so that a plausible mistake fails it:
 * the RL-on-output composition of runReconstruction (SimAlgorithm.java:
   587-686): the effective SIM OTF at DC (band 0 weighted 1/D per direction,
   every higher band twice: at +k and -k), its symmetry and support, and the
-  filter with 0 iterations returning the unfiltered sum.
+  filter with 0 iterations returning the unfiltered sum;
+* the live refit (ReconstructionRunner.doRefit, ReconstructionRunner.java:
+  642-746): known-answer recovery of the highest band's shift, identity with
+  estimateParameters where the two flows coincide, and OTF band numbers that
+  change the fit correlation with a three-band OTF.
 """
 import ctypes as C
 import math
@@ -241,3 +245,85 @@ def test_rl_output_with_zero_iterations_is_the_unfiltered_sum():
     assert fp64.rel_l2(got, want) < 1e-12
     two = fp64.run_reconstruction64(p, in_fft, vals, filter_style="RLout", rl_iterations=2)
     assert fp64.rel_l2(two, want) > 1e-3
+
+
+def _three_band_otf(na=1.4, lam=525.0):
+    """A measured-style three-band OTF: bands of different curvature."""
+    tabs = [fp64.otf_from_estimate(na, lam, a)[0] for a in (1.0, 0.6, 0.3)]
+    cut, cpm = fp64.otf_from_estimate(na, lam, 1.0)[1:]
+    return ref.OtfBands32(np.stack(tabs), cut, cpm)
+
+
+@pytest.mark.parametrize("nphase,nband,k_frac,h2", [(3, 2, 0.85, 0.0), (5, 3, 0.45, 0.3)])
+def test_refit_known_answer(nphase, nband, k_frac, h2):
+    """The live refit recovers the pattern: the highest band's shift (twice
+    the band-1 shift with three bands) within 0.1 px, phase offset near 0
+    (the data's phases are 2 pi p / P; the binning of the fine grid moves the
+    pattern's phase at the raw pixel centres a little) within 0.2 rad, as for
+    estimateParameters."""
+    from fsim import synthetic
+    n = 128
+    raw, sp = synthetic.sim_stack(n=n, nphase=nphase, seed=31, nbeads=60, nfil=8, harmonic2=h2, k_frac=k_frac)
+    fft_in = ref.input_ffts32_live(raw.reshape(3, nphase, n, n))
+    otf = ref.Otf32(1.4, 525.0, 1.0)
+    got = ref.refit32(fft_in, nband, otf, 1 / (n * 0.08))
+    scale = nband - 1
+    for d in range(3):
+        g = got[d]
+        assert abs(g["px"] - scale * sp["px"][d]) < 0.1 and abs(g["py"] - scale * sp["py"][d]) < 0.1
+        assert abs(g["pha_off"]) < 0.2
+        assert 0.3 < g["mod1"] < 1.1
+        assert "mod2" not in g
+
+
+def test_refit_equals_estimate_where_the_flows_coincide():
+    """Two bands: the refit is estimateParameters with fit band 1 and
+    fitExclude 0.8 (same arithmetic, so identical results).  Three bands with
+    a single-band OTF: fit band 2 gives the same shift, phase and modulation
+    1 (the OTF band numbers do not matter for one table)."""
+    from fsim import synthetic
+    n = 128
+    otf = ref.Otf32(1.4, 525.0, 1.0)
+    raw, _ = synthetic.sim_stack(n=n, seed=32, nbeads=60, nfil=8)
+    fft_in = ref.input_ffts32_live(raw.reshape(3, 3, n, n))
+    a = ref.refit32(fft_in, 2, otf, 1 / (n * 0.08))
+    b = ref.estimate32(fft_in, 2, 1, 0.8, otf, 1 / (n * 0.08))
+    for d in range(3):
+        for k in ("px", "py", "pha_off", "mod1"):
+            assert a[d][k] == b[d][k]
+    raw, _ = synthetic.sim_stack(n=n, nphase=5, seed=33, nbeads=60, nfil=8, harmonic2=0.3, k_frac=0.45)
+    fft_in = ref.input_ffts32_live(raw.reshape(3, 5, n, n))
+    a = ref.refit32(fft_in, 3, otf, 1 / (n * 0.08))
+    b = ref.estimate32(fft_in, 3, 2, 0.8, otf, 1 / (n * 0.08))
+    for d in range(3):
+        for k in ("px", "py", "pha_off", "mod1"):
+            assert a[d][k] == b[d][k]
+    # the live input differs from the GUI's fadeBorderCos on the image only
+    # in float rounding
+    gui = ref.input_ffts32(raw.reshape(3, 5, n, n).astype(np.float32))
+    assert 0 < fp64.rel_l2(ref.to_complex(fft_in), ref.to_complex(gui)) < 1e-6
+
+
+def test_three_band_otf_band_numbers_matter():
+    """With a three-band OTF, fitPeak's band numbers select the tables of the
+    commonRegion weights: the refit's (0, 1) and estimateParameters' (0, 2)
+    give different correlations of the same bands."""
+    from fsim import synthetic
+    n = 128
+    otf = _three_band_otf()
+    raw, sp = synthetic.sim_stack(n=n, nphase=5, seed=34, nbeads=60, nfil=8, harmonic2=0.3, k_frac=0.45)
+    fft_in = ref.input_ffts32_live(raw.reshape(3, 5, n, n))
+    S = np.zeros(2 * 5 * 5)
+    ph = np.array([2 * np.pi * p / 5 for p in range(5)])
+    assert ref.lib().ref_separation_matrix(ref.ptr(ph), 5, 3, None, ref.ptr(S)) == 0
+    Sf = S.astype(np.float32)
+    sep = np.zeros((5, n, n, 2), np.float32)
+    ins = (C.c_void_p * 5)(*[fft_in[0, p].ctypes.data for p in range(5)])
+    outs = (C.c_void_p * 5)(*[sep[c].ctypes.data for c in range(5)])
+    ref.lib().ref_separate_bands(ins, outs, n * n, ref.ptr(Sf), 5, 5)
+    k = (-2 * sp["px"][0], -2 * sp["py"][0])
+    f1 = ref.fit_peak32(sep[0], sep[3], otf, 1 / (n * 0.08), k[0], k[1], 0.05, 2.5, bn1=1)
+    f2 = ref.fit_peak32(sep[0], sep[3], otf, 1 / (n * 0.08), k[0], k[1], 0.05, 2.5, bn1=2)
+    assert abs(f1[3] - f2[3]) > 1e-3 * abs(f2[3])
+    got = ref.refit32(fft_in, 3, otf, 1 / (n * 0.08))
+    assert abs(got[0]["px"] - 2 * sp["px"][0]) < 0.1
