@@ -789,6 +789,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--px-um", type=float, default=0.080,
+                    help="raw pixel size of the synthetic stacks (micron)")
     ap.add_argument("--plain-runs", type=int, default=20)
     ap.add_argument("--latency-runs", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -856,11 +858,11 @@ def main():
     # B distinct stacks: 4 seeded samples, each imaged B/4 times with
     # independent photon noise (stacks[i] is stack i of the batch)
     per = max(2, (B + 3) // 4)
-    series = [synthetic.sim_stack(n=n, seed=s, timepoints=per) for s in range(4)]
+    series = [synthetic.sim_stack(n=n, seed=s, timepoints=per, px_um=args.px_um) for s in range(4)]
     nst = max(B, 4)
     stacks = [series[i % 4][0][i // 4] for i in range(nst)]
     sps = [series[i % 4][1] for i in range(nst)]
-    param = synthetic.make_param(sps[0], n)
+    param = synthetic.make_param(sps[0], n, px_um=args.px_um)
     rec = Reconstructor(param)
     tuned = None
     if TUNE:
@@ -901,7 +903,7 @@ def main():
     from oracle import fp64, ref
     log("parity check vs FP32 oracle")
     got = rec.reconstruct_host(stacks[0])
-    p32 = ref.recon_dict(sps[0], n)
+    p32 = ref.recon_dict(sps[0], n, px_um=args.px_um)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     threads = max(1, min(threads, os.cpu_count() or 1))
     t0 = time.perf_counter()

@@ -523,11 +523,15 @@ struct ColShiftTile {
     static constexpr int cw = cw0 < CW0 ? cw0 : CW0;
     static constexpr int value = t * cw;
     static constexpr int lds = FftLds<N, e>::len * cw;
-    // register twiddles (no LDS table; about 28 more VGPRs: at most 5 waves
-    // per SIMD keep them without spills)
-    static constexpr bool tw_fits = (lds + N) * 8 <= 163840;
+    // the LDS twiddle table only where it costs no block per CU (at 2048^2 a
+    // 4-column block would otherwise run alone on its CU, with no other
+    // block loading while it transforms); sincospi twiddles otherwise
+    static constexpr bool tw_fits =
+        (lds + N) * 8 <= 163840 && 163840 / ((lds + N) * 8) == 163840 / (lds * 8);
+    // register twiddles from that table: about 28 more VGPRs, at most 5
+    // waves per SIMD keep them without spills
     static constexpr bool rt = FSIM_COL_TW == 2 && tw_fits && tw_reg_fits<N, e>();
-    static constexpr int wpe = lds_waves_per_simd(value, (lds + N) * 8, rt ? 5 : FSIM_COL_WPE);
+    static constexpr int wpe = lds_waves_per_simd(value, (lds + (tw_fits ? N : 0)) * 8, rt ? 5 : FSIM_COL_WPE);
 };
 
 template <int N, int E0, int CW0, bool WC>
@@ -558,11 +562,15 @@ __global__ void __launch_bounds__((ColShiftTile<N, E0, CW0, WC>::value), (ColShi
     if (FSIM_TILE_GROUPS) {
         constexpr int TG = tiles < 8 ? tiles : 8;
         tile_group_order<TG>(w, total / tiles, u, tile);
-        // group g holds tiles g, g + G, g + 2G, ... (G = tiles / TG groups):
-        // each XCD's share spans the whole column range, so the zero tiles
-        // beyond the OTF cutoff (a contiguous range) spread evenly over the
-        // XCDs instead of emptying one XCD's share
-        tile = tile / TG + (tiles / TG) * (tile % TG);
+        // group g holds tiles g, g + G, g + 2G, ... (G groups), in runs of S
+        // adjacent tiles that share A's 8-column runs (S = 8 / CW): each
+        // XCD's share spans the whole column range, so the zero tiles beyond
+        // the OTF cutoff (a contiguous range) spread evenly over the XCDs
+        // instead of emptying one XCD's share
+        constexpr int S = CW < kAT ? kAT / CW : 1;
+        constexpr int TGB = TG / S > 0 ? TG / S : 1, NB = tiles / S;
+        const int j = tile / S;
+        tile = (j / TGB + (NB / TGB) * (j % TGB)) * S + tile % S;
     } else {
         tile = w % tiles;
         u = w / tiles;
@@ -1203,6 +1211,14 @@ static void stage_mark(int stage, hipStream_t s, hipEvent_t &start) {
 // 0: the size's default element count): 0 (0, 8), 1 (16, 8), 2 (16, 16),
 // 3 (8, 16), 4 (16, 4), 5 (8, 8); 6 and 7: one column per wavefront
 // (WaveColE elements per thread), 8 and 4 columns per block.
+// Default columns per block of the column pass: 8, and 4 at 2048^2, where
+// two 4-column blocks (without the LDS twiddle table) share a CU and one loads
+// while the other transforms (0.58 against 0.64 ms for 8 stacks).
+template <int N>
+struct DefColCW {
+    static constexpr int value = N >= 2048 ? 4 : FSIM_COL_CW;
+};
+
 template <int N, int E0, int CW0, bool WC = false>
 static void launch_col_shift(const JobList &jl, int njobs, int D, int C, const float2 *twn, hipStream_t s) {
     using CT = ColShiftTile<N, E0, CW0, WC>;
@@ -1221,7 +1237,7 @@ static void col_candidate_shape(int col, int &e, int &cw) {
         case 5: e = ColShiftTile<N, 8, 8>::e; cw = ColShiftTile<N, 8, 8>::cw; break;
         case 6: e = ColShiftTile<N, 0, 8, true>::e; cw = -ColShiftTile<N, 0, 8, true>::cw; break;
         case 7: e = ColShiftTile<N, 0, 4, true>::e; cw = -ColShiftTile<N, 0, 4, true>::cw; break;
-        default: e = ColShiftTile<N, 0, FSIM_COL_CW>::e; cw = ColShiftTile<N, 0, FSIM_COL_CW>::cw; break;
+        default: e = ColShiftTile<N, 0, DefColCW<N>::value>::e; cw = ColShiftTile<N, 0, DefColCW<N>::value>::cw; break;
     }
 }
 
@@ -1267,7 +1283,7 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int P, int C, 
             case 5: launch_col_shift<N, 8, 8>(jl, njobs, D, C, twn, s); break;
             case 6: launch_col_shift<N, 0, 8, true>(jl, njobs, D, C, twn, s); break;
             case 7: launch_col_shift<N, 0, 4, true>(jl, njobs, D, C, twn, s); break;
-            default: launch_col_shift<N, 0, FSIM_COL_CW>(jl, njobs, D, C, twn, s); break;
+            default: launch_col_shift<N, 0, DefColCW<N>::value>(jl, njobs, D, C, twn, s); break;
         }
         stage_mark(1, s, ev);
         constexpr int E = FftE<N>::value, T = N / E;
