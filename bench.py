@@ -373,18 +373,26 @@ def batched_mixed(nper=32, reps=10):
     b_alg = sum(434 * n * n for n, _ in jobs)
     b_moved = sum(sum(stage_bytes(n, Cs=2, half=True, dc_sum=True)) for n, _ in jobs)
 
-    batch = ReconstructionBatch([recs[k] for k in jobs], use_graph=True)
-    batch.run(raw_ptrs, out_ptrs)            # captures the graph
-    call("fsim_sync")
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        batch.run(raw_ptrs, out_ptrs)
+    def time_batch(persistent):
+        batch = ReconstructionBatch([recs[k] for k in jobs], use_graph=True, persistent=persistent)
+        batch.run(raw_ptrs, out_ptrs)            # captures the graph
         call("fsim_sync")
-        times.append(time.perf_counter() - t0)
-    el = float(np.median(times))
-    info = batch.info()
-    batch.close()
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            batch.run(raw_ptrs, out_ptrs)
+            call("fsim_sync")
+            times.append(time.perf_counter() - t0)
+        info = batch.info()
+        batch.close()
+        return float(np.median(times)), info
+
+    # persistent work-list kernels: one launch per stage and group of sizes;
+    # its images must be the per-class form's bits
+    el_p, _ = time_batch(True)
+    firsts = {n: outs[n][0].download((2 * n, 2 * n), np.float32) for n in sizes}
+    el, info = time_batch(False)
+    same_bits = all(np.array_equal(firsts[n], outs[n][0].download((2 * n, 2 * n), np.float32)) for n in sizes)
     # parity of the first slot of each size (channel 0) from the batch output
     parity = {}
     for n in sizes:
@@ -418,6 +426,15 @@ def batched_mixed(nper=32, reps=10):
             "reference_dataflow_equivalent_frac": round(b_alg / el / 8e12, 4),
             "autotune": {n: (t["sep"], t["col_elems"], t["col_width"]) for n, t in tuned.items()},
             "rel_l2": parity,
+            "persistent_work_list": {"form": "the same batch through persistent work-list kernels (per stage one "
+                                             "launch for 256/512 and one for 1024), replayed hipGraph",
+                                     "ms_per_batch": round(el_p * 1e3, 3),
+                                     "recons_per_s": round(len(jobs) / el_p, 1),
+                                     "executed_bytes_frac": round(b_moved / el_p / 8e12, 4),
+                                     "bitwise_equal_to_class_launches":
+                                         same_bits if all(t["col"] == 0 and (t["sep"] == 1 or n > 512)
+                                                          for n, t in tuned.items())
+                                         else "not compared: tuned column choices differ from the defaults"},
             "served_one_by_one": {"ms_per_batch": round(el_s * 1e3, 3),
                                   "recons_per_s": round(len(jobs) / el_s, 1),
                                   "executed_bytes_frac": round(b_moved / el_s / 8e12, 4),

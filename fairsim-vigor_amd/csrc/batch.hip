@@ -14,6 +14,14 @@
 // pinned host memory, stream-ordered before the graph) -- the graph holds only
 // kernels, no memory set nodes.
 //
+// With FSIM_BATCH_PERSISTENT the classes of sizes 256 to 2048 run instead
+// through persistent work-list kernels: per launch group (n = 256 and 512;
+// n = 1024 and 2048) and stage, one kernel walks a device list of (class,
+// block) items that holds the blocks of every class's own launch of that
+// stage, larger size first, claiming them through a cursor that a small
+// kernel zeroes at the start of each run (see launch_persistent).  The two groups run on
+// streams of their own, concurrently; other sizes keep their class launches.
+//
 // The reference runs the same reconstruction per image from several threads
 // (livemode/ReconstructionRunner.java:243-251, 380-413); batching images of
 // different sizes and channels into one device launch has no counterpart
@@ -36,6 +44,19 @@ struct BatchClass {
     hipEvent_t done = nullptr;
 };
 
+// A launch group of the persistent form: its classes (device copies, which
+// the item lists index), one item list per stage, a stream.
+struct PGroupRun {
+    std::vector<PClass> hcls;
+    PClass *cls_d = nullptr;
+    int4 *items_d = nullptr;
+    int *cur_d = nullptr;        // the stage lists' cursors
+    int off[kPStages] = {};
+    int ends[kPStages][4] = {};
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+};
+
 bool stage_timing_enabled();
 
 }  // namespace fsim
@@ -51,7 +72,8 @@ struct fsim_batch_s {
     Job *jobs_d = nullptr;
     hipStream_t origin = nullptr;
     hipEvent_t fork = nullptr, staged = nullptr, joined = nullptr, start = nullptr;
-    bool use_graph = false, captured = false;
+    bool use_graph = false, captured = false, persistent = false;
+    PGroupRun pg[kPGroups];
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int64_t device_bytes = 0;
@@ -66,6 +88,13 @@ static void batch_free(fsim_batch_s *b) {
         if (c.stream) (void)hipStreamDestroy(c.stream);
         if (c.done) (void)hipEventDestroy(c.done);
     }
+    for (PGroupRun &g : b->pg) {
+        (void)hipFree(g.cls_d);
+        (void)hipFree(g.items_d);
+        (void)hipFree(g.cur_d);
+        if (g.stream) (void)hipStreamDestroy(g.stream);
+        if (g.done) (void)hipEventDestroy(g.done);
+    }
     (void)hipFree(b->jobs_d);
     if (b->jobs_h) (void)hipHostFree(b->jobs_h);
     if (b->origin) (void)hipStreamDestroy(b->origin);
@@ -74,11 +103,29 @@ static void batch_free(fsim_batch_s *b) {
     delete b;
 }
 
-// The launch sequence of one run on b->origin: fork into the class streams,
-// one pipeline per class, join.
+static bool in_persistent_group(const fsim_batch_s *b, const BatchClass &c) {
+    return b->persistent && persistent_group(c.n) >= 0;
+}
+
+// The launch sequence of one run on b->origin: fork into the group and class
+// streams, the persistent stages per group and one pipeline per other class,
+// join.
 static int batch_launch(fsim_batch_s *b) {
     FSIM_HIP(hipEventRecord(b->start, b->origin));
+    for (int gi = 0; gi < kPGroups; gi++) {
+        PGroupRun &g = b->pg[gi];
+        if (g.hcls.empty()) continue;
+        FSIM_HIP(hipStreamWaitEvent(g.stream, b->start, 0));
+        int rc = launch_persistent_reset(g.cur_d, kPStages * kPListCursors, g.stream);
+        for (int st = 0; st < kPStages && !rc; st++)
+            rc = launch_persistent(gi, st, g.cls_d, g.items_d + g.off[st], g.ends[st], g.cur_d + st * kPListCursors,
+                                   g.stream);
+        if (rc) return rc;
+        FSIM_HIP(hipEventRecord(g.done, g.stream));
+        FSIM_HIP(hipStreamWaitEvent(b->origin, g.done, 0));
+    }
     for (BatchClass &c : b->cls) {
+        if (in_persistent_group(b, c)) continue;
         FSIM_HIP(hipStreamWaitEvent(c.stream, b->start, 0));
         JobList jl{};
         jl.jobs = b->jobs_d + c.off;
@@ -93,7 +140,7 @@ static int batch_launch(fsim_batch_s *b) {
 
 extern "C" {
 
-int fsim_batch_create(const fsim_channel *channels, int count, int use_graph, fsim_batch *out) {
+int fsim_batch_create(const fsim_channel *channels, int count, int flags, fsim_batch *out) {
     int rc = ensure_init();
     if (rc) return rc;
     if (!channels || !out || count < 1 || count > 65535) return fail(FSIM_ERR_ARG, "batch_create: bad arguments");
@@ -102,7 +149,12 @@ int fsim_batch_create(const fsim_channel *channels, int count, int use_graph, fs
     fsim_batch_s *b = new fsim_batch_s();
     b->count = count;
     b->chans.assign(channels, channels + count);
-    b->use_graph = use_graph != 0;
+    if (flags & ~(FSIM_BATCH_GRAPH | FSIM_BATCH_PERSISTENT)) {
+        delete b;
+        return fail(FSIM_ERR_ARG, "batch_create: unknown flags");
+    }
+    b->use_graph = (flags & FSIM_BATCH_GRAPH) != 0;
+    b->persistent = (flags & FSIM_BATCH_PERSISTENT) != 0;
     // size classes, in order of first appearance
     for (int i = 0; i < count; i++) {
         const ReconConst &K = channel_host_const(channels[i]);
@@ -151,6 +203,43 @@ int fsim_batch_create(const fsim_channel *channels, int count, int use_graph, fs
             J.A = c.A + ae * j;
             J.B = c.B + be * j;
         }
+    }
+    // persistent groups: class table and the stages' item lists
+    for (BatchClass &c : b->cls) {
+        if (!ok || !in_persistent_group(b, c)) continue;
+        PGroupRun &g = b->pg[persistent_group(c.n)];
+        PClass pc{};
+        pc.jl.jobs = b->jobs_d + c.off;
+        pc.jl.raw_aligned = true;
+        pc.n = c.n;
+        pc.njobs = (int)c.items.size();
+        pc.D = c.D;
+        pc.C = c.Cs;
+        pc.half = c.half ? 1 : 0;
+        pc.sep_all = sep_all_eligible(c.n, c.D, c.P) ? 1 : 0;
+        pc.twn = twiddles(c.n);
+        pc.twm = twiddles(2 * c.n);
+        ok = pc.twn && pc.twm;
+        g.hcls.push_back(pc);
+    }
+    for (PGroupRun &g : b->pg) {
+        if (!ok || g.hcls.empty()) continue;
+        std::vector<int4> all, st;
+        for (int k = 0; k < kPStages; k++) {
+            persistent_list(g.hcls.data(), (int)g.hcls.size(), k, st, g.ends[k]);
+            g.off[k] = (int)all.size();
+            all.insert(all.end(), st.begin(), st.end());
+        }
+        const size_t cb = sizeof(PClass) * g.hcls.size(), ib = sizeof(int4) * std::max<size_t>(all.size(), 1);
+        const size_t kb = sizeof(int) * kPStages * kPListCursors;
+        ok = hipMalloc(&g.cls_d, cb) == hipSuccess && hipMalloc(&g.items_d, ib) == hipSuccess &&
+             hipMalloc(&g.cur_d, kb) == hipSuccess &&
+             hipMemcpy(g.cls_d, g.hcls.data(), cb, hipMemcpyHostToDevice) == hipSuccess &&
+             (all.empty() || hipMemcpy(g.items_d, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice) ==
+                                 hipSuccess) &&
+             hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&g.done, hipEventDisableTiming) == hipSuccess;
+        b->device_bytes += (int64_t)(cb + ib + kb);
     }
     if (!ok) {
         (void)hipGetLastError();

@@ -398,27 +398,57 @@ struct TwLds {
     static constexpr int len = tl > 0 ? tl : 1;
 };
 
+// The thread's index in its block, passed through an empty asm statement so
+// that the optimizer keeps the index arithmetic at its uses: the persistent
+// work-list kernels run block functions in loops, and arithmetic that
+// depends on the thread alone would otherwise be hoisted out of every loop
+// and held in registers for all of the functions at once.
+__device__ __forceinline__ int item_tid() {
+    unsigned v = threadIdx.x;
+    asm volatile("" : "+v"(v));
+    __builtin_assume(v < 1024u);
+    return (int)v;
+}
+
 // Fills the LDS table twl of length TL (> 0) from the global table
 // g[s] = exp(-2 pi i s / TL) (rounded from double) and synchronises the
 // block; every one of the NT threads must call it.  All loads are issued
 // before the first LDS write.  With TL = 0 it does nothing.
 template <int TL, int NT>
-__device__ __forceinline__ const float2 *tw_table(float2 *twl, const float2 *g) {
+__device__ __forceinline__ const float2 *tw_table(float2 *twl, const float2 *g, int tid) {
     if constexpr (TL > 0) {
         constexpr int R = (TL + NT - 1) / NT;
         const FSIM_GLB float2 *gg = glb(g);
         float2 tmp[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int i = (int)threadIdx.x + r * NT;
+            const int i = tid + r * NT;
             if (TL % NT == 0 || i < TL) tmp[r] = ld(gg + i);
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int i = (int)threadIdx.x + r * NT;
+            const int i = tid + r * NT;
             if (TL % NT == 0 || i < TL) twl[i] = tmp[r];
         }
         __syncthreads();
+        return twl;
+    } else {
+        return g;
+    }
+}
+template <int TL, int NT>
+__device__ __forceinline__ const float2 *tw_table(float2 *twl, const float2 *g) {
+    return tw_table<TL, NT>(twl, g, (int)threadIdx.x);
+}
+// tw_table for blocks that run several items with the same table: loads it
+// (and sets loaded) only when loaded is false.  loaded must be uniform.
+template <int TL, int NT>
+__device__ __forceinline__ const float2 *tw_table(float2 *twl, const float2 *g, int tid, bool &loaded) {
+    if constexpr (TL > 0) {
+        if (!loaded) {
+            tw_table<TL, NT>(twl, g, tid);
+            loaded = true;
+        }
         return twl;
     } else {
         return g;

@@ -108,6 +108,36 @@ int launch_pipeline_cfg(int n, const JobList &jl, int njobs, int D, int P, int C
 bool recon_size_ok(int n);
 int gather_cw(int n);   // column tile width of the gather at raw side n
 int launch_pipeline(int n, const JobList &jl, int njobs, int D, int P, int Cs, bool half, hipStream_t s);
+// Persistent work-list execution of a batch's pipeline.  A launch group is
+// a set of sizes whose stage kernels share a block shape and about the same
+// registers and LDS: group 0 holds n = 256 and 512, group 1 n = 1024 and
+// 2048 (other sizes run their own launches).  Per group and stage one device
+// list of work items (class, bx, by, unused) holds the blocks of every
+// class's own launch of that stage, and one persistent kernel runs the list.
+struct PClass {
+    JobList jl;                 // the class's jobs (device job array)
+    int n, njobs, D, C, half;   // C: stored components per direction
+    int sep_all;                // separation with all units of a row in one wavefront
+    const float2 *twn, *twm;    // twiddle tables of n and 2n points
+};
+// stages: separation, column pass, row shift, gather, last row pass
+constexpr int kPStages = 5;
+constexpr int kPGroups = 2;
+int persistent_group(int n);    // -1: n runs its own launches
+// The item list of a stage over classes cls[0, ncls) of one group (host
+// copies; item class indices refer to this array): four segments, the
+// group's larger size (variants 0, 1) then the smaller, each class's run
+// padded with (-1, 0, 0, 0) to a multiple of 8; ends: the segment ends.
+void persistent_list(const PClass *cls, int ncls, int stage, std::vector<int4> &items, int ends[4]);
+// One persistent launch of a group's stage over a list (device copies);
+// cur: the list's 8 cursors (one per XCD, kPCursorStride ints apart), zero
+// before the launch (launch_persistent_reset).
+int launch_persistent(int group, int stage, const PClass *cls_d, const int4 *items_d, const int ends[4], int *cur,
+                      hipStream_t s);
+int launch_persistent_reset(int *cur, int count, hipStream_t s);
+// ints between two list cursors (256 bytes), and per list
+constexpr int kPCursorStride = 64;
+constexpr int kPListCursors = 8 * kPCursorStride;
 size_t workspace_a_elems(int n, int D, int C);
 size_t workspace_b_elems(int n);
 

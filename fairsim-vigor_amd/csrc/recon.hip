@@ -35,6 +35,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <map>
 #include <set>
 #include <utility>
@@ -109,10 +110,11 @@ __device__ __forceinline__ Job get_job(const JobList &jl, int j) {
 // dealt to them round robin by linear id) walks one contiguous range of
 // [0, total): neighbouring tiles of a component run on one XCD.  The grid
 // has xcd_grid(total) blocks; indices >= total have no work.
-__device__ __forceinline__ int xcd_linear(int total) {
+__device__ __forceinline__ int xcd_linear_at(unsigned b, int total) {
     const int per = (total + 7) / 8;
-    return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    return (int)(b % 8) * per + (int)(b / 8);
 }
+__device__ __forceinline__ int xcd_linear(int total) { return xcd_linear_at(blockIdx.x, total); }
 static inline unsigned xcd_grid(int total) { return (unsigned)((total + 7) / 8) * 8; }
 
 // Work index w -> (unit, tile) in tile-group-major order: groups of TG
@@ -192,7 +194,8 @@ __device__ __forceinline__ size_t at_row(int y) {
 // of the block (NT = n * RW / E threads, E elements each) calls them; the
 // caller places the block barriers.
 template <int N, int RW, int NT>
-__device__ __forceinline__ void tiled_rows_load(const FSIM_GLB float2 *a, int y0, float2 *rows, int RP, int ntiles) {
+__device__ __forceinline__ void tiled_rows_load(const FSIM_GLB float2 *a, int y0, float2 *rows, int RP, int ntiles,
+                                                int tid) {
     constexpr int RUN = RW * kAT, PER = N * RW / NT;
     static_assert(ATile<N>::rh % RW == 0, "a block's rows lie in one tile row");
     const FSIM_GLB float2 *base = a + at_row<N>(y0);
@@ -200,12 +203,12 @@ __device__ __forceinline__ void tiled_rows_load(const FSIM_GLB float2 *a, int y0
     float2 u[PER];
 #pragma unroll
     for (int r = 0; r < PER; r++) {
-        const int e = (int)threadIdx.x + NT * r;
+        const int e = tid + NT * r;
         if (e < lim) u[r] = ld(base + at_col<N>((e / RUN) * kAT) + e % RUN);
     }
 #pragma unroll
     for (int r = 0; r < PER; r++) {
-        const int e = (int)threadIdx.x + NT * r;
+        const int e = tid + NT * r;
         if (e < lim) {
             const int q = e % RUN;
             rows[(q / kAT) * RP + (e / RUN) * kAT + q % kAT] = u[r];
@@ -228,14 +231,15 @@ __device__ __forceinline__ void tiled_row_store_regs(FSIM_GLB float2 *a, int y, 
 }
 
 template <int N, int RW, int NT>
-__device__ __forceinline__ void tiled_rows_store(FSIM_GLB float2 *a, int y0, const float2 *rows, int RP, int ntiles) {
+__device__ __forceinline__ void tiled_rows_store(FSIM_GLB float2 *a, int y0, const float2 *rows, int RP, int ntiles,
+                                                 int tid) {
     constexpr int RUN = RW * kAT, PER = N * RW / NT;
     static_assert(ATile<N>::rh % RW == 0, "a block's rows lie in one tile row");
     FSIM_GLB float2 *base = a + at_row<N>(y0);
     const int lim = ntiles * RUN;
 #pragma unroll
     for (int r = 0; r < PER; r++) {
-        const int e = (int)threadIdx.x + NT * r;
+        const int e = tid + NT * r;
         if (e < lim) {
             const int q = e % RUN;
             st(base + at_col<N>((e / RUN) * kAT) + q, rows[(q / kAT) * RP + (e / RUN) * kAT + q % kAT]);
@@ -243,21 +247,34 @@ __device__ __forceinline__ void tiled_rows_store(FSIM_GLB float2 *a, int y0, con
     }
 }
 
+// LDS of the row kernels' items (float2 elements): the rows' transform
+// regions and the twiddle table (tl elements) and, for the complex-to-real
+// pass, the row-major copy of the block's rows.
+template <int N>
+struct RowItemLds {
+    static constexpr int E = FftE<N>::value, T = N / E;
+    static constexpr int RW = T >= 256 ? 1 : 256 / T;
+    static constexpr int rows = RW * FftLds<N, E>::len;
+};
+
+// One (stack, row block, unit) item of the one-unit-per-block separation
+// (the kernel k_sep_rows1, and the persistent work-list kernels): bx = the
+// block's position L among the stack's row blocks and units, by = the stack.
 template <int N, bool VEC>
-__global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int nrb, int units, int half,
-                                                   const float2 *__restrict__ tw) {
+__device__ __forceinline__ void sep_rows1_item(const JobList &jl, int nrb, int units, int half,
+                                               const float2 *__restrict__ tw, unsigned bx, unsigned by, float2 *lds,
+                                               float2 *twl, bool &twk) {
     constexpr int E = FftE<N>::value, T = N / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int LP = FftLds<N, E>::len;
-    __shared__ float2 lds[RW * LP];
-    const Job J = get_job(jl, blockIdx.y);
+    const Job J = get_job(jl, by);
     KRef K = cst(J.c);
     const int C = K.Cs, P = K.P;
     // unit 0 of a row block: the DC component summed over the directions;
     // unit 1 + d (C - 1) + (c - 1): stored component c >= 1 of direction d.
     // The units of one row block run on one XCD (blocks L, L + 8, ...), so
     // the raw rows they share are read from L2.
-    const int L = blockIdx.x;
+    const int L = bx;
     const int rb = (L / 8 / units) * 8 + (L % 8);
     const int u = (L / 8) % units;
     if (rb >= nrb) return;
@@ -270,9 +287,9 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int nrb, int unit
     }
     const int crow = K.srow[c];          // separation-matrix row
     using TW = TwLds<N, FSIM_SEP_TW != 0>;
-    __shared__ float2 twl[TW::len];
-    const float2 *tws = tw_table<TW::tl, T * RW>(twl, tw);
-    const int t = threadIdx.x % T, rr = threadIdx.x / T;
+    const int tid = item_tid();
+    const float2 *tws = tw_table<TW::tl, T * RW>(twl, tw, tid, twk);
+    const int t = tid % T, rr = tid / T;
     const int y = rb * RW + rr;
     const size_t nn = (size_t)N * N;
     // frames are angle-major: frame f = d * P + p
@@ -392,29 +409,40 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int nrb, int unit
     tiled_row_store_regs<N, E>(glb(J.A) + (size_t)(d * C + c) * nn, y, t, v, ntiles * kAT - 1);
 }
 
+template <int N, bool VEC>
+__global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int nrb, int units, int half,
+                                                   const float2 *__restrict__ tw) {
+    __shared__ float2 lds[RowItemLds<N>::rows];
+    __shared__ float2 twl[TwLds<N, FSIM_SEP_TW != 0>::len];
+    bool twk = false;
+    sep_rows1_item<N, VEC>(jl, nrb, units, half, tw, blockIdx.x, blockIdx.y, lds, twl, twk);
+}
+
 // K1, all units of a row in one wavefront: the row's D*P raw frames are
 // loaded once (4 pixels per load, all loads in flight together) and kept in
 // registers as 16-bit pixels; the DC sum and every stored shifted component
 // are then combined, transformed and stored in turn.  FR bounds the frames
 // held (D*P <= FR); used for n <= 512 (8 pixels per lane) and D*P <= 9,
 // where the frames fit the register budget.
+// One (stack, row block) item of the all-units separation: bx = the row
+// block, by = the stack.
 template <int N, int FR>
-__global__ void __launch_bounds__(256) k_sep_rows_all(JobList jl, int half, const float2 *__restrict__ tw) {
+__device__ __forceinline__ void sep_rows_all_item(const JobList &jl, int half, const float2 *__restrict__ tw, unsigned bx,
+                                                  unsigned by, float2 *lds, float2 *twl, bool &twk) {
     constexpr int E = FftE<N>::value, T = N / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int LP = FftLds<N, E>::len;
     static_assert(E % 4 == 0, "4 pixels per load");
     constexpr int Q = E / 4;
     using PL = FftLds<N, E>;
-    __shared__ float2 lds[RW * LP];
-    const Job J = get_job(jl, blockIdx.y);
+    const Job J = get_job(jl, by);
     KRef K = cst(J.c);
     const int C = K.Cs, P = K.P, nf = K.D * P;
     using TW = TwLds<N, FSIM_SEP_TW != 0>;
-    __shared__ float2 twl[TW::len];
-    const float2 *tws = tw_table<TW::tl, T * RW>(twl, tw);
-    const int t = threadIdx.x % T, rr = threadIdx.x / T;
-    const int y = blockIdx.x * RW + rr;
+    const int tid = item_tid();
+    const float2 *tws = tw_table<TW::tl, T * RW>(twl, tw, tid, twk);
+    const int t = tid % T, rr = tid / T;
+    const int y = bx * RW + rr;
     const size_t nn = (size_t)N * N;
     const FSIM_GLB uint16_t *raw = glb(J.raw) + (size_t)y * N;
     const FSIM_GLB double *flo = glb(K.fade_lo), *fhi = glb(K.fade_hi);
@@ -491,6 +519,14 @@ __global__ void __launch_bounds__(256) k_sep_rows_all(JobList jl, int half, cons
     }
 }
 
+template <int N, int FR>
+__global__ void __launch_bounds__(256) k_sep_rows_all(JobList jl, int half, const float2 *__restrict__ tw) {
+    __shared__ float2 lds[RowItemLds<N>::rows];
+    __shared__ float2 twl[TwLds<N, FSIM_SEP_TW != 0>::len];
+    bool twk = false;
+    sep_rows_all_item<N, FR>(jl, half, tw, blockIdx.x, blockIdx.y, lds, twl, twk);
+}
+
 // ---------------------------------------------------------------------------
 // K2: shifted components: column FFT, * conj(OTF_band), column IFFT (1/n),
 // column factor ry(y) of the sub-pixel ramp, column FFT.  Blocks own
@@ -534,30 +570,32 @@ struct ColShiftTile {
     static constexpr int wpe = lds_waves_per_simd(value, (lds + (tw_fits ? N : 0)) * 8, rt ? 5 : FSIM_COL_WPE);
 };
 
+// One block's work of the column pass: b = the block's position in the
+// pass's grid (blockIdx.x of k_col_shift).
 template <int N, int E0, int CW0, bool WC>
-__global__ void __launch_bounds__((ColShiftTile<N, E0, CW0, WC>::value), (ColShiftTile<N, E0, CW0, WC>::wpe))
-    k_col_shift(JobList jl, int nsh_all, int total, const float2 *__restrict__ tw) {
+__device__ __forceinline__ void col_shift_item(const JobList &jl, int nsh_all, int total,
+                                               const float2 *__restrict__ tw, unsigned b, float2 *lds, float2 *twl,
+                                               bool &twk) {
     using CT = ColShiftTile<N, E0, CW0, WC>;
     constexpr int E = CT::e, T = N / E;
     constexpr int CW = CT::cw;
     constexpr int tiles = N / CW;
     constexpr int LP = FftLds<N, E>::len;
-    __shared__ float2 lds[CT::lds];
     // nsh_all = D * (Cs - 1) shifted components per stack; total = stacks *
     // nsh_all * tiles
     int w;
     if (FSIM_COL_XCD) {
-        w = xcd_linear(total);
+        w = xcd_linear_at(b, total);
         if (w >= total) return;
     } else {
-        w = blockIdx.x;
+        w = b;
     }
     // twiddles: in registers, loaded once from the global table, where
     // they fit (L <= 512); else the LDS table
     constexpr bool RT = CT::rt;
     using TW = TwLds<N, CT::tw_fits>;
-    __shared__ float2 twl[TW::len];
-    const float2 *tws = tw_table<TW::tl, CT::value>(twl, tw);
+    const int tid = item_tid();
+    const float2 *tws = tw_table<TW::tl, CT::value>(twl, tw, tid, twk);
     int tile, u;
     if (FSIM_TILE_GROUPS) {
         constexpr int TG = tiles < 8 ? tiles : 8;
@@ -579,8 +617,8 @@ __global__ void __launch_bounds__((ColShiftTile<N, E0, CW0, WC>::value), (ColShi
     KRef K = cst(J.c);
     const int C = K.Cs, nsh = C - 1, dc = u % nsh_all;
     const int d = dc / nsh, c = 1 + dc % nsh;   // stored shifted component
-    const int cl = WC ? (int)threadIdx.x / T : (int)threadIdx.x % CW;
-    const int t = WC ? (int)threadIdx.x % T : (int)threadIdx.x / CW;
+    const int cl = WC ? tid / T : tid % CW;
+    const int t = WC ? tid % T : tid / CW;
     // LDS region of the column: interleaved with the tile's other columns,
     // or (WC) a contiguous one per column
     float2 *reg = WC ? lds + cl * LP : lds + cl;
@@ -641,28 +679,40 @@ __global__ void __launch_bounds__((ColShiftTile<N, E0, CW0, WC>::value), (ColShi
     for (int m = 0; m < E; m++) st(col + at_row<N>(t + T * m), v[m]);
 }
 
+template <int N, int E0, int CW0, bool WC>
+__global__ void __launch_bounds__((ColShiftTile<N, E0, CW0, WC>::value), (ColShiftTile<N, E0, CW0, WC>::wpe))
+    k_col_shift(JobList jl, int nsh_all, int total, const float2 *__restrict__ tw) {
+    using CT = ColShiftTile<N, E0, CW0, WC>;
+    __shared__ float2 lds[CT::lds];
+    __shared__ float2 twl[TwLds<N, CT::tw_fits>::len];
+    bool twk = false;
+    col_shift_item<N, E0, CW0, WC>(jl, nsh_all, total, tw, blockIdx.x, lds, twl, twk);
+}
+
 // ---------------------------------------------------------------------------
 // K3: shifted components: row IFFT (1/n), row factor rx(x) of the sub-pixel
 // ramp, row FFT.
 // ---------------------------------------------------------------------------
+// One (stack, shifted component, row block) item: bx = component x row
+// blocks + row block, by = the stack.
 template <int N>
-__global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__restrict__ tw) {
+__device__ __forceinline__ void row_shift_item(const JobList &jl, const float2 *__restrict__ tw, unsigned bx,
+                                               unsigned by, float2 *lds, float2 *twl, bool &twk) {
     constexpr int E = FftE<N>::value, T = N / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int rblocks = N / RW;
     constexpr int LP = FftLds<N, E>::len;
-    __shared__ float2 lds[RW * LP];
-    const Job J = get_job(jl, blockIdx.y);
+    const Job J = get_job(jl, by);
     KRef K = cst(J.c);
     const int C = K.Cs, nsh = C - 1;
     using TW = TwLds<N, FSIM_ROW_TW != 0>;
-    __shared__ float2 twl[TW::len];
-    const float2 *tws = tw_table<TW::tl, T * RW>(twl, tw);
-    const int rb = blockIdx.x % rblocks, dc = blockIdx.x / rblocks;
+    const int tid = item_tid();
+    const float2 *tws = tw_table<TW::tl, T * RW>(twl, tw, tid, twk);
+    const int rb = bx % rblocks, dc = bx / rblocks;
     const int d = dc / nsh, c = 1 + dc % nsh;   // stored shifted component
-    const int t = threadIdx.x % T, rr = threadIdx.x / T;
+    const int t = tid % T, rr = tid / T;
     FSIM_GLB float2 *comp = glb(J.A) + (size_t)(d * C + c) * N * N;
-    tiled_rows_load<N, RW, T * RW>(comp, rb * RW, lds, LP, N / kAT);
+    tiled_rows_load<N, RW, T * RW>(comp, rb * RW, lds, LP, N / kAT, tid);
     __syncthreads();
     constexpr bool RT = FSIM_ROW_TW == 2 && tw_reg_fits<N, E>();
     TwReg<N, E> twr;
@@ -692,7 +742,15 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
 #pragma unroll
     for (int m = 0; m < E; m++) lds[rr * LP + t + T * m] = v[m];
     __syncthreads();
-    tiled_rows_store<N, RW, T * RW>(comp, rb * RW, lds, LP, N / kAT);
+    tiled_rows_store<N, RW, T * RW>(comp, rb * RW, lds, LP, N / kAT, tid);
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__restrict__ tw) {
+    __shared__ float2 lds[RowItemLds<N>::rows];
+    __shared__ float2 twl[TwLds<N, FSIM_ROW_TW != 0>::len];
+    bool twk = false;
+    row_shift_item<N>(jl, tw, blockIdx.x, blockIdx.y, lds, twl, twk);
 }
 
 // Block -> (stack, column tile) for the gather kernel.  With >= 8 stacks in a
@@ -704,8 +762,8 @@ __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__r
 // XCD.  (Tile-group order, as in the column pass, keeps the Wiener x
 // apodization tiles in L2 but spreads the concurrent blocks over 8 stacks;
 // measured slower.)
-__device__ __forceinline__ bool gather_block(int tiles, int njobs, int &job, int &tile) {
-    const int L = blockIdx.x;
+__device__ __forceinline__ bool gather_block(unsigned b, int tiles, int njobs, int &job, int &tile) {
+    const int L = (int)b;
     if (njobs >= 8 && FSIM_JOBMAP) {
         const int xcd = L % 8, k = L / 8;
         job = xcd + 8 * (k / tiles);
@@ -792,27 +850,32 @@ struct GatherTiles {
     static constexpr int value = HALF ? N / GatherE<N>::cw + 1 : 2 * N / GatherE<N>::cw;
 };
 
+// One table of length m serves both of the gather's transforms (the n-point
+// one at stride 2), where the LDS holds it beside the tile.
+template <int N>
+using GatherTw = TwLds<2 * N, (GatherE<N>::lds + 2 * N) * 8 <= 163840>;
+
+// One block's work of the gather: b = the block's position in the gather's
+// grid (blockIdx.x of k_gather).
 template <int N, bool HALF>
-__global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::wpe) k_gather(JobList jl, int njobs, const float2 *__restrict__ twn,
-                                                 const float2 *__restrict__ twm) {
+__device__ __forceinline__ void gather_item(const JobList &jl, int njobs, const float2 *__restrict__ twn,
+                                            const float2 *__restrict__ twm, unsigned b, float2 *lds, float2 *twl,
+                                            bool &twk) {
     constexpr int M = 2 * N;
     constexpr int EN = GatherE<N>::en, TN = N / EN;
     constexpr int EM = GatherE<N>::em, TM = M / EM;
     constexpr int CW = GatherE<N>::cw;
     static_assert(!HALF || N + CW <= M, "the Nyquist tile must fit the row");
-    __shared__ float2 lds[GatherE<N>::lds];
     constexpr int tiles = GatherTiles<N, HALF>::value;
     int job, tile;
-    if (!gather_block(tiles, njobs, job, tile)) return;
-    // one table of length m serves both transforms (the n-point one at
-    // stride 2), where the LDS holds it beside the tile
-    using TW = TwLds<M, (GatherE<N>::lds + M) * 8 <= 163840>;
-    __shared__ float2 twl[TW::len];
-    const float2 *tws = tw_table<TW::tl, GatherE<N>::tk * GatherE<N>::cw>(twl, twm);
+    if (!gather_block(b, tiles, njobs, job, tile)) return;
+    using TW = GatherTw<N>;
+    const int tid = item_tid();
+    const float2 *tws = tw_table<TW::tl, GatherE<N>::tk * GatherE<N>::cw>(twl, twm, tid, twk);
     const Job J = get_job(jl, job);
     KRef K = cst(J.c);
     const int D = K.D, C = K.C, Cs = K.Cs;
-    const int cl = threadIdx.x % CW, t = threadIdx.x / CW;
+    const int cl = tid % CW, t = tid / CW;
     const int xo = tile * CW + cl;
     const FSIM_GLB float2 *A = glb(J.A);
     float2 acc[EM];
@@ -962,20 +1025,32 @@ __global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::w
     }
 }
 
+template <int N, bool HALF>
+__global__ void __launch_bounds__(GatherE<N>::tk * GatherE<N>::cw, GatherE<N>::wpe)
+    k_gather(JobList jl, int njobs, const float2 *__restrict__ twn, const float2 *__restrict__ twm) {
+    __shared__ float2 lds[GatherE<N>::lds];
+    __shared__ float2 twl[GatherTw<N>::len];
+    bool twk = false;
+    gather_item<N, HALF>(jl, njobs, twn, twm, blockIdx.x, lds, twl, twk);
+}
+
 // ---------------------------------------------------------------------------
 // K6: row IFFT of the doubled grid, real part out.
 // ---------------------------------------------------------------------------
+// One (stack, row block) item of the full-width last pass: bx = the row
+// block, by = the stack.
 template <int M>
-__global__ void __launch_bounds__(256) k_row_real(JobList jl, const float2 *__restrict__ tw) {
+__device__ __forceinline__ void row_real_item(const JobList &jl, const float2 *__restrict__ tw, unsigned bx,
+                                              unsigned by, float2 *lds) {
     constexpr int E = FftE<M>::value, T = M / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int LP = FftLds<M, E>::len;
-    __shared__ float2 lds[RW * LP];
     using TW = TwLds<M, false>;
     const float2 *tws = tw;
-    const Job J = get_job(jl, blockIdx.y);
-    const int t = threadIdx.x % T, rr = threadIdx.x / T;
-    const int y = blockIdx.x * RW + rr;
+    const Job J = get_job(jl, by);
+    const int tid = item_tid();
+    const int t = tid % T, rr = tid / T;
+    const int y = bx * RW + rr;
     const FSIM_GLB float2 *row = glb(J.B) + (size_t)y * M;
     float2 v[E];
 #pragma unroll
@@ -985,6 +1060,12 @@ __global__ void __launch_bounds__(256) k_row_real(JobList jl, const float2 *__re
     FSIM_GLB float *o = glb(J.out) + (size_t)y * M;
 #pragma unroll
     for (int m = 0; m < E; m++) o[t + T * m] = v[m].x * sc;
+}
+
+template <int M>
+__global__ void __launch_bounds__(256) k_row_real(JobList jl, const float2 *__restrict__ tw) {
+    __shared__ float2 lds[RowItemLds<M>::rows];
+    row_real_item<M>(jl, tw, blockIdx.x, blockIdx.y, lds);
 }
 
 // K6 for a Hermitian result: B holds columns 0..n of the m-point row
@@ -1002,42 +1083,50 @@ __global__ void __launch_bounds__(256) k_row_real(JobList jl, const float2 *__re
 // from which each row's threads take X[k] and X[n - k].  Blocks walk the
 // rows in XCD-contiguous ranges (the grid's x extent is a multiple of 8),
 // so rows that share cache lines are read on one XCD.
+// Row-major LDS copy of columns [0, n + cw) of a complex-to-real block's rows.
 template <int N>
-__global__ void __launch_bounds__(256) k_row_c2r(JobList jl, const float2 *__restrict__ twm) {
+struct C2rRows {
+    static constexpr int CW = GatherE<N>::cw;
+    static constexpr int RL = N + CW + 1;   // odd row pitch
+    static constexpr int value = RowItemLds<N>::RW * RL;
+};
+
+// One (stack, row group) item of the complex-to-real pass: grp = the row
+// group, by = the stack.
+template <int N>
+__device__ __forceinline__ void row_c2r_item(const JobList &jl, const float2 *__restrict__ twm, int grp, unsigned by,
+                                             float2 *lds, float2 *twl, float2 *rows, bool &twk) {
     constexpr int M = 2 * N;
     constexpr int E = FftE<N>::value, T = N / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int LP = FftLds<N, E>::len;
     constexpr int CW = GatherE<N>::cw;
     constexpr int NT = T * RW;
-    __shared__ float2 lds[RW * LP];
     // the twiddles of the n-point transform from the m-point table
     // (TL = m, a multiple of n)
     using TW = TwLds<M, FSIM_C2R_TW != 0>;
-    __shared__ float2 twl[TW::len];
-    const float2 *tws = tw_table<TW::tl, NT>(twl, twm);
-    const Job J = get_job(jl, blockIdx.y);
-    const int t = threadIdx.x % T, rr = threadIdx.x / T;
-    const int grp = (M / RW) % 8 ? (int)blockIdx.x : xcd_linear(M / RW);
+    const int tid = item_tid();
+    const float2 *tws = tw_table<TW::tl, NT>(twl, twm, tid, twk);
+    const Job J = get_job(jl, by);
+    const int t = tid % T, rr = tid / T;
     const int y = grp * RW + rr;
     const FSIM_GLB float2 *tg = glb(twm);
     float2 a[E], b[E], w[E];
     // row-major LDS copy of columns [0, n + cw) of the block's rows
-    constexpr int RL = N + CW + 1;              // odd row pitch
+    constexpr int RL = C2rRows<N>::RL;
     constexpr int RUN = RW * CW;                // elements per tile run
     constexpr int TOT = (N / CW + 1) * RUN;
     constexpr int PER = (TOT + NT - 1) / NT;
-    __shared__ float2 rows[RW * RL];
     const FSIM_GLB float2 *bt = glb(J.B) + (size_t)grp * RUN;
     float2 u[PER];
 #pragma unroll
     for (int r = 0; r < PER; r++) {
-        const int e = (int)threadIdx.x + NT * r;
+        const int e = tid + NT * r;
         if (TOT % NT == 0 || e < TOT) u[r] = ld(bt + (size_t)(e / RUN) * M * CW + e % RUN);
     }
 #pragma unroll
     for (int r = 0; r < PER; r++) {
-        const int e = (int)threadIdx.x + NT * r;
+        const int e = tid + NT * r;
         if (TOT % NT == 0 || e < TOT) {
             const int q = e % RUN;
             rows[(q / CW) * RL + (e / RUN) * CW + q % CW] = u[r];
@@ -1075,6 +1164,17 @@ __global__ void __launch_bounds__(256) k_row_c2r(JobList jl, const float2 *__res
     FSIM_GLB float2 *o = reinterpret_cast<FSIM_GLB float2 *>(glb(J.out) + (size_t)y * M);
 #pragma unroll
     for (int m = 0; m < E; m++) st(o + t + T * m, cscale(v[m], sc));
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_row_c2r(JobList jl, const float2 *__restrict__ twm) {
+    constexpr int RW = RowItemLds<N>::RW;
+    __shared__ float2 lds[RowItemLds<N>::rows];
+    __shared__ float2 twl[TwLds<2 * N, FSIM_C2R_TW != 0>::len];
+    __shared__ float2 rows[C2rRows<N>::value];
+    const int grp = (2 * N / RW) % 8 ? (int)blockIdx.x : xcd_linear(2 * N / RW);
+    bool twk = false;
+    row_c2r_item<N>(jl, twm, grp, blockIdx.y, lds, twl, rows, twk);
 }
 
 // Tile-major position of the tables the gather reads: element j of an
@@ -1346,6 +1446,401 @@ int launch_pipeline(int n, const JobList &jl, int njobs, int D, int P, int C, bo
     PipeCfg cfg;
     if (!pipe_cfg_lookup(n, D, P, C, half, cfg)) cfg = PipeCfg{};
     return launch_pipeline_cfg(n, jl, njobs, D, P, C, half, cfg, s);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent work-list kernels.  For each stage, one kernel per launch group
+// runs a device list of items (class k, bx, by): item (k, bx, by) is the work
+// of block (bx, by) of class k's own launch of that stage, run by the same
+// block function.  The blocks (as many as stay resident) claim items
+// through cursors and synchronise between items (the LDS is reused); every
+// class's run of items starts at a multiple of 8, so an item's position
+// modulo 8 is its block's index modulo 8 in the class's launch where that
+// grid's x extent is a multiple of 8, and it runs on the same XCD.  The
+// lists put the larger size first, so the small items fill the end of the
+// launch.
+// ---------------------------------------------------------------------------
+template <int G>
+struct PSizes;
+template <>
+struct PSizes<0> {
+    static constexpr int a = 256, b = 512;
+};
+template <>
+struct PSizes<1> {
+    static constexpr int a = 1024, b = 2048;
+};
+constexpr int cmax(int x, int y) { return x > y ? x : y; }
+constexpr int cmin(int x, int y) { return x < y ? x : y; }
+
+int persistent_group(int n) {
+    switch (n) {
+        case 256: case 512: return 0;
+        case 1024: case 2048: return 1;
+        default: return -1;
+    }
+}
+
+// Column pass tile of the persistent kernels: 512-thread blocks in both
+// groups (16 columns at n = 256, the same 8-point radix as its 8-column
+// default; the default tile elsewhere).
+template <int N>
+struct PCol {
+    static constexpr int e0 = N == 256 ? 8 : 0;
+    static constexpr int cw0 = N == 256 ? 16 : DefColCW<N>::value;
+    using CT = ColShiftTile<N, e0, cw0, false>;
+};
+
+// The item at position i (uniform across the block); false for padding.
+__device__ __forceinline__ bool p_item(const int4 *__restrict__ items, int i, int &k, unsigned &bx, unsigned &by) {
+    const int4 v = items[i];
+    k = __builtin_amdgcn_readfirstlane(v.x);
+    bx = (unsigned)__builtin_amdgcn_readfirstlane(v.y);
+    by = (unsigned)__builtin_amdgcn_readfirstlane(v.z);
+    return k >= 0;
+}
+
+// List walking.  Positions p of a list with p % 8 == x belong to XCD x
+// (blocks are dealt to the 8 XCDs round robin by block index): the blocks of
+// XCD x claim them one at a time through cursor x of the list (claim j is
+// position 8 j + x), so an item runs on the XCD of its block in the class's
+// own launch, and blocks that start late, behind other work on their CUs,
+// take less.  A block claims its next position while it runs the current
+// item (thread 0 issues the atomic add, and the block reads the claim from
+// LDS after the item's final barrier; two slots, so that a wave still
+// reading the previous claim is never overwritten).
+struct PWalk {
+    int *cur;     // this XCD's cursor
+    int *slot;    // two LDS ints
+    int x;        // this block's XCD
+    int par;      // slot of the next claim
+    int p;        // current position (uniform)
+};
+
+__device__ __forceinline__ void p_start(PWalk &w, int *cur, int *slot) {
+    w.x = (int)(blockIdx.x % 8);
+    w.cur = cur + w.x * kPCursorStride;
+    w.slot = slot;
+    if (threadIdx.x == 0) slot[0] = atomicAdd(w.cur, 1);
+    __syncthreads();
+    w.p = __builtin_amdgcn_readfirstlane(8 * slot[0] + w.x);
+    w.par = 1;
+}
+
+// Runs the block's items while its position lies in one list segment
+// (positions below end; segments end on multiples of 8, and a block's
+// claims grow, so the next segment starts where this one leaves the
+// position).  Each segment runs one block function in a loop of its own, so
+// the registers of the other segments' functions are not held beside it.
+template <class F>
+__device__ __forceinline__ void p_walk(const int4 *__restrict__ items, PWalk &w, int end, F &&f) {
+    // the segment's twiddle table stays in LDS from its first item on
+    bool twk = false;
+    while (w.p < end) {
+        int k;
+        unsigned bx, by;
+        const bool ok = p_item(items, w.p, k, bx, by);
+        int next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(w.cur, 1);
+        if (ok) f(k, bx, by, twk);
+        if (threadIdx.x == 0) w.slot[w.par] = next;
+        __syncthreads();
+        w.p = __builtin_amdgcn_readfirstlane(8 * w.slot[w.par] + w.x);
+        w.par ^= 1;
+    }
+}
+
+// Zeroes the list cursors before a run (a kernel, so that a captured graph
+// holds no memory set node).
+__global__ void k_p_reset(int *cur, int count) {
+    for (int i = threadIdx.x; i < count; i += blockDim.x) cur[i] = 0;
+}
+
+// The lists have four segments, ends e.x <= e.y <= e.z <= e.w: the larger
+// size (variant 0, variant 1), then the smaller.  Variants: separation with
+// all units of a row in one wavefront / one unit per block; gather and last
+// pass of a Hermitian / full result; the column pass and row shift have
+// variant 0 only.
+template <int G>
+__global__ void __launch_bounds__(256) k_p_sep(const PClass *__restrict__ cls, const int4 *__restrict__ items,
+                                               int4 e, int *cur) {
+    constexpr int A = PSizes<G>::a, B = PSizes<G>::b;
+    __shared__ float2 lds[cmax(RowItemLds<A>::rows, RowItemLds<B>::rows)];
+    __shared__ float2 twl[cmax(TwLds<A, FSIM_SEP_TW != 0>::len, TwLds<B, FSIM_SEP_TW != 0>::len)];
+    __shared__ int slot[2];
+    PWalk w;
+    p_start(w, cur, slot);
+    auto all = [&](auto n) {
+        constexpr int N = decltype(n)::value;
+        return [&](int k, unsigned bx, unsigned by, bool &twk) {
+            const PClass &c = cls[k];
+            sep_rows_all_item<N, 9>(c.jl, c.half, c.twn, bx, by, lds, twl, twk);
+        };
+    };
+    auto one = [&](auto n) {
+        constexpr int N = decltype(n)::value;
+        return [&](int k, unsigned bx, unsigned by, bool &twk) {
+            constexpr int E = FftE<N>::value, T = N / E, RW = T >= 256 ? 1 : 256 / T;
+            const PClass &c = cls[k];
+            sep_rows1_item<N, true>(c.jl, N / RW, 1 + c.D * (c.C - 1), c.half, c.twn, bx, by, lds, twl, twk);
+        };
+    };
+    if constexpr (B <= 512) p_walk(items, w, e.x, all(std::integral_constant<int, B>{}));
+    p_walk(items, w, e.y, one(std::integral_constant<int, B>{}));
+    if constexpr (A <= 512) p_walk(items, w, e.z, all(std::integral_constant<int, A>{}));
+    p_walk(items, w, e.w, one(std::integral_constant<int, A>{}));
+}
+
+template <int G>
+__global__ void __launch_bounds__(PCol<PSizes<G>::a>::CT::value,
+                                  cmin(PCol<PSizes<G>::a>::CT::wpe, PCol<PSizes<G>::b>::CT::wpe))
+    k_p_col(const PClass *__restrict__ cls, const int4 *__restrict__ items, int4 e, int *cur) {
+    constexpr int A = PSizes<G>::a, B = PSizes<G>::b;
+    using CA = typename PCol<A>::CT;
+    using CB = typename PCol<B>::CT;
+    static_assert(CA::value == CB::value, "one block shape per group");
+    __shared__ float2 lds[cmax(CA::lds, CB::lds)];
+    __shared__ float2 twl[cmax(TwLds<A, CA::tw_fits>::len, TwLds<B, CB::tw_fits>::len)];
+    __shared__ int slot[2];
+    PWalk w;
+    p_start(w, cur, slot);
+    auto col = [&](auto n) {
+        constexpr int N = decltype(n)::value;
+        return [&](int k, unsigned bx, unsigned, bool &twk) {
+            using P = PCol<N>;
+            const PClass &c = cls[k];
+            const int nsh_all = c.D * (c.C - 1);
+            col_shift_item<N, P::e0, P::cw0, false>(c.jl, nsh_all, c.njobs * nsh_all * (N / P::CT::cw), c.twn, bx,
+                                                    lds, twl, twk);
+        };
+    };
+    p_walk(items, w, e.y, col(std::integral_constant<int, B>{}));
+    p_walk(items, w, e.w, col(std::integral_constant<int, A>{}));
+}
+
+template <int G>
+__global__ void __launch_bounds__(256) k_p_row_shift(const PClass *__restrict__ cls, const int4 *__restrict__ items,
+                                                     int4 e, int *cur) {
+    constexpr int A = PSizes<G>::a, B = PSizes<G>::b;
+    __shared__ float2 lds[cmax(RowItemLds<A>::rows, RowItemLds<B>::rows)];
+    __shared__ float2 twl[cmax(TwLds<A, FSIM_ROW_TW != 0>::len, TwLds<B, FSIM_ROW_TW != 0>::len)];
+    __shared__ int slot[2];
+    PWalk w;
+    p_start(w, cur, slot);
+    auto rs = [&](auto n) {
+        constexpr int N = decltype(n)::value;
+        return [&](int k, unsigned bx, unsigned by, bool &twk) { row_shift_item<N>(cls[k].jl, cls[k].twn, bx, by, lds, twl, twk); };
+    };
+    p_walk(items, w, e.y, rs(std::integral_constant<int, B>{}));
+    p_walk(items, w, e.w, rs(std::integral_constant<int, A>{}));
+}
+
+template <int G>
+__global__ void __launch_bounds__(GatherE<PSizes<G>::a>::tk * GatherE<PSizes<G>::a>::cw,
+                                  cmin(GatherE<PSizes<G>::a>::wpe, GatherE<PSizes<G>::b>::wpe))
+    k_p_gather(const PClass *__restrict__ cls, const int4 *__restrict__ items, int4 e, int *cur) {
+    constexpr int A = PSizes<G>::a, B = PSizes<G>::b;
+    static_assert(GatherE<A>::tk * GatherE<A>::cw == GatherE<B>::tk * GatherE<B>::cw, "one block shape per group");
+    __shared__ float2 lds[cmax(GatherE<A>::lds, GatherE<B>::lds)];
+    __shared__ float2 twl[cmax(GatherTw<A>::len, GatherTw<B>::len)];
+    __shared__ int slot[2];
+    PWalk w;
+    p_start(w, cur, slot);
+    auto ga = [&](auto n, auto h) {
+        constexpr int N = decltype(n)::value;
+        constexpr bool H = decltype(h)::value;
+        return [&](int k, unsigned bx, unsigned, bool &twk) {
+            const PClass &c = cls[k];
+            gather_item<N, H>(c.jl, c.njobs, c.twn, c.twm, bx, lds, twl, twk);
+        };
+    };
+    p_walk(items, w, e.x, ga(std::integral_constant<int, B>{}, std::true_type{}));
+    p_walk(items, w, e.y, ga(std::integral_constant<int, B>{}, std::false_type{}));
+    p_walk(items, w, e.z, ga(std::integral_constant<int, A>{}, std::true_type{}));
+    p_walk(items, w, e.w, ga(std::integral_constant<int, A>{}, std::false_type{}));
+}
+
+// LDS of the last pass's items: a complex-to-real item's row regions,
+// twiddle table and row-major copy, or a full-width item's row regions,
+// carved from one array (an item uses one or the other).
+template <int N>
+struct PLastLds {
+    static constexpr int lds = RowItemLds<N>::rows, twl = TwLds<2 * N, FSIM_C2R_TW != 0>::len;
+    static constexpr int rows = (lds + twl + 1) / 2 * 2;   // 16-byte aligned
+    static constexpr int c2r = rows + C2rRows<N>::value;
+    static constexpr int value = cmax(c2r, RowItemLds<2 * N>::rows);
+};
+
+template <int G>
+__global__ void __launch_bounds__(256) k_p_last(const PClass *__restrict__ cls, const int4 *__restrict__ items,
+                                                int4 e, int *cur) {
+    constexpr int A = PSizes<G>::a, B = PSizes<G>::b;
+    __shared__ __align__(16) float2 buf[cmax(PLastLds<A>::value, PLastLds<B>::value)];
+    __shared__ int slot[2];
+    PWalk w;
+    p_start(w, cur, slot);
+    auto c2r = [&](auto n) {
+        constexpr int N = decltype(n)::value;
+        return [&](int k, unsigned bx, unsigned by, bool &twk) {
+            using L = PLastLds<N>;
+            constexpr int M = 2 * N, E = FftE<N>::value, T = N / E, RW = T >= 256 ? 1 : 256 / T;
+            const int grp = (M / RW) % 8 ? (int)bx : xcd_linear_at(bx, M / RW);
+            row_c2r_item<N>(cls[k].jl, cls[k].twm, grp, by, buf, buf + L::lds, buf + L::rows, twk);
+        };
+    };
+    auto real = [&](auto n) {
+        constexpr int N = decltype(n)::value;
+        return [&](int k, unsigned bx, unsigned by, bool &) { row_real_item<2 * N>(cls[k].jl, cls[k].twm, bx, by, buf); };
+    };
+    p_walk(items, w, e.x, c2r(std::integral_constant<int, B>{}));
+    p_walk(items, w, e.y, real(std::integral_constant<int, B>{}));
+    p_walk(items, w, e.z, c2r(std::integral_constant<int, A>{}));
+    p_walk(items, w, e.w, real(std::integral_constant<int, A>{}));
+}
+
+// Grid (gx, gy) of class c's own launch of a stage (0 x 0: the stage does
+// not run for the class).
+template <int N>
+static void pstage_grid(const PClass &c, int stage, int &gx, int &gy) {
+    constexpr int M = 2 * N;
+    constexpr int E = FftE<N>::value, T = N / E, RW = T >= 256 ? 1 : 256 / T;
+    gx = gy = 0;
+    switch (stage) {
+        case 0: {
+            const int nrb = N / RW, units = 1 + c.D * (c.C - 1);
+            gx = c.sep_all ? nrb : ((nrb + 7) / 8) * 8 * units;
+            gy = c.njobs;
+            break;
+        }
+        case 1:
+            if (c.C > 1) {
+                const int total = c.njobs * c.D * (c.C - 1) * (N / PCol<N>::CT::cw);
+                gx = FSIM_COL_XCD ? (int)xcd_grid(total) : total;
+                gy = 1;
+            }
+            break;
+        case 2:
+            if (c.C > 1) {
+                gx = c.D * (c.C - 1) * (N / RW);
+                gy = c.njobs;
+            }
+            break;
+        case 3:
+            gx = (int)gather_grid(c.half ? GatherTiles<N, true>::value : GatherTiles<N, false>::value, c.njobs);
+            gy = 1;
+            break;
+        default:
+            if (c.half) {
+                gx = M / RW;
+            } else {
+                constexpr int EM = FftE<M>::value, TM = M / EM, RWM = TM >= 256 ? 1 : 256 / TM;
+                gx = M / RWM;
+            }
+            gy = c.njobs;
+            break;
+    }
+}
+
+// Segment of class c's items in a stage's list (0, 1: the group's larger
+// size, variants 0 and 1; 2, 3: the smaller size).
+static int pstage_segment(const PClass &c, int stage) {
+    const int g = persistent_group(c.n);
+    const int big = g == 0 ? 512 : 2048;
+    int v = 0;
+    if (stage == 0) v = c.sep_all ? 0 : 1;
+    if (stage == 3 || stage == 4) v = c.half ? 0 : 1;
+    return (c.n == big ? 0 : 2) + v;
+}
+
+void persistent_list(const PClass *cls, int ncls, int stage, std::vector<int4> &items, int ends[4]) {
+    items.clear();
+    for (int seg = 0; seg < 4; seg++) {
+        for (int k = 0; k < ncls; k++) {
+            const PClass &c = cls[k];
+            if (persistent_group(c.n) < 0 || pstage_segment(c, stage) != seg) continue;
+            int gx = 0, gy = 0;
+            switch (c.n) {
+                case 256: pstage_grid<256>(c, stage, gx, gy); break;
+                case 512: pstage_grid<512>(c, stage, gx, gy); break;
+                case 1024: pstage_grid<1024>(c, stage, gx, gy); break;
+                default: pstage_grid<2048>(c, stage, gx, gy); break;
+            }
+            for (int y = 0; y < gy; y++)
+                for (int x = 0; x < gx; x++) items.push_back(make_int4(k, x, y, 0));
+            while (items.size() % 8) items.push_back(make_int4(-1, 0, 0, 0));
+        }
+        ends[seg] = (int)items.size();
+    }
+}
+
+// Kernel and block size of a group's stage.
+static void pstage_kernel(int group, int stage, const void *&fn, int &threads) {
+    threads = 256;
+    switch (group * kPStages + stage) {
+        case 0: fn = (const void *)k_p_sep<0>; break;
+        case 1: fn = (const void *)k_p_col<0>; threads = PCol<256>::CT::value; break;
+        case 2: fn = (const void *)k_p_row_shift<0>; break;
+        case 3: fn = (const void *)k_p_gather<0>; threads = GatherE<256>::tk * GatherE<256>::cw; break;
+        case 4: fn = (const void *)k_p_last<0>; break;
+        case 5: fn = (const void *)k_p_sep<1>; break;
+        case 6: fn = (const void *)k_p_col<1>; threads = PCol<1024>::CT::value; break;
+        case 7: fn = (const void *)k_p_row_shift<1>; break;
+        case 8: fn = (const void *)k_p_gather<1>; threads = GatherE<1024>::tk * GatherE<1024>::cw; break;
+        default: fn = (const void *)k_p_last<1>; break;
+    }
+}
+
+int launch_persistent_reset(int *cur, int count, hipStream_t s) {
+    hipLaunchKernelGGL(k_p_reset, dim3(1), dim3(256), 0, s, cur, count);
+    FSIM_HIP(hipGetLastError());
+    return FSIM_OK;
+}
+
+int launch_persistent(int group, int stage, const PClass *cls_d, const int4 *items_d, const int ends[4], int *cur,
+                      hipStream_t s) {
+    if (group < 0 || group >= kPGroups || stage < 0 || stage >= kPStages)
+        return fail(FSIM_ERR_ARG, "launch_persistent: bad group or stage");
+    const int nitems = ends[3];
+    if (nitems <= 0) return FSIM_OK;
+    const int4 e = make_int4(ends[0], ends[1], ends[2], ends[3]);
+    // resident blocks per CU of each kernel, looked up once per device
+    static std::mutex mu;
+    static int resident[64][kPGroups * kPStages] = {};
+    static int cus[64] = {};
+    int dev = 0;
+    FSIM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(FSIM_ERR_ARG, "launch_persistent: device index out of range");
+    const void *fn = nullptr;
+    int threads = 256;
+    pstage_kernel(group, stage, fn, threads);
+    int per_cu, ncu;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        int &r = resident[dev][group * kPStages + stage];
+        if (!cus[dev]) FSIM_HIP(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
+        if (!r) {
+            FSIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, fn, threads, 0));
+            if (r < 1) return fail(FSIM_ERR_HIP, "launch_persistent: kernel does not fit a CU");
+        }
+        per_cu = r;
+        ncu = cus[dev];
+    }
+    // a multiple of 8 blocks, an equal share per XCD
+    const int grid = std::max(8, std::min(per_cu * ncu, nitems) / 8 * 8);
+    switch (group * kPStages + stage) {
+        case 0: hipLaunchKernelGGL(k_p_sep<0>, dim3(grid), dim3(threads), 0, s, cls_d, items_d, e, cur); break;
+        case 1: hipLaunchKernelGGL(k_p_col<0>, dim3(grid), dim3(threads), 0, s, cls_d, items_d, e, cur); break;
+        case 2: hipLaunchKernelGGL(k_p_row_shift<0>, dim3(grid), dim3(threads), 0, s, cls_d, items_d, e, cur); break;
+        case 3: hipLaunchKernelGGL(k_p_gather<0>, dim3(grid), dim3(threads), 0, s, cls_d, items_d, e, cur); break;
+        case 4: hipLaunchKernelGGL(k_p_last<0>, dim3(grid), dim3(threads), 0, s, cls_d, items_d, e, cur); break;
+        case 5: hipLaunchKernelGGL(k_p_sep<1>, dim3(grid), dim3(threads), 0, s, cls_d, items_d, e, cur); break;
+        case 6: hipLaunchKernelGGL(k_p_col<1>, dim3(grid), dim3(threads), 0, s, cls_d, items_d, e, cur); break;
+        case 7: hipLaunchKernelGGL(k_p_row_shift<1>, dim3(grid), dim3(threads), 0, s, cls_d, items_d, e, cur); break;
+        case 8: hipLaunchKernelGGL(k_p_gather<1>, dim3(grid), dim3(threads), 0, s, cls_d, items_d, e, cur); break;
+        default: hipLaunchKernelGGL(k_p_last<1>, dim3(grid), dim3(threads), 0, s, cls_d, items_d, e, cur); break;
+    }
+    FSIM_HIP(hipGetLastError());
+    return FSIM_OK;
 }
 
 int launch_pipeline_cfg(int n, const JobList &jl, int njobs, int D, int P, int C, bool half, const PipeCfg &cfg,

@@ -107,15 +107,21 @@ class ReconstructionBatch:
     one device raw address and one device output address per slot (8-byte
     aligned); the class pipelines run concurrently and, with
     ``use_graph``, as one replayed hipGraph whose job lists carry the
-    addresses of the run."""
+    addresses of the run.  With ``persistent`` the slots of sides 256 to
+    2048 run through persistent work-list kernels: per stage one launch per
+    group of sides (256/512, 1024/2048) walks the (slot class, tile or row
+    block) items of all its slots."""
 
-    def __init__(self, reconstructors, use_graph=True):
+    GRAPH, PERSISTENT = 1, 2
+
+    def __init__(self, reconstructors, use_graph=True, persistent=False):
         _lib.init()
         self._recs = list(reconstructors)
         n = len(self._recs)
         hs = (C.c_void_p * n)(*[r.handle.value for r in self._recs])
         self._h = C.c_void_p()
-        call("fsim_batch_create", hs, n, 1 if use_graph else 0, C.byref(self._h))
+        flags = (self.GRAPH if use_graph else 0) | (self.PERSISTENT if persistent else 0)
+        call("fsim_batch_create", hs, n, flags, C.byref(self._h))
         self.count = n
 
     def run(self, raw_ptrs, out_ptrs, stream=None):

@@ -462,13 +462,21 @@ int fsim_server_get_stats(fsim_server s, fsim_server_stats *st, int reset);
  * reconstructs raw[i] (device, D*P frames of n*n uint16) into out[i] (device,
  * (2n)^2 float32) for every slot, stream-ordered on `stream` (NULL: the
  * library stream): the class pipelines run concurrently, and with
- * use_graph != 0 the launch sequence is captured as a hipGraph on the first
- * run and replayed afterwards, the slot addresses being rewritten in the
- * device job lists before each replay.  raw[i] and out[i] must be 8-byte
+ * flags & FSIM_BATCH_GRAPH the launch sequence is captured as a hipGraph on
+ * the first run and replayed afterwards, the slot addresses being rewritten
+ * in the device job lists before each replay.  With flags &
+ * FSIM_BATCH_PERSISTENT the slots of sides 256 to 2048 run through
+ * persistent kernels instead of per-class launches: for each pipeline stage
+ * one launch per group of sides (256 and 512; 1024 and 2048) walks a device
+ * list of (slot class, tile or row block) work items, larger side first;
+ * other sides keep their class launches.  raw[i] and out[i] must be 8-byte
  * aligned.  One run at a time per batch (runs on one stream serialise).
- * Results are bitwise those of fsim_reconstruct on the same stacks. */
+ * Results are bitwise those of fsim_reconstruct on the same stacks with the
+ * default launch choices (no autotuned choice set for the class). */
+#define FSIM_BATCH_GRAPH 1
+#define FSIM_BATCH_PERSISTENT 2
 typedef struct fsim_batch_s *fsim_batch;
-int fsim_batch_create(const fsim_channel *channels, int count, int use_graph, fsim_batch *out);
+int fsim_batch_create(const fsim_channel *channels, int count, int flags, fsim_batch *out);
 int fsim_batch_run(fsim_batch b, const uint16_t *const *raw, float *const *out, void *stream);
 int fsim_batch_info(fsim_batch b, int *nclasses, int *captured, int64_t *device_bytes);
 int fsim_batch_destroy(fsim_batch b);

@@ -150,3 +150,67 @@ def test_batch_and_params_argument_errors(setup):
     with pytest.raises(FsimError):
         srv.submit_params(cid, raw, out, 0, SIM_PARAMS())
     srv.close()
+
+
+def test_persistent_batch_every_group_and_variant(fsim_lib):
+    """Persistent work-list kernels (FSIM_BATCH_PERSISTENT): one batch with
+    both launch groups (256/512 and 1024/2048 sides), every list segment
+    (all-units and one-unit separation; Hermitian and full-width gather and
+    last pass), a class of nine slots (the gather's cross-stack block order)
+    and a 128^2 slot that keeps its own class launches.  Eager and as a
+    replayed graph with the stacks swapped between replays.  Every slot is
+    bitwise equal to fsim_reconstruct of its stack: the items run the block
+    functions of the per-class kernels, and multiply-adds fuse within one
+    source expression only (-ffp-contract=on), so they round alike wherever
+    they are inlined."""
+    from fsim import synthetic
+    from fsim._lib import call
+    from fsim.linalg import DeviceBuffer
+    from fsim.livemode import ReconstructionBatch
+    from fsim.sim_algorithm import Reconstructor
+    # (n, wavelength, phases, bands, mode, slots); mode 1: no Hermitian pairing
+    specs = [(2048, 525.0, 3, 2, 0, 1), (1024, 600.0, 3, 2, 0, 2), (1024, 525.0, 3, 2, 1, 1),
+             (512, 670.0, 3, 2, 0, 2), (512, 525.0, 5, 3, 0, 2), (256, 600.0, 3, 2, 0, 9),
+             (256, 525.0, 3, 2, 1, 2), (128, 525.0, 3, 2, 0, 2)]
+    recs, pairs, order = [], [], []
+    for i, (n, lam, nph, nb, mode, nslots) in enumerate(specs):
+        k = 0.45 if nb == 3 else 0.85
+        stacks = []
+        for t in range(2):
+            raw, sp = synthetic.sim_stack(n=n, nphase=nph, seed=700 + 10 * i + t, lam=lam, nbeads=max(30, 300 * n // 512),
+                                          nfil=max(5, 40 * n // 512), harmonic2=0.3 if nb == 3 else 0.0, k_frac=k)
+            stacks.append(raw)
+        rec = Reconstructor(synthetic.make_param(sp, n, lam=lam, nphase=nph, nband=nb), mode=mode)
+        recs.append(rec)
+        exp = [rec.reconstruct_host(s) for s in stacks]
+        bufs = []
+        for s in stacks:
+            b = DeviceBuffer(s.nbytes)
+            b.upload(s)
+            bufs.append(b)
+        pairs.append((bufs, exp))
+        order += [(i, j % 2) for j in range(nslots)]
+    rng = np.random.default_rng(5)
+    order = [order[j] for j in rng.permutation(len(order))]
+    outs = [DeviceBuffer(16 * specs[i][0] ** 2) for i, _ in order]
+    worst = {}
+    for use_graph in (False, True):
+        b = ReconstructionBatch([recs[i] for i, _ in order], use_graph=use_graph, persistent=True)
+        for rep in range(3 if use_graph else 1):
+            sel = [(i, t ^ (rep % 2)) for i, t in order]
+            print(f"[persistent] graph={use_graph} run {rep}: {len(sel)} slots", flush=True)
+            b.run([pairs[i][0][t].value for i, t in sel], [o.value for o in outs])
+            call("fsim_sync")
+            bad = []
+            for o, (i, t) in zip(outs, sel):
+                n = specs[i][0]
+                got = o.download((2 * n, 2 * n), np.float32)
+                want = pairs[i][1][t]
+                e = float(np.abs(got - want).max() / np.abs(want).max())
+                worst[specs[i][0]] = max(worst.get(specs[i][0], 0.0), e)
+                if not np.array_equal(got, want):
+                    bad.append((specs[i], t, e))
+            assert not bad, f"graph={use_graph} run {rep}: {bad}"
+        assert b.info()["captured"] == use_graph
+        b.close()
+    print(f"[persistent] max difference from the single calls per side: {worst}")

@@ -7,7 +7,8 @@ fitted grid positions exactly):
   3 bands fitting band 2, batched (fsim_estimate);
 * a mixed batch of 256^2, 512^2 and 1024^2 stacks over three emission
   channels (525 / 600 / 670 nm, each with its own OTF and k0), one fixed-shape
-  batch replayed as a hipGraph (fsim_batch_*);
+  batch replayed as a hipGraph (fsim_batch_*), with per-class launches and
+  through the persistent work-list kernels;
 * a served Poisson stream of those sizes and channels (fsim_server_*), the
   concurrency ReconstructionRunner's threads model
   (ReconstructionRunner.java:243-251);
@@ -96,10 +97,12 @@ def mixed(fsim_lib):
     return stacks, recs, single, oracle
 
 
-def test_mixed_size_batch_three_channels(mixed):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_mixed_size_batch_three_channels(mixed, persistent):
     """One fixed-shape batch of 15 distinct stacks (256^2 / 512^2 / 1024^2 x
     525 / 600 / 670 nm) in mixed slot order, replayed as a graph with the
-    slots' stacks permuted between replays; every slot bitwise equal to its
+    slots' stacks permuted between replays, with per-class launches or
+    through the persistent work-list kernels; every slot bitwise equal to its
     stack's single call and within 1e-4 of the oracle."""
     from fsim._lib import call
     from fsim.linalg import DeviceBuffer
@@ -113,7 +116,7 @@ def test_mixed_size_batch_three_channels(mixed):
         bufs[k] = DeviceBuffer(stacks[k].nbytes)
         bufs[k].upload(stacks[k])
     outs = [DeviceBuffer(16 * k[0] * k[0]) for k in order]
-    b = ReconstructionBatch([recs[k[:2]] for k in order], use_graph=True)
+    b = ReconstructionBatch([recs[k[:2]] for k in order], use_graph=True, persistent=persistent)
     for rep in range(3):
         # slots of one (size, channel) exchange their stacks between replays
         perm = list(order)
@@ -123,7 +126,7 @@ def test_mixed_size_batch_three_channels(mixed):
                     perm[i] = (k[0], k[1], 1)
                 elif k[2] == 1:
                     perm[i] = (k[0], k[1], 0)
-        print(f"[batch] replay {rep}: {len(order)} slots", flush=True)
+        print(f"[batch] persistent={persistent} replay {rep}: {len(order)} slots", flush=True)
         b.run([bufs[k].value for k in perm], [o.value for o in outs])
         call("fsim_sync")
         for i, k in enumerate(perm):
