@@ -117,15 +117,19 @@ def recon_dict(sp, n, nband=2, na=1.4, lam=525.0, px_um=0.080, a=1.0, att_str=0.
                att_fwhm=1.2, use_att=True, wiener=0.05, apo_bend=0.9, apo_cut=2.0,
                fade_px=10, fast_shift=1, modulation=None, otf_n=512):
     """Plain-dict description of a reconstruction from synthetic parameters
-    (fsim.synthetic.sim_stack); setPxPy divides by nband-1 (SimParam.java:536-541)."""
+    (fsim.synthetic.sim_stack).  px/py are the band-1 shift pX/pY
+    (SimParam.java:566-569: band b sits at b * pX): sim_stack reports the
+    band-1 shift, and fsim.synthetic.make_param hands the highest band's
+    shift sp['px'] * (nband - 1) to setPxPy, which divides it by nband - 1
+    (SimParam.java:536-541), so both describe the same pattern."""
     ndir = len(sp["px"])
     nphase = len(sp["phases"][0])
     cutoff = 1000 / (lam / na / 2)
     m = sp["modul"] if modulation is None else modulation
     return dict(
         n=n, ndir=ndir, nphase=nphase, nband=nband,
-        px=[sp["px"][d] * (nband - 1) / (nband - 1) for d in range(ndir)],
-        py=[sp["py"][d] * (nband - 1) / (nband - 1) for d in range(ndir)],
+        px=[sp["px"][d] for d in range(ndir)],
+        py=[sp["py"][d] for d in range(ndir)],
         phases=[list(sp["phases"][d]) for d in range(ndir)],
         modul=[[1.0] + [m] * (nband - 1) for _ in range(ndir)],
         otf_n=otf_n, otf_cpm=cutoff / otf_n, otf_cutoff=cutoff,

@@ -22,9 +22,7 @@ def _case(n, seed, nphase=3, nband=2, harmonic2=0.0):
     from fsim import synthetic
     raw, sp = synthetic.sim_stack(n=n, nphase=nphase, seed=seed, nbeads=max(30, 300 * n // 512),
                                   nfil=max(5, 40 * n // 512), harmonic2=harmonic2)
-    p = ref.recon_dict(sp, n, nband=nband)
-    if nband == 3:
-        p["px"] = [v / 2 * 2 for v in sp["px"]]
+    p = ref.recon_dict(sp, n, nband=nband)   # band-1 shift, also with 3 bands
     vals = fp64.otf_from_estimate(NA, LAM, 1.0)
     if isinstance(vals, tuple):
         vals = vals[0]
