@@ -271,6 +271,28 @@ def test_filters_match_oracle(vf):
     assert np.array_equal(f.vectorData(), ef)
 
 
+@pytest.mark.parametrize("kx,ky", [(0.0, 0.0), (9.25, -6.5), (-40.0, 31.75)])
+def test_mask_otf_bit_exact(vf, kx, ky):
+    """maskOtf (OtfProvider.java:538-559) on random data, centred and at
+    fractional band centres, against the oracle's mask (pinned as the cutoff
+    disc by the oracle tests): bitwise (the kept pixels are the input, the
+    others zero)."""
+    n = 128
+    o = _otf()
+    vec_cpm = 1 / (n * 0.08)
+    o.setPixelSize(vec_cpm)
+    rng = np.random.default_rng(21)
+    z = rnd_c(rng, n, n)
+    v = cvec(vf, z)
+    o.maskOtf(v, kx, ky)
+    want = np.ascontiguousarray(z.view(np.float32).reshape(n, n, 2))
+    ref.lib().ref_mask_otf(ref.ptr(want), n, n, vec_cpm, o.cutoff, kx, ky)
+    got = v.vectorData()
+    assert np.array_equal(got.view(np.float32).reshape(n, n, 2), want)
+    kept = np.count_nonzero(want[..., 0])
+    assert 0 < kept < n * n       # both sides of the cutoff occur
+
+
 def test_locate_peak_bit_exact(vf):
     from fsim.sim_algorithm import locate_peak
     rng = np.random.default_rng(10)
