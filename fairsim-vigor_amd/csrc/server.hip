@@ -68,6 +68,7 @@ struct Slot {
     std::vector<Request *> reqs;
     bool busy = false;
     double t_launch = 0;
+    uint64_t last_use = 0;     // order of the slot's last batch (eviction picks the oldest)
 };
 
 static size_t per_stack_bytes(const ClassKey &k) {
@@ -93,6 +94,7 @@ struct Server {
     std::vector<Slot *> inflight;
     size_t bytes = 0;
     uint64_t next_ticket = 1;
+    uint64_t use_clock = 0;
     bool stop = false;
     std::thread sched, compl_;
     // statistics
@@ -154,31 +156,50 @@ struct Server {
         return s;
     }
 
-    // Pick a free slot for class k: reuse one, allocate within the budget, or
-    // evict free slots of other classes.  Called with mu held.
+    int busy_slots() const {
+        int b = 0;
+        for (const Slot *s : slots) b += s->busy ? 1 : 0;
+        return b;
+    }
+
+    // Pick a free slot for class k: at most nslots batches are in flight; an
+    // idle slot of class k is reused; else a new one is allocated within
+    // mem_budget, evicting idle slots of other classes, least recently used
+    // first, only as far as the budget needs (freeing device memory waits
+    // for the device, so idle slots keep their memory while it fits).
+    // Called with mu held.
     Slot *acquire_slot(const ClassKey &k) {
+        if (busy_slots() >= cfg.nslots) return nullptr;
         for (Slot *s : slots)
             if (!s->busy && s->key == k) return s;
         const size_t need = slot_bytes(k, cap_for(k));
-        if ((int)slots.size() >= cfg.nslots || bytes + need > (size_t)cfg.mem_budget) {
-            for (size_t i = 0; i < slots.size();) {
-                Slot *s = slots[i];
-                if (!s->busy && !(s->key == k)) {
-                    bytes -= s->bytes;
-                    free_slot(s);
-                    slots.erase(slots.begin() + i);
-                    if ((int)slots.size() < cfg.nslots && bytes + need <= (size_t)cfg.mem_budget) break;
-                } else {
-                    i++;
-                }
-            }
+        while (bytes + need > (size_t)cfg.mem_budget) {
+            Slot *victim = nullptr;
+            for (Slot *s : slots)
+                if (!s->busy && !(s->key == k) && (!victim || s->last_use < victim->last_use)) victim = s;
+            if (!victim) return nullptr;
+            bytes -= victim->bytes;
+            slots.erase(std::find(slots.begin(), slots.end(), victim));
+            free_slot(victim);
         }
-        if ((int)slots.size() >= cfg.nslots || bytes + need > (size_t)cfg.mem_budget) return nullptr;
         Slot *s = alloc_slot(k);
         if (!s) return nullptr;
         slots.push_back(s);
         bytes += s->bytes;
         return s;
+    }
+
+    // A slot for the class of a newly added channel, so that its first
+    // requests do not wait for the allocation (skipped when the budget has
+    // no room for it beside the existing slots).  Called with mu held.
+    void warm_slot(const ClassKey &k) {
+        for (Slot *s : slots)
+            if (s->key == k) return;
+        if (bytes + slot_bytes(k, cap_for(k)) > (size_t)cfg.mem_budget) return;
+        Slot *s = alloc_slot(k);
+        if (!s) return;
+        slots.push_back(s);
+        bytes += s->bytes;
     }
 
     ClassKey key_of(const Request *r) const {
@@ -291,6 +312,7 @@ struct Server {
                 continue;
             }
             s->busy = true;
+            s->last_use = ++use_clock;
             s->reqs.clear();
             for (auto it = pending.begin(); it != pending.end() && (int)s->reqs.size() < s->cap;) {
                 if (key_of(*it) == k) {
@@ -401,6 +423,9 @@ int fsim_server_add_channel(fsim_server h, fsim_channel ch, int *id) {
     std::lock_guard<std::mutex> lk(h->s.mu);
     h->s.channels.push_back(ch);
     *id = (int)h->s.channels.size() - 1;
+    Request probe;
+    probe.chan = *id;
+    h->s.warm_slot(h->s.key_of(&probe));
     return FSIM_OK;
 }
 

@@ -408,11 +408,15 @@ int fsim_register_images(fsim_registration r, int widefield, const float *src_de
  * requests are pending, as the reference drops frames when its queue is full
  * (:366-370) -- and wait() takes a result as finalRecon.take() does (:57-58).
  * A scheduler thread batches pending requests of one size class (n, directions,
- * phases, bands; channels may differ within a batch), admits a batch when a
- * device slot of that class is free or mem_budget allows a new one (free slots
- * of other classes are evicted first), and runs each batch on its own stream,
- * so that up to nslots batches overlap on the device; requests arriving while
- * batches run join the next free slot.  Channels stay owned by the caller and
+ * phases, bands; channels may differ within a batch), admits a batch when
+ * fewer than nslots batches are in flight and an idle device slot of that
+ * class exists or mem_budget allows a new one (idle slots of other classes
+ * are freed, least recently used first, only as far as the budget needs:
+ * freeing device memory waits for the device), and runs each batch on its
+ * slot's stream, so that up to nslots batches overlap on the device;
+ * requests arriving while batches run join the next free slot.
+ * add_channel allocates a slot for the channel's class when none exists and
+ * the budget has room, so the first requests do not wait for it.  Channels stay owned by the caller and
  * must outlive the server.  Host buffers are copied asynchronously: pass
  * pinned memory (fsim_alloc_host) for overlap.  FSIM_SUBMIT_DEVICE: raw/out
  * are device pointers.  Results are bitwise independent of batch composition.

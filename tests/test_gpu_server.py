@@ -95,6 +95,26 @@ def test_small_memory_budget_evicts_and_completes(setup):
     srv.close()
 
 
+def test_add_channel_warms_a_slot_within_budget(setup):
+    """add_channel allocates a slot for its class when the budget has room
+    (one per class, not per channel) and none when it has not."""
+    from fsim.livemode import ReconstructionServer
+    srv = ReconstructionServer(mem_budget=1 << 30, max_batch=4, max_queue=16, nslots=2)
+    assert srv.stats()["device_bytes"] == 0
+    srv.add_channel(setup[(128, 600.0)][1])
+    one = srv.stats()["device_bytes"]
+    assert one > 0
+    srv.add_channel(setup[(128, 670.0)][1])     # same size class
+    assert srv.stats()["device_bytes"] == one
+    srv.add_channel(setup[(256, 525.0)][1])
+    assert srv.stats()["device_bytes"] > one
+    srv.close()
+    tiny = ReconstructionServer(mem_budget=1 << 20, max_batch=4, max_queue=16, nslots=1)
+    tiny.add_channel(setup[(256, 525.0)][1])
+    assert tiny.stats()["device_bytes"] == 0
+    tiny.close()
+
+
 def test_device_resident_submit(setup):
     from fsim.linalg import DeviceBuffer
     from fsim.livemode import ReconstructionServer
