@@ -16,8 +16,9 @@
 // products) costs about 70 double-precision operations per pixel and
 // candidate.  Each round instead first evaluates every candidate's
 // correlation A(c) = sum q * e^{i phi_c} (q = b1 conj(b0)) separably, with
-// accurate phasors: a 10-frequency DFT along x per row, then 10 along y —
-// about 60 operations per pixel for all 100 candidates.  The reference value
+// accurate phasors: a 10-frequency DFT along y per column, then 10 along x —
+// about 60 operations per pixel for all 100 candidates, fused with the
+// inverse column transform that produces the pixels.  The reference value
 // J(c) differs from A(c) by at most
 //   D = sum|q| * (2^-24 * phi_max + 2e-6) * 1.5
 // (2^-24 |phi|: rounding of the phase to float; 2e-6 covers the polynomial,
@@ -25,14 +26,19 @@
 // hold the maximum of |J| only if |A(c)| >= max|A| - 2D.  Those candidates
 // (typically a handful) are then evaluated in the reference's arithmetic and
 // the first maximum in y-outer, x-inner order is taken among them, which is
-// the candidate the full evaluation picks.
+// the candidate the full evaluation picks.  When the filter keeps one
+// candidate it is the strict maximum and nothing is evaluated: fitPeak's
+// phase and magnitude of a grid round are not used by estimateParameters
+// (SimAlgorithm.java:114-159) or the live refit.
 //
 // Device layout for G = count * ndir groups, nn = n * n:
 //   X  [G][P][nn] complex: input spectra; after separation slots 0, 1 (, 2)
 //      hold components 0, 1 (, 3) in place
 //   W  [2][Q][nn] complex: coarse correlation (Q = G: attenuated component 0
 //      and fit-band component), then the commonRegion copies of Q pairs
-//      (band 0 copies first, then band 1 copies)
+//      (band 0 copies first, then band 1 copies) after the inverse row pass;
+//      the column kernels transform their tiles in registers and write
+//      nothing back
 #include "fsim_internal.h"
 #include "fft_core.h"
 #include "fft_launch.h"
@@ -520,151 +526,208 @@ static int launch_region_rows(int n, const EbRegion &r, const float2 *dct, EstPa
     }
 }
 
-// Phasor tables of the approximate candidate evaluation, per pair (20 n
-// float2 reserved): TY[j][y] = e^{-2 pi i ypos_j y / n} as float (10 n
-// float2), then as double2 TXB[x] = e^{+2 pi i xpos_0 x / n} and
-// TXS[x] = e^{+2 pi i (search / 4.5) x / n}, so that
-// TX_i(x) = e^{+2 pi i xpos_i x / n} = TXB[x] TXS[x]^i (the grid's positions
-// are xpos_0 + i search / 4.5; the shift of b1 by (xpos, -ypos)).  Phases
-// reduced mod 1 and evaluated in double.
-template <typename F>
-__device__ __forceinline__ auto apx_txb(F *tabs, int p, int n) {
-    using D = std::conditional_t<std::is_const_v<F>, const double2, double2>;
-    return reinterpret_cast<D *>(tabs + (size_t)p * 20 * n + 10 * n);
-}
-__global__ void k_eb_apx_tab(const EstPair *__restrict__ pairs, int n, float2 *__restrict__ tabs) {
-    const int p = blockIdx.y;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= 12 * n) return;
-    const double search = pairs[p].search;
-    double pos, sn, cs;
-    int v;
-    if (k < 10 * n) {
-        pos = grid_pos(pairs[p].ky, k / n, search);
-        v = k % n;
-    } else {
-        v = (k - 10 * n) % n;
-        pos = k < 11 * n ? grid_pos(pairs[p].kx, 0, search) : search / 4.5;
+// The fused column kernels of a round (the inverse column transform of both
+// bands of a pair, then the candidate filter or the exact evaluation) use
+// the stand-alone column pass's tile width, capped at 512 threads per block
+// so that a thread may hold up to 256 VGPRs, and its twiddle source, so
+// that the transforms return its values bit for bit.
+template <int L>
+struct EbCols {
+    static constexpr int E = FftE<L>::value, T = L / E;
+    static constexpr int CW0 = ColW<L>::value;
+    static constexpr int CW = T * CW0 <= 512 ? CW0 : (512 / T >= 1 ? 512 / T : 1);
+    static constexpr int NT = T * CW, NW = (NT + 63) / 64;
+    static constexpr int NTILE = L / CW;
+    using TW = TwLds<L, (FftLds<L, E>::len * CW0 + L) * 8 <= 163840>;
+    static constexpr int LDS = FftLds<L, E>::len * CW;
+};
+
+// Inverse column transform of column tile `tile` of both bands of pair p
+// (their rows transformed by the region pass; band 0 of pair p at
+// W[p], band 1 at W[npairs + p]), as the stand-alone column pass computes
+// it, scaled by 1 / L: a[m], b[m] = band 0, band 1 at column tile * CW + c,
+// row t + T m.  Every thread of the block calls it (barriers inside).
+template <int L>
+__device__ __forceinline__ void eb_cols_load(const float2 *__restrict__ W, int npairs, int p, int tile, int c, int t,
+                                             float2 (&a)[EbCols<L>::E], float2 (&b)[EbCols<L>::E], float2 *lds,
+                                             const float2 *tws) {
+    using C = EbCols<L>;
+    constexpr size_t nn = (size_t)L * L;
+    const float2 *pa = W + (size_t)p * nn + tile * C::CW + c;
+    const float2 *pb = W + ((size_t)npairs + p) * nn + tile * C::CW + c;
+#pragma unroll
+    for (int m = 0; m < C::E; m++) {
+        a[m] = pa[(size_t)(t + C::T * m) * L];
+        b[m] = pb[(size_t)(t + C::T * m) * L];
     }
-    double u = pos * v / n;
-    u -= floor(u);
-    sincospi(2 * u, &sn, &cs);
-    if (k < 10 * n)
-        tabs[(size_t)p * 20 * n + k] = make_float2((float)cs, (float)-sn);
-    else
-        apx_txb(tabs, p, n)[k - 10 * n] = make_double2(cs, sn);
+    // fft_regs ends with a barrier after its last LDS reads, so the second
+    // transform may reuse the region at once
+    fft_regs<L, C::E, true, false, C::TW::tl>(a, lds + c, C::CW, t, tws);
+    fft_regs<L, C::E, true, false, C::TW::tl>(b, lds + c, C::CW, t, tws);
+    constexpr float sc = 1.0f / L;
+#pragma unroll
+    for (int m = 0; m < C::E; m++) {
+        a[m] = cscale(a[m], sc);
+        b[m] = cscale(b[m], sc);
+    }
 }
 
-// Approximate correlations A(c) of all 100 candidates and sum|q| of each
-// pair; grid (n / rows, npairs), 256 threads as 16 row groups of 16 lanes.
-// Per row: R_i = sum_x TX_i(x) q(x, y) (lanes split x, then a 16-lane sum;
-// TX_i(x) q(x, y) by the recurrence z_0 = TXB q, z_{i+1} = z_i TXS), kept per
-// row in LDS; then thread j * 10 + i forms candidate (i, j) as
-// sum_y TY_j(y) R_i(y) over the block's rows.  Partial sums
-// apart[(pair * kPairSlots + c) * nblk + block], sum|q| in slot kNormSlot.
-constexpr int kApxTile = 256, kApxRows = 32;
-__global__ void __launch_bounds__(256, 4) k_eb_approx(const float2 *__restrict__ T, int n,
-                                                      const EstPair *__restrict__ pairs, int npairs,
-                                                      const float2 *__restrict__ tabs, int rows, int nblk,
-                                                      double2 *__restrict__ apart) {
-    __shared__ double2 txb[kApxTile], txs[kApxTile];
-    __shared__ float2 ty[10 * kApxRows];
-    __shared__ double2 rsum[kApxRows][10];
-    __shared__ double ared[4];
-    const int p = blockIdx.y, blk = blockIdx.x;
-    const int t = threadIdx.x, rg = t / 16, l = t % 16;
-    const size_t nn = (size_t)n * n;
-    const float2 *b0 = T + (size_t)p * nn, *b1 = T + ((size_t)npairs + p) * nn;
-    const float2 *TY = tabs + (size_t)p * 20 * n;
-    const double2 *TXB = apx_txb(tabs, p, n), *TXS = TXB + n;
-    const int y0 = blk * rows;
-    for (int k = t; k < 10 * rows; k += 256) ty[k] = TY[(k / rows) * n + y0 + k % rows];
-    for (int k = t; k < 10 * rows; k += 256) rsum[k / 10][k % 10] = make_double2(0.0, 0.0);
-    double aq = 0.0;
-    const int tw = n < kApxTile ? n : kApxTile;
-    for (int xt0 = 0; xt0 < n; xt0 += tw) {
-        __syncthreads();
-        for (int k = t; k < tw; k += 256) {
-            txb[k] = TXB[xt0 + k];
-            txs[k] = TXS[xt0 + k];
+// Sum of v over the block's NT threads, valid in thread 0; sh holds a
+// double2 per wavefront.  Called by all threads (barriers inside).
+template <int NT>
+__device__ __forceinline__ double2 eb_block_sum(double2 v, double2 *sh) {
+    constexpr int NW = (NT + 63) / 64;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        if (o < NT) {
+            v.x += __shfl_down(v.x, o, 64);
+            v.y += __shfl_down(v.y, o, 64);
         }
-        __syncthreads();
-        for (int r = rg; r < rows; r += 16) {
-            const size_t row = (size_t)(y0 + r) * n + xt0;
-            double2 R[10];
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    double2 r = make_double2(0.0, 0.0);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NW; i++) {
+            r.x += sh[i].x;
+            r.y += sh[i].y;
+        }
+    __syncthreads();
+    return r;
+}
+
+// Phasor tables of the candidate filter, per pair (4 n double2): TYB[y] =
+// e^{-2 pi i ypos_0 y / n}, TYS[y] = e^{-2 pi i (search / 4.5) y / n},
+// TXB[x] = e^{+2 pi i xpos_0 x / n}, TXS[x] = e^{+2 pi i (search / 4.5) x / n}.
+// The grid's positions are pos_0 + i search / 4.5, so TY_j = TYB TYS^j and
+// TX_i = TXB TXS^i (the shift of b1 by (xpos, -ypos)).  Phases reduced mod 1
+// and evaluated in double.
+__global__ void k_eb_apx_tab(const EstPair *__restrict__ pairs, int n, double2 *__restrict__ tabs) {
+    const int p = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 4 * n) return;
+    const int which = k / n, v = k % n;
+    const double search = pairs[p].search;
+    const double pos = which == 0   ? grid_pos(pairs[p].ky, 0, search)
+                       : which == 2 ? grid_pos(pairs[p].kx, 0, search)
+                                    : search / 4.5;
+    double u = pos * v / n;
+    u -= floor(u);
+    double sn, cs;
+    sincospi(2 * u, &sn, &cs);
+    tabs[(size_t)p * 4 * n + k] = make_double2(cs, which < 2 ? -sn : sn);
+}
+
+// A round's inverse column pass fused with the approximate correlation of
+// all 100 candidates (the filter described at the top of this file): per
+// column x, C_j(x) = sum_y TY_j(y) q(x, y), q = b1 conj(b0) in double, by
+// the recurrence z_{j+1} = z_j TYS(y); then A_tile(i, j) = sum_x TX_i(x)
+// C_j(x) over the tile's columns, and the tile's sum |q|.  Nothing is
+// written back: the exact evaluation transforms the columns again.  Grid
+// xcd_grid(npairs * NTILE): neighbouring tiles of a pair run on one XCD and
+// share the cache lines of their rows.
+// apart[(pair * kPairSlots + c) * NTILE + tile], sum |q| in slot kNormSlot.
+template <int L>
+__global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__restrict__ W, int npairs,
+                                                               const double2 *__restrict__ tabs,
+                                                               const float2 *__restrict__ tw,
+                                                               double2 *__restrict__ apart) {
+    using C = EbCols<L>;
+    constexpr int E = C::E, T = C::T, CW = C::CW, NT = C::NT, NW = C::NW, NTILE = C::NTILE;
+    __shared__ float2 lds[C::LDS];
+    __shared__ float2 twl[C::TW::len];
+    __shared__ double2 red[NW][CW][10];
+    __shared__ double2 col[CW][10], txi[CW][10];
+    __shared__ double ared[NW];
+    const int w = xcd_linear(npairs * NTILE);
+    if (w >= npairs * NTILE) return;   // block-uniform, before any barrier
+    const int p = w / NTILE, tile = w % NTILE;
+    const float2 *tws = tw_table<C::TW::tl, NT>(twl, tw);
+    const int c = threadIdx.x % CW, t = threadIdx.x / CW;
+    float2 a[E], b[E];
+    eb_cols_load<L>(W, npairs, p, tile, c, t, a, b, lds, tws);
+    const double2 *TYB = tabs + (size_t)p * 4 * L, *TYS = TYB + L, *TXB = TYS + L, *TXS = TXB + L;
+    double2 S[10];
 #pragma unroll
-            for (int i = 0; i < 10; i++) R[i] = make_double2(0.0, 0.0);
-            // four pixels' loads in flight before their sums (the sums keep
-            // the pixel order: x, x + 16, ...)
-            for (int x0 = l; x0 < tw; x0 += 64) {
-                float2 av[4], bv[4];
+    for (int j = 0; j < 10; j++) S[j] = make_double2(0.0, 0.0);
+    double aq = 0.0;
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int x = x0 + 16 * u;
-                    if (x < tw) {
-                        av[u] = b0[row + x];
-                        bv[u] = b1[row + x];
-                    }
-                }
+    for (int m = 0; m < E; m++) {
+        const int y = t + T * m;
+        const float2 u = a[m], v = b[m];
+        const double qr = (double)v.x * u.x + (double)v.y * u.y;
+        const double qi = (double)v.y * u.x - (double)v.x * u.y;
+        // |q| for the bound's sum |q| in single precision: its relative
+        // error (~1e-7) is far inside the bound's factor 1.5
+        aq += (double)sqrtf((float)(qr * qr + qi * qi));
+        const double2 e = TYB[y], s = TYS[y];
+        double zr = qr * e.x - qi * e.y, zi = qr * e.y + qi * e.x;
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int x = x0 + 16 * u;
-                    if (x >= tw) break;
-                    const float2 a = av[u], b = bv[u];
-                    const double qr = (double)b.x * a.x + (double)b.y * a.y;
-                    const double qi = (double)b.y * a.x - (double)b.x * a.y;
-                    // |q| for the bound's sum|q| in single precision: its
-                    // relative error (~1e-7) is far inside the bound's
-                    // factor 1.5
-                    aq += (double)sqrtf((float)(qr * qr + qi * qi));
-                    const double2 e = txb[x], w = txs[x];
-                    double zr = qr * e.x - qi * e.y, zi = qr * e.y + qi * e.x;
-#pragma unroll
-                    for (int i = 0; i < 10; i++) {
-                        R[i].x += zr;
-                        R[i].y += zi;
-                        if (i < 9) {
-                            const double nr = zr * w.x - zi * w.y;
-                            zi = zr * w.y + zi * w.x;
-                            zr = nr;
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int o = 8; o > 0; o >>= 1)
-#pragma unroll
-                for (int i = 0; i < 10; i++) {
-                    R[i].x += __shfl_xor(R[i].x, o, 16);
-                    R[i].y += __shfl_xor(R[i].y, o, 16);
-                }
-            double2 Ri = R[0];
-#pragma unroll
-            for (int i = 1; i < 10; i++)
-                if (l == i) Ri = R[i];
-            // row r belongs to this row group in every x tile
-            if (l < 10) {
-                rsum[r][l].x += Ri.x;
-                rsum[r][l].y += Ri.y;
+        for (int j = 0; j < 10; j++) {
+            S[j].x += zr;
+            S[j].y += zi;
+            if (j < 9) {
+                const double nr = zr * s.x - zi * s.y;
+                zi = zr * s.y + zi * s.x;
+                zr = nr;
             }
         }
     }
-    for (int o = 32; o > 0; o >>= 1) aq += __shfl_down(aq, o, 64);
-    if (t % 64 == 0) ared[t / 64] = aq;
+    // sum over the rows: the lanes of one column within the wavefront
+    // (lane = t' CW + c), then over the wavefronts
+#pragma unroll
+    for (int o = CW; o < 64; o <<= 1)
+        if (o < NT)
+#pragma unroll
+            for (int j = 0; j < 10; j++) {
+                S[j].x += __shfl_xor(S[j].x, o, 64);
+                S[j].y += __shfl_xor(S[j].y, o, 64);
+            }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        if (o < NT) aq += __shfl_down(aq, o, 64);
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+    if (lane < CW)
+#pragma unroll
+        for (int j = 0; j < 10; j++) red[wv][lane][j] = S[j];
+    if (lane == 0) ared[wv] = aq;
     __syncthreads();
-    double2 *out = apart + (size_t)p * kPairSlots * nblk;
-    if (t < kMaxCand) {
-        const int i = t % 10, j = t / 10;
+    for (int k = threadIdx.x; k < CW * 10; k += NT) {
+        const int cc = k / 10, j = k % 10;
         double2 s = make_double2(0.0, 0.0);
-        for (int r = 0; r < rows; r++) {
-            const float2 e = ty[j * rows + r];
-            const double2 R = rsum[r][i];
+        for (int i = 0; i < NW; i++) {
+            s.x += red[i][cc][j].x;
+            s.y += red[i][cc][j].y;
+        }
+        col[cc][j] = s;
+        // TX_j at this column (here j counts x candidates): TXB TXS^j
+        const int x = tile * CW + cc;
+        const double2 e = TXB[x], st = TXS[x];
+        double zr = e.x, zi = e.y;
+        for (int r = 0; r < j; r++) {
+            const double nr = zr * st.x - zi * st.y;
+            zi = zr * st.y + zi * st.x;
+            zr = nr;
+        }
+        txi[cc][j] = make_double2(zr, zi);
+    }
+    __syncthreads();
+    double2 *out = apart + (size_t)p * kPairSlots * NTILE;
+    for (int k = threadIdx.x; k < kMaxCand; k += NT) {
+        const int i = k % 10, j = k / 10;
+        double2 s = make_double2(0.0, 0.0);
+        for (int cc = 0; cc < CW; cc++) {
+            const double2 e = txi[cc][i], R = col[cc][j];
             s.x += R.x * e.x - R.y * e.y;
             s.y += R.x * e.y + R.y * e.x;
         }
-        out[(size_t)t * nblk + blk] = s;
+        out[(size_t)k * NTILE + tile] = s;
     }
-    if (t == 0) out[(size_t)kNormSlot * nblk + blk] = make_double2(ared[0] + ared[1] + ared[2] + ared[3], 0.0);
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < NW; i++) s += ared[i];
+        out[(size_t)kNormSlot * NTILE + tile] = make_double2(s, 0.0);
+    }
 }
 
 // Candidates that can hold the maximum (see the top of this file); all 100
@@ -700,32 +763,118 @@ __global__ void __launch_bounds__(128) k_eb_select(EstPair *__restrict__ pairs, 
     pairs[p].nsel = ns;
 }
 
-// The selected candidates of every pair in the reference's arithmetic, and
-// |b0|^2: grid (nchunk, npairs); part[(pair * kPairSlots + c) * nchunk + chunk].
-// Candidate c = yi * 10 + xi of the grid, or (kx, ky) alone.
-__global__ void __launch_bounds__(kCorrThreads) k_eb_corr(const float2 *__restrict__ T, int n,
-                                                          const EstPair *__restrict__ pairs, int npairs, int nchunk,
-                                                          double2 *__restrict__ part) {
-    __shared__ double2 sh[kCorrThreads / 64];
-    const size_t nn = (size_t)n * n;
-    const int p = blockIdx.y, chunk = blockIdx.x;
+// The exact evaluation of a round: the inverse column pass again (the same
+// values), then |b0|^2 and each selected candidate in the reference's
+// arithmetic (corr_pixel), summed per tile;
+// part[(pair * kPairSlots + c) * NTILE + tile].  Candidate c = yi * 10 + xi
+// of the grid, or (kx, ky) alone.  A fitPeak round whose filter kept a
+// single candidate needs no evaluation -- that candidate is the strict
+// maximum, and a grid round's phase and magnitude are not used -- so such
+// pairs return at once (k_eb_step takes the candidate's position).
+template <int L>
+__global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_corr(const float2 *__restrict__ W, int npairs,
+                                                                const EstPair *__restrict__ pairs,
+                                                                const float2 *__restrict__ tw,
+                                                                double2 *__restrict__ part) {
+    using C = EbCols<L>;
+    constexpr int E = C::E, T = C::T, CW = C::CW, NT = C::NT, NTILE = C::NTILE;
+    __shared__ float2 lds[C::LDS];
+    __shared__ float2 twl[C::TW::len];
+    __shared__ double2 sh[C::NW];
+    const int w = xcd_linear(npairs * NTILE);
+    if (w >= npairs * NTILE) return;   // block-uniform, before any barrier
+    const int p = w / NTILE, tile = w % NTILE;
     const double kx = pairs[p].kx, ky = pairs[p].ky, search = pairs[p].search;
     const int nsel = pairs[p].nsel;
-    CorrPix px;
-    corr_load(px, T + (size_t)p * nn, T + ((size_t)npairs + p) * nn, n, chunk);
-    double2 *out = part + (size_t)p * kPairSlots * nchunk;
-    const double2 s0 = block_sum(make_double2(corr_norm2(px), 0.0), sh);
-    if (threadIdx.x == 0) out[(size_t)kNormSlot * nchunk + chunk] = s0;
+    if (search > 0 && nsel == 1) return;   // block-uniform
+    const float2 *tws = tw_table<C::TW::tl, NT>(twl, tw);
+    const int c = threadIdx.x % CW, t = threadIdx.x / CW;
+    const int x = tile * CW + c;
+    float2 a[E], b[E];
+    eb_cols_load<L>(W, npairs, p, tile, c, t, a, b, lds, tws);
+    double2 *out = part + (size_t)p * kPairSlots * NTILE;
+    double n2 = 0.0;
+#pragma unroll
+    for (int m = 0; m < E; m++) n2 = corr_norm2_add(n2, a[m]);
+    const double2 s0 = eb_block_sum<NT>(make_double2(n2, 0.0), sh);
+    if (threadIdx.x == 0) out[(size_t)kNormSlot * NTILE + tile] = s0;
+    constexpr double inv_n = 1.0 / L;
     for (int k = 0; k < nsel; k++) {
-        const int c = pairs[p].sel[k];
-        const double xpos = search > 0 ? grid_pos(kx, c % 10, search) : kx;
-        const double ypos = search > 0 ? grid_pos(ky, c / 10, search) : ky;
-        const double2 s = block_sum(corr_candidate(px, xpos, ypos, n), sh);
-        if (threadIdx.x == 0) out[(size_t)c * nchunk + chunk] = s;
+        const int cc = pairs[p].sel[k];
+        const double xpos = search > 0 ? grid_pos(kx, cc % 10, search) : kx;
+        const double ypos = search > 0 ? grid_pos(ky, cc / 10, search) : ky;
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const float2 q = corr_pixel(a[m], b[m], x, t + T * m, xpos, ypos, inv_n);
+            acc.x += (double)q.x;
+            acc.y += (double)q.y;
+        }
+        const double2 s = eb_block_sum<NT>(acc, sh);
+        if (threadIdx.x == 0) out[(size_t)cc * NTILE + tile] = s;
     }
 }
 
-// Per pair: selected candidate sums in chunk order, corr = sum * (1 / |b0|^2)
+#define FSIM_EB_SWITCH(n, FN, ...)                                            \
+    switch (n) {                                                              \
+        case 16: return FN<16>(__VA_ARGS__);                                  \
+        case 32: return FN<32>(__VA_ARGS__);                                  \
+        case 64: return FN<64>(__VA_ARGS__);                                  \
+        case 128: return FN<128>(__VA_ARGS__);                                \
+        case 256: return FN<256>(__VA_ARGS__);                                \
+        case 512: return FN<512>(__VA_ARGS__);                                \
+        case 1024: return FN<1024>(__VA_ARGS__);                              \
+        case 2048: return FN<2048>(__VA_ARGS__);                              \
+        case 4096: return FN<4096>(__VA_ARGS__);                              \
+        default: return fail(FSIM_ERR_ARG, "estimate: unsupported size");     \
+    }
+
+// Column tiles per image of the fused column kernels (partial sums per pair
+// and candidate); 0 for an unsupported size.
+static int eb_ntile(int n) {
+    switch (n) {
+        case 16: return EbCols<16>::NTILE;
+        case 32: return EbCols<32>::NTILE;
+        case 64: return EbCols<64>::NTILE;
+        case 128: return EbCols<128>::NTILE;
+        case 256: return EbCols<256>::NTILE;
+        case 512: return EbCols<512>::NTILE;
+        case 1024: return EbCols<1024>::NTILE;
+        case 2048: return EbCols<2048>::NTILE;
+        case 4096: return EbCols<4096>::NTILE;
+        default: return 0;
+    }
+}
+
+template <int L>
+static int launch_cols_apx_t(const float2 *W, int npairs, const double2 *tabs, double2 *apart, hipStream_t s) {
+    using C = EbCols<L>;
+    const float2 *tw = twiddles(L);
+    if (!tw) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
+    hipLaunchKernelGGL(k_eb_cols_apx<L>, dim3(xcd_grid(npairs * C::NTILE)), dim3(C::NT), 0, s, W, npairs, tabs, tw,
+                       apart);
+    FSIM_HIP(hipGetLastError());
+    return FSIM_OK;
+}
+static int launch_cols_apx(int n, const float2 *W, int npairs, const double2 *tabs, double2 *apart, hipStream_t s) {
+    FSIM_EB_SWITCH(n, launch_cols_apx_t, W, npairs, tabs, apart, s)
+}
+
+template <int L>
+static int launch_cols_corr_t(const float2 *W, int npairs, const EstPair *pairs, double2 *part, hipStream_t s) {
+    using C = EbCols<L>;
+    const float2 *tw = twiddles(L);
+    if (!tw) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
+    hipLaunchKernelGGL(k_eb_cols_corr<L>, dim3(xcd_grid(npairs * C::NTILE)), dim3(C::NT), 0, s, W, npairs, pairs,
+                       tw, part);
+    FSIM_HIP(hipGetLastError());
+    return FSIM_OK;
+}
+static int launch_cols_corr(int n, const float2 *W, int npairs, const EstPair *pairs, double2 *part, hipStream_t s) {
+    FSIM_EB_SWITCH(n, launch_cols_corr_t, W, npairs, pairs, part, s)
+}
+
+// Per pair: selected candidate sums in tile order, corr = sum * (1 / |b0|^2)
 // as Cplx.mult with a real Cplx.Double; for a grid, the first maximum of
 // hypot(corr) in y-outer, x-inner order becomes the new shift and the search
 // radius shrinks by 5 (Correlation.java:99-131).  One block per pair.
@@ -735,6 +884,17 @@ __global__ void __launch_bounds__(128) k_eb_step(EstPair *__restrict__ pairs, co
     const int p = blockIdx.x, t = threadIdx.x;
     const double search = pairs[p].search;
     const int nsel = pairs[p].nsel;
+    if (search > 0 && nsel == 1) {
+        // the filter kept one candidate: it is the maximum (no sums made)
+        if (t == 0) {
+            const int cc = pairs[p].sel[0];
+            pairs[p].kx = grid_pos(pairs[p].kx, cc % 10, search);
+            pairs[p].ky = grid_pos(pairs[p].ky, cc / 10, search);
+            pairs[p].search = search / 5;
+            pairs[p].re = pairs[p].im = 0.0;
+        }
+        return;
+    }
     const double2 *in = part + (size_t)p * kPairSlots * nchunk;
     const int c = t < nsel ? pairs[p].sel[t] : (t == 127 ? kNormSlot : -1);
     if (c >= 0) {
@@ -875,26 +1035,28 @@ static int eb_grow(void **p, size_t &have, size_t need) {
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-// One commonRegion / inverse FFT / correlation round for npairs pairs; with
-// `grid`, the 10 x 10 candidates of fitPeak through the approximate filter
-// (tabs: 20 n phasors per pair, apart: partial sums of n / rows blocks).
+// One commonRegion / inverse FFT / correlation round for npairs pairs:
+// commonRegion fused with the inverse row pass, then with `grid` (fitPeak's
+// 10 x 10 candidates) the inverse column pass fused with the candidate
+// filter and the selection, then the exact evaluation of what the filter
+// kept (the inverse column pass once more), then the step.  tabs: 4 n
+// phasors per pair; apart, part: partial sums of eb_ntile(n) tiles.
 static int eb_round(const EbRegion &r, const float2 *dct, EstPair *pairs, int npairs, bool grid, int nphase, int n,
-                    int nchunk, double2 *part, float2 *tabs, double2 *apart, hipStream_t s) {
+                    double2 *part, double2 *tabs, double2 *apart, hipStream_t s) {
+    const int ntile = eb_ntile(n);
     hipLaunchKernelGGL(k_eb_ring, dim3(cdiv(npairs, 64)), dim3(64), 0, s, pairs, npairs, n, 0.15);
     int rc = launch_region_rows(n, r, dct, pairs, npairs, nphase, g_eb.X, g_eb.W, s);
-    if (!rc) rc = fft_cols_batch_device(g_eb.W, n, n, 2 * npairs, 1, s);
     if (rc) return rc;
     if (grid) {
-        const int rows = n < kApxRows ? n : kApxRows, nblk = n / rows;
-        hipLaunchKernelGGL(k_eb_apx_tab, dim3(cdiv((size_t)12 * n, 256), npairs), dim3(256), 0, s, pairs, n, tabs);
-        hipLaunchKernelGGL(k_eb_approx, dim3(nblk, npairs), dim3(256), 0, s, g_eb.W, n, pairs, npairs, tabs, rows,
-                           nblk, apart);
-        hipLaunchKernelGGL(k_eb_select, dim3(npairs), dim3(128), 0, s, pairs, apart, nblk);
+        hipLaunchKernelGGL(k_eb_apx_tab, dim3(cdiv((size_t)4 * n, 256), npairs), dim3(256), 0, s, pairs, n, tabs);
+        rc = launch_cols_apx(n, g_eb.W, npairs, tabs, apart, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_eb_select, dim3(npairs), dim3(128), 0, s, pairs, apart, ntile);
         FSIM_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_eb_corr, dim3(nchunk, npairs), dim3(kCorrThreads), 0, s, g_eb.W, n, pairs, npairs, nchunk,
-                       part);
-    hipLaunchKernelGGL(k_eb_step, dim3(npairs), dim3(128), 0, s, pairs, part, nchunk);
+    rc = launch_cols_corr(n, g_eb.W, npairs, pairs, part, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_eb_step, dim3(npairs), dim3(128), 0, s, pairs, part, ntile);
     FSIM_HIP(hipGetLastError());
     return FSIM_OK;
 }
@@ -948,7 +1110,7 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     sep.att_fwhm = 0.15 * o.cutoff;
     EbRegion reg{o.vals, o.ns, o.nband, o.cpm, o.cutoff, p->vec_cpm};
 
-    const int nchunk = (int)((nn + kCorrPix - 1) / kCorrPix);
+    const int ntile = eb_ntile(n);
     const int npair_max = npk * G;
     std::lock_guard<std::mutex> lk(g_eb.mu);
     rc = eb_grow((void **)&g_eb.X, g_eb.x_bytes, (size_t)G * P * nn * sizeof(float2));
@@ -958,15 +1120,14 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     const size_t o_fit = align256(o_grp + G * sizeof(EstGroup));
     const size_t o_gp = align256(o_fit + G * sizeof(EstPair));
     const size_t o_part = align256(o_gp + (size_t)npair_max * sizeof(EstPair));
-    const size_t o_peak = align256(o_part + (size_t)npair_max * kPairSlots * nchunk * sizeof(double2));
+    const size_t o_peak = align256(o_part + (size_t)npair_max * kPairSlots * ntile * sizeof(double2));
     const size_t o_shift = align256(o_peak + 2 * G * sizeof(double));
     const size_t o_res = align256(o_shift + 2 * G * sizeof(double));
     const size_t o_err = align256(o_res + G * sizeof(fsim_estimate_result));
-    const int nblk = n / (n < 32 ? n : 32);
     const size_t o_tabs = align256(o_err + G * sizeof(int));
-    const size_t o_apart = align256(o_tabs + (size_t)G * 20 * n * sizeof(float2));
+    const size_t o_apart = align256(o_tabs + (size_t)G * 4 * n * sizeof(double2));
     const int ndc = o.nband == 1 ? 1 : (o.nband < 3 ? o.nband : 3);
-    const size_t o_dct = align256(o_apart + (size_t)G * kPairSlots * nblk * sizeof(double2));
+    const size_t o_dct = align256(o_apart + (size_t)G * kPairSlots * ntile * sizeof(double2));
     const size_t small = align256(o_dct + (size_t)ndc * nn * sizeof(float2));
     if (!rc) rc = eb_grow(&g_eb.small, g_eb.small_bytes, small);
     if (rc) return rc;
@@ -980,7 +1141,7 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     double *shift = reinterpret_cast<double *>(sb + o_shift);
     fsim_estimate_result *resd = reinterpret_cast<fsim_estimate_result *>(sb + o_res);
     int *errd = reinterpret_cast<int *>(sb + o_err);
-    float2 *tabs = reinterpret_cast<float2 *>(sb + o_tabs);
+    double2 *tabs = reinterpret_cast<double2 *>(sb + o_tabs);
     double2 *apart = reinterpret_cast<double2 *>(sb + o_apart);
     float2 *dct = reinterpret_cast<float2 *>(sb + o_dct);
 
@@ -1012,13 +1173,13 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     FSIM_HIP(hipGetLastError());
     // fitPeak: three rounds of commonRegion, inverse FFTs, candidate filter, argmax
     for (int iter = 0; iter < 3; iter++) {
-        rc = eb_round(reg, dct, fit, G, true, P, n, nchunk, part, tabs, apart, s);
+        rc = eb_round(reg, dct, fit, G, true, P, n, part, tabs, apart, s);
         if (rc) return rc;
     }
     // getPeak of band 1 (and band 2)
     hipLaunchKernelGGL(k_eb_gp_init, dim3(gb), dim3(64), 0, s, fit, gp, G, B, fit_band, live ? 1 : 0, shift);
     FSIM_HIP(hipGetLastError());
-    rc = eb_round(reg, dct, gp, npk * G, false, P, n, nchunk, part, tabs, apart, s);
+    rc = eb_round(reg, dct, gp, npk * G, false, P, n, part, tabs, apart, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_eb_finish, dim3(gb), dim3(64), 0, s, fit, gp, G, npk, live ? 1 : 0, peaks, shift, resd,
                        errd);

@@ -134,18 +134,30 @@ __device__ __forceinline__ void corr_load(CorrPix &c, const float2 *__restrict__
     }
 }
 
+// One pixel's term of sum(shift(b1, xpos, -ypos) * conj(b0)) for a
+// power-of-two size N (inv_n = 1 / N): fourierShift's phase and rotation,
+// then timesConj, in the reference's float operations.
+__device__ __forceinline__ float2 corr_pixel(float2 p0, float2 p1, int x, int y, double xpos, double ypos,
+                                             double inv_n) {
+    float co, si;
+    shift_cos_sin_pow2(xpos, -ypos, x, y, inv_n, co, si);
+    return times_conj_java(rot_java(p1, co, si), p0);
+}
+
 // This thread's part of sum(shift(b1, xpos, -ypos) * conj(b0)).
 __device__ __forceinline__ double2 corr_candidate(const CorrPix &c, double xpos, double ypos, int N) {
     double2 acc = make_double2(0.0, 0.0);
     const bool pow2 = (N & (N - 1)) == 0;
 #pragma unroll
     for (int j = 0; j < kCorrPpt; j++) {
-        float co, si;
-        if (pow2)
-            shift_cos_sin_pow2(xpos, -ypos, c.px[j], c.py[j], 1.0 / N, co, si);
-        else
+        float2 q;
+        if (pow2) {
+            q = corr_pixel(c.p0[j], c.p1[j], c.px[j], c.py[j], xpos, ypos, 1.0 / N);
+        } else {
+            float co, si;
             shift_cos_sin(xpos, -ypos, c.px[j], c.py[j], N, co, si);
-        const float2 q = times_conj_java(rot_java(c.p1[j], co, si), c.p0[j]);
+            q = times_conj_java(rot_java(c.p1[j], co, si), c.p0[j]);
+        }
         if (c.ok[j]) {
             acc.x += (double)q.x;
             acc.y += (double)q.y;
@@ -154,17 +166,21 @@ __device__ __forceinline__ double2 corr_candidate(const CorrPix &c, double xpos,
     return acc;
 }
 
-// This thread's part of b0.norm2() (AbstractVectorCplx.java:241-247): float
-// squares summed in double.
+// b0.norm2() (AbstractVectorCplx.java:241-247) accumulation of one pixel:
+// float squares added in double.
+__device__ __forceinline__ double corr_norm2_add(double a, float2 v) {
+    const float re2 = v.x * v.x, im2 = v.y * v.y;
+    a += (double)re2;
+    a += (double)im2;
+    return a;
+}
+
+// This thread's part of b0.norm2().
 __device__ __forceinline__ double corr_norm2(const CorrPix &c) {
     double a = 0.0;
 #pragma unroll
     for (int j = 0; j < kCorrPpt; j++)
-        if (c.ok[j]) {
-            const float re2 = c.p0[j].x * c.p0[j].x, im2 = c.p0[j].y * c.p0[j].y;
-            a += (double)re2;
-            a += (double)im2;
-        }
+        if (c.ok[j]) a = corr_norm2_add(a, c.p0[j]);
     return a;
 }
 

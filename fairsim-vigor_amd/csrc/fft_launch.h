@@ -22,4 +22,15 @@ struct ColTile {
     static constexpr int value = v0 < v1 ? v0 : v1;
 };
 
+// Work index of this block such that each of the 8 XCDs (workgroups are
+// dealt to them round robin by linear id) walks one contiguous range of
+// [0, total): neighbouring tiles of a component run on one XCD.  The grid
+// has xcd_grid(total) blocks; indices >= total have no work.
+__device__ __forceinline__ int xcd_linear_at(unsigned b, int total) {
+    const int per = (total + 7) / 8;
+    return (int)(b % 8) * per + (int)(b / 8);
+}
+__device__ __forceinline__ int xcd_linear(int total) { return xcd_linear_at(blockIdx.x, total); }
+static inline unsigned xcd_grid(int total) { return (unsigned)((total + 7) / 8) * 8; }
+
 }  // namespace fsim

@@ -106,17 +106,6 @@ __device__ __forceinline__ Job get_job(const JobList &jl, int j) {
     return J;
 }
 
-// Work index of this block such that each of the 8 XCDs (workgroups are
-// dealt to them round robin by linear id) walks one contiguous range of
-// [0, total): neighbouring tiles of a component run on one XCD.  The grid
-// has xcd_grid(total) blocks; indices >= total have no work.
-__device__ __forceinline__ int xcd_linear_at(unsigned b, int total) {
-    const int per = (total + 7) / 8;
-    return (int)(b % 8) * per + (int)(b / 8);
-}
-__device__ __forceinline__ int xcd_linear(int total) { return xcd_linear_at(blockIdx.x, total); }
-static inline unsigned xcd_grid(int total) { return (unsigned)((total + 7) / 8) * 8; }
-
 // Work index w -> (unit, tile) in tile-group-major order: groups of TG
 // adjacent tiles, and within a group every unit (component) with its TG
 // tiles in turn.  With xcd_linear, an XCD walks one group for all units
