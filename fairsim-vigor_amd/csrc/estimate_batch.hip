@@ -19,10 +19,11 @@
 // accurate phasors: a 10-frequency DFT along y per column, then 10 along x —
 // about 60 operations per pixel for all 100 candidates, fused with the
 // inverse column transform that produces the pixels.  The reference value
-// J(c) differs from A(c) by at most
-//   D = sum|q| * (2^-24 * phi_max + 2e-6) * 1.5
-// (2^-24 |phi|: rounding of the phase to float; 2e-6 covers the polynomial,
-// the float products and the phasor tables with margin), so a candidate can
+// J(c) differs from the computed A(c) by at most
+//   D = sum|q| * (2^-24 * phi_max + 2e-6 + 32 * 2^-24) * 1.5
+// (2^-24 |phi|: rounding of the phase to float; 2e-6 covers the polynomial
+// and the reference's float products with margin; 32 * 2^-24 the float part
+// of the approximate sums, kApxArith), so a candidate can
 // hold the maximum of |J| only if |A(c)| >= max|A| - 2D.  Those candidates
 // (typically a handful) are then evaluated in the reference's arithmetic and
 // the first maximum in y-outer, x-inner order is taken among them, which is
@@ -418,16 +419,138 @@ __global__ void k_eb_ring(EstPair *__restrict__ pairs, int npairs, int n, double
     pairs[p].ring_ok = 1;
 }
 
+// commonRegion's test on the OTF weight centred at the shifted position,
+// |w| >= 0.05 (Correlation.java:262-264), depends on a pixel only through
+// the squared distance d2 = (xh - kx)^2 + (yh - ky)^2 that the reference
+// computes (d2_at): the radius is sqrt(d2) * vec_cpm, then the table index
+// floor(radius / cpm), the weight zero at and beyond the cutoff or the
+// table's end.  That index -- the key, with ns for zero -- is a
+// non-decreasing function of d2 (rounded square root, rounded products and
+// quotient by positive constants), so the test's outcome per table sample
+// forms runs in d2.  Per OTF band, once per call, k_eb_runs classifies each
+// sample (pass: |w| >= the limit whatever the interpolation fraction; fail:
+// below it whatever the fraction, or zero; undecided: within a few rounding
+// errors of the limit) and finds each run's first d2 by binary search with
+// the same arithmetic; the region pass compares d2 with the run starts and
+// forms the weight only in undecided runs (every pixel when a table has
+// more than kMaxRuns runs).
+constexpr int kMaxRuns = 16;
+struct EbRuns {
+    int n;                    // runs; 0: no classes, the weight at every pixel
+    int cls[kMaxRuns];        // 0 fail, 1 pass, 2 undecided
+    double start[kMaxRuns];   // first d2 of the run (start[0] = 0)
+};
+
+// Table key at squared distance d2 (see above): the sample index the weight
+// interpolates from, or ns where the weight is zero.
+__device__ __forceinline__ int otf_key(double d2, const EbRegion &r) {
+    const double cycl = cycl_of_d2(d2, r.vec_cpm);
+    if (cycl > r.cutoff) return r.ns;   // otf_pixel: zero
+    float f;
+    const int l = otf_index(r.ns, r.cpm, r.cutoff, cycl, f);
+    return l < 0 ? r.ns : l;
+}
+
+// Class of sample v under the test abs_java(otf_weigh(v, f)) < wlim for
+// every fraction f in [0, 1].  otf_weigh returns each part as
+// v (1 - f) + v f in float: within 2.5 u of v (u = 2^-24: 1 - f rounds by at
+// most u / 2, two products and a sum by u each, relative to |v|); abs_java
+// adds u + 2^-52; so |w| lies within 3.6 u of |v|, and a margin of 8 u
+// around the limit decides the sample for every f.  Zero samples fail.
+__device__ __forceinline__ int sample_class(float2 v, double wlim) {
+    const double m = sqrt((double)v.x * v.x + (double)v.y * v.y);
+    if (m >= wlim * (1 + 8 * 0x1p-24)) return 1;
+    if (m < wlim * (1 - 8 * 0x1p-24)) return 0;
+    return 2;
+}
+
+// First d2 >= 0 with otf_key(d2) >= l: binary search over the bit patterns
+// of non-negative doubles (ordered as the values) up to hi, which bounds
+// every pixel's d2; +inf when no d2 up to hi reaches l.
+__device__ __forceinline__ double first_d2(int l, const EbRegion &r, double hi) {
+    unsigned long long a = 0, b = (unsigned long long)__double_as_longlong(hi) + 1;
+    if (otf_key(hi, r) < l) return __longlong_as_double(0x7FF0000000000000ll);
+    while (a < b) {
+        const unsigned long long m = a + (b - a) / 2;
+        if (otf_key(__longlong_as_double((long long)m), r) >= l)
+            b = m;
+        else
+            a = m + 1;
+    }
+    return __longlong_as_double((long long)a);
+}
+
+// The run table of OTF band blockIdx.x (256 threads).  Run boundaries are
+// the keys l whose class differs from that of l - 1, found chunk by chunk
+// with a block prefix count; each run's first d2 by first_d2.
+__global__ void __launch_bounds__(256) k_eb_runs(EbRegion r, int n, double wlim, EbRuns *__restrict__ out) {
+    __shared__ int bl[kMaxRuns], bc[kMaxRuns], scan[256];
+    __shared__ int cnt, overflow;
+    const int b = blockIdx.x, tid = threadIdx.x, ns = r.ns;
+    const float2 *tab = r.tabs + (size_t)b * ns;
+    const int k0 = otf_key(0.0, r);
+    auto cls = [&](int l) { return l >= ns ? 0 : sample_class(tab[l], wlim); };
+    if (tid == 0) {
+        bl[0] = k0;
+        bc[0] = cls(k0);
+        cnt = 1;
+        overflow = 0;
+    }
+    __syncthreads();
+    for (int l0 = k0 + 1; l0 <= ns; l0 += 256) {
+        const int l = l0 + tid;
+        int c = 0;
+        const int flag = (l <= ns && (c = cls(l)) != cls(l - 1)) ? 1 : 0;
+        scan[tid] = flag;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const int v = tid >= o ? scan[tid - o] : 0;
+            __syncthreads();
+            scan[tid] += v;
+            __syncthreads();
+        }
+        const int pos = cnt + scan[tid] - flag;
+        if (flag) {
+            if (pos < kMaxRuns) {
+                bl[pos] = l;
+                bc[pos] = c;
+            } else {
+                overflow = 1;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) cnt += scan[255];
+        __syncthreads();
+    }
+    const int nr = cnt < kMaxRuns ? cnt : kMaxRuns;
+    const double hi = 64.0 * n * n;   // every pixel's d2 is below (1.5 n)^2 * 2
+    if (tid < nr && !overflow) {
+        out[b].cls[tid] = bc[tid];
+        out[b].start[tid] = tid == 0 ? 0.0 : first_d2(bl[tid], r, hi);
+    }
+    if (tid == 0) out[b].n = overflow ? 0 : nr;
+}
+
+// The band value of commonRegion where the DC weight's reciprocal dc is
+// valid and the shifted weight passes (region_dc without the cut).
+__device__ __forceinline__ float2 region_pass(float2 v, float2 dc) {
+#pragma clang fp contract(off)
+    return make_float2(v.x * dc.x - v.y * dc.y, v.x * dc.y + v.y * dc.x);
+}
+
 // Correlation.commonRegion(b0, b1, 0, bn1, otf, kx, ky, 0.15, 0.05, true)
 // on copies of each pair's bands (Correlation.java:60-63, 189-192), fused
 // with the row pass of the inverse 2-D FFT that follows (the same row
 // transform as fft2d's, which runs rows first).  Grid (ceil(n / RW), 2 * npairs):
 // blockIdx.y = 2 * pair + band; band 0 rows go to T[pair], band 1 rows to
 // T[npairs + pair].
-template <int L>
+// RUNS: the shifted weight's test through the run tables (k_eb_runs);
+// otherwise the weight at every pixel of the common region.
+template <int L, bool RUNS>
 __global__ void __launch_bounds__(256) k_eb_region_rows(const float2 *__restrict__ X, float2 *__restrict__ T,
                                                         EstPair *__restrict__ pairs, int npairs, int nphase,
                                                         EbRegion r, const float2 *__restrict__ dct,
+                                                        const EbRuns *__restrict__ runs,
                                                         const float2 *__restrict__ tw) {
     constexpr int E = FftE<L>::value, Tn = L / E;
     constexpr int RW = Tn >= 256 ? 1 : 256 / Tn;
@@ -465,6 +588,8 @@ __global__ void __launch_bounds__(256) k_eb_region_rows(const float2 *__restrict
         const float2 *dc = dct + (size_t)(band && r.otf_nband > 1 ? bn1 : 0) * nn + (size_t)row * L;
         const float2 *tab = band ? a.tab1 : a.tab0;
         const double sx = band ? -kx : kx, sy = band ? -ky : ky;
+        const EbRuns &R = runs[band ? (r.otf_nband == 1 ? 0 : bn1) : 0];
+        const int nrun = RUNS ? R.n : 0;
 #pragma unroll
         for (int m = 0; m < E; m++) {
             const int x = t + Tn * m;
@@ -482,8 +607,21 @@ __global__ void __launch_bounds__(256) k_eb_region_rows(const float2 *__restrict
             const float2 dcv = dc[x];
             float2 out = make_float2(0.f, 0.f);
             if (!cut && dcv.x == dcv.x) {
-                const float2 wt = otf_pixel(tab, a.ns, a.cpm, a.cutoff, x, row, L, L, sx, sy, a.vec_cpm);
-                out = region_dc(src[x], dcv, wt, false, a.wlim);
+                const double d2 = d2_at(x, row, L, L, sx, sy);
+                int cls = 2;
+                if (nrun > 0) {
+                    cls = R.cls[0];
+                    for (int k = 1; k < nrun; k++)
+                        if (d2 >= R.start[k]) cls = R.cls[k];
+                }
+                if (cls == 2) {
+                    const double cycl = cycl_of_d2(d2, a.vec_cpm);
+                    const float2 wt = cycl > a.cutoff ? make_float2(0.f, 0.f)
+                                                      : otf_val(tab, a.ns, a.cpm, a.cutoff, cycl);
+                    out = region_dc(src[x], dcv, wt, false, a.wlim);
+                } else if (cls == 1) {
+                    out = region_pass(src[x], dcv);
+                }
             }
             v[m] = out;
         }
@@ -497,31 +635,38 @@ __global__ void __launch_bounds__(256) k_eb_region_rows(const float2 *__restrict
     }
 }
 
+// Test hook: the weight at every pixel instead of the run tables.
+static int g_region_exact = 0;
+
 template <int L>
-static int launch_region_rows_t(const EbRegion &r, const float2 *dct, EstPair *pairs, int npairs, int nphase,
-                                float2 *X, float2 *W, hipStream_t s) {
+static int launch_region_rows_t(const EbRegion &r, const float2 *dct, const EbRuns *runs, EstPair *pairs,
+                                int npairs, int nphase, float2 *X, float2 *W, hipStream_t s) {
     constexpr int E = FftE<L>::value, Tn = L / E;
     constexpr int RW = Tn >= 256 ? 1 : 256 / Tn;
     const float2 *tw = twiddles(L);
     if (!tw) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
-    hipLaunchKernelGGL(k_eb_region_rows<L>, dim3(cdiv(L, RW), 2 * npairs), dim3(Tn * RW), 0, s, X, W, pairs,
-                       npairs, nphase, r, dct, tw);
+    if (g_region_exact)
+        hipLaunchKernelGGL((k_eb_region_rows<L, false>), dim3(cdiv(L, RW), 2 * npairs), dim3(Tn * RW), 0, s, X, W,
+                           pairs, npairs, nphase, r, dct, runs, tw);
+    else
+        hipLaunchKernelGGL((k_eb_region_rows<L, true>), dim3(cdiv(L, RW), 2 * npairs), dim3(Tn * RW), 0, s, X, W,
+                           pairs, npairs, nphase, r, dct, runs, tw);
     FSIM_HIP(hipGetLastError());
     return FSIM_OK;
 }
 
-static int launch_region_rows(int n, const EbRegion &r, const float2 *dct, EstPair *pairs, int npairs, int nphase,
-                              float2 *X, float2 *W, hipStream_t s) {
+static int launch_region_rows(int n, const EbRegion &r, const float2 *dct, const EbRuns *runs, EstPair *pairs,
+                              int npairs, int nphase, float2 *X, float2 *W, hipStream_t s) {
     switch (n) {
-        case 16: return launch_region_rows_t<16>(r, dct, pairs, npairs, nphase, X, W, s);
-        case 32: return launch_region_rows_t<32>(r, dct, pairs, npairs, nphase, X, W, s);
-        case 64: return launch_region_rows_t<64>(r, dct, pairs, npairs, nphase, X, W, s);
-        case 128: return launch_region_rows_t<128>(r, dct, pairs, npairs, nphase, X, W, s);
-        case 256: return launch_region_rows_t<256>(r, dct, pairs, npairs, nphase, X, W, s);
-        case 512: return launch_region_rows_t<512>(r, dct, pairs, npairs, nphase, X, W, s);
-        case 1024: return launch_region_rows_t<1024>(r, dct, pairs, npairs, nphase, X, W, s);
-        case 2048: return launch_region_rows_t<2048>(r, dct, pairs, npairs, nphase, X, W, s);
-        case 4096: return launch_region_rows_t<4096>(r, dct, pairs, npairs, nphase, X, W, s);
+        case 16: return launch_region_rows_t<16>(r, dct, runs, pairs, npairs, nphase, X, W, s);
+        case 32: return launch_region_rows_t<32>(r, dct, runs, pairs, npairs, nphase, X, W, s);
+        case 64: return launch_region_rows_t<64>(r, dct, runs, pairs, npairs, nphase, X, W, s);
+        case 128: return launch_region_rows_t<128>(r, dct, runs, pairs, npairs, nphase, X, W, s);
+        case 256: return launch_region_rows_t<256>(r, dct, runs, pairs, npairs, nphase, X, W, s);
+        case 512: return launch_region_rows_t<512>(r, dct, runs, pairs, npairs, nphase, X, W, s);
+        case 1024: return launch_region_rows_t<1024>(r, dct, runs, pairs, npairs, nphase, X, W, s);
+        case 2048: return launch_region_rows_t<2048>(r, dct, runs, pairs, npairs, nphase, X, W, s);
+        case 4096: return launch_region_rows_t<4096>(r, dct, runs, pairs, npairs, nphase, X, W, s);
         default: return fail(FSIM_ERR_ARG, "estimate: unsupported size");
     }
 }
@@ -596,83 +741,130 @@ __device__ __forceinline__ double2 eb_block_sum(double2 v, double2 *sh) {
     return r;
 }
 
-// Phasor tables of the candidate filter, per pair (4 n double2): TYB[y] =
-// e^{-2 pi i ypos_0 y / n}, TYS[y] = e^{-2 pi i (search / 4.5) y / n},
-// TXB[x] = e^{+2 pi i xpos_0 x / n}, TXS[x] = e^{+2 pi i (search / 4.5) x / n}.
-// The grid's positions are pos_0 + i search / 4.5, so TY_j = TYB TYS^j and
-// TX_i = TXB TXS^i (the shift of b1 by (xpos, -ypos)).  Phases reduced mod 1
-// and evaluated in double.
-__global__ void k_eb_apx_tab(const EstPair *__restrict__ pairs, int n, double2 *__restrict__ tabs) {
+// Phasor tables of the candidate filter, per pair (kApxTab n float2):
+// TY[y][j] = e^{-2 pi i ypos_j y / n} rounded to float (10 n float2), then
+// as double2 TXB[x] = e^{+2 pi i xpos_0 x / n} and TXS[x] =
+// e^{+2 pi i (search / 4.5) x / n}: the grid's positions are
+// pos_0 + i search / 4.5, so TX_i = TXB TXS^i (the shift of b1 by
+// (xpos, -ypos)).  Phases reduced mod 1 and evaluated in double.
+constexpr int kApxTab = 14;
+__device__ __forceinline__ const double2 *apx_txb(const float2 *tabs, int p, int n) {
+    return reinterpret_cast<const double2 *>(tabs + (size_t)p * kApxTab * n + 10 * n);
+}
+__global__ void k_eb_apx_tab(const EstPair *__restrict__ pairs, int n, float2 *__restrict__ tabs) {
     const int p = blockIdx.y;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= 4 * n) return;
-    const int which = k / n, v = k % n;
+    if (k >= 12 * n) return;
     const double search = pairs[p].search;
-    const double pos = which == 0   ? grid_pos(pairs[p].ky, 0, search)
-                       : which == 2 ? grid_pos(pairs[p].kx, 0, search)
-                                    : search / 4.5;
+    double pos;
+    int v;
+    if (k < 10 * n) {
+        pos = grid_pos(pairs[p].ky, k / n, search);
+        v = k % n;
+    } else {
+        v = (k - 10 * n) % n;
+        pos = k < 11 * n ? grid_pos(pairs[p].kx, 0, search) : search / 4.5;
+    }
     double u = pos * v / n;
     u -= floor(u);
     double sn, cs;
     sincospi(2 * u, &sn, &cs);
-    tabs[(size_t)p * 4 * n + k] = make_double2(cs, which < 2 ? -sn : sn);
+    float2 *t = tabs + (size_t)p * kApxTab * n;
+    if (k < 10 * n)
+        t[v * 10 + k / n] = make_float2((float)cs, (float)-sn);
+    else
+        reinterpret_cast<double2 *>(t + 10 * n)[k - 10 * n] = make_double2(cs, sn);
 }
 
 // A round's inverse column pass fused with the approximate correlation of
 // all 100 candidates (the filter described at the top of this file): per
-// column x, C_j(x) = sum_y TY_j(y) q(x, y), q = b1 conj(b0) in double, by
-// the recurrence z_{j+1} = z_j TYS(y); then A_tile(i, j) = sum_x TX_i(x)
-// C_j(x) over the tile's columns, and the tile's sum |q|.  Nothing is
-// written back: the exact evaluation transforms the columns again.  Grid
-// xcd_grid(npairs * NTILE): neighbouring tiles of a pair run on one XCD and
-// share the cache lines of their rows.
+// column x, C_j(x) = sum_y TY_j(y) q(x, y), q = b1 conj(b0), each thread's
+// terms in float and then in double; A_tile(i, j) = sum_x TX_i(x) C_j(x)
+// over the tile's columns in double, and the tile's sum |q|.  The float part
+// stays within kApxArith sum|q| of the exact sums (bound in k_eb_select).
+// Nothing is written back: the exact evaluation transforms the columns
+// again.  Grid xcd_grid(npairs * NTILE): neighbouring tiles of a pair run
+// on one XCD and share the cache lines of their rows.
 // apart[(pair * kPairSlots + c) * NTILE + tile], sum |q| in slot kNormSlot.
 template <int L>
 __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__restrict__ W, int npairs,
-                                                               const double2 *__restrict__ tabs,
+                                                               const float2 *__restrict__ tabs,
                                                                const float2 *__restrict__ tw,
                                                                double2 *__restrict__ apart) {
     using C = EbCols<L>;
     constexpr int E = C::E, T = C::T, CW = C::CW, NT = C::NT, NW = C::NW, NTILE = C::NTILE;
-    __shared__ float2 lds[C::LDS];
+    // the pair's phasor table TY (10 L float2) moves into the transforms'
+    // LDS region once they are done, where it fits (L <= 512): read from
+    // global memory, its ten entries per pixel were the kernel's bottleneck
+    constexpr bool TY_LDS = 10 * L * (int)sizeof(float2) <= 65536;
+    constexpr int LDSN = TY_LDS && 10 * L > C::LDS ? 10 * L : C::LDS;
+    __shared__ __align__(16) float2 lds[LDSN];
     __shared__ float2 twl[C::TW::len];
-    __shared__ double2 red[NW][CW][10];
+    // the row sums per wavefront reuse the same region where it fits
+    constexpr bool red_in_lds = NW * CW * 10 * sizeof(double2) <= LDSN * sizeof(float2);
+    __shared__ double2 red_own[red_in_lds ? 1 : NW * CW * 10];
+    double2 *red = red_in_lds ? reinterpret_cast<double2 *>(lds) : red_own;
     __shared__ double2 col[CW][10], txi[CW][10];
     __shared__ double ared[NW];
     const int w = xcd_linear(npairs * NTILE);
     if (w >= npairs * NTILE) return;   // block-uniform, before any barrier
     const int p = w / NTILE, tile = w % NTILE;
+    const float2 *TY = tabs + (size_t)p * kApxTab * L;
+    const double2 *TXB = apx_txb(tabs, p, L), *TXS = TXB + L;
+    // the table's loads are issued before the transforms and held in registers
+    constexpr int RP = TY_LDS ? (10 * L + NT - 1) / NT : 1;
+    float2 typ[RP];
+    if constexpr (TY_LDS) {
+#pragma unroll
+        for (int r = 0; r < RP; r++) {
+            const int k = threadIdx.x + r * NT;
+            if (k < 10 * L) typ[r] = TY[k];
+        }
+    }
     const float2 *tws = tw_table<C::TW::tl, NT>(twl, tw);
     const int c = threadIdx.x % CW, t = threadIdx.x / CW;
     float2 a[E], b[E];
     eb_cols_load<L>(W, npairs, p, tile, c, t, a, b, lds, tws);
-    const double2 *TYB = tabs + (size_t)p * 4 * L, *TYS = TYB + L, *TXB = TYS + L, *TXS = TXB + L;
-    double2 S[10];
+    if constexpr (TY_LDS) {
+        // eb_cols_load ends with a barrier after the transforms' last LDS reads
 #pragma unroll
-    for (int j = 0; j < 10; j++) S[j] = make_double2(0.0, 0.0);
-    double aq = 0.0;
+        for (int r = 0; r < RP; r++) {
+            const int k = threadIdx.x + r * NT;
+            if (k < 10 * L) lds[k] = typ[r];
+        }
+        __syncthreads();
+    }
+    auto ty_at = [&](int i) -> float2 {
+        if constexpr (TY_LDS)
+            return lds[i];
+        else
+            return TY[i];
+    };
+    float2 Sf[10];
+#pragma unroll
+    for (int j = 0; j < 10; j++) Sf[j] = make_float2(0.f, 0.f);
+    float aqf = 0.f;
 #pragma unroll
     for (int m = 0; m < E; m++) {
         const int y = t + T * m;
         const float2 u = a[m], v = b[m];
-        const double qr = (double)v.x * u.x + (double)v.y * u.y;
-        const double qi = (double)v.y * u.x - (double)v.x * u.y;
-        // |q| for the bound's sum |q| in single precision: its relative
-        // error (~1e-7) is far inside the bound's factor 1.5
-        aq += (double)sqrtf((float)(qr * qr + qi * qi));
-        const double2 e = TYB[y], s = TYS[y];
-        double zr = qr * e.x - qi * e.y, zi = qr * e.y + qi * e.x;
+        const float qr = v.x * u.x + v.y * u.y;
+        const float qi = v.y * u.x - v.x * u.y;
+        // |q| for the bound's sum |q|: a few ulps, far inside its factor 1.5
+        aqf += __builtin_amdgcn_sqrtf(qr * qr + qi * qi);
 #pragma unroll
         for (int j = 0; j < 10; j++) {
-            S[j].x += zr;
-            S[j].y += zi;
-            if (j < 9) {
-                const double nr = zr * s.x - zi * s.y;
-                zi = zr * s.y + zi * s.x;
-                zr = nr;
-            }
+            const float2 e = ty_at(y * 10 + j);
+            Sf[j].x += qr * e.x - qi * e.y;
+            Sf[j].y += qr * e.y + qi * e.x;
         }
     }
+    // the table's last reads come before the row sums overwrite the region
+    if constexpr (TY_LDS && red_in_lds) __syncthreads();
+    double2 S[10];
+#pragma unroll
+    for (int j = 0; j < 10; j++) S[j] = make_double2((double)Sf[j].x, (double)Sf[j].y);
+    double aq = (double)aqf;
     // sum over the rows: the lanes of one column within the wavefront
     // (lane = t' CW + c), then over the wavefronts
 #pragma unroll
@@ -689,15 +881,15 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__r
     const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
     if (lane < CW)
 #pragma unroll
-        for (int j = 0; j < 10; j++) red[wv][lane][j] = S[j];
+        for (int j = 0; j < 10; j++) red[(wv * CW + lane) * 10 + j] = S[j];
     if (lane == 0) ared[wv] = aq;
     __syncthreads();
     for (int k = threadIdx.x; k < CW * 10; k += NT) {
         const int cc = k / 10, j = k % 10;
         double2 s = make_double2(0.0, 0.0);
         for (int i = 0; i < NW; i++) {
-            s.x += red[i][cc][j].x;
-            s.y += red[i][cc][j].y;
+            s.x += red[(i * CW + cc) * 10 + j].x;
+            s.y += red[(i * CW + cc) * 10 + j].y;
         }
         col[cc][j] = s;
         // TX_j at this column (here j counts x candidates): TXB TXS^j
@@ -730,6 +922,12 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__r
     }
 }
 
+// Distance of the approximate sums from the exact ones, relative to sum|q|,
+// from their float part (u = 2^-24): q = b1 conj(b0) in float (<= 3 u |q|),
+// the float phasor (<= u), the product (<= 3 u), a thread's E <= 16 terms
+// summed in float (<= 15 u); the rest is double.  32 u leaves margin.
+constexpr double kApxArith = 32 * 0x1p-24;
+
 // Candidates that can hold the maximum (see the top of this file); all 100
 // when the bound cannot separate (no positive threshold, or NaN).
 __global__ void __launch_bounds__(128) k_eb_select(EstPair *__restrict__ pairs, const double2 *__restrict__ apart,
@@ -754,7 +952,7 @@ __global__ void __launch_bounds__(128) k_eb_select(EstPair *__restrict__ pairs, 
         if (mag[c] > M) M = mag[c];
     }
     const double phimax = 2 * M_PI * (fabs(pairs[p].kx) + fabs(pairs[p].ky) + 2 * pairs[p].search);
-    const double delta = 1.5 * mag[kNormSlot] * (phimax * 0x1p-24 * (1 + 1e-6) + 2e-6);
+    const double delta = 1.5 * mag[kNormSlot] * (phimax * 0x1p-24 * (1 + 1e-6) + 2e-6 + kApxArith);
     const double thr = M - 2 * delta - 1e-12 * M;
     const bool all = bad || !(thr > 0.0) || !(delta == delta);
     int ns = 0;
@@ -847,7 +1045,7 @@ static int eb_ntile(int n) {
 }
 
 template <int L>
-static int launch_cols_apx_t(const float2 *W, int npairs, const double2 *tabs, double2 *apart, hipStream_t s) {
+static int launch_cols_apx_t(const float2 *W, int npairs, const float2 *tabs, double2 *apart, hipStream_t s) {
     using C = EbCols<L>;
     const float2 *tw = twiddles(L);
     if (!tw) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
@@ -856,7 +1054,7 @@ static int launch_cols_apx_t(const float2 *W, int npairs, const double2 *tabs, d
     FSIM_HIP(hipGetLastError());
     return FSIM_OK;
 }
-static int launch_cols_apx(int n, const float2 *W, int npairs, const double2 *tabs, double2 *apart, hipStream_t s) {
+static int launch_cols_apx(int n, const float2 *W, int npairs, const float2 *tabs, double2 *apart, hipStream_t s) {
     FSIM_EB_SWITCH(n, launch_cols_apx_t, W, npairs, tabs, apart, s)
 }
 
@@ -1039,16 +1237,16 @@ static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 // commonRegion fused with the inverse row pass, then with `grid` (fitPeak's
 // 10 x 10 candidates) the inverse column pass fused with the candidate
 // filter and the selection, then the exact evaluation of what the filter
-// kept (the inverse column pass once more), then the step.  tabs: 4 n
-// phasors per pair; apart, part: partial sums of eb_ntile(n) tiles.
-static int eb_round(const EbRegion &r, const float2 *dct, EstPair *pairs, int npairs, bool grid, int nphase, int n,
-                    double2 *part, double2 *tabs, double2 *apart, hipStream_t s) {
+// kept (the inverse column pass once more), then the step.  tabs: kApxTab n
+// float2 of phasors per pair; apart, part: partial sums of eb_ntile(n) tiles.
+static int eb_round(const EbRegion &r, const float2 *dct, const EbRuns *runs, EstPair *pairs, int npairs, bool grid,
+                    int nphase, int n, double2 *part, float2 *tabs, double2 *apart, hipStream_t s) {
     const int ntile = eb_ntile(n);
     hipLaunchKernelGGL(k_eb_ring, dim3(cdiv(npairs, 64)), dim3(64), 0, s, pairs, npairs, n, 0.15);
-    int rc = launch_region_rows(n, r, dct, pairs, npairs, nphase, g_eb.X, g_eb.W, s);
+    int rc = launch_region_rows(n, r, dct, runs, pairs, npairs, nphase, g_eb.X, g_eb.W, s);
     if (rc) return rc;
     if (grid) {
-        hipLaunchKernelGGL(k_eb_apx_tab, dim3(cdiv((size_t)4 * n, 256), npairs), dim3(256), 0, s, pairs, n, tabs);
+        hipLaunchKernelGGL(k_eb_apx_tab, dim3(cdiv((size_t)12 * n, 256), npairs), dim3(256), 0, s, pairs, n, tabs);
         rc = launch_cols_apx(n, g_eb.W, npairs, tabs, apart, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_eb_select, dim3(npairs), dim3(128), 0, s, pairs, apart, ntile);
@@ -1125,10 +1323,11 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     const size_t o_res = align256(o_shift + 2 * G * sizeof(double));
     const size_t o_err = align256(o_res + G * sizeof(fsim_estimate_result));
     const size_t o_tabs = align256(o_err + G * sizeof(int));
-    const size_t o_apart = align256(o_tabs + (size_t)G * 4 * n * sizeof(double2));
+    const size_t o_apart = align256(o_tabs + (size_t)G * kApxTab * n * sizeof(float2));
     const int ndc = o.nband == 1 ? 1 : (o.nband < 3 ? o.nband : 3);
     const size_t o_dct = align256(o_apart + (size_t)G * kPairSlots * ntile * sizeof(double2));
-    const size_t small = align256(o_dct + (size_t)ndc * nn * sizeof(float2));
+    const size_t o_runs = align256(o_dct + (size_t)ndc * nn * sizeof(float2));
+    const size_t small = align256(o_runs + (size_t)ndc * sizeof(EbRuns));
     if (!rc) rc = eb_grow(&g_eb.small, g_eb.small_bytes, small);
     if (rc) return rc;
     char *sb = reinterpret_cast<char *>(g_eb.small);
@@ -1141,9 +1340,10 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     double *shift = reinterpret_cast<double *>(sb + o_shift);
     fsim_estimate_result *resd = reinterpret_cast<fsim_estimate_result *>(sb + o_res);
     int *errd = reinterpret_cast<int *>(sb + o_err);
-    double2 *tabs = reinterpret_cast<double2 *>(sb + o_tabs);
+    float2 *tabs = reinterpret_cast<float2 *>(sb + o_tabs);
     double2 *apart = reinterpret_cast<double2 *>(sb + o_apart);
     float2 *dct = reinterpret_cast<float2 *>(sb + o_dct);
+    EbRuns *runs = reinterpret_cast<EbRuns *>(sb + o_runs);
 
     hipStream_t s = stream();
     g_eb.fade_h.resize(2 * (size_t)n);
@@ -1153,6 +1353,7 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     hipLaunchKernelGGL(k_eb_init, dim3(gb), dim3(64), 0, s, grp, G);
     hipLaunchKernelGGL(k_eb_dc_tab, dim3((unsigned)std::min<size_t>((ndc * nn + 255) / 256, 4096)), dim3(256), 0, s,
                        dct, n, reg, ndc);
+    hipLaunchKernelGGL(k_eb_runs, dim3(ndc), dim3(256), 0, s, reg, n, 0.05, runs);
     rc = launch_load_rows(n, images, g_eb.X, G * P * n, fade, fade + n, live, s);
     if (rc) return rc;
     rc = fft_cols_batch_device(g_eb.X, n, n, G * P, 0, s);
@@ -1173,13 +1374,13 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     FSIM_HIP(hipGetLastError());
     // fitPeak: three rounds of commonRegion, inverse FFTs, candidate filter, argmax
     for (int iter = 0; iter < 3; iter++) {
-        rc = eb_round(reg, dct, fit, G, true, P, n, part, tabs, apart, s);
+        rc = eb_round(reg, dct, runs, fit, G, true, P, n, part, tabs, apart, s);
         if (rc) return rc;
     }
     // getPeak of band 1 (and band 2)
     hipLaunchKernelGGL(k_eb_gp_init, dim3(gb), dim3(64), 0, s, fit, gp, G, B, fit_band, live ? 1 : 0, shift);
     FSIM_HIP(hipGetLastError());
-    rc = eb_round(reg, dct, gp, npk * G, false, P, n, part, tabs, apart, s);
+    rc = eb_round(reg, dct, runs, gp, npk * G, false, P, n, part, tabs, apart, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_eb_finish, dim3(gb), dim3(64), 0, s, fit, gp, G, npk, live ? 1 : 0, peaks, shift, resd,
                        errd);
@@ -1191,6 +1392,11 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     for (int g = 0; g < G; g++)
         if (err[g]) return fail(FSIM_ERR_ARG, "estimate: commonRegion shift larger than the vector (stack " +
                                                   std::to_string(g / D) + ", direction " + std::to_string(g % D) + ")");
+    return FSIM_OK;
+}
+
+extern "C" int fsim_debug_region_exact(int on) {
+    g_region_exact = on ? 1 : 0;
     return FSIM_OK;
 }
 

@@ -113,12 +113,20 @@ __device__ __forceinline__ float att_val(const float *__restrict__ only_att, dou
     return retl + reth;
 }
 
-// Radius in cycles/micron of pixel (x, y) from centre (kx, ky):
-// MTool.fhypot(xh-kx, yh-ky) * vecCyclesPerMicron (OtfProvider.java:347-349).
-__device__ __forceinline__ double cycl_at(int x, int y, int w, int h, double kx, double ky, double vec_cpm) {
+// Squared distance of pixel (x, y) from centre (kx, ky) in pixels, and the
+// radius in cycles/micron from it: MTool.fhypot(xh-kx, yh-ky) *
+// vecCyclesPerMicron (OtfProvider.java:347-349).
+__device__ __forceinline__ double d2_at(int x, int y, int w, int h, double kx, double ky) {
 #pragma clang fp contract(off)
     const double dx = xh_of(x, w) - kx, dy = yh_of(y, h) - ky;
-    return sqrt(dx * dx + dy * dy) * vec_cpm;
+    return dx * dx + dy * dy;
+}
+__device__ __forceinline__ double cycl_of_d2(double d2, double vec_cpm) {
+#pragma clang fp contract(off)
+    return sqrt(d2) * vec_cpm;
+}
+__device__ __forceinline__ double cycl_at(int x, int y, int w, int h, double kx, double ky, double vec_cpm) {
+    return cycl_of_d2(d2_at(x, y, w, h, kx, ky), vec_cpm);
 }
 
 // otfToVector value at a pixel (OtfProvider.java:351-365): 0 above cutoff.

@@ -218,6 +218,7 @@ _SIGS = {
     "fsim_stage_timing": [_I],
     "fsim_stage_times": [_PD, _PI, _I],
     "fsim_debug_fail_alloc": [_I],
+    "fsim_debug_region_exact": [_I],
 }
 _RET = {
     "fsim_last_error": (C.c_char_p, []),

@@ -264,6 +264,11 @@ typedef struct {
  *   commonRegion (the reference fails there with an index exception). */
 int fsim_estimate(const float *images, int count, const fsim_estimate_params *p, fsim_otf otf,
                   fsim_estimate_result *res);
+/* Test hook, no counterpart in the reference: on != 0 makes fsim_estimate's
+ * commonRegion pass form the shifted OTF weight at every pixel instead of
+ * classifying pixels by squared distance through per-table run starts; the
+ * two give the same bits.  For tests that compare them. */
+int fsim_debug_region_exact(int on);
 
 /* ---- fused fast-shift reconstruction --------------------------------------
  * One reconstruction = livemode/ReconstructionRunner.java:456-542 /

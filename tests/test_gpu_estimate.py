@@ -310,3 +310,62 @@ def test_live_refit_keeps_modulation_two(fsim_lib):
         assert np.isnan(res_live[d].mod2) and not np.isnan(res_est[d].mod2)
         assert live.modul[d][2] == 0.42 and est.modul[d][2] == res_est[d].mod2
         assert live.modul[d][1] == res_live[d].mod1 and live.px[d] == res_live[d].px / 2
+
+
+def _region_otf(kind, n):
+    """OTFs for the commonRegion run tables: the estimated one (one pass run,
+    then fail); a measured three-band table whose magnitude crosses the 0.05
+    limit several times and holds samples at the limit and one float step on
+    either side (undecided runs), with imaginary parts; and one that crosses
+    it every other sample (more runs than the tables hold: every pixel
+    evaluated)."""
+    from fsim.sim_algorithm import OtfProvider
+    est = OtfProvider.fromEstimate(NA, LAM, A_OTF)
+    if kind == "estimate":
+        o = est
+    else:
+        base = est.band_values()[0]
+        ns = base.shape[0]
+        lim = np.float32(0.05)
+        bands = []
+        for b in range(3):
+            v = base.astype(np.complex64).copy()
+            if kind == "crossings":
+                k = np.arange(ns // 5, ns // 5 + 9)
+                v[k] = [0.04, 0.06, lim, np.nextafter(lim, np.float32(1)), np.nextafter(lim, np.float32(0)),
+                        0.2, 0.03, 0.05 * np.exp(0.3j), 0.07]
+                v[ns // 2 + b] = 0.049999
+                v[ns // 2 + 3 + b] = 0.5
+            else:   # alternating
+                k = np.arange(ns // 6, ns // 6 + 40)
+                v[k] = np.where(k % 2 == 0, 0.03, 0.3)
+            v = v * np.exp(1j * 0.2 * b * np.arange(ns) / ns)
+            bands.append(v.astype(np.complex64))
+        o = OtfProvider.fromData(NA, LAM, est.cpm, np.stack(bands))
+    o.setPixelSize(1 / (n * PX_UM))
+    return o
+
+
+@pytest.mark.parametrize("kind", ["estimate", "crossings", "alternating"])
+def test_region_runs_equal_per_pixel_weight(fsim_lib, kind):
+    """fsim_estimate's commonRegion pass classifies pixels by squared distance
+    through per-table run starts (pass / fail / undecided); with the test
+    hook it forms the shifted weight at every pixel instead.  The estimates
+    are bitwise the same for a monotone OTF, for tables with runs of every
+    class (samples at the limit, one float step either side, complex
+    values, several crossings) and for a table with too many runs."""
+    from fsim._lib import lib
+    from fsim.sim_algorithm import SimParam, estimate_batch
+    n = 256
+    images, _ = _stacks(2, 5, (0.45, 0.42), 0.3, n=n, seed0=23)
+    param = SimParam.create(3, 3, 5, n, PX_UM, _region_otf(kind, n))
+    L = lib()
+    outs = []
+    for exact in (0, 1):
+        L.fsim_debug_region_exact(exact)
+        try:
+            res = estimate_batch(param, 2, 0.6, images=images, raw=True)
+            outs.append(bytes(res))
+        finally:
+            L.fsim_debug_region_exact(0)
+    assert outs[0] == outs[1]
