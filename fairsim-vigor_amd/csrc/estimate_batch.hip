@@ -181,121 +181,229 @@ static int launch_load_rows(int L, const float *img, float2 *X, int nrows, const
     }
 }
 
-// BandSeparation.separateBands with equidistant phases
-// (BandSeparation.java:131-165, 183-197) for the components the estimation
-// reads, in place; the attenuated copies of component 0 and of the fit
-// band's component (SimAlgorithm.java:84-92: duplicate, times(otfAtt) with
-// writeAttenuationVector(.99, 0.15 * cutoff)) go to W.  Grid (pixels, G).
-__global__ void k_eb_separate(float2 *__restrict__ X, float2 *__restrict__ W, int n, int G, EbSep a) {
-    const size_t nn = (size_t)n * n;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nn) return;
-    const int g = blockIdx.y;
-    float2 *in = X + (size_t)g * a.nphase * nn;
-    float2 x[kMaxPhase];
-#pragma unroll
-    for (int p = 0; p < kMaxPhase; p++)
-        if (p < a.nphase) x[p] = in[(size_t)p * nn + i];
-    float2 acc[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        acc[c] = make_float2(0.f, 0.f);
-        if (c < a.ncomp)
-#pragma unroll
-            for (int p = 0; p < kMaxPhase; p++)
-                if (p < a.nphase) acc[c] = sep_axpy(acc[c], a.S[c][p], x[p]);
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-        if (c < a.ncomp) in[(size_t)c * nn + i] = acc[c];
-    const int px = (int)(i % n), py = (int)(i / n);
-    const float att = attenuation_pixel(px, py, n, n, a.vec_cpm, .99, a.att_fwhm, 0.0, 0.0);
-    const float2 f = a.fb == 2 ? acc[2] : acc[1];
-    W[(size_t)g * nn + i] = make_float2(acc[0].x * att, acc[0].y * att);
-    W[((size_t)G + g) * nn + i] = make_float2(f.x * att, f.y * att);
+// The fused column kernels (the input's separation, the coarse correlation's
+// peak, and each round's inverse column transform of both bands of a pair
+// with the candidate filter or the exact evaluation) use the stand-alone
+// column pass's tile width, capped at 512 threads per block so that a
+// thread may hold up to 256 VGPRs, and its twiddle source, so that the
+// transforms return its values bit for bit.
+template <int L>
+struct EbCols {
+    static constexpr int E = FftE<L>::value, T = L / E;
+    static constexpr int CW0 = ColW<L>::value;
+    static constexpr int CW = T * CW0 <= 512 ? CW0 : (512 / T >= 1 ? 512 / T : 1);
+    static constexpr int NT = T * CW, NW = (NT + 63) / 64;
+    static constexpr int NTILE = L / CW;
+    using TW = TwLds<L, (FftLds<L, E>::len * CW0 + L) * 8 <= 163840>;
+    static constexpr int LDS = FftLds<L, E>::len * CW;
+};
+
+// Attenuation vector of the coarse correlation (SimAlgorithm.java:84-92:
+// writeAttenuationVector(.99, 0.15 * cutoff), OtfProvider.java:503-529) as a
+// table over the squared integer radius d2 = dx^2 + dy^2 in [0, n^2 / 2]:
+// centred at 0 the per-pixel value depends on d2 alone (attenuation_pixel
+// computes the same d2 exactly), so the column kernel reads one float per
+// pixel instead of evaluating a double exp.
+__global__ void k_eb_att_tab(float *__restrict__ tab, int nd2, double vec_cpm, double fwhm) {
+    for (int d2 = blockIdx.x * blockDim.x + threadIdx.x; d2 < nd2; d2 += gridDim.x * blockDim.x)
+        tab[d2] = attenuation_d2((double)d2, vec_cpm, .99, fwhm);
 }
 
-// The column pass of the coarse correlation's inverse transforms fused with
-// c_fb.timesConj(c0) (SimAlgorithm.java:85-101, AbstractVectorCplx.java:
-// 288-304): one block transforms the same column tile of both images of a
-// group, each column exactly as the stand-alone column pass does (scaled by
-// 1 / L), and writes the product over c_fb; c0 in real space is not read
-// again.  Saves the product's pass over three images and the write of c0.
-// Grid (L / CW, G); W holds c0 of the G groups, then c_fb (Gnn = G * L * L
-// apart).
-template <int L>
-__global__ void __launch_bounds__(1024) k_eb_cols_conj(float2 *__restrict__ W, size_t Gnn,
-                                                       const float2 *__restrict__ tw) {
-    constexpr int E = FftE<L>::value, T = L / E;
-    constexpr int CW = ColW<L>::value;
-    __shared__ float2 lds[FftLds<L, E>::len * CW];
-    using TW = TwLds<L, (FftLds<L, E>::len * CW + L) * 8 <= 163840>;
-    __shared__ float2 twl[TW::len];
-    const float2 *tws = tw_table<TW::tl, T * CW>(twl, tw);
+// The input's column pass fused with the separation and the first half of
+// the coarse correlation.  Per group and column tile: each phase image's
+// forward column transform (as the stand-alone column pass; the rows were
+// transformed by k_eb_load_rows), accumulated into the components the flow
+// reads in the phase order of BandSeparation.separateBands with
+// equidistant phases (BandSeparation.java:131-165, 183-197); the components
+// go back to X slots 0, 1 (, 2) = components 0, 1 (, 3) in place (a block
+// reads its tile of every phase before it writes).  Then the attenuated
+// copies of component 0 and of the fit band's component
+// (SimAlgorithm.java:84-92: duplicate, times(otfAtt)) are taken to real
+// space by their inverse column transforms here (the inverse rows follow in
+// k_eb_rows_corr) and go to W[g] and W[G + g].  NC: stored components.
+template <int L, int NC>
+__global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_sep(float2 *__restrict__ X, float2 *__restrict__ W, int G,
+                                                               EbSep a, const float *__restrict__ att_tab,
+                                                               const float2 *__restrict__ tw) {
+    using C = EbCols<L>;
+    constexpr int E = C::E, T = C::T, CW = C::CW, NT = C::NT, NTILE = C::NTILE;
+    constexpr size_t nn = (size_t)L * L;
+    __shared__ float2 lds[C::LDS];
+    __shared__ float2 twl[C::TW::len];
+    const int w = xcd_linear(G * NTILE);
+    if (w >= G * NTILE) return;   // block-uniform, before any barrier
+    const int g = w / NTILE, tile = w % NTILE;
+    const float2 *tws = tw_table<C::TW::tl, NT>(twl, tw);
     const int c = threadIdx.x % CW, t = threadIdx.x / CW;
-    const float2 *a = W + (size_t)blockIdx.y * L * L + blockIdx.x * CW + c;
-    float2 *b = W + Gnn + (size_t)blockIdx.y * L * L + blockIdx.x * CW + c;
-    float2 v[E], u[E];
+    const int x = tile * CW + c;
+    float2 *in = X + (size_t)g * a.nphase * nn + x;
+    float2 acc[NC][E], nxt[E];
+#pragma unroll
+    for (int k = 0; k < NC; k++)
+#pragma unroll
+        for (int m = 0; m < E; m++) acc[k][m] = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int m = 0; m < E; m++) nxt[m] = in[(size_t)(t + T * m) * L];
+    for (int p = 0; p < a.nphase; p++) {
+        float2 v[E];
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = nxt[m];
+        // the next phase's loads stay in flight across this transform
+        if (p + 1 < a.nphase)
+#pragma unroll
+            for (int m = 0; m < E; m++) nxt[m] = in[(size_t)(p + 1) * nn + (size_t)(t + T * m) * L];
+        fft_regs<L, E, false, false, C::TW::tl>(v, lds + c, CW, t, tws);
+#pragma unroll
+        for (int k = 0; k < NC; k++)
+#pragma unroll
+            for (int m = 0; m < E; m++) acc[k][m] = sep_axpy(acc[k][m], a.S[k][p], v[m]);
+    }
+#pragma unroll
+    for (int k = 0; k < NC; k++)
+#pragma unroll
+        for (int m = 0; m < E; m++) in[(size_t)k * nn + (size_t)(t + T * m) * L] = acc[k][m];
+    float2 u[E], f[E];
 #pragma unroll
     for (int m = 0; m < E; m++) {
-        v[m] = a[(size_t)(t + T * m) * L];
-        u[m] = b[(size_t)(t + T * m) * L];
+        const int y = t + T * m;
+        const int dx = x < L / 2 ? x : x - L, dy = y < L / 2 ? y : y - L;
+        const float att = att_tab[dx * dx + dy * dy];
+        const float2 c0 = acc[0][m], cf = NC == 3 && a.fb == 2 ? acc[NC - 1][m] : acc[1][m];
+        u[m] = make_float2(c0.x * att, c0.y * att);
+        f[m] = make_float2(cf.x * att, cf.y * att);
     }
-    // fft_regs ends with a barrier after its last LDS reads, so the second
-    // transform may reuse the region at once
-    fft_regs<L, E, true, false, TW::tl>(v, lds + c, CW, t, tws);
-    fft_regs<L, E, true, false, TW::tl>(u, lds + c, CW, t, tws);
+    fft_regs<L, E, true, false, C::TW::tl>(u, lds + c, CW, t, tws);
+    fft_regs<L, E, true, false, C::TW::tl>(f, lds + c, CW, t, tws);
+    constexpr float sc = 1.0f / L;
+    float2 *w0 = W + (size_t)g * nn + x, *w1 = W + ((size_t)G + g) * nn + x;
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        w0[(size_t)(t + T * m) * L] = cscale(u[m], sc);
+        w1[(size_t)(t + T * m) * L] = cscale(f[m], sc);
+    }
+}
+
+// The coarse correlation's row pass: the inverse row transforms of both
+// attenuated copies (their columns were transformed by k_eb_cols_sep; the
+// stand-alone row pass's transform and 1 / L scale), c_fb.timesConj(c0)
+// (AbstractVectorCplx.java:288-304), and the forward row transform of the
+// product (SimAlgorithm.java:94-101), written over c_fb.  One row per
+// T = L / E threads; grid (ceil(G L / RW)).
+template <int L>
+__global__ void __launch_bounds__(256) k_eb_rows_corr(float2 *__restrict__ W, int G) {
+    constexpr int E = FftE<L>::value, T = L / E;
+    constexpr int RW = T >= 256 ? 1 : 256 / T;
+    constexpr int LP = FftLds<L, E>::len;
+    __shared__ float2 lds[RW * LP];
+    using TW = TwLds<L, false>;   // the row pass's twiddle source
+    constexpr size_t nn = (size_t)L * L;
+    const int t = threadIdx.x % T, rr = threadIdx.x / T;
+    const int row = blockIdx.x * RW + rr;   // row of G images of L rows
+    const bool active = row < G * L;
+    float2 *p0 = W + (size_t)row * L, *p1 = W + (size_t)G * nn + (size_t)row * L;
+    float2 v[E], u[E];
+    if (active) {
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            v[m] = p0[t + T * m];
+            u[m] = p1[t + T * m];
+        }
+    }
+    fft_regs<L, E, true, RowWave<L>::value, TW::tl>(v, lds + rr * LP, 1, t, nullptr, active);
+    fft_regs<L, E, true, RowWave<L>::value, TW::tl>(u, lds + rr * LP, 1, t, nullptr, active);
     constexpr float sc = 1.0f / L;
 #pragma unroll
-    for (int m = 0; m < E; m++) b[(size_t)(t + T * m) * L] = times_conj_java(cscale(u[m], sc), cscale(v[m], sc));
+    for (int m = 0; m < E; m++) u[m] = times_conj_java(cscale(u[m], sc), cscale(v[m], sc));
+    fft_regs<L, E, false, RowWave<L>::value, TW::tl>(u, lds + rr * LP, 1, t, nullptr, active);
+    if (active) {
+#pragma unroll
+        for (int m = 0; m < E; m++) p1[t + T * m] = u[m];
+    }
+}
+
+// The coarse correlation's forward column pass fused with
+// Correlation.locatePeak (Correlation.java:144-167): each group's key (the
+// magnitude bits and inverted index, peak_key) is the maximum over its
+// tiles, one atomicMax per block; the correlation map itself is never
+// written.
+template <int L>
+__global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_peak(const float2 *__restrict__ C1, int G, double kmin,
+                                                                EstGroup *__restrict__ grp,
+                                                                const float2 *__restrict__ tw) {
+    using C = EbCols<L>;
+    constexpr int E = C::E, T = C::T, CW = C::CW, NT = C::NT, NW = C::NW, NTILE = C::NTILE;
+    constexpr size_t nn = (size_t)L * L;
+    __shared__ float2 lds[C::LDS];
+    __shared__ float2 twl[C::TW::len];
+    __shared__ unsigned long long sh[NW];
+    const int w = xcd_linear(G * NTILE);
+    if (w >= G * NTILE) return;   // block-uniform, before any barrier
+    const int g = w / NTILE, tile = w % NTILE;
+    const float2 *tws = tw_table<C::TW::tl, NT>(twl, tw);
+    const int c = threadIdx.x % CW, t = threadIdx.x / CW;
+    const int x = tile * CW + c;
+    const float2 *src = C1 + (size_t)g * nn + x;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) v[m] = src[(size_t)(t + T * m) * L];
+    fft_regs<L, E, false, false, C::TW::tl>(v, lds + c, CW, t, tws);
+    unsigned long long k = 0;
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const int y = t + T * m;
+        const unsigned long long key = peak_key(v[m], x, y, L, L, kmin, (unsigned)(y * L + x));
+        if (key > k) k = key;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        if (o < NT) {
+            const unsigned long long q = __shfl_down(k, o, 64);
+            if (q > k) k = q;
+        }
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = k;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < NW; i++)
+            if (sh[i] > k) k = sh[i];
+        if (k) atomicMax(&grp[g].key, k);
+    }
 }
 
 template <int L>
-static int launch_cols_conj_t(float2 *W, int G, hipStream_t s) {
-    constexpr int E = FftE<L>::value, T = L / E;
-    constexpr int CW = ColW<L>::value;
-    static_assert(L % CW == 0, "column tiles cover the image");
+static int launch_front_t(float2 *X, float2 *W, int G, const EbSep &sep, const float *att_tab, double kmin,
+                          EstGroup *grp, hipStream_t s) {
+    using C = EbCols<L>;
     const float2 *tw = twiddles(L);
     if (!tw) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
-    hipLaunchKernelGGL(k_eb_cols_conj<L>, dim3(L / CW, G), dim3(T * CW), 0, s, W, (size_t)G * L * L, tw);
+    const unsigned gc = xcd_grid(G * C::NTILE);
+    if (sep.ncomp == 3)
+        hipLaunchKernelGGL((k_eb_cols_sep<L, 3>), dim3(gc), dim3(C::NT), 0, s, X, W, G, sep, att_tab, tw);
+    else
+        hipLaunchKernelGGL((k_eb_cols_sep<L, 2>), dim3(gc), dim3(C::NT), 0, s, X, W, G, sep, att_tab, tw);
+    constexpr int E = FftE<L>::value, T = L / E;
+    constexpr int RW = T >= 256 ? 1 : 256 / T;
+    hipLaunchKernelGGL(k_eb_rows_corr<L>, dim3(cdiv((size_t)G * L, RW)), dim3(T * RW), 0, s, W, G);
+    hipLaunchKernelGGL(k_eb_cols_peak<L>, dim3(gc), dim3(C::NT), 0, s, W + (size_t)G * L * L, G, kmin, grp, tw);
     FSIM_HIP(hipGetLastError());
     return FSIM_OK;
 }
 
-static int launch_cols_conj(int n, float2 *W, int G, hipStream_t s) {
+// The input's column pass with the separation, then the coarse correlation
+// to its peak (SimAlgorithm.java:84-110): three launches.
+static int launch_front(int n, float2 *X, float2 *W, int G, const EbSep &sep, const float *att_tab, double kmin,
+                        EstGroup *grp, hipStream_t s) {
     switch (n) {
-        case 16: return launch_cols_conj_t<16>(W, G, s);
-        case 32: return launch_cols_conj_t<32>(W, G, s);
-        case 64: return launch_cols_conj_t<64>(W, G, s);
-        case 128: return launch_cols_conj_t<128>(W, G, s);
-        case 256: return launch_cols_conj_t<256>(W, G, s);
-        case 512: return launch_cols_conj_t<512>(W, G, s);
-        case 1024: return launch_cols_conj_t<1024>(W, G, s);
-        case 2048: return launch_cols_conj_t<2048>(W, G, s);
-        case 4096: return launch_cols_conj_t<4096>(W, G, s);
+        case 16: return launch_front_t<16>(X, W, G, sep, att_tab, kmin, grp, s);
+        case 32: return launch_front_t<32>(X, W, G, sep, att_tab, kmin, grp, s);
+        case 64: return launch_front_t<64>(X, W, G, sep, att_tab, kmin, grp, s);
+        case 128: return launch_front_t<128>(X, W, G, sep, att_tab, kmin, grp, s);
+        case 256: return launch_front_t<256>(X, W, G, sep, att_tab, kmin, grp, s);
+        case 512: return launch_front_t<512>(X, W, G, sep, att_tab, kmin, grp, s);
+        case 1024: return launch_front_t<1024>(X, W, G, sep, att_tab, kmin, grp, s);
+        case 2048: return launch_front_t<2048>(X, W, G, sep, att_tab, kmin, grp, s);
+        case 4096: return launch_front_t<4096>(X, W, G, sep, att_tab, kmin, grp, s);
         default: return fail(FSIM_ERR_ARG, "estimate: unsupported size");
     }
-}
-
-// Correlation.locatePeak (Correlation.java:144-167) key of each group's
-// correlation map; grid (blocks, G).
-__global__ void __launch_bounds__(256) k_eb_peak(const float2 *__restrict__ C, int n, double kmin,
-                                                 EstGroup *__restrict__ grp) {
-    __shared__ unsigned long long sh[256];
-    const size_t nn = (size_t)n * n;
-    const float2 *v = C + (size_t)blockIdx.y * nn;
-    unsigned long long k = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x) {
-        const unsigned long long key = peak_key(v[i], (int)(i % n), (int)(i / n), n, n, kmin, (unsigned)i);
-        if (key > k) k = key;
-    }
-    sh[threadIdx.x] = k;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s && sh[threadIdx.x + s] > sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0 && sh[0]) atomicMax(&grp[blockIdx.y].key, sh[0]);
 }
 
 // locatePeak's result in its coordinate convention (Correlation.java:145-166):
@@ -670,22 +778,6 @@ static int launch_region_rows(int n, const EbRegion &r, const float2 *dct, const
         default: return fail(FSIM_ERR_ARG, "estimate: unsupported size");
     }
 }
-
-// The fused column kernels of a round (the inverse column transform of both
-// bands of a pair, then the candidate filter or the exact evaluation) use
-// the stand-alone column pass's tile width, capped at 512 threads per block
-// so that a thread may hold up to 256 VGPRs, and its twiddle source, so
-// that the transforms return its values bit for bit.
-template <int L>
-struct EbCols {
-    static constexpr int E = FftE<L>::value, T = L / E;
-    static constexpr int CW0 = ColW<L>::value;
-    static constexpr int CW = T * CW0 <= 512 ? CW0 : (512 / T >= 1 ? 512 / T : 1);
-    static constexpr int NT = T * CW, NW = (NT + 63) / 64;
-    static constexpr int NTILE = L / CW;
-    using TW = TwLds<L, (FftLds<L, E>::len * CW0 + L) * 8 <= 163840>;
-    static constexpr int LDS = FftLds<L, E>::len * CW;
-};
 
 // Inverse column transform of column tile `tile` of both bands of pair p
 // (their rows transformed by the region pass; band 0 of pair p at
@@ -1327,7 +1419,8 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     const int ndc = o.nband == 1 ? 1 : (o.nband < 3 ? o.nband : 3);
     const size_t o_dct = align256(o_apart + (size_t)G * kPairSlots * ntile * sizeof(double2));
     const size_t o_runs = align256(o_dct + (size_t)ndc * nn * sizeof(float2));
-    const size_t small = align256(o_runs + (size_t)ndc * sizeof(EbRuns));
+    const size_t o_att = align256(o_runs + (size_t)ndc * sizeof(EbRuns));
+    const size_t small = align256(o_att + (2 * (size_t)(n / 2) * (n / 2) + 1) * sizeof(float));
     if (!rc) rc = eb_grow(&g_eb.small, g_eb.small_bytes, small);
     if (rc) return rc;
     char *sb = reinterpret_cast<char *>(g_eb.small);
@@ -1344,6 +1437,7 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     double2 *apart = reinterpret_cast<double2 *>(sb + o_apart);
     float2 *dct = reinterpret_cast<float2 *>(sb + o_dct);
     EbRuns *runs = reinterpret_cast<EbRuns *>(sb + o_runs);
+    float *att_tab = reinterpret_cast<float *>(sb + o_att);
 
     hipStream_t s = stream();
     g_eb.fade_h.resize(2 * (size_t)n);
@@ -1356,20 +1450,15 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     hipLaunchKernelGGL(k_eb_runs, dim3(ndc), dim3(256), 0, s, reg, n, 0.05, runs);
     rc = launch_load_rows(n, images, g_eb.X, G * P * n, fade, fade + n, live, s);
     if (rc) return rc;
-    rc = fft_cols_batch_device(g_eb.X, n, n, G * P, 0, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_eb_separate, dim3(cdiv(nn, 256), G), dim3(256), 0, s, g_eb.X, g_eb.W, n, G, sep);
-    FSIM_HIP(hipGetLastError());
-    // coarse correlation: both to real space (row pass, then the column pass
-    // fused with c_fb * conj(c0)), back
-    rc = fft_rows_batch_device(g_eb.W, n, n, 2 * G, 1, s);
-    if (!rc) rc = launch_cols_conj(n, g_eb.W, G, s);
-    if (rc) return rc;
-    rc = fft2d_batch_device(g_eb.W + (size_t)G * nn, n, n, G, 0, s);
-    if (rc) return rc;
+    // column pass with the separation, the attenuated copies' inverse
+    // columns; their inverse rows, product and forward rows; the forward
+    // columns with locatePeak
+    const int nd2 = 2 * (n / 2) * (n / 2) + 1;
+    hipLaunchKernelGGL(k_eb_att_tab, dim3((unsigned)std::min(cdiv(nd2, 256), 4096u)), dim3(256), 0, s, att_tab, nd2,
+                       sep.vec_cpm, sep.att_fwhm);
     const double kmin = p->fit_exclude * o.cutoff / p->vec_cpm;
-    const unsigned pkb = cdiv(nn, 256 * 8);
-    hipLaunchKernelGGL(k_eb_peak, dim3(pkb, G), dim3(256), 0, s, g_eb.W + (size_t)G * nn, n, kmin, grp);
+    rc = launch_front(n, g_eb.X, g_eb.W, G, sep, att_tab, kmin, grp, s);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_eb_fit_init, dim3(gb), dim3(64), 0, s, grp, fit, G, n, sep.fb, fit_otf_band, peaks);
     FSIM_HIP(hipGetLastError());
     // fitPeak: three rounds of commonRegion, inverse FFTs, candidate filter, argmax

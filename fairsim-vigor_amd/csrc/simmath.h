@@ -139,11 +139,15 @@ __device__ __forceinline__ float2 otf_pixel(const float2 *__restrict__ tab, int 
 
 // OtfProvider.writeAttenuationVector value at pixel (x, y)
 // (OtfProvider.java:503-529): 1 - str * exp(-cycl^2 / (2 (fwhm / 2.355)^2)).
+// attenuation_d2: the same from the squared distance d2_at returns.
+__device__ __forceinline__ float attenuation_d2(double d2, double vec_cpm, double str, double fwhm) {
+#pragma clang fp contract(off)
+    const double cycl = cycl_of_d2(d2, vec_cpm);
+    return (float)(1 - str * exp(-(cycl * cycl) / (2 * ((fwhm / 2.355) * (fwhm / 2.355)))));
+}
 __device__ __forceinline__ float attenuation_pixel(int x, int y, int w, int h, double vec_cpm, double str,
                                                    double fwhm, double kx, double ky) {
-#pragma clang fp contract(off)
-    const double cycl = cycl_at(x, y, w, h, kx, ky, vec_cpm);
-    return (float)(1 - str * exp(-(cycl * cycl) / (2 * ((fwhm / 2.355) * (fwhm / 2.355)))));
+    return attenuation_d2(d2_at(x, y, w, h, kx, ky), vec_cpm, str, fwhm);
 }
 
 // Correlation.locatePeak (Correlation.java:144-167) ranking key of pixel i =
