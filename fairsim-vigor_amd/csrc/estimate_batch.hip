@@ -20,9 +20,9 @@
 // about 60 operations per pixel for all 100 candidates, fused with the
 // inverse column transform that produces the pixels.  The reference value
 // J(c) differs from the computed A(c) by at most
-//   D = sum|q| * (2^-24 * phi_max + 2e-6 + 32 * 2^-24) * 1.5
+//   D = sum|q| * (2^-24 * phi_max + 2e-6 + 64 * 2^-24) * 1.5
 // (2^-24 |phi|: rounding of the phase to float; 2e-6 covers the polynomial
-// and the reference's float products with margin; 32 * 2^-24 the float part
+// and the reference's float products with margin; 64 * 2^-24 the float part
 // of the approximate sums, kApxArith), so a candidate can
 // hold the maximum of |J| only if |A(c)| >= max|A| - 2D.  Those candidates
 // (typically a handful) are then evaluated in the reference's arithmetic and
@@ -91,98 +91,128 @@ __global__ void k_eb_init(EstGroup *__restrict__ g, int G) {
     if (i < G) g[i].key = 0ull;
 }
 
-// Input preparation of every frame (sim_gui/ImageControl.java:819-840) fused
-// with the forward row transform: fadeBorderCos on the float image (top,
-// bottom, left, right; each `float *= double` rounds once,
-// SimUtils.java:162-185) and the copy to the real part of a complex image, then
-// the row pass of the 2-D FFT (the same transform code as
-// fft2d_batch_device's row pass, hence the same values).  One row of one
-// frame per T = n/E threads.
+// Input preparation of every frame (sim_gui/ImageControl.java:819-840) and
+// the band separation, fused with the forward row transform.  Per row of a
+// group: each phase frame's fadeBorderCos (top, bottom, left, right; each
+// `float *= double` rounds once, SimUtils.java:162-185), the components the
+// flow reads accumulated in BandSeparation.separateBands' phase order
+// (BandSeparation.java:131-165, 183-197; S complex, the frames real), then
+// each component's row transform (the stand-alone row pass's code).  The
+// separation is linear and moves in front of the transform, as in the
+// fused reconstruction: the components equal the reference's up to float
+// rounding, and NC transforms per row remain instead of P.  X slots 0, 1
+// (, 2) of a group = components 0, 1 (, 3).  One row per T = n/E threads.
 // LIVE: the live refit's preparation instead (ReconstructionRunner.java:
 // 614-617, 661-664): the image times the float vector borderDampen, a vector
 // of ones faded by fadeBorderCos -- per pixel
 // (float)((float)((float)((float)(1 * lo[y]) * hi[y]) * lo[x]) * hi[x]),
 // each factor applied as `float *= double` -- in one float product.
-template <int L, bool LIVE>
-__global__ void __launch_bounds__(256) k_eb_load_rows(const float *__restrict__ img, float2 *__restrict__ X,
-                                                      int nrows, const double *__restrict__ lo,
-                                                      const double *__restrict__ hi) {
+template <int L, bool LIVE, int NC>
+__global__ void __launch_bounds__(256) k_eb_load_sep(const float *__restrict__ img, float2 *__restrict__ X, int G,
+                                                     EbSep a, const double *__restrict__ lo,
+                                                     const double *__restrict__ hi) {
     constexpr int E = FftE<L>::value, T = L / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int LP = FftLds<L, E>::len;
+    constexpr size_t nn = (size_t)L * L;
     __shared__ float2 lds[RW * LP];
     using TW = TwLds<L, false>;
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
-    const int row = blockIdx.x * RW + rr;
-    const bool active = row < nrows;
-    const int y = row % L;
-    float2 v[E];
+    const int row = blockIdx.x * RW + rr;   // row of G groups of L rows
+    const bool active = row < G * L;
+    const int g = row / L, y = row % L;
+    float2 acc[NC][E];
+#pragma unroll
+    for (int k = 0; k < NC; k++)
+#pragma unroll
+        for (int m = 0; m < E; m++) acc[k][m] = make_float2(0.f, 0.f);
     if (active) {
-        const float *src = img + (size_t)row * L;
+        const float *src = img + (size_t)g * a.nphase * nn + (size_t)y * L;
+        // every phase's pixels in flight before the first use
+        float d[kMaxPhase][E];
+#pragma unroll
+        for (int p = 0; p < kMaxPhase; p++)
+            if (p < a.nphase)
+#pragma unroll
+                for (int m = 0; m < E; m++) d[p][m] = src[(size_t)p * nn + t + T * m];
         const double ly = lo[y], hy = hi[y];
-        float d[E];
 #pragma unroll
-        for (int m = 0; m < E; m++) d[m] = src[t + T * m];
+        for (int p = 0; p < kMaxPhase; p++) {
+            if (p >= a.nphase) break;
 #pragma unroll
-        for (int m = 0; m < E; m++) {
-            const int x = t + T * m;
-            if constexpr (LIVE) {
-                float b = 1.0f;
-                b = (float)((double)b * ly);
-                b = (float)((double)b * hy);
-                b = (float)((double)b * lo[x]);
-                b = (float)((double)b * hi[x]);
-                v[m] = make_float2(d[m] * b, 0.f);
-            } else {
-                float e = (float)((double)d[m] * ly);
-                e = (float)((double)e * hy);
-                e = (float)((double)e * lo[x]);
-                e = (float)((double)e * hi[x]);
-                v[m] = make_float2(e, 0.f);
+            for (int m = 0; m < E; m++) {
+                const int x = t + T * m;
+                float e;
+                if constexpr (LIVE) {
+                    float b = 1.0f;
+                    b = (float)((double)b * ly);
+                    b = (float)((double)b * hy);
+                    b = (float)((double)b * lo[x]);
+                    b = (float)((double)b * hi[x]);
+                    e = d[p][m] * b;
+                } else {
+                    e = (float)((double)d[p][m] * ly);
+                    e = (float)((double)e * hy);
+                    e = (float)((double)e * lo[x]);
+                    e = (float)((double)e * hi[x]);
+                }
+#pragma unroll
+                for (int k = 0; k < NC; k++) acc[k][m] = sep_axpy(acc[k][m], a.S[k][p], make_float2(e, 0.f));
             }
         }
     }
-    fft_regs<L, E, false, RowWave<L>::value, TW::tl>(v, lds + rr * LP, 1, t, nullptr, active);
-    if (active) {
-        float2 *dst = X + (size_t)row * L;
 #pragma unroll
-        for (int m = 0; m < E; m++) dst[t + T * m] = v[m];
+    for (int k = 0; k < NC; k++) {
+        fft_regs<L, E, false, RowWave<L>::value, TW::tl>(acc[k], lds + rr * LP, 1, t, nullptr, active);
+        if (active) {
+            float2 *dst = X + ((size_t)g * a.nphase + k) * nn + (size_t)y * L;
+#pragma unroll
+            for (int m = 0; m < E; m++) dst[t + T * m] = acc[k][m];
+        }
     }
 }
 
-template <int L>
-static int launch_load_rows_t(const float *img, float2 *X, int nrows, const double *lo, const double *hi, bool live,
-                              hipStream_t s) {
+template <int L, int NC>
+static void launch_load_sep_tn(const float *img, float2 *X, int G, const EbSep &sep, const double *lo,
+                               const double *hi, bool live, hipStream_t s) {
     constexpr int E = FftE<L>::value, T = L / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
+    const dim3 grid(cdiv((size_t)G * L, RW)), block(T * RW);
     if (live)
-        hipLaunchKernelGGL((k_eb_load_rows<L, true>), dim3(cdiv(nrows, RW)), dim3(T * RW), 0, s, img, X, nrows, lo,
-                           hi);
+        hipLaunchKernelGGL((k_eb_load_sep<L, true, NC>), grid, block, 0, s, img, X, G, sep, lo, hi);
     else
-        hipLaunchKernelGGL((k_eb_load_rows<L, false>), dim3(cdiv(nrows, RW)), dim3(T * RW), 0, s, img, X, nrows, lo,
-                           hi);
+        hipLaunchKernelGGL((k_eb_load_sep<L, false, NC>), grid, block, 0, s, img, X, G, sep, lo, hi);
+}
+
+template <int L>
+static int launch_load_sep_t(const float *img, float2 *X, int G, const EbSep &sep, const double *lo, const double *hi,
+                             bool live, hipStream_t s) {
+    if (sep.ncomp == 3)
+        launch_load_sep_tn<L, 3>(img, X, G, sep, lo, hi, live, s);
+    else
+        launch_load_sep_tn<L, 2>(img, X, G, sep, lo, hi, live, s);
     FSIM_HIP(hipGetLastError());
     return FSIM_OK;
 }
 
-static int launch_load_rows(int L, const float *img, float2 *X, int nrows, const double *lo, const double *hi,
-                            bool live, hipStream_t s) {
+static int launch_load_sep(int L, const float *img, float2 *X, int G, const EbSep &sep, const double *lo,
+                           const double *hi, bool live, hipStream_t s) {
     switch (L) {
-        case 16: return launch_load_rows_t<16>(img, X, nrows, lo, hi, live, s);
-        case 32: return launch_load_rows_t<32>(img, X, nrows, lo, hi, live, s);
-        case 64: return launch_load_rows_t<64>(img, X, nrows, lo, hi, live, s);
-        case 128: return launch_load_rows_t<128>(img, X, nrows, lo, hi, live, s);
-        case 256: return launch_load_rows_t<256>(img, X, nrows, lo, hi, live, s);
-        case 512: return launch_load_rows_t<512>(img, X, nrows, lo, hi, live, s);
-        case 1024: return launch_load_rows_t<1024>(img, X, nrows, lo, hi, live, s);
-        case 2048: return launch_load_rows_t<2048>(img, X, nrows, lo, hi, live, s);
-        case 4096: return launch_load_rows_t<4096>(img, X, nrows, lo, hi, live, s);
+        case 16: return launch_load_sep_t<16>(img, X, G, sep, lo, hi, live, s);
+        case 32: return launch_load_sep_t<32>(img, X, G, sep, lo, hi, live, s);
+        case 64: return launch_load_sep_t<64>(img, X, G, sep, lo, hi, live, s);
+        case 128: return launch_load_sep_t<128>(img, X, G, sep, lo, hi, live, s);
+        case 256: return launch_load_sep_t<256>(img, X, G, sep, lo, hi, live, s);
+        case 512: return launch_load_sep_t<512>(img, X, G, sep, lo, hi, live, s);
+        case 1024: return launch_load_sep_t<1024>(img, X, G, sep, lo, hi, live, s);
+        case 2048: return launch_load_sep_t<2048>(img, X, G, sep, lo, hi, live, s);
+        case 4096: return launch_load_sep_t<4096>(img, X, G, sep, lo, hi, live, s);
         default: return fail(FSIM_ERR_ARG, "estimate: unsupported size");
     }
 }
 
-// The fused column kernels (the input's separation, the coarse correlation's
-// peak, and each round's inverse column transform of both bands of a pair
+// The fused column kernels (the components' column pass, the coarse
+// correlation's peak, and each round's inverse column transform of both bands of a pair
 // with the candidate filter or the exact evaluation) use the stand-alone
 // column pass's tile width, capped at 512 threads per block so that a
 // thread may hold up to 256 VGPRs, and its twiddle source, so that the
@@ -209,18 +239,14 @@ __global__ void k_eb_att_tab(float *__restrict__ tab, int nd2, double vec_cpm, d
         tab[d2] = attenuation_d2((double)d2, vec_cpm, .99, fwhm);
 }
 
-// The input's column pass fused with the separation and the first half of
-// the coarse correlation.  Per group and column tile: each phase image's
-// forward column transform (as the stand-alone column pass; the rows were
-// transformed by k_eb_load_rows), accumulated into the components the flow
-// reads in the phase order of BandSeparation.separateBands with
-// equidistant phases (BandSeparation.java:131-165, 183-197); the components
-// go back to X slots 0, 1 (, 2) = components 0, 1 (, 3) in place (a block
-// reads its tile of every phase before it writes).  Then the attenuated
-// copies of component 0 and of the fit band's component
-// (SimAlgorithm.java:84-92: duplicate, times(otfAtt)) are taken to real
-// space by their inverse column transforms here (the inverse rows follow in
-// k_eb_rows_corr) and go to W[g] and W[G + g].  NC: stored components.
+// The components' column pass fused with the first half of the coarse
+// correlation.  Per group and column tile: each component's forward column
+// transform (as the stand-alone column pass; k_eb_load_sep transformed the
+// rows), written back in place; then the attenuated copies of component 0
+// and of the fit band's component (SimAlgorithm.java:84-92: duplicate,
+// times(otfAtt)) are taken to real space by their inverse column transforms
+// here (the inverse rows follow in k_eb_rows_corr) and go to W[g] and
+// W[G + g].  NC: stored components.
 template <int L, int NC>
 __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_sep(float2 *__restrict__ X, float2 *__restrict__ W, int G,
                                                                EbSep a, const float *__restrict__ att_tab,
@@ -236,41 +262,37 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_sep(float2 *__restric
     const float2 *tws = tw_table<C::TW::tl, NT>(twl, tw);
     const int c = threadIdx.x % CW, t = threadIdx.x / CW;
     const int x = tile * CW + c;
+    const int fslot = NC == 3 && a.fb == 2 ? 2 : 1;
     float2 *in = X + (size_t)g * a.nphase * nn + x;
-    float2 acc[NC][E], nxt[E];
-#pragma unroll
-    for (int k = 0; k < NC; k++)
-#pragma unroll
-        for (int m = 0; m < E; m++) acc[k][m] = make_float2(0.f, 0.f);
+    float2 u[E], f[E], nxt[E];
 #pragma unroll
     for (int m = 0; m < E; m++) nxt[m] = in[(size_t)(t + T * m) * L];
-    for (int p = 0; p < a.nphase; p++) {
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
         float2 v[E];
 #pragma unroll
         for (int m = 0; m < E; m++) v[m] = nxt[m];
-        // the next phase's loads stay in flight across this transform
-        if (p + 1 < a.nphase)
+        // the next component's loads stay in flight across this transform
+        if (k + 1 < NC)
 #pragma unroll
-            for (int m = 0; m < E; m++) nxt[m] = in[(size_t)(p + 1) * nn + (size_t)(t + T * m) * L];
+            for (int m = 0; m < E; m++) nxt[m] = in[(size_t)(k + 1) * nn + (size_t)(t + T * m) * L];
         fft_regs<L, E, false, false, C::TW::tl>(v, lds + c, CW, t, tws);
 #pragma unroll
-        for (int k = 0; k < NC; k++)
+        for (int m = 0; m < E; m++) in[(size_t)k * nn + (size_t)(t + T * m) * L] = v[m];
+        if (k == 0)
 #pragma unroll
-            for (int m = 0; m < E; m++) acc[k][m] = sep_axpy(acc[k][m], a.S[k][p], v[m]);
+            for (int m = 0; m < E; m++) u[m] = v[m];
+        if (k == fslot)
+#pragma unroll
+            for (int m = 0; m < E; m++) f[m] = v[m];
     }
-#pragma unroll
-    for (int k = 0; k < NC; k++)
-#pragma unroll
-        for (int m = 0; m < E; m++) in[(size_t)k * nn + (size_t)(t + T * m) * L] = acc[k][m];
-    float2 u[E], f[E];
 #pragma unroll
     for (int m = 0; m < E; m++) {
         const int y = t + T * m;
         const int dx = x < L / 2 ? x : x - L, dy = y < L / 2 ? y : y - L;
         const float att = att_tab[dx * dx + dy * dy];
-        const float2 c0 = acc[0][m], cf = NC == 3 && a.fb == 2 ? acc[NC - 1][m] : acc[1][m];
-        u[m] = make_float2(c0.x * att, c0.y * att);
-        f[m] = make_float2(cf.x * att, cf.y * att);
+        u[m] = make_float2(u[m].x * att, u[m].y * att);
+        f[m] = make_float2(f[m].x * att, f[m].y * att);
     }
     fft_regs<L, E, true, false, C::TW::tl>(u, lds + c, CW, t, tws);
     fft_regs<L, E, true, false, C::TW::tl>(f, lds + c, CW, t, tws);
@@ -388,8 +410,8 @@ static int launch_front_t(float2 *X, float2 *W, int G, const EbSep &sep, const f
     return FSIM_OK;
 }
 
-// The input's column pass with the separation, then the coarse correlation
-// to its peak (SimAlgorithm.java:84-110): three launches.
+// The components' column pass, then the coarse correlation to its peak
+// (SimAlgorithm.java:84-110): three launches.
 static int launch_front(int n, float2 *X, float2 *W, int G, const EbSep &sep, const float *att_tab, double kmin,
                         EstGroup *grp, hipStream_t s) {
     switch (n) {
@@ -946,9 +968,12 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__r
         aqf += __builtin_amdgcn_sqrtf(qr * qr + qi * qi);
 #pragma unroll
         for (int j = 0; j < 10; j++) {
+            // q e accumulated by four fused multiply-adds
             const float2 e = ty_at(y * 10 + j);
-            Sf[j].x += qr * e.x - qi * e.y;
-            Sf[j].y += qr * e.y + qi * e.x;
+            Sf[j].x = fmaf(qr, e.x, Sf[j].x);
+            Sf[j].x = fmaf(-qi, e.y, Sf[j].x);
+            Sf[j].y = fmaf(qr, e.y, Sf[j].y);
+            Sf[j].y = fmaf(qi, e.x, Sf[j].y);
         }
     }
     // the table's last reads come before the row sums overwrite the region
@@ -1015,10 +1040,11 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__r
 }
 
 // Distance of the approximate sums from the exact ones, relative to sum|q|,
-// from their float part (u = 2^-24): q = b1 conj(b0) in float (<= 3 u |q|),
-// the float phasor (<= u), the product (<= 3 u), a thread's E <= 16 terms
-// summed in float (<= 15 u); the rest is double.  32 u leaves margin.
-constexpr double kApxArith = 32 * 0x1p-24;
+// from their float part (u = 2^-24): q = b1 conj(b0) in float (<= 2 sqrt 2 u
+// |q|), the float phasor (<= u), a thread's 2 E <= 32 fused multiply-adds per
+// part, each rounding the running sum of terms bounded by sqrt 2 |q|
+// (<= 2 sqrt 2 E u |q|); the rest is double.  Under 50 u for E = 16; 64 u.
+constexpr double kApxArith = 64 * 0x1p-24;
 
 // Candidates that can hold the maximum (see the top of this file); all 100
 // when the bound cannot separate (no positive threshold, or NaN).
@@ -1448,9 +1474,9 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     hipLaunchKernelGGL(k_eb_dc_tab, dim3((unsigned)std::min<size_t>((ndc * nn + 255) / 256, 4096)), dim3(256), 0, s,
                        dct, n, reg, ndc);
     hipLaunchKernelGGL(k_eb_runs, dim3(ndc), dim3(256), 0, s, reg, n, 0.05, runs);
-    rc = launch_load_rows(n, images, g_eb.X, G * P * n, fade, fade + n, live, s);
+    rc = launch_load_sep(n, images, g_eb.X, G, sep, fade, fade + n, live, s);
     if (rc) return rc;
-    // column pass with the separation, the attenuated copies' inverse
+    // the components' column pass and the attenuated copies' inverse
     // columns; their inverse rows, product and forward rows; the forward
     // columns with locatePeak
     const int nd2 = 2 * (n / 2) * (n / 2) + 1;

@@ -11,7 +11,7 @@ MTool.java:34-59; float products; double sum) by
 
 and evaluates exactly only the candidates with |A| >= max|A| - 2 D, where
 its own A carries a float part (per-thread sums of q * TY_j(y) in single
-precision) that adds at most 32 * 2^-24 * sum|q| to D.  Here
+precision) that adds at most 64 * 2^-24 * sum|q| to D.  Here
 the unscaled bound (without the 1.5) is checked against every candidate of
 three fitPeak-like grids on a commonRegion'd, inverse-transformed band pair
 with a known shift, and the selected set must contain the reference's
@@ -28,7 +28,7 @@ import pytest
 from oracle import ref
 
 NA, LAM, A_OTF, PX_UM = 1.4, 525.0, 1.0, 0.08
-APX_ARITH = 32 * 2.0 ** -24   # kApxArith: the float part of the approximate sums
+APX_ARITH = 64 * 2.0 ** -24   # kApxArith: the float part of the approximate sums
 
 
 def _pair(n, kx, ky, seed):
@@ -73,8 +73,9 @@ def test_candidate_filter_bound(n, kx, ky, seed):
 def _float_part_sums(b0, b1, n, xpos0, ypos, step, T):
     """The approximate sums as the column kernel forms them: per column x,
     thread t adds q(x, y) * TY_j(y) for y = t, t + T, ... in float32
-    (q = b1 conj(b0) in float32, TY_j rounded from double), the threads'
-    sums are added in double, then sum_x TX_i(x) C_j(x) in double."""
+    (q = b1 conj(b0) in float32, TY_j rounded from double; each product
+    term added with one rounding, as the kernel's fused multiply-adds), the
+    threads' sums are added in double, then sum_x TX_i(x) C_j(x) in double."""
     f = np.float32
     ux, uy = b0[..., 0], b0[..., 1]
     vx, vy = b1[..., 0], b1[..., 1]
@@ -88,14 +89,19 @@ def _float_part_sums(b0, b1, n, xpos0, ypos, step, T):
         u -= np.floor(u)
         ex = np.cos(2 * np.pi * u).astype(f)[:, None]
         ey = (-np.sin(2 * np.pi * u)).astype(f)[:, None]
-        tr = (qr * ex).astype(f) - (qi * ey).astype(f)
-        ti = (qr * ey).astype(f) + (qi * ex).astype(f)
-        # rows y = t + T m of a column: thread t's float sum over m, in order
+        d = np.float64
+        a1, a2 = qr.astype(d) * ex, -(qi.astype(d) * ey)   # exact products
+        b1, b2 = qr.astype(d) * ey, qi.astype(d) * ex
+        # rows y = t + T m of a column: thread t's float sum over m, in order,
+        # each product added exactly and the sum rounded (fmaf)
         sr = np.zeros((T, n), f)
         si = np.zeros((T, n), f)
         for m in range(n // T):
-            sr = (sr + tr[m * T:(m + 1) * T]).astype(f)
-            si = (si + ti[m * T:(m + 1) * T]).astype(f)
+            k = slice(m * T, (m + 1) * T)
+            sr = (sr.astype(d) + a1[k]).astype(f)
+            sr = (sr.astype(d) + a2[k]).astype(f)
+            si = (si.astype(d) + b1[k]).astype(f)
+            si = (si.astype(d) + b2[k]).astype(f)
         C[j] = sr.astype(np.float64).sum(0) + 1j * si.astype(np.float64).sum(0)
     x = np.arange(n)
     for j in range(10):
@@ -107,7 +113,7 @@ def _float_part_sums(b0, b1, n, xpos0, ypos, step, T):
 @pytest.mark.parametrize("n,kx,ky,seed", [(128, 40.37, -21.62, 3), (128, -47.3, 8.9, 5)])
 def test_candidate_filter_float_part(n, kx, ky, seed):
     """The float part of the kernel's approximate sums stays within
-    32 * 2^-24 * sum|q| of the exact candidate sums (T = 16 threads per
+    64 * 2^-24 * sum|q| of the exact candidate sums (T = 16 threads per
     column at n = 128, eight terms each); a wrong phasor sign, axis or
     table row misses by orders of magnitude."""
     b0, b1 = _pair(n, kx, ky, seed)
