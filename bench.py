@@ -545,22 +545,27 @@ def estimation(runs=3, batch=64):
 
 
 def estimation_batched(nphase, nband, fb, k_frac, h2, batch=64, runs=3):
-    """Estimation plus reconstruction as a throughput: `batch` 512^2 stacks per call (4 distinct
-    seeded stacks repeated), float32 images and uint16 raw frames resident on
-    the device; fsim_estimate for all of them, then fsim_reconstruct_params
-    with every stack's own estimate.  Timed: the whole sequence including the
-    host step that turns estimates into per-stack parameters."""
+    """Estimation plus reconstruction as a throughput: `batch` distinct 512^2 stacks per call
+    (4 seeded samples, each imaged batch / 4 times with independent seeded
+    photon noise; stack s is sample s % 4), float32 images and uint16 raw
+    frames resident on the device; fsim_estimate for all of them, then
+    fsim_reconstruct_params with every stack's own estimate.  Timed: the
+    whole sequence including the host step that turns estimates into
+    per-stack parameters."""
     import numpy as np
     from fsim import synthetic
     from fsim.linalg import DeviceBuffer
     from fsim.sim_algorithm import Reconstructor, estimate_batch, params_from_estimates
     n = 512
+    tp = batch // 4
     raws, sps = [], []
     for s in range(4):
-        raw, sp = synthetic.sim_stack(n=n, nphase=nphase, seed=11 + s, harmonic2=h2, k_frac=k_frac)
+        raw, sp = synthetic.sim_stack(n=n, nphase=nphase, seed=11 + s, harmonic2=h2, k_frac=k_frac, timepoints=tp)
         raws.append(raw)
         sps.append(sp)
-    raw_b = np.ascontiguousarray(np.concatenate([np.stack(raws)] * (batch // 4)))
+    # stack k = timepoint k // 4 of sample k % 4
+    raw_b = np.ascontiguousarray(np.stack(raws, 1).reshape(batch, *raws[0].shape[1:]))
+    raws = [r[0] for r in raws]
     img_b = raw_b.reshape(batch, 3, nphase, n, n).astype(np.float32)
     param = synthetic.make_param(sps[0], n, nphase=nphase, nband=nband)
     rec = Reconstructor(param)

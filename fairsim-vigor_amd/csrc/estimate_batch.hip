@@ -20,10 +20,12 @@
 // about 60 operations per pixel for all 100 candidates, fused with the
 // inverse column transform that produces the pixels.  The reference value
 // J(c) differs from the computed A(c) by at most
-//   D = sum|q| * (2^-24 * phi_max + 2e-6 + 64 * 2^-24) * 1.5
-// (2^-24 |phi|: rounding of the phase to float; 2e-6 covers the polynomial
-// and the reference's float products with margin; 64 * 2^-24 the float part
-// of the approximate sums, kApxArith), so a candidate can
+//   D = (sum |q| 2^-24 phi_max(x, y) + sum|q| (2e-6 + 64 * 2^-24)) * 1.5
+// (2^-24 |phi|: rounding of the phase to float, with the pixel's largest
+// phase magnitude over the grid phi_max(x, y) = 2 pi (ax x + ay y) / n,
+// ax = |kx| + search, ay = |ky| + search; 2e-6 covers the polynomial and the
+// reference's float products with margin; 64 * 2^-24 the float part of the
+// approximate sums, kApxArith), so a candidate can
 // hold the maximum of |J| only if |A(c)| >= max|A| - 2D.  Those candidates
 // (typically a handful) are then evaluated in the reference's arithmetic and
 // the first maximum in y-outer, x-inner order is taken among them, which is
@@ -83,13 +85,37 @@ struct EbRegion {
     double cpm, cutoff, vec_cpm;
 };
 
-constexpr int kNormSlot = kMaxCand;            // partial slot of |b0|^2
-constexpr int kPairSlots = kMaxCand + 1;
+constexpr int kNormSlot = kMaxCand;            // partial slot of |b0|^2 (exact), sum |q| (filter)
+constexpr int kWqSlot = kMaxCand + 1;          // filter: sum |q| (ax x + ay y), the phase bound's weight
+constexpr int kPairSlots = kMaxCand + 2;
 
 __global__ void k_eb_init(EstGroup *__restrict__ g, int G) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < G) g[i].key = 0ull;
 }
+
+// Row transforms of the estimation's row kernels.  REG: each thread's
+// twiddles in registers, loaded once from the exact-rounded table (tw,
+// length L) where they fit (L <= 512) -- worth it for a kernel that runs
+// several transforms per thread; else sincospi of the first twiddle of
+// each butterfly and its powers (one transform per thread: the register
+// loads would cost more L2 traffic than they save).
+template <int L, bool REG>
+struct EbRowTw {
+    static constexpr int E = FftE<L>::value;
+    static constexpr bool reg = REG && tw_reg_fits<L, E>();
+    TwReg<L, E> r;
+    __device__ __forceinline__ EbRowTw(const float2 *tw, int t) {
+        if constexpr (reg) tw_reg_load<L, E, L>(r, t, glb(tw));
+    }
+    template <bool INV>
+    __device__ __forceinline__ void fft(float2 (&v)[E], float2 *lds, int t, bool active) const {
+        if constexpr (reg)
+            fft_regs_r<L, E, INV, RowWave<L>::value>(v, lds, 1, t, r, active);
+        else
+            fft_regs<L, E, INV, RowWave<L>::value, 0>(v, lds, 1, t, nullptr, active);
+    }
+};
 
 // Input preparation of every frame (sim_gui/ImageControl.java:819-840) and
 // the band separation, fused with the forward row transform.  Per row of a
@@ -97,7 +123,7 @@ __global__ void k_eb_init(EstGroup *__restrict__ g, int G) {
 // `float *= double` rounds once, SimUtils.java:162-185), the components the
 // flow reads accumulated in BandSeparation.separateBands' phase order
 // (BandSeparation.java:131-165, 183-197; S complex, the frames real), then
-// each component's row transform (the stand-alone row pass's code).  The
+// each component's row transform.  The
 // separation is linear and moves in front of the transform, as in the
 // fused reconstruction: the components equal the reference's up to float
 // rounding, and NC transforms per row remain instead of P.  X slots 0, 1
@@ -110,17 +136,17 @@ __global__ void k_eb_init(EstGroup *__restrict__ g, int G) {
 template <int L, bool LIVE, int NC>
 __global__ void __launch_bounds__(256) k_eb_load_sep(const float *__restrict__ img, float2 *__restrict__ X, int G,
                                                      EbSep a, const double *__restrict__ lo,
-                                                     const double *__restrict__ hi) {
+                                                     const double *__restrict__ hi, const float2 *__restrict__ tw) {
     constexpr int E = FftE<L>::value, T = L / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int LP = FftLds<L, E>::len;
     constexpr size_t nn = (size_t)L * L;
     __shared__ float2 lds[RW * LP];
-    using TW = TwLds<L, false>;
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int row = blockIdx.x * RW + rr;   // row of G groups of L rows
     const bool active = row < G * L;
     const int g = row / L, y = row % L;
+    const EbRowTw<L, false> rtw(tw, t);
     float2 acc[NC][E];
 #pragma unroll
     for (int k = 0; k < NC; k++)
@@ -128,33 +154,38 @@ __global__ void __launch_bounds__(256) k_eb_load_sep(const float *__restrict__ i
         for (int m = 0; m < E; m++) acc[k][m] = make_float2(0.f, 0.f);
     if (active) {
         const float *src = img + (size_t)g * a.nphase * nn + (size_t)y * L;
-        // every phase's pixels in flight before the first use
-        float d[kMaxPhase][E];
-#pragma unroll
-        for (int p = 0; p < kMaxPhase; p++)
-            if (p < a.nphase)
-#pragma unroll
-                for (int m = 0; m < E; m++) d[p][m] = src[(size_t)p * nn + t + T * m];
         const double ly = lo[y], hy = hi[y];
+        double lx[E], hx[E];
 #pragma unroll
-        for (int p = 0; p < kMaxPhase; p++) {
-            if (p >= a.nphase) break;
+        for (int m = 0; m < E; m++) {
+            lx[m] = lo[t + T * m];
+            hx[m] = hi[t + T * m];
+        }
+        // the next phase's pixels are in flight while this one is used
+        float d[E], nxt[E];
+#pragma unroll
+        for (int m = 0; m < E; m++) nxt[m] = src[t + T * m];
+        for (int p = 0; p < a.nphase; p++) {
+#pragma unroll
+            for (int m = 0; m < E; m++) d[m] = nxt[m];
+            if (p + 1 < a.nphase)
+#pragma unroll
+                for (int m = 0; m < E; m++) nxt[m] = src[(size_t)(p + 1) * nn + t + T * m];
 #pragma unroll
             for (int m = 0; m < E; m++) {
-                const int x = t + T * m;
                 float e;
                 if constexpr (LIVE) {
                     float b = 1.0f;
                     b = (float)((double)b * ly);
                     b = (float)((double)b * hy);
-                    b = (float)((double)b * lo[x]);
-                    b = (float)((double)b * hi[x]);
-                    e = d[p][m] * b;
+                    b = (float)((double)b * lx[m]);
+                    b = (float)((double)b * hx[m]);
+                    e = d[m] * b;
                 } else {
-                    e = (float)((double)d[p][m] * ly);
+                    e = (float)((double)d[m] * ly);
                     e = (float)((double)e * hy);
-                    e = (float)((double)e * lo[x]);
-                    e = (float)((double)e * hi[x]);
+                    e = (float)((double)e * lx[m]);
+                    e = (float)((double)e * hx[m]);
                 }
 #pragma unroll
                 for (int k = 0; k < NC; k++) acc[k][m] = sep_axpy(acc[k][m], a.S[k][p], make_float2(e, 0.f));
@@ -163,7 +194,7 @@ __global__ void __launch_bounds__(256) k_eb_load_sep(const float *__restrict__ i
     }
 #pragma unroll
     for (int k = 0; k < NC; k++) {
-        fft_regs<L, E, false, RowWave<L>::value, TW::tl>(acc[k], lds + rr * LP, 1, t, nullptr, active);
+        rtw.template fft<false>(acc[k], lds + rr * LP, t, active);
         if (active) {
             float2 *dst = X + ((size_t)g * a.nphase + k) * nn + (size_t)y * L;
 #pragma unroll
@@ -178,15 +209,17 @@ static void launch_load_sep_tn(const float *img, float2 *X, int G, const EbSep &
     constexpr int E = FftE<L>::value, T = L / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     const dim3 grid(cdiv((size_t)G * L, RW)), block(T * RW);
+    const float2 *tw = twiddles(L);
     if (live)
-        hipLaunchKernelGGL((k_eb_load_sep<L, true, NC>), grid, block, 0, s, img, X, G, sep, lo, hi);
+        hipLaunchKernelGGL((k_eb_load_sep<L, true, NC>), grid, block, 0, s, img, X, G, sep, lo, hi, tw);
     else
-        hipLaunchKernelGGL((k_eb_load_sep<L, false, NC>), grid, block, 0, s, img, X, G, sep, lo, hi);
+        hipLaunchKernelGGL((k_eb_load_sep<L, false, NC>), grid, block, 0, s, img, X, G, sep, lo, hi, tw);
 }
 
 template <int L>
 static int launch_load_sep_t(const float *img, float2 *X, int G, const EbSep &sep, const double *lo, const double *hi,
                              bool live, hipStream_t s) {
+    if (!twiddles(L)) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
     if (sep.ncomp == 3)
         launch_load_sep_tn<L, 3>(img, X, G, sep, lo, hi, live, s);
     else
@@ -306,18 +339,17 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_sep(float2 *__restric
 }
 
 // The coarse correlation's row pass: the inverse row transforms of both
-// attenuated copies (their columns were transformed by k_eb_cols_sep; the
-// stand-alone row pass's transform and 1 / L scale), c_fb.timesConj(c0)
+// attenuated copies (their columns were transformed by k_eb_cols_sep; scaled
+// by 1 / L as the stand-alone row pass scales), c_fb.timesConj(c0)
 // (AbstractVectorCplx.java:288-304), and the forward row transform of the
 // product (SimAlgorithm.java:94-101), written over c_fb.  One row per
 // T = L / E threads; grid (ceil(G L / RW)).
 template <int L>
-__global__ void __launch_bounds__(256) k_eb_rows_corr(float2 *__restrict__ W, int G) {
+__global__ void __launch_bounds__(256) k_eb_rows_corr(float2 *__restrict__ W, int G, const float2 *__restrict__ tw) {
     constexpr int E = FftE<L>::value, T = L / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int LP = FftLds<L, E>::len;
     __shared__ float2 lds[RW * LP];
-    using TW = TwLds<L, false>;   // the row pass's twiddle source
     constexpr size_t nn = (size_t)L * L;
     const int t = threadIdx.x % T, rr = threadIdx.x / T;
     const int row = blockIdx.x * RW + rr;   // row of G images of L rows
@@ -331,12 +363,13 @@ __global__ void __launch_bounds__(256) k_eb_rows_corr(float2 *__restrict__ W, in
             u[m] = p1[t + T * m];
         }
     }
-    fft_regs<L, E, true, RowWave<L>::value, TW::tl>(v, lds + rr * LP, 1, t, nullptr, active);
-    fft_regs<L, E, true, RowWave<L>::value, TW::tl>(u, lds + rr * LP, 1, t, nullptr, active);
+    const EbRowTw<L, true> rtw(tw, t);
+    rtw.template fft<true>(v, lds + rr * LP, t, active);
+    rtw.template fft<true>(u, lds + rr * LP, t, active);
     constexpr float sc = 1.0f / L;
 #pragma unroll
     for (int m = 0; m < E; m++) u[m] = times_conj_java(cscale(u[m], sc), cscale(v[m], sc));
-    fft_regs<L, E, false, RowWave<L>::value, TW::tl>(u, lds + rr * LP, 1, t, nullptr, active);
+    rtw.template fft<false>(u, lds + rr * LP, t, active);
     if (active) {
 #pragma unroll
         for (int m = 0; m < E; m++) p1[t + T * m] = u[m];
@@ -404,7 +437,7 @@ static int launch_front_t(float2 *X, float2 *W, int G, const EbSep &sep, const f
         hipLaunchKernelGGL((k_eb_cols_sep<L, 2>), dim3(gc), dim3(C::NT), 0, s, X, W, G, sep, att_tab, tw);
     constexpr int E = FftE<L>::value, T = L / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
-    hipLaunchKernelGGL(k_eb_rows_corr<L>, dim3(cdiv((size_t)G * L, RW)), dim3(T * RW), 0, s, W, G);
+    hipLaunchKernelGGL(k_eb_rows_corr<L>, dim3(cdiv((size_t)G * L, RW)), dim3(T * RW), 0, s, W, G, tw);
     hipLaunchKernelGGL(k_eb_cols_peak<L>, dim3(gc), dim3(C::NT), 0, s, W + (size_t)G * L * L, G, kmin, grp, tw);
     FSIM_HIP(hipGetLastError());
     return FSIM_OK;
@@ -670,8 +703,9 @@ __device__ __forceinline__ float2 region_pass(float2 v, float2 dc) {
 
 // Correlation.commonRegion(b0, b1, 0, bn1, otf, kx, ky, 0.15, 0.05, true)
 // on copies of each pair's bands (Correlation.java:60-63, 189-192), fused
-// with the row pass of the inverse 2-D FFT that follows (the same row
-// transform as fft2d's, which runs rows first).  Grid (ceil(n / RW), 2 * npairs):
+// with the row pass of the inverse 2-D FFT that follows (fft2d runs rows
+// first).  Grid
+// (ceil(n / RW), 2 * npairs):
 // blockIdx.y = 2 * pair + band; band 0 rows go to T[pair], band 1 rows to
 // T[npairs + pair].
 // RUNS: the shifted weight's test through the run tables (k_eb_runs);
@@ -686,7 +720,6 @@ __global__ void __launch_bounds__(256) k_eb_region_rows(const float2 *__restrict
     constexpr int RW = Tn >= 256 ? 1 : 256 / Tn;
     constexpr int LP = FftLds<L, E>::len;
     __shared__ float2 lds[RW * LP];
-    using TW = TwLds<L, false>;
     constexpr size_t nn = (size_t)L * L;
     const int t = threadIdx.x % Tn, rr = threadIdx.x / Tn;
     const int row = blockIdx.x * RW + rr;
@@ -756,7 +789,8 @@ __global__ void __launch_bounds__(256) k_eb_region_rows(const float2 *__restrict
             v[m] = out;
         }
     }
-    fft_regs<L, E, true, RowWave<L>::value, TW::tl>(v, lds + rr * LP, 1, t, tw, active);
+    const EbRowTw<L, false> rtw(tw, t);
+    rtw.template fft<true>(v, lds + rr * LP, t, active);
     if (active) {
         float2 *dst = T + (size_t)(band ? npairs + p : p) * nn + (size_t)row * L;
         constexpr float scale = 1.0f / L;
@@ -899,9 +933,11 @@ __global__ void k_eb_apx_tab(const EstPair *__restrict__ pairs, int n, float2 *_
 // Nothing is written back: the exact evaluation transforms the columns
 // again.  Grid xcd_grid(npairs * NTILE): neighbouring tiles of a pair run
 // on one XCD and share the cache lines of their rows.
-// apart[(pair * kPairSlots + c) * NTILE + tile], sum |q| in slot kNormSlot.
+// apart[(pair * kPairSlots + c) * NTILE + tile], sum |q| in slot kNormSlot,
+// sum |q| (ax x + ay y) in slot kWqSlot.
 template <int L>
 __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__restrict__ W, int npairs,
+                                                               const EstPair *__restrict__ pairs,
                                                                const float2 *__restrict__ tabs,
                                                                const float2 *__restrict__ tw,
                                                                double2 *__restrict__ apart) {
@@ -919,10 +955,13 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__r
     __shared__ double2 red_own[red_in_lds ? 1 : NW * CW * 10];
     double2 *red = red_in_lds ? reinterpret_cast<double2 *>(lds) : red_own;
     __shared__ double2 col[CW][10], txi[CW][10];
-    __shared__ double ared[NW];
+    __shared__ double2 ared[NW];
     const int w = xcd_linear(npairs * NTILE);
     if (w >= npairs * NTILE) return;   // block-uniform, before any barrier
     const int p = w / NTILE, tile = w % NTILE;
+    // the phase bound's weights: |xpos| <= ax, |ypos| <= ay on the grid
+    const float ax = (float)(fabs(pairs[p].kx) + pairs[p].search) * (1 + 0x1p-20f);
+    const float ay = (float)(fabs(pairs[p].ky) + pairs[p].search) * (1 + 0x1p-20f);
     const float2 *TY = tabs + (size_t)p * kApxTab * L;
     const double2 *TXB = apx_txb(tabs, p, L), *TXS = TXB + L;
     // the table's loads are issued before the transforms and held in registers
@@ -957,15 +996,18 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__r
     float2 Sf[10];
 #pragma unroll
     for (int j = 0; j < 10; j++) Sf[j] = make_float2(0.f, 0.f);
-    float aqf = 0.f;
+    float aqf = 0.f, wqf = 0.f;
+    const float bx = ax * (float)(tile * CW + c);
 #pragma unroll
     for (int m = 0; m < E; m++) {
         const int y = t + T * m;
         const float2 u = a[m], v = b[m];
         const float qr = v.x * u.x + v.y * u.y;
         const float qi = v.y * u.x - v.x * u.y;
-        // |q| for the bound's sum |q|: a few ulps, far inside its factor 1.5
-        aqf += __builtin_amdgcn_sqrtf(qr * qr + qi * qi);
+        // |q| for the bound's sums: a few ulps, far inside its factor 1.5
+        const float aqp = __builtin_amdgcn_sqrtf(qr * qr + qi * qi);
+        aqf += aqp;
+        wqf = fmaf(aqp, fmaf(ay, (float)y, bx), wqf);
 #pragma unroll
         for (int j = 0; j < 10; j++) {
             // q e accumulated by four fused multiply-adds
@@ -981,7 +1023,7 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__r
     double2 S[10];
 #pragma unroll
     for (int j = 0; j < 10; j++) S[j] = make_double2((double)Sf[j].x, (double)Sf[j].y);
-    double aq = (double)aqf;
+    double aq = (double)aqf, wq = (double)wqf;
     // sum over the rows: the lanes of one column within the wavefront
     // (lane = t' CW + c), then over the wavefronts
 #pragma unroll
@@ -994,12 +1036,15 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__r
             }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
-        if (o < NT) aq += __shfl_down(aq, o, 64);
+        if (o < NT) {
+            aq += __shfl_down(aq, o, 64);
+            wq += __shfl_down(wq, o, 64);
+        }
     const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
     if (lane < CW)
 #pragma unroll
         for (int j = 0; j < 10; j++) red[(wv * CW + lane) * 10 + j] = S[j];
-    if (lane == 0) ared[wv] = aq;
+    if (lane == 0) ared[wv] = make_double2(aq, wq);
     __syncthreads();
     for (int k = threadIdx.x; k < CW * 10; k += NT) {
         const int cc = k / 10, j = k % 10;
@@ -1033,9 +1078,13 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__r
         out[(size_t)k * NTILE + tile] = s;
     }
     if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int i = 0; i < NW; i++) s += ared[i];
+        double s = 0.0, sw = 0.0;
+        for (int i = 0; i < NW; i++) {
+            s += ared[i].x;
+            sw += ared[i].y;
+        }
         out[(size_t)kNormSlot * NTILE + tile] = make_double2(s, 0.0);
+        out[(size_t)kWqSlot * NTILE + tile] = make_double2(sw, 0.0);
     }
 }
 
@@ -1049,7 +1098,7 @@ constexpr double kApxArith = 64 * 0x1p-24;
 // Candidates that can hold the maximum (see the top of this file); all 100
 // when the bound cannot separate (no positive threshold, or NaN).
 __global__ void __launch_bounds__(128) k_eb_select(EstPair *__restrict__ pairs, const double2 *__restrict__ apart,
-                                                   int nblk) {
+                                                   int nblk, int n) {
     __shared__ double mag[kPairSlots];
     const int p = blockIdx.x, t = threadIdx.x;
     const double2 *in = apart + (size_t)p * kPairSlots * nblk;
@@ -1059,7 +1108,7 @@ __global__ void __launch_bounds__(128) k_eb_select(EstPair *__restrict__ pairs, 
             s.x += in[(size_t)t * nblk + b].x;
             s.y += in[(size_t)t * nblk + b].y;
         }
-        mag[t] = t == kNormSlot ? s.x : hypot(s.x, s.y);
+        mag[t] = t >= kNormSlot ? s.x : hypot(s.x, s.y);
     }
     __syncthreads();
     if (t != 0) return;
@@ -1069,8 +1118,9 @@ __global__ void __launch_bounds__(128) k_eb_select(EstPair *__restrict__ pairs, 
         if (!(mag[c] >= 0.0)) bad = true;
         if (mag[c] > M) M = mag[c];
     }
-    const double phimax = 2 * M_PI * (fabs(pairs[p].kx) + fabs(pairs[p].ky) + 2 * pairs[p].search);
-    const double delta = 1.5 * mag[kNormSlot] * (phimax * 0x1p-24 * (1 + 1e-6) + 2e-6 + kApxArith);
+    // the phase of pixel (x, y) is at most 2 pi (ax x + ay y) / n in magnitude
+    const double phisum = 2 * M_PI * mag[kWqSlot] / n;
+    const double delta = 1.5 * (phisum * 0x1p-24 * (1 + 1e-6) + mag[kNormSlot] * (2e-6 + kApxArith));
     const double thr = M - 2 * delta - 1e-12 * M;
     const bool all = bad || !(thr > 0.0) || !(delta == delta);
     int ns = 0;
@@ -1163,17 +1213,19 @@ static int eb_ntile(int n) {
 }
 
 template <int L>
-static int launch_cols_apx_t(const float2 *W, int npairs, const float2 *tabs, double2 *apart, hipStream_t s) {
+static int launch_cols_apx_t(const float2 *W, int npairs, const EstPair *pairs, const float2 *tabs, double2 *apart,
+                             hipStream_t s) {
     using C = EbCols<L>;
     const float2 *tw = twiddles(L);
     if (!tw) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
-    hipLaunchKernelGGL(k_eb_cols_apx<L>, dim3(xcd_grid(npairs * C::NTILE)), dim3(C::NT), 0, s, W, npairs, tabs, tw,
-                       apart);
+    hipLaunchKernelGGL(k_eb_cols_apx<L>, dim3(xcd_grid(npairs * C::NTILE)), dim3(C::NT), 0, s, W, npairs, pairs, tabs,
+                       tw, apart);
     FSIM_HIP(hipGetLastError());
     return FSIM_OK;
 }
-static int launch_cols_apx(int n, const float2 *W, int npairs, const float2 *tabs, double2 *apart, hipStream_t s) {
-    FSIM_EB_SWITCH(n, launch_cols_apx_t, W, npairs, tabs, apart, s)
+static int launch_cols_apx(int n, const float2 *W, int npairs, const EstPair *pairs, const float2 *tabs,
+                           double2 *apart, hipStream_t s) {
+    FSIM_EB_SWITCH(n, launch_cols_apx_t, W, npairs, pairs, tabs, apart, s)
 }
 
 template <int L>
@@ -1365,9 +1417,9 @@ static int eb_round(const EbRegion &r, const float2 *dct, const EbRuns *runs, Es
     if (rc) return rc;
     if (grid) {
         hipLaunchKernelGGL(k_eb_apx_tab, dim3(cdiv((size_t)12 * n, 256), npairs), dim3(256), 0, s, pairs, n, tabs);
-        rc = launch_cols_apx(n, g_eb.W, npairs, tabs, apart, s);
+        rc = launch_cols_apx(n, g_eb.W, npairs, pairs, tabs, apart, s);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_eb_select, dim3(npairs), dim3(128), 0, s, pairs, apart, ntile);
+        hipLaunchKernelGGL(k_eb_select, dim3(npairs), dim3(128), 0, s, pairs, apart, ntile, n);
         FSIM_HIP(hipGetLastError());
     }
     rc = launch_cols_corr(n, g_eb.W, npairs, pairs, part, s);

@@ -7,7 +7,10 @@ computes (Correlation.fitPeak -> Vec2d.Cplx.fourierShift with the phase
 rounded to float and MTool's sine polynomial, BasicVectors.java:309-339,
 MTool.java:34-59; float products; double sum) by
 
-    D = 1.5 * sum|q| * (2^-24 * 2 pi (|kx| + |ky| + 2 search) + 2e-6),
+    D = 1.5 * (2^-24 * 2 pi sum_xy |q| (ax x + ay y) / n + 2e-6 sum|q|),
+    ax = |kx| + search, ay = |ky| + search (the phase of pixel (x, y) is at
+    most 2 pi (ax x + ay y) / n over the grid; the older, looser form used
+    2 pi (|kx| + |ky| + 2 search) for every pixel),
 
 and evaluates exactly only the candidates with |A| >= max|A| - 2 D, where
 its own A carries a float part (per-thread sums of q * TY_j(y) in single
@@ -57,10 +60,16 @@ def test_candidate_filter_bound(n, kx, ky, seed):
         A = np.array([(np.exp(2j * np.pi * xp * x / n)[None, :] * np.exp(-2j * np.pi * yp * x / n)[:, None] * q).sum()
                       for xp, yp in pos])
         D = sum_abs * (2.0 ** -24 * 2 * math.pi * (abs(k0x) + abs(k0y) + 2 * search) + 2e-6)
+        # the kernel's form: each pixel's largest phase over the grid,
+        # 2 pi (ax x + ay y) / n, instead of the largest over the image
+        ax, ay = abs(k0x) + search, abs(k0y) + search
+        wq = (np.abs(q) * (ax * x[None, :] + ay * x[:, None])).sum()
+        Dp = 2.0 ** -24 * 2 * math.pi * wq / n + 2e-6 * sum_abs
         err = np.abs(J - A).max()
         print(f"search {search}: max|J-A| = {err:.3e}, bound {D:.3e} ({err / D:.2f}), "
-              f"max|J| = {np.abs(J).max():.3e}")
-        assert err <= D
+              f"per-pixel bound {Dp:.3e} ({err / Dp:.2f}), max|J| = {np.abs(J).max():.3e}")
+        assert err <= Dp <= D
+        D = Dp
         # the selected set holds the reference's first maximum (y-outer, x-inner)
         thr = np.abs(A).max() - 2 * 1.5 * (D + APX_ARITH * sum_abs)
         sel = np.nonzero(np.abs(A) >= thr)[0]
