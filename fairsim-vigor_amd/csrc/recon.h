@@ -167,6 +167,16 @@ struct MultiWork {
 };
 void multiwork_release(MultiWork &w);
 size_t multiwork_bytes(int n, int count, int nsets);
+// Chunked execution of reconstruct_params_with: the stacks run as chunks of
+// k on ns library streams (chunk_streams_locked, the chunk mutex held),
+// chunk c on stream c % ns with that stream's work buffers a, b of k stacks.
+struct ParamsChunks {
+    int k = 0, ns = 0;
+    float2 *a[kMaxChunkStreams] = {}, *b[kMaxChunkStreams] = {};
+    PipeCfg cfg;
+};
+// pc: nullptr for one launch with the work buffers A and B of count stacks.
 int reconstruct_params_with(const fsim_channel_s *ch, MultiWork &w, float2 *A, float2 *B, const uint16_t *raw, float *out,
-                            int count, const fsim_sim_params *sp, int sp_step, hipStream_t s);
+                            int count, const fsim_sim_params *sp, int sp_step, hipStream_t s,
+                            const ParamsChunks *pc = nullptr);
 }  // namespace fsim

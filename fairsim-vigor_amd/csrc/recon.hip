@@ -1907,10 +1907,15 @@ int launch_chunked(int n, const JobList &jl, int count, int D, int P, int Cs, bo
     for (int c = 0, first = 0; first < count; c++, first += K) {
         const int i = c % S, kc = std::min(K, count - first);
         JobList cj = jl;
-        cj.base.raw += (size_t)first * jl.raw_stride;
-        cj.base.out += (size_t)first * jl.out_stride;
-        cj.base.A = ws_a[i];
-        cj.base.B = ws_b[i];
+        if (jl.jobs) {
+            // a device job list: its entries carry their chunk's work buffers
+            cj.jobs = jl.jobs + first;
+        } else {
+            cj.base.raw += (size_t)first * jl.raw_stride;
+            cj.base.out += (size_t)first * jl.out_stride;
+            cj.base.A = ws_a[i];
+            cj.base.B = ws_b[i];
+        }
         const int rc = launch_pipeline_cfg(n, cj, kc, D, P, Cs, half, cfg, g_chunk.s[i]);
         if (rc) return rc;
     }
@@ -2145,7 +2150,7 @@ size_t multiwork_bytes(int n, int count, int nsets) {
 // timepoints of one fit share them).  A and B: work buffers for `count`
 // stacks; w: the state that grows with the sets and jobs.
 int reconstruct_params_with(const fsim_channel_s *ch, MultiWork &w, float2 *A, float2 *B, const uint16_t *raw, float *out,
-                            int count, const fsim_sim_params *sp, int sp_step, hipStream_t s) {
+                            int count, const fsim_sim_params *sp, int sp_step, hipStream_t s, const ParamsChunks *pc) {
     const ReconConst &K0 = ch->K;
     const int n = K0.n, M = 2 * n;
     const size_t mm = (size_t)M * M, rper = (size_t)kMaxDir * kMaxComp * 2 * n;
@@ -2225,8 +2230,14 @@ int reconstruct_params_with(const fsim_channel_s *ch, MultiWork &w, float2 *A, f
         Job &j = w.hjobs[st];
         j.c = w.dK + set_of[st];
         j.raw = raw + (size_t)st * K0.D * K0.P * n * n;
-        j.A = A + st * workspace_a_elems(n, K0.D, K0.Cs);
-        j.B = B + st * workspace_b_elems(n);
+        if (pc) {
+            const int c = st / pc->k, r = st - c * pc->k;
+            j.A = pc->a[c % pc->ns] + r * workspace_a_elems(n, K0.D, K0.Cs);
+            j.B = pc->b[c % pc->ns] + r * workspace_b_elems(n);
+        } else {
+            j.A = A + st * workspace_a_elems(n, K0.D, K0.Cs);
+            j.B = B + st * workspace_b_elems(n);
+        }
         j.out = out + (size_t)st * 4 * n * n;
         aligned = aligned && ((uintptr_t)j.raw & 7) == 0;
     }
@@ -2249,6 +2260,7 @@ int reconstruct_params_with(const fsim_channel_s *ch, MultiWork &w, float2 *A, f
     JobList jl{};
     jl.jobs = w.jobs;
     jl.raw_aligned = aligned;
+    if (pc) return launch_chunked(n, jl, count, K0.D, K0.P, K0.Cs, half, pc->cfg, pc->k, pc->ns, s, nullptr, nullptr);
     return launch_pipeline(n, jl, count, K0.D, K0.P, K0.Cs, half, s);
 }
 
@@ -2474,7 +2486,34 @@ int fsim_reconstruct_params(fsim_channel ch, const uint16_t *raw, float *out, in
     if (!ch || !raw || !out || !sp || count < 1) return fail(FSIM_ERR_ARG, "reconstruct_params: bad arguments");
     if (count > 65535) return fail(FSIM_ERR_ARG, "reconstruct_params: at most 65535 stacks per call");
     hipStream_t s = strm ? (hipStream_t)strm : stream();
+    const ReconConst &K = ch->K;
+    const bool half = K.half && ((uintptr_t)out & 7) == 0;
+    PipeCfg cfg;
+    if (!pipe_cfg_lookup(K.n, K.D, K.P, K.Cs, half, cfg)) cfg = PipeCfg{};
+    int kc = count, ns = 1;
+    chunk_plan(K.n, K.D, K.Cs, count, cfg, kc, ns);
     Workspace *w = nullptr;
+    if (kc < count) {
+        // chunks on the library streams as in fsim_reconstruct, each with its
+        // own work buffers; the parameter state stays with the caller's stream
+        std::lock_guard<std::mutex> lk(chunk_mutex());
+        const hipStream_t *cs = chunk_streams_locked();
+        if (!cs) return fail(FSIM_ERR_HIP, "reconstruct_params: chunk streams could not be created");
+        ParamsChunks pc;
+        pc.k = kc;
+        pc.ns = ns;
+        pc.cfg = cfg;
+        for (int i = 0; i < ns; i++) {
+            Workspace *wi = nullptr;
+            rc = ensure_workspace(ch, kc, cs[i], &wi);
+            if (rc) return rc;
+            pc.a[i] = wi->A;
+            pc.b[i] = wi->B;
+        }
+        rc = ensure_workspace(ch, 1, s, &w);
+        if (rc) return rc;
+        return reconstruct_params_with(ch, w->mw, nullptr, nullptr, raw, out, count, sp, 1, s, &pc);
+    }
     rc = ensure_workspace(ch, count, s, &w);
     if (rc) return rc;
     return reconstruct_params_with(ch, w->mw, w->A, w->B, raw, out, count, sp, 1, s);

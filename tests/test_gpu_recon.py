@@ -255,6 +255,34 @@ def test_reconstruct_params_equals_own_channel(fsim_lib):
     ob.free()
 
 
+def test_reconstruct_params_chunked_equals_own_channel(fsim_lib):
+    """At 512^2 a call of more stacks than one chunk (8) runs as chunks on
+    the library streams, each stream with its own work buffers and the
+    job list carrying them; every stack is still bitwise the image of a
+    channel built for its parameters, in a first call and a repeated one."""
+    from fsim.linalg import DeviceBuffer
+    from fsim.sim_algorithm import Reconstructor
+    from fsim._lib import call
+    n = 512
+    raws, params, _ = _multi_stacks(n, [(0.85, 3.0, 0.0, 0.7), (0.78, 11.0, 0.4, 0.55), (0.8, -7.0, -1.1, 0.9)])
+    order = [0, 1, 2, 2, 0, 1, 1, 0, 2, 0, 1]   # 11 stacks: chunks of 8 and 3
+    raw_b = np.ascontiguousarray(raws[order])
+    prm_b = [params[k] for k in order]
+    rec = Reconstructor(params[0])
+    rb = DeviceBuffer(raw_b.nbytes)
+    rb.upload(raw_b)
+    ob = DeviceBuffer(len(order) * rec.out_bytes())
+    own = [Reconstructor(prm).reconstruct_host(raws[k]) for k, prm in enumerate(params)]
+    for _ in range(2):
+        rec.reconstruct_params(rb.value, ob.value, prm_b)
+        call("fsim_sync")
+        got = ob.download((len(order), 2 * n, 2 * n), np.float32)
+        for s, k in enumerate(order):
+            assert np.array_equal(got[s], own[k]), f"stack {s}"
+    rb.free()
+    ob.free()
+
+
 def test_estimate_then_reconstruct_matches_oracle(fsim_lib):
     """Estimate-then-reconstruct flow: parameters estimated per stack (fsim_estimate), then
     every stack reconstructed with its own estimate (fsim_reconstruct_params);
