@@ -89,9 +89,25 @@ constexpr int kNormSlot = kMaxCand;            // partial slot of |b0|^2 (exact)
 constexpr int kWqSlot = kMaxCand + 1;          // filter: sum |q| (ax x + ay y), the phase bound's weight
 constexpr int kPairSlots = kMaxCand + 2;
 
-__global__ void k_eb_init(EstGroup *__restrict__ g, int G) {
+// Clears the groups' locatePeak keys, and finds locatePeak's exclusion as
+// an integer bound: sqrt((double)d2) > kmin (Correlation.java:150-152) is a
+// non-decreasing test of the integer squared radius d2, true exactly from
+// *d2min on (binary search with the same arithmetic; 2 (n/2)^2 + 1 when no
+// pixel of the image qualifies).
+__global__ void k_eb_init(EstGroup *__restrict__ g, int G, int n, double kmin, int *__restrict__ d2min) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < G) g[i].key = 0ull;
+    if (i == 0) {
+        int a = 0, b = 2 * (n / 2) * (n / 2) + 1;
+        while (a < b) {
+            const int m = a + (b - a) / 2;
+            if (sqrt((double)m) > kmin)
+                b = m;
+            else
+                a = m + 1;
+        }
+        *d2min = a;
+    }
 }
 
 // Row transforms of the estimation's row kernels.  REG: each thread's
@@ -156,10 +172,12 @@ __global__ void __launch_bounds__(256) k_eb_load_sep(const float *__restrict__ i
         const float *src = img + (size_t)g * a.nphase * nn + (size_t)y * L;
         const double ly = lo[y], hy = hi[y];
         double lx[E], hx[E];
+        bool one[E];   // the pixel lies inside the faded border
 #pragma unroll
         for (int m = 0; m < E; m++) {
             lx[m] = lo[t + T * m];
             hx[m] = hi[t + T * m];
+            one[m] = ly == 1.0 && hy == 1.0 && lx[m] == 1.0 && hx[m] == 1.0;
         }
         // the next phase's pixels are in flight while this one is used
         float d[E], nxt[E];
@@ -174,7 +192,10 @@ __global__ void __launch_bounds__(256) k_eb_load_sep(const float *__restrict__ i
 #pragma unroll
             for (int m = 0; m < E; m++) {
                 float e;
-                if constexpr (LIVE) {
+                if (one[m]) {
+                    // every factor is 1.0: each rounding returns the pixel
+                    e = d[m];
+                } else if constexpr (LIVE) {
                     float b = 1.0f;
                     b = (float)((double)b * ly);
                     b = (float)((double)b * hy);
@@ -250,11 +271,13 @@ static int launch_load_sep(int L, const float *img, float2 *X, int G, const EbSe
 // column pass's tile width, capped at 512 threads per block so that a
 // thread may hold up to 256 VGPRs, and its twiddle source, so that the
 // transforms return its values bit for bit.
-template <int L>
+// NTMAX = 1024 for kernels whose registers fit 128 VGPRs: the full tile
+// width (128-byte row segments) of the stand-alone pass.
+template <int L, int NTMAX = 512>
 struct EbCols {
     static constexpr int E = FftE<L>::value, T = L / E;
     static constexpr int CW0 = ColW<L>::value;
-    static constexpr int CW = T * CW0 <= 512 ? CW0 : (512 / T >= 1 ? 512 / T : 1);
+    static constexpr int CW = T * CW0 <= NTMAX ? CW0 : (NTMAX / T >= 1 ? NTMAX / T : 1);
     static constexpr int NT = T * CW, NW = (NT + 63) / 64;
     static constexpr int NTILE = L / CW;
     using TW = TwLds<L, (FftLds<L, E>::len * CW0 + L) * 8 <= 163840>;
@@ -281,10 +304,10 @@ __global__ void k_eb_att_tab(float *__restrict__ tab, int nd2, double vec_cpm, d
 // here (the inverse rows follow in k_eb_rows_corr) and go to W[g] and
 // W[G + g].  NC: stored components.
 template <int L, int NC>
-__global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_sep(float2 *__restrict__ X, float2 *__restrict__ W, int G,
-                                                               EbSep a, const float *__restrict__ att_tab,
-                                                               const float2 *__restrict__ tw) {
-    using C = EbCols<L>;
+__global__ void __launch_bounds__((EbCols<L, 1024>::NT)) k_eb_cols_sep(float2 *__restrict__ X, float2 *__restrict__ W,
+                                                                     int G, EbSep a, const float *__restrict__ att_tab,
+                                                                     const float2 *__restrict__ tw) {
+    using C = EbCols<L, 1024>;
     constexpr int E = C::E, T = C::T, CW = C::CW, NT = C::NT, NTILE = C::NTILE;
     constexpr size_t nn = (size_t)L * L;
     __shared__ float2 lds[C::LDS];
@@ -382,15 +405,17 @@ __global__ void __launch_bounds__(256) k_eb_rows_corr(float2 *__restrict__ W, in
 // tiles, one atomicMax per block; the correlation map itself is never
 // written.
 template <int L>
-__global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_peak(const float2 *__restrict__ C1, int G, double kmin,
-                                                                EstGroup *__restrict__ grp,
-                                                                const float2 *__restrict__ tw) {
-    using C = EbCols<L>;
+__global__ void __launch_bounds__((EbCols<L, 1024>::NT)) k_eb_cols_peak(const float2 *__restrict__ C1, int G,
+                                                                      const int *__restrict__ d2min,
+                                                                      EstGroup *__restrict__ grp,
+                                                                      const float2 *__restrict__ tw) {
+    using C = EbCols<L, 1024>;
     constexpr int E = C::E, T = C::T, CW = C::CW, NT = C::NT, NW = C::NW, NTILE = C::NTILE;
     constexpr size_t nn = (size_t)L * L;
     __shared__ float2 lds[C::LDS];
     __shared__ float2 twl[C::TW::len];
     __shared__ unsigned long long sh[NW];
+    __shared__ double shs[NW];
     const int w = xcd_linear(G * NTILE);
     if (w >= G * NTILE) return;   // block-uniform, before any barrier
     const int g = w / NTILE, tile = w % NTILE;
@@ -402,12 +427,53 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_peak(const float2 *__
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = src[(size_t)(t + T * m) * L];
     fft_regs<L, E, false, false, C::TW::tl>(v, lds + c, CW, t, tws);
-    unsigned long long k = 0;
+    // peak_key's magnitude (float)sqrt(s), s = re^2 + im^2 in double, is a
+    // non-decreasing function of s: the block's largest key has the largest
+    // s* among the pixels beyond the exclusion, so the square roots are taken
+    // only for pixels whose s lies within a float step of s* (the smallest
+    // index among those with the same magnitude wins, as in peak_key)
+    const int dm = *d2min;
+    const int dx2 = (x < L / 2) ? (x * x) : ((x - L) * (x - L));
+    double sv[E];
+    bool okv[E];
+    double smax = -1.0;
 #pragma unroll
     for (int m = 0; m < E; m++) {
         const int y = t + T * m;
-        const unsigned long long key = peak_key(v[m], x, y, L, L, kmin, (unsigned)(y * L + x));
-        if (key > k) k = key;
+        const int dy2 = (y < L / 2) ? (y * y) : ((y - L) * (y - L));
+        const float2 cv = v[m];
+        sv[m] = (double)cv.x * cv.x + (double)cv.y * cv.y;
+        okv[m] = dx2 + dy2 >= dm;
+        if (okv[m] && sv[m] > smax) smax = sv[m];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        if (o < NT) {
+            const double q = __shfl_down(smax, o, 64);
+            if (q > smax) smax = q;
+        }
+    if (threadIdx.x % 64 == 0) shs[threadIdx.x / 64] = smax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < NW; i++)
+            if (shs[i] > smax) smax = shs[i];
+        shs[0] = smax;
+    }
+    __syncthreads();
+    smax = shs[0];
+    unsigned long long k = 0;
+    if (smax >= 0.0) {
+        const float amax = (float)sqrt(smax);
+        const double slo = smax * (1 - 0x1p-20);
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            if (okv[m] && sv[m] >= slo && (float)sqrt(sv[m]) == amax) {
+                const unsigned i = (unsigned)((t + T * m) * L + x);
+                const unsigned long long key =
+                    ((unsigned long long)__float_as_uint(amax) << 32) | (unsigned long long)(0xFFFFFFFFu - i);
+                if (key > k) k = key;
+            }
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
@@ -425,9 +491,9 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_peak(const float2 *__
 }
 
 template <int L>
-static int launch_front_t(float2 *X, float2 *W, int G, const EbSep &sep, const float *att_tab, double kmin,
+static int launch_front_t(float2 *X, float2 *W, int G, const EbSep &sep, const float *att_tab, const int *d2min,
                           EstGroup *grp, hipStream_t s) {
-    using C = EbCols<L>;
+    using C = EbCols<L, 1024>;
     const float2 *tw = twiddles(L);
     if (!tw) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
     const unsigned gc = xcd_grid(G * C::NTILE);
@@ -438,25 +504,25 @@ static int launch_front_t(float2 *X, float2 *W, int G, const EbSep &sep, const f
     constexpr int E = FftE<L>::value, T = L / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     hipLaunchKernelGGL(k_eb_rows_corr<L>, dim3(cdiv((size_t)G * L, RW)), dim3(T * RW), 0, s, W, G, tw);
-    hipLaunchKernelGGL(k_eb_cols_peak<L>, dim3(gc), dim3(C::NT), 0, s, W + (size_t)G * L * L, G, kmin, grp, tw);
+    hipLaunchKernelGGL(k_eb_cols_peak<L>, dim3(gc), dim3(C::NT), 0, s, W + (size_t)G * L * L, G, d2min, grp, tw);
     FSIM_HIP(hipGetLastError());
     return FSIM_OK;
 }
 
 // The components' column pass, then the coarse correlation to its peak
 // (SimAlgorithm.java:84-110): three launches.
-static int launch_front(int n, float2 *X, float2 *W, int G, const EbSep &sep, const float *att_tab, double kmin,
+static int launch_front(int n, float2 *X, float2 *W, int G, const EbSep &sep, const float *att_tab, const int *d2min,
                         EstGroup *grp, hipStream_t s) {
     switch (n) {
-        case 16: return launch_front_t<16>(X, W, G, sep, att_tab, kmin, grp, s);
-        case 32: return launch_front_t<32>(X, W, G, sep, att_tab, kmin, grp, s);
-        case 64: return launch_front_t<64>(X, W, G, sep, att_tab, kmin, grp, s);
-        case 128: return launch_front_t<128>(X, W, G, sep, att_tab, kmin, grp, s);
-        case 256: return launch_front_t<256>(X, W, G, sep, att_tab, kmin, grp, s);
-        case 512: return launch_front_t<512>(X, W, G, sep, att_tab, kmin, grp, s);
-        case 1024: return launch_front_t<1024>(X, W, G, sep, att_tab, kmin, grp, s);
-        case 2048: return launch_front_t<2048>(X, W, G, sep, att_tab, kmin, grp, s);
-        case 4096: return launch_front_t<4096>(X, W, G, sep, att_tab, kmin, grp, s);
+        case 16: return launch_front_t<16>(X, W, G, sep, att_tab, d2min, grp, s);
+        case 32: return launch_front_t<32>(X, W, G, sep, att_tab, d2min, grp, s);
+        case 64: return launch_front_t<64>(X, W, G, sep, att_tab, d2min, grp, s);
+        case 128: return launch_front_t<128>(X, W, G, sep, att_tab, d2min, grp, s);
+        case 256: return launch_front_t<256>(X, W, G, sep, att_tab, d2min, grp, s);
+        case 512: return launch_front_t<512>(X, W, G, sep, att_tab, d2min, grp, s);
+        case 1024: return launch_front_t<1024>(X, W, G, sep, att_tab, d2min, grp, s);
+        case 2048: return launch_front_t<2048>(X, W, G, sep, att_tab, d2min, grp, s);
+        case 4096: return launch_front_t<4096>(X, W, G, sep, att_tab, d2min, grp, s);
         default: return fail(FSIM_ERR_ARG, "estimate: unsupported size");
     }
 }
@@ -1498,7 +1564,8 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     const size_t o_dct = align256(o_apart + (size_t)G * kPairSlots * ntile * sizeof(double2));
     const size_t o_runs = align256(o_dct + (size_t)ndc * nn * sizeof(float2));
     const size_t o_att = align256(o_runs + (size_t)ndc * sizeof(EbRuns));
-    const size_t small = align256(o_att + (2 * (size_t)(n / 2) * (n / 2) + 1) * sizeof(float));
+    const size_t o_d2 = align256(o_att + (2 * (size_t)(n / 2) * (n / 2) + 1) * sizeof(float));
+    const size_t small = align256(o_d2 + sizeof(int));
     if (!rc) rc = eb_grow(&g_eb.small, g_eb.small_bytes, small);
     if (rc) return rc;
     char *sb = reinterpret_cast<char *>(g_eb.small);
@@ -1516,13 +1583,15 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     float2 *dct = reinterpret_cast<float2 *>(sb + o_dct);
     EbRuns *runs = reinterpret_cast<EbRuns *>(sb + o_runs);
     float *att_tab = reinterpret_cast<float *>(sb + o_att);
+    int *d2min = reinterpret_cast<int *>(sb + o_d2);
 
     hipStream_t s = stream();
     g_eb.fade_h.resize(2 * (size_t)n);
     fade_factors_host(n, p->fade_px, g_eb.fade_h.data(), g_eb.fade_h.data() + n);
     FSIM_HIP(hipMemcpyAsync(fade, g_eb.fade_h.data(), 2 * n * sizeof(double), hipMemcpyHostToDevice, s));
     const unsigned gb = cdiv(G, 64);
-    hipLaunchKernelGGL(k_eb_init, dim3(gb), dim3(64), 0, s, grp, G);
+    const double kmin = p->fit_exclude * o.cutoff / p->vec_cpm;
+    hipLaunchKernelGGL(k_eb_init, dim3(gb), dim3(64), 0, s, grp, G, n, kmin, d2min);
     hipLaunchKernelGGL(k_eb_dc_tab, dim3((unsigned)std::min<size_t>((ndc * nn + 255) / 256, 4096)), dim3(256), 0, s,
                        dct, n, reg, ndc);
     hipLaunchKernelGGL(k_eb_runs, dim3(ndc), dim3(256), 0, s, reg, n, 0.05, runs);
@@ -1534,8 +1603,7 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     const int nd2 = 2 * (n / 2) * (n / 2) + 1;
     hipLaunchKernelGGL(k_eb_att_tab, dim3((unsigned)std::min(cdiv(nd2, 256), 4096u)), dim3(256), 0, s, att_tab, nd2,
                        sep.vec_cpm, sep.att_fwhm);
-    const double kmin = p->fit_exclude * o.cutoff / p->vec_cpm;
-    rc = launch_front(n, g_eb.X, g_eb.W, G, sep, att_tab, kmin, grp, s);
+    rc = launch_front(n, g_eb.X, g_eb.W, G, sep, att_tab, d2min, grp, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_eb_fit_init, dim3(gb), dim3(64), 0, s, grp, fit, G, n, sep.fb, fit_otf_band, peaks);
     FSIM_HIP(hipGetLastError());

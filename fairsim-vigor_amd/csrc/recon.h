@@ -1,5 +1,6 @@
 // Fused reconstruction: per-channel constants and job descriptors.
 #pragma once
+#include <functional>
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <vector>
@@ -62,6 +63,7 @@ struct JobList {
     Job base;
     size_t raw_stride, a_stride, b_stride, out_stride;
     bool raw_aligned;
+    hipEvent_t gather_after;   // host side: the gather's stream waits for it (nullptr: no wait)
 };
 
 // Launch choices of the pipeline that the autotuner picks per size class:
@@ -94,11 +96,16 @@ void chunk_plan(int n, int D, int Cs, int count, const PipeCfg &cfg, int &K, int
 // be created.
 std::mutex &chunk_mutex();
 const hipStream_t *chunk_streams_locked();
-// With chunk_mutex() held: runs `count` stacks (jl: base job plus strides;
-// its A/B unused) as chunks of K on the first S chunk streams, forked from
-// and joined into s; ws_a / ws_b: the work buffers (K stacks each) of stream i.
+// With chunk_mutex() held: runs `count` stacks (jl: base job plus strides,
+// its A/B unused; or a device job list whose entries carry their chunk's
+// work buffers) as chunks of K on the first S chunk streams, forked from
+// and joined into s; ws_a / ws_b: the work buffers (K stacks each) of stream
+// i.  gather_input: work enqueued on s after the fork (the Wiener tables the
+// gathers read), which the chunks' first passes overlap: each chunk's gather
+// waits for it.
 int launch_chunked(int n, const JobList &jl, int count, int D, int P, int Cs, bool half, const PipeCfg &cfg, int K,
-                   int S, hipStream_t s, float2 *const *ws_a, float2 *const *ws_b);
+                   int S, hipStream_t s, float2 *const *ws_a, float2 *const *ws_b,
+                   const std::function<int(hipStream_t)> *gather_input = nullptr);
 bool pipe_cfg_lookup(int n, int D, int P, int Cs, bool half, PipeCfg &cfg);
 int col_shape(int n, int col, int *e, int *cw);
 bool sep_all_eligible(int n, int D, int P);

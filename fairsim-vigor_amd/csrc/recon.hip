@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <type_traits>
 #include <map>
@@ -1382,6 +1383,7 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int P, int C, 
     }
     {
         constexpr int TK = GatherE<N>::tk, CW = GatherE<N>::cw;
+        if (jl.gather_after) FSIM_HIP(hipStreamWaitEvent(s, jl.gather_after, 0));
         if (half)
             hipLaunchKernelGGL((k_gather<N, true>), dim3(gather_grid(GatherTiles<N, true>::value, njobs)),
                                dim3(TK * CW), 0, s, jl, njobs, twn, twm);
@@ -1874,7 +1876,7 @@ struct ChunkStreams {
     std::mutex mu;
     bool made = false, failed = false;
     hipStream_t s[kMaxChunkStreams] = {};
-    hipEvent_t fork = nullptr;
+    hipEvent_t fork = nullptr, aux = nullptr;
     hipEvent_t join[kMaxChunkStreams] = {};
 };
 static ChunkStreams g_chunk;
@@ -1882,7 +1884,8 @@ static ChunkStreams g_chunk;
 static bool chunk_streams_ready() {
     if (g_chunk.made) return true;
     if (g_chunk.failed) return false;
-    bool ok = hipEventCreateWithFlags(&g_chunk.fork, hipEventDisableTiming) == hipSuccess;
+    bool ok = hipEventCreateWithFlags(&g_chunk.fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&g_chunk.aux, hipEventDisableTiming) == hipSuccess;
     for (int i = 0; i < kMaxChunkStreams && ok; i++)
         ok = hipStreamCreateWithFlags(&g_chunk.s[i], hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&g_chunk.join[i], hipEventDisableTiming) == hipSuccess;
@@ -1900,13 +1903,21 @@ std::mutex &chunk_mutex() { return g_chunk.mu; }
 const hipStream_t *chunk_streams_locked() { return chunk_streams_ready() ? g_chunk.s : nullptr; }
 
 int launch_chunked(int n, const JobList &jl, int count, int D, int P, int Cs, bool half, const PipeCfg &cfg, int K,
-                   int S, hipStream_t s, float2 *const *ws_a, float2 *const *ws_b) {
+                   int S, hipStream_t s, float2 *const *ws_a, float2 *const *ws_b,
+                   const std::function<int(hipStream_t)> *gather_input) {
     if (!chunk_streams_ready()) return fail(FSIM_ERR_HIP, "reconstruct: chunk streams could not be created");
     FSIM_HIP(hipEventRecord(g_chunk.fork, s));
     for (int i = 0; i < S; i++) FSIM_HIP(hipStreamWaitEvent(g_chunk.s[i], g_chunk.fork, 0));
+    JobList base = jl;
+    if (gather_input) {
+        const int rc = (*gather_input)(s);
+        if (rc) return rc;
+        FSIM_HIP(hipEventRecord(g_chunk.aux, s));
+        base.gather_after = g_chunk.aux;
+    }
     for (int c = 0, first = 0; first < count; c++, first += K) {
         const int i = c % S, kc = std::min(K, count - first);
-        JobList cj = jl;
+        JobList cj = base;
         if (jl.jobs) {
             // a device job list: its entries carry their chunk's work buffers
             cj.jobs = jl.jobs + first;
@@ -2253,14 +2264,23 @@ int reconstruct_params_with(const fsim_channel_s *ch, MultiWork &w, float2 *A, f
     const int cw = gather_cw(n);
     // the half gather reads the tiles of columns [0, n + cw)
     const size_t len = half ? (size_t)(n + cw) * M : mm;
-    hipLaunchKernelGGL(k_wiener_multi, dim3((unsigned)std::min<size_t>((len + 255) / 256, 2048), nset), dim3(256),
-                       0, s, w.wa, ch->apo, M, cw, len, w.sh, o.vals, o.only_att, o.ns, o.nband, o.cpm, o.cutoff,
-                       ch->p.vec_cpm, ch->p.use_att, (float)(ch->p.wiener * ch->p.wiener));
-    FSIM_HIP(hipGetLastError());
+    const std::function<int(hipStream_t)> wiener = [&](hipStream_t ws) {
+        hipLaunchKernelGGL(k_wiener_multi, dim3((unsigned)std::min<size_t>((len + 255) / 256, 2048), nset),
+                           dim3(256), 0, ws, w.wa, ch->apo, M, cw, len, w.sh, o.vals, o.only_att, o.ns, o.nband,
+                           o.cpm, o.cutoff, ch->p.vec_cpm, ch->p.use_att, (float)(ch->p.wiener * ch->p.wiener));
+        FSIM_HIP(hipGetLastError());
+        return FSIM_OK;
+    };
     JobList jl{};
     jl.jobs = w.jobs;
     jl.raw_aligned = aligned;
-    if (pc) return launch_chunked(n, jl, count, K0.D, K0.P, K0.Cs, half, pc->cfg, pc->k, pc->ns, s, nullptr, nullptr);
+    // chunked: the Wiener tables (FP64 work) on the caller's stream while
+    // the chunks' separation and shift passes (memory work) run
+    if (pc)
+        return launch_chunked(n, jl, count, K0.D, K0.P, K0.Cs, half, pc->cfg, pc->k, pc->ns, s, nullptr, nullptr,
+                              &wiener);
+    const int rc2 = wiener(s);
+    if (rc2) return rc2;
     return launch_pipeline(n, jl, count, K0.D, K0.P, K0.Cs, half, s);
 }
 
