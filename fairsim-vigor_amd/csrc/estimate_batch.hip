@@ -616,12 +616,10 @@ __device__ __forceinline__ double ring_ratio(int d2, double kmax) {
 // d2 in [0, n^2 / 2] find lo and hi with the same arithmetic the per-pixel
 // test uses, and the pass then compares integers.  |k| zero or not finite
 // keeps the per-pixel test.
-__global__ void k_eb_ring(EstPair *__restrict__ pairs, int npairs, int n, double dist) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
-    const double kmax = k_len(pairs[p].kx, pairs[p].ky);
+__device__ __forceinline__ void eb_ring_pair(EstPair &P, int n, double dist) {
+    const double kmax = k_len(P.kx, P.ky);
     if (!(kmax > 0.0 && kmax <= 1e300)) {
-        pairs[p].ring_ok = 0;
+        P.ring_ok = 0;
         return;
     }
     const int dmax = 2 * (n / 2) * (n / 2);
@@ -643,9 +641,9 @@ __global__ void k_eb_ring(EstPair *__restrict__ pairs, int npairs, int n, double
         else
             a = m + 1;
     }
-    pairs[p].ring_lo = lo;
-    pairs[p].ring_hi = a - 1;
-    pairs[p].ring_ok = 1;
+    P.ring_lo = lo;
+    P.ring_hi = a - 1;
+    P.ring_ok = 1;
 }
 
 // commonRegion's test on the OTF weight centred at the shifted position,
@@ -965,29 +963,92 @@ constexpr int kApxTab = 14;
 __device__ __forceinline__ const double2 *apx_txb(const float2 *tabs, int p, int n) {
     return reinterpret_cast<const double2 *>(tabs + (size_t)p * kApxTab * n + 10 * n);
 }
-__global__ void k_eb_apx_tab(const EstPair *__restrict__ pairs, int n, float2 *__restrict__ tabs) {
-    const int p = blockIdx.y;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= 12 * n) return;
-    const double search = pairs[p].search;
-    double pos;
-    int v;
-    if (k < 10 * n) {
-        pos = grid_pos(pairs[p].ky, k / n, search);
-        v = k % n;
-    } else {
-        v = (k - 10 * n) % n;
-        pos = k < 11 * n ? grid_pos(pairs[p].kx, 0, search) : search / 4.5;
-    }
-    double u = pos * v / n;
-    u -= floor(u);
-    double sn, cs;
-    sincospi(2 * u, &sn, &cs);
+__device__ __forceinline__ void eb_tabs_pair(double kx, double ky, double search, int p, int n,
+                                             float2 *__restrict__ tabs, int tid, int nt) {
     float2 *t = tabs + (size_t)p * kApxTab * n;
-    if (k < 10 * n)
-        t[v * 10 + k / n] = make_float2((float)cs, (float)-sn);
-    else
-        reinterpret_cast<double2 *>(t + 10 * n)[k - 10 * n] = make_double2(cs, sn);
+    for (int k = tid; k < 12 * n; k += nt) {
+        double pos;
+        int v;
+        if (k < 10 * n) {
+            pos = grid_pos(ky, k / n, search);
+            v = k % n;
+        } else {
+            v = (k - 10 * n) % n;
+            pos = k < 11 * n ? grid_pos(kx, 0, search) : search / 4.5;
+        }
+        double u = pos * v / n;
+        u -= floor(u);
+        double sn, cs;
+        sincospi(2 * u, &sn, &cs);
+        if (k < 10 * n)
+            t[v * 10 + k / n] = make_float2((float)cs, (float)-sn);
+        else
+            reinterpret_cast<double2 *>(t + 10 * n)[k - 10 * n] = make_double2(cs, sn);
+    }
+}
+
+// The ring bounds (thread 0) and, with `tables`, the filter's phasor
+// tables of pair p for the round about to run, from its current shift and
+// search radius.  Called by the nt threads of a block.
+__device__ __forceinline__ void eb_prep_pair(EstPair *__restrict__ pairs, int p, int n, float2 *__restrict__ tabs,
+                                             bool tables, double kx, double ky, double search, int tid, int nt) {
+    if (tid == 0) eb_ring_pair(pairs[p], n, 0.15);
+    if (tables) eb_tabs_pair(kx, ky, search, p, n, tabs, tid, nt);
+}
+
+// A round's set-up where no earlier kernel of the flow did it (the first
+// fitPeak round and the getPeak round): one block per pair.
+__global__ void __launch_bounds__(256) k_eb_prep(EstPair *__restrict__ pairs, int n, float2 *__restrict__ tabs,
+                                                 int tables) {
+    const int p = blockIdx.x;
+    eb_prep_pair(pairs, p, n, tabs, tables != 0, pairs[p].kx, pairs[p].ky, pairs[p].search, threadIdx.x, 256);
+}
+
+// Distance of the approximate sums from the exact ones, relative to sum|q|,
+// from their float part (u = 2^-24): q = b1 conj(b0) in float (<= 2 sqrt 2 u
+// |q|), the float phasor (<= u), a thread's 2 E <= 32 fused multiply-adds per
+// part, each rounding the running sum of terms bounded by sqrt 2 |q|
+// (<= 2 sqrt 2 E u |q|); the rest is double.  Under 50 u for E = 16; 64 u.
+constexpr double kApxArith = 64 * 0x1p-24;
+
+// Candidates that can hold the maximum (see the top of this file); all 100
+// when the bound cannot separate (no positive threshold, or NaN).
+// Called by the nt threads of one block per pair; mag: kPairSlots doubles
+// of shared memory.
+__device__ __forceinline__ void eb_select_pair(EstPair *__restrict__ pairs, int p, const double2 *__restrict__ apart,
+                                               int nblk, int n, double *mag, int tid, int nt) {
+    const double2 *in = apart + (size_t)p * kPairSlots * nblk;
+    for (int t = tid; t < kPairSlots; t += nt) {
+        double2 s = make_double2(0.0, 0.0);
+        for (int b = 0; b < nblk; b++) {
+            s.x += in[(size_t)t * nblk + b].x;
+            s.y += in[(size_t)t * nblk + b].y;
+        }
+        mag[t] = t >= kNormSlot ? s.x : hypot(s.x, s.y);
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    double M = 0.0;
+    bool bad = false;
+    for (int c = 0; c < kMaxCand; c++) {
+        if (!(mag[c] >= 0.0)) bad = true;
+        if (mag[c] > M) M = mag[c];
+    }
+    // the phase of pixel (x, y) is at most 2 pi (ax x + ay y) / n in magnitude
+    const double phisum = 2 * M_PI * mag[kWqSlot] / n;
+    const double delta = 1.5 * (phisum * 0x1p-24 * (1 + 1e-6) + mag[kNormSlot] * (2e-6 + kApxArith));
+    const double thr = M - 2 * delta - 1e-12 * M;
+    const bool all = bad || !(thr > 0.0) || !(delta == delta);
+    int ns = 0;
+    for (int c = 0; c < kMaxCand; c++)
+        if (all || mag[c] >= thr) pairs[p].sel[ns++] = (unsigned char)c;
+    pairs[p].nsel = ns;
+}
+
+__global__ void __launch_bounds__(128) k_eb_select(EstPair *__restrict__ pairs, const double2 *__restrict__ apart,
+                                                   int nblk, int n) {
+    __shared__ double mag[kPairSlots];
+    eb_select_pair(pairs, blockIdx.x, apart, nblk, n, mag, threadIdx.x, 128);
 }
 
 // A round's inverse column pass fused with the approximate correlation of
@@ -995,7 +1056,7 @@ __global__ void k_eb_apx_tab(const EstPair *__restrict__ pairs, int n, float2 *_
 // column x, C_j(x) = sum_y TY_j(y) q(x, y), q = b1 conj(b0), each thread's
 // terms in float and then in double; A_tile(i, j) = sum_x TX_i(x) C_j(x)
 // over the tile's columns in double, and the tile's sum |q|.  The float part
-// stays within kApxArith sum|q| of the exact sums (bound in k_eb_select).
+// stays within kApxArith sum|q| of the exact sums (bound in eb_select_pair).
 // Nothing is written back: the exact evaluation transforms the columns
 // again.  Grid xcd_grid(npairs * NTILE): neighbouring tiles of a pair run
 // on one XCD and share the cache lines of their rows.
@@ -1154,45 +1215,85 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__r
     }
 }
 
-// Distance of the approximate sums from the exact ones, relative to sum|q|,
-// from their float part (u = 2^-24): q = b1 conj(b0) in float (<= 2 sqrt 2 u
-// |q|), the float phasor (<= u), a thread's 2 E <= 32 fused multiply-adds per
-// part, each rounding the running sum of terms bounded by sqrt 2 |q|
-// (<= 2 sqrt 2 E u |q|); the rest is double.  Under 50 u for E = 16; 64 u.
-constexpr double kApxArith = 64 * 0x1p-24;
-
-// Candidates that can hold the maximum (see the top of this file); all 100
-// when the bound cannot separate (no positive threshold, or NaN).
-__global__ void __launch_bounds__(128) k_eb_select(EstPair *__restrict__ pairs, const double2 *__restrict__ apart,
-                                                   int nblk, int n) {
-    __shared__ double mag[kPairSlots];
-    const int p = blockIdx.x, t = threadIdx.x;
-    const double2 *in = apart + (size_t)p * kPairSlots * nblk;
-    if (t < kPairSlots) {
-        double2 s = make_double2(0.0, 0.0);
-        for (int b = 0; b < nblk; b++) {
-            s.x += in[(size_t)t * nblk + b].x;
-            s.y += in[(size_t)t * nblk + b].y;
+// Per pair: selected candidate sums in tile order, corr = sum * (1 / |b0|^2)
+// as Cplx.mult with a real Cplx.Double; for a grid, the first maximum of
+// hypot(corr) in y-outer, x-inner order becomes the new shift and the search
+// radius shrinks by 5 (Correlation.java:99-131).
+// Called by the nt threads of one block per pair (uniform branches only);
+// sums: kPairSlots double2 of shared memory.  The pair's new state is left
+// in pairs[p], written by thread 0.
+__device__ __forceinline__ void eb_step_pair(EstPair *__restrict__ pairs, int p, const double2 *__restrict__ part,
+                                             int nchunk, double2 *sums, int tid, int nt) {
+    const double search = pairs[p].search;
+    const int nsel = pairs[p].nsel;
+    if (search > 0 && nsel == 1) {
+        // the filter kept one candidate: it is the maximum (no sums made)
+        if (tid == 0) {
+            const int cc = pairs[p].sel[0];
+            pairs[p].kx = grid_pos(pairs[p].kx, cc % 10, search);
+            pairs[p].ky = grid_pos(pairs[p].ky, cc / 10, search);
+            pairs[p].search = search / 5;
+            pairs[p].re = pairs[p].im = 0.0;
         }
-        mag[t] = t >= kNormSlot ? s.x : hypot(s.x, s.y);
+        return;
+    }
+    const double2 *in = part + (size_t)p * kPairSlots * nchunk;
+    for (int t = tid; t <= nsel; t += nt) {
+        const int c = t < nsel ? pairs[p].sel[t] : kNormSlot;
+        double2 s = make_double2(0.0, 0.0);
+        for (int k = 0; k < nchunk; k++) {
+            s.x += in[(size_t)c * nchunk + k].x;
+            s.y += in[(size_t)c * nchunk + k].y;
+        }
+        sums[c] = s;
     }
     __syncthreads();
-    if (t != 0) return;
-    double M = 0.0;
-    bool bad = false;
-    for (int c = 0; c < kMaxCand; c++) {
-        if (!(mag[c] >= 0.0)) bad = true;
-        if (mag[c] > M) M = mag[c];
+    if (tid != 0) return;
+    const double scal = 1. / sums[kNormSlot].x;
+    if (search <= 0) {
+        pairs[p].re = sums[0].x * scal - sums[0].y * 0.0;
+        pairs[p].im = sums[0].x * 0.0 + sums[0].y * scal;
+        return;
     }
-    // the phase of pixel (x, y) is at most 2 pi (ax x + ay y) / n in magnitude
-    const double phisum = 2 * M_PI * mag[kWqSlot] / n;
-    const double delta = 1.5 * (phisum * 0x1p-24 * (1 + 1e-6) + mag[kNormSlot] * (2e-6 + kApxArith));
-    const double thr = M - 2 * delta - 1e-12 * M;
-    const bool all = bad || !(thr > 0.0) || !(delta == delta);
-    int ns = 0;
-    for (int c = 0; c < kMaxCand; c++)
-        if (all || mag[c] >= thr) pairs[p].sel[ns++] = (unsigned char)c;
-    pairs[p].nsel = ns;
+    const double kx = pairs[p].kx, ky = pairs[p].ky;
+    double max = 0, newKx = 0, newKy = 0, bre = 0, bim = 0;
+    for (int k = 0; k < nsel; k++) {
+        const int cc = pairs[p].sel[k], xi = cc % 10, yi = cc / 10;
+        const double2 s = sums[cc];
+        const double re = s.x * scal - s.y * 0.0, im = s.x * 0.0 + s.y * scal;
+        const double mag = hypot(re, im);
+        if (mag > max) {
+            max = mag;
+            newKx = grid_pos(kx, xi, search);
+            newKy = grid_pos(ky, yi, search);
+            bre = re;
+            bim = im;
+        }
+    }
+    pairs[p].kx = newKx;
+    pairs[p].ky = newKy;
+    pairs[p].search = search / 5;
+    pairs[p].re = bre;
+    pairs[p].im = bim;
+}
+
+// The step of a round, one block per pair; with prep_next also the next
+// fitPeak round's set-up at the new shift (ring bounds, phasor tables).
+__global__ void __launch_bounds__(256) k_eb_step(EstPair *__restrict__ pairs, const double2 *__restrict__ part,
+                                                 int nchunk, int n, float2 *__restrict__ tabs, int prep_next) {
+    __shared__ double2 sums[kPairSlots];
+    __shared__ double nk[3];
+    const int p = blockIdx.x;
+    eb_step_pair(pairs, p, part, nchunk, sums, threadIdx.x, 256);
+    if (!prep_next) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        nk[0] = pairs[p].kx;
+        nk[1] = pairs[p].ky;
+        nk[2] = pairs[p].search;
+    }
+    __syncthreads();
+    eb_prep_pair(pairs, p, n, tabs, true, nk[0], nk[1], nk[2], threadIdx.x, 256);
 }
 
 // The exact evaluation of a round: the inverse column pass again (the same
@@ -1202,7 +1303,7 @@ __global__ void __launch_bounds__(128) k_eb_select(EstPair *__restrict__ pairs, 
 // of the grid, or (kx, ky) alone.  A fitPeak round whose filter kept a
 // single candidate needs no evaluation -- that candidate is the strict
 // maximum, and a grid round's phase and magnitude are not used -- so such
-// pairs return at once (k_eb_step takes the candidate's position).
+// pairs return at once (eb_step_pair takes the candidate's position).
 template <int L>
 __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_corr(const float2 *__restrict__ W, int npairs,
                                                                 const EstPair *__restrict__ pairs,
@@ -1219,31 +1320,33 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_corr(const float2 *__
     const double kx = pairs[p].kx, ky = pairs[p].ky, search = pairs[p].search;
     const int nsel = pairs[p].nsel;
     if (search > 0 && nsel == 1) return;   // block-uniform
-    const float2 *tws = tw_table<C::TW::tl, NT>(twl, tw);
-    const int c = threadIdx.x % CW, t = threadIdx.x / CW;
-    const int x = tile * CW + c;
-    float2 a[E], b[E];
-    eb_cols_load<L>(W, npairs, p, tile, c, t, a, b, lds, tws);
-    double2 *out = part + (size_t)p * kPairSlots * NTILE;
-    double n2 = 0.0;
+    {
+        const float2 *tws = tw_table<C::TW::tl, NT>(twl, tw);
+        const int c = threadIdx.x % CW, t = threadIdx.x / CW;
+        const int x = tile * CW + c;
+        float2 a[E], b[E];
+        eb_cols_load<L>(W, npairs, p, tile, c, t, a, b, lds, tws);
+        double2 *out = part + (size_t)p * kPairSlots * NTILE;
+        double n2 = 0.0;
 #pragma unroll
-    for (int m = 0; m < E; m++) n2 = corr_norm2_add(n2, a[m]);
-    const double2 s0 = eb_block_sum<NT>(make_double2(n2, 0.0), sh);
-    if (threadIdx.x == 0) out[(size_t)kNormSlot * NTILE + tile] = s0;
-    constexpr double inv_n = 1.0 / L;
-    for (int k = 0; k < nsel; k++) {
-        const int cc = pairs[p].sel[k];
-        const double xpos = search > 0 ? grid_pos(kx, cc % 10, search) : kx;
-        const double ypos = search > 0 ? grid_pos(ky, cc / 10, search) : ky;
-        double2 acc = make_double2(0.0, 0.0);
+        for (int m = 0; m < E; m++) n2 = corr_norm2_add(n2, a[m]);
+        const double2 s0 = eb_block_sum<NT>(make_double2(n2, 0.0), sh);
+        if (threadIdx.x == 0) out[(size_t)kNormSlot * NTILE + tile] = s0;
+        constexpr double inv_n = 1.0 / L;
+        for (int k = 0; k < nsel; k++) {
+            const int cc = pairs[p].sel[k];
+            const double xpos = search > 0 ? grid_pos(kx, cc % 10, search) : kx;
+            const double ypos = search > 0 ? grid_pos(ky, cc / 10, search) : ky;
+            double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
-        for (int m = 0; m < E; m++) {
-            const float2 q = corr_pixel(a[m], b[m], x, t + T * m, xpos, ypos, inv_n);
-            acc.x += (double)q.x;
-            acc.y += (double)q.y;
+            for (int m = 0; m < E; m++) {
+                const float2 q = corr_pixel(a[m], b[m], x, t + T * m, xpos, ypos, inv_n);
+                acc.x += (double)q.x;
+                acc.y += (double)q.y;
+            }
+            const double2 s = eb_block_sum<NT>(acc, sh);
+            if (threadIdx.x == 0) out[(size_t)cc * NTILE + tile] = s;
         }
-        const double2 s = eb_block_sum<NT>(acc, sh);
-        if (threadIdx.x == 0) out[(size_t)cc * NTILE + tile] = s;
     }
 }
 
@@ -1308,66 +1411,6 @@ static int launch_cols_corr(int n, const float2 *W, int npairs, const EstPair *p
     FSIM_EB_SWITCH(n, launch_cols_corr_t, W, npairs, pairs, part, s)
 }
 
-// Per pair: selected candidate sums in tile order, corr = sum * (1 / |b0|^2)
-// as Cplx.mult with a real Cplx.Double; for a grid, the first maximum of
-// hypot(corr) in y-outer, x-inner order becomes the new shift and the search
-// radius shrinks by 5 (Correlation.java:99-131).  One block per pair.
-__global__ void __launch_bounds__(128) k_eb_step(EstPair *__restrict__ pairs, const double2 *__restrict__ part,
-                                                 int nchunk) {
-    __shared__ double2 sums[kPairSlots];
-    const int p = blockIdx.x, t = threadIdx.x;
-    const double search = pairs[p].search;
-    const int nsel = pairs[p].nsel;
-    if (search > 0 && nsel == 1) {
-        // the filter kept one candidate: it is the maximum (no sums made)
-        if (t == 0) {
-            const int cc = pairs[p].sel[0];
-            pairs[p].kx = grid_pos(pairs[p].kx, cc % 10, search);
-            pairs[p].ky = grid_pos(pairs[p].ky, cc / 10, search);
-            pairs[p].search = search / 5;
-            pairs[p].re = pairs[p].im = 0.0;
-        }
-        return;
-    }
-    const double2 *in = part + (size_t)p * kPairSlots * nchunk;
-    const int c = t < nsel ? pairs[p].sel[t] : (t == 127 ? kNormSlot : -1);
-    if (c >= 0) {
-        double2 s = make_double2(0.0, 0.0);
-        for (int k = 0; k < nchunk; k++) {
-            s.x += in[(size_t)c * nchunk + k].x;
-            s.y += in[(size_t)c * nchunk + k].y;
-        }
-        sums[c] = s;
-    }
-    __syncthreads();
-    if (t != 0) return;
-    const double scal = 1. / sums[kNormSlot].x;
-    if (search <= 0) {
-        pairs[p].re = sums[0].x * scal - sums[0].y * 0.0;
-        pairs[p].im = sums[0].x * 0.0 + sums[0].y * scal;
-        return;
-    }
-    const double kx = pairs[p].kx, ky = pairs[p].ky;
-    double max = 0, newKx = 0, newKy = 0, bre = 0, bim = 0;
-    for (int k = 0; k < nsel; k++) {
-        const int cc = pairs[p].sel[k], xi = cc % 10, yi = cc / 10;
-        const double2 s = sums[cc];
-        const double re = s.x * scal - s.y * 0.0, im = s.x * 0.0 + s.y * scal;
-        const double mag = hypot(re, im);
-        if (mag > max) {
-            max = mag;
-            newKx = grid_pos(kx, xi, search);
-            newKy = grid_pos(ky, yi, search);
-            bre = re;
-            bim = im;
-        }
-    }
-    pairs[p].kx = newKx;
-    pairs[p].ky = newKy;
-    pairs[p].search = search / 5;
-    pairs[p].re = bre;
-    pairs[p].im = bim;
-}
 
 // getPeak pairs from the fitted peak (SimAlgorithm.java:118-159): two bands,
 // getPeak(sep0, sep1, 0, 1, peak); three bands, the peak doubled when fitted
@@ -1470,19 +1513,25 @@ static int eb_grow(void **p, size_t &have, size_t need) {
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 // One commonRegion / inverse FFT / correlation round for npairs pairs:
-// commonRegion fused with the inverse row pass, then with `grid` (fitPeak's
-// 10 x 10 candidates) the inverse column pass fused with the candidate
-// filter and the selection, then the exact evaluation of what the filter
-// kept (the inverse column pass once more), then the step.  tabs: kApxTab n
-// float2 of phasors per pair; apart, part: partial sums of eb_ntile(n) tiles.
+// (prep: the ring bounds and, with `grid`, the filter's phasor tables --
+// otherwise the previous round's step made them) commonRegion fused with
+// the inverse row pass, then with `grid` (fitPeak's 10 x 10 candidates) the
+// inverse column pass fused with the candidate filter, and the selection;
+// the exact evaluation of what the filter kept (the inverse column pass
+// once more); the step, with prep_next also the next round's set-up.
+// tabs: kApxTab n float2 of phasors per pair; apart, part: partial sums of
+// eb_ntile(n) tiles.
 static int eb_round(const EbRegion &r, const float2 *dct, const EbRuns *runs, EstPair *pairs, int npairs, bool grid,
-                    int nphase, int n, double2 *part, float2 *tabs, double2 *apart, hipStream_t s) {
+                    int nphase, int n, double2 *part, float2 *tabs, double2 *apart, bool prep, bool prep_next,
+                    hipStream_t s) {
     const int ntile = eb_ntile(n);
-    hipLaunchKernelGGL(k_eb_ring, dim3(cdiv(npairs, 64)), dim3(64), 0, s, pairs, npairs, n, 0.15);
+    if (prep) {
+        hipLaunchKernelGGL(k_eb_prep, dim3(npairs), dim3(256), 0, s, pairs, n, tabs, grid ? 1 : 0);
+        FSIM_HIP(hipGetLastError());
+    }
     int rc = launch_region_rows(n, r, dct, runs, pairs, npairs, nphase, g_eb.X, g_eb.W, s);
     if (rc) return rc;
     if (grid) {
-        hipLaunchKernelGGL(k_eb_apx_tab, dim3(cdiv((size_t)12 * n, 256), npairs), dim3(256), 0, s, pairs, n, tabs);
         rc = launch_cols_apx(n, g_eb.W, npairs, pairs, tabs, apart, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_eb_select, dim3(npairs), dim3(128), 0, s, pairs, apart, ntile, n);
@@ -1490,7 +1539,7 @@ static int eb_round(const EbRegion &r, const float2 *dct, const EbRuns *runs, Es
     }
     rc = launch_cols_corr(n, g_eb.W, npairs, pairs, part, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_eb_step, dim3(npairs), dim3(128), 0, s, pairs, part, ntile);
+    hipLaunchKernelGGL(k_eb_step, dim3(npairs), dim3(256), 0, s, pairs, part, ntile, n, tabs, prep_next ? 1 : 0);
     FSIM_HIP(hipGetLastError());
     return FSIM_OK;
 }
@@ -1609,13 +1658,13 @@ extern "C" int fsim_estimate(const float *images, int count, const fsim_estimate
     FSIM_HIP(hipGetLastError());
     // fitPeak: three rounds of commonRegion, inverse FFTs, candidate filter, argmax
     for (int iter = 0; iter < 3; iter++) {
-        rc = eb_round(reg, dct, runs, fit, G, true, P, n, part, tabs, apart, s);
+        rc = eb_round(reg, dct, runs, fit, G, true, P, n, part, tabs, apart, iter == 0, iter < 2, s);
         if (rc) return rc;
     }
     // getPeak of band 1 (and band 2)
     hipLaunchKernelGGL(k_eb_gp_init, dim3(gb), dim3(64), 0, s, fit, gp, G, B, fit_band, live ? 1 : 0, shift);
     FSIM_HIP(hipGetLastError());
-    rc = eb_round(reg, dct, runs, gp, npk * G, false, P, n, part, tabs, apart, s);
+    rc = eb_round(reg, dct, runs, gp, npk * G, false, P, n, part, tabs, apart, true, false, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_eb_finish, dim3(gb), dim3(64), 0, s, fit, gp, G, npk, live ? 1 : 0, peaks, shift, resd,
                        errd);
