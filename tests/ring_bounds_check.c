@@ -1,4 +1,4 @@
-/* Host check of the DC-ring bounds of the batched estimation (k_eb_ring in
+/* Host check of the DC-ring bounds of the batched estimation (eb_ring_pair in
  * fairsim-vigor_amd/csrc/estimate_batch.hip): commonRegion zeroes a pixel
  * when ratio = sqrt(d2) / |k| (rounded as the reference rounds it) is below
  * dist or above 1 - dist, d2 = dx^2 + dy^2 an integer.  The kernel replaces

@@ -1,5 +1,5 @@
-"""The batched estimation's DC-ring bounds (k_eb_ring) agree with
-commonRegion's per-pixel ring test for every squared radius.
+"""The batched estimation's DC-ring bounds (eb_ring_pair, run by the step and
+set-up kernels) agree with commonRegion's per-pixel ring test for every squared radius.
 
 The ratio sqrt(d2) / |k| is a non-decreasing function of the integer d2, so
 the set the ring cuts is d2 < lo or d2 > hi; the host restatement of the
