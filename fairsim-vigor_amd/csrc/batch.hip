@@ -216,7 +216,9 @@ int fsim_batch_create(const fsim_channel *channels, int count, int flags, fsim_b
         pc.D = c.D;
         pc.C = c.Cs;
         pc.half = c.half ? 1 : 0;
-        pc.sep_all = sep_all_eligible(c.n, c.D, c.P) ? 1 : 0;
+        // the work-list kernels hold up to 9 frames per row (the one-unit
+        // separation gives the same bits otherwise)
+        pc.sep_all = c.n <= 512 && c.D * c.P <= 9 ? 1 : 0;
         pc.twn = twiddles(c.n);
         pc.twm = twiddles(2 * c.n);
         ok = pc.twn && pc.twm;

@@ -412,8 +412,8 @@ __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int nrb, int unit
 // loaded once (4 pixels per load, all loads in flight together) and kept in
 // registers as 16-bit pixels; the DC sum and every stored shifted component
 // are then combined, transformed and stored in turn.  FR bounds the frames
-// held (D*P <= FR); used for n <= 512 (8 pixels per lane) and D*P <= 9,
-// where the frames fit the register budget.
+// held (D*P <= FR: 9, or 15 for five phases in three directions); used for
+// n <= 512 (8 pixels per lane), where the frames fit the register budget.
 // One (stack, row block) item of the all-units separation: bx = the row
 // block, by = the stack.
 template <int N, int FR>
@@ -1353,6 +1353,11 @@ static int launch_pipeline_t(const JobList &jl, int njobs, int D, int P, int C, 
             if (cfg.sep != 0 && jl.raw_aligned && D * P <= 9) {
                 hipLaunchKernelGGL((k_sep_rows_all<N, 9>), dim3(nrb, njobs), dim3(T * RW), 0, s, jl, (int)half, twn);
                 done = true;
+            } else if (cfg.sep != 0 && jl.raw_aligned && D * P <= 15) {
+                // five phases in three directions
+                hipLaunchKernelGGL((k_sep_rows_all<N, 15>), dim3(nrb, njobs), dim3(T * RW), 0, s, jl, (int)half,
+                                   twn);
+                done = true;
             }
         }
         if (done) {
@@ -1431,7 +1436,7 @@ int col_shape(int n, int col, int *e, int *cw) {
     }
 }
 
-bool sep_all_eligible(int n, int D, int P) { return n <= 512 && D * P <= 9; }
+bool sep_all_eligible(int n, int D, int P) { return n <= 512 && D * P <= 15; }
 
 int launch_pipeline(int n, const JobList &jl, int njobs, int D, int P, int C, bool half, hipStream_t s) {
     PipeCfg cfg;
