@@ -927,8 +927,7 @@ __device__ __forceinline__ void gather_item(const JobList &jl, int njobs, const 
     for (int d = 0; d < D; d++) {
         // shifted components: full spectra in A, read in place; a mirrored
         // (neg) contribution reads its pos partner at the mirrored index and
-        // conjugates.  Loads first (skipped where the row is not pasted; the
-        // register then stays zero), sums after, so the loads of a
+        // conjugates.  Loads first, sums after, so the loads of a
         // contribution are in flight together.
         for (int c = 1; c < C; c++) {
             const int offx = K.offx[d][c];
@@ -945,46 +944,41 @@ __device__ __forceinline__ void gather_item(const JobList &jl, int njobs, const 
             if (mir) xs = (K.mdx[d][c] - xs) & (N - 1);
             const FSIM_GLB float2 *comp = A + (size_t)(d * Cs + K.src[c]) * nn + at_col<N>(xs);
             const float sg = mir ? -1.0f : 1.0f;
-            if constexpr (HALF) {
+            // Rows y and y + n of the doubled grid read the same component
+            // row (the paste map is periodic in n), and each paste window
+            // holds exactly one of the two (it spans n of the 2n rows).  So
+            // a lane loads rows y = t + TM * i, i < EM / 2, once each,
+            // unconditionally when its column is pasted (every such row
+            // feeds element i or i + EM / 2), and each element takes the
+            // value or zero by its own window test: the same bytes as
+            // per-element conditional loads, without a branch per element.
+            constexpr int EH = EM / 2;
+            float2 sv[EH];
 #pragma unroll
-                for (int h = 0; h < EM; h += EM / 2) {
-                    float2 sv[EM / 2];
-                    float wt[EM / 2];
+            for (int i = 0; i < EH; i++) sv[i] = make_float2(0.f, 0.f);
+            if (colok || colok2) {
 #pragma unroll
-                    for (int i = 0; i < EM / 2; i++) {
-                        sv[i] = make_float2(0.f, 0.f);
-                        const int y = t + TM * (h + i);
-                        const int py = paste_p<N>(y, oy);
-                        int ys = py & (N - 1);
-                        if (mir) ys = (my - ys) & (N - 1);
-                        const int r = (colok && paste_ok<N>(py)) + (colok2 && paste_ok<N>(paste_p<N>(-y, offy2)));
-                        wt[i] = 0.5f * (float)r;
-                        if (r) sv[i] = ld(comp + at_row<N>(ys));
-                    }
-#pragma unroll
-                    for (int i = 0; i < EM / 2; i++) {
-                        acc[h + i].x += sv[i].x * wt[i];
-                        acc[h + i].y += sg * sv[i].y * wt[i];
-                    }
-                }
-                continue;
-            }
-            // two halves of EM / 2 loads each bound the registers in flight
-#pragma unroll
-            for (int h = 0; h < EM; h += EM / 2) {
-                float2 sv[EM / 2];
-#pragma unroll
-                for (int i = 0; i < EM / 2; i++) {
-                    sv[i] = make_float2(0.f, 0.f);
-                    const int py = paste_p<N>(t + TM * (h + i), oy);
-                    int ys = py & (N - 1);
+                for (int i = 0; i < EH; i++) {
+                    int ys = paste_p<N>(t + TM * i, oy) & (N - 1);
                     if (mir) ys = (my - ys) & (N - 1);
-                    if (colok && paste_ok<N>(py)) sv[i] = ld(comp + at_row<N>(ys));
+                    sv[i] = ld(comp + at_row<N>(ys));
                 }
+            }
 #pragma unroll
-                for (int i = 0; i < EM / 2; i++) {
-                    acc[h + i].x += sv[i].x;
-                    acc[h + i].y += sg * sv[i].y;
+            for (int m = 0; m < EM; m++) {
+                const int y = t + TM * m;
+                const float2 z = make_float2(0.f, 0.f);
+                if constexpr (HALF) {
+                    const int r = (colok && paste_ok<N>(paste_p<N>(y, oy))) +
+                                  (colok2 && paste_ok<N>(paste_p<N>(-y, offy2)));
+                    const float wt = 0.5f * (float)r;
+                    const float2 v = r ? sv[m % EH] : z;
+                    acc[m].x += v.x * wt;
+                    acc[m].y += sg * v.y * wt;
+                } else {
+                    const float2 v = (colok && paste_ok<N>(paste_p<N>(y, oy))) ? sv[m % EH] : z;
+                    acc[m].x += v.x;
+                    acc[m].y += sg * v.y;
                 }
             }
         }
