@@ -388,6 +388,83 @@ __device__ __forceinline__ void fft_regs_r(float2 (&v)[E], float2 *lds, int ls, 
     fft_pass<L, E, 1, INV, true, WAVE, TwRegRef<L, E>>(v, lds, ls, t, ts, active);
 }
 
+// fft_pass with each LDS exchange in two halves, the real parts and then the
+// imaginary parts, through a float region of FftLds<L, E>::len elements
+// (times ls): half the LDS of fft_pass for twice the barriers.  The
+// butterflies, twiddles and their order are those of fft_pass, and the
+// exchange only moves values, so the results are bitwise equal.
+template <int L, int E, int NS, bool INV, bool WAVE, typename TS>
+__device__ __forceinline__ void fft_pass_split(float2 (&v)[E], float *lds, int ls, int t, const TS &ts,
+                                               bool active) {
+    constexpr int REM = L / NS;
+    constexpr int RE = E > 16 ? 16 : E;
+    constexpr int R = REM >= RE ? RE : REM;
+    constexpr bool LAST = (NS * R == L);
+    constexpr int T = L / E;
+    constexpr int Q = E / R;
+    using PL = FftLds<L, E>;
+    int pb[Q];
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            float2 u[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) u[r] = v[q + r * Q];
+            const int j = t + T * q;
+            int k = j & (NS - 1);
+            asm volatile("" : "+v"(k));
+            if constexpr (NS > 1) ts.template apply<L, E, NS, R, INV>(u, k, q);
+            dft_reg<R, INV>(u);
+#pragma unroll
+            for (int r = 0; r < R; r++) v[q + r * Q] = u[r];
+            pb[q] = PL::pad((j - k) * R + k);
+        }
+    }
+    if constexpr (!LAST) {
+        constexpr int NSP = NS + (PL::shift ? (NS >> PL::shift) : 0);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            if (active) {
+#pragma unroll
+                for (int q = 0; q < Q; q++)
+#pragma unroll
+                    for (int r = 0; r < R; r++) lds[(pb[q] + r * NSP) * ls] = h ? v[q + r * Q].y : v[q + r * Q].x;
+            }
+            pass_sync<WAVE>();
+            if (active) {
+                if constexpr (PL::shift == 0 || (T % PL::re) == 0) {
+                    constexpr int TP = T + (PL::shift ? (T >> PL::shift) : 0);
+                    const int pt = PL::pad(t);
+#pragma unroll
+                    for (int m = 0; m < E; m++) {
+                        const float x = lds[(pt + TP * m) * ls];
+                        if (h) v[m].y = x; else v[m].x = x;
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < E; m++) {
+                        const float x = lds[PL::pad(t + T * m) * ls];
+                        if (h) v[m].y = x; else v[m].x = x;
+                    }
+                }
+            }
+            pass_sync<WAVE>();
+        }
+        fft_pass_split<L, E, NS * R, INV, WAVE, TS>(v, lds, ls, t, ts, active);
+    }
+}
+
+// fft_regs through a float LDS region (fft_pass_split).
+template <int L, int E, bool INV, bool WAVE, int TL>
+__device__ __forceinline__ void fft_regs_split(float2 (&v)[E], float *lds, int ls, int t,
+                                               const float2 *__restrict__ tw, bool active = true) {
+    static_assert(L % E == 0, "E must divide L");
+    static_assert(TL == 0 || TL % L == 0, "twiddle table length must be a multiple of L");
+    static_assert(!WAVE || (L / E) <= 64, "wave synchronisation needs the transform inside one wavefront");
+    const TwTab<TL> ts{tw};
+    fft_pass_split<L, E, 1, INV, WAVE, TwTab<TL>>(v, lds, ls, t, ts, active);
+}
+
 // Twiddle table of a kernel whose largest transform has L points: tl is the
 // table length passed to fft_regs (0: no table), len the LDS elements the
 // kernel reserves for it.  use = false forces the sincospi form (a kernel

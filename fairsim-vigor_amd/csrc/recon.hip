@@ -788,12 +788,22 @@ struct GatherE {
     static constexpr int tk = tn > tm ? tn : tm;
     static constexpr int cw0 = ColTile<2 * N, tk, (en > em ? en : em)>::value;
     static constexpr int cw = cw0 < FSIM_GATHER_CW ? cw0 : FSIM_GATHER_CW;
-    // LDS elements: the larger padded transform, or the n x cw staging tile
+    // LDS elements (float2): the larger padded transform, or the n x cw
+    // staging tile.  split: the 2n-point transform exchanges its real and
+    // imaginary parts in turn through a float region (half its LDS, twice
+    // its barriers), chosen where that lets more blocks share a CU.
     static constexpr int lpn = FftLds<N, en>::len, lpm = FftLds<2 * N, em>::len;
-    static constexpr int lpk = lpm > lpn ? lpm : lpn;
-    static constexpr int lds = (lpk > N ? lpk : N) * cw;
-    // at most 4 waves per SIMD (128 VGPRs)
-    static constexpr int wpe = lds_waves_per_simd(tk * cw, (lds + 2 * N) * 8, 4);
+    static constexpr int stage = (lpn > N ? lpn : N) * cw;
+    static constexpr int full = (lpm * cw > stage ? lpm * cw : stage);
+    static constexpr int half = ((lpm * cw + 1) / 2 > stage ? (lpm * cw + 1) / 2 : stage);
+    // bytes of a block: the region, and the 2n-entry twiddle table where it fits
+    static constexpr int bytes(int l) { return (l + 2 * N) * 8 <= 163840 ? (l + 2 * N) * 8 : l * 8; }
+    static constexpr bool split =
+        lds_waves_per_simd(tk * cw, bytes(half), 8) > lds_waves_per_simd(tk * cw, bytes(full), 8);
+    static constexpr int lds = split ? half : full;
+    // registers planned for at most 4 waves per SIMD (128 VGPRs), 6 (80
+    // VGPRs) where the split exchange lets three 512-thread blocks share a CU
+    static constexpr int wpe = lds_waves_per_simd(tk * cw, (lds + 2 * N) * 8, split ? 6 : 4);
 };
 
 // pasteFreq from the n grid into the 2n grid (BasicVectors.java:273-278), one
@@ -993,7 +1003,10 @@ __device__ __forceinline__ void gather_item(const JobList &jl, int njobs, const 
 #pragma unroll
         for (int m = 0; m < EM; m++) acc[m] = cscale(acc[m], w[m]);
     }
-    fft_regs<M, EM, true, false, TW::tl>(acc, lds + cl, CW, t, tws, t < TM);
+    if constexpr (GatherE<N>::split)
+        fft_regs_split<M, EM, true, false, TW::tl>(acc, reinterpret_cast<float *>(lds) + cl, CW, t, tws, t < TM);
+    else
+        fft_regs<M, EM, true, false, TW::tl>(acc, lds + cl, CW, t, tws, t < TM);
     if (t < TM) {
         const float sc = 1.0f / M;
         if constexpr (HALF) {
