@@ -753,10 +753,13 @@ def side_outputs(n=512, batch=64, reps=20):
 
 def filter_styles(n=512, iters=10, reps=5):
     """Filter styles beyond Wiener: fsim_rl_deconvolve on a (2n)^2 spectrum
-    (the RL-on-output size) -- iterations/s and HBM rate against 168 B per
-    pixel and iteration (four 2-D FFTs at one read + write each, three fused
-    element-wise passes) -- and SimAlgorithm.runReconstruction's slow-shift
-    flow per filter style through the vector API (one 512^2 3x3 stack)."""
+    (the RL-on-output size) -- iterations/s and the HBM rate on the 104 B
+    per pixel and iteration the fused steps move (two column passes: read,
+    write, OTF; two row passes: read, write and the image, or the deconvolved
+    estimate read and written), beside the 232 B the transforms and
+    element-wise passes one at a time would move -- and
+    SimAlgorithm.runReconstruction's slow-shift flow per filter style through
+    the vector API (one 512^2 3x3 stack)."""
     import numpy as np
     from fsim import synthetic
     from fsim.linalg import Vec2dCplx
@@ -771,7 +774,7 @@ def filter_styles(n=512, iters=10, reps=5):
     for _ in range(reps):
         RLDeconvolution.deconvolve(img, otf, iters)
     t = (time.perf_counter() - a) / (reps * iters)
-    gbs = 168 * M * M / t / 1e9
+    gbs = 104 * M * M / t / 1e9
     raw, sp = synthetic.sim_stack(n=n, seed=9)
     param = synthetic.make_param(sp, n)
     inFFT = input_ffts(raw.reshape(3, 3, n, n).astype(np.float32))
@@ -784,7 +787,8 @@ def filter_styles(n=512, iters=10, reps=5):
         _lib_sync()
         styles[style] = round((time.perf_counter() - a) * 1e3, 2)
     return {"rl_1024": {"iterations_per_s": round(1 / t, 1), "GB_per_s": round(gbs, 1),
-                        "hbm_frac": round(gbs / 8000, 4)},
+                        "hbm_frac": round(gbs / 8000, 4), "bytes_per_pixel_iteration": 104,
+                        "unfused_equivalent_GB_per_s": round(232 * M * M / t / 1e9, 1)},
             "run_reconstruction_512_ms": styles}
 
 

@@ -3,13 +3,23 @@
 // for the "RL on input / output / both" filter styles.
 //
 // Per iteration the reference runs four 2-D FFTs and five element-wise
-// vector operations; here the element-wise work between the transforms is
-// fused into three kernels (same float operations, same order):
+// vector operations:
 //   next = IFFT(FFT(next) * otf);  next = reciproc(next) * img
 //   next = IFFT(FFT(next) * otf);  next = next * deconv; deconv = next
-// The reference's per-iteration change estimate (errEst, a norm2 that only
+// Here an iteration is four passes over the image, each transform pair
+// meeting its element-wise work in registers (the 2-D inverse transform
+// runs columns first, then rows):
+//   columns: FFT, * otf, IFFT            rows: IFFT, reciproc * img, FFT
+//   columns: FFT, * otf, IFFT            rows: IFFT, * deconv (stored), FFT
+// so next stays row-transformed between iterations (one row FFT before the
+// first, none after the last): 104 bytes per pixel and iteration against
+// 232 for the transforms and element-wise passes one at a time.  The float
+// operations of the element-wise steps are the reference's.  The
+// reference's per-iteration change estimate (errEst, a norm2 that only
 // feeds a trace message) is not computed.
 #include "fsim_internal.h"
+#include "fft_core.h"
+#include "fft_launch.h"
 
 #include <mutex>
 
@@ -25,27 +35,161 @@ __device__ __forceinline__ float2 times_java(float2 y, float2 x) {
     return make_float2((x.x * y.x) - (x.y * y.y), (x.y * y.x) + (x.x * y.y));
 }
 
-__global__ void k_rl_otf(float2 *__restrict__ v, const float2 *__restrict__ otf, size_t n) {
-    RL_GRID_STRIDE(i, n) v[i] = times_java(v[i], otf[i]);
-}
-
-// reciproc() (AbstractVectorCplx.java:208-216), then times(img)
-__global__ void k_rl_ratio(float2 *__restrict__ v, const float2 *__restrict__ img, size_t n) {
+// reciproc() (AbstractVectorCplx.java:208-216), then times(img), one element
+__device__ __forceinline__ float2 rl_ratio(float2 a, float2 im) {
 #pragma clang fp contract(off)
-    RL_GRID_STRIDE(i, n) {
-        const float2 a = v[i];
-        const float s = a.x * a.x + a.y * a.y;
-        v[i] = times_java(make_float2(a.x / s, -a.y / s), img[i]);
+    const float s = a.x * a.x + a.y * a.y;
+    return times_java(make_float2(a.x / s, -a.y / s), im);
+}
+
+// Transform shapes of the RL steps.  One image is a small grid for the
+// device (a 1024^2 image is 1M pixels), so the steps use 8 elements per
+// thread up to L = 1024 (2048 for rows) and narrow column tiles, which
+// spreads an image over more wavefronts than the batched transforms do.
+template <int L>
+struct RlShape {
+    static constexpr int re = L <= 512 ? FftE<L>::value : (L <= 2048 ? 8 : 16);   // rows
+    static constexpr int rt = L / re;
+    static constexpr int rw = rt >= 256 ? 1 : 256 / rt;
+    static constexpr int ce = L <= 512 ? FftE<L>::value : (L <= 1024 ? 8 : 16);   // columns
+    static constexpr int ct = L / ce;
+    static constexpr int cw0 = 512 / ct < 1 ? 1 : 512 / ct;
+    static constexpr int cw = cw0 < ColW<L>::value ? cw0 : ColW<L>::value;
+};
+
+// Column step of an iteration on an h x w row-major image (L = h): forward
+// column FFT, * otf, inverse column FFT scaled by 1/h.  Tiles of CW columns,
+// neighbouring tiles on one XCD (they share cache lines).
+template <int L>
+__global__ void __launch_bounds__(RlShape<L>::ct * RlShape<L>::cw)
+    k_rl_cols(float2 *__restrict__ data, const float2 *__restrict__ otf, int ncols, const float2 *__restrict__ tw) {
+    constexpr int E = RlShape<L>::ce, T = L / E, CW = RlShape<L>::cw;
+    constexpr int LB = FftLds<L, E>::len * CW * 8;   // bytes of the transform region
+    __shared__ float2 lds[FftLds<L, E>::len * CW];
+    // the twiddle table where it costs no block per CU
+    using TW = TwLds<L, LB + L * 8 <= 163840 && 163840 / (LB + L * 8) == 163840 / LB>;
+    __shared__ float2 twl[TW::len];
+    const int ntiles = (ncols + CW - 1) / CW;
+    const int tile = xcd_linear(ntiles);
+    if (tile >= ntiles) return;   // the whole block: before any barrier
+    const float2 *tws = tw_table<TW::tl, T * CW>(twl, tw);
+    const int c = threadIdx.x % CW, t = threadIdx.x / CW;
+    const int col = tile * CW + c;
+    const bool active = col < ncols;
+    float2 v[E];
+    if (active) {
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = data[(size_t)(t + T * m) * ncols + col];
+    }
+    fft_regs<L, E, false, false, TW::tl>(v, lds + c, CW, t, tws, active);
+    if (active) {
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = times_java(v[m], otf[(size_t)(t + T * m) * ncols + col]);
+    }
+    fft_regs<L, E, true, false, TW::tl>(v, lds + c, CW, t, tws, active);
+    if (active) {
+        const float sc = 1.0f / L;
+#pragma unroll
+        for (int m = 0; m < E; m++) data[(size_t)(t + T * m) * ncols + col] = cscale(v[m], sc);
     }
 }
 
-// times(deconvImg), then deconvImg.copy(nextImg)
-__global__ void k_rl_update(float2 *__restrict__ v, float2 *__restrict__ u, size_t n) {
-    RL_GRID_STRIDE(i, n) {
-        const float2 r = times_java(v[i], u[i]);
-        v[i] = r;
-        u[i] = r;
+// Row step of an iteration (L = w): inverse row FFT scaled by 1/w, then
+// MODE 0: reciproc * img, forward row FFT (next);
+// MODE 1: * deconv, stored to deconv, forward row FFT (next);
+// MODE 2: * deconv, stored to deconv only (the last iteration).
+template <int L, int MODE>
+__global__ void __launch_bounds__(RlShape<L>::rt * RlShape<L>::rw)
+    k_rl_rows(float2 *__restrict__ next, float2 *__restrict__ deconv, const float2 *__restrict__ img, int nrows,
+              const float2 *__restrict__ tw) {
+    constexpr int E = RlShape<L>::re, T = L / E, RW = RlShape<L>::rw;
+    constexpr int LP = FftLds<L, E>::len;
+    constexpr bool WAVE = T <= 64;
+    __shared__ float2 lds[RW * LP];
+    // row transforms: small blocks, twiddles from sincospi (no table fill)
+    using TW = TwLds<L, false>;
+    const int t = threadIdx.x % T, rr = threadIdx.x / T;
+    const int row = blockIdx.x * RW + rr;
+    const bool active = row < nrows;
+    const size_t base = (size_t)row * L;
+    float2 v[E];
+    if (active) {
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = next[base + t + T * m];
     }
+    fft_regs<L, E, true, WAVE, TW::tl>(v, lds + rr * LP, 1, t, tw, active);
+    if (active) {
+        const float sc = 1.0f / L;
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const size_t i = base + t + T * m;
+            const float2 a = cscale(v[m], sc);
+            if constexpr (MODE == 0) {
+                v[m] = rl_ratio(a, img[i]);
+            } else {
+                v[m] = times_java(a, deconv[i]);
+                deconv[i] = v[m];
+            }
+        }
+    }
+    if constexpr (MODE != 2) {
+        fft_regs<L, E, false, WAVE, TW::tl>(v, lds + rr * LP, 1, t, tw, active);
+        if (active) {
+#pragma unroll
+            for (int m = 0; m < E; m++) next[base + t + T * m] = v[m];
+        }
+    }
+}
+
+template <int L>
+static int rl_cols_t(float2 *d, const float2 *otf, int ncols, hipStream_t s) {
+    using S = RlShape<L>;
+    const float2 *tw = twiddles(L);
+    if (!tw) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
+    hipLaunchKernelGGL(k_rl_cols<L>, dim3(xcd_grid(cdiv(ncols, S::cw))), dim3(S::ct * S::cw), 0, s, d, otf, ncols,
+                       tw);
+    return FSIM_OK;
+}
+
+template <int L>
+static int rl_rows_t(float2 *next, float2 *deconv, const float2 *img, int nrows, int mode, hipStream_t s) {
+    using S = RlShape<L>;
+    const float2 *tw = twiddles(L);
+    if (!tw) return fail(FSIM_ERR_OOM, "twiddle table allocation failed");
+    const dim3 g(cdiv(nrows, S::rw)), b(S::rt * S::rw);
+    if (mode == 0)
+        hipLaunchKernelGGL((k_rl_rows<L, 0>), g, b, 0, s, next, deconv, img, nrows, tw);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_rl_rows<L, 1>), g, b, 0, s, next, deconv, img, nrows, tw);
+    else
+        hipLaunchKernelGGL((k_rl_rows<L, 2>), g, b, 0, s, next, deconv, img, nrows, tw);
+    return FSIM_OK;
+}
+
+#define FSIM_RL_SWITCH(L, FN, ...)                                                          \
+    switch (L) {                                                                            \
+        case 2: return FN<2>(__VA_ARGS__);                                                  \
+        case 4: return FN<4>(__VA_ARGS__);                                                  \
+        case 8: return FN<8>(__VA_ARGS__);                                                  \
+        case 16: return FN<16>(__VA_ARGS__);                                                \
+        case 32: return FN<32>(__VA_ARGS__);                                                \
+        case 64: return FN<64>(__VA_ARGS__);                                                \
+        case 128: return FN<128>(__VA_ARGS__);                                              \
+        case 256: return FN<256>(__VA_ARGS__);                                              \
+        case 512: return FN<512>(__VA_ARGS__);                                              \
+        case 1024: return FN<1024>(__VA_ARGS__);                                            \
+        case 2048: return FN<2048>(__VA_ARGS__);                                            \
+        case 4096: return FN<4096>(__VA_ARGS__);                                            \
+        default: return fail(FSIM_ERR_ARG, "rl_deconvolve: size must be a power of two in [2, 4096]"); \
+    }
+
+// column step on an h x w image
+static int rl_cols(float2 *d, const float2 *otf, int w, int h, hipStream_t s) {
+    FSIM_RL_SWITCH(h, rl_cols_t, d, otf, w, s)
+}
+// row step (mode as k_rl_rows) on an h x w image
+static int rl_rows(float2 *next, float2 *deconv, const float2 *img, int w, int h, int mode, hipStream_t s) {
+    FSIM_RL_SWITCH(w, rl_rows_t, next, deconv, img, h, mode, s)
 }
 
 static dim3 rl_grid(size_t n) {
@@ -96,25 +240,19 @@ extern "C" int fsim_rl_deconvolve(fsim_vec imgh, fsim_vec otfh, int steps, int i
         if (rc) return rc;
     }
     FSIM_HIP(hipMemcpyAsync(deconv, d, n * sizeof(float2), hipMemcpyDeviceToDevice, s));
-    FSIM_HIP(hipMemcpyAsync(next, d, n * sizeof(float2), hipMemcpyDeviceToDevice, s));
-    const dim3 g = rl_grid(n);
+    if (steps > 0) {
+        FSIM_HIP(hipMemcpyAsync(next, d, n * sizeof(float2), hipMemcpyDeviceToDevice, s));
+        // next row-transformed; each iteration leaves it so for the next
+        rc = fft_rows_batch_device(next, w, h, 1, 0, s);
+        if (rc) return rc;
+    }
     for (int it = 0; it < steps; it++) {
-        // 1: u_j * psf
-        rc = fft2d_device(next, w, h, 0, s);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_rl_otf, g, dim3(256), 0, s, next, o, n);
-        rc = fft2d_device(next, w, h, 1, s);
-        if (rc) return rc;
-        // 2: d / [1]
-        hipLaunchKernelGGL(k_rl_ratio, g, dim3(256), 0, s, next, d, n);
-        // 3: [2] * psf'
-        rc = fft2d_device(next, w, h, 0, s);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_rl_otf, g, dim3(256), 0, s, next, o, n);
-        rc = fft2d_device(next, w, h, 1, s);
-        if (rc) return rc;
-        // 4: u_j x [3] -> u_{j+1}
-        hipLaunchKernelGGL(k_rl_update, g, dim3(256), 0, s, next, deconv, n);
+        // 1: u_j * psf (2-D FFT: the row part is in place; inverse: columns,
+        // then rows); 2: d / [1], and the row FFT of step 3
+        if ((rc = rl_cols(next, o, w, h, s)) || (rc = rl_rows(next, deconv, d, w, h, 0, s))) return rc;
+        // 3: [2] * psf'; 4: u_j x [3] -> u_{j+1}, and the next iteration's row FFT
+        if ((rc = rl_cols(next, o, w, h, s)) || (rc = rl_rows(next, deconv, d, w, h, it + 1 < steps ? 1 : 2, s)))
+            return rc;
         FSIM_HIP(hipGetLastError());
     }
     FSIM_HIP(hipMemcpyAsync(d, deconv, n * sizeof(float2), hipMemcpyDeviceToDevice, s));

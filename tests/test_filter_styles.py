@@ -119,6 +119,35 @@ def test_rl_deconvolve_matches_oracle(fsim_lib):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("w,h,steps", [(256, 64, 3), (64, 512, 1), (1024, 1024, 2), (4096, 16, 2), (32, 32, 0)])
+def test_rl_deconvolve_shapes(fsim_lib, w, h, steps):
+    """The fused iteration (column FFT * otf IFFT, row IFFT with the ratio or
+    the update and the next row FFT) on non-square images, the largest row
+    length, one iteration (no next-row transform after it) and none (the
+    input back unchanged)."""
+    from fsim.linalg import Vec2dCplx
+    from fsim.sim_algorithm import RLDeconvolution
+    rng = np.random.default_rng(w * 7 + h + steps)
+    xh, yh = fp64.coords(w, h)
+    otf = np.exp(-(xh ** 2 + yh ** 2) / (2 * 9.0 ** 2))
+    d = fp64.ifft2(fp64.fft2(rng.uniform(1, 50, (h, w))) * otf).real + 5
+    x = d.astype(np.complex128)
+    img, o = Vec2dCplx(w, h), Vec2dCplx(w, h)
+    img.setData(np.stack([x.real, x.imag], -1).astype(np.float32))
+    o.setData(np.stack([otf, np.zeros_like(otf)], -1).astype(np.float32))
+    RLDeconvolution.deconvolve(img, o, steps, False)
+    g = img.vectorData()
+    got = g[..., 0] + 1j * g[..., 1]
+    if steps == 0:
+        assert np.array_equal(got, x.astype(np.complex64))
+        return
+    want = fp64.rl_deconvolve64(x, otf, steps, False)
+    e = fp64.rel_l2(got, want)
+    print(f"{w}x{h}, {steps} iterations: rel L2 {e:.2e}")
+    assert e < 1e-5
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("style,before,clip", [("Wiener", False, "NONE"), ("Wiener", True, "NONE"),
                                                ("RLin", False, "NONE"), ("RLout", False, "NONE"),
                                                ("RLboth", False, "BOTH"), ("Wiener", False, "CLIP")])
