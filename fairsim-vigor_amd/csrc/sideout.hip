@@ -50,59 +50,100 @@ __global__ void k_widefield(const uint16_t *__restrict__ raw, float *__restrict_
         e = (float)((double)e * hi[x0 + j]);
         d[j] = e;
     }
+    // frames in groups of G: a group's loads in flight together, the sums
+    // in frame order
+    constexpr int G = 9;
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-    for (int f = 0; f < nf; f++) {
-        const ushort4 v = r[(size_t)f * nq + q];
-        a0 = a0 + (float)v.x * d[0];
-        a1 = a1 + (float)v.y * d[1];
-        a2 = a2 + (float)v.z * d[2];
-        a3 = a3 + (float)v.w * d[3];
+    for (int f0 = 0; f0 < nf; f0 += G) {
+        ushort4 v[G];
+#pragma unroll
+        for (int g = 0; g < G; g++)
+            if (f0 + g < nf) v[g] = r[(size_t)(f0 + g) * nq + q];
+#pragma unroll
+        for (int g = 0; g < G; g++)
+            if (f0 + g < nf) {
+                a0 = a0 + (float)v[g].x * d[0];
+                a1 = a1 + (float)v[g].y * d[1];
+                a2 = a2 + (float)v[g].z * d[2];
+                a3 = a3 + (float)v[g].w * d[3];
+            }
     }
     o[q] = make_float4(a0 * scale, a1 * scale, a2 * scale, a3 * scale);
 }
 
 // Registration.transPixel (Registration.java:239-268) at target (x, y) of a
 // w x h image in tables of width tw (the image centred in them at offx,
-// offy).  0 where the source falls outside [0, w-1) x [0, h-1) (the
-// reference's fresh result vector).
-__device__ __forceinline__ float trans_pixel(const float *__restrict__ s, const float *__restrict__ xt,
-                                             const float *__restrict__ yt, int tw, int w, int h, int offx, int offy,
-                                             int x, int y) {
+// offy), in two parts: the source position, its four neighbours and their
+// weights (reg_tap, from the tables alone), then the weighted sum over one
+// image (reg_gather).  0 where the source falls outside [0, w-1) x [0, h-1)
+// (the reference's fresh result vector).
+struct RegTap {
+    int tl, tr, bl, br;          // neighbour indices; tl < 0: outside
+    float tlf, trf, blf, brf;    // their weights
+};
+__device__ __forceinline__ RegTap reg_tap(const float *__restrict__ xt, const float *__restrict__ yt, int tw, int w,
+                                          int h, int offx, int offy, int x, int y) {
 #pragma clang fp contract(off)
     const size_t t = (size_t)(y + offy) * tw + (x + offx);
     const float sx = xt[t] - offx, sy = yt[t] - offy;
-    float v = 0.f;
+    RegTap r;
+    r.tl = -1;
+    r.tr = r.bl = r.br = 0;
+    r.tlf = r.trf = r.blf = r.brf = 0.f;
     if (sx >= 0 && sy >= 0 && sx < w - 1 && sy < h - 1) {
         const int left = (int)sx, top = (int)sy, right = (int)sx + 1, bottom = (int)sy + 1;
         const float xf = sx - left, yf = sy - top;
-        const float tlf = (1 - xf) * (1 - yf), trf = xf * (1 - yf), blf = (1 - xf) * yf, brf = xf * yf;
-        const float tlv = s[(size_t)top * w + left] * tlf;
-        const float trv = s[(size_t)top * w + right] * trf;
-        const float blv = s[(size_t)bottom * w + left] * blf;
-        const float brv = s[(size_t)bottom * w + right] * brf;
-        v = tlv + trv + blv + brv;
+        r.tlf = (1 - xf) * (1 - yf);
+        r.trf = xf * (1 - yf);
+        r.blf = (1 - xf) * yf;
+        r.brf = xf * yf;
+        r.tl = top * w + left;
+        r.tr = top * w + right;
+        r.bl = bottom * w + left;
+        r.br = bottom * w + right;
     }
-    return v;
+    return r;
+}
+__device__ __forceinline__ float reg_gather(const float *__restrict__ s, const RegTap &r) {
+#pragma clang fp contract(off)
+    if (r.tl < 0) return 0.f;
+    const float tlv = s[r.tl] * r.tlf;
+    const float trv = s[r.tr] * r.trf;
+    const float blv = s[r.bl] * r.blf;
+    const float brv = s[r.br] * r.brf;
+    return tlv + trv + blv + brv;
 }
 
-// All target pixels of `count` images: grid (blocks of threads, images),
-// each thread kRegPix pixels a block width apart (coalesced table reads and
-// result writes, several gathers in flight per thread).
-constexpr int kRegPix = 4;
+// Target pixels of `count` images: grid (blocks of threads, groups of
+// kRegImg images).  Each thread takes kRegPix pixels a block width apart
+// (coalesced table reads and result writes, several gathers in flight) and
+// applies each pixel's tap to every image of its group, so the tables are
+// read once per group instead of once per image.
+constexpr int kRegPix = 4, kRegImg = 4;
 __global__ void __launch_bounds__(256) k_register(float *__restrict__ dst, const float *__restrict__ src,
                                                   const float *__restrict__ xt, const float *__restrict__ yt, int tw,
-                                                  int w, int h, int offx, int offy) {
+                                                  int w, int h, int offx, int offy, int count) {
     const size_t nn = (size_t)w * h;
-    const float *s = src + (size_t)blockIdx.y * nn;
-    float *d = dst + (size_t)blockIdx.y * nn;
     const size_t base = (size_t)blockIdx.x * 256 * kRegPix + threadIdx.x;
+    RegTap tap[kRegPix];
 #pragma unroll
     for (int j = 0; j < kRegPix; j++) {
         const size_t i = base + (size_t)j * 256;
-        if (i < nn) d[i] = trans_pixel(s, xt, yt, tw, w, h, offx, offy, (int)(i % w), (int)(i / w));
+        tap[j] = reg_tap(xt, yt, tw, w, h, offx, offy, i < nn ? (int)(i % w) : 0, i < nn ? (int)(i / w) : 0);
+    }
+    const int g0 = blockIdx.y * kRegImg;
+#pragma unroll
+    for (int g = 0; g < kRegImg; g++) {
+        if (g0 + g >= count) break;
+        const float *s = src + (size_t)(g0 + g) * nn;
+        float *d = dst + (size_t)(g0 + g) * nn;
+#pragma unroll
+        for (int j = 0; j < kRegPix; j++) {
+            const size_t i = base + (size_t)j * 256;
+            if (i < nn) d[i] = reg_gather(s, tap[j]);
+        }
     }
 }
-
 }  // namespace fsim
 
 using namespace fsim;
@@ -195,9 +236,9 @@ int fsim_register_images(fsim_registration r, int widefield, const float *src, f
         return fail(FSIM_ERR_ARG, "register_images: Image registration failed, because of differences in dimensions");
     hipStream_t s = strm ? (hipStream_t)strm : stream();
     const size_t nn = (size_t)w * h;
-    hipLaunchKernelGGL(k_register, dim3(cdiv(nn, 256 * kRegPix), count), dim3(256), 0, s, dst, src,
-                       widefield ? r->xw : r->xr,
-                       widefield ? r->yw : r->yr, tw, w, h, (tw - w) / 2, (th - h) / 2);
+    hipLaunchKernelGGL(k_register, dim3(cdiv(nn, 256 * kRegPix), cdiv(count, kRegImg)), dim3(256), 0, s, dst, src,
+                       widefield ? r->xw : r->xr, widefield ? r->yw : r->yr, tw, w, h, (tw - w) / 2, (th - h) / 2,
+                       count);
     FSIM_HIP(hipGetLastError());
     return FSIM_OK;
 }

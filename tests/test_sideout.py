@@ -70,15 +70,18 @@ def test_registration_file_reader(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,fade,raw_output", [(64, 10, 0), (128, 0, 0), (64, 10, 5)])
-def test_widefield_matches_oracle(fsim_lib, n, fade, raw_output):
+@pytest.mark.parametrize("n,fade,raw_output,nphase", [(64, 10, 0, 3), (128, 0, 0, 3), (64, 10, 5, 3),
+                                                     (64, 10, 0, 5)])
+def test_widefield_matches_oracle(fsim_lib, n, fade, raw_output, nphase):
+    """3 x 3 frames, and 3 x 5 (more frames than one group of loads)."""
     from fsim import synthetic
     from fsim.linalg import DeviceBuffer
     from fsim.sim_algorithm import Reconstructor
     from fsim._lib import call
-    stacks = [synthetic.sim_stack(n=n, seed=s, nbeads=40, nfil=6)[0] for s in range(3)]
-    _, sp = synthetic.sim_stack(n=n, seed=0, nbeads=40, nfil=6)
-    rec = Reconstructor(synthetic.make_param(sp, n), fade_px=fade)
+    kw = dict(nbeads=40, nfil=6, nphase=nphase)
+    stacks = [synthetic.sim_stack(n=n, seed=s, **kw)[0] for s in range(3)]
+    _, sp = synthetic.sim_stack(n=n, seed=0, **kw)
+    rec = Reconstructor(synthetic.make_param(sp, n, nphase=nphase), fade_px=fade)
     raw = np.stack(stacks)
     rb, ob = DeviceBuffer(raw.nbytes), DeviceBuffer(3 * n * n * 4)
     rb.upload(raw)
@@ -86,7 +89,7 @@ def test_widefield_matches_oracle(fsim_lib, n, fade, raw_output):
     call("fsim_sync")
     got = ob.download((3, n, n), np.float32)
     for s in range(3):
-        assert np.array_equal(got[s], so.widefield32(stacks[s], 3, 3, fade, raw_output)), s
+        assert np.array_equal(got[s], so.widefield32(stacks[s], 3, nphase, fade, raw_output)), s
     rb.free()
     ob.free()
 
@@ -109,13 +112,15 @@ def test_registration_matches_oracle(fsim_lib, widefield):
     txt = xt[::2, ::2] / np.float32(2) if widefield else xt
     tyt = yt[::2, ::2] / np.float32(2) if widefield else yt
     h, w = (36, 44) if widefield else (70, 90)   # smaller than the tables: centred
-    src = rng.uniform(0, 1000, (2, h, w)).astype(np.float32)
+    # five images: one full group of four (the tables read once for the
+    # group) and a partial one
+    src = rng.uniform(0, 1000, (5, h, w)).astype(np.float32)
     sb, db = DeviceBuffer(src.nbytes), DeviceBuffer(src.nbytes)
     sb.upload(src)
-    reg.register(sb.value, db.value, w, h, 2, widefield)
+    reg.register(sb.value, db.value, w, h, 5, widefield)
     call("fsim_sync")
-    got = db.download((2, h, w), np.float32)
-    for k in range(2):
+    got = db.download((5, h, w), np.float32)
+    for k in range(5):
         want = so.register32(src[k], txt, tyt)
         assert np.array_equal(got[k], want), k
         assert (want != 0).mean() > 0.8
