@@ -240,11 +240,16 @@ __device__ __forceinline__ void tiled_rows_store(FSIM_GLB float2 *a, int y0, con
 // LDS of the row kernels' items (float2 elements): the rows' transform
 // regions and the twiddle table (tl elements) and, for the complex-to-real
 // pass, the row-major copy of the block's rows.
+// pitch: the rows' regions lie pitch elements apart, 8 more than a region
+// where a block holds several rows, so that the tile-run staging (8 lanes
+// per row of a run) puts a wave's rows on different LDS banks (a pitch that
+// is a multiple of 32 banks would put all of them on the same ones).
 template <int N>
 struct RowItemLds {
     static constexpr int E = FftE<N>::value, T = N / E;
     static constexpr int RW = T >= 256 ? 1 : 256 / T;
-    static constexpr int rows = RW * FftLds<N, E>::len;
+    static constexpr int pitch = RW > 1 ? FftLds<N, E>::len + 8 : FftLds<N, E>::len;
+    static constexpr int rows = RW * pitch;
 };
 
 // One (stack, row block, unit) item of the one-unit-per-block separation
@@ -691,7 +696,7 @@ __device__ __forceinline__ void row_shift_item(const JobList &jl, const float2 *
     constexpr int E = FftE<N>::value, T = N / E;
     constexpr int RW = T >= 256 ? 1 : 256 / T;
     constexpr int rblocks = N / RW;
-    constexpr int LP = FftLds<N, E>::len;
+    constexpr int LP = RowItemLds<N>::pitch;
     const Job J = get_job(jl, by);
     KRef K = cst(J.c);
     const int C = K.Cs, nsh = C - 1;
