@@ -119,12 +119,13 @@ def test_rl_deconvolve_matches_oracle(fsim_lib):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("w,h,steps", [(256, 64, 3), (64, 512, 1), (1024, 1024, 2), (4096, 16, 2), (32, 32, 0)])
+@pytest.mark.parametrize("w,h,steps", [(256, 64, 3), (64, 512, 1), (1024, 1024, 2), (4096, 16, 2), (8, 4, 2),
+                                         (32, 32, 0)])
 def test_rl_deconvolve_shapes(fsim_lib, w, h, steps):
     """The fused iteration (column FFT * otf IFFT, row IFFT with the ratio or
     the update and the next row FFT) on non-square images, the largest row
-    length, one iteration (no next-row transform after it) and none (the
-    input back unchanged)."""
+    length, a few pixels (one thread per column transform), one iteration (no
+    next-row transform after it) and none (the input back unchanged)."""
     from fsim.linalg import Vec2dCplx
     from fsim.sim_algorithm import RLDeconvolution
     rng = np.random.default_rng(w * 7 + h + steps)
