@@ -65,9 +65,10 @@
 // that table.  Column pass: 1 LDS table per pass, 2 registers from the LDS
 // table.  Defaults measured per kernel at 512^2 (64 stacks): the all-units
 // separation runs four transforms per thread (registers: 0.166 ms against
-// 0.178 with sincospi), the row shift is no faster with fewer VALU
-// instructions (0.17 ms in all modes; the table leaves more issue slots to
-// concurrent kernels), both separation kernels share one mode (the same
+// 0.178 with sincospi), the row shift takes the table only from n = 2048
+// on (sincospi: 0.159 against 0.169 ms at 512^2, 0.169 against 0.187 at
+// 1024^2 with 16 stacks, where the table costs a block per CU; 0.411
+// against 0.398 ms at 2048^2 with 8), both separation kernels share one mode (the same
 // arithmetic, so the two forms give bitwise equal spectra), the complex-to-real pass loses to a table fill
 // (0.093 ms sincospi, 0.114 table), the column pass loses with register
 // twiddles (0.195 ms table, 0.24 registers: occupancy).
@@ -688,6 +689,13 @@ __global__ void __launch_bounds__((ColShiftTile<N, E0, CW0, WC>::value), (ColShi
 // K3: shifted components: row IFFT (1/n), row factor rx(x) of the sub-pixel
 // ramp, row FFT.
 // ---------------------------------------------------------------------------
+// The row shift's twiddles: the LDS table (FSIM_ROW_TW) from n = 2048 on,
+// sincospi below, where the table costs the kernel a block per CU.
+template <int N>
+struct RowShiftTw {
+    static constexpr bool value = FSIM_ROW_TW != 0 && N >= 2048;
+};
+
 // One (stack, shifted component, row block) item: bx = component x row
 // blocks + row block, by = the stack.
 template <int N>
@@ -700,7 +708,7 @@ __device__ __forceinline__ void row_shift_item(const JobList &jl, const float2 *
     const Job J = get_job(jl, by);
     KRef K = cst(J.c);
     const int C = K.Cs, nsh = C - 1;
-    using TW = TwLds<N, FSIM_ROW_TW != 0>;
+    using TW = TwLds<N, RowShiftTw<N>::value>;
     const int tid = item_tid();
     const float2 *tws = tw_table<TW::tl, T * RW>(twl, tw, tid, twk);
     const int rb = bx % rblocks, dc = bx / rblocks;
@@ -709,7 +717,7 @@ __device__ __forceinline__ void row_shift_item(const JobList &jl, const float2 *
     FSIM_GLB float2 *comp = glb(J.A) + (size_t)(d * C + c) * N * N;
     tiled_rows_load<N, RW, T * RW>(comp, rb * RW, lds, LP, N / kAT, tid);
     __syncthreads();
-    constexpr bool RT = FSIM_ROW_TW == 2 && tw_reg_fits<N, E>();
+    constexpr bool RT = FSIM_ROW_TW == 2 && RowShiftTw<N>::value && tw_reg_fits<N, E>();
     TwReg<N, E> twr;
     if constexpr (RT) tw_reg_load<N, E, TW::tl>(twr, t, tws);
     float2 v[E];
@@ -743,7 +751,7 @@ __device__ __forceinline__ void row_shift_item(const JobList &jl, const float2 *
 template <int N>
 __global__ void __launch_bounds__(256) k_row_shift(JobList jl, const float2 *__restrict__ tw) {
     __shared__ float2 lds[RowItemLds<N>::rows];
-    __shared__ float2 twl[TwLds<N, FSIM_ROW_TW != 0>::len];
+    __shared__ float2 twl[TwLds<N, RowShiftTw<N>::value>::len];
     bool twk = false;
     row_shift_item<N>(jl, tw, blockIdx.x, blockIdx.y, lds, twl, twk);
 }
@@ -1631,7 +1639,7 @@ __global__ void __launch_bounds__(256) k_p_row_shift(const PClass *__restrict__ 
                                                      int4 e, int *cur) {
     constexpr int A = PSizes<G>::a, B = PSizes<G>::b;
     __shared__ float2 lds[cmax(RowItemLds<A>::rows, RowItemLds<B>::rows)];
-    __shared__ float2 twl[cmax(TwLds<A, FSIM_ROW_TW != 0>::len, TwLds<B, FSIM_ROW_TW != 0>::len)];
+    __shared__ float2 twl[cmax(TwLds<A, RowShiftTw<A>::value>::len, TwLds<B, RowShiftTw<B>::value>::len)];
     __shared__ int slot[2];
     PWalk w;
     p_start(w, cur, slot);
