@@ -72,6 +72,10 @@
 // arithmetic, so the two forms give bitwise equal spectra), the complex-to-real pass loses to a table fill
 // (0.093 ms sincospi, 0.114 table), the column pass loses with register
 // twiddles (0.195 ms table, 0.24 registers: occupancy).
+// the row shift reads the LDS table from this n on (sincospi below)
+#ifndef FSIM_ROW_TW_MIN
+#define FSIM_ROW_TW_MIN 2048
+#endif
 #ifndef FSIM_ROW_TW
 #define FSIM_ROW_TW 1
 #endif
@@ -693,7 +697,7 @@ __global__ void __launch_bounds__((ColShiftTile<N, E0, CW0, WC>::value), (ColShi
 // sincospi below, where the table costs the kernel a block per CU.
 template <int N>
 struct RowShiftTw {
-    static constexpr bool value = FSIM_ROW_TW != 0 && N >= 2048;
+    static constexpr bool value = FSIM_ROW_TW != 0 && N >= FSIM_ROW_TW_MIN;
 };
 
 // One (stack, shifted component, row block) item: bx = component x row
