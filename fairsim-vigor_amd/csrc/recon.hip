@@ -991,21 +991,36 @@ __device__ __forceinline__ void gather_item(const JobList &jl, int njobs, const 
                     sv[i] = ld(comp + at_row<N>(ys));
                 }
             }
+            // The window tests in closed form: with y = t + TM m and
+            // TM EH = n, ((y - oy + n/2) mod 2n) < n holds exactly when bit
+            // log2(EH) of ad + m is clear, ad = ((t - oy + n/2) mod 2n) / TM;
+            // the mirror test ((-y - offy2 + n/2) mod 2n) < n when that bit
+            // of am - m is clear (am offset by 2 EM to stay positive).
+            // Elements i and i + EH fall on opposite sides of each window,
+            // so their weights are r and (colok + colok2) - r.
+            constexpr int LE = ilog2c(EH);
+            const int ad = ((t - oy + N / 2) & (2 * N - 1)) / TM;
+            const int am = ((-t - offy2 + N / 2) & (2 * N - 1)) / TM + 2 * EM;
+            const int rs = (int)colok + (int)colok2;
+            const float2 z = make_float2(0.f, 0.f);
 #pragma unroll
-            for (int m = 0; m < EM; m++) {
-                const int y = t + TM * m;
-                const float2 z = make_float2(0.f, 0.f);
-                if constexpr (HALF) {
-                    const int r = (colok && paste_ok<N>(paste_p<N>(y, oy))) +
-                                  (colok2 && paste_ok<N>(paste_p<N>(-y, offy2)));
-                    const float wt = 0.5f * (float)r;
-                    const float2 v = r ? sv[m % EH] : z;
-                    acc[m].x += v.x * wt;
-                    acc[m].y += sg * v.y * wt;
-                } else {
-                    const float2 v = (colok && paste_ok<N>(paste_p<N>(y, oy))) ? sv[m % EH] : z;
-                    acc[m].x += v.x;
-                    acc[m].y += sg * v.y;
+            for (int i = 0; i < EH; i++) {
+                const bool ind = (((ad + i) >> LE) & 1) == 0;
+                const bool inm = (((am - i) >> LE) & 1) == 0;
+                const int r0 = (int)(colok && ind) + (int)(HALF && colok2 && inm);
+                const int rr[2] = {r0, rs - r0};
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    const int m = i + k * EH;
+                    const float2 v = rr[k] ? sv[i] : z;
+                    if constexpr (HALF) {
+                        const float wt = 0.5f * (float)rr[k];
+                        acc[m].x += v.x * wt;
+                        acc[m].y += sg * v.y * wt;
+                    } else {
+                        acc[m].x += v.x;
+                        acc[m].y += sg * v.y;
+                    }
                 }
             }
         }
