@@ -79,6 +79,9 @@
 #ifndef FSIM_ROW_TW
 #define FSIM_ROW_TW 1
 #endif
+#ifndef FSIM_SEP_TW_MAX
+#define FSIM_SEP_TW_MAX 1024
+#endif
 #ifndef FSIM_SEP_TW
 #define FSIM_SEP_TW 2
 #endif
@@ -257,6 +260,18 @@ struct RowItemLds {
     static constexpr int rows = RW * pitch;
 };
 
+// The separation kernels' twiddles (one choice per n for both forms, so
+// that they give bitwise equal spectra): below FSIM_SEP_TW_MAX the LDS table
+// (FSIM_SEP_TW 1) or each thread's twiddles loaded from it into registers
+// (2, where they fit); from FSIM_SEP_TW_MAX on sincospi, which frees the
+// table's LDS for another block per CU (one-unit kernel at 2048^2: 0.445
+// against 0.477 ms per 8 stacks; 1024^2: 0.210 against 0.219 ms per 16).
+template <int N>
+struct SepTw {
+    static constexpr bool table = FSIM_SEP_TW != 0 && N < FSIM_SEP_TW_MAX;
+    static constexpr bool regs = FSIM_SEP_TW == 2 && table && tw_reg_fits<N, FftE<N>::value>();
+};
+
 // One (stack, row block, unit) item of the one-unit-per-block separation
 // (the kernel k_sep_rows1, and the persistent work-list kernels): bx = the
 // block's position L among the stack's row blocks and units, by = the stack.
@@ -286,7 +301,7 @@ __device__ __forceinline__ void sep_rows1_item(const JobList &jl, int nrb, int u
         nf = P;
     }
     const int crow = K.srow[c];          // separation-matrix row
-    using TW = TwLds<N, FSIM_SEP_TW != 0>;
+    using TW = TwLds<N, SepTw<N>::table>;
     const int tid = item_tid();
     const float2 *tws = tw_table<TW::tl, T * RW>(twl, tw, tid, twk);
     const int t = tid % T, rr = tid / T;
@@ -397,7 +412,7 @@ __device__ __forceinline__ void sep_rows1_item(const JobList &jl, int nrb, int u
                 }
         }
     }
-    if constexpr (FSIM_SEP_TW == 2 && tw_reg_fits<N, E>()) {
+    if constexpr (SepTw<N>::regs) {
         TwReg<N, E> twr;
         tw_reg_load<N, E, TW::tl>(twr, t, tws);
         fft_regs_r<N, E, false, RowWave<N>::value>(v, lds + rr * LP, 1, t, twr);
@@ -413,7 +428,7 @@ template <int N, bool VEC>
 __global__ void __launch_bounds__(256) k_sep_rows1(JobList jl, int nrb, int units, int half,
                                                    const float2 *__restrict__ tw) {
     __shared__ float2 lds[RowItemLds<N>::rows];
-    __shared__ float2 twl[TwLds<N, FSIM_SEP_TW != 0>::len];
+    __shared__ float2 twl[TwLds<N, SepTw<N>::table>::len];
     bool twk = false;
     sep_rows1_item<N, VEC>(jl, nrb, units, half, tw, blockIdx.x, blockIdx.y, lds, twl, twk);
 }
@@ -438,7 +453,7 @@ __device__ __forceinline__ void sep_rows_all_item(const JobList &jl, int half, c
     const Job J = get_job(jl, by);
     KRef K = cst(J.c);
     const int C = K.Cs, P = K.P, nf = K.D * P;
-    using TW = TwLds<N, FSIM_SEP_TW != 0>;
+    using TW = TwLds<N, SepTw<N>::table>;
     const int tid = item_tid();
     const float2 *tws = tw_table<TW::tl, T * RW>(twl, tw, tid, twk);
     const int t = tid % T, rr = tid / T;
@@ -467,7 +482,7 @@ __device__ __forceinline__ void sep_rows_all_item(const JobList &jl, int half, c
     float2 *row = lds + rr * LP;
     const int units = 1 + K.D * (C - 1);
     // one set of twiddles serves every unit's transform
-    constexpr bool RT = FSIM_SEP_TW == 2 && tw_reg_fits<N, E>();
+    constexpr bool RT = SepTw<N>::regs;
     TwReg<N, E> twr;
     if constexpr (RT) tw_reg_load<N, E, TW::tl>(twr, t, tws);
     for (int u = 0; u < units; u++) {
@@ -522,7 +537,7 @@ __device__ __forceinline__ void sep_rows_all_item(const JobList &jl, int half, c
 template <int N, int FR>
 __global__ void __launch_bounds__(256) k_sep_rows_all(JobList jl, int half, const float2 *__restrict__ tw) {
     __shared__ float2 lds[RowItemLds<N>::rows];
-    __shared__ float2 twl[TwLds<N, FSIM_SEP_TW != 0>::len];
+    __shared__ float2 twl[TwLds<N, SepTw<N>::table>::len];
     bool twk = false;
     sep_rows_all_item<N, FR>(jl, half, tw, blockIdx.x, blockIdx.y, lds, twl, twk);
 }
@@ -1601,7 +1616,7 @@ __global__ void __launch_bounds__(256) k_p_sep(const PClass *__restrict__ cls, c
                                                int4 e, int *cur) {
     constexpr int A = PSizes<G>::a, B = PSizes<G>::b;
     __shared__ float2 lds[cmax(RowItemLds<A>::rows, RowItemLds<B>::rows)];
-    __shared__ float2 twl[cmax(TwLds<A, FSIM_SEP_TW != 0>::len, TwLds<B, FSIM_SEP_TW != 0>::len)];
+    __shared__ float2 twl[cmax(TwLds<A, SepTw<A>::table>::len, TwLds<B, SepTw<B>::table>::len)];
     __shared__ int slot[2];
     PWalk w;
     p_start(w, cur, slot);
