@@ -1135,6 +1135,19 @@ struct C2rRows {
     static constexpr int value = RowItemLds<N>::RW * RL;
 };
 
+// LDS of a complex-to-real block: the rows' transform regions, then the
+// row copy (rows = its offset); from n = 1024 on the regions overlay the
+// copy once it has been read (half the LDS, twice the blocks per CU: 0.100
+// against 0.115 ms at 1024^2 with 16 stacks, 0.217 against 0.256 at 2048^2
+// with 8; at 512^2 the extra blocks were 3 % slower).
+template <int N>
+struct C2rLds {
+    static constexpr bool overlay = N >= 1024;
+    static constexpr int rows = overlay ? 0 : RowItemLds<N>::rows;
+    static constexpr int value = overlay ? (RowItemLds<N>::rows > C2rRows<N>::value ? RowItemLds<N>::rows : C2rRows<N>::value)
+                                         : RowItemLds<N>::rows + C2rRows<N>::value;
+};
+
 // One (stack, row group) item of the complex-to-real pass: grp = the row
 // group, by = the stack.
 template <int N>
@@ -1184,6 +1197,9 @@ __device__ __forceinline__ void row_c2r_item(const JobList &jl, const float2 *__
         b[m] = rows[rr * RL + N - k];
         w[m] = ld(tg + k);   // exp(-2 pi i k / m)
     }
+    // where the kernels overlay the transforms' region on the row copy,
+    // every thread has read its values before the first exchange writes
+    if constexpr (C2rLds<N>::overlay) __syncthreads();
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) {
@@ -1213,12 +1229,11 @@ __device__ __forceinline__ void row_c2r_item(const JobList &jl, const float2 *__
 template <int N>
 __global__ void __launch_bounds__(256) k_row_c2r(JobList jl, const float2 *__restrict__ twm) {
     constexpr int RW = RowItemLds<N>::RW;
-    __shared__ float2 lds[RowItemLds<N>::rows];
+    __shared__ __align__(16) float2 buf[C2rLds<N>::value];
     __shared__ float2 twl[TwLds<2 * N, FSIM_C2R_TW != 0>::len];
-    __shared__ float2 rows[C2rRows<N>::value];
     const int grp = (2 * N / RW) % 8 ? (int)blockIdx.x : xcd_linear(2 * N / RW);
     bool twk = false;
-    row_c2r_item<N>(jl, twm, grp, blockIdx.y, lds, twl, rows, twk);
+    row_c2r_item<N>(jl, twm, grp, blockIdx.y, buf, twl, buf + C2rLds<N>::rows, twk);
 }
 
 // Tile-major position of the tables the gather reads: element j of an
@@ -1715,10 +1730,10 @@ __global__ void __launch_bounds__(GatherE<PSizes<G>::a>::tk * GatherE<PSizes<G>:
 // carved from one array (an item uses one or the other).
 template <int N>
 struct PLastLds {
-    static constexpr int lds = RowItemLds<N>::rows, twl = TwLds<2 * N, FSIM_C2R_TW != 0>::len;
-    static constexpr int rows = (lds + twl + 1) / 2 * 2;   // 16-byte aligned
-    static constexpr int c2r = rows + C2rRows<N>::value;
-    static constexpr int value = cmax(c2r, RowItemLds<2 * N>::rows);
+    // the complex-to-real items: the transform regions and the row copy
+    // (C2rLds), then the twiddle table
+    static constexpr int reg = C2rLds<N>::value, twl = TwLds<2 * N, FSIM_C2R_TW != 0>::len;
+    static constexpr int value = cmax(reg + twl, RowItemLds<2 * N>::rows);
 };
 
 template <int G>
@@ -1735,7 +1750,7 @@ __global__ void __launch_bounds__(256) k_p_last(const PClass *__restrict__ cls, 
             using L = PLastLds<N>;
             constexpr int M = 2 * N, E = FftE<N>::value, T = N / E, RW = T >= 256 ? 1 : 256 / T;
             const int grp = (M / RW) % 8 ? (int)bx : xcd_linear_at(bx, M / RW);
-            row_c2r_item<N>(cls[k].jl, cls[k].twm, grp, by, buf, buf + L::lds, buf + L::rows, twk);
+            row_c2r_item<N>(cls[k].jl, cls[k].twm, grp, by, buf, buf + L::reg, buf + C2rLds<N>::rows, twk);
         };
     };
     auto real = [&](auto n) {
