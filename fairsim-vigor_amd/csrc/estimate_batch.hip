@@ -817,21 +817,32 @@ __global__ void __launch_bounds__(256) k_eb_region_rows(const float2 *__restrict
         const double sx = band ? -kx : kx, sy = band ? -ky : ky;
         const EbRuns &R = runs[band ? (r.otf_nband == 1 ? 0 : bn1) : 0];
         const int nrun = RUNS ? R.n : 0;
+        // ring: band 0 at its own pixel, band 1 at its source pixel
+        // ((x + ikx) mod n, (y - iky) mod n).  The ring test needs integers
+        // only, so the band values of the pixels inside the ring are loaded
+        // with the DC weights, in one memory round trip.
+        bool cuts[E];
+        float2 dcs[E], vs[E];
 #pragma unroll
         for (int m = 0; m < E; m++) {
             const int x = t + Tn * m;
-            // ring: band 0 at its own pixel, band 1 at its source pixel
-            // ((x + ikx) mod n, (y - iky) mod n)
             const int xq = band ? ((x + a.ikx) % L + L) % L : x, yq = band ? ((row - a.iky) % L + L) % L : row;
             const int dx = xq < L / 2 ? xq : xq - L, dy = yq < L / 2 ? yq : yq - L, d2 = dx * dx + dy * dy;
-            const bool cut = ring_ok ? (d2 < ring_lo || d2 > ring_hi)
-                                     : (ring_ratio(d2, kmax) < a.dist || ring_ratio(d2, kmax) > 1 - a.dist);
+            cuts[m] = ring_ok ? (d2 < ring_lo || d2 > ring_hi)
+                              : (ring_ratio(d2, kmax) < a.dist || ring_ratio(d2, kmax) > 1 - a.dist);
+            dcs[m] = dc[x];
+            vs[m] = cuts[m] ? make_float2(0.f, 0.f) : src[x];
+        }
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int x = t + Tn * m;
+            const bool cut = cuts[m];
             // a pixel in the ring's cut or outside the DC weight's support is
             // zero whatever the shifted weight: the weight (FP64 radius and
-            // table lookup) and the band value are formed only where they
-            // matter (a wavefront's lanes hold 64 adjacent pixels, so whole
-            // runs outside the common region skip it)
-            const float2 dcv = dc[x];
+            // table lookup) is formed only where it matters (a wavefront's
+            // lanes hold 64 adjacent pixels, so whole runs outside the
+            // common region skip it)
+            const float2 dcv = dcs[m];
             float2 out = make_float2(0.f, 0.f);
             if (!cut && dcv.x == dcv.x) {
                 const double d2 = d2_at(x, row, L, L, sx, sy);
@@ -845,9 +856,9 @@ __global__ void __launch_bounds__(256) k_eb_region_rows(const float2 *__restrict
                     const double cycl = cycl_of_d2(d2, a.vec_cpm);
                     const float2 wt = cycl > a.cutoff ? make_float2(0.f, 0.f)
                                                       : otf_val(tab, a.ns, a.cpm, a.cutoff, cycl);
-                    out = region_dc(src[x], dcv, wt, false, a.wlim);
+                    out = region_dc(vs[m], dcv, wt, false, a.wlim);
                 } else if (cls == 1) {
-                    out = region_pass(src[x], dcv);
+                    out = region_pass(vs[m], dcv);
                 }
             }
             v[m] = out;
