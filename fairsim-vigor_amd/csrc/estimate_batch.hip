@@ -1019,7 +1019,10 @@ __global__ void __launch_bounds__(256) k_eb_prep(EstPair *__restrict__ pairs, in
 // from their float part (u = 2^-24): q = b1 conj(b0) in float (<= 2 sqrt 2 u
 // |q|), the float phasor (<= u), a thread's 2 E <= 32 fused multiply-adds per
 // part, each rounding the running sum of terms bounded by sqrt 2 |q|
-// (<= 2 sqrt 2 E u |q|); the rest is double.  Under 50 u for E = 16; 64 u.
+// (<= 2 sqrt 2 E u |q|), then at most 5 float additions across the lanes of
+// a column in its wavefront, each rounding a partial sum bounded by
+// sqrt 2 sum|q| (<= 5 sqrt 2 u); the rest is double.  Under 57 u for
+// E = 16; 64 u.
 constexpr double kApxArith = 64 * 0x1p-24;
 
 // Candidates that can hold the maximum (see the top of this file); all 100
@@ -1158,26 +1161,28 @@ __global__ void __launch_bounds__(EbCols<L>::NT) k_eb_cols_apx(const float2 *__r
     }
     // the table's last reads come before the row sums overwrite the region
     if constexpr (TY_LDS && red_in_lds) __syncthreads();
-    double2 S[10];
-#pragma unroll
-    for (int j = 0; j < 10; j++) S[j] = make_double2((double)Sf[j].x, (double)Sf[j].y);
-    double aq = (double)aqf, wq = (double)wqf;
     // sum over the rows: the lanes of one column within the wavefront
-    // (lane = t' CW + c), then over the wavefronts
+    // (lane = t' CW + c) in float (half the cross-lane moves of double
+    // sums; the bound's float part covers them), then over the wavefronts
+    // in double
 #pragma unroll
     for (int o = CW; o < 64; o <<= 1)
         if (o < NT)
 #pragma unroll
             for (int j = 0; j < 10; j++) {
-                S[j].x += __shfl_xor(S[j].x, o, 64);
-                S[j].y += __shfl_xor(S[j].y, o, 64);
+                Sf[j].x += __shfl_xor(Sf[j].x, o, 64);
+                Sf[j].y += __shfl_xor(Sf[j].y, o, 64);
             }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
         if (o < NT) {
-            aq += __shfl_down(aq, o, 64);
-            wq += __shfl_down(wq, o, 64);
+            aqf += __shfl_down(aqf, o, 64);
+            wqf += __shfl_down(wqf, o, 64);
         }
+    double2 S[10];
+#pragma unroll
+    for (int j = 0; j < 10; j++) S[j] = make_double2((double)Sf[j].x, (double)Sf[j].y);
+    const double aq = (double)aqf, wq = (double)wqf;
     const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
     if (lane < CW)
 #pragma unroll

@@ -79,12 +79,27 @@ def test_candidate_filter_bound(n, kx, ky, seed):
         assert len(sel) <= 25
 
 
-def _float_part_sums(b0, b1, n, xpos0, ypos, step, T):
+def _wave_tree(s, lpw):
+    """The kernel's cross-lane float sums within a wavefront: the threads
+    t' of a column sit in groups of lpw lanes (lane = t' CW + c), and each
+    lane adds its partner's value at lane distance CW, 2 CW, ... (xor), in
+    float32; returns the group sums, (T / lpw, n)."""
+    w = s.reshape(s.shape[0] // lpw, lpw, -1)
+    o = 1
+    while o < lpw:
+        w = (w + w[:, np.arange(lpw) ^ o]).astype(np.float32)
+        o <<= 1
+    return w[:, 0]
+
+
+def _float_part_sums(b0, b1, n, xpos0, ypos, step, T, lpw=4):
     """The approximate sums as the column kernel forms them: per column x,
     thread t adds q(x, y) * TY_j(y) for y = t, t + T, ... in float32
     (q = b1 conj(b0) in float32, TY_j rounded from double; each product
     term added with one rounding, as the kernel's fused multiply-adds), the
-    threads' sums are added in double, then sum_x TX_i(x) C_j(x) in double."""
+    sums of the lpw threads of a column in one wavefront are added in
+    float32 (xor tree), the wavefronts' sums in double, then
+    sum_x TX_i(x) C_j(x) in double."""
     f = np.float32
     ux, uy = b0[..., 0], b0[..., 1]
     vx, vy = b1[..., 0], b1[..., 1]
@@ -111,7 +126,7 @@ def _float_part_sums(b0, b1, n, xpos0, ypos, step, T):
             sr = (sr.astype(d) + a2[k]).astype(f)
             si = (si.astype(d) + b1[k]).astype(f)
             si = (si.astype(d) + b2[k]).astype(f)
-        C[j] = sr.astype(np.float64).sum(0) + 1j * si.astype(np.float64).sum(0)
+        C[j] = _wave_tree(sr, lpw).astype(np.float64).sum(0) + 1j * _wave_tree(si, lpw).astype(np.float64).sum(0)
     x = np.arange(n)
     for j in range(10):
         for i in range(10):
@@ -123,7 +138,8 @@ def _float_part_sums(b0, b1, n, xpos0, ypos, step, T):
 def test_candidate_filter_float_part(n, kx, ky, seed):
     """The float part of the kernel's approximate sums stays within
     64 * 2^-24 * sum|q| of the exact candidate sums (T = 16 threads per
-    column at n = 128, eight terms each); a wrong phasor sign, axis or
+    column at n = 128, eight terms each, 16-column tiles: four threads of a
+    column per wavefront, summed in float); a wrong phasor sign, axis or
     table row misses by orders of magnitude."""
     b0, b1 = _pair(n, kx, ky, seed)
     z0 = b0[..., 0].astype(np.float64) + 1j * b0[..., 1]
