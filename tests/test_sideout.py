@@ -95,8 +95,8 @@ def test_widefield_matches_oracle(fsim_lib, n, fade, raw_output, nphase):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("widefield", [False, True])
-def test_registration_matches_oracle(fsim_lib, widefield):
+@pytest.mark.parametrize("widefield,count", [(False, 5), (True, 5), (False, 67)])
+def test_registration_matches_oracle(fsim_lib, widefield, count):
     from fsim.linalg import DeviceBuffer
     from fsim.livemode import Registration
     from fsim._lib import call
@@ -112,15 +112,16 @@ def test_registration_matches_oracle(fsim_lib, widefield):
     txt = xt[::2, ::2] / np.float32(2) if widefield else xt
     tyt = yt[::2, ::2] / np.float32(2) if widefield else yt
     h, w = (36, 44) if widefield else (70, 90)   # smaller than the tables: centred
-    # five images: one full group of four (the tables read once for the
-    # group) and a partial one
-    src = rng.uniform(0, 1000, (5, h, w)).astype(np.float32)
+    # five images: a partial group (the kernel takes up to 64 images per
+    # group and reads the tables once for it); 67: a full group and a
+    # partial one
+    src = rng.uniform(0, 1000, (count, h, w)).astype(np.float32)
     sb, db = DeviceBuffer(src.nbytes), DeviceBuffer(src.nbytes)
     sb.upload(src)
-    reg.register(sb.value, db.value, w, h, 5, widefield)
+    reg.register(sb.value, db.value, w, h, count, widefield)
     call("fsim_sync")
-    got = db.download((5, h, w), np.float32)
-    for k in range(5):
+    got = db.download((count, h, w), np.float32)
+    for k in range(count):
         want = so.register32(src[k], txt, tyt)
         assert np.array_equal(got[k], want), k
         assert (want != 0).mean() > 0.8
